@@ -1,0 +1,140 @@
+/*
+ * mbrec.h -- C ABI of the MI355X-native model-based MSOT operators and
+ * non-negative least-squares solvers.
+ *
+ * Two layers:
+ *
+ * 1. MEX drop-ins.  One entry point per reference MEX file, same argument
+ *    order, dtypes and output shapes (FP64 host buffers in, FP64 host buffer
+ *    out, column-major MATLAB layouts).  A MATLAB gateway (mexFunction) only
+ *    unpacks mxArrays and calls these; it raises the same error IDs as the
+ *    reference when a type check fails.  Device-resident tables are cached by
+ *    (pointer, dims, content hash), so tables are uploaded once.
+ *
+ * 2. Device API.  A plan holds the device-resident geometry (travel times,
+ *    SIR tables, impulse response, prefilter) and runs the operators and
+ *    solvers on device buffers (FP32 bulk, FP64 scalars) on a caller stream.
+ *
+ * All functions return MBREC_OK (0) or a negative status; mbrec_last_error()
+ * and mbrec_last_error_id() describe the last failure of the calling thread.
+ *
+ * Layouts.  MEX image [W x P] (wavelength fastest), MEX sinogram [W x nT x D],
+ * travel times / SIR indices [P x D] (pixel fastest, tt[d*P + x]), SIR table
+ * [Ls x Nsir], SIR windows [2 x Nsir] (1-based, inclusive).  Device images are
+ * planar [B][P] with pixel x = iz + Nz*ix, device sinograms planar [B][D][nT]
+ * (the MATLAB vector layout of MSOTmodel.m:194,202,226).
+ */
+#ifndef MBREC_H
+#define MBREC_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MBREC_OK 0
+#define MBREC_ERR_ARG (-1)      /* invalid argument / shape */
+#define MBREC_ERR_TYPE (-2)     /* wrong input class (MEX type checks) */
+#define MBREC_ERR_HIP (-3)      /* HIP runtime error */
+#define MBREC_ERR_NODEV (-4)    /* no usable GPU */
+#define MBREC_ERR_RANGE (-5)    /* travel times outside the fixed-point range */
+
+typedef struct mbrec_plan_s *mbrec_plan;  /* opaque device-resident geometry */
+typedef void *mbrec_stream;                /* hipStream_t; NULL = null stream */
+
+/* ---- library ------------------------------------------------------------ */
+const char *mbrec_last_error(void);
+const char *mbrec_last_error_id(void);
+int mbrec_version(void);
+/* Selects and initialises a device (hipSetDevice); 0 on success. */
+int mbrec_init(int device);
+
+/* ---- 1. MEX drop-ins ----------------------------------------------------- */
+
+/* forwardMex (reconstruction_model_based/models/forwardMex.cpp:16-162):
+ * sig = forwardMex(image[W x P], sirIndices[P x D] int32, travelTimes[P x D],
+ *                  sir[Ls x Nsir], numTimesteps int32, sirTimestepsStartAndEnd[2 x Nsir] int32, R)
+ * out: [W x nT x D], overwritten (the MEX zero-fills, :84). */
+int mbrec_forwardMex(const double *image, int W, int P, const int32_t *sirIndices,
+                     const double *travelTimes, int D, const double *sir, int Ls, int Nsir,
+                     int32_t numTimesteps, const int32_t *sirTimestepsStartAndEnd,
+                     double R, double *out);
+
+/* transposeMex (reconstruction_model_based/models/transposeMex.cpp:16-158):
+ * img = transposeMex(sigMat[W x nT x D], sirIndices, travelTimes, sir, sirSE, R)
+ * out: [W x P]. */
+int mbrec_transposeMex(const double *sigMat, int W, int nT, int D, const int32_t *sirIndices,
+                       const double *travelTimes, int P, const double *sir, int Ls, int Nsir,
+                       const int32_t *sirTimestepsStartAndEnd, double R, double *out);
+
+/* forwardDelayAndSumMex (reconstruction_model_based/models/forwardDelayAndSumMex.cpp:15-104):
+ * sig = forwardDelayAndSumMex(image[W x P], travelTimes[P x D], numTimesteps int32, R)
+ * Samples outside [0, nT) are dropped (the reference writes out of bounds, :80-99). */
+int mbrec_forwardDelayAndSumMex(const double *image, int W, int P, const double *travelTimes,
+                                int D, int32_t numTimesteps, double R, double *out);
+
+/* transposeDelayAndSumMex (reconstruction_model_based/models/transposeDelayAndSumMex.cpp:15-98):
+ * img = transposeDelayAndSumMex(sigMat[W x nT x D], travelTimes[P x D], R) */
+int mbrec_transposeDelayAndSumMex(const double *sigMat, int W, int nT, int D,
+                                  const double *travelTimes, int P, double R, double *out);
+
+/* Type-check helper shared by the MEX gateways: maps an argument position and
+ * an observed class to the reference error ID ("forwardMex:notDouble" /
+ * "forwardMex:notInt32", forwardMex.cpp:30-50) and sets the last error. */
+int mbrec_mex_type_error(const char *mex_name, int arg_index, int expect_int32);
+
+/* Drops every cached device table (the gateways call this from mexAtExit). */
+void mbrec_mex_cache_clear(void);
+
+/* ---- 2. Device API -------------------------------------------------------- */
+
+/* Geometry: travelTimes [P x D] FP64 host (0-based fractional samples, as
+ * MSOTmodel.m:380-381 produces them).  Nz, Nx give the pixel grid (P = Nz*Nx)
+ * and enable the tiled device pixel order; pass Nz = P, Nx = 1 when unknown.
+ * Replaces the per-call table marshalling of MSOTmodel.m:195-252. */
+int mbrec_plan_create(mbrec_plan *plan, int Nz, int Nx, int D, int nT, double R,
+                      const double *travelTimes);
+/* SIR tables as forwardMex takes them (MSOTmodel.m:196-200). */
+int mbrec_plan_set_sir(mbrec_plan plan, const int32_t *sirIndices, const double *sir, int Ls,
+                       int Nsir, const int32_t *sirTimestepsStartAndEnd);
+/* Impulse response IR [Lir] or [Lir x D] (per_detector), MSOTmodel.m:259-269. */
+int mbrec_plan_set_ir(mbrec_plan plan, const double *ir, int Lir, int per_detector);
+/* Separable circular prefilter (Discretization.m:54-72): taps kz[0..2Kz],
+ * kx[0..2Kx] for offsets -K..K of the kernel k[n] = ifft(1/phi_hat)[n]. */
+int mbrec_plan_set_prefilter(mbrec_plan plan, const double *kz, int Kz, const double *kx, int Kx);
+/* model_normalization_factor (MSOTmodel.m:271-287). */
+int mbrec_plan_set_sigma(mbrec_plan plan, double sigma);
+int mbrec_plan_destroy(mbrec_plan plan);
+/* Bytes the plan holds on the device (tables + workspace). */
+size_t mbrec_plan_device_bytes(mbrec_plan plan);
+/* Fixed-point travel-time format: number of fractional bits. */
+int mbrec_plan_tau_frac_bits(mbrec_plan plan);
+
+/* Bare MEX loop (no prefilter, IR or 1/sigma), batch of B planar images
+ * [B][P] -> sinograms [B][D][nT] and back. */
+int mbrec_mex_forward_dev(mbrec_plan plan, const float *img, float *sino, int B, mbrec_stream s);
+int mbrec_mex_transpose_dev(mbrec_plan plan, const float *sino, float *img, int B, mbrec_stream s);
+
+/* Normalised model operators model.Funcs.applyForward / applyTranspose
+ * (MSOTmodel.m:285-286): prefilter, MEX loop, IR convolution, 1/sigma. */
+int mbrec_model_forward_dev(mbrec_plan plan, const float *img, float *sino, int B, mbrec_stream s);
+int mbrec_model_transpose_dev(mbrec_plan plan, const float *sino, float *img, int B, mbrec_stream s);
+
+/* rec_nn_with_L2_reg (reconstruction/rec_nn_with_L2_reg.m:1-66) with RegL2 = identity
+ * and nnls_L2_less (nnls_L2_less.m:1-101): 50-style outer splitting, 2 LSQR inner
+ * iterations, lambda = 1, beta = 1e-3.  data [B][D][nT], rec [B][P].
+ * coupled != 0 solves the B sinograms as one stacked problem with shared LSQR
+ * scalars (the reference's multi-wavelength call); coupled == 0 solves each
+ * independently.  init (nullable) seeds the outer iterate w0 (warm start; the
+ * reference ignores its initial image, nnls_L2_less.m:28-29).  lcurve (host,
+ * nullable) receives per group {||A rec - y||^2, ||rec||^2}. */
+int mbrec_rec_nn_with_L2_reg_dev(mbrec_plan plan, const float *data, int B, int coupled,
+                                 int num_iter, double lambdaL2, const float *init,
+                                 float *rec, double *lcurve, mbrec_stream s);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MBREC_H */

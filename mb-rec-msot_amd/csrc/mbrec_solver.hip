@@ -1,0 +1,497 @@
+// mbrec_solver.hip -- device-resident non-negative least-squares loops.
+//
+// nnls_L2_less (reconstruction/nnls_L2_less.m:28-72) around MATLAB-semantics
+// LSQR (Paige & Saunders 1982, in the d-vector form of MATLAB's lsqr: initial
+// residual b - A x0, one transpose product at the end of every iteration,
+// residual estimate normr = |s| normr, stopping tests normr <= tol ||b|| and
+// ||A'r|| / (||A|| ||r||) <= tol, stagnation counter).  The augmented operator
+// is A_aug = [sqrt(l) A_reg; sqrt(b) I] (add_nn_constraint_term_to_model_matrix,
+// nnls_L2_less.m:79-101) with A_reg = [A; sqrt(lambdaL2) I]
+// (combine_forward_and_transpose_with_reg.m:6-14, RegL2 = identity).
+//
+// Vectors are FP32 on the device; every inner product, norm and recurrence
+// scalar is FP64 and lives on the device, one scalar set per group (all B
+// sinograms in one group = the reference's stacked multi-wavelength call;
+// one group per sinogram = independent batched solves).  No host round trip
+// happens inside a solve, so the launch sequence is graph-capturable.
+#include "mbrec_internal.h"
+
+#include <cmath>
+#include <vector>
+
+namespace mbrec {
+
+constexpr int kRedBlocks = 64;     // reduction blocks per batch item
+constexpr int kMaxQ = 6;           // reduction slots
+
+enum { Q_A = 0, Q_B = 1, Q_C = 2, Q_D = 3, Q_E = 4, Q_F = 5 };
+
+struct Scal {
+    double alpha, beta, c, s, phibar, phi, thet, rho;
+    double normr, norma, normar, tolb, n2b;
+    double inv_u, inv_v;           // pending normalisations of u and v
+    int stag, flag, active, iter;
+};
+
+// block-wide FP64 sum, written to part[q][b][blk]
+__device__ __forceinline__ void block_sum_store(double v, double *part, int q, int b, int B)
+{
+    __shared__ double sh[8];
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) sh[w] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (int i = 0; i < (int)(blockDim.x >> 6); ++i) t += sh[i];
+        part[((size_t)q * B + b) * kRedBlocks + blockIdx.x] = t;
+    }
+    __syncthreads();
+}
+
+__device__ __forceinline__ double group_sum(const double *part, int q, int g, int B, int coupled)
+{
+    double t = 0.0;
+    const int b0 = coupled ? 0 : g, b1 = coupled ? B : g + 1;
+    for (int b = b0; b < b1; ++b)
+        for (int i = 0; i < kRedBlocks; ++i) t += part[((size_t)q * B + b) * kRedBlocks + i];
+    return t;
+}
+
+struct Consts {
+    float sl, sb, sr;   // sqrt(lambda), sqrt(beta), sqrt(lambda)*sqrt(lambdaL2)
+    int reg;            // lambdaL2 > 0
+    double tol;
+    int coupled;
+};
+
+// ---- outer-iteration setup: u = b - A_aug x0 with x0 = w --------------------
+// sinogram block: u_s = sl*(y - A w); ||u||^2 -> Q_A, ||b||^2 -> Q_B
+__global__ __launch_bounds__(256) void k_init_sino(const float *__restrict__ y, const float *__restrict__ aw,
+                                                   float *__restrict__ us, size_t n, int B, Consts k,
+                                                   double *part)
+{
+    const int b = blockIdx.y;
+    const size_t off = (size_t)b * n;
+    double uu = 0.0, bb = 0.0;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const float yy = y[off + i];
+        const float u = k.sl * (yy - aw[off + i]);
+        us[off + i] = u;
+        uu += (double)u * u;
+        const double bv = (double)k.sl * yy;
+        bb += bv * bv;
+    }
+    block_sum_store(uu, part, Q_A, b, B);
+    block_sum_store(bb, part, Q_B, b, B);
+}
+
+// pixel blocks: u_r = -sr*w, u_c = sb*vm, x = w, d = 0; ||u||^2 -> Q_C, ||b||^2 -> Q_D
+__global__ __launch_bounds__(256) void k_init_pix(const float *__restrict__ w, const float *__restrict__ vm,
+                                                  float *__restrict__ ur, float *__restrict__ uc,
+                                                  float *__restrict__ x, float *__restrict__ dv,
+                                                  size_t n, int B, Consts k, double *part)
+{
+    const int b = blockIdx.y;
+    const size_t off = (size_t)b * n;
+    double uu = 0.0, bb = 0.0;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const float wi = w[off + i], vi = vm[off + i];
+        if (k.reg) {
+            const float r = -k.sr * wi;
+            ur[off + i] = r;
+            uu += (double)r * r;
+        }
+        const float bc = k.sb * (wi + vi);
+        const float c = bc - k.sb * wi;
+        uc[off + i] = c;
+        uu += (double)c * c;
+        bb += (double)bc * bc;
+        x[off + i] = wi;
+        dv[off + i] = 0.0f;
+    }
+    block_sum_store(uu, part, Q_C, b, B);
+    block_sum_store(bb, part, Q_D, b, B);
+}
+
+__global__ void k_scal_init(Scal *sc, const double *part, int G, int B, Consts k)
+{
+    for (int g = threadIdx.x; g < G; g += blockDim.x) {
+        Scal &S = sc[g];
+        const double bb = group_sum(part, Q_B, g, B, k.coupled) + group_sum(part, Q_D, g, B, k.coupled);
+        const double uu = group_sum(part, Q_A, g, B, k.coupled) + group_sum(part, Q_C, g, B, k.coupled);
+        S.n2b = sqrt(bb);
+        S.tolb = k.tol * S.n2b;
+        S.beta = sqrt(uu);
+        S.normr = S.beta;
+        S.c = 1.0;
+        S.s = 0.0;
+        S.phibar = S.beta;
+        S.inv_u = S.beta != 0.0 ? 1.0 / S.beta : 1.0;
+        S.norma = 0.0;
+        S.stag = 0;
+        S.iter = 0;
+        S.flag = 1;
+        S.active = 1;
+        if (S.n2b == 0.0) { S.flag = 0; S.active = 0; S.iter = -1; }  // x = 0 (b == 0)
+    }
+}
+
+// scale the three u blocks by inv_u (pending normalisation) of their group
+__global__ __launch_bounds__(256) void k_scale_u(float *__restrict__ us, size_t ns, float *__restrict__ ur,
+                                                 float *__restrict__ uc, size_t np, int B, Consts k,
+                                                 const Scal *sc)
+{
+    const int b = blockIdx.y;
+    const Scal &S = sc[k.coupled ? 0 : b];
+    if (!S.active) return;
+    const float f = (float)S.inv_u;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < ns; i += (size_t)gridDim.x * blockDim.x)
+        us[(size_t)b * ns + i] *= f;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < np; i += (size_t)gridDim.x * blockDim.x) {
+        if (k.reg) ur[(size_t)b * np + i] *= f;
+        uc[(size_t)b * np + i] *= f;
+    }
+}
+
+// v = sl*A'u_s + sr*u_r + sb*u_c - beta*v (first=1: without the beta term); ||v||^2 -> Q_E
+__global__ __launch_bounds__(256) void k_v_update(const float *__restrict__ atu, const float *__restrict__ ur,
+                                                  const float *__restrict__ uc, float *__restrict__ v,
+                                                  size_t n, int B, Consts k, const Scal *sc, int first,
+                                                  double *part)
+{
+    const int b = blockIdx.y;
+    const Scal &S = sc[k.coupled ? 0 : b];
+    const size_t off = (size_t)b * n;
+    double vv = 0.0;
+    if (S.active) {
+        const float beta = first ? 0.0f : (float)S.beta;
+        for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+            float t = k.sl * atu[off + i] + k.sb * uc[off + i];
+            if (k.reg) t += k.sr * ur[off + i];
+            const float vn = t - beta * v[off + i];
+            v[off + i] = vn;
+            vv += (double)vn * vn;
+        }
+    }
+    block_sum_store(vv, part, Q_E, b, B);
+}
+
+__global__ void k_scal_v0(Scal *sc, const double *part, int G, int B, Consts k)
+{
+    for (int g = threadIdx.x; g < G; g += blockDim.x) {
+        Scal &S = sc[g];
+        if (!S.active) continue;
+        const double vv = group_sum(part, Q_E, g, B, k.coupled);
+        S.alpha = sqrt(vv);
+        S.inv_v = S.alpha != 0.0 ? 1.0 / S.alpha : 1.0;
+        S.normar = S.alpha * S.beta;
+        if (S.normar == 0.0) { S.flag = 0; S.active = 0; }
+    }
+}
+
+__global__ __launch_bounds__(256) void k_scale_v(float *__restrict__ v, size_t n, int B, Consts k,
+                                                 const Scal *sc)
+{
+    const int b = blockIdx.y;
+    const Scal &S = sc[k.coupled ? 0 : b];
+    if (!S.active) return;
+    const float f = (float)S.inv_v;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        v[(size_t)b * n + i] *= f;
+}
+
+// u_s = sl*(A v) - alpha*u_s; ||.||^2 -> Q_A
+__global__ __launch_bounds__(256) void k_u_sino(const float *__restrict__ av, float *__restrict__ us, size_t n,
+                                                int B, Consts k, const Scal *sc, double *part)
+{
+    const int b = blockIdx.y;
+    const Scal &S = sc[k.coupled ? 0 : b];
+    const size_t off = (size_t)b * n;
+    double uu = 0.0;
+    if (S.active) {
+        const float a = (float)S.alpha;
+        for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+            const float u = k.sl * av[off + i] - a * us[off + i];
+            us[off + i] = u;
+            uu += (double)u * u;
+        }
+    }
+    block_sum_store(uu, part, Q_A, b, B);
+}
+
+// u_r = sr*v - alpha*u_r, u_c = sb*v - alpha*u_c; ||.||^2 -> Q_C
+__global__ __launch_bounds__(256) void k_u_pix(const float *__restrict__ v, float *__restrict__ ur,
+                                               float *__restrict__ uc, size_t n, int B, Consts k,
+                                               const Scal *sc, double *part)
+{
+    const int b = blockIdx.y;
+    const Scal &S = sc[k.coupled ? 0 : b];
+    const size_t off = (size_t)b * n;
+    double uu = 0.0;
+    if (S.active) {
+        const float a = (float)S.alpha;
+        for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+            const float vi = v[off + i];
+            if (k.reg) {
+                const float r = k.sr * vi - a * ur[off + i];
+                ur[off + i] = r;
+                uu += (double)r * r;
+            }
+            const float c = k.sb * vi - a * uc[off + i];
+            uc[off + i] = c;
+            uu += (double)c * c;
+        }
+    }
+    block_sum_store(uu, part, Q_C, b, B);
+}
+
+__global__ void k_scal_u(Scal *sc, const double *part, int G, int B, Consts k)
+{
+    for (int g = threadIdx.x; g < G; g += blockDim.x) {
+        Scal &S = sc[g];
+        if (!S.active) continue;
+        const double uu = group_sum(part, Q_A, g, B, k.coupled) + group_sum(part, Q_C, g, B, k.coupled);
+        S.beta = sqrt(uu);
+        S.inv_u = 1.0 / S.beta;
+        S.norma = sqrt(S.norma * S.norma + S.alpha * S.alpha + S.beta * S.beta);
+        S.thet = -S.s * S.alpha;
+        const double rhot = S.c * S.alpha;
+        S.rho = sqrt(rhot * rhot + S.beta * S.beta);
+        S.c = rhot / S.rho;
+        S.s = -S.beta / S.rho;
+        S.phi = S.c * S.phibar;
+        if (S.phi == 0.0) S.stag = 1;
+        S.phibar = S.s * S.phibar;
+    }
+}
+
+// ||x_old||^2 -> Q_B, d = (v - thet*d)/rho, ||d||^2 -> Q_D, x += phi*d
+__global__ __launch_bounds__(256) void k_dx(const float *__restrict__ v, float *__restrict__ dv,
+                                            float *__restrict__ x, size_t n, int B, Consts k,
+                                            const Scal *sc, double *part)
+{
+    const int b = blockIdx.y;
+    const Scal &S = sc[k.coupled ? 0 : b];
+    const size_t off = (size_t)b * n;
+    double xx = 0.0, dd = 0.0;
+    if (S.active) {
+        const float thet = (float)S.thet, irho = (float)(1.0 / S.rho), phi = (float)S.phi;
+        for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+            const float xo = x[off + i];
+            xx += (double)xo * xo;
+            const float dn = (v[off + i] - thet * dv[off + i]) * irho;
+            dv[off + i] = dn;
+            dd += (double)dn * dn;
+            x[off + i] = xo + phi * dn;
+        }
+    }
+    block_sum_store(xx, part, Q_B, b, B);
+    block_sum_store(dd, part, Q_D, b, B);
+}
+
+__global__ void k_scal_v(Scal *sc, const double *part, int G, int B, Consts k, int ii)
+{
+    for (int g = threadIdx.x; g < G; g += blockDim.x) {
+        Scal &S = sc[g];
+        if (!S.active) continue;
+        const double xx = group_sum(part, Q_B, g, B, k.coupled);
+        const double dd = group_sum(part, Q_D, g, B, k.coupled);
+        if (fabs(S.phi) * sqrt(dd) < 2.220446049250313e-16 * sqrt(xx)) S.stag++;
+        else S.stag = 0;
+        const double vv = group_sum(part, Q_E, g, B, k.coupled);
+        S.alpha = sqrt(vv);
+        S.inv_v = 1.0 / S.alpha;
+        S.normr = fabs(S.s) * S.normr;
+        S.normar = S.alpha * fabs(S.s * S.phi);
+        S.iter = ii;
+        if (S.normr <= S.tolb) { S.flag = 0; S.active = 0; }
+        else if (S.normar / (S.norma * S.normr) <= k.tol) { S.flag = 0; S.active = 0; }
+        else if (S.stag >= 3) { S.flag = 3; S.active = 0; }
+    }
+}
+
+// w = max(x - vm, 0); vm += w - x  (nnls_L2_less.m:60,64).  Groups whose
+// right-hand side vanished (iter == -1) take x = 0 (MATLAB lsqr with b == 0).
+__global__ __launch_bounds__(256) void k_nnls_update(const float *__restrict__ x, float *__restrict__ w,
+                                                     float *__restrict__ vm, size_t n, int B, Consts k,
+                                                     const Scal *sc)
+{
+    const int b = blockIdx.y;
+    const Scal &S = sc[k.coupled ? 0 : b];
+    const bool zero = S.iter == -1;
+    const size_t off = (size_t)b * n;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const float xi = zero ? 0.0f : x[off + i];
+        const float vi = vm[off + i];
+        const float wn = fmaxf(xi - vi, 0.0f);
+        w[off + i] = wn;
+        vm[off + i] = vi + wn - xi;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_copy_or_zero(const float *__restrict__ src, float *__restrict__ dst,
+                                                      size_t n)
+{
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        dst[i] = src ? src[i] : 0.0f;
+}
+
+// residual / L-curve terms: ||f - y||^2 -> Q_A ; ||w||^2 -> Q_B
+__global__ __launch_bounds__(256) void k_resid(const float *__restrict__ f, const float *__restrict__ y,
+                                               size_t n, int B, double *part)
+{
+    const int b = blockIdx.y;
+    double e = 0.0;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const double r = (double)f[(size_t)b * n + i] - (double)y[(size_t)b * n + i];
+        e += r * r;
+    }
+    block_sum_store(e, part, Q_A, b, B);
+}
+
+__global__ __launch_bounds__(256) void k_sumsq(const float *__restrict__ x, size_t n, int B, int q, double *part)
+{
+    const int b = blockIdx.y;
+    double e = 0.0;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const double v = x[(size_t)b * n + i];
+        e += v * v;
+    }
+    block_sum_store(e, part, q, b, B);
+}
+
+__global__ void k_group_sums(const double *part, int q, int G, int B, int coupled, double *out)
+{
+    for (int g = threadIdx.x; g < G; g += blockDim.x) out[g] = group_sum(part, q, g, B, coupled);
+}
+
+// ---------------------------------------------------------------------------
+// host driver
+// ---------------------------------------------------------------------------
+struct Workspace {
+    float *x = nullptr, *w = nullptr, *vm = nullptr, *d = nullptr, *v = nullptr;
+    float *ur = nullptr, *uc = nullptr, *us = nullptr, *ts = nullptr, *tp = nullptr;
+    double *part = nullptr;
+    Scal *sc = nullptr;
+    double *gs = nullptr;
+    std::vector<void *> all;
+    ~Workspace()
+    {
+        for (void *p : all) (void)hipFree(p);
+    }
+    template <typename T>
+    int alloc(T **p, size_t n)
+    {
+        hipError_t e = hipMalloc((void **)p, (n ? n : 1) * sizeof(T));
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            return set_error(MBREC_ERR_HIP, "mbrec:alloc", "solver workspace allocation failed: %s",
+                             hipGetErrorString(e));
+        }
+        all.push_back(*p);
+        return MBREC_OK;
+    }
+};
+
+static inline dim3 vgrid(int B) { return dim3(kRedBlocks, B); }
+
+static int lsqr_solve(Plan *p, Workspace &ws, const float *y, int B, int G, int maxit, Consts k,
+                      hipStream_t s)
+{
+    const size_t np = (size_t)p->P, ns = (size_t)p->D * p->nT;
+    int rc;
+    // u = b - A_aug w
+    if ((rc = op_forward(p, ws.w, ws.ts, B, true, s))) return rc;
+    hipLaunchKernelGGL(k_init_sino, vgrid(B), dim3(256), 0, s, y, ws.ts, ws.us, ns, B, k, ws.part);
+    hipLaunchKernelGGL(k_init_pix, vgrid(B), dim3(256), 0, s, ws.w, ws.vm, ws.ur, ws.uc, ws.x, ws.d, np, B, k,
+                       ws.part);
+    hipLaunchKernelGGL(k_scal_init, dim3(1), dim3(256), 0, s, ws.sc, ws.part, G, B, k);
+    hipLaunchKernelGGL(k_scale_u, vgrid(B), dim3(256), 0, s, ws.us, ns, ws.ur, ws.uc, np, B, k, ws.sc);
+    // v = A_aug' u
+    if ((rc = op_transpose(p, ws.us, ws.tp, B, true, s))) return rc;
+    hipLaunchKernelGGL(k_v_update, vgrid(B), dim3(256), 0, s, ws.tp, ws.ur, ws.uc, ws.v, np, B, k, ws.sc, 1,
+                       ws.part);
+    hipLaunchKernelGGL(k_scal_v0, dim3(1), dim3(256), 0, s, ws.sc, ws.part, G, B, k);
+    hipLaunchKernelGGL(k_scale_v, vgrid(B), dim3(256), 0, s, ws.v, np, B, k, ws.sc);
+    for (int ii = 1; ii <= maxit; ++ii) {
+        if ((rc = op_forward(p, ws.v, ws.ts, B, true, s))) return rc;
+        hipLaunchKernelGGL(k_u_sino, vgrid(B), dim3(256), 0, s, ws.ts, ws.us, ns, B, k, ws.sc, ws.part);
+        hipLaunchKernelGGL(k_u_pix, vgrid(B), dim3(256), 0, s, ws.v, ws.ur, ws.uc, np, B, k, ws.sc, ws.part);
+        hipLaunchKernelGGL(k_scal_u, dim3(1), dim3(256), 0, s, ws.sc, ws.part, G, B, k);
+        hipLaunchKernelGGL(k_scale_u, vgrid(B), dim3(256), 0, s, ws.us, ns, ws.ur, ws.uc, np, B, k, ws.sc);
+        hipLaunchKernelGGL(k_dx, vgrid(B), dim3(256), 0, s, ws.v, ws.d, ws.x, np, B, k, ws.sc, ws.part);
+        if ((rc = op_transpose(p, ws.us, ws.tp, B, true, s))) return rc;
+        hipLaunchKernelGGL(k_v_update, vgrid(B), dim3(256), 0, s, ws.tp, ws.ur, ws.uc, ws.v, np, B, k, ws.sc, 0,
+                           ws.part);
+        hipLaunchKernelGGL(k_scal_v, dim3(1), dim3(256), 0, s, ws.sc, ws.part, G, B, k, ii);
+        if (ii < maxit) hipLaunchKernelGGL(k_scale_v, vgrid(B), dim3(256), 0, s, ws.v, np, B, k, ws.sc);
+    }
+    MBREC_HIP(hipGetLastError());
+    return MBREC_OK;
+}
+
+int nnls_l2(Plan *p, const float *y, int B, int coupled, int num_iter, double lambdaL2,
+            const float *init, float *rec, double *lcurve, hipStream_t s)
+{
+    const size_t np = (size_t)p->P, ns = (size_t)p->D * p->nT;
+    const int G = coupled ? 1 : B;
+    Workspace ws;
+    int rc;
+    const int reg = lambdaL2 > 0.0;
+    if ((rc = ws.alloc(&ws.x, np * B)) || (rc = ws.alloc(&ws.vm, np * B)) ||
+        (rc = ws.alloc(&ws.d, np * B)) || (rc = ws.alloc(&ws.v, np * B)) ||
+        (rc = ws.alloc(&ws.uc, np * B)) || (rc = ws.alloc(&ws.tp, np * B)) ||
+        (rc = ws.alloc(&ws.ur, reg ? np * B : 1)) || (rc = ws.alloc(&ws.us, ns * B)) ||
+        (rc = ws.alloc(&ws.ts, ns * B)) || (rc = ws.alloc(&ws.part, (size_t)kMaxQ * B * kRedBlocks)) ||
+        (rc = ws.alloc(&ws.sc, G)) || (rc = ws.alloc(&ws.gs, 2 * (size_t)G)))
+        return rc;
+    ws.w = rec;   // the outer iterate lives in the output buffer
+    Consts k;
+    k.sl = 1.0f;                                   // lambda = 1 (rec_nn_with_L2_reg.m:33)
+    k.sb = (float)std::sqrt(1e-3);                 // beta = 1e-3 (:34)
+    k.sr = (float)(reg ? std::sqrt(lambdaL2) : 0.0);
+    k.reg = reg;
+    k.tol = 1e-8;                                  // inner_tol (nnls_L2_less.m:33)
+    k.coupled = coupled;
+    const int nb = 1024;
+    // w0 = init (warm start) or 0 (nnls_L2_less.m:28); v0 = 0 (:29)
+    hipLaunchKernelGGL(k_copy_or_zero, dim3(nb), dim3(256), 0, s, init, ws.w, np * B);
+    hipLaunchKernelGGL(k_copy_or_zero, dim3(nb), dim3(256), 0, s, (const float *)nullptr, ws.vm, np * B);
+    for (int it = 0; it < num_iter; ++it) {
+        if ((rc = lsqr_solve(p, ws, y, B, G, 2, k, s))) return rc;
+        hipLaunchKernelGGL(k_nnls_update, vgrid(B), dim3(256), 0, s, ws.x, ws.w, ws.vm, np, B, k, ws.sc);
+    }
+    MBREC_HIP(hipGetLastError());
+    if (lcurve) {
+        // rec_nn_with_L2_reg.m:51,54: ||A rec - y||^2 and ||RegL2 rec||^2 per group
+        if ((rc = op_forward(p, ws.w, ws.ts, B, true, s))) return rc;
+        hipLaunchKernelGGL(k_resid, vgrid(B), dim3(256), 0, s, ws.ts, y, ns, B, ws.part);
+        hipLaunchKernelGGL(k_sumsq, vgrid(B), dim3(256), 0, s, ws.w, np, B, (int)Q_B, ws.part);
+        hipLaunchKernelGGL(k_group_sums, dim3(1), dim3(256), 0, s, ws.part, (int)Q_A, G, B, coupled, ws.gs);
+        hipLaunchKernelGGL(k_group_sums, dim3(1), dim3(256), 0, s, ws.part, (int)Q_B, G, B, coupled, ws.gs + G);
+        std::vector<double> h(2 * (size_t)G);
+        MBREC_HIP(hipMemcpyAsync(h.data(), ws.gs, sizeof(double) * 2 * G, hipMemcpyDeviceToHost, s));
+        MBREC_HIP(hipStreamSynchronize(s));
+        for (int g = 0; g < G; ++g) {
+            lcurve[2 * g] = h[g];
+            lcurve[2 * g + 1] = reg ? h[G + g] : 0.0;
+        }
+    } else {
+        MBREC_HIP(hipStreamSynchronize(s));   // workspace is freed on return
+    }
+    return MBREC_OK;
+}
+
+}  // namespace mbrec
+
+using namespace mbrec;
+
+extern "C" int mbrec_rec_nn_with_L2_reg_dev(mbrec_plan h, const float *data, int B, int coupled,
+                                            int num_iter, double lambdaL2, const float *init,
+                                            float *rec, double *lcurve, mbrec_stream s)
+{
+    if (!h || !data || !rec || B <= 0 || num_iter < 0)
+        return set_error(MBREC_ERR_ARG, "mbrec:args", "rec_nn_with_L2_reg: invalid arguments");
+    return nnls_l2(&h->p, data, B, coupled, num_iter, lambdaL2, init, rec, lcurve, (hipStream_t)s);
+}

@@ -1,0 +1,242 @@
+"""MSOTmodel: geometry tables and the normalised forward / transpose operators
+(reconstruction_model_based/models/MSOTmodel.m).
+
+Setup (travel times, interpolation radius, impulse response, SIR tables,
+prefilter) runs on the host once; the tables are then uploaded into a device
+plan of libmbrec and ``Funcs.applyForward`` / ``Funcs.applyTranspose`` run the
+HIP operators (MSOTmodel.m:285-286).  Arrays use MATLAB memory order with
+reversed axes: images (W, Nx, Nz), sinograms (W, D, nT), travel times (D, Nx, Nz).
+"""
+from __future__ import annotations
+
+import ctypes as C
+from types import SimpleNamespace
+
+import numpy as np
+
+from . import _lib
+from .discretization import Discretization
+from .irf import impulse_response_with_sir, impulse_response_without_sir
+from .probe import Probe
+from .sir import sir_tables_for_model
+
+
+def travel_time_indices(disc: Discretization, probe: Probe, c: float, ncrop: int) -> np.ndarray:
+    """Single speed of sound (refraction.m:20-24 same-side branch, MSOTmodel.m:380-381):
+    tau[d, x] = |p_x - q_d| / c * fs - delay - ncrop, shape (D, Nx, Nz)."""
+    X = disc.region["xPositionsFov"].T          # (Nx, Nz): memory order x = iz + Nz*ix
+    Z = disc.region["zPositionsFov"].T
+    xd = probe.detector.xPositionsOfTransducers
+    zd = probe.detector.zPositionsOfTransducers
+    dx = xd[:, None, None] - X[None]
+    dz = zd[:, None, None] - Z[None]
+    t = np.sqrt(dx * dx + dz * dz) / c
+    return t * probe.DAC.frequency - probe.DAC.delayBeforeRecording - ncrop
+
+
+class DevicePlan:
+    """Owns one libmbrec plan (device-resident tables)."""
+
+    def __init__(self, Nz, Nx, D, nT, R, tt, sir=None, ir=None, prefilter=None, sigma=1.0):
+        L = _lib.lib()
+        _lib.check(L.mbrec_init(0))
+        h = C.c_void_p()
+        tt = np.ascontiguousarray(tt, dtype=np.float64)
+        _lib.check(L.mbrec_plan_create(C.byref(h), int(Nz), int(Nx), int(D), int(nT), float(R),
+                                       tt.ctypes.data))
+        self.h = h
+        self.Nz, self.Nx, self.D, self.nT, self.R = Nz, Nx, D, nT, R
+        self.P = Nz * Nx
+        if sir is not None:
+            sidx, table, se = sir
+            sidx = np.ascontiguousarray(sidx, dtype=np.int32)
+            table = np.ascontiguousarray(table, dtype=np.float64)
+            se = np.ascontiguousarray(se, dtype=np.int32)
+            _lib.check(L.mbrec_plan_set_sir(h, sidx.ctypes.data, table.ctypes.data,
+                                            int(table.shape[1]), int(table.shape[0]), se.ctypes.data))
+        if ir is not None:
+            ir = np.ascontiguousarray(ir, dtype=np.float64)
+            per_det = 1 if ir.ndim == 2 else 0
+            Lir = ir.shape[-1]
+            _lib.check(L.mbrec_plan_set_ir(h, ir.ctypes.data, int(Lir), per_det))
+        if prefilter is not None:
+            kz, kx = (np.ascontiguousarray(k, dtype=np.float64) for k in prefilter)
+            _lib.check(L.mbrec_plan_set_prefilter(h, kz.ctypes.data, (len(kz) - 1) // 2,
+                                                  kx.ctypes.data, (len(kx) - 1) // 2))
+        self.set_sigma(sigma)
+
+    def set_sigma(self, sigma):
+        self.sigma = float(sigma)
+        _lib.check(_lib.lib().mbrec_plan_set_sigma(self.h, self.sigma))
+
+    def close(self):
+        if getattr(self, "h", None):
+            _lib.lib().mbrec_plan_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def device_bytes(self):
+        return int(_lib.lib().mbrec_plan_device_bytes(self.h))
+
+    @property
+    def tau_frac_bits(self):
+        return int(_lib.lib().mbrec_plan_tau_frac_bits(self.h))
+
+    # device-tensor operators --------------------------------------------------
+    def _run(self, fn, src, dst, B):
+        _lib.check(fn(self.h, src.data_ptr(), dst.data_ptr(), int(B), _lib.stream_handle()))
+
+    def forward(self, img, out=None, model=True):
+        import torch
+        B = img.numel() // self.P
+        if out is None:
+            out = torch.empty((B, self.D, self.nT), dtype=torch.float32, device=img.device)
+        L = _lib.lib()
+        self._run(L.mbrec_model_forward_dev if model else L.mbrec_mex_forward_dev,
+                  img.contiguous(), out, B)
+        return out
+
+    def transpose(self, sino, out=None, model=True):
+        import torch
+        B = sino.numel() // (self.D * self.nT)
+        if out is None:
+            out = torch.empty((B, self.Nx, self.Nz), dtype=torch.float32, device=sino.device)
+        L = _lib.lib()
+        self._run(L.mbrec_model_transpose_dev if model else L.mbrec_mex_transpose_dev,
+                  sino.contiguous(), out, B)
+        return out
+
+
+class MSOTmodel:
+    """MSOTmodel(probe, discretization, speed_of_sound, include_sir, use_single_sos,
+    num_cropped_samples, filt_cutoff_min, filt_cutoff_max, model_normalization_factor)
+    (MSOTmodel.m:18-48)."""
+
+    def __init__(self, probe: Probe, discretization: Discretization, speed_of_sound_below_membrane: float,
+                 include_sir: bool, use_single_speed_of_sound: bool = True,
+                 num_cropped_samples_at_sinogram_start: int = 0, filt_cutoff_min: float = 1e5,
+                 filt_cutoff_max: float = 12e6, model_normalization_factor=None,
+                 sigma_iterations: int = 60):
+        if not use_single_speed_of_sound:
+            raise NotImplementedError("dual speed of sound travel times are not provided")
+        self.Probe = probe
+        self.Discretization = discretization
+        self.Medium = SimpleNamespace(speedOfSound=float(speed_of_sound_below_membrane))
+        self.includeSir = bool(include_sir)
+        self.useSingleSpeedOfSound = True
+        ncrop = int(num_cropped_samples_at_sinogram_start)
+        self.DataPreprocessing = SimpleNamespace(numCroppedSamplesAtSinogramStart=ncrop,
+                                                 filtCutoffMin=filt_cutoff_min,
+                                                 filtCutoffMax=filt_cutoff_max)
+        c = self.Medium.speedOfSound
+        fs = probe.DAC.frequency
+        nT = int(probe.DAC.numRecordedSamplesPerTransducer - ncrop)
+        tt = travel_time_indices(discretization, probe, c, ncrop)
+        Z = discretization.region["zPositionsFov"]
+        R = (Z[1, 0] - Z[0, 0]) / (c / fs)                      # MSOTmodel.m:36-38
+        self.Data = SimpleNamespace(numTimeSamplesPerTransducer=nT, absoluteTravelTimeIndices=tt,
+                                    circularInterpolationRadiusInSamples=R)
+        if self.includeSir:
+            ir = impulse_response_with_sir(probe.detector.EIR, fs, filt_cutoff_min, filt_cutoff_max)
+            X = discretization.region["xPositionsFov"].T.ravel()
+            Zf = discretization.region["zPositionsFov"].T.ravel()
+            res = min(discretization.dx, discretization.dz)
+            table, se, sidx, info = sir_tables_for_model(
+                X, Zf, probe.detector.xPositionsOfTransducers, probe.detector.zPositionsOfTransducers,
+                res, c, fs, probe.detector.elementWidth, probe.detector.numSubElements,
+                discretization.pixelModel)
+            self.Data.SPMR = table
+            self.Data.SPMRTimestepIndexStartAndEnd = se
+            self.Data.SPMRIndices = sidx
+            self.Data.SPMRGrid = info
+            sir = (sidx, table, se)
+        else:
+            ir = impulse_response_without_sir(probe.detector.EIR, fs, c, discretization.pixelModel,
+                                              filt_cutoff_min, filt_cutoff_max)
+            sir = None
+        self.Data.IR = ir
+        D = probe.detector.numOfTransducers
+        self.plan = DevicePlan(discretization.Nz, discretization.Nx, D, nT, R,
+                               tt.reshape(D, -1), sir=sir, ir=ir,
+                               prefilter=(discretization.kz, discretization.kx), sigma=1.0)
+        if model_normalization_factor is None:
+            model_normalization_factor = self.estimate_sigma(sigma_iterations)
+        self.normalizationFactor = float(model_normalization_factor)
+        self.plan.set_sigma(self.normalizationFactor)
+        self.Funcs = SimpleNamespace(applyForward=self.applyForward, applyTranspose=self.applyTranspose)
+
+    # sizes
+    @property
+    def shape_image(self):
+        return (self.Discretization.Nx, self.Discretization.Nz)
+
+    @property
+    def shape_sinogram(self):
+        return (self.Probe.detector.numOfTransducers, self.Data.numTimeSamplesPerTransducer)
+
+    def estimate_sigma(self, iters: int = 60, seed: int = 0) -> float:
+        """Largest singular value of the unnormalised operator by power iteration on A'A
+        (replaces svds(..., 1, 'largest') at MSOTmodel.m:278; FP64 norms)."""
+        import torch
+        self.plan.set_sigma(1.0)
+        g = torch.Generator(device="cpu").manual_seed(seed)
+        x = torch.rand((1, *self.shape_image), generator=g, dtype=torch.float32).cuda()
+        x /= torch.linalg.vector_norm(x.double()).float()
+        lam = 0.0
+        for _ in range(iters):
+            y = self.plan.transpose(self.plan.forward(x))
+            nrm = torch.linalg.vector_norm(y.double()).item()
+            lam_new = nrm
+            x = (y.double() / nrm).float()
+            if abs(lam_new - lam) <= 1e-9 * lam_new:
+                lam = lam_new
+                break
+            lam = lam_new
+        return float(np.sqrt(lam))
+
+    # operators (MSOTmodel.m:285-286) --------------------------------------------
+    def applyForward(self, p0):
+        return self._apply(p0, forward=True)
+
+    def applyTranspose(self, sig):
+        return self._apply(sig, forward=False)
+
+    def _apply(self, x, forward: bool):
+        import torch
+        host = isinstance(x, np.ndarray)
+        t = torch.as_tensor(np.ascontiguousarray(x, dtype=np.float32)).cuda() if host else x
+        if t.dtype != torch.float32:
+            raise TypeError("device tensors must be float32")
+        t = t.contiguous()
+        if forward:
+            B = t.numel() // self.Discretization.numberOfPixels
+            out = self.plan.forward(t.view(B, -1))
+            out = out.view(B, *self.shape_sinogram)
+        else:
+            B = t.numel() // (self.shape_sinogram[0] * self.shape_sinogram[1])
+            out = self.plan.transpose(t.view(B, -1))
+            out = out.view(B, *self.shape_image)
+        if host:
+            out = out.cpu().numpy().astype(np.float64)
+            if np.ndim(x) in (2,) or (np.ndim(x) == 1):
+                out = out[0]
+        return out
+
+
+def define_model_for_reconstruction(field_of_view, number_of_grid_points_fov, device_probe_id,
+                                    use_eir, use_indiv_eir, use_sir, use_single_speed_of_sound,
+                                    speed_of_sound_tissue, num_cropped_samples_at_sinogram_start,
+                                    filt_cutoff_min, filt_cutoff_max, model_normalization_factor=None,
+                                    custom_pixel_model=None):
+    """define_model_for_reconstruction.m:1-11."""
+    disc = Discretization(field_of_view, number_of_grid_points_fov, custom_pixel_model)
+    probe = Probe(device_probe_id, use_eir, use_indiv_eir)
+    return MSOTmodel(probe, disc, speed_of_sound_tissue, use_sir, use_single_speed_of_sound,
+                     num_cropped_samples_at_sinogram_start, filt_cutoff_min, filt_cutoff_max,
+                     model_normalization_factor)

@@ -1,0 +1,84 @@
+"""Non-negative reconstruction methods (reconstruction_model_based/reconstruction/).
+
+The solver loops run on the device (libmbrec); these functions mirror the
+reference's MATLAB entry points: same argument order, same constants, same
+outputs.  ``data_filt`` is (W, D, nT) or (D, nT) (MATLAB [nT x D x W] memory
+order); ``rec_img`` is (W, Nx, Nz) or (Nx, Nz).
+"""
+from __future__ import annotations
+
+import time
+
+import numpy as np
+
+from . import _lib
+
+
+def _is_identity(f):
+    return f is None or f == "identity" or getattr(f, "is_identity", False)
+
+
+def identity(x):
+    return x
+
+
+identity.is_identity = True
+
+
+def _to_device(x):
+    import torch
+    if isinstance(x, np.ndarray):
+        return torch.as_tensor(np.ascontiguousarray(x, dtype=np.float32)).cuda(), True
+    if x.dtype != torch.float32:
+        raise TypeError("device tensors must be float32")
+    return x.contiguous(), False
+
+
+def rec_nn_with_L2_reg(model, data_filt, num_iter_nn_outer, lambdaL2=0.0, RegL2=None, RegL2T=None,
+                       lambdaL2_2=None, RegL2_2=None, RegL2T_2=None, *, coupled=True, init=None):
+    """rec_nn_with_L2_reg.m:1-66 (RegL2 = identity).
+
+    Returns (rec_img, lCurveErrImg, lCurveErrReg, lCurveErrReg_2, err, t) like the
+    reference; ``err`` stays zero (calculate_objective_per_iter = false, :35) and
+    ``t`` holds the solve's wall time split evenly over the outer iterations.
+    ``coupled=True`` is the reference's semantics for a multi-wavelength call
+    (shared LSQR scalars, nnls_L2_less.m:56 on stacked vectors); ``coupled=False``
+    solves every sinogram independently.  ``init`` (optional image) seeds the
+    outer iterate (warm start); the reference's own start value is ignored by
+    nnls_L2_less.m:28-29 and so is not computed here.
+    """
+    import torch
+    if lambdaL2 and lambdaL2 > 0 and not (_is_identity(RegL2) and _is_identity(RegL2T)):
+        raise NotImplementedError("only RegL2 = identity is provided on the device")
+    if lambdaL2_2 and lambdaL2_2 > 0:
+        raise NotImplementedError("the second (Laplacian) L2 term is not provided on the device")
+    y, host = _to_device(data_filt)
+    D, nT = model.shape_sinogram
+    B = y.numel() // (D * nT)
+    y = y.view(B, D, nT)
+    rec = torch.empty((B, *model.shape_image), dtype=torch.float32, device=y.device)
+    initp = None
+    if init is not None:
+        it, _ = _to_device(init)
+        if it.numel() != rec.numel():
+            raise ValueError("init has the wrong number of pixels")
+        initp = it.contiguous()
+    G = 1 if coupled else B
+    lc = np.zeros(2 * G, dtype=np.float64)
+    L = _lib.lib()
+    t0 = time.perf_counter()
+    _lib.check(L.mbrec_rec_nn_with_L2_reg_dev(
+        model.plan.h, y.data_ptr(), int(B), int(bool(coupled)), int(num_iter_nn_outer),
+        float(lambdaL2 or 0.0), initp.data_ptr() if initp is not None else None, rec.data_ptr(),
+        lc.ctypes.data, _lib.stream_handle()))
+    el = time.perf_counter() - t0
+    errimg = lc[0::2] if not coupled else lc[0]
+    errreg = lc[1::2] if not coupled else lc[1]
+    err = np.zeros(int(num_iter_nn_outer))
+    t = np.full(int(num_iter_nn_outer), el / max(1, int(num_iter_nn_outer)))
+    if host:
+        out = rec.cpu().numpy().astype(np.float64)
+        if np.ndim(data_filt) == 2:
+            out = out[0]
+        rec = out
+    return rec, errimg, errreg, 0.0, err, t

@@ -1,0 +1,713 @@
+/*
+ * mbrec_oracle.c -- FP64 CPU restatement of the reference's hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is the checker: only tests/,
+ * __graft_entry__.smoke() and bench.py's cpu_baseline leg may load it, and
+ * never as the thing measured or shipped.  The product path (the HIP library
+ * under mb-rec-msot_amd/csrc) does not link or call anything here.
+ *
+ * Every function restates a reference routine in plain C with the same loop
+ * order and the same FP64 expressions, and cites the file:line it follows
+ * (paths relative to the reference checkout).  Built with the reference MEX
+ * flags (reconstruction_model_based/models/compile_mex_files.m:2-6):
+ *     -O3 -fopenmp -mavx -fwrapv -DNDEBUG
+ * and the same parallel decomposition (forward over detectors, SIR transpose
+ * over pixels, DAS transpose serial over detectors x parallel over pixels).
+ *
+ * Layouts (all column-major as in MATLAB, leading index fastest):
+ *   MEX image      [W x P]        image[x*W + w]
+ *   MEX sinogram   [W x nT x D]   sig[d*W*nT + t*W + w]
+ *   travel times   [P x D]        tt[d*P + x]          (0-based fractional sample)
+ *   SIR indices    [P x D] int32  sirIdx[d*P + x]      (1-based)
+ *   SIR table      [Ls x Nsir]    sir[k*Ls + s]
+ *   SIR windows    [2 x Nsir]     se[2k], se[2k+1]     (1-based, inclusive)
+ *   model image    [P x W]        img[w*P + x],  x = iz + Nz*ix
+ *   model sinogram [nT x D x W]   sino[(w*D + d)*nT + t]
+ *
+ * Deliberate difference from the reference: the two delay-and-sum loops
+ * (forwardDelayAndSumMex.cpp:80-99, transposeDelayAndSumMex.cpp:75-94) have
+ * no sample-range check and read/write out of bounds for travel times near
+ * the sinogram ends.  Here out-of-range samples are dropped, exactly as the
+ * SIR loops do (forwardMex.cpp:139, transposeMex.cpp:298).
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+#include <float.h>
+#include <omp.h>
+
+#define ORC_API __attribute__((visibility("default")))
+
+/* ------------------------------------------------------------------------- */
+/* MEX loops                                                                  */
+/* ------------------------------------------------------------------------- */
+
+/* forwardDelayAndSumMex.cpp:73-101 (out zero-filled like mxCreateNumericArray, :64) */
+ORC_API void orc_forward_das(const double *image, int W, int P, const double *tt,
+                             int D, int nT, double R, double *out)
+{
+    memset(out, 0, sizeof(double) * (size_t)W * nT * D);
+#pragma omp parallel for schedule(static)
+    for (int d = 0; d < D; ++d) {
+        double *row = out + (size_t)d * W * nT;
+        const double *ttd = tt + (size_t)d * P;
+        for (int x = 0; x < P; ++x) {
+            const double tau = ttd[x];
+            const int first = (int)ceil(tau - R);
+            const int last = (int)floor(tau + R);
+            for (int r = first; r <= last; ++r) {
+                if (r < 0 || r >= nT) continue;
+                const double factor = 1.0 - fabs(0.0 + r - tau) / R;
+                for (int w = 0; w < W; ++w)
+                    row[(size_t)r * W + w] += factor * image[(size_t)x * W + w];
+            }
+        }
+    }
+}
+
+/* transposeDelayAndSumMex.cpp:171-200: serial detectors, parallel pixels */
+ORC_API void orc_transpose_das(const double *sig, int W, int nT, int D,
+                               const double *tt, int P, double R, double *out)
+{
+    memset(out, 0, sizeof(double) * (size_t)W * P);
+    for (int d = 0; d < D; ++d) {
+        const double *row = sig + (size_t)d * W * nT;
+        const double *ttd = tt + (size_t)d * P;
+#pragma omp parallel for schedule(static)
+        for (int x = 0; x < P; ++x) {
+            const double tau = ttd[x];
+            const int first = (int)ceil(tau - R);
+            const int last = (int)floor(tau + R);
+            for (int r = first; r <= last; ++r) {
+                if (r < 0 || r >= nT) continue;
+                const double factor = 1.0 - fabs(0.0 + r - tau) / R;
+                for (int w = 0; w < W; ++w)
+                    out[(size_t)x * W + w] = out[(size_t)x * W + w] + factor * row[(size_t)r * W + w];
+            }
+        }
+    }
+}
+
+/* forwardMex.cpp:118-161 */
+ORC_API void orc_forward_sir(const double *image, int W, int P, const int32_t *sirIdx,
+                             const double *tt, int D, const double *sir, int Ls,
+                             int nT, const int32_t *se, double R, double *out)
+{
+    memset(out, 0, sizeof(double) * (size_t)W * nT * D);
+#pragma omp parallel for schedule(static)
+    for (int d = 0; d < D; ++d) {
+        double *row = out + (size_t)d * W * nT;
+        for (int x = 0; x < P; ++x) {
+            const int k = sirIdx[(size_t)d * P + x] - 1;
+            const double tau = tt[(size_t)d * P + x];
+            const int first = (int)ceil(tau - R);
+            const int last = (int)floor(tau + R);
+            for (int s = se[2 * (size_t)k] - 1; s <= se[2 * (size_t)k + 1] - 1; ++s) {
+                const double h = sir[(size_t)k * Ls + s];
+                for (int r = first; r <= last; ++r) {
+                    const int t = r + s - Ls / 2;
+                    if (t >= 0 && t < nT) {
+                        const double factor = 1.0 - fabs(0.0 + r - tau) / R;
+                        for (int w = 0; w < W; ++w)
+                            row[(size_t)t * W + w] += factor * h * image[(size_t)x * W + w];
+                    }
+                }
+            }
+        }
+    }
+}
+
+/* transposeMex.cpp:277-320 */
+ORC_API void orc_transpose_sir(const double *sig, int W, int nT, int D,
+                               const int32_t *sirIdx, const double *tt, int P,
+                               const double *sir, int Ls, const int32_t *se,
+                               double R, double *out)
+{
+    memset(out, 0, sizeof(double) * (size_t)W * P);
+#pragma omp parallel for schedule(static)
+    for (int x = 0; x < P; ++x) {
+        for (int d = 0; d < D; ++d) {
+            const double *row = sig + (size_t)d * W * nT;
+            const int k = sirIdx[(size_t)d * P + x] - 1;
+            const double tau = tt[(size_t)d * P + x];
+            const int first = (int)ceil(tau - R);
+            const int last = (int)floor(tau + R);
+            for (int s = se[2 * (size_t)k] - 1; s <= se[2 * (size_t)k + 1] - 1; ++s) {
+                const double h = sir[(size_t)k * Ls + s];
+                for (int r = first; r <= last; ++r) {
+                    const int t = r + s - Ls / 2;
+                    if (t >= 0 && t < nT) {
+                        const double factor = 1.0 - fabs(0.0 + r - tau) / R;
+                        for (int w = 0; w < W; ++w)
+                            out[(size_t)x * W + w] = out[(size_t)x * W + w] + factor * h * row[(size_t)t * W + w];
+                    }
+                }
+            }
+        }
+    }
+}
+
+/* ------------------------------------------------------------------------- */
+/* EIR convolution (MSOTmodel.m:259-269)                                       */
+/* ------------------------------------------------------------------------- */
+/* convolve_with_IR: zero-padded FFT of length nT+Lir-1 is the full linear
+ * convolution; cropping half+1 : end-half keeps y[t] = sum_m s[m] IR[t+half-m].
+ * ir is [Lir] (shared) or [D x Lir] (ir_per_det, MSOTmodel.m:260 broadcast). */
+ORC_API void orc_eir_conv(const double *in, int W, int D, int nT, const double *ir,
+                          int Lir, int ir_per_det, double *out)
+{
+    const int half = Lir / 2;
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int w = 0; w < W; ++w)
+        for (int d = 0; d < D; ++d) {
+            const double *s = in + ((size_t)w * D + d) * nT;
+            const double *h = ir + (ir_per_det ? (size_t)d * Lir : 0);
+            double *y = out + ((size_t)w * D + d) * nT;
+            for (int t = 0; t < nT; ++t) {
+                double acc = 0.0;
+                int mlo = t + half - (Lir - 1); if (mlo < 0) mlo = 0;
+                int mhi = t + half; if (mhi > nT - 1) mhi = nT - 1;
+                for (int m = mlo; m <= mhi; ++m) acc += s[m] * h[t + half - m];
+                y[t] = acc;
+            }
+        }
+}
+
+/* transpose_convolve_with_IR: same crop with flip(IR):
+ * z[t] = sum_m y[m] IRflip[t+half-m] = sum_m y[m] IR[Lir-1-half-t+m]. */
+ORC_API void orc_eir_corr(const double *in, int W, int D, int nT, const double *ir,
+                          int Lir, int ir_per_det, double *out)
+{
+    const int half = Lir / 2;
+    const int off = Lir - 1 - half;
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int w = 0; w < W; ++w)
+        for (int d = 0; d < D; ++d) {
+            const double *s = in + ((size_t)w * D + d) * nT;
+            const double *h = ir + (ir_per_det ? (size_t)d * Lir : 0);
+            double *y = out + ((size_t)w * D + d) * nT;
+            for (int t = 0; t < nT; ++t) {
+                double acc = 0.0;
+                int mlo = t - off; if (mlo < 0) mlo = 0;
+                int mhi = t - off + Lir - 1; if (mhi > nT - 1) mhi = nT - 1;
+                for (int m = mlo; m <= mhi; ++m) acc += s[m] * h[off - t + m];
+                y[t] = acc;
+            }
+        }
+}
+
+/* ------------------------------------------------------------------------- */
+/* Prefilter (Discretization.m:47-72, shift_inv_proj.m:66-71)                  */
+/* ------------------------------------------------------------------------- */
+/* With lattice [1 1] the prefilter conj(phi_hat ./ fftn(autocorr)) is 1./phi_hat.
+ * For a separable pixel model phi = gz (x) gx this is (1./gz_hat) (x) (1./gx_hat),
+ * i.e. a separable circular convolution with k = real(ifft(1./fft(g))).
+ * g is the pixel profile already fftshift-ed (peak at index 0 for even N).
+ * The DFTs are evaluated directly in FP64 (O(N^2), N <= a few thousand). */
+ORC_API void orc_prefilter_kernel_1d(const double *g, int N, double *k)
+{
+    double *re = (double *)malloc(sizeof(double) * N);
+    double *im = (double *)malloc(sizeof(double) * N);
+    for (int f = 0; f < N; ++f) {
+        double a = 0.0, b = 0.0;
+        for (int n = 0; n < N; ++n) {
+            const double ang = -2.0 * M_PI * (double)(((long long)f * n) % N) / N;
+            a += g[n] * cos(ang);
+            b += g[n] * sin(ang);
+        }
+        const double den = a * a + b * b;   /* 1/(a+ib) = (a-ib)/den */
+        re[f] = a / den;
+        im[f] = -b / den;
+    }
+    for (int n = 0; n < N; ++n) {
+        double a = 0.0;
+        for (int f = 0; f < N; ++f) {
+            const double ang = 2.0 * M_PI * (double)(((long long)f * n) % N) / N;
+            a += re[f] * cos(ang) - im[f] * sin(ang);
+        }
+        k[n] = a / N;   /* 'symmetric' ifft: real part */
+    }
+    free(re);
+    free(im);
+}
+
+/* Circular convolution coeffs[n] = sum_m p[m] K[(n-m) mod N], along z then x,
+ * per wavelength.  transp != 0 uses the conjugate prefilter, i.e. K[(m-n) mod N]
+ * (Discretization.m:64-72). */
+ORC_API void orc_prefilter_apply(const double *img, int W, int Nz, int Nx,
+                                 const double *kz, const double *kx, int transp,
+                                 double *out)
+{
+    const size_t P = (size_t)Nz * Nx;
+    double *tmp = (double *)malloc(sizeof(double) * P * W);
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int w = 0; w < W; ++w)
+        for (int ix = 0; ix < Nx; ++ix) {
+            const double *col = img + w * P + (size_t)ix * Nz;
+            double *o = tmp + w * P + (size_t)ix * Nz;
+            for (int iz = 0; iz < Nz; ++iz) {
+                double acc = 0.0;
+                for (int m = 0; m < Nz; ++m) {
+                    int j = transp ? (m - iz) : (iz - m);
+                    j %= Nz; if (j < 0) j += Nz;
+                    acc += col[m] * kz[j];
+                }
+                o[iz] = acc;
+            }
+        }
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int w = 0; w < W; ++w)
+        for (int ix = 0; ix < Nx; ++ix) {
+            double *o = out + w * P + (size_t)ix * Nz;
+            for (int iz = 0; iz < Nz; ++iz) {
+                double acc = 0.0;
+                for (int m = 0; m < Nx; ++m) {
+                    int j = transp ? (m - ix) : (ix - m);
+                    j %= Nx; if (j < 0) j += Nx;
+                    acc += tmp[w * P + (size_t)m * Nz + iz] * kx[j];
+                }
+                o[iz] = acc;
+            }
+        }
+    free(tmp);
+}
+
+/* ------------------------------------------------------------------------- */
+/* Composed operator (MSOTmodel.m:189-257, 285-286)                            */
+/* ------------------------------------------------------------------------- */
+typedef struct {
+    int Nz, Nx, D, nT;
+    const double *tt;        /* [P x D] */
+    double R;
+    int use_sir;
+    const int32_t *sirIdx;   /* [P x D] */
+    const double *sir;       /* [Ls x Nsir] */
+    int Ls;
+    const int32_t *se;       /* [2 x Nsir] */
+    const double *kz, *kx;   /* prefilter kernels, NULL = identity */
+    const double *ir;        /* [Lir] or [D x Lir], NULL = identity */
+    int Lir, ir_per_det;
+    double sigma;            /* model_normalization_factor */
+} orc_model;
+
+static void permute_PW_to_WP(const double *a, int P, int W, double *b)
+{   /* [P x W] (pixel fastest) -> [W x P] (wavelength fastest) */
+    for (int w = 0; w < W; ++w)
+        for (int x = 0; x < P; ++x) b[(size_t)x * W + w] = a[(size_t)w * P + x];
+}
+
+/* forward / forwardWithoutSIR then applyForward = forward(p0)/sigma */
+ORC_API void orc_model_forward(const orc_model *m, const double *img, int W, double *sino)
+{
+    const int P = m->Nz * m->Nx;
+    const size_t nS = (size_t)m->nT * m->D;
+    double *c = (double *)malloc(sizeof(double) * (size_t)P * W);
+    double *cw = (double *)malloc(sizeof(double) * (size_t)P * W);
+    double *s = (double *)malloc(sizeof(double) * nS * W);
+    double *sp = (double *)malloc(sizeof(double) * nS * W);
+    if (m->kz) orc_prefilter_apply(img, W, m->Nz, m->Nx, m->kz, m->kx, 0, c);
+    else memcpy(c, img, sizeof(double) * (size_t)P * W);
+    permute_PW_to_WP(c, P, W, cw);                                   /* :194 */
+    if (m->use_sir)
+        orc_forward_sir(cw, W, P, m->sirIdx, m->tt, m->D, m->sir, m->Ls, m->nT, m->se, m->R, s);
+    else
+        orc_forward_das(cw, W, P, m->tt, m->D, m->nT, m->R, s);
+    for (int w = 0; w < W; ++w)                                      /* :202 permute [2 3 1] */
+        for (int d = 0; d < m->D; ++d)
+            for (int t = 0; t < m->nT; ++t)
+                sp[((size_t)w * m->D + d) * m->nT + t] = s[(size_t)d * W * m->nT + (size_t)t * W + w];
+    if (m->ir) orc_eir_conv(sp, W, m->D, m->nT, m->ir, m->Lir, m->ir_per_det, sino);
+    else memcpy(sino, sp, sizeof(double) * nS * W);
+    for (size_t i = 0; i < nS * W; ++i) sino[i] = sino[i] / m->sigma;  /* :285 */
+    free(c); free(cw); free(s); free(sp);
+}
+
+/* transpose / transposeWithoutSIR then applyTranspose = transpose(sig)/sigma */
+ORC_API void orc_model_transpose(const orc_model *m, const double *sino, int W, double *img)
+{
+    const int P = m->Nz * m->Nx;
+    const size_t nS = (size_t)m->nT * m->D;
+    double *z = (double *)malloc(sizeof(double) * nS * W);
+    double *zw = (double *)malloc(sizeof(double) * nS * W);
+    double *o = (double *)malloc(sizeof(double) * (size_t)P * W);
+    double *op = (double *)malloc(sizeof(double) * (size_t)P * W);
+    if (m->ir) orc_eir_corr(sino, W, m->D, m->nT, m->ir, m->Lir, m->ir_per_det, z);
+    else memcpy(z, sino, sizeof(double) * nS * W);
+    for (int w = 0; w < W; ++w)                                      /* :230 permute [3 1 2] */
+        for (int d = 0; d < m->D; ++d)
+            for (int t = 0; t < m->nT; ++t)
+                zw[(size_t)d * W * m->nT + (size_t)t * W + w] = z[((size_t)w * m->D + d) * m->nT + t];
+    if (m->use_sir)
+        orc_transpose_sir(zw, W, m->nT, m->D, m->sirIdx, m->tt, P, m->sir, m->Ls, m->se, m->R, o);
+    else
+        orc_transpose_das(zw, W, m->nT, m->D, m->tt, P, m->R, o);
+    for (int w = 0; w < W; ++w)                                      /* :237 */
+        for (int x = 0; x < P; ++x) op[(size_t)w * P + x] = o[(size_t)x * W + w];
+    if (m->kz) orc_prefilter_apply(op, W, m->Nz, m->Nx, m->kz, m->kx, 1, img);
+    else memcpy(img, op, sizeof(double) * (size_t)P * W);
+    for (size_t i = 0; i < (size_t)P * W; ++i) img[i] = img[i] / m->sigma; /* :286 */
+    free(z); free(zw); free(o); free(op);
+}
+
+/* ------------------------------------------------------------------------- */
+/* Linear-operator protocol (basic_tools/combine_forward_and_transpose*.m)     */
+/* ------------------------------------------------------------------------- */
+typedef void (*orc_apply_fn)(void *ctx, const double *in, double *out, int transp);
+
+static double dot(const double *a, const double *b, size_t n)
+{
+    double s = 0.0;
+    for (size_t i = 0; i < n; ++i) s += a[i] * b[i];
+    return s;
+}
+static double nrm2(const double *a, size_t n) { return sqrt(dot(a, a, n)); }
+
+/* Stacked data-fit + L2 term of incorporate_l2_reg_into_model_functions.m:14-17
+ * with RegL2 = identity (combine_forward_and_transpose_with_reg.m:6-14):
+ *   A_reg x = [A x ; sqrt(lambdaL2) * x],  A_reg' [u1; u2] = A' u1 + sqrt(lambdaL2) * u2
+ * lambdaL2 <= 0 gives the plain combine_forward_and_transpose.m. */
+typedef struct {
+    const orc_model *m;
+    int W;
+    double sqrt_lambda_l2;   /* 0 => no reg block */
+} orc_regop;
+
+static void regop_apply(void *ctx, const double *in, double *out, int transp)
+{
+    const orc_regop *r = (const orc_regop *)ctx;
+    const size_t P = (size_t)r->m->Nz * r->m->Nx * r->W;
+    const size_t S = (size_t)r->m->nT * r->m->D * r->W;
+    if (!transp) {
+        orc_model_forward(r->m, in, r->W, out);
+        if (r->sqrt_lambda_l2 > 0)
+            for (size_t i = 0; i < P; ++i) out[S + i] = r->sqrt_lambda_l2 * in[i];
+    } else {
+        orc_model_transpose(r->m, in, r->W, out);
+        if (r->sqrt_lambda_l2 > 0)
+            for (size_t i = 0; i < P; ++i) out[i] = out[i] + r->sqrt_lambda_l2 * in[S + i];
+    }
+}
+
+/* ------------------------------------------------------------------------- */
+/* LSQR with MATLAB lsqr semantics (external built-in, called at               */
+/* reconstruction/nnls_L2_less.m:56).  Paige & Saunders, ACM TOMS 8(1), 1982,  */
+/* in the d-vector form MATLAB uses: the initial residual b - A x0, a          */
+/* transpose product at the end of every iteration, residual-norm estimate     */
+/* normr = |s| normr, and the stopping tests normr <= tol*||b|| and            */
+/* ||A' r|| / (||A|| ||r||) <= tol.  On non-convergence MATLAB returns the     */
+/* iterate of minimal residual; the estimate is non-increasing, so that is the */
+/* last iterate.  Returns the flag (0 converged, 1 maxit, 3 stagnation).       */
+/* ------------------------------------------------------------------------- */
+typedef struct {
+    orc_apply_fn fn;
+    void *ctx;
+    size_t m, n;
+    int nfwd, nadj;   /* operator application counters */
+} orc_op;
+
+static void op_apply(orc_op *op, const double *in, double *out, int transp)
+{
+    op->fn(op->ctx, in, out, transp);
+    if (transp) op->nadj++; else op->nfwd++;
+}
+
+ORC_API int orc_lsqr(orc_op *op, const double *b, double tol, int maxit,
+                     const double *x0, double *x, int *iter_out)
+{
+    const size_t m = op->m, n = op->n;
+    const double n2b = nrm2(b, m);
+    if (n2b == 0.0) {
+        memset(x, 0, sizeof(double) * n);
+        if (iter_out) *iter_out = 0;
+        return 0;
+    }
+    double *u = (double *)malloc(sizeof(double) * m);
+    double *v = (double *)malloc(sizeof(double) * n);
+    double *d = (double *)calloc(n, sizeof(double));
+    double *z = (double *)malloc(sizeof(double) * n);
+    double *tm = (double *)malloc(sizeof(double) * m);
+    double *tn = (double *)malloc(sizeof(double) * n);
+    double *xmin = (double *)malloc(sizeof(double) * n);
+    if (x0) memcpy(x, x0, sizeof(double) * n); else memset(x, 0, sizeof(double) * n);
+    const double tolb = tol * n2b;
+    op_apply(op, x, tm, 0);
+    for (size_t i = 0; i < m; ++i) u[i] = b[i] - tm[i];
+    double beta = nrm2(u, m);
+    double normr = beta;
+    if (beta != 0.0) for (size_t i = 0; i < m; ++i) u[i] /= beta;
+    double c = 1.0, s = 0.0, phibar = beta;
+    op_apply(op, u, v, 1);
+    double alpha = nrm2(v, n);
+    if (alpha != 0.0) for (size_t i = 0; i < n; ++i) v[i] /= alpha;
+    double normar = alpha * beta;
+    int flag = 1, iter = 0;
+    if (normar == 0.0) {
+        flag = 0;
+        goto done;
+    }
+    {
+        double norma = 0.0;
+        int stag = 0;
+        const int maxstagsteps = 3;
+        double normrmin = normr;
+        int imin = 0;
+        memcpy(xmin, x, sizeof(double) * n);
+        for (int ii = 1; ii <= maxit; ++ii) {
+            memcpy(z, v, sizeof(double) * n);
+            op_apply(op, z, tm, 0);
+            for (size_t i = 0; i < m; ++i) u[i] = tm[i] - alpha * u[i];
+            beta = nrm2(u, m);
+            for (size_t i = 0; i < m; ++i) u[i] /= beta;
+            norma = sqrt(norma * norma + alpha * alpha + beta * beta);
+            const double thet = -s * alpha;
+            const double rhot = c * alpha;
+            const double rho = sqrt(rhot * rhot + beta * beta);
+            c = rhot / rho;
+            s = -beta / rho;
+            const double phi = c * phibar;
+            if (phi == 0.0) stag = 1;
+            phibar = s * phibar;
+            for (size_t i = 0; i < n; ++i) d[i] = (z[i] - thet * d[i]) / rho;
+            if (fabs(phi) * nrm2(d, n) < DBL_EPSILON * nrm2(x, n)) stag++;
+            else stag = 0;
+            for (size_t i = 0; i < n; ++i) x[i] = x[i] + phi * d[i];
+            normr = fabs(s) * normr;
+            op_apply(op, u, tn, 1);
+            for (size_t i = 0; i < n; ++i) v[i] = tn[i] - beta * v[i];
+            alpha = nrm2(v, n);
+            for (size_t i = 0; i < n; ++i) v[i] /= alpha;
+            normar = alpha * fabs(s * phi);
+            iter = ii;
+            if (normr <= tolb) { flag = 0; break; }
+            if (normar / (norma * normr) <= tol) { flag = 0; break; }
+            if (stag >= maxstagsteps) { flag = 3; break; }
+            if (normr < normrmin) { normrmin = normr; memcpy(xmin, x, sizeof(double) * n); imin = ii; }
+        }
+        if (flag != 0 && normrmin < normr) { memcpy(x, xmin, sizeof(double) * n); iter = imin; }
+    }
+done:
+    if (iter_out) *iter_out = iter;
+    free(u); free(v); free(d); free(z); free(tm); free(tn); free(xmin);
+    return flag;
+}
+
+/* ------------------------------------------------------------------------- */
+/* nnls_L2_less (reconstruction/nnls_L2_less.m:28-72, 79-101)                  */
+/* ------------------------------------------------------------------------- */
+typedef struct {
+    orc_op *inner;            /* model_func */
+    double sl, sb;            /* sqrt(lambda), sqrt(beta) */
+} orc_augop;
+
+/* add_nn_constraint_term_to_model_matrix: [sqrt(l) A; sqrt(b) I] and its transpose */
+static void augop_apply(void *ctx, const double *in, double *out, int transp)
+{
+    orc_augop *a = (orc_augop *)ctx;
+    const size_t m = a->inner->m, n = a->inner->n;
+    if (!transp) {
+        op_apply(a->inner, in, out, 0);
+        for (size_t i = 0; i < m; ++i) out[i] = a->sl * out[i];
+        for (size_t i = 0; i < n; ++i) out[m + i] = a->sb * in[i];
+    } else {
+        op_apply(a->inner, in, out, 1);
+        for (size_t i = 0; i < n; ++i) out[i] = a->sl * out[i] + a->sb * in[m + i];
+    }
+}
+
+ORC_API void orc_nnls_L2_less(orc_op *model, const double *target, double lambda,
+                              double beta, int nIter, int nIter_inner, double *rec)
+{
+    const size_t m = model->m, n = model->n;
+    double *w = (double *)calloc(n, sizeof(double));     /* img_out_iter, :28 */
+    double *v = (double *)calloc(n, sizeof(double));     /* lagrange multiplier, :29 */
+    double *u = (double *)malloc(sizeof(double) * n);
+    double *baug = (double *)malloc(sizeof(double) * (m + n));
+    orc_augop aug = {model, sqrt(lambda), sqrt(beta)};
+    orc_op augop = {augop_apply, &aug, m + n, n, 0, 0};
+    for (int it = 0; it < nIter; ++it) {
+        for (size_t i = 0; i < m; ++i) baug[i] = sqrt(lambda) * target[i];
+        for (size_t i = 0; i < n; ++i) baug[m + i] = sqrt(beta) * (w[i] + v[i]);
+        orc_lsqr(&augop, baug, 1e-8, nIter_inner, w, u, NULL);       /* :56 */
+        for (size_t i = 0; i < n; ++i) {
+            const double wn = u[i] - v[i];
+            w[i] = wn > 0.0 ? wn : 0.0;                                 /* :60 max(.,0) */
+        }
+        for (size_t i = 0; i < n; ++i) v[i] = v[i] + w[i] - u[i];       /* :64 */
+    }
+    memcpy(rec, w, sizeof(double) * n);
+    free(w); free(v); free(u); free(baug);
+}
+
+/* rec_nn_with_L2_reg.m:31-51 with RegL2 = identity.  Outputs the image
+ * [P x W] and the L-curve data term ||A rec - y||^2 (:51) and reg term
+ * ||rec||^2 (:54).  counts[0..1] receive the model fwd/adj application counts
+ * (including the discarded start value A'y of :41). */
+ORC_API void orc_rec_nn_with_L2_reg(const orc_model *mdl, const double *data, int W,
+                                    int num_iter, double lambdaL2, double *rec,
+                                    double *lcurve, int *counts)
+{
+    const size_t P = (size_t)mdl->Nz * mdl->Nx * W;
+    const size_t S = (size_t)mdl->nT * mdl->D * W;
+    const int reg = lambdaL2 > 0.0;
+    orc_regop rop = {mdl, W, reg ? sqrt(lambdaL2) : 0.0};
+    const size_t m = S + (reg ? P : 0);
+    orc_op op = {regop_apply, &rop, m, P, 0, 0};
+    double *target = (double *)calloc(m, sizeof(double));
+    memcpy(target, data, sizeof(double) * S);
+    double *start = (double *)malloc(sizeof(double) * P);
+    op_apply(&op, target, start, 1);   /* :41, value unused by nnls_L2_less.m:28 */
+    orc_nnls_L2_less(&op, target, 1.0, 1e-3, num_iter, 2, rec);
+    if (lcurve) {
+        double *f = (double *)malloc(sizeof(double) * S);
+        orc_model_forward(mdl, rec, W, f);
+        double e = 0.0;
+        for (size_t i = 0; i < S; ++i) e += (f[i] - data[i]) * (f[i] - data[i]);
+        lcurve[0] = e;
+        lcurve[1] = reg ? dot(rec, rec, P) : 0.0;
+        free(f);
+    }
+    if (counts) { counts[0] = op.nfwd; counts[1] = op.nadj; }
+    free(target); free(start);
+}
+
+/* ------------------------------------------------------------------------- */
+/* CGS with MATLAB cgs semantics (external built-in, called at                 */
+/* reconstruction/nnls_L1_eye_matrix_less.m:45 with x0 = 0).  Sonneveld,       */
+/* SIAM J. Sci. Stat. Comput. 10(1), 1989.  Tracks the iterate of minimal      */
+/* residual and, on non-convergence, returns it when its recomputed residual   */
+/* is not larger than the last one.  The exact MATLAB return rule cannot be    */
+/* checked without MATLAB (parity unpinned for that rule).                     */
+/* ------------------------------------------------------------------------- */
+typedef struct {
+    orc_op *inner;
+    double lam;
+} orc_l1op;
+
+/* L1_method_operator = lambda * A'(A v) + v  (nnls_L1_eye_matrix_less.m:33) */
+static void l1op_apply(orc_l1op *o, const double *in, double *out, double *tmp)
+{
+    op_apply(o->inner, in, tmp, 0);
+    op_apply(o->inner, tmp, out, 1);
+    for (size_t i = 0; i < o->inner->n; ++i) out[i] = o->lam * out[i] + in[i];
+}
+
+ORC_API int orc_cgs_l1(orc_op *inner, double lam, const double *b, double tol,
+                       int maxit, double *x)
+{
+    const size_t n = inner->n;
+    orc_l1op o = {inner, lam};
+    double *tmp = (double *)malloc(sizeof(double) * inner->m);
+    double *r = (double *)malloc(sizeof(double) * n);
+    double *rt = (double *)malloc(sizeof(double) * n);
+    double *u = (double *)malloc(sizeof(double) * n);
+    double *p = (double *)malloc(sizeof(double) * n);
+    double *q = (double *)malloc(sizeof(double) * n);
+    double *vh = (double *)malloc(sizeof(double) * n);
+    double *uh = (double *)malloc(sizeof(double) * n);
+    double *qh = (double *)malloc(sizeof(double) * n);
+    double *xmin = (double *)malloc(sizeof(double) * n);
+    const double n2b = nrm2(b, n);
+    int flag = 1;
+    memset(x, 0, sizeof(double) * n);
+    if (n2b == 0.0) { flag = 0; goto out; }
+    {
+        const double tolb = tol * n2b;
+        memcpy(r, b, sizeof(double) * n);   /* x0 = 0 => r = b, no product */
+        double normr = nrm2(r, n), normr_act = normr;
+        if (normr <= tolb) { flag = 0; goto out; }
+        memcpy(rt, r, sizeof(double) * n);
+        double normrmin = normr;
+        memcpy(xmin, x, sizeof(double) * n);
+        int stag = 0;
+        const int maxstagsteps = 3;
+        double rho = 1.0;
+        for (int ii = 1; ii <= maxit; ++ii) {
+            const double rho1 = rho;
+            rho = dot(rt, r, n);
+            if (rho == 0.0 || isinf(rho)) { flag = 4; break; }
+            if (ii == 1) {
+                memcpy(u, r, sizeof(double) * n);
+                memcpy(p, u, sizeof(double) * n);
+            } else {
+                const double bet = rho / rho1;
+                if (bet == 0.0 || isinf(bet)) { flag = 4; break; }
+                for (size_t i = 0; i < n; ++i) u[i] = r[i] + bet * q[i];
+                for (size_t i = 0; i < n; ++i) p[i] = u[i] + bet * (q[i] + bet * p[i]);
+            }
+            l1op_apply(&o, p, vh, tmp);
+            const double rtvh = dot(rt, vh, n);
+            if (rtvh == 0.0) { flag = 4; break; }
+            const double alpha = rho / rtvh;
+            if (isinf(alpha)) { flag = 4; break; }
+            for (size_t i = 0; i < n; ++i) q[i] = u[i] - alpha * vh[i];
+            for (size_t i = 0; i < n; ++i) uh[i] = u[i] + q[i];
+            if (fabs(alpha) * nrm2(uh, n) < DBL_EPSILON * nrm2(x, n)) stag++;
+            else stag = 0;
+            for (size_t i = 0; i < n; ++i) x[i] = x[i] + alpha * uh[i];
+            l1op_apply(&o, uh, qh, tmp);
+            for (size_t i = 0; i < n; ++i) r[i] = r[i] - alpha * qh[i];
+            normr = nrm2(r, n);
+            normr_act = normr;
+            if (normr <= tolb || stag >= maxstagsteps) {
+                l1op_apply(&o, x, qh, tmp);
+                for (size_t i = 0; i < n; ++i) r[i] = b[i] - qh[i];
+                normr_act = nrm2(r, n);
+                if (normr_act <= tolb) { flag = 0; break; }
+                if (stag >= maxstagsteps) { flag = 3; break; }
+            }
+            if (normr_act < normrmin) { normrmin = normr_act; memcpy(xmin, x, sizeof(double) * n); }
+        }
+        if (flag != 0) {
+            l1op_apply(&o, xmin, qh, tmp);
+            for (size_t i = 0; i < n; ++i) r[i] = b[i] - qh[i];
+            if (nrm2(r, n) <= normr_act) memcpy(x, xmin, sizeof(double) * n);
+        }
+    }
+out:
+    free(tmp); free(r); free(rt); free(u); free(p); free(q); free(vh); free(uh); free(qh); free(xmin);
+    return flag;
+}
+
+/* nnls_L1_eye_matrix_less.m:24-55 */
+ORC_API void orc_nnls_L1_eye(orc_op *model, const double *target, const double *initial,
+                             double etaReg, double lambdaParam, int nIter, int cgIter,
+                             double *rec)
+{
+    const size_t n = model->n;
+    double *b = (double *)calloc(n, sizeof(double));
+    double *u = (double *)malloc(sizeof(double) * n);
+    double *dp = (double *)malloc(sizeof(double) * n);
+    double *bin = (double *)malloc(sizeof(double) * n);
+    double *aty = (double *)malloc(sizeof(double) * n);
+    memcpy(u, initial, sizeof(double) * n);                       /* :28 */
+    for (int it = 0; it < nIter; ++it) {
+        for (size_t i = 0; i < n; ++i) { const double t = u[i] - b[i]; dp[i] = (t > 0) * t; } /* :40 */
+        op_apply(model, target, aty, 1);                             /* :43 */
+        for (size_t i = 0; i < n; ++i)
+            bin[i] = ((dp[i] > 0) * dp[i]) + b[i] + lambdaParam * aty[i] - lambdaParam * etaReg * 1.0;
+        orc_cgs_l1(model, lambdaParam, bin, 1e-8, cgIter, u);         /* :45 */
+        for (size_t i = 0; i < n; ++i) b[i] = b[i] + ((dp[i] > 0) * dp[i]) - u[i];  /* :47 */
+    }
+    memcpy(rec, u, sizeof(double) * n);
+    free(b); free(u); free(dp); free(bin); free(aty);
+}
+
+/* rec_nn_with_L1_eye_reg.m:21-36 */
+ORC_API void orc_rec_nn_with_L1_eye_reg(const orc_model *mdl, const double *data, int W,
+                                        int num_iter, double regParam, double *rec,
+                                        int *counts)
+{
+    const size_t P = (size_t)mdl->Nz * mdl->Nx * W;
+    const size_t S = (size_t)mdl->nT * mdl->D * W;
+    orc_regop rop = {mdl, W, 0.0};
+    orc_op op = {regop_apply, &rop, S, P, 0, 0};
+    double *start = (double *)malloc(sizeof(double) * P);
+    op_apply(&op, data, start, 1);                                    /* :32 */
+    orc_nnls_L1_eye(&op, data, start, regParam, 1.0, num_iter, 2, rec);
+    if (counts) { counts[0] = op.nfwd; counts[1] = op.nadj; }
+    free(start);
+}
+
+/* Number of OpenMP threads the oracle uses (reported as cpu_baseline.cores). */
+ORC_API int orc_num_threads(void) { return omp_get_max_threads(); }

@@ -1,0 +1,202 @@
+"""GPU parity tests of the HIP operators and solvers against the FP64 oracle.
+
+Tolerances (FP32 device vs FP64 oracle): forward / transpose outputs within
+1e-5 relative L2; adjoint identity <Ax, y> = <x, A'y> within 1e-6 relative;
+reconstructions within 1e-3 relative L2 at equal iterations.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from helpers import geometry_tt, random_sir, random_tt, rel_l2, rng
+
+pytestmark = pytest.mark.gpu
+
+TOL_OP = 1e-5
+TOL_ADJ = 1e-6
+TOL_REC = 1e-3
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _gpu(require_gpu):
+    import mbrec
+    mbrec.lib()
+    yield
+
+
+def _log(msg):
+    print(msg, flush=True)
+
+
+# --- MEX drop-ins (host FP64 in/out) -------------------------------------------------
+@pytest.mark.parametrize("W,P,D,nT,R,spill", [(1, 1000, 16, 300, 2.6666666666666665, 0.0),
+                                              (3, 777, 9, 257, 3.3333333333333335, 0.15),
+                                              (2, 64, 5, 128, 1.6, 0.3),
+                                              (1, 130, 4, 96, 2.0, 0.2)])
+def test_mex_das_matches_oracle(W, P, D, nT, R, spill):
+    from mbrec.mex import forwardDelayAndSumMex, transposeDelayAndSumMex
+    _log(f"mex_das W={W} P={P} D={D} nT={nT} R={R}")
+    tt = random_tt(P, D, nT, R, seed=P + D, spill=spill)
+    x = rng(1).uniform(0, 1, (W, P))
+    y = forwardDelayAndSumMex(x, tt, np.int32(nT), R)
+    yo = O.forward_das(x, tt, nT, R)
+    assert y.shape == (W, nT, D)
+    assert rel_l2(y, yo) < TOL_OP
+    s = rng(2).standard_normal((W, nT, D))
+    xt = transposeDelayAndSumMex(s, tt, R)
+    assert xt.shape == (W, P)
+    assert rel_l2(xt, O.transpose_das(s, tt, R)) < TOL_OP
+
+
+@pytest.mark.parametrize("W,P,D,nT,R,Ls", [(1, 500, 8, 200, 2.6666666666666665, 9),
+                                           (2, 333, 6, 150, 1.6, 15),
+                                           (5, 70, 3, 90, 3.3333333333333335, 4)])
+def test_mex_sir_matches_oracle(W, P, D, nT, R, Ls):
+    from mbrec.mex import forwardMex, transposeMex
+    _log(f"mex_sir W={W} P={P} D={D} nT={nT} R={R} Ls={Ls}")
+    tt = random_tt(P, D, nT, R, seed=7, spill=0.1)
+    sidx, sir, se = random_sir(P, D, Ls=Ls, Nsir=11, seed=8)
+    x = rng(3).uniform(0, 1, (W, P))
+    y = forwardMex(x, sidx, tt, sir, np.int32(nT), se, R)
+    assert rel_l2(y, O.forward_sir(x, sidx, tt, sir, nT, se, R)) < TOL_OP
+    s = rng(4).standard_normal((W, nT, D))
+    xt = transposeMex(s, sidx, tt, sir, se, R)
+    assert rel_l2(xt, O.transpose_sir(s, sidx, tt, sir, se, R)) < TOL_OP
+
+
+def test_mex_known_answers():
+    from mbrec.mex import forwardDelayAndSumMex, forwardMex
+    y = forwardDelayAndSumMex(np.array([[2.0]]), np.array([[10.25]]), np.int32(20), 2.5)
+    want = np.zeros(20); want[8:13] = [0.2, 1.0, 1.8, 1.4, 0.6]
+    np.testing.assert_allclose(y[0, :, 0], want, atol=1e-6)
+    sir = np.array([[0.0, 1.0, 2.0, 3.0, 0.0]]).T
+    y = forwardMex(np.array([[1.0]]), np.array([[1]], dtype=np.int32), np.array([[10.0]]), sir, np.int32(16),
+                   np.array([[2], [4]], dtype=np.int32), 1.5)
+    want = np.zeros(16); want[8:13] = [1 / 3, 5 / 3, 10 / 3, 11 / 3, 1.0]
+    np.testing.assert_allclose(y[0, :, 0], want, atol=1e-6)
+
+
+# --- device plans (tiled pixel order) ------------------------------------------------
+def _plan_problem(Nz, Nx, D, nT, R, seed, sir=False):
+    import torch
+    from mbrec.model import DevicePlan
+    P = Nz * Nx
+    tt = random_tt(P, D, nT, R, seed=seed, spill=0.05)
+    kw = {}
+    if sir:
+        sidx, s, se = random_sir(P, D, Ls=13, Nsir=9, seed=seed + 1)
+        kw["sir"] = (sidx.T, s.T, se.T)
+    plan = DevicePlan(Nz, Nx, D, nT, R, tt.T, **kw)
+    return plan, tt, kw, torch
+
+
+@pytest.mark.parametrize("sir", [False, True])
+@pytest.mark.parametrize("B", [1, 3, 6])
+def test_device_mex_ops_batched(sir, B):
+    Nz, Nx, D, nT, R = 37, 29, 12, 220, 2.6666666666666665
+    _log(f"device_mex sir={sir} B={B}")
+    plan, tt, kw, torch = _plan_problem(Nz, Nx, D, nT, R, seed=20 + B, sir=sir)
+    x = rng(5).uniform(0, 1, (B, Nx, Nz))
+    y = plan.forward(torch.tensor(x, dtype=torch.float32).cuda(), model=False).cpu().numpy()
+    xo = x.reshape(B, -1)
+    if sir:
+        sidx, s, se = (a.T for a in kw["sir"])
+        yo = O.forward_sir(xo, sidx, tt, s, nT, se, R)
+    else:
+        yo = O.forward_das(xo, tt, nT, R)
+    yo = np.transpose(yo, (0, 2, 1))          # [W x nT x D] -> (W, D, nT)
+    assert rel_l2(y, yo) < TOL_OP
+    sg = rng(6).standard_normal((B, D, nT))
+    xt = plan.transpose(torch.tensor(sg, dtype=torch.float32).cuda(), model=False).cpu().numpy()
+    sgm = np.transpose(sg, (0, 2, 1))
+    xto = O.transpose_sir(sgm, sidx, tt, s, se, R) if sir else O.transpose_das(sgm, tt, R)
+    assert rel_l2(xt.reshape(B, -1), xto) < TOL_OP
+
+
+def _geometry_model(Nz, Nx, D, nT, h, sir=False, sigma=None):
+    """Model with the benchmark geometry scaled down, built by the product setup code."""
+    from mbrec.discretization import Discretization
+    from mbrec.model import MSOTmodel
+    from mbrec.probe import Probe
+    half = (Nx - 1) / 2 * h
+    disc = Discretization([-half, half, -(Nz - 1) / 2 * h, (Nz - 1) / 2 * h], [Nx, Nz])
+    probe = Probe("arc256_r40_174", True, False, num_detectors=D)
+    probe.DAC.numRecordedSamplesPerTransducer = nT
+    # shift the recording window so the small field of view lands mid-sinogram
+    probe.DAC.delayBeforeRecording = int(round(probe.detector.radius / 1500.0 * probe.DAC.frequency)) - nT // 2
+    return MSOTmodel(probe, disc, 1500.0, sir, True, 0, 1e5, 12e6, sigma)
+
+
+def _oracle_of(model):
+    """Oracle model on the same tables; the prefilter kernels come from the oracle's own
+    direct DFT of the Gaussian pixel (sigma = round(h/2 um), Discretization.m:43)."""
+    d = model.Discretization
+    Nz, Nx = d.Nz, d.Nx
+    h = d.dz
+    sig_px = np.floor(h / 2 * 1e6 + 0.5) / 1e6
+    kz, kx = O.full_kernels(Nz, Nx, d.dz, d.dx, sig_px)
+    kw = dict(kz=kz, kx=kx, ir=model.Data.IR)
+    if model.includeSir:
+        kw.update(sidx=model.Data.SPMRIndices, sir=model.Data.SPMR, se=model.Data.SPMRTimestepIndexStartAndEnd)
+    D = model.Probe.detector.numOfTransducers
+    return O.Model(Nz, Nx, model.Data.absoluteTravelTimeIndices.reshape(D, -1),
+                   model.Data.numTimeSamplesPerTransducer, model.Data.circularInterpolationRadiusInSamples,
+                   sigma=model.normalizationFactor, **kw)
+
+
+@pytest.mark.parametrize("sir", [False, True])
+def test_model_operators_match_oracle(sir):
+    import torch
+    _log(f"model_ops sir={sir}")
+    m = _geometry_model(48, 40, 32, 700, 1e-4, sir=sir, sigma=2.5)
+    orc = _oracle_of(m)
+    x = np.maximum(rng(9).standard_normal((2, 40, 48)), 0)
+    y = m.applyForward(x)
+    yo = orc.forward(x)
+    assert rel_l2(y, yo) < TOL_OP, rel_l2(y, yo)
+    s = rng(10).standard_normal((2, 32, 700))
+    xt = m.applyTranspose(s)
+    assert rel_l2(xt, orc.transpose(s)) < TOL_OP
+    # adjoint identity of the device pair, FP64 inner products of FP32 results
+    xd = torch.tensor(x, dtype=torch.float32).cuda()
+    sd = m.applyForward(torch.tensor(np.maximum(rng(11).standard_normal((2, 40, 48)), 0), dtype=torch.float32).cuda())
+    lhs = torch.sum(m.applyForward(xd).double() * sd.double()).item()
+    rhs = torch.sum(xd.double() * m.applyTranspose(sd).double()).item()
+    assert abs(lhs - rhs) <= TOL_ADJ * abs(lhs), (lhs, rhs)
+
+
+def test_nnls_L2_matches_oracle():
+    import torch
+    from mbrec.solvers import rec_nn_with_L2_reg
+    from mbrec.phantoms import disks, add_noise
+    _log("nnls_L2 parity")
+    m = _geometry_model(40, 40, 32, 600, 1.25e-4, sir=False, sigma=3.0)
+    orc = _oracle_of(m)
+    ph = np.stack([disks(m.Discretization, n=6, seed=s) for s in (1, 2)])
+    y = add_noise(orc.forward(ph), 0.01, seed=3)
+    lam = 1e-2 * np.linalg.norm(orc.transpose(y))
+    for coupled in (True, False):
+        rec, e_img, e_reg, *_ = rec_nn_with_L2_reg(m, y, 20, lam, coupled=coupled)
+        if coupled:
+            ro, lc, _ = orc.rec_nn_with_L2_reg(y, 20, lam)
+            assert rel_l2(rec, ro) < TOL_REC
+            assert abs(e_img - lc[0]) < 1e-3 * lc[0]
+        else:
+            for w in range(2):
+                ro, lc, _ = orc.rec_nn_with_L2_reg(y[w:w + 1], 20, lam)
+                assert rel_l2(rec[w], ro) < TOL_REC
+                assert abs(e_img[w] - lc[0]) < 1e-3 * lc[0]
+
+
+def test_nnls_warm_start_seeds_outer_iterate():
+    """init really seeds w0 (the hook the learned warm start uses): with 0 outer
+    iterations the result is the seed itself, without a seed it is zero."""
+    import torch
+    from mbrec.solvers import rec_nn_with_L2_reg
+    m = _geometry_model(24, 24, 16, 400, 1.25e-4, sir=False, sigma=3.0)
+    y = rng(12).standard_normal((2, 16, 400))
+    seed = rng(13).uniform(0, 1, (2, 24, 24)).astype(np.float32)
+    r0, *_ = rec_nn_with_L2_reg(m, y, 0, 0.0, coupled=False, init=seed)
+    np.testing.assert_array_equal(r0.astype(np.float32), seed)
+    z0, *_ = rec_nn_with_L2_reg(m, y, 0, 0.0, coupled=False)
+    assert not np.any(z0)

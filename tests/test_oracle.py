@@ -1,0 +1,294 @@
+"""CPU tests that pin the FP64 oracle (no GPU).
+
+Each oracle function is checked against something it does not share code with:
+hand-derived known answers, a different formulation (dense hat-function
+matrices, FFT forms, exact Krylov minimisers, textbook filter design) or a
+mathematical invariant.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import oracle as O
+from helpers import rel_l2, random_sir, random_tt, rng
+
+GOLD = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "known_answers.json")))
+
+
+# --- known answers -----------------------------------------------------------
+def test_das_single_pixel_known_answer():
+    c = GOLD["das_single_pixel"]
+    y = O.forward_das(np.array([[c["value"]]]), np.array([[c["tau"]]]), c["nT"], c["R"])
+    want = np.zeros(c["nT"])
+    for k, v in c["expect"].items():
+        want[int(k)] = v
+    np.testing.assert_allclose(y[0, :, 0], want, atol=1e-14)
+
+
+def test_das_edge_samples_dropped():
+    c = GOLD["das_edge_drop"]
+    y = O.forward_das(np.array([[c["value"]]]), np.array([[c["tau"]]]), c["nT"], c["R"])
+    want = np.array([c["expect"][str(i)] for i in range(c["nT"])])
+    np.testing.assert_allclose(y[0, :, 0], want, atol=1e-14)
+
+
+def test_sir_single_pixel_known_answer():
+    c = GOLD["sir_single_pixel"]
+    sir = np.array(c["sir"]).reshape(-1, 1)
+    se = np.array(c["window"], dtype=np.int32).reshape(2, 1)
+    y = O.forward_sir(np.array([[c["value"]]]), np.array([[1]], dtype=np.int32), np.array([[c["tau"]]]),
+                      sir, c["nT"], se, c["R"])
+    want = np.zeros(c["nT"])
+    for k, v in c["expect"].items():
+        want[int(k)] = v
+    np.testing.assert_allclose(y[0, :, 0], want, atol=1e-13)
+
+
+def test_das_transpose_known_answer():
+    c = GOLD["das_transpose_single_detector"]
+    sig = np.array(c["signal"], dtype=np.float64).reshape(1, -1, 1)
+    x = O.transpose_das(sig, np.array([[c["tau"]]]), c["R"])
+    assert abs(x[0, 0] - c["expect"]) < 1e-13
+
+
+# --- different formulation: dense hat-function matrices ------------------------
+@pytest.mark.parametrize("R", [2.6666666666666665, 3.3333333333333335, 1.6, 2.0, 0.75])
+def test_das_matches_dense_hat_matrix(R):
+    P, D, nT = 37, 5, 64
+    tt = random_tt(P, D, nT, R, seed=1, spill=0.2)
+    A = O.dense_das(tt, nT, R)
+    x = rng(2).standard_normal(P)
+    y = O.forward_das(x.reshape(1, P), tt, nT, R)
+    np.testing.assert_allclose(y[0].T.ravel(), A @ x, rtol=1e-12, atol=1e-12)
+    s = rng(3).standard_normal((1, nT, D))
+    xt = O.transpose_das(s, tt, R)
+    np.testing.assert_allclose(xt[0], A.T @ s[0].T.ravel(), rtol=1e-12, atol=1e-12)
+
+
+@pytest.mark.parametrize("R", [2.6666666666666665, 1.6])
+def test_sir_matches_dense_hat_matrix(R):
+    P, D, nT = 23, 4, 48
+    tt = random_tt(P, D, nT, R, seed=4, spill=0.2)
+    sidx, sir, se = random_sir(P, D, Ls=9, Nsir=6, seed=5)
+    A = O.dense_sir(sidx, tt, sir, se, nT, R)
+    x = rng(6).standard_normal(P)
+    y = O.forward_sir(x.reshape(1, P), sidx, tt, sir, nT, se, R)
+    np.testing.assert_allclose(y[0].T.ravel(), A @ x, rtol=1e-12, atol=1e-12)
+    s = rng(7).standard_normal((1, nT, D))
+    xt = O.transpose_sir(s, sidx, tt, sir, se, R)
+    np.testing.assert_allclose(xt[0], A.T @ s[0].T.ravel(), rtol=1e-12, atol=1e-12)
+
+
+def test_sir_delta_reduces_to_das():
+    """A unit tap at floor(Ls/2) with that single-tap window makes forwardMex == DAS."""
+    P, D, nT, R, Ls = 50, 6, 80, 2.6666666666666665, 7
+    tt = random_tt(P, D, nT, R, seed=8, spill=0.1)
+    sir = np.zeros((Ls, 1)); sir[Ls // 2, 0] = 1.0
+    se = np.array([[Ls // 2 + 1], [Ls // 2 + 1]], dtype=np.int32)
+    sidx = np.ones((P, D), dtype=np.int32)
+    x = rng(9).standard_normal((2, P))
+    np.testing.assert_allclose(O.forward_sir(x, sidx, tt, sir, nT, se, R), O.forward_das(x, tt, nT, R), atol=1e-13)
+    s = rng(10).standard_normal((2, nT, D))
+    np.testing.assert_allclose(O.transpose_sir(s, sidx, tt, sir, se, R), O.transpose_das(s, tt, R), atol=1e-13)
+
+
+def test_wavelength_batch_equals_single_calls():
+    P, D, nT, R = 40, 5, 60, 2.5
+    tt = random_tt(P, D, nT, R, seed=11)
+    sidx, sir, se = random_sir(P, D, seed=12)
+    x = rng(13).standard_normal((5, P))
+    yb = O.forward_sir(x, sidx, tt, sir, nT, se, R)
+    for w in range(5):
+        np.testing.assert_allclose(yb[w], O.forward_sir(x[w:w + 1], sidx, tt, sir, nT, se, R)[0], atol=1e-13)
+
+
+def test_adjoint_identity_fp64():
+    P, D, nT, R = 60, 7, 90, 2.6666666666666665
+    tt = random_tt(P, D, nT, R, seed=14, spill=0.1)
+    sidx, sir, se = random_sir(P, D, seed=15)
+    x = rng(16).standard_normal((3, P))
+    s = rng(17).standard_normal((3, nT, D))
+    for fwd, adj in ((lambda v: O.forward_das(v, tt, nT, R), lambda v: O.transpose_das(v, tt, R)),
+                     (lambda v: O.forward_sir(v, sidx, tt, sir, nT, se, R),
+                      lambda v: O.transpose_sir(v, sidx, tt, sir, se, R))):
+        lhs = np.vdot(fwd(x), s)
+        rhs = np.vdot(x, adj(s))
+        assert abs(lhs - rhs) <= 1e-12 * abs(lhs)
+
+
+# --- EIR -----------------------------------------------------------------------
+def test_eir_conv_matches_numpy_and_fft_forms():
+    g = rng(18)
+    s = g.standard_normal((2, 3, 150))
+    ir = g.standard_normal(201)
+    y = O.eir_conv(s, ir)
+    ref = np.stack([[np.convolve(s[w, d], ir)[100:250] for d in range(3)] for w in range(2)])
+    np.testing.assert_allclose(y, ref, rtol=1e-12, atol=1e-12)
+    np.testing.assert_allclose(y, O.eir_conv_fft(s, ir), rtol=1e-10, atol=1e-10)
+    z = O.eir_corr(s, ir)
+    np.testing.assert_allclose(z, O.eir_corr_fft(s, ir), rtol=1e-10, atol=1e-10)
+    t = g.standard_normal(s.shape)
+    assert abs(np.vdot(O.eir_conv(t, ir), s) - np.vdot(t, z)) < 1e-10 * abs(np.vdot(t, z))
+
+
+def test_eir_per_detector():
+    g = rng(19)
+    s = g.standard_normal((1, 4, 120))
+    ir = g.standard_normal((4, 201))
+    y = O.eir_conv(s, ir, per_det=True)
+    for d in range(4):
+        np.testing.assert_allclose(y[0, d], np.convolve(s[0, d], ir[d])[100:220], atol=1e-12)
+
+
+# --- prefilter -----------------------------------------------------------------
+@pytest.mark.parametrize("N,h,sig", [(40, 100e-6, 50e-6), (33, 125e-6, 62.5e-6), (16, 60e-6, 30e-6)])
+def test_prefilter_kernel_dft_vs_fft(N, h, sig):
+    g = O.gaussian_profile_shifted(N, h, sig)
+    k = O.prefilter_kernel_1d(g)
+    np.testing.assert_allclose(k, np.real(np.fft.ifft(1.0 / np.fft.fft(g))), rtol=1e-10, atol=1e-12)
+    # deconvolution property: g (*) k = delta (circular)
+    conv = np.real(np.fft.ifft(np.fft.fft(g) * np.fft.fft(k)))
+    e = np.zeros(N); e[0] = 1
+    np.testing.assert_allclose(conv, e, atol=1e-10)
+
+
+@pytest.mark.parametrize("Nz,Nx", [(24, 20), (17, 22)])
+def test_prefilter_separable_matches_reference_fft(Nz, Nx):
+    h, sig = 100e-6, 50e-6
+    kz, kx = O.full_kernels(Nz, Nx, h, h, sig)
+    img = rng(20).standard_normal((2, Nx, Nz))
+    for tr in (False, True):
+        a = O.prefilter_apply(img, kz, kx, transp=tr)
+        b = O.prefilter_fft(img, Nz, Nx, h, h, sig, transp=tr)
+        np.testing.assert_allclose(a, b, rtol=1e-9, atol=1e-9)
+
+
+# --- IR design -----------------------------------------------------------------
+def test_butterworth_design_matches_textbook():
+    from mbrec.irf import butter_zero_phase_response
+    fs = 4e7
+    for n in (201, 256):
+        a = butter_zero_phase_response(n, fs, 1e5, 12e6)
+        b = O.zero_phase_response_textbook(n, fs, 1e5, 12e6)
+        np.testing.assert_allclose(a, b, rtol=1e-9, atol=1e-12)
+    # |H|^2 of the 8th-order low-pass alone is 1/2 at its cut-off (fmax on a bin: n = 40)
+    lp = O.zero_phase_response_textbook(40, fs, 0.0, 12e6)
+    assert abs(lp[12] - 0.5) < 1e-9
+
+
+def test_gauspuls_definition():
+    from mbrec.irf import gauspuls_eir
+    fs, fc = 4e7, 4e6
+    e = gauspuls_eir(fs, fc, 1.0, -6.0)
+    assert e.shape == (201,) and e[100] == 1.0
+    # the -6 dB point of the envelope lies at f = fc*(1 +/- bw/2) in the spectrum
+    spec = np.abs(np.fft.rfft(e, 1 << 16))
+    f = np.fft.rfftfreq(1 << 16, 1 / fs)
+    pk = spec.max()
+    for fe in (fc * 0.5, fc * 1.5):
+        v = np.interp(fe, f, spec) / pk
+        assert abs(20 * np.log10(v) + 6.0) < 0.2
+
+
+# --- solvers ---------------------------------------------------------------------
+def _tiny_model(seed=21, sir=False, Nz=8, Nx=7, D=5, nT=70):
+    R = 2.6666666666666665
+    P = Nz * Nx
+    tt = random_tt(P, D, nT, R, seed=seed)
+    kz, kx = O.full_kernels(Nz, Nx, 1e-4, 1e-4, 5e-5)
+    ir = rng(seed + 1).standard_normal(9)
+    kw = dict(kz=kz, kx=kx, ir=ir)
+    if sir:
+        sidx, s, se = random_sir(P, D, seed=seed + 2)
+        kw.update(sidx=sidx.T, sir=s.T, se=se.T)
+    return O.Model(Nz, Nx, tt.T, nT, R, sigma=3.0, **kw)
+
+
+def _dense_of(m):
+    P = m.Nz * m.Nx
+    A = np.zeros((m.D * m.nT, P))
+    e = np.zeros((1, m.Nx, m.Nz))
+    for j in range(P):
+        e.ravel()[:] = 0
+        e.ravel()[j] = 1
+        A[:, j] = m.forward(e).ravel()
+    return A
+
+
+def test_model_transpose_is_adjoint():
+    for sir in (False, True):
+        m = _tiny_model(sir=sir)
+        x = rng(30).standard_normal((2, m.Nx, m.Nz))
+        y = rng(31).standard_normal((2, m.D, m.nT))
+        lhs, rhs = np.vdot(m.forward(x), y), np.vdot(x, m.transpose(y))
+        assert abs(lhs - rhs) <= 1e-11 * abs(lhs)
+
+
+def test_model_composition_matches_parts():
+    """A = crop(IR * MEX(F p)) / sigma composed from the separately pinned parts."""
+    m = _tiny_model(seed=40)
+    x = rng(41).standard_normal((1, m.Nx, m.Nz))
+    c = O.prefilter_apply(x, m.kz, m.kx)
+    tt = m.tt.T
+    s = O.forward_das(c.reshape(1, -1), tt, m.nT, m.R)          # [1 x nT x D]
+    s = np.transpose(s, (0, 2, 1))                               # (1, D, nT)
+    want = O.eir_conv(s, m.ir) / 3.0
+    np.testing.assert_allclose(m.forward(x), want, rtol=1e-12, atol=1e-12)
+
+
+def _nnls_L2_reference(A, y, nIter, lamL2):
+    """nnls_L2_less with each LSQR call replaced by its exact Krylov minimiser."""
+    m, n = A.shape
+    Areg = np.vstack([A, np.sqrt(lamL2) * np.eye(n)]) if lamL2 > 0 else A
+    b = np.concatenate([y, np.zeros(n)]) if lamL2 > 0 else y
+    sb = np.sqrt(1e-3)
+    Aaug = np.vstack([Areg, sb * np.eye(n)])
+    w = np.zeros(n); v = np.zeros(n)
+    for _ in range(nIter):
+        baug = np.concatenate([b, sb * (w + v)])
+        u = O.lsqr_krylov(Aaug, baug, 2, w)
+        w = np.maximum(u - v, 0)
+        v = v + w - u
+    return w
+
+
+@pytest.mark.parametrize("lamL2", [0.0, 0.05])
+def test_nnls_L2_matches_exact_krylov_restatement(lamL2):
+    m = _tiny_model(seed=50)
+    A = _dense_of(m)
+    xt = np.maximum(rng(51).standard_normal(A.shape[1]), 0)
+    y = A @ xt + 0.01 * rng(52).standard_normal(A.shape[0])
+    rec, lc, cnt = m.rec_nn_with_L2_reg(y.reshape(1, m.D, m.nT), 12, lamL2)
+    want = _nnls_L2_reference(A, y, 12, lamL2)
+    assert rel_l2(rec.ravel(), want) < 1e-8
+    # 1 A'y start value + 12 x (3 A + 3 A') + 1 A for the L-curve term
+    assert tuple(cnt) == (12 * 3, 12 * 3 + 1)
+    assert abs(lc[0] - np.sum((A @ rec.ravel() - y) ** 2)) < 1e-9 * lc[0]
+
+
+def test_lsqr_restatement_matches_krylov_and_scipy():
+    from scipy.sparse.linalg import lsqr as sp_lsqr
+    g = rng(60)
+    A = g.standard_normal((30, 12)); b = g.standard_normal(30); x0 = g.standard_normal(12)
+    for k in (1, 2, 3):
+        a = O.lsqr_matlab(A, b, 1e-12, k, x0)
+        np.testing.assert_allclose(a, O.lsqr_krylov(A, b, k, x0), rtol=1e-9, atol=1e-11)
+        s = sp_lsqr(A, b, atol=0, btol=0, iter_lim=k, x0=x0)[0]
+        np.testing.assert_allclose(a, s, rtol=1e-9, atol=1e-11)
+
+
+def test_nnls_L2_fixed_point_is_nonnegative_least_squares():
+    """After many outer iterations the splitting reaches the bound-constrained minimiser."""
+    from scipy.optimize import lsq_linear
+    m = _tiny_model(seed=70, Nz=5, Nx=4, D=4, nT=40)
+    A = _dense_of(m)
+    g = rng(71)
+    y = A @ np.maximum(g.standard_normal(A.shape[1]), 0) + 0.05 * g.standard_normal(A.shape[0])
+    lam = 0.02
+    rec, _, _ = m.rec_nn_with_L2_reg(y.reshape(1, m.D, m.nT), 10000, lam)
+    Aa = np.vstack([A, np.sqrt(lam) * np.eye(A.shape[1])])
+    ya = np.concatenate([y, np.zeros(A.shape[1])])
+    ref = lsq_linear(Aa, ya, bounds=(0, np.inf), tol=1e-14, lsmr_tol="auto", max_iter=10000).x
+    assert rel_l2(rec.ravel(), ref) < 1e-3
