@@ -1,0 +1,186 @@
+"""Benchmark: NNLS-L2 reconstructions per second at 400^2 px x 256 channels on one MI355X.
+
+Workload (one "step"): rec_nn_with_L2_reg, 50 outer iterations (2 LSQR inner
+iterations each, lambda = 1, beta = 1e-3) for a batch of B independent seeded
+sinograms on the 400 x 400 / 100 um grid with SIR, 256-element 40 mm / 174 deg
+arc, fs 40 MHz, 2030 samples, c 1500 m/s, delay 0, synthetic gauspuls EIR
+(fc 4 MHz, 100 %), band 0.1-12 MHz, explicit sigma_max.  Phantoms: 20 seeded
+disks per sinogram, 1 % Gaussian noise, lambdaL2 = 1e-2 * ||A' y_0|| with
+RegL2 = identity.  All inputs are resident in HBM before the timed region.
+
+Prints one JSON line (rank 0).  value = reconstructions / s over the timed
+steps.  roofline: the dominant operator kernel (by time per solve), its
+algorithmic bytes per launch (travel-time table + SIR index + referenced SIR
+entries + image + sinogram, FP32) over its average duration measured with HIP
+events on the launch stream.  cpu_baseline: the FP64 C oracle (same loops and
+OpenMP decomposition as the reference MEX files) on a bounded sample of the
+same operator, scaled to one solve.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path[:0] = [os.path.join(ROOT, "mb-rec-msot_amd"), ROOT]
+
+METRIC = "NNLS recons/sec 400²px×256ch at rel.err≤1e-3 (learned vs plain); op HBM GB/s"
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--batch", type=int, default=16)
+    ap.add_argument("--iters", type=int, default=50)
+    ap.add_argument("--grid", type=int, default=400)
+    ap.add_argument("--no-sir", action="store_true")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-sample-reps", type=int, default=1)
+    return ap.parse_args()
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(f"bench.py runs on one GPU only; --gpus {a.gpus} is not supported in this build "
+              "(batches of independent solves shard across GPUs with no exchange; see DESIGN.md)",
+              flush=True)
+        sys.exit(2)
+    import numpy as np
+    import torch
+
+    from mbrec.discretization import Discretization
+    from mbrec.model import MSOTmodel
+    from mbrec.phantoms import disks
+    from mbrec.probe import Probe
+    from mbrec.solvers import rec_nn_with_L2_reg
+
+    use_sir = not a.no_sir
+    N, h = a.grid, 1e-4
+    half = (N - 1) / 2 * h
+    t0 = time.time()
+    disc = Discretization([-half, half], N)
+    probe = Probe("arc256_r40_174", True, False)
+    model = MSOTmodel(probe, disc, 1500.0, use_sir, True, 0, 1e5, 12e6, None, sigma_iterations=40)
+    log(f"setup {time.time() - t0:.1f} s, sigma {model.normalizationFactor:.6g}, "
+        f"device tables {model.plan.device_bytes / 1e6:.1f} MB")
+    D, nT = model.shape_sinogram
+    B = a.batch
+    ph = np.stack([disks(disc, n=20, seed=1000 + i) for i in range(B)]).astype(np.float32)
+    x_true = torch.tensor(ph).cuda()
+    y = model.applyForward(x_true)
+    g = torch.Generator(device="cuda").manual_seed(7)
+    noise = torch.randn(y.shape, generator=g, device="cuda")
+    y = (y + 0.01 * y.abs().amax(dim=(1, 2), keepdim=True) * noise).contiguous()
+    lam = 1e-2 * torch.linalg.vector_norm(model.applyTranspose(y[:1]).double()).item()
+    torch.cuda.synchronize()
+
+    def step():
+        return rec_nn_with_L2_reg(model, y, a.iters, lam, coupled=False)
+
+    for _ in range(a.warmup):
+        step()
+    torch.cuda.synchronize()
+    s = torch.cuda.current_stream()
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t_wall = time.perf_counter()
+    ev0.record(s)
+    for _ in range(a.steps):
+        rec = step()[0]
+    ev1.record(s)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t_wall
+    ms_step = ev0.elapsed_time(ev1) / a.steps
+    value = B / (ms_step / 1e3)
+    rel_err_truth = (torch.linalg.vector_norm((rec - x_true).double()) /
+                     torch.linalg.vector_norm(x_true.double())).item()
+    log(f"{a.steps} steps: {ms_step:.1f} ms/step (wall {wall / a.steps * 1e3:.1f}), {value:.2f} recon/s")
+
+    # ---- roofline of the operator kernels (HIP events on the launch stream) ----
+    P = N * N
+    sir_bytes = 0
+    if use_sir:
+        se = model.Data.SPMRTimestepIndexStartAndEnd
+        ref = np.unique(model.Data.SPMRIndices) - 1
+        sir_bytes = 4 * int(np.sum(np.maximum(se[ref, 1] - se[ref, 0] + 1, 0)))
+    alg_bytes = 4 * P * D * (2 if use_sir else 1) + sir_bytes + 4 * B * P + 4 * B * D * nT
+    reps = 10
+    t_gather = model.plan.time_stage("gather", B, reps)
+    t_scatter = model.plan.time_stage("scatter", B, reps)
+    t_s2b = model.plan.time_stage("sino_to_bins", B, reps)
+    t_b2s = model.plan.time_stage("bins_to_sino", B, reps)
+    # per 50-iteration solve: 151 forward and 150 transpose applications
+    dom = "gather" if t_gather * 150 >= t_scatter * 151 else "scatter"
+    t_dom = t_gather if dom == "gather" else t_scatter
+    achieved = alg_bytes / (t_dom * 1e-3) / 1e9
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", "traffic.json")
+    if os.path.exists(tfile):
+        try:
+            tr = json.load(open(tfile))
+            key = f"{'sir' if use_sir else 'das'}_{dom}_B{B}_N{N}"
+            traffic = tr.get(key)
+        except Exception:
+            traffic = None
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                "kernel": f"k_{dom}<{'SIR' if use_sir else 'DAS'}>", "algorithmic_bytes": int(alg_bytes),
+                "kernel_ms": round(t_dom, 4),
+                "stage_ms": {"scatter": round(t_scatter, 4), "bins_to_sino": round(t_b2s, 4),
+                             "sino_to_bins": round(t_s2b, 4), "gather": round(t_gather, 4)}}
+    log(f"stages (ms, B={B}): scatter {t_scatter:.3f} bins_to_sino {t_b2s:.3f} "
+        f"sino_to_bins {t_s2b:.3f} gather {t_gather:.3f}; {dom} {achieved:.0f} GB/s")
+
+    # ---- CPU baseline: FP64 oracle on a bounded sample --------------------------
+    cpu = None
+    if not a.no_cpu_baseline:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as O
+        sig_px = np.floor(h / 2 * 1e6 + 0.5) / 1e6
+        kz, kx = O.full_kernels(N, N, h, h, sig_px)
+        kw = dict(kz=kz, kx=kx, ir=model.Data.IR)
+        if use_sir:
+            kw.update(sidx=model.Data.SPMRIndices, sir=model.Data.SPMR,
+                      se=model.Data.SPMRTimestepIndexStartAndEnd)
+        orc = O.Model(N, N, model.Data.absoluteTravelTimeIndices.reshape(D, -1), nT,
+                      model.Data.circularInterpolationRadiusInSamples, sigma=model.normalizationFactor, **kw)
+        x0 = ph[:1].astype(np.float64)
+        y0 = y[:1].double().cpu().numpy()
+        tc = time.perf_counter()
+        for _ in range(a.cpu_sample_reps):
+            orc.forward(x0)
+        t_f = (time.perf_counter() - tc) / a.cpu_sample_reps
+        tc = time.perf_counter()
+        for _ in range(a.cpu_sample_reps):
+            orc.transpose(y0)
+        t_a = (time.perf_counter() - tc) / a.cpu_sample_reps
+        t_solve = 151 * t_f + 150 * t_a
+        cpu = {"value": round(1.0 / t_solve, 6), "unit": "recon/s", "cores": O.num_threads(), "kind": "port",
+               "sample": f"{a.cpu_sample_reps} forward + {a.cpu_sample_reps} transpose FP64 applications of the "
+                         f"same {N}x{N} model (A: {t_f:.2f} s, A': {t_a:.2f} s), scaled to one 50-iteration "
+                         f"solve = 151 A + 150 A'"}
+        log(f"cpu baseline: A {t_f:.2f} s, A' {t_a:.2f} s -> {1 / t_solve:.5f} recon/s on {O.num_threads()} threads")
+
+    out = {"metric": METRIC, "value": round(value, 4), "unit": "recon/s", "n_gpus": 1, "steps": a.steps,
+           "warmup": a.warmup, "ms_per_step": round(ms_step, 3), "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+           "config": {"workload": f"rec_nn_with_L2_reg {a.iters} it, {N}x{N} px at 100 um, "
+                                  f"{'SIR' if use_sir else 'no SIR'}, {D} ch x {nT} samples, batch {B} "
+                                  "independent seeded sinograms (plain cold start)",
+                      "batch": B, "grid": N, "detectors": D, "samples": nT, "sir": use_sir,
+                      "outer_iterations": a.iters, "lambdaL2": lam},
+           "rel_err_vs_phantom": round(rel_err_truth, 5),
+           "roofline": roofline, "cpu_baseline": cpu}
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

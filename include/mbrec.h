@@ -112,6 +112,12 @@ size_t mbrec_plan_device_bytes(mbrec_plan plan);
 /* Fixed-point travel-time format: number of fractional bits. */
 int mbrec_plan_tau_frac_bits(mbrec_plan plan);
 
+/* Profiling: average duration (ms, HIP events on stream s) of one operator
+ * stage over reps launches for a batch of B: 0 = forward pair scatter,
+ * 1 = bins -> samples (+IR), 2 = samples -> bins (+IR), 3 = transpose pair
+ * gather.  Inputs are zero buffers; timing does not depend on their values. */
+int mbrec_plan_time_stage(mbrec_plan plan, int stage, int B, int reps, mbrec_stream s, float *ms);
+
 /* Bare MEX loop (no prefilter, IR or 1/sigma), batch of B planar images
  * [B][P] -> sinograms [B][D][nT] and back. */
 int mbrec_mex_forward_dev(mbrec_plan plan, const float *img, float *sino, int B, mbrec_stream s);
@@ -133,6 +139,13 @@ int mbrec_model_transpose_dev(mbrec_plan plan, const float *sino, float *img, in
 int mbrec_rec_nn_with_L2_reg_dev(mbrec_plan plan, const float *data, int B, int coupled,
                                  int num_iter, double lambdaL2, const float *init,
                                  float *rec, double *lcurve, mbrec_stream s);
+
+/* rec_nn_with_L1_eye_reg (reconstruction/rec_nn_with_L1_eye_reg.m:1-50) and
+ * nnls_L1_eye_matrix_less (nnls_L1_eye_matrix_less.m:1-67): split Bregman with
+ * 2 CGS iterations of (A'A + I) per outer iteration, lambdaParam = 1, etaReg =
+ * regParam, started from A'y.  data [B][D][nT], rec [B][P]; coupled as above. */
+int mbrec_rec_nn_with_L1_eye_reg_dev(mbrec_plan plan, const float *data, int B, int coupled,
+                                     int num_iter, double regParam, float *rec, mbrec_stream s);
 
 #ifdef __cplusplus
 }

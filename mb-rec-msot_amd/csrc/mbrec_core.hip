@@ -362,4 +362,11 @@ int mbrec_model_transpose_dev(mbrec_plan h, const float *sino, float *img, int B
     return op_transpose(&h->p, sino, img, B, true, (hipStream_t)s);
 }
 
+int mbrec_plan_time_stage(mbrec_plan h, int stage, int B, int reps, mbrec_stream s, float *ms)
+{
+    if (!h || !ms || B <= 0 || reps <= 0)
+        return set_error(MBREC_ERR_ARG, "mbrec:args", "plan_time_stage: invalid arguments");
+    return time_stage(&h->p, stage, B, reps, (hipStream_t)s, ms);
+}
+
 }  // extern "C"

@@ -96,6 +96,8 @@ int launch_gather(Plan *p, float *img, int B, float scale, hipStream_t s);      
 int launch_prefilter(Plan *p, const float *in, float *out, float *tmp, int B, bool transp,
                      hipStream_t s);
 
+int time_stage(Plan *p, int which, int B, int reps, hipStream_t s, float *ms);
+
 int op_forward(Plan *p, const float *img, float *sino, int B, bool model, hipStream_t s);
 int op_transpose(Plan *p, const float *sino, float *img, int B, bool model, hipStream_t s);
 

@@ -468,4 +468,44 @@ int op_transpose(Plan *p, const float *sino, float *img, int B, bool model, hipS
     return launch_gather(p, img, B, scale, s);
 }
 
+// ---------------------------------------------------------------------------
+// profiling: average duration of one pipeline stage, HIP events on stream s
+// ---------------------------------------------------------------------------
+int time_stage(Plan *p, int which, int B, int reps, hipStream_t s, float *ms)
+{
+    int rc = ensure_bins(p, B);
+    if (rc) return rc;
+    if ((rc = ensure_tmp(p, B))) return rc;
+    float *img = p->d_img_tmp;
+    float *sino = nullptr;
+    MBREC_HIP(hipMalloc((void **)&sino, sizeof(float) * (size_t)B * p->D * p->nT));
+    MBREC_HIP(hipMemsetAsync(sino, 0, sizeof(float) * (size_t)B * p->D * p->nT, s));
+    MBREC_HIP(hipMemsetAsync(img, 0, sizeof(float) * (size_t)B * p->P, s));
+    auto run = [&]() -> int {
+        switch (which) {
+        case 0: return launch_scatter(p, img, B, s);
+        case 1: return launch_bins_to_sino(p, sino, B, true, s);
+        case 2: return launch_sino_to_bins(p, sino, B, true, s);
+        case 3: return launch_gather(p, img, B, 1.0f, s);
+        default: return set_error(MBREC_ERR_ARG, "mbrec:args", "unknown stage %d", which);
+        }
+    };
+    hipEvent_t a, b;
+    MBREC_HIP(hipEventCreate(&a));
+    MBREC_HIP(hipEventCreate(&b));
+    if ((rc = run())) return rc;   // warm-up
+    MBREC_HIP(hipEventRecord(a, s));
+    for (int i = 0; i < reps; ++i)
+        if ((rc = run())) return rc;
+    MBREC_HIP(hipEventRecord(b, s));
+    MBREC_HIP(hipEventSynchronize(b));
+    float t = 0.0f;
+    MBREC_HIP(hipEventElapsedTime(&t, a, b));
+    *ms = t / reps;
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    MBREC_HIP(hipFree(sino));
+    return MBREC_OK;
+}
+
 }  // namespace mbrec

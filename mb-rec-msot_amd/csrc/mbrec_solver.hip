@@ -14,17 +14,12 @@
 // sinograms in one group = the reference's stacked multi-wavelength call;
 // one group per sinogram = independent batched solves).  No host round trip
 // happens inside a solve, so the launch sequence is graph-capturable.
-#include "mbrec_internal.h"
+#include "mbrec_reduce.h"
 
 #include <cmath>
 #include <vector>
 
 namespace mbrec {
-
-constexpr int kRedBlocks = 64;     // reduction blocks per batch item
-constexpr int kMaxQ = 6;           // reduction slots
-
-enum { Q_A = 0, Q_B = 1, Q_C = 2, Q_D = 3, Q_E = 4, Q_F = 5 };
 
 struct Scal {
     double alpha, beta, c, s, phibar, phi, thet, rho;
@@ -32,31 +27,6 @@ struct Scal {
     double inv_u, inv_v;           // pending normalisations of u and v
     int stag, flag, active, iter;
 };
-
-// block-wide FP64 sum, written to part[q][b][blk]
-__device__ __forceinline__ void block_sum_store(double v, double *part, int q, int b, int B)
-{
-    __shared__ double sh[8];
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    if (lane == 0) sh[w] = v;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        double t = 0.0;
-        for (int i = 0; i < (int)(blockDim.x >> 6); ++i) t += sh[i];
-        part[((size_t)q * B + b) * kRedBlocks + blockIdx.x] = t;
-    }
-    __syncthreads();
-}
-
-__device__ __forceinline__ double group_sum(const double *part, int q, int g, int B, int coupled)
-{
-    double t = 0.0;
-    const int b0 = coupled ? 0 : g, b1 = coupled ? B : g + 1;
-    for (int b = b0; b < b1; ++b)
-        for (int i = 0; i < kRedBlocks; ++i) t += part[((size_t)q * B + b) * kRedBlocks + i];
-    return t;
-}
 
 struct Consts {
     float sl, sb, sr;   // sqrt(lambda), sqrt(beta), sqrt(lambda)*sqrt(lambdaL2)
@@ -393,8 +363,6 @@ struct Workspace {
         return MBREC_OK;
     }
 };
-
-static inline dim3 vgrid(int B) { return dim3(kRedBlocks, B); }
 
 static int lsqr_solve(Plan *p, Workspace &ws, const float *y, int B, int G, int maxit, Consts k,
                       hipStream_t s)

@@ -46,6 +46,7 @@ SIGNATURES = [
     ("mbrec_plan_destroy", C.c_int, [c_void_p]),
     ("mbrec_plan_device_bytes", C.c_size_t, [c_void_p]),
     ("mbrec_plan_tau_frac_bits", C.c_int, [c_void_p]),
+    ("mbrec_plan_time_stage", C.c_int, [c_void_p, C.c_int, C.c_int, C.c_int, c_void_p, c_float_p]),
     ("mbrec_mex_forward_dev", C.c_int, [c_void_p, c_void_p, c_void_p, C.c_int, c_void_p]),
     ("mbrec_mex_transpose_dev", C.c_int, [c_void_p, c_void_p, c_void_p, C.c_int, c_void_p]),
     ("mbrec_model_forward_dev", C.c_int, [c_void_p, c_void_p, c_void_p, C.c_int, c_void_p]),
@@ -53,6 +54,8 @@ SIGNATURES = [
     ("mbrec_rec_nn_with_L2_reg_dev", C.c_int,
      [c_void_p, c_void_p, C.c_int, C.c_int, C.c_int, C.c_double, c_void_p, c_void_p, c_void_p,
       c_void_p]),
+    ("mbrec_rec_nn_with_L1_eye_reg_dev", C.c_int,
+     [c_void_p, c_void_p, C.c_int, C.c_int, C.c_int, C.c_double, c_void_p, c_void_p]),
 ]
 
 

@@ -88,6 +88,15 @@ class DevicePlan:
     def tau_frac_bits(self):
         return int(_lib.lib().mbrec_plan_tau_frac_bits(self.h))
 
+    STAGES = {"scatter": 0, "bins_to_sino": 1, "sino_to_bins": 2, "gather": 3}
+
+    def time_stage(self, stage, B=1, reps=20):
+        """Average device time (ms) of one operator stage, HIP events on the current stream."""
+        ms = C.c_float()
+        _lib.check(_lib.lib().mbrec_plan_time_stage(self.h, self.STAGES[stage], int(B), int(reps),
+                                                    _lib.stream_handle(), C.byref(ms)))
+        return float(ms.value)
+
     # device-tensor operators --------------------------------------------------
     def _run(self, fn, src, dst, B):
         _lib.check(fn(self.h, src.data_ptr(), dst.data_ptr(), int(B), _lib.stream_handle()))

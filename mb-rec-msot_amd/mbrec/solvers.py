@@ -82,3 +82,35 @@ def rec_nn_with_L2_reg(model, data_filt, num_iter_nn_outer, lambdaL2=0.0, RegL2=
             out = out[0]
         rec = out
     return rec, errimg, errreg, 0.0, err, t
+
+
+def rec_nn_with_L1_eye_reg(model, data_filt, num_iter, regParam, *, coupled=True):
+    """rec_nn_with_L1_eye_reg.m:1-50: non-negative L1 (identity) reconstruction by split
+    Bregman (nnls_L1_eye_matrix_less.m) with MATLAB-cgs inner solves, started from A'y.
+
+    Returns (rec_img, lCurveErrImg, lCurveErrReg, err, t); lCurveErrImg = ||A rec - y||^2
+    (:38) and lCurveErrReg = ||rec||_1 (:40), per sinogram when coupled=False."""
+    import torch
+    y, host = _to_device(data_filt)
+    D, nT = model.shape_sinogram
+    B = y.numel() // (D * nT)
+    y = y.view(B, D, nT)
+    rec = torch.empty((B, *model.shape_image), dtype=torch.float32, device=y.device)
+    t0 = time.perf_counter()
+    _lib.check(_lib.lib().mbrec_rec_nn_with_L1_eye_reg_dev(
+        model.plan.h, y.data_ptr(), int(B), int(bool(coupled)), int(num_iter), float(regParam),
+        rec.data_ptr(), _lib.stream_handle()))
+    el = time.perf_counter() - t0
+    f = model.plan.forward(rec.view(B, -1))
+    res = ((f.double() - y.double()) ** 2).sum(dim=(1, 2))
+    l1 = rec.double().abs().sum(dim=(1, 2))
+    if coupled:
+        e_img, e_reg = float(res.sum()), float(l1.sum())
+    else:
+        e_img, e_reg = res.cpu().numpy(), l1.cpu().numpy()
+    err = np.zeros(int(num_iter))
+    t = np.full(int(num_iter), el / max(1, int(num_iter)))
+    if host:
+        out = rec.cpu().numpy().astype(np.float64)
+        rec = out[0] if np.ndim(data_filt) == 2 else out
+    return rec, e_img, e_reg, err, t

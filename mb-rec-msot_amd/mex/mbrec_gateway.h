@@ -1,0 +1,46 @@
+/*
+ * mbrec_gateway.h -- shared helpers of the four MEX gateways.
+ *
+ * Each gateway keeps the reference MEX's name, argument order, type checks and
+ * error identifiers (reconstruction_model_based/models/forwardMex.cpp:30-50
+ * and the corresponding lines of the other three files), allocates the output
+ * with mxCreateNumericArray / mxCreateDoubleMatrix like the reference, and
+ * forwards to libmbrec (include/mbrec.h).  Build inside MATLAB, e.g.
+ *   mex -R2018a -I<repo>/include forwardMex.cpp -L<repo>/mb-rec-msot_amd/mbrec -lmbrec
+ */
+#pragma once
+#include "mex.h"
+#include "mbrec.h"
+
+static void mbrec_gateway_fail(void)
+{
+    mexErrMsgIdAndTxt(mbrec_last_error_id(), "%s", mbrec_last_error());
+}
+
+static void mbrec_gateway_at_exit(void) { mbrec_mex_cache_clear(); }
+
+static void mbrec_gateway_register_exit(void)
+{
+    static int done = 0;
+    if (!done) {
+        mexAtExit(mbrec_gateway_at_exit);
+        done = 1;
+    }
+}
+
+/* reference check: mxIsDouble && !mxIsComplex, else <id> "<name> must be type double." */
+static void mbrec_need_double(const char *mexname, const mxArray *a, int pos)
+{
+    if (!mxIsDouble(a) || mxIsComplex(a)) {
+        mbrec_mex_type_error(mexname, pos, 0);
+        mbrec_gateway_fail();
+    }
+}
+
+static void mbrec_need_int32(const char *mexname, const mxArray *a, int pos)
+{
+    if (!mxIsInt32(a)) {
+        mbrec_mex_type_error(mexname, pos, 1);
+        mbrec_gateway_fail();
+    }
+}

@@ -292,3 +292,80 @@ def test_nnls_L2_fixed_point_is_nonnegative_least_squares():
     ya = np.concatenate([y, np.zeros(A.shape[1])])
     ref = lsq_linear(Aa, ya, bounds=(0, np.inf), tol=1e-14, lsmr_tol="auto", max_iter=10000).x
     assert rel_l2(rec.ravel(), ref) < 1e-3
+
+
+# --- L1 split Bregman with MATLAB-cgs inner solves ---------------------------------
+def _cgs_by_cg_polynomial(M, b, k):
+    """CGS iterates for symmetric M with shadow residual r0 = b, x0 = 0, from the identity
+    r_k(CGS) = phi_k(M)^2 r0 where phi_k is CG's residual polynomial (Sonneveld 1989):
+    phi_k is recovered from textbook CG and applied once more."""
+    r0 = b.copy()
+    xs, rs = [], []
+    x = np.zeros_like(b); r = r0.copy(); p = r.copy()
+    krylov = [r0]
+    for _ in range(k):
+        krylov.append(M @ krylov[-1])
+    for j in range(1, k + 1):
+        a = (r @ r) / (p @ (M @ p))
+        x = x + a * p
+        rn = r - a * (M @ p)
+        p = rn + ((rn @ rn) / (r @ r)) * p
+        r = rn
+        # phi_j coefficients: r_CG = sum_i c_i M^i r0 with c_0 = 1
+        K = np.stack(krylov[1:j + 1], axis=1)
+        c, *_ = np.linalg.lstsq(K, r - r0, rcond=None)
+        Kr = [r]
+        for _ in range(j):
+            Kr.append(M @ Kr[-1])
+        phi_r = r + np.stack(Kr[1:], axis=1) @ c      # phi_j(M) r_CG = phi_j(M)^2 r0
+        rs.append(phi_r)
+        xs.append(np.linalg.solve(M, r0 - phi_r))
+    return xs, rs
+
+
+def _cgs_matlab_return(M, b, k):
+    xs, rs = _cgs_by_cg_polynomial(M, b, k)
+    norms = [np.linalg.norm(r) for r in rs]
+    imin = int(np.argmin([np.linalg.norm(b)] + norms)) - 1      # -1 -> x0 = 0
+    xmin = np.zeros_like(b) if imin < 0 else xs[imin]
+    rcomp = np.linalg.norm(b - M @ xmin)
+    return xmin if rcomp <= norms[-1] else xs[-1]
+
+
+def _nnls_L1_reference(A, y, u0, eta, nIter):
+    """nnls_L1_eye_matrix_less.m:24-55 with lambda = 1 and the CGS polynomial form."""
+    n = A.shape[1]
+    M = A.T @ A + np.eye(n)
+    aty = A.T @ y
+    b = np.zeros(n); u = u0.copy()
+    for _ in range(nIter):
+        d = np.maximum(u - b, 0)
+        bin_ = d + b + aty - eta
+        u = _cgs_matlab_return(M, bin_, 2)
+        b = b + d - u
+    return u
+
+
+def test_cgs_polynomial_identity_matches_recurrence():
+    g = rng(90)
+    Q = g.standard_normal((15, 15)); M = Q @ Q.T + np.eye(15)
+    b = g.standard_normal(15)
+    xs, rs = _cgs_by_cg_polynomial(M, b, 2)
+    # one CGS step by its recurrence: x1 = alpha (2 r0 - alpha M r0)
+    a = (b @ b) / (b @ (M @ b))
+    np.testing.assert_allclose(xs[0], a * (2 * b - a * (M @ b)), rtol=1e-10)
+    for x, r in zip(xs, rs):
+        np.testing.assert_allclose(M @ x, b - r, rtol=1e-9, atol=1e-10)
+
+
+@pytest.mark.parametrize("eta", [0.0, 0.05])
+def test_nnls_L1_matches_cg_polynomial_restatement(eta):
+    m = _tiny_model(seed=80, Nz=5, Nx=4, D=4, nT=40)
+    A = _dense_of(m)
+    g = rng(81)
+    y = A @ np.maximum(g.standard_normal(A.shape[1]), 0) + 0.05 * g.standard_normal(A.shape[0])
+    rec, cnt = m.rec_nn_with_L1_eye_reg(y.reshape(1, m.D, m.nT), 15, eta)
+    want = _nnls_L1_reference(A, y, A.T @ y, eta, 15)
+    assert rel_l2(rec.ravel(), want) < 1e-8
+    # 1 start A'y + per outer iteration: A'y, 2 x 2 products of (A'A + I), 1 check product
+    assert tuple(cnt) == (15 * 5, 1 + 15 * 6)
