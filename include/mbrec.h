@@ -147,6 +147,46 @@ int mbrec_rec_nn_with_L2_reg_dev(mbrec_plan plan, const float *data, int B, int 
 int mbrec_rec_nn_with_L1_eye_reg_dev(mbrec_plan plan, const float *data, int B, int coupled,
                                      int num_iter, double regParam, float *rec, mbrec_stream s);
 
+/* rec_nn_with_TV_reg (reconstruction/rec_nn_with_TV_reg.m:1-55) and
+ * nnls_sparsa_based_reg_matrix_less (nnls_sparsa_based_reg_matrix_less.m:1-86): the
+ * outer splitting with SpaRSA inner solves (inner_iterations = MaxiterA = 2 in the
+ * reference), Psi = split-Bregman isotropic-TV prox (SB_ITV2.m), Phi = isotropic TV
+ * (ITV.m).  Square grids only.  lcurve (host, nullable) receives per group
+ * {||A rec - y||^2, Phi(rec)}. */
+int mbrec_rec_nn_with_TV_reg_dev(mbrec_plan plan, const float *data, int B, int coupled, int num_iter,
+                                 double regParam, int inner_iterations, float *rec, double *lcurve,
+                                 mbrec_stream s);
+
+/* ---- 3. Secondary reconstruction paths (FP64) ------------------------------ */
+
+/* dmas_cf (reconstruction_backprojection/dmas_cf.m:19-107): delay-multiply-and-sum
+ * with coherence factor.  xpix/zpix: device pixel coordinates [P] (fov_x(:), fov_z(:));
+ * sig: device FP32 sinograms [W][D][nT] (MATLAB [nT x D x W]); out: device FP64
+ * [4][W][P] = {dmas_cf, dmas, cf, das}.  An arrival sample outside [1, nT] is an
+ * error ("dmas_cf:indexOutOfRange"), as MATLAB's indexing would be. */
+int mbrec_dmas_cf_dev(const double *xpix, const double *zpix, int P, const float *sig, int W, int D,
+                      int nT, double c, double fs, int cropped, double coverage_deg, double radius,
+                      double *out, mbrec_stream s);
+
+/* backproject_waveeq (reconstruction_backprojection/backproject_waveeq.m:1-50).
+ * xv [Nx], zv [Nz]: host grid vectors of the meshgrid fov_x / fov_y; sig: device FP64
+ * [W][D][nT]; psf: host [pm x pn] (time x detector); K: device [Nt x Nt] column-major
+ * with Nt >= nT + cropped (bp_kernel output); out: device FP64 [W][Nx][Nz]. */
+int mbrec_backproject_waveeq_dev(const double *xv, const double *zv, int Nz, int Nx, const double *sig,
+                                 int W, int D, int nT, double c, double fs, int cropped,
+                                 double coverage_deg, double radius, const double *psf, int pm, int pn,
+                                 const double *K, int Nt, double *out, mbrec_stream s);
+
+/* bp_kernel (reconstruction_backprojection/bp_kernel.m:1-12): K = -Im(H diag(w lambda) H^H)
+ * with H = H0^(1)(t_j lambda_l), trapezoid weights w.  t: host [Nt]; K: device [Nt x Nt]. */
+int mbrec_bp_kernel_dev(const double *t, int Nt, double *K, mbrec_stream s);
+
+/* upsample_detectors_in_sinogram (load_and_preprocess/upsample_detectors_in_sinogram.m:1-13)
+ * as a linear map: out_b [nT x Dup] = sig_b [nT x D] * U', U [Dup x D] column-major
+ * (device, the not-a-knot spline map), for nbatch sinograms [nbatch][D][nT]. */
+int mbrec_upsample_detectors_dev(const double *U, int Dup, int D, const double *sig, int nT, int nbatch,
+                                 double *out, mbrec_stream s);
+
 #ifdef __cplusplus
 }
 #endif

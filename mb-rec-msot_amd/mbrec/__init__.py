@@ -9,7 +9,7 @@ from .discretization import Discretization  # noqa: F401
 from .probe import Probe  # noqa: F401
 
 __all__ = ["MbrecError", "lib", "LIB_PATH", "Discretization", "Probe", "MSOTmodel",
-           "define_model_for_reconstruction", "rec_nn_with_L2_reg", "rec_nn_with_L1_eye_reg", "forwardMex", "transposeMex",
+           "define_model_for_reconstruction", "rec_nn_with_L2_reg", "rec_nn_with_L1_eye_reg", "rec_nn_with_TV_reg", "forwardMex", "transposeMex",
            "forwardDelayAndSumMex", "transposeDelayAndSumMex"]
 
 
@@ -17,7 +17,7 @@ def __getattr__(name):
     if name in ("MSOTmodel", "define_model_for_reconstruction", "DevicePlan", "travel_time_indices"):
         from . import model
         return getattr(model, name)
-    if name in ("rec_nn_with_L2_reg", "rec_nn_with_L1_eye_reg"):
+    if name in ("rec_nn_with_L2_reg", "rec_nn_with_L1_eye_reg", "rec_nn_with_TV_reg"):
         from . import solvers
         return getattr(solvers, name)
     if name in ("forwardMex", "transposeMex", "forwardDelayAndSumMex", "transposeDelayAndSumMex", "MexError"):

@@ -114,3 +114,32 @@ def rec_nn_with_L1_eye_reg(model, data_filt, num_iter, regParam, *, coupled=True
         out = rec.cpu().numpy().astype(np.float64)
         rec = out[0] if np.ndim(data_filt) == 2 else out
     return rec, e_img, e_reg, err, t
+
+
+def rec_nn_with_TV_reg(model, data_filt, num_iter, regParam, *, coupled=True, inner_iterations=2):
+    """rec_nn_with_TV_reg.m:1-55: non-negative TV reconstruction (square grids only).
+
+    Returns (rec_img, lCurveErrImg, lCurveErrReg, err, t) with lCurveErrImg = ||A rec - y||^2
+    (:51) and lCurveErrReg = Phi(rec) = isotropic TV (:52), per sinogram when coupled=False.
+    The inner SpaRSA runs ``inner_iterations`` (MaxiterA, :27) iterations."""
+    import torch
+    y, host = _to_device(data_filt)
+    D, nT = model.shape_sinogram
+    B = y.numel() // (D * nT)
+    y = y.view(B, D, nT)
+    rec = torch.empty((B, *model.shape_image), dtype=torch.float32, device=y.device)
+    G = 1 if coupled else B
+    lc = np.zeros(2 * G)
+    t0 = time.perf_counter()
+    _lib.check(_lib.lib().mbrec_rec_nn_with_TV_reg_dev(
+        model.plan.h, y.data_ptr(), int(B), int(bool(coupled)), int(num_iter), float(regParam),
+        int(inner_iterations), rec.data_ptr(), lc.ctypes.data, _lib.stream_handle()))
+    el = time.perf_counter() - t0
+    e_img = lc[0] if coupled else lc[0::2]
+    e_reg = lc[1] if coupled else lc[1::2]
+    err = np.zeros(int(num_iter))
+    t = np.full(int(num_iter), el / max(1, int(num_iter)))
+    if host:
+        out = rec.cpu().numpy().astype(np.float64)
+        rec = out[0] if np.ndim(data_filt) == 2 else out
+    return rec, e_img, e_reg, err, t

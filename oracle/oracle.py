@@ -411,3 +411,310 @@ def lsqr_krylov(A, b, maxit, x0):
     Kb = np.stack(K, axis=1)
     coef, *_ = np.linalg.lstsq(A @ Kb, r0, rcond=None)
     return x0 + Kb @ coef
+
+
+# ---------------------------------------------------------------------------
+# secondary paths (numpy FP64, statement-by-statement restatements)
+# ---------------------------------------------------------------------------
+def matlab_round(v):
+    return np.sign(v) * np.floor(np.abs(v) + 0.5)
+
+
+def _bp_angles(D, coverage):
+    off = (180.0 - coverage) / 2.0
+    return -(off + np.arange(D) * (coverage / (D - 1))) * np.pi / 180.0
+
+
+def dmas_cf(sig, fov_x, fov_z, c, fs, cropped, coverage, radius):
+    """dmas_cf.m:19-107 (FP64).  sig (W, D, nT); fov_x/fov_z (Nx, Nz).  Pinned by the
+    pairwise-product identity of DMAS and the Cauchy-Schwarz bound of CF (tests)."""
+    sig = np.asarray(sig, dtype=np.float64)
+    W, D, nT = sig.shape
+    dt = 1.0 / fs
+    ang = _bp_angles(D, coverage)
+    t1 = (1 + cropped) * dt
+    s = sig / np.max(np.abs(sig))                               # :99
+    sq = np.sign(s) * np.sqrt(np.abs(s))                        # :103
+    fx, fz = np.asarray(fov_x).ravel(), np.asarray(fov_z).ravel()
+    out = np.zeros((4, W, fx.size))
+    for w in range(W):
+        ssq = np.zeros(fx.size); ssum = np.zeros(fx.size); num = np.zeros(fx.size); den = np.zeros(fx.size)
+        for p in range(D):
+            T = matlab_round((np.sqrt(np.abs(fx - np.cos(ang[p]) * radius) ** 2 +
+                                      np.abs(fz - np.sin(ang[p]) * radius) ** 2) / c - t1) * fs + 1).astype(np.int64)
+            if T.min() < 1 or T.max() > nT:
+                raise IndexError("arrival sample outside the sinogram")
+            ssq = ssq + sq[w, p, T - 1] ** 2
+            ssum = ssum + sq[w, p, T - 1]
+            num = num + s[w, p, T - 1]
+            den = den + s[w, p, T - 1] ** 2
+        dmas = 0.5 * (ssum ** 2 - ssq)
+        cf = num ** 2 / (1e-6 + D * den)
+        out[:, w] = (dmas * cf, dmas, cf, num)
+    return out.reshape((4, W) + np.shape(fov_x))
+
+
+def bp_kernel(t):
+    """bp_kernel.m:5-12 with scipy's Hankel function (tiny Nt only: O(Nt^3))."""
+    from scipy.special import hankel1
+    t = np.asarray(t, dtype=np.float64).ravel()
+    Nt = t.size
+    dl = np.pi / (t[-1] - t[0])
+    lam = dl * np.arange(1, Nt + 1)
+    H = hankel1(0, np.outer(t, lam))
+    K = np.zeros((Nt, Nt), dtype=complex)
+    for j in range(Nt):
+        K[j] = np.trapezoid(H[j][None, :] * np.conj(H) * lam[None, :], lam, axis=1)
+    return -np.imag(K)
+
+
+def divergence(xv, yv, U, V):
+    """MATLAB divergence(X, Y, U, V) for meshgrid data given as (Nx, Nz) arrays:
+    central differences inside, one-sided at the edges (gradient)."""
+    return np.gradient(U, xv, axis=0, edge_order=1) + np.gradient(V, yv, axis=1, edge_order=1)
+
+
+def backproject_waveeq(sig, fov_x, fov_y, c, fs, cropped, coverage, radius, psf, K):
+    """backproject_waveeq.m:13-50 (FP64).  sig (W, D, nT); fov_x, fov_y (Nx, Nz); psf (pm, pn)."""
+    from scipy.signal import convolve
+    sig = np.asarray(sig, dtype=np.float64)
+    W, D, nT = sig.shape
+    dt = 1.0 / fs
+    ang = _bp_angles(D, coverage)
+    tsam = np.arange(1, nT + 1) + cropped
+    tsec = tsam * dt
+    s = sig / c
+    cum = np.concatenate([np.zeros((W, D, 1)), np.cumsum(np.diff(tsec)[None, None] * (s[..., :-1] + s[..., 1:]) / 2,
+                                                         axis=-1)], axis=-1)
+    S = cum / tsec[None, None, :]
+    ps = np.atleast_2d(np.asarray(psf, dtype=np.float64))
+    Sc = np.empty_like(S)
+    for w in range(W):
+        full = convolve(S[w].T, ps, mode="full")        # (nT, D) with psf (pm, pn)
+        r0, c0 = ps.shape[0] // 2, ps.shape[1] // 2
+        Sc[w] = full[r0:r0 + nT, c0:c0 + D].T
+    Sc = Sc / ps.sum()
+    fx, fy = np.asarray(fov_x, dtype=np.float64), np.asarray(fov_y, dtype=np.float64)
+    out = np.zeros((W,) + fx.shape)
+    Ksub = np.asarray(K)[np.ix_(tsam - 1, tsam - 1)]
+    for w in range(W):
+        Swl = Ksub @ Sc[w].T * dt                         # (nT, D)
+        px = np.zeros(fx.shape); py = np.zeros(fx.shape)
+        for p in range(D):
+            T = matlab_round((np.sqrt(np.abs(fx - np.cos(ang[p]) * radius) ** 2 +
+                                      np.abs(fy - np.sin(ang[p]) * radius) ** 2) / c - tsec[0]) * fs + 1).astype(np.int64)
+            px = px + Swl[T - 1, p] * np.cos(ang[p]) * dt * c
+            py = py + Swl[T - 1, p] * np.sin(ang[p]) * dt * c
+        div = divergence(fx[:, 0], fy[0, :], px, py)
+        out[w] = div[::-1, :]                             # fliplr: reverse x (columns in MATLAB)
+    return out
+
+
+def not_a_knot_spline_matrix(D, factor):
+    """Interpolation matrix of the not-a-knot cubic spline on knots 1:f:D*f evaluated at
+    1..D*f, built from its defining equations (C2 continuity + not-a-knot ends), solved
+    densely -- an independent construction of the upsampling map."""
+    x = np.arange(1, D * factor + 1, factor, dtype=np.float64)
+    q = np.arange(1, D * factor + 1, dtype=np.float64)
+    n = D
+    h = np.diff(x)
+    U = np.zeros((q.size, n))
+    for j in range(n):
+        y = np.zeros(n); y[j] = 1.0
+        # unknown second derivatives M_0..M_{n-1}
+        A = np.zeros((n, n)); rhs = np.zeros(n)
+        for i in range(1, n - 1):
+            A[i, i - 1] = h[i - 1]; A[i, i] = 2 * (h[i - 1] + h[i]); A[i, i + 1] = h[i]
+            rhs[i] = 6 * ((y[i + 1] - y[i]) / h[i] - (y[i] - y[i - 1]) / h[i - 1])
+        # not-a-knot: third derivative continuous at x_1 and x_{n-2}
+        A[0, 0] = h[1]; A[0, 1] = -(h[0] + h[1]); A[0, 2] = h[0]
+        A[n - 1, n - 3] = h[n - 2]; A[n - 1, n - 2] = -(h[n - 3] + h[n - 2]); A[n - 1, n - 1] = h[n - 3]
+        M = np.linalg.solve(A, rhs)
+        k = np.clip(np.searchsorted(x, q, side="right") - 1, 0, n - 2)
+        t = q - x[k]
+        hk = h[k]
+        a = y[k]
+        b = (y[k + 1] - y[k]) / hk - hk * (2 * M[k] + M[k + 1]) / 6
+        cc = M[k] / 2
+        d = (M[k + 1] - M[k]) / (6 * hk)
+        U[:, j] = a + b * t + cc * t ** 2 + d * t ** 3
+    return U
+
+
+# ---------------------------------------------------------------------------
+# TV path: DiffOper, ITV, SB_ITV2, SpaRSA, nnls_sparsa_based_reg_matrix_less
+# ---------------------------------------------------------------------------
+def diff_oper(N):
+    """libraries/SplitBregmanTVdenoising/DiffOper.m:1-7: D = backward difference with a zero
+    first row; B = [kron(I, D); kron(D, I)] (sparse, structural zeros removed)."""
+    import scipy.sparse as sp
+    D = sp.lil_matrix((N, N))
+    for i in range(1, N):
+        D[i, i] = 1.0
+        D[i, i - 1] = -1.0
+    D = D.tocsr()
+    I = sp.identity(N, format="csr")
+    B = sp.vstack([sp.kron(I, D), sp.kron(D, I)]).tocsr()
+    B.eliminate_zeros()
+    return B
+
+
+def cgs_matlab(Aop, b, tol, maxit):
+    """Numpy restatement of MATLAB cgs with x0 = 0 (same recurrences as orc_cgs_l1)."""
+    n2b = np.linalg.norm(b)
+    x = np.zeros_like(b)
+    if n2b == 0:
+        return x
+    tolb = tol * n2b
+    r = b.copy(); normr = np.linalg.norm(r); normr_act = normr
+    if normr <= tolb:
+        return x
+    rt = r.copy(); normrmin = normr; xmin = x.copy(); flag = 1
+    rho = 1.0; stag = 0
+    q = p = u = None
+    for ii in range(1, maxit + 1):
+        rho1 = rho
+        rho = rt @ r
+        if rho == 0 or np.isinf(rho):
+            flag = 4; break
+        if ii == 1:
+            u = r.copy(); p = u.copy()
+        else:
+            beta = rho / rho1
+            if beta == 0 or np.isinf(beta):
+                flag = 4; break
+            u = r + beta * q
+            p = u + beta * (q + beta * p)
+        vh = Aop(p)
+        rtvh = rt @ vh
+        if rtvh == 0:
+            flag = 4; break
+        alpha = rho / rtvh
+        q = u - alpha * vh
+        uh = u + q
+        stag = stag + 1 if abs(alpha) * np.linalg.norm(uh) < np.finfo(float).eps * np.linalg.norm(x) else 0
+        x = x + alpha * uh
+        r = r - alpha * Aop(uh)
+        normr = np.linalg.norm(r); normr_act = normr
+        if normr <= tolb or stag >= 3:
+            r = b - Aop(x); normr_act = np.linalg.norm(r)
+            if normr_act <= tolb:
+                flag = 0; break
+            if stag >= 3:
+                flag = 3; break
+        if normr_act < normrmin:
+            normrmin = normr_act; xmin = x.copy()
+    if flag != 0 and np.linalg.norm(b - Aop(xmin)) <= normr_act:
+        x = xmin
+    return x
+
+
+def neumann_solver(N):
+    """Exact solve of (I + B'B) u = r for the DiffOper B: B'B is the 2-D Neumann Laplacian,
+    diagonalised by the orthonormal DCT-II basis (eigenvalues 4 sin^2(pi k / 2N))."""
+    k = np.arange(N)
+    Cm = np.cos(np.pi * np.outer(np.arange(N) + 0.5, k) / N) * np.sqrt(2.0 / N)
+    Cm[:, 0] = np.sqrt(1.0 / N)
+    lam = 4 * np.sin(np.pi * k / (2 * N)) ** 2
+    den = 1.0 + lam[:, None] + lam[None, :]
+
+    def solve(r):     # r in MATLAB column-major order (z fastest) of an N x N image
+        R = r.reshape(N, N, order="F")
+        return (Cm @ ((Cm.T @ R @ Cm) / den) @ Cm.T).ravel(order="F")
+    return solve
+
+
+def sb_itv2(g, mu, B, exact=False):
+    """SB_ITV2.m:20-40: arg min_u 1/2||u - g||^2 + mu ITV(u) by split Bregman, inner
+    cgs(I + B'B, g - B'(b - d), 1e-5, 100) (or the exact Neumann solve when exact=True).
+    Shrinkage 0/0 at s = 0 follows MATLAB (NaN, confined to structurally empty rows)."""
+    g = np.asarray(g, dtype=np.float64).ravel(order="F")
+    n = g.size
+    N = int(round(np.sqrt(n)))
+    Bt = B.T.tocsr()
+    M = lambda v: v + Bt @ (B @ v)
+    ex = neumann_solver(N) if exact else None
+    b = np.zeros(2 * n); d = b.copy(); u = g.copy(); err = 1.0
+    it = 0
+    while err > 1e-3:
+        up = u
+        rhs = g - Bt @ (b - d)
+        u = ex(rhs) if exact else cgs_matlab(M, rhs, 1e-5, 100)
+        Bub = B @ u + b
+        s = np.sqrt(Bub[:n] ** 2 + Bub[n:] ** 2)
+        with np.errstate(invalid="ignore", divide="ignore"):
+            f = np.fmax(s - mu, 0.0)
+            d = np.concatenate([f * Bub[:n] / s, f * Bub[n:] / s])
+        b = Bub - d
+        err = np.linalg.norm(up - u) / np.linalg.norm(u)
+        it += 1
+    return u, it
+
+
+def itv(x):
+    """ITV.m:5-6 on an [Nz x Nx] image given as (Nx, Nz): MATLAB gradient (central inside)."""
+    X = np.asarray(x, dtype=np.float64).T
+    dy, dx = np.gradient(X)
+    return np.sum(np.sqrt(dx ** 2 + dy ** 2))
+
+
+def sparsa(y, A, AT, tau, psi, phi, x0, maxiter=2, tolA=1e-8):
+    """SpaRSA 2.0 (Wright, Nowak, Figueiredo, IEEE TSP 57(7), 2009; called at
+    nnls_sparsa_based_reg_matrix_less.m:64) as published: step from the separable
+    approximation x+ = psi(x - grad/alpha, tau/alpha), every step accepted (no monotone
+    or safeguard test, the SpaRSA defaults), Barzilai-Borwein alpha = ||A dx||^2/||dx||^2
+    clipped to [1e-30, 1e30], initial alpha = 1, stop criterion 1 (relative change of the
+    objective <= tolA) or maxiter iterations.  PARITY UNPINNED: SpaRSA is not vendored and
+    its exact defaults (MiniterA, initial alpha) cannot be checked here."""
+    x = x0.copy()
+    resid = A(x) - y
+    f = 0.5 * resid @ resid + tau * phi(x)
+    alpha = 1.0
+    for _ in range(maxiter):
+        grad = AT(resid)
+        prev_x, prev_resid, prev_f = x, resid, f
+        x = psi(prev_x - grad / alpha, tau / alpha)
+        dx = x - prev_x
+        Adx = A(dx)
+        resid = prev_resid + Adx
+        f = 0.5 * resid @ resid + tau * phi(x)
+        dd = dx @ dx
+        alpha = min(1e30, max(1e-30, (Adx @ Adx) / (np.finfo(float).tiny + dd)))
+        if abs(f - prev_f) / prev_f <= tolA:
+            break
+    return x
+
+
+def rec_nn_with_TV_reg(model, data, num_iter, regParam, exact_prox=False, inner=2):
+    """rec_nn_with_TV_reg.m:20-53 -> nnls_sparsa_based_reg_matrix_less.m:24-86 (FP64).
+    model: an oracle.Model (square grid); data (W, D, nT).  Returns (rec (W, Nx, Nz), lcurve)."""
+    data = np.asarray(data, dtype=np.float64)
+    W = data.size // (model.D * model.nT)
+    N = model.Nz
+    assert model.Nx == N, "This TV implementation works only with square images!"
+    B = diff_oper(N)
+    P = N * N
+    m = data.size
+    tgt = data.ravel()
+    sl, sb = 1.0, np.sqrt(1e-3)
+    fwd = lambda x: model.forward(x.reshape(W, N, N)).ravel()
+    adj = lambda s: model.transpose(s.reshape(W, model.D, model.nT)).ravel()
+    A = lambda x: np.concatenate([sl * fwd(x), sb * x])
+    AT = lambda u: sl * adj(u[:m]) + sb * u[m:]
+
+    def psi(x, tau):
+        out = np.empty_like(x)
+        for w in range(W):     # image w in memory order (Nx, Nz) == MATLAB [Nz x Nx] column-major
+            out[w * P:(w + 1) * P] = sb_itv2(x[w * P:(w + 1) * P], tau, B, exact=exact_prox)[0]
+        return out
+
+    phi = lambda x: sum(itv(x[w * P:(w + 1) * P].reshape(N, N)) for w in range(W))
+    w_ = np.zeros(P * W); v_ = np.zeros(P * W)
+    for _ in range(num_iter):
+        y = np.concatenate([sl * tgt, sb * (w_ + v_)])
+        u = sparsa(y, A, AT, regParam, psi, phi, w_, maxiter=inner, tolA=1e-8)
+        w_ = np.maximum(u - v_, 0)
+        v_ = v_ + w_ - u
+    rec = w_.reshape(W, N, N)
+    res = fwd(w_) - tgt
+    return rec, (res @ res, phi(w_))
