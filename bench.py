@@ -113,13 +113,15 @@ def main():
         sir_bytes = 4 * int(np.sum(np.maximum(se[ref, 1] - se[ref, 0] + 1, 0)))
     alg_bytes = 4 * P * D * (2 if use_sir else 1) + sir_bytes + 4 * B * P + 4 * B * D * nT
     reps = 10
-    t_gather = model.plan.time_stage("gather", B, reps)
-    t_scatter = model.plan.time_stage("scatter", B, reps)
-    t_s2b = model.plan.time_stage("sino_to_bins", B, reps)
-    t_b2s = model.plan.time_stage("bins_to_sino", B, reps)
-    # per 50-iteration solve: 151 forward and 150 transpose applications
-    dom = "gather" if t_gather * 150 >= t_scatter * 151 else "scatter"
-    t_dom = t_gather if dom == "gather" else t_scatter
+    st = {k: model.plan.time_stage(k, B, reps) for k in ("scatter", "bins_to_y", "fir", "z_to_bins", "gather")}
+    # launches per 50-iteration solve: 151 forward (scatter, bins_to_y, fir) and 150 transpose
+    # (fir, z_to_bins, gather) applications
+    n_forward, n_transpose = 3 * a.iters + 1, 3 * a.iters
+    per_solve = {"scatter": n_forward * st["scatter"], "bins_to_y": n_forward * st["bins_to_y"],
+                 "fir": (n_forward + n_transpose) * st["fir"], "z_to_bins": n_transpose * st["z_to_bins"],
+                 "gather": n_transpose * st["gather"]}
+    dom = "gather" if per_solve["gather"] >= per_solve["scatter"] else "scatter"
+    t_dom = st[dom]
     achieved = alg_bytes / (t_dom * 1e-3) / 1e9
     traffic = None
     tfile = os.path.join(ROOT, "profiles", "traffic.json")
@@ -130,14 +132,13 @@ def main():
             traffic = tr.get(key)
         except Exception:
             traffic = None
+    kname = {"gather": "k_gather", "scatter": "k_scatter_bk"}[dom]
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                "kernel": f"k_{dom}<{'SIR' if use_sir else 'DAS'}>", "algorithmic_bytes": int(alg_bytes),
-                "kernel_ms": round(t_dom, 4),
-                "stage_ms": {"scatter": round(t_scatter, 4), "bins_to_sino": round(t_b2s, 4),
-                             "sino_to_bins": round(t_s2b, 4), "gather": round(t_gather, 4)}}
-    log(f"stages (ms, B={B}): scatter {t_scatter:.3f} bins_to_sino {t_b2s:.3f} "
-        f"sino_to_bins {t_s2b:.3f} gather {t_gather:.3f}; {dom} {achieved:.0f} GB/s")
+                "kernel": f"{kname}<{'SIR' if use_sir else 'DAS'}>", "algorithmic_bytes": int(alg_bytes),
+                "kernel_ms": round(t_dom, 4), "stage_ms": {k: round(v, 4) for k, v in st.items()},
+                "ms_per_solve_by_stage": {k: round(v, 2) for k, v in per_solve.items()}}
+    log("stages (ms, B=%d): %s; %s %.0f GB/s" % (B, " ".join(f"{k} {v:.3f}" for k, v in st.items()), dom, achieved))
 
     # ---- CPU baseline: FP64 oracle on a bounded sample --------------------------
     cpu = None

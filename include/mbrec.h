@@ -113,9 +113,10 @@ size_t mbrec_plan_device_bytes(mbrec_plan plan);
 int mbrec_plan_tau_frac_bits(mbrec_plan plan);
 
 /* Profiling: average duration (ms, HIP events on stream s) of one operator
- * stage over reps launches for a batch of B: 0 = forward pair scatter,
- * 1 = bins -> samples (+IR), 2 = samples -> bins (+IR), 3 = transpose pair
- * gather.  Inputs are zero buffers; timing does not depend on their values. */
+ * stage over reps launches for a batch of B: 0 = forward pair scatter into
+ * bucketed bin windows, 1 = bins -> samples (window sum + hat filter),
+ * 2 = IR convolution (FIR), 3 = samples -> bins (hat filter), 4 = transpose
+ * pair gather.  Inputs are zero buffers; timing does not depend on their values. */
 int mbrec_plan_time_stage(mbrec_plan plan, int stage, int B, int reps, mbrec_stream s, float *ms);
 
 /* Bare MEX loop (no prefilter, IR or 1/sigma), batch of B planar images

@@ -50,32 +50,123 @@ static int upload(T *dst, const T *src, size_t n)
     return MBREC_OK;
 }
 
-int ensure_bins(Plan *p, int B)
+static void free_work(Plan *p)
 {
-    if (B <= p->bins_bcap) return MBREC_OK;
-    if (p->d_bins) {
+    float *bufs[] = {p->d_bins, p->d_hpart, p->d_imgI, p->d_img_tmp, p->d_sino_tmp};
+    for (float *q : bufs)
+        if (q) (void)hipFree(q);
+    p->d_bins = p->d_hpart = p->d_imgI = p->d_img_tmp = p->d_sino_tmp = nullptr;
+    p->bcap = 0;
+    p->work_bytes = 0;
+}
+
+
+int ensure_work(Plan *p, int B)
+{
+    const size_t need_h = (size_t)p->D * p->nbk * p->WL;
+    if (B <= p->bcap && p->d_hpart && p->work_bytes && need_h * batch_stride(p->bcap) * sizeof(float) <=
+                                                             p->work_bytes)
+        return MBREC_OK;
+    if (p->bcap) {
         MBREC_HIP(hipDeviceSynchronize());
-        MBREC_HIP(hipFree(p->d_bins));
-        p->d_bins = nullptr;
+        free_work(p);
     }
-    const size_t n = (size_t)p->S * B * p->D * p->geom.NB;
-    int rc = dalloc(&p->d_bins, n, nullptr);
-    if (rc) { p->bins_bcap = 0; return rc; }
-    p->bins_bcap = B;
+    const int Bp = batch_stride(B);
+    size_t acc = 0;
+    int rc;
+    if ((rc = dalloc(&p->d_bins, (size_t)p->D * p->geom.NB * Bp, &acc)) ||
+        (rc = dalloc(&p->d_hpart, need_h * Bp, nullptr)) ||
+        (rc = dalloc(&p->d_imgI, (size_t)p->P * Bp, &acc)) ||
+        (rc = dalloc(&p->d_img_tmp, (size_t)2 * B * p->P, &acc)) ||
+        (rc = dalloc(&p->d_sino_tmp, (size_t)B * p->D * p->nT, &acc))) {
+        free_work(p);
+        return rc;
+    }
+    p->bcap = B;
+    p->work_bytes = need_h * Bp * sizeof(float);   // hpart size (reallocated when the window grows)
     return MBREC_OK;
 }
 
-int ensure_tmp(Plan *p, int B)
+void host_decode(int32_t q, const BinGeom &g, int &nn, int &m)
 {
-    if (B <= p->tmp_bcap) return MBREC_OK;
-    if (p->d_img_tmp) {
-        MBREC_HIP(hipDeviceSynchronize());
-        MBREC_HIP(hipFree(p->d_img_tmp));
-        p->d_img_tmp = nullptr;
+    // identical arithmetic to decode_tau on the device (exact float fraction, float compares)
+    const int base = q >> g.F;
+    const float fr = (float)(q & ((1 << g.F) - 1)) * std::ldexp(1.0f, -g.F);
+    m = (fr >= g.phi1) + (fr >= g.phi2);
+    nn = base + g.K;
+}
+
+// Bucket the (pixel, travel time) pairs of every detector by bin j = 3 nn + m
+// (bucket floor(j / Wb), clamped to the first / last bucket), keeping the device
+// tile order inside a bucket.  Without SIR a pair contributes iff its sample
+// nn lies in [0, nT + 2K); with SIR (sidx, win given) iff one of its taps
+// nn + s - Ls/2, s in its window, does, and its SIR index is stored with it.
+// The window margins lo_off / hi_off must already cover the tap displacements.
+int build_records(Plan *p, const std::vector<int32_t> &slot_pix, const std::vector<int32_t> &tq,
+                  const int32_t *sidx, const int32_t *win, int Ls)
+{
+    const BinGeom &g = p->geom;
+    const int span = g.nT + 2 * g.K;
+    const int half = Ls / 2;
+    const int nbk = (g.NB + p->Wb - 1) / p->Wb;
+    std::vector<int32_t> off((size_t)p->D * (nbk + 1), 0);
+    std::vector<int32_t> kbuf((size_t)p->D * p->Pd, -1);
+    for (int d = 0; d < p->D; ++d)
+        for (int sl = 0; sl < p->Pd; ++sl) {
+            if (slot_pix[sl] < 0) continue;
+            const size_t at = (size_t)d * p->Pd + sl;
+            int nn, m;
+            host_decode(tq[at], g, nn, m);
+            if (sidx) {
+                const int k = sidx[at];
+                const long long s0 = win[2 * k], len = win[2 * k + 1];
+                if (len <= 0 || (long long)nn + s0 - half >= span || (long long)nn + s0 + len - 1 - half < 0)
+                    continue;
+            } else if ((unsigned)nn >= (unsigned)span) {
+                continue;
+            }
+            const long long j = 3LL * nn + m;
+            const long long kb = j < 0 ? 0 : std::min<long long>(nbk - 1, j / p->Wb);
+            kbuf[at] = (int32_t)kb;
+            off[(size_t)d * (nbk + 1) + kb + 1]++;
+        }
+    size_t total = 0;
+    for (int d = 0; d < p->D; ++d) {
+        int32_t *o = &off[(size_t)d * (nbk + 1)];
+        if (total + (size_t)p->Pd > (size_t)INT32_MAX)
+            return set_error(MBREC_ERR_ARG, "mbrec:size", "too many (pixel, detector) pairs for int32 offsets");
+        o[0] = (int32_t)total;
+        for (int k = 0; k < nbk; ++k) o[k + 1] += o[k];
+        total = o[nbk];
     }
-    int rc = dalloc(&p->d_img_tmp, (size_t)2 * B * p->P, nullptr);
-    if (rc) { p->tmp_bcap = 0; return rc; }
-    p->tmp_bcap = B;
+    std::vector<int2> rec(std::max<size_t>(total, 1));
+    std::vector<int32_t> rk(sidx ? std::max<size_t>(total, 1) : 0);
+    std::vector<int32_t> fill(off);
+    for (int d = 0; d < p->D; ++d)
+        for (int sl = 0; sl < p->Pd; ++sl) {
+            const size_t at = (size_t)d * p->Pd + sl;
+            if (kbuf[at] < 0) continue;
+            const int32_t r = fill[(size_t)d * (nbk + 1) + kbuf[at]]++;
+            rec[r] = make_int2(slot_pix[sl], tq[at]);
+            if (sidx) rk[r] = sidx[at];
+        }
+    // replace the previous tables
+    void *old[] = {p->d_rec, p->d_rec_off, p->d_rec_k};
+    for (void *q : old)
+        if (q) (void)hipFree(q);
+    p->table_bytes -= p->rec_bytes;
+    p->rec_bytes = 0;
+    p->d_rec = nullptr;
+    p->d_rec_off = p->d_rec_k = nullptr;
+    p->nbk = nbk;
+    p->nrec = total;
+    int rc;
+    if ((rc = dalloc(&p->d_rec, rec.size(), &p->rec_bytes)) || (rc = dalloc(&p->d_rec_off, off.size(), &p->rec_bytes)) ||
+        (rc = upload(p->d_rec, rec.data(), rec.size())) || (rc = upload(p->d_rec_off, off.data(), off.size())))
+        return rc;
+    if (sidx && ((rc = dalloc(&p->d_rec_k, rk.size(), &p->rec_bytes)) || (rc = upload(p->d_rec_k, rk.data(), rk.size()))))
+        return rc;
+    p->table_bytes += p->rec_bytes;
     return MBREC_OK;
 }
 
@@ -146,10 +237,11 @@ static void make_pixel_order(Plan *p, std::vector<int32_t> &slot_pix)
 
 static void free_plan(Plan *p)
 {
-    void *ptrs[] = {p->d_pix, p->d_tq, p->d_sidx, p->d_sir, p->d_swin, p->d_ir,
-                    p->d_kz,  p->d_kx, p->d_bins, p->d_img_tmp, p->d_sino_tmp};
+    void *ptrs[] = {p->d_pix, p->d_tq, p->d_sidx, p->d_sir, p->d_swin, p->d_ir, p->d_irf,
+                    p->d_kz,  p->d_kx, p->d_rec, p->d_rec_off, p->d_rec_k};
     for (void *q : ptrs)
         if (q) (void)hipFree(q);
+    free_work(p);
 }
 
 }  // namespace mbrec
@@ -212,7 +304,8 @@ int mbrec_plan_create(mbrec_plan *out, int Nz, int Nx, int D, int nT, double R,
     if ((rc = dalloc(&p->d_pix, p->Pd, &p->table_bytes)) ||
         (rc = dalloc(&p->d_tq, (size_t)D * p->Pd, &p->table_bytes)) ||
         (rc = upload(p->d_pix, slot_pix.data(), slot_pix.size())) ||
-        (rc = upload(p->d_tq, tq.data(), tq.size()))) {
+        (rc = upload(p->d_tq, tq.data(), tq.size())) ||
+        (rc = build_records(p, slot_pix, tq, nullptr, nullptr, 0))) {
         free_plan(p);
         delete h;
         return rc;
@@ -264,6 +357,22 @@ int mbrec_plan_set_sir(mbrec_plan h, const int32_t *sirIdx, const double *sir, i
         (rc = upload(p->d_sir, sirf.data(), sirf.size())) ||
         (rc = upload(p->d_swin, win.data(), win.size())))
         return rc;
+    // deposit margins: a record at bin j reaches j + 3 (s - Ls/2) + {0, 1} for s in its window
+    int smin = INT32_MAX, smax = INT32_MIN;
+    for (int k = 0; k < Nsir; ++k)
+        if (win[2 * (size_t)k + 1] > 0) {
+            smin = std::min(smin, (int)win[2 * (size_t)k]);
+            smax = std::max(smax, (int)(win[2 * (size_t)k] + win[2 * (size_t)k + 1] - 1));
+        }
+    if (smin > smax) smin = smax = Ls / 2;
+    p->lo_off = std::min(0, 3 * (smin - Ls / 2));
+    p->hi_off = std::max(1, 3 * (smax - Ls / 2) + 1);
+    p->WL = p->Wb + p->hi_off - p->lo_off;
+    {
+        std::vector<int32_t> tq((size_t)p->D * p->Pd);
+        MBREC_HIP(hipMemcpy(tq.data(), p->d_tq, sizeof(int32_t) * tq.size(), hipMemcpyDeviceToHost));
+        if ((rc = build_records(p, slot_pix, tq, sidx.data(), win.data(), Ls))) return rc;
+    }
     p->use_sir = 1;
     p->Ls = Ls;
     p->Nsir = Nsir;
@@ -280,9 +389,15 @@ int mbrec_plan_set_ir(mbrec_plan h, const double *ir, int Lir, int per_det)
     std::vector<float> f(n);
     // host layout [Lir x D] column-major = per detector contiguous
     for (size_t i = 0; i < n; ++i) f[i] = (float)ir[i];
-    if (p->d_ir) (void)hipFree(p->d_ir);
+    std::vector<float> ff(n);
+    const int rows = per_det ? p->D : 1;
+    for (int r = 0; r < rows; ++r)
+        for (int i = 0; i < Lir; ++i) ff[(size_t)r * Lir + i] = f[(size_t)r * Lir + (Lir - 1 - i)];
+    if (p->d_ir) { (void)hipFree(p->d_ir); (void)hipFree(p->d_irf); }
     int rc;
-    if ((rc = dalloc(&p->d_ir, n, &p->table_bytes)) || (rc = upload(p->d_ir, f.data(), n))) return rc;
+    if ((rc = dalloc(&p->d_ir, n, &p->table_bytes)) || (rc = upload(p->d_ir, f.data(), n)) ||
+        (rc = dalloc(&p->d_irf, n, &p->table_bytes)) || (rc = upload(p->d_irf, ff.data(), n)))
+        return rc;
     p->Lir = Lir;
     p->ir_per_det = per_det ? 1 : 0;
     return MBREC_OK;
@@ -332,8 +447,11 @@ size_t mbrec_plan_device_bytes(mbrec_plan h)
 {
     if (!h) return 0;
     const Plan *p = &h->p;
-    return p->table_bytes + (size_t)p->S * p->bins_bcap * p->D * p->geom.NB * sizeof(float) +
-           (size_t)2 * p->tmp_bcap * p->P * sizeof(float);
+    if (!p->bcap) return p->table_bytes;
+    const size_t Bp = batch_stride(p->bcap);
+    return p->table_bytes + p->work_bytes +
+           sizeof(float) * ((size_t)p->D * p->geom.NB * Bp + (size_t)p->P * Bp + (size_t)2 * p->bcap * p->P +
+                            (size_t)p->bcap * p->D * p->nT);
 }
 
 int mbrec_plan_tau_frac_bits(mbrec_plan h) { return h ? h->p.geom.F : -1; }

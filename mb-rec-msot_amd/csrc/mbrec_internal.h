@@ -19,6 +19,7 @@
 #include <stdint.h>
 #include <stddef.h>
 #include <string>
+#include <vector>
 #include "../../include/mbrec.h"
 
 #define MBREC_VERSION 1
@@ -49,31 +50,44 @@ struct Plan {
     int TZ = 8, TX = 8;          // device pixel tile
     int Pd = 0;                  // device pixel slots (tile-padded)
     int32_t *d_pix = nullptr;    // [Pd] slot -> pixel index or -1
-    int32_t *d_tq = nullptr;     // [D][Pd] fixed-point travel times
+    int32_t *d_tq = nullptr;     // [D][Pd] fixed-point travel times (transpose gather)
     BinGeom geom{};
     HatTaps taps{};
+    // forward scatter: per detector, (pixel, travel time) records grouped into buckets
+    // of Wb consecutive bins; bucket k of detector d deposits into bins
+    // [k*Wb + lo_off, k*Wb + lo_off + WL)
+    int Wb = 384, nbk = 0, lo_off = 0, hi_off = 1, WL = 0;
+    int2 *d_rec = nullptr;       // [records] (pixel, fixed-point travel time)
+    int32_t *d_rec_off = nullptr;   // [D][nbk + 1] record offsets
+    int32_t *d_rec_k = nullptr;  // [records] 0-based SIR index (SIR plans)
+    size_t nrec = 0;
+    size_t rec_bytes = 0;              // device bytes of d_rec / d_rec_off / d_rec_k
     // SIR
     int use_sir = 0, Ls = 0, Nsir = 0;
     int32_t *d_sidx = nullptr;   // [D][Pd] 0-based SIR index
     float *d_sir = nullptr;      // [Nsir][Ls]
     int32_t *d_swin = nullptr;   // [Nsir][2] 0-based first tap, tap count
     int sir_maxlen = 0;
-    // impulse response
+    // impulse response (taps of the convolution and of the correlation = flipped IR)
     float *d_ir = nullptr;       // [Lir] or [D][Lir]
+    float *d_irf = nullptr;      // flipped
     int Lir = 0, ir_per_det = 0;
     // prefilter
     float *d_kz = nullptr, *d_kx = nullptr;
     int Kz = -1, Kx = -1;        // -1 = no prefilter
     double sigma = 1.0;
-    // workspace
-    int S = 4;                   // pixel splits of the forward scatter
-    float *d_bins = nullptr;     // [S][Bcap][D][NB]
-    int bins_bcap = 0;
-    float *d_img_tmp = nullptr;  // [Bcap][P] x2
-    float *d_sino_tmp = nullptr; // [Bcap][D][nT]
-    int tmp_bcap = 0;
+    // workspace, sized for bcap sinograms
+    int bcap = 0;
+    float *d_bins = nullptr;     // [D][NB][Bp]      transpose bins
+    float *d_hpart = nullptr;    // [D][nbk][WL][Bp] forward bucket windows
+    float *d_imgI = nullptr;     // [Pd..P][Bp]      interleaved image (scatter input)
+    float *d_img_tmp = nullptr;  // 2 x [B][P]       prefilter temporaries
+    float *d_sino_tmp = nullptr; // [B][D][nT]       samples before/after the IR filter
     size_t table_bytes = 0;
+    size_t work_bytes = 0;
 };
+
+static inline int batch_stride(int B) { return B <= 1 ? 1 : (B + 3) / 4 * 4; }
 
 // error state -------------------------------------------------------------
 int set_error(int code, const char *id, const char *fmt, ...);
@@ -85,16 +99,10 @@ int hip_check(hipError_t e, const char *what);
     } while (0)
 
 // plan workspace ----------------------------------------------------------
-int ensure_bins(Plan *p, int B);
-int ensure_tmp(Plan *p, int B);
-
-// operator launchers (device buffers, planar layouts) ----------------------
-int launch_scatter(Plan *p, const float *img, int B, hipStream_t s);           // img -> bins partials
-int launch_bins_to_sino(Plan *p, float *sino, int B, bool model, hipStream_t s);
-int launch_sino_to_bins(Plan *p, const float *sino, int B, bool model, hipStream_t s);
-int launch_gather(Plan *p, float *img, int B, float scale, hipStream_t s);      // bins -> img
-int launch_prefilter(Plan *p, const float *in, float *out, float *tmp, int B, bool transp,
-                     hipStream_t s);
+int ensure_work(Plan *p, int B);
+int build_records(Plan *p, const std::vector<int32_t> &slot_pix, const std::vector<int32_t> &tq,
+                  const int32_t *sidx, const int32_t *win, int Ls);
+void host_decode(int32_t q, const BinGeom &g, int &nn, int &m);
 
 int time_stage(Plan *p, int which, int B, int reps, hipStream_t s, float *ms);
 

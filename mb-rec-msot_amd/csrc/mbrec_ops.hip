@@ -1,15 +1,23 @@
 // mbrec_ops.hip -- gfx950 kernels of the forward / transpose model operators.
 //
-// Pipeline (planar device layouts: image [B][P], sinogram [B][D][nT]):
-//   forward   img --prefilter--> coeffs --k_scatter--> bin partials [S][B][D][NB]
-//                 --k_bins_to_sino (polyphase hat filter, IR convolution, 1/sigma)--> sino
-//   transpose sino --k_sino_to_bins (IR correlation, polyphase hat filter)--> bins [B][D][NB]
-//                 --k_gather (1 interpolation per pair, Kahan sum, 1/sigma)--> coeffs
-//                 --prefilter^T--> img
+// Pipeline (device layouts: images planar [B][P], sinograms planar [B][D][nT];
+// internal bins batch-interleaved [D][NB][Bp], Bp = B rounded up to 4):
+//
+//   forward    img --prefilter (2 passes, 2nd writes [P][Bp])--> coeffs
+//                  --k_scatter_bk (bucketed, LDS window atomics)--> windows [D][nbk][WL][Bp]
+//                  --k_bins_to_y (window sums + polyphase hat filter)--> y [B][D][nT]
+//                  --k_fir (IR convolution, 1/sigma)--> sino
+//   transpose  sino --k_fir (IR correlation)--> z --k_z_to_bins (polyphase hat filter)--> bins
+//                  --k_gather (one interpolation per pair, Kahan, 1/sigma)--> coeffs
+//                  --prefilter^T--> img
+//
 // The bin formulation is explained in mbrec_internal.h; it is exact, so the
-// composite equals MSOTmodel.m:189-257 (A = crop(IR * MEX(F p)) / sigma) up
-// to FP32 rounding.
+// composite equals MSOTmodel.m:189-257 (A = crop(IR * MEX(F p)) / sigma) up to
+// FP32 rounding.  Sums that cancel (the band-pass IR filter, the prefilter)
+// accumulate in FP64; the transpose pair sum uses Kahan compensation.
 #include "mbrec_internal.h"
+
+#include <algorithm>
 
 namespace mbrec {
 
@@ -45,271 +53,24 @@ __device__ __forceinline__ void kahan_add(float &s, float &c, float v)
 }
 
 // ---------------------------------------------------------------------------
-// forward scatter: one workgroup per (detector, pixel split, batch chunk);
-// the detector's bin row lives in LDS and receives LDS float atomics, then is
-// written once (no global atomics).  Partials of the S splits are summed in a
-// fixed order by k_bins_to_sino.
+// planar [B][P] -> interleaved [P][Bp] (padding columns zeroed)
 // ---------------------------------------------------------------------------
-template <int BC, bool SIR>
-__global__ __launch_bounds__(256) void k_scatter(
-    const int32_t *__restrict__ tq, const int32_t *__restrict__ pix,
-    const float *__restrict__ img, float *__restrict__ hpart,
-    const int32_t *__restrict__ sidx, const float *__restrict__ sir,
-    const int32_t *__restrict__ swin, int Ls, BinGeom g, int Pd, int P, int D, int B,
-    int S, int chunk)
+__global__ void k_to_interleaved(const float *__restrict__ in, float *__restrict__ out, int P, int B, int Bp)
 {
-    extern __shared__ float h[];
-    const int d = blockIdx.x, sp = blockIdx.y, b0 = blockIdx.z * BC;
-    const int nb = min(BC, B - b0);
-    const int NB = g.NB;
-    for (int i = threadIdx.x; i < BC * NB; i += blockDim.x) h[i] = 0.0f;
-    __syncthreads();
-    const int lo = sp * chunk, hi = min(Pd, lo + chunk);
-    const int span = g.nT + 2 * g.K;
-    const int32_t *tqd = tq + (size_t)d * Pd;
-    for (int slot = lo + (int)threadIdx.x; slot < hi; slot += blockDim.x) {
-        const int px = pix[slot];
-        if (px < 0) continue;
-        const Decoded dc = decode_tau(tqd[slot], g);
-        float c[BC];
-#pragma unroll
-        for (int b = 0; b < BC; ++b) c[b] = (b < nb) ? img[(size_t)(b0 + b) * P + px] : 0.0f;
-        if (!SIR) {
-            if ((unsigned)dc.nn >= (unsigned)span) continue;
-            const int j = 3 * dc.nn + dc.m;
-#pragma unroll
-            for (int b = 0; b < BC; ++b) {
-                atomicAdd(&h[b * NB + j], c[b] * (1.0f - dc.u));
-                atomicAdd(&h[b * NB + j + 1], c[b] * dc.u);
-            }
-        } else {
-            const int k = sidx[(size_t)d * Pd + slot];
-            const int s0 = swin[2 * k], len = swin[2 * k + 1];
-            const float *hk = sir + (size_t)k * Ls;
-            const int shift = dc.nn - Ls / 2;
-            for (int s = s0; s < s0 + len; ++s) {
-                const int nn = shift + s;
-                if ((unsigned)nn >= (unsigned)span) continue;
-                const float wv = hk[s];
-                const int j = 3 * nn + dc.m;
-#pragma unroll
-                for (int b = 0; b < BC; ++b) {
-                    const float v = c[b] * wv;
-                    atomicAdd(&h[b * NB + j], v * (1.0f - dc.u));
-                    atomicAdd(&h[b * NB + j + 1], v * dc.u);
-                }
-            }
-        }
-    }
-    __syncthreads();
-    for (int b = 0; b < nb; ++b) {
-        float *dst = hpart + (((size_t)sp * B + b0 + b) * D + d) * NB;
-        for (int j = threadIdx.x; j < NB; j += blockDim.x) dst[j] = h[b * NB + j];
-    }
-}
-
-// ---------------------------------------------------------------------------
-// bins -> samples (M^T), then optional IR convolution (MSOTmodel.m:259-263)
-// and scale.  One workgroup per (time tile, detector, batch item).
-// ---------------------------------------------------------------------------
-constexpr int kTimeTile = 512;
-
-__global__ __launch_bounds__(256) void k_bins_to_sino(
-    const float *__restrict__ hpart, int S, float *__restrict__ sino, HatTaps taps, BinGeom g,
-    const float *__restrict__ ir, int Lir, int ir_per_det, int D, int B, float scale)
-{
-    extern __shared__ float sm[];
-    const int d = blockIdx.y, b = blockIdx.z;
-    const int nT = g.nT, NB = g.NB;
-    const int t0 = blockIdx.x * kTimeTile, t1 = min(nT, t0 + kTimeTile);
-    const int half = Lir / 2;
-    int rlo, rhi;
-    if (ir) {
-        rlo = max(0, t0 + half - (Lir - 1));
-        rhi = min(nT - 1, t1 - 1 + half);
-    } else {
-        rlo = t0;
-        rhi = t1 - 1;
-    }
-    const int kmax = taps.kmin + taps.nw - 1;
-    const int jlo = 3 * (rlo - kmax + g.K);
-    const int jhi = 3 * (rhi - taps.kmin + g.K) + 2;
-    const int nj = jhi - jlo + 1;
-    float *hs = sm;
-    float *ys = sm + nj;
-    float *irs = ys + (rhi - rlo + 1);
-    for (int j = threadIdx.x; j < nj; j += blockDim.x) {
-        float acc = 0.0f;
-        for (int sp = 0; sp < S; ++sp)
-            acc += hpart[(((size_t)sp * B + b) * D + d) * NB + jlo + j];
-        hs[j] = acc;
-    }
-    if (ir) {
-        const float *irow = ir + (ir_per_det ? (size_t)d * Lir : 0);
-        for (int i = threadIdx.x; i < Lir; i += blockDim.x) irs[i] = irow[i];
-    }
-    __syncthreads();
-    for (int r = rlo + (int)threadIdx.x; r <= rhi; r += blockDim.x) {
-        float acc = 0.0f;
-        for (int m = 0; m < 3; ++m) {
-            const int jb = 3 * (r - taps.kmin + g.K) + m - jlo;
-            for (int i = 0; i < taps.nw; ++i) acc = fmaf(taps.w[m][i], hs[jb - 3 * i], acc);
-        }
-        ys[r - rlo] = acc;
-    }
-    __syncthreads();
-    float *out = sino + ((size_t)b * D + d) * nT;
-    for (int t = t0 + (int)threadIdx.x; t < t1; t += blockDim.x) {
-        float acc;
-        if (ir) {
-            // y[t] = sum_r ys[r] IR[t + half - r],  r in [t+half-Lir+1, t+half] cap [rlo, rhi]
-            const int ra = max(rlo, t + half - (Lir - 1));
-            const int rb = min(rhi, t + half);
-            acc = 0.0f;
-            for (int r = ra; r <= rb; ++r) acc = fmaf(ys[r - rlo], irs[t + half - r], acc);
-        } else {
-            acc = ys[t - rlo];
-        }
-        out[t] = scale * acc;
-    }
-}
-
-// ---------------------------------------------------------------------------
-// samples -> bins (M), with optional IR correlation in front
-// (MSOTmodel.m:265-269).  One workgroup per (bin tile, detector, batch item).
-// ---------------------------------------------------------------------------
-constexpr int kBinTile = 1536;   // 512 samples x 3 phases
-
-__global__ __launch_bounds__(256) void k_sino_to_bins(
-    const float *__restrict__ sino, float *__restrict__ bins, HatTaps taps, BinGeom g,
-    const float *__restrict__ ir, int Lir, int ir_per_det, int D, int B)
-{
-    extern __shared__ float sm[];
-    const int d = blockIdx.y, b = blockIdx.z;
-    const int nT = g.nT, NB = g.NB;
-    const int j0 = blockIdx.x * kBinTile, j1 = min(NB, j0 + kBinTile);
-    const int nlo = j0 / 3 - g.K, nhi = (j1 - 1) / 3 - g.K;
-    const int kmax = taps.kmin + taps.nw - 1;
-    const int zlo = max(0, nlo + taps.kmin), zhi = min(nT - 1, nhi + kmax);
-    const float *srow = sino + ((size_t)b * D + d) * nT;
-    float *zs = sm;
-    const int nz = max(0, zhi - zlo + 1);
-    if (ir) {
-        // z[t] = sum_m s[m] IR[off - t + m], m in [t-off, t-off+Lir-1] cap [0, nT-1]
-        const int off = Lir - 1 - Lir / 2;
-        const int slo = max(0, zlo - off), shi = min(nT - 1, zhi - off + Lir - 1);
-        float *ss = zs + nz;
-        float *irs = ss + max(0, shi - slo + 1);
-        const float *irow = ir + (ir_per_det ? (size_t)d * Lir : 0);
-        for (int i = threadIdx.x; i <= shi - slo; i += blockDim.x) ss[i] = srow[slo + i];
-        for (int i = threadIdx.x; i < Lir; i += blockDim.x) irs[i] = irow[i];
-        __syncthreads();
-        for (int t = zlo + (int)threadIdx.x; t <= zhi; t += blockDim.x) {
-            const int ma = max(slo, t - off), mb = min(shi, t - off + Lir - 1);
-            float acc = 0.0f;
-            for (int m = ma; m <= mb; ++m) acc = fmaf(ss[m - slo], irs[off - t + m], acc);
-            zs[t - zlo] = acc;
-        }
-    } else {
-        for (int i = threadIdx.x; i < nz; i += blockDim.x) zs[i] = srow[zlo + i];
-    }
-    __syncthreads();
-    float *brow = bins + ((size_t)b * D + d) * NB;
-    for (int j = j0 + (int)threadIdx.x; j < j1; j += blockDim.x) {
-        const int n = j / 3 - g.K, m = j - 3 * (j / 3);
-        float acc = 0.0f;
-        for (int i = 0; i < taps.nw; ++i) {
-            const int r = n + taps.kmin + i;
-            if (r >= zlo && r <= zhi) acc = fmaf(taps.w[m][i], zs[r - zlo], acc);
-        }
-        brow[j] = acc;
-    }
-}
-
-// ---------------------------------------------------------------------------
-// transpose gather: one workgroup per 64-slot pixel tile; its 4 waves split
-// the detectors and are reduced through LDS.  FP32 with Kahan compensation.
-// ---------------------------------------------------------------------------
-template <int BC, bool SIR>
-__global__ __launch_bounds__(256) void k_gather(
-    const int32_t *__restrict__ tq, const int32_t *__restrict__ pix,
-    const float *__restrict__ bins, float *__restrict__ out,
-    const int32_t *__restrict__ sidx, const float *__restrict__ sir,
-    const int32_t *__restrict__ swin, int Ls, BinGeom g, int Pd, int P, int D, int B, float scale)
-{
-    __shared__ float red[4][BC][64];
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    const int slot = blockIdx.x * 64 + lane;
-    const int b0 = blockIdx.y * BC;
-    const int nb = min(BC, B - b0);
-    const int span = g.nT + 2 * g.K;
-    const size_t bstride = (size_t)D * g.NB;
-    float acc[BC], cmp[BC];
-#pragma unroll
-    for (int b = 0; b < BC; ++b) { acc[b] = 0.0f; cmp[b] = 0.0f; }
-    for (int d = wv; d < D; d += 4) {
-        const Decoded dc = decode_tau(tq[(size_t)d * Pd + slot], g);
-        const float *gb = bins + (size_t)b0 * bstride + (size_t)d * g.NB;
-        if (!SIR) {
-            if ((unsigned)dc.nn < (unsigned)span) {
-                const int j = 3 * dc.nn + dc.m;
-#pragma unroll
-                for (int b = 0; b < BC; ++b) {
-                    if (b < nb) {
-                        const float g0 = gb[b * bstride + j], g1 = gb[b * bstride + j + 1];
-                        kahan_add(acc[b], cmp[b], fmaf(dc.u, g1 - g0, g0));
-                    }
-                }
-            }
-        } else {
-            const int k = sidx[(size_t)d * Pd + slot];
-            const int s0 = swin[2 * k], len = swin[2 * k + 1];
-            const float *hk = sir + (size_t)k * Ls;
-            const int shift = dc.nn - Ls / 2;
-            float part[BC];
-#pragma unroll
-            for (int b = 0; b < BC; ++b) part[b] = 0.0f;
-            for (int s = s0; s < s0 + len; ++s) {
-                const int nn = shift + s;
-                if ((unsigned)nn >= (unsigned)span) continue;
-                const float wvl = hk[s];
-                const int j = 3 * nn + dc.m;
-#pragma unroll
-                for (int b = 0; b < BC; ++b) {
-                    if (b < nb) {
-                        const float g0 = gb[b * bstride + j], g1 = gb[b * bstride + j + 1];
-                        part[b] = fmaf(wvl, fmaf(dc.u, g1 - g0, g0), part[b]);
-                    }
-                }
-            }
-#pragma unroll
-            for (int b = 0; b < BC; ++b) kahan_add(acc[b], cmp[b], part[b]);
-        }
-    }
-#pragma unroll
-    for (int b = 0; b < BC; ++b) red[wv][b][lane] = acc[b] - cmp[b];
-    __syncthreads();
-    if (wv == 0) {
-        const int px = pix[slot];
-        if (px >= 0) {
-#pragma unroll
-            for (int b = 0; b < BC; ++b) {
-                if (b < nb) {
-                    const float v = (red[0][b][lane] + red[1][b][lane]) + (red[2][b][lane] + red[3][b][lane]);
-                    out[(size_t)(b0 + b) * P + px] = scale * v;
-                }
-            }
-        }
-    }
+    const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (size_t)P * Bp) return;
+    const int px = (int)(idx / Bp), b = (int)(idx - (size_t)px * Bp);
+    out[idx] = b < B ? in[(size_t)b * P + px] : 0.0f;
 }
 
 // ---------------------------------------------------------------------------
 // separable circular prefilter (Discretization.m:54-72): one pass per axis.
-// forward: out[i] = sum_a k[a] in[(i - a) mod N];  transpose: in[(i + a) mod N]
+// forward: out[i] = sum_a k[a] in[(i - a) mod N];  transpose: in[(i + a) mod N].
+// interleaved_out: write [P][Bp] instead of [B][P].
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_filt_axis(
     const float *__restrict__ in, float *__restrict__ out, const float *__restrict__ k, int K,
-    int Nz, int Nx, int B, int axis, int transp)
+    int Nz, int Nx, int B, int axis, int transp, int interleaved_out, int Bp)
 {
     const size_t P = (size_t)Nz * Nx;
     const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -321,116 +82,426 @@ __global__ __launch_bounds__(256) void k_filt_axis(
     const int N = axis == 0 ? Nz : Nx;
     const int i0 = axis == 0 ? iz : ix;
     const int sgn = transp ? 1 : -1;
-    float acc = 0.0f;
+    double acc = 0.0;
     for (int a = -K; a <= K; ++a) {
         int j = i0 + sgn * a;
         j %= N;
         if (j < 0) j += N;
         const float v = axis == 0 ? src[(size_t)ix * Nz + j] : src[(size_t)j * Nz + iz];
-        acc = fmaf(k[a + K], v, acc);
+        acc = fma((double)k[a + K], (double)v, acc);
     }
-    out[idx] = acc;
+    if (interleaved_out) out[(size_t)pxl * Bp + b] = (float)acc;
+    else out[idx] = (float)acc;
+}
+
+// ---------------------------------------------------------------------------
+// forward scatter, bucketed: one workgroup per (detector, bucket, batch chunk).
+// The bucket's records (pixel, travel time) are listed at plan time; every
+// record deposits its two interpolation weights (times each SIR tap) into an
+// LDS window of WL bins x Bc sinograms with LDS float atomics; the window is
+// then written once.  Windows of neighbouring buckets overlap and are summed
+// in a fixed order by k_bins_to_y.
+// ---------------------------------------------------------------------------
+template <bool SIR, bool VEC>
+__global__ __launch_bounds__(256) void k_scatter_bk(
+    const int2 *__restrict__ rec, const int32_t *__restrict__ rec_off, const int32_t *__restrict__ rec_k,
+    const float *__restrict__ imgI, float *__restrict__ hpart, const float *__restrict__ sir,
+    const int32_t *__restrict__ swin, int Ls, BinGeom g, int Wb, int nbk, int lo_off, int WL, int WLp,
+    int Bp, int B, int Bc)
+{
+    extern __shared__ float L[];   // [Bc][WLp]
+    const int d = blockIdx.x, kb = blockIdx.y, b0 = blockIdx.z * Bc;
+    const int nb = min(Bc, B - b0);
+    for (int i = threadIdx.x; i < Bc * WLp; i += blockDim.x) L[i] = 0.0f;
+    __syncthreads();
+    const int base = kb * Wb + lo_off;
+    const int r0 = rec_off[(size_t)d * (nbk + 1) + kb], r1 = rec_off[(size_t)d * (nbk + 1) + kb + 1];
+    const int span = g.nT + 2 * g.K;
+    for (int r = r0 + (int)threadIdx.x; r < r1; r += blockDim.x) {
+        const int2 e = rec[r];
+        const Decoded dc = decode_tau(e.y, g);
+        const int jl = 3 * dc.nn + dc.m - base;
+        const float *c = imgI + (size_t)e.x * Bp + b0;
+        const float u1 = dc.u, u0 = 1.0f - dc.u;
+        if (!SIR) {
+            if ((unsigned)jl >= (unsigned)(WL - 1)) continue;   // cannot happen for a consistent plan
+            if (VEC) {
+                for (int b = 0; b < nb; b += 4) {
+                    const float4 v = *reinterpret_cast<const float4 *>(c + b);
+                    const float cv[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) {
+                        if (b + q < nb) {
+                            atomicAdd(&L[(b + q) * WLp + jl], cv[q] * u0);
+                            atomicAdd(&L[(b + q) * WLp + jl + 1], cv[q] * u1);
+                        }
+                    }
+                }
+            } else {
+                for (int b = 0; b < nb; ++b) {
+                    const float cv = c[b];
+                    atomicAdd(&L[b * WLp + jl], cv * u0);
+                    atomicAdd(&L[b * WLp + jl + 1], cv * u1);
+                }
+            }
+        } else {
+            const int k = rec_k[r];
+            const int s0 = swin[2 * k], len = swin[2 * k + 1];
+            const float *hk = sir + (size_t)k * Ls;
+            const int half = Ls / 2;
+            for (int s = s0; s < s0 + len; ++s) {
+                const int nn = dc.nn + s - half;
+                if ((unsigned)nn >= (unsigned)span) continue;
+                const int jj = jl + 3 * (s - half);
+                if ((unsigned)jj >= (unsigned)(WL - 1)) continue;   // cannot happen for a consistent plan
+                const float wv = hk[s];
+                const float w0 = wv * u0, w1 = wv * u1;
+                for (int b = 0; b < nb; ++b) {
+                    const float cv = c[b];
+                    atomicAdd(&L[b * WLp + jj], cv * w0);
+                    atomicAdd(&L[b * WLp + jj + 1], cv * w1);
+                }
+            }
+        }
+    }
+    __syncthreads();
+    float *dst = hpart + (((size_t)d * nbk + kb) * WL) * Bp + b0;
+    for (int i = threadIdx.x; i < WL * nb; i += blockDim.x) {
+        const int jl = i / nb, b = i - jl * nb;
+        dst[(size_t)jl * Bp + b] = L[b * WLp + jl];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// window sums + polyphase hat filter (M^T): y[b][d][r] = sum_m sum_i w[m][i] bins(3(r-kmin-i+K)+m)
+// One workgroup per (time tile, detector, batch chunk).
+// ---------------------------------------------------------------------------
+constexpr int kYTile = 128;
+
+__global__ __launch_bounds__(256) void k_bins_to_y(
+    const float *__restrict__ hpart, float *__restrict__ y, HatTaps taps, BinGeom g, int Wb, int nbk,
+    int lo_off, int WL, int Bp, int D, int B, int Bc, float scale)
+{
+    extern __shared__ float sm[];
+    const int d = blockIdx.y, b0 = blockIdx.z * Bc;
+    const int nb = min(Bc, B - b0);
+    const int nT = g.nT;
+    const int t0 = blockIdx.x * kYTile, t1 = min(nT, t0 + kYTile);
+    const int kmax = taps.kmin + taps.nw - 1;
+    const int jlo = 3 * (t0 - kmax + g.K);
+    const int jhi = 3 * (t1 - 1 - taps.kmin + g.K) + 2;
+    const int nj = jhi - jlo + 1;
+    const int njp = nj | 1;
+    // load summed bins for the tile: bs[b][j - jlo]; window k covers bins [k*Wb + lo_off, k*Wb + lo_off + WL)
+    for (int i = threadIdx.x; i < nj * nb; i += blockDim.x) {
+        const int jj = i / nb, b = i - jj * nb;
+        const int rel = jlo + jj - lo_off;          // >= 0: jlo >= 3 and lo_off <= 0
+        const int ka = rel >= WL - 1 ? (rel - (WL - 1) + Wb - 1) / Wb : 0;
+        const int kz = min(nbk - 1, rel / Wb);
+        float acc = 0.0f;
+        for (int k = ka; k <= kz; ++k)
+            acc += hpart[(((size_t)d * nbk + k) * WL + (rel - k * Wb)) * Bp + b0 + b];
+        sm[b * njp + jj] = acc;
+    }
+    __syncthreads();
+    const int nr = t1 - t0;
+    for (int i = threadIdx.x; i < nr * nb; i += blockDim.x) {
+        const int b = i / nr, r = t0 + (i - b * nr);
+        const float *bs = sm + b * njp;
+        float acc = 0.0f;
+        for (int m = 0; m < 3; ++m) {
+            const int jb = 3 * (r - taps.kmin + g.K) + m - jlo;
+            for (int q = 0; q < taps.nw; ++q) acc = fmaf(taps.w[m][q], bs[jb - 3 * q], acc);
+        }
+        y[((size_t)(b0 + b) * D + d) * nT + r] = scale * acc;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// FIR along sinogram rows (MSOTmodel.m:259-269): out[t] = scale * sum_i taps[i] x[t + off - i],
+// x = 0 outside [0, nT).  Convolution: taps = IR, correlation: taps = flip(IR).
+// Each thread owns 4 consecutive outputs with a sliding register window; FP64 sums.
+// ---------------------------------------------------------------------------
+constexpr int kFirTile = 1024;   // outputs per workgroup (256 threads x 4)
+
+__global__ __launch_bounds__(256) void k_fir(const float *__restrict__ x, float *__restrict__ out,
+                                             const float *__restrict__ taps, int L, int per_det, int off,
+                                             int D, int nT, float scale)
+{
+    extern __shared__ float sm[];
+    const int row = blockIdx.y;                   // b * D + d
+    const int d = row % D;
+    const int t0 = blockIdx.x * kFirTile;
+    const int xbase = t0 + off - (L - 1);         // first input sample needed
+    const int nx = kFirTile + L + 2;
+    float *xs = sm;
+    float *ts = sm + nx;
+    const float *xr = x + (size_t)row * nT;
+    for (int i = threadIdx.x; i < nx; i += blockDim.x) {
+        const int t = xbase + i;
+        xs[i] = (t >= 0 && t < nT) ? xr[t] : 0.0f;
+    }
+    const float *tp = taps + (per_det ? (size_t)d * L : 0);
+    for (int i = threadIdx.x; i < L; i += blockDim.x) ts[i] = tp[i];
+    __syncthreads();
+    // output t = t0 + 4*tid + k needs x[t + off - i] = xs[4*tid + k + L - 1 - i]
+    const int o = 4 * threadIdx.x;
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+    float v0 = xs[o + L - 1], v1 = xs[o + L], v2 = xs[o + L + 1], v3 = xs[o + L + 2];
+    for (int i = 0; i < L; ++i) {
+        const double w = (double)ts[i];
+        a0 = fma(w, (double)v0, a0);
+        a1 = fma(w, (double)v1, a1);
+        a2 = fma(w, (double)v2, a2);
+        a3 = fma(w, (double)v3, a3);
+        v3 = v2; v2 = v1; v1 = v0;
+        v0 = (i + 1 < L) ? xs[o + L - 2 - i] : 0.0f;
+    }
+    float *orow = out + (size_t)row * nT;
+    const int t = t0 + o;
+    if (t < nT) orow[t] = scale * (float)a0;
+    if (t + 1 < nT) orow[t + 1] = scale * (float)a1;
+    if (t + 2 < nT) orow[t + 2] = scale * (float)a2;
+    if (t + 3 < nT) orow[t + 3] = scale * (float)a3;
+}
+
+// ---------------------------------------------------------------------------
+// samples -> bins (M): bins[d][j][b] = sum_i w[m][i] z[b][d][n + kmin + i], z = 0 outside [0, nT).
+// One workgroup per (bin tile, detector, batch chunk).
+// ---------------------------------------------------------------------------
+constexpr int kBinTile = 384;   // 128 samples x 3 phases
+
+__global__ __launch_bounds__(256) void k_z_to_bins(const float *__restrict__ z, float *__restrict__ bins,
+                                                   HatTaps taps, BinGeom g, int D, int B, int Bp, int Bc)
+{
+    extern __shared__ float sm[];
+    const int d = blockIdx.y, b0 = blockIdx.z * Bc;
+    const int nb = min(Bc, B - b0);
+    const int nT = g.nT, NB = g.NB;
+    const int j0 = blockIdx.x * kBinTile, j1 = min(NB, j0 + kBinTile);
+    const int nlo = j0 / 3 - g.K, nhi = (j1 - 1) / 3 - g.K;
+    const int zlo = nlo + taps.kmin, zhi = nhi + taps.kmin + taps.nw - 1;
+    const int nz = zhi - zlo + 1;
+    const int nzp = nz | 1;
+    for (int i = threadIdx.x; i < nz * nb; i += blockDim.x) {
+        const int b = i / nz, q = i - b * nz;
+        const int t = zlo + q;
+        sm[b * nzp + q] = (t >= 0 && t < nT) ? z[((size_t)(b0 + b) * D + d) * nT + t] : 0.0f;
+    }
+    __syncthreads();
+    float *brow = bins + (size_t)d * NB * Bp + b0;
+    const int njt = j1 - j0;
+    for (int i = threadIdx.x; i < njt * nb; i += blockDim.x) {
+        const int jj = i / nb, b = i - jj * nb;
+        const int j = j0 + jj;
+        const int n = j / 3 - g.K, m = j - 3 * (j / 3);
+        const float *zs = sm + b * nzp + (n + taps.kmin - zlo);
+        float acc = 0.0f;
+        for (int q = 0; q < taps.nw; ++q) acc = fmaf(taps.w[m][q], zs[q], acc);
+        brow[(size_t)j * Bp + b] = acc;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// transpose gather: one workgroup per 64-slot pixel tile; its 4 waves split the
+// detectors (unrolled by U so several travel-time loads are in flight) and are
+// reduced through LDS.  BC sinograms per pass (float4 loads of interleaved bins).
+// ---------------------------------------------------------------------------
+template <int BC>
+__device__ __forceinline__ void load_bins(const float *p, float (&v)[BC])
+{
+    if constexpr (BC % 4 == 0) {
+#pragma unroll
+        for (int q = 0; q < BC; q += 4) {
+            const float4 t = *reinterpret_cast<const float4 *>(p + q);
+            v[q] = t.x; v[q + 1] = t.y; v[q + 2] = t.z; v[q + 3] = t.w;
+        }
+    } else {
+#pragma unroll
+        for (int q = 0; q < BC; ++q) v[q] = p[q];
+    }
+}
+
+template <int BC, int U, bool SIR>
+__global__ __launch_bounds__(256) void k_gather(
+    const int32_t *__restrict__ tq, const int32_t *__restrict__ pix, const float *__restrict__ bins,
+    float *__restrict__ out, const int32_t *__restrict__ sidx, const float *__restrict__ sir,
+    const int32_t *__restrict__ swin, int Ls, BinGeom g, int Pd, int P, int D, int B, int Bp, float scale)
+{
+    __shared__ float red[4][BC][64];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int slot = blockIdx.x * 64 + lane;
+    const int b0 = blockIdx.y * BC;
+    const int span = g.nT + 2 * g.K;
+    const size_t rowstride = (size_t)g.NB * Bp;
+    float acc[BC], cmp[BC];
+#pragma unroll
+    for (int b = 0; b < BC; ++b) { acc[b] = 0.0f; cmp[b] = 0.0f; }
+    for (int d0 = wv; d0 < D; d0 += 4 * U) {
+        int32_t q[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int d = d0 + 4 * u;
+            q[u] = d < D ? tq[(size_t)d * Pd + slot] : INT32_MIN;
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int d = d0 + 4 * u;
+            const Decoded dc = decode_tau(q[u], g);
+            const float *gb = bins + (size_t)d * rowstride + b0;
+            if (!SIR) {
+                if ((unsigned)dc.nn < (unsigned)span) {
+                    const int j = 3 * dc.nn + dc.m;
+                    float g0[BC], g1[BC];
+                    load_bins<BC>(gb + (size_t)j * Bp, g0);
+                    load_bins<BC>(gb + (size_t)(j + 1) * Bp, g1);
+#pragma unroll
+                    for (int b = 0; b < BC; ++b) kahan_add(acc[b], cmp[b], fmaf(dc.u, g1[b] - g0[b], g0[b]));
+                }
+            } else if (d < D) {
+                const int k = sidx[(size_t)d * Pd + slot];
+                const int s0 = swin[2 * k], len = swin[2 * k + 1];
+                const float *hk = sir + (size_t)k * Ls;
+                const int shift = dc.nn - Ls / 2;
+                float part[BC];
+#pragma unroll
+                for (int b = 0; b < BC; ++b) part[b] = 0.0f;
+                for (int s = s0; s < s0 + len; ++s) {
+                    const int nn = shift + s;
+                    if ((unsigned)nn >= (unsigned)span) continue;
+                    const float wvl = hk[s];
+                    const int j = 3 * nn + dc.m;
+                    float g0[BC], g1[BC];
+                    load_bins<BC>(gb + (size_t)j * Bp, g0);
+                    load_bins<BC>(gb + (size_t)(j + 1) * Bp, g1);
+#pragma unroll
+                    for (int b = 0; b < BC; ++b) part[b] = fmaf(wvl, fmaf(dc.u, g1[b] - g0[b], g0[b]), part[b]);
+                }
+#pragma unroll
+                for (int b = 0; b < BC; ++b) kahan_add(acc[b], cmp[b], part[b]);
+            }
+        }
+    }
+#pragma unroll
+    for (int b = 0; b < BC; ++b) red[wv][b][lane] = acc[b] - cmp[b];
+    __syncthreads();
+    if (wv == 0) {
+        const int px = pix[slot];
+        if (px >= 0) {
+#pragma unroll
+            for (int b = 0; b < BC; ++b) {
+                if (b0 + b < B) {
+                    const float v = (red[0][b][lane] + red[1][b][lane]) + (red[2][b][lane] + red[3][b][lane]);
+                    out[(size_t)(b0 + b) * P + px] = scale * v;
+                }
+            }
+        }
+    }
 }
 
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
-template <int BC>
-static void scatter_bc(Plan *p, const float *img, int B, hipStream_t s, int chunk)
+static inline unsigned nblocks(size_t n, int t = 256) { return (unsigned)((n + t - 1) / t); }
+constexpr size_t kLdsBudget = 160 * 1024;   // LDS per workgroup on gfx950
+
+static int scatter_chunk(const Plan *p, int Bp)
 {
-    dim3 grid(p->D, p->S, (B + BC - 1) / BC);
-    const size_t lds = sizeof(float) * BC * p->geom.NB;
+    const int WLp = p->WL | 1;
+    if (Bp == 1) return 1;
+    int bc = (int)(kLdsBudget / (sizeof(float) * (size_t)WLp));
+    bc = std::min(bc, Bp) / 4 * 4;
+    return std::max(bc, 4);
+}
+
+static int launch_scatter(Plan *p, const float *imgI, int B, int Bp, hipStream_t s)
+{
+    const int WLp = p->WL | 1;
+    const int Bc = scatter_chunk(p, Bp);
+    if ((size_t)Bc * WLp * sizeof(float) > kLdsBudget)
+        return set_error(MBREC_ERR_ARG, "mbrec:size", "bucket window too long for LDS (WL=%d)", p->WL);
+    dim3 grid(p->D, p->nbk, (B + Bc - 1) / Bc);
+    const size_t lds = sizeof(float) * (size_t)Bc * WLp;
     if (p->use_sir)
-        hipLaunchKernelGGL((k_scatter<BC, true>), grid, dim3(256), lds, s, p->d_tq, p->d_pix, img,
-                           p->d_bins, p->d_sidx, p->d_sir, p->d_swin, p->Ls, p->geom, p->Pd,
-                           p->P, p->D, B, p->S, chunk);
+        hipLaunchKernelGGL((k_scatter_bk<true, false>), grid, dim3(256), lds, s, p->d_rec, p->d_rec_off, p->d_rec_k,
+                           imgI, p->d_hpart, p->d_sir, p->d_swin, p->Ls, p->geom, p->Wb, p->nbk, p->lo_off, p->WL,
+                           WLp, Bp, B, Bc);
+    else if (Bp % 4 == 0)
+        hipLaunchKernelGGL((k_scatter_bk<false, true>), grid, dim3(256), lds, s, p->d_rec, p->d_rec_off, p->d_rec_k,
+                           imgI, p->d_hpart, p->d_sir, p->d_swin, p->Ls, p->geom, p->Wb, p->nbk, p->lo_off, p->WL,
+                           WLp, Bp, B, Bc);
     else
-        hipLaunchKernelGGL((k_scatter<BC, false>), grid, dim3(256), lds, s, p->d_tq, p->d_pix,
-                           img, p->d_bins, p->d_sidx, p->d_sir, p->d_swin, p->Ls, p->geom,
-                           p->Pd, p->P, p->D, B, p->S, chunk);
-}
-
-int launch_scatter(Plan *p, const float *img, int B, hipStream_t s)
-{
-    int chunk = (p->Pd + p->S - 1) / p->S;
-    chunk = (chunk + 63) / 64 * 64;
-    // LDS budget 64 KiB per workgroup: BC * NB floats
-    const size_t row = sizeof(float) * p->geom.NB;
-    if (row * 2 <= 65536 && B >= 2)
-        scatter_bc<2>(p, img, B, s, chunk);
-    else if (row <= 65536)
-        scatter_bc<1>(p, img, B, s, chunk);
-    else
-        return set_error(MBREC_ERR_ARG, "mbrec:size", "sinogram too long for one LDS bin row (nT=%d)", p->nT);
+        hipLaunchKernelGGL((k_scatter_bk<false, false>), grid, dim3(256), lds, s, p->d_rec, p->d_rec_off, p->d_rec_k,
+                           imgI, p->d_hpart, p->d_sir, p->d_swin, p->Ls, p->geom, p->Wb, p->nbk, p->lo_off, p->WL,
+                           WLp, Bp, B, Bc);
     MBREC_HIP(hipGetLastError());
     return MBREC_OK;
 }
 
-int launch_bins_to_sino(Plan *p, float *sino, int B, bool model, hipStream_t s)
+static int launch_bins_to_y(Plan *p, float *y, int B, int Bp, float scale, hipStream_t s)
 {
-    const bool use_ir = model && p->d_ir;
-    const int Lir = use_ir ? p->Lir : 0;
-    const int tiles = (p->nT + kTimeTile - 1) / kTimeTile;
-    const int span_r = kTimeTile + (use_ir ? Lir : 0) + 2;
-    const int nj = 3 * (span_r + p->taps.nw + 2) + 3;
-    const size_t lds = sizeof(float) * (nj + span_r + Lir + 8);
-    dim3 grid(tiles, p->D, B);
-    const float scale = model ? (float)(1.0 / p->sigma) : 1.0f;
-    hipLaunchKernelGGL(k_bins_to_sino, grid, dim3(256), lds, s, p->d_bins, p->S, sino, p->taps,
-                       p->geom, use_ir ? p->d_ir : nullptr, Lir, p->ir_per_det, p->D, B, scale);
+    const int nj = 3 * (kYTile + p->taps.nw) + 3;
+    const int Bc = std::max(1, std::min(B, (int)((96 * 1024) / (sizeof(float) * (size_t)(nj | 1)))));
+    dim3 grid((p->nT + kYTile - 1) / kYTile, p->D, (B + Bc - 1) / Bc);
+    hipLaunchKernelGGL(k_bins_to_y, grid, dim3(256), sizeof(float) * (size_t)Bc * (nj | 1), s, p->d_hpart, y,
+                       p->taps, p->geom, p->Wb, p->nbk, p->lo_off, p->WL, Bp, p->D, B, Bc, scale);
     MBREC_HIP(hipGetLastError());
     return MBREC_OK;
 }
 
-int launch_sino_to_bins(Plan *p, const float *sino, int B, bool model, hipStream_t s)
+static int launch_fir(Plan *p, const float *x, float *out, int B, bool corr, float scale, hipStream_t s)
 {
-    const bool use_ir = model && p->d_ir;
-    const int Lir = use_ir ? p->Lir : 0;
-    const int tiles = (p->geom.NB + kBinTile - 1) / kBinTile;
-    const int nz = kBinTile / 3 + p->taps.nw + 4;
-    const size_t lds = sizeof(float) * (nz + (use_ir ? (nz + 2 * Lir + 4) : 0) + 8);
-    dim3 grid(tiles, p->D, B);
-    hipLaunchKernelGGL(k_sino_to_bins, grid, dim3(256), lds, s, sino, p->d_bins, p->taps, p->geom,
-                       use_ir ? p->d_ir : nullptr, Lir, p->ir_per_det, p->D, B);
+    const int L = p->Lir;
+    const int off = L / 2;            // odd L: the same offset serves the convolution and the flipped-IR correlation
+    dim3 grid((p->nT + kFirTile - 1) / kFirTile, B * p->D);
+    const size_t lds = sizeof(float) * (size_t)(kFirTile + 2 * L + 2);
+    hipLaunchKernelGGL(k_fir, grid, dim3(256), lds, s, x, out, corr ? p->d_irf : p->d_ir, L, p->ir_per_det, off,
+                       p->D, p->nT, scale);
     MBREC_HIP(hipGetLastError());
     return MBREC_OK;
 }
 
-template <int BC>
-static void gather_bc(Plan *p, float *img, int B, float scale, hipStream_t s)
+static int launch_z_to_bins(Plan *p, const float *z, int B, int Bp, hipStream_t s)
 {
-    dim3 grid(p->Pd / 64, (B + BC - 1) / BC);
+    const int nz = kBinTile / 3 + p->taps.nw + 2;
+    const int Bc = std::max(1, std::min(B, (int)((96 * 1024) / (sizeof(float) * (size_t)(nz | 1)))));
+    dim3 grid((p->geom.NB + kBinTile - 1) / kBinTile, p->D, (B + Bc - 1) / Bc);
+    hipLaunchKernelGGL(k_z_to_bins, grid, dim3(256), sizeof(float) * (size_t)Bc * (nz | 1), s, z, p->d_bins, p->taps,
+                       p->geom, p->D, B, Bp, Bc);
+    MBREC_HIP(hipGetLastError());
+    return MBREC_OK;
+}
+
+template <int BC, int U>
+static void gather_bc(Plan *p, float *img, int B, int Bp, float scale, hipStream_t s)
+{
+    dim3 grid(p->Pd / 64, (Bp + BC - 1) / BC);
     if (p->use_sir)
-        hipLaunchKernelGGL((k_gather<BC, true>), grid, dim3(256), 0, s, p->d_tq, p->d_pix,
-                           p->d_bins, img, p->d_sidx, p->d_sir, p->d_swin, p->Ls, p->geom, p->Pd,
-                           p->P, p->D, B, scale);
+        hipLaunchKernelGGL((k_gather<BC, 1, true>), grid, dim3(256), 0, s, p->d_tq, p->d_pix, p->d_bins, img,
+                           p->d_sidx, p->d_sir, p->d_swin, p->Ls, p->geom, p->Pd, p->P, p->D, B, Bp, scale);
     else
-        hipLaunchKernelGGL((k_gather<BC, false>), grid, dim3(256), 0, s, p->d_tq, p->d_pix,
-                           p->d_bins, img, p->d_sidx, p->d_sir, p->d_swin, p->Ls, p->geom, p->Pd,
-                           p->P, p->D, B, scale);
+        hipLaunchKernelGGL((k_gather<BC, U, false>), grid, dim3(256), 0, s, p->d_tq, p->d_pix, p->d_bins, img,
+                           p->d_sidx, p->d_sir, p->d_swin, p->Ls, p->geom, p->Pd, p->P, p->D, B, Bp, scale);
 }
 
-int launch_gather(Plan *p, float *img, int B, float scale, hipStream_t s)
+static int launch_gather(Plan *p, float *img, int B, int Bp, float scale, hipStream_t s)
 {
-    if (B >= 4)
-        gather_bc<4>(p, img, B, scale, s);
-    else if (B >= 2)
-        gather_bc<2>(p, img, B, scale, s);
-    else
-        gather_bc<1>(p, img, B, scale, s);
+    if (Bp == 1) gather_bc<1, 8>(p, img, B, Bp, scale, s);
+    else if (Bp % 16 == 0) gather_bc<16, 1>(p, img, B, Bp, scale, s);
+    else if (Bp % 8 == 0) gather_bc<8, 2>(p, img, B, Bp, scale, s);
+    else gather_bc<4, 4>(p, img, B, Bp, scale, s);
     MBREC_HIP(hipGetLastError());
     return MBREC_OK;
 }
 
-int launch_prefilter(Plan *p, const float *in, float *out, float *tmp, int B, bool transp,
-                     hipStream_t s)
+static int launch_prefilter(Plan *p, const float *in, float *out, float *tmp, int B, bool transp,
+                            bool interleaved_out, int Bp, hipStream_t s)
 {
     const size_t n = (size_t)p->P * B;
-    const int blocks = (int)((n + 255) / 256);
-    hipLaunchKernelGGL(k_filt_axis, dim3(blocks), dim3(256), 0, s, in, tmp, p->d_kz, p->Kz, p->Nz,
-                       p->Nx, B, 0, transp ? 1 : 0);
-    hipLaunchKernelGGL(k_filt_axis, dim3(blocks), dim3(256), 0, s, (const float *)tmp, out,
-                       p->d_kx, p->Kx, p->Nz, p->Nx, B, 1, transp ? 1 : 0);
+    hipLaunchKernelGGL(k_filt_axis, dim3(nblocks(n)), dim3(256), 0, s, in, tmp, p->d_kz, p->Kz, p->Nz, p->Nx, B, 0,
+                       transp ? 1 : 0, 0, Bp);
+    hipLaunchKernelGGL(k_filt_axis, dim3(nblocks(n)), dim3(256), 0, s, (const float *)tmp, out, p->d_kx, p->Kx,
+                       p->Nz, p->Nx, B, 1, transp ? 1 : 0, interleaved_out ? 1 : 0, Bp);
     MBREC_HIP(hipGetLastError());
     return MBREC_OK;
 }
@@ -440,53 +511,66 @@ int launch_prefilter(Plan *p, const float *in, float *out, float *tmp, int B, bo
 // ---------------------------------------------------------------------------
 int op_forward(Plan *p, const float *img, float *sino, int B, bool model, hipStream_t s)
 {
-    int rc = ensure_bins(p, B);
+    int rc = ensure_work(p, B);
     if (rc) return rc;
-    const float *src = img;
+    const int Bp = batch_stride(B);
     if (model && p->Kz >= 0) {
-        if ((rc = ensure_tmp(p, B))) return rc;
-        float *t1 = p->d_img_tmp, *t2 = p->d_img_tmp + (size_t)p->tmp_bcap * p->P;
-        if ((rc = launch_prefilter(p, img, t1, t2, B, false, s))) return rc;
-        src = t1;
+        float *t2 = p->d_img_tmp + (size_t)p->bcap * p->P;
+        if ((rc = launch_prefilter(p, img, p->d_imgI, t2, B, false, true, Bp, s))) return rc;
+    } else {
+        hipLaunchKernelGGL(k_to_interleaved, dim3(nblocks((size_t)p->P * Bp)), dim3(256), 0, s, img, p->d_imgI, p->P,
+                           B, Bp);
     }
-    if ((rc = launch_scatter(p, src, B, s))) return rc;
-    return launch_bins_to_sino(p, sino, B, model, s);
+    if ((rc = launch_scatter(p, p->d_imgI, B, Bp, s))) return rc;
+    const float scale = model ? (float)(1.0 / p->sigma) : 1.0f;
+    if (model && p->d_ir) {
+        if ((rc = launch_bins_to_y(p, p->d_sino_tmp, B, Bp, 1.0f, s))) return rc;
+        return launch_fir(p, p->d_sino_tmp, sino, B, false, scale, s);
+    }
+    return launch_bins_to_y(p, sino, B, Bp, scale, s);
 }
 
 int op_transpose(Plan *p, const float *sino, float *img, int B, bool model, hipStream_t s)
 {
-    int rc = ensure_bins(p, B);
+    int rc = ensure_work(p, B);
     if (rc) return rc;
-    if ((rc = launch_sino_to_bins(p, sino, B, model, s))) return rc;
+    const int Bp = batch_stride(B);
+    const float *z = sino;
+    if (model && p->d_ir) {
+        if ((rc = launch_fir(p, sino, p->d_sino_tmp, B, true, 1.0f, s))) return rc;
+        z = p->d_sino_tmp;
+    }
+    if ((rc = launch_z_to_bins(p, z, B, Bp, s))) return rc;
     const float scale = model ? (float)(1.0 / p->sigma) : 1.0f;
     if (model && p->Kz >= 0) {
-        if ((rc = ensure_tmp(p, B))) return rc;
-        float *t1 = p->d_img_tmp, *t2 = p->d_img_tmp + (size_t)p->tmp_bcap * p->P;
-        if ((rc = launch_gather(p, t1, B, scale, s))) return rc;
-        return launch_prefilter(p, t1, img, t2, B, true, s);
+        float *t1 = p->d_img_tmp, *t2 = p->d_img_tmp + (size_t)p->bcap * p->P;
+        if ((rc = launch_gather(p, t1, B, Bp, scale, s))) return rc;
+        return launch_prefilter(p, t1, img, t2, B, true, false, Bp, s);
     }
-    return launch_gather(p, img, B, scale, s);
+    return launch_gather(p, img, B, Bp, scale, s);
 }
 
 // ---------------------------------------------------------------------------
 // profiling: average duration of one pipeline stage, HIP events on stream s
+// (0 scatter, 1 bins->samples, 2 IR filter, 3 samples->bins, 4 gather)
 // ---------------------------------------------------------------------------
 int time_stage(Plan *p, int which, int B, int reps, hipStream_t s, float *ms)
 {
-    int rc = ensure_bins(p, B);
+    int rc = ensure_work(p, B);
     if (rc) return rc;
-    if ((rc = ensure_tmp(p, B))) return rc;
-    float *img = p->d_img_tmp;
+    const int Bp = batch_stride(B);
     float *sino = nullptr;
     MBREC_HIP(hipMalloc((void **)&sino, sizeof(float) * (size_t)B * p->D * p->nT));
     MBREC_HIP(hipMemsetAsync(sino, 0, sizeof(float) * (size_t)B * p->D * p->nT, s));
-    MBREC_HIP(hipMemsetAsync(img, 0, sizeof(float) * (size_t)B * p->P, s));
+    MBREC_HIP(hipMemsetAsync(p->d_imgI, 0, sizeof(float) * (size_t)p->P * Bp, s));
+    MBREC_HIP(hipMemsetAsync(p->d_img_tmp, 0, sizeof(float) * (size_t)p->P * B, s));
     auto run = [&]() -> int {
         switch (which) {
-        case 0: return launch_scatter(p, img, B, s);
-        case 1: return launch_bins_to_sino(p, sino, B, true, s);
-        case 2: return launch_sino_to_bins(p, sino, B, true, s);
-        case 3: return launch_gather(p, img, B, 1.0f, s);
+        case 0: return launch_scatter(p, p->d_imgI, B, Bp, s);
+        case 1: return launch_bins_to_y(p, sino, B, Bp, 1.0f, s);
+        case 2: return p->d_ir ? launch_fir(p, sino, p->d_sino_tmp, B, true, 1.0f, s) : MBREC_OK;
+        case 3: return launch_z_to_bins(p, sino, B, Bp, s);
+        case 4: return launch_gather(p, p->d_img_tmp, B, Bp, 1.0f, s);
         default: return set_error(MBREC_ERR_ARG, "mbrec:args", "unknown stage %d", which);
         }
     };

@@ -88,7 +88,7 @@ class DevicePlan:
     def tau_frac_bits(self):
         return int(_lib.lib().mbrec_plan_tau_frac_bits(self.h))
 
-    STAGES = {"scatter": 0, "bins_to_sino": 1, "sino_to_bins": 2, "gather": 3}
+    STAGES = {"scatter": 0, "bins_to_y": 1, "fir": 2, "z_to_bins": 3, "gather": 4}
 
     def time_stage(self, stage, B=1, reps=20):
         """Average device time (ms) of one operator stage, HIP events on the current stream."""

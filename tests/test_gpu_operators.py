@@ -206,16 +206,17 @@ def test_nnls_L1_matches_oracle():
     from mbrec.solvers import rec_nn_with_L1_eye_reg
     from mbrec.phantoms import disks, add_noise
     _log("nnls_L1 parity")
-    m = _geometry_model(40, 40, 32, 600, 1.25e-4, sir=False, sigma=3.0)
+    m = _geometry_model(40, 40, 32, 600, 1.25e-4, sir=False, sigma=None)
     orc = _oracle_of(m)
     ph = np.stack([disks(m.Discretization, n=6, seed=s) for s in (5, 6)])
     y = add_noise(orc.forward(ph), 0.01, seed=7)
+    eta = 0.05 * np.abs(orc.transpose(y)).max()     # a shrinkage that leaves a non-trivial image
     for coupled in (True, False):
-        rec, e_img, e_reg, *_ = rec_nn_with_L1_eye_reg(m, y, 15, 1e-3, coupled=coupled)
+        rec, e_img, e_reg, *_ = rec_nn_with_L1_eye_reg(m, y, 15, eta, coupled=coupled)
         if coupled:
-            ro, _ = orc.rec_nn_with_L1_eye_reg(y, 15, 1e-3)
+            ro, _ = orc.rec_nn_with_L1_eye_reg(y, 15, eta)
             assert rel_l2(rec, ro) < TOL_REC
         else:
             for w in range(2):
-                ro, _ = orc.rec_nn_with_L1_eye_reg(y[w:w + 1], 15, 1e-3)
+                ro, _ = orc.rec_nn_with_L1_eye_reg(y[w:w + 1], 15, eta)
                 assert rel_l2(rec[w], ro) < TOL_REC
