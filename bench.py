@@ -1,9 +1,9 @@
 """Benchmark: NNLS-L2 reconstructions per second at 400^2 px x 256 channels on one MI355X.
 
 Workload (one "step"): rec_nn_with_L2_reg, 50 outer iterations (2 LSQR inner
-iterations each, lambda = 1, beta = 1e-3) for a batch of B independent seeded
-sinograms on the 400 x 400 / 100 um grid with SIR, 256-element 40 mm / 174 deg
-arc, fs 40 MHz, 2030 samples, c 1500 m/s, delay 0, synthetic gauspuls EIR
+iterations each, lambda = 1, beta = 1e-3) for a batch of B = 64 independent
+seeded sinograms on the 400 x 400 / 100 um grid without SIR (--sir adds the
+SIR model), 256-element 40 mm / 174 deg arc, fs 40 MHz, 2030 samples, c 1500 m/s, delay 0, synthetic gauspuls EIR
 (fc 4 MHz, 100 %), band 0.1-12 MHz, explicit sigma_max.  Phantoms: 20 seeded
 disks per sinogram, 1 % Gaussian noise, lambdaL2 = 1e-2 * ||A' y_0|| with
 RegL2 = identity.  All inputs are resident in HBM before the timed region.
@@ -34,10 +34,10 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--batch", type=int, default=16)
+    ap.add_argument("--batch", type=int, default=64)
     ap.add_argument("--iters", type=int, default=50)
     ap.add_argument("--grid", type=int, default=400)
-    ap.add_argument("--no-sir", action="store_true")
+    ap.add_argument("--sir", action="store_true", help="400x400 with SIR (the SIR parity configuration)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample-reps", type=int, default=1)
     return ap.parse_args()
@@ -63,7 +63,7 @@ def main():
     from mbrec.probe import Probe
     from mbrec.solvers import rec_nn_with_L2_reg
 
-    use_sir = not a.no_sir
+    use_sir = a.sir
     N, h = a.grid, 1e-4
     half = (N - 1) / 2 * h
     t0 = time.time()

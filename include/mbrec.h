@@ -141,6 +141,12 @@ int mbrec_rec_nn_with_L2_reg_dev(mbrec_plan plan, const float *data, int B, int 
                                  int num_iter, double lambdaL2, const float *init,
                                  float *rec, double *lcurve, mbrec_stream s);
 
+/* Independent solves of B sinograms with a lambdaL2 of their own (host array
+ * [B]), as B separate rec_nn_with_L2_reg.m calls would use; otherwise as
+ * mbrec_rec_nn_with_L2_reg_dev with coupled = 0 (lcurve: 2 values per sinogram). */
+int mbrec_rec_nn_with_L2_reg_batch_dev(mbrec_plan plan, const float *data, int B, int num_iter,
+                                       const double *lambdaL2, const float *init, float *rec, double *lcurve,
+                                       mbrec_stream s);
 /* rec_nn_with_L1_eye_reg (reconstruction/rec_nn_with_L1_eye_reg.m:1-50) and
  * nnls_L1_eye_matrix_less (nnls_L1_eye_matrix_less.m:1-67): split Bregman with
  * 2 CGS iterations of (A'A + I) per outer iteration, lambdaParam = 1, etaReg =
@@ -187,6 +193,36 @@ int mbrec_bp_kernel_dev(const double *t, int Nt, double *K, mbrec_stream s);
  * (device, the not-a-knot spline map), for nbatch sinograms [nbatch][D][nT]. */
 int mbrec_upsample_detectors_dev(const double *U, int Dup, int D, const double *sig, int nT, int nbatch,
                                  double *out, mbrec_stream s);
+
+/* ---- learned warm start: 4-level convolutional U-Net ----------------------
+ * No counterpart in the reference: its NNLS loops start from zero
+ * (nnls_L2_less.m:28-29); the network's estimate enters through the initial
+ * image of mbrec_rec_nn_with_L2_reg_dev (hook at rec_nn_with_L2_reg.m:41).
+ * Images are FP32 [B][H][W] device arrays, H and W multiples of 8.  Filters
+ * and activations are bfloat16 on the matrix cores with FP32 accumulation;
+ * master weights and Adam moments are FP32.  Parameters are one FP32 vector
+ * (mbrec_unet_layout gives each layer's filter [Cout][k][k][Cin] and bias
+ * offsets).  Calls on one network must not overlap. */
+typedef struct mbrec_unet_s *mbrec_unet;
+int mbrec_unet_create(mbrec_unet *net, int base_channels, unsigned long long seed);
+int mbrec_unet_destroy(mbrec_unet net);
+size_t mbrec_unet_num_params(mbrec_unet net);
+int mbrec_unet_layout(mbrec_unet net, int layer, int *cin, int *cout, int *ksize, size_t *w_off, size_t *b_off);
+int mbrec_unet_get_params(mbrec_unet net, float *host, size_t n);
+int mbrec_unet_set_params(mbrec_unet net, const float *host, size_t n);   /* also resets Adam */
+/* out = s * net(in / s), s = rms of each input image. */
+int mbrec_unet_forward(mbrec_unet net, const float *in, float *out, int B, int H, int W, mbrec_stream s);
+/* loss = mean((net(in/s) - target/s)^2); gradients of it (host FP32, may be NULL). */
+int mbrec_unet_gradients(mbrec_unet net, const float *in, const float *target, int B, int H, int W,
+                         float *grad_host, double *loss_host, mbrec_stream s);
+/* One Adam step on that loss; loss_host (may be NULL) receives the loss before the step. */
+int mbrec_unet_train_step(mbrec_unet net, const float *in, const float *target, int B, int H, int W, float lr,
+                          float beta1, float beta2, float eps, double *loss_host, mbrec_stream s);
+/* One 3x3 zero-padded convolution on the matrix cores: x bf16 [N][H][W][Cin],
+ * w bf16 [Cout][3][3][Cin], y bf16 [N][H][W][Cout]; relu=1: y = relu(conv + bias),
+ * relu=0: y = conv (bias must be NULL).  Cin, Cout multiples of 32. */
+int mbrec_conv3x3_bf16(const uint16_t *x, const uint16_t *w, const float *bias, uint16_t *y, int N, int H, int W,
+                       int Cin, int Cout, int relu, mbrec_stream s);
 
 #ifdef __cplusplus
 }

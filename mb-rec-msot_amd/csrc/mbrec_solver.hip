@@ -33,7 +33,10 @@ struct Consts {
     int reg;            // lambdaL2 > 0
     double tol;
     int coupled;
+    const float *srv;   // per-sinogram sqrt(lambdaL2) (independent solves), or null: sr for all
 };
+
+__device__ __forceinline__ float sr_of(const Consts &k, int b) { return k.srv ? k.srv[b] : k.sr; }
 
 // ---- outer-iteration setup: u = b - A_aug x0 with x0 = w --------------------
 // sinogram block: u_s = sl*(y - A w); ||u||^2 -> Q_A, ||b||^2 -> Q_B
@@ -68,7 +71,7 @@ __global__ __launch_bounds__(256) void k_init_pix(const float *__restrict__ w, c
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
         const float wi = w[off + i], vi = vm[off + i];
         if (k.reg) {
-            const float r = -k.sr * wi;
+            const float r = -sr_of(k, b) * wi;
             ur[off + i] = r;
             uu += (double)r * r;
         }
@@ -138,7 +141,7 @@ __global__ __launch_bounds__(256) void k_v_update(const float *__restrict__ atu,
         const float beta = first ? 0.0f : (float)S.beta;
         for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
             float t = k.sl * atu[off + i] + k.sb * uc[off + i];
-            if (k.reg) t += k.sr * ur[off + i];
+            if (k.reg) t += sr_of(k, b) * ur[off + i];
             const float vn = t - beta * v[off + i];
             v[off + i] = vn;
             vv += (double)vn * vn;
@@ -204,7 +207,7 @@ __global__ __launch_bounds__(256) void k_u_pix(const float *__restrict__ v, floa
         for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
             const float vi = v[off + i];
             if (k.reg) {
-                const float r = k.sr * vi - a * ur[off + i];
+                const float r = sr_of(k, b) * vi - a * ur[off + i];
                 ur[off + i] = r;
                 uu += (double)r * r;
             }
@@ -345,6 +348,7 @@ struct Workspace {
     double *part = nullptr;
     Scal *sc = nullptr;
     double *gs = nullptr;
+    float *srv = nullptr;
     std::vector<void *> all;
     ~Workspace()
     {
@@ -399,21 +403,36 @@ static int lsqr_solve(Plan *p, Workspace &ws, const float *y, int B, int G, int 
     return MBREC_OK;
 }
 
+// lambda_b (host, [B], may be null): per-sinogram lambdaL2 of independent solves (coupled = 0)
 int nnls_l2(Plan *p, const float *y, int B, int coupled, int num_iter, double lambdaL2,
-            const float *init, float *rec, double *lcurve, hipStream_t s)
+            const float *init, float *rec, double *lcurve, hipStream_t s, const double *lambda_b = nullptr)
 {
     const size_t np = (size_t)p->P, ns = (size_t)p->D * p->nT;
     const int G = coupled ? 1 : B;
     Workspace ws;
     int rc;
-    const int reg = lambdaL2 > 0.0;
+    int reg = lambdaL2 > 0.0;
+    std::vector<float> srv;
+    if (lambda_b) {
+        reg = 0;
+        srv.resize(B);
+        for (int b = 0; b < B; ++b) {
+            if (!(lambda_b[b] >= 0.0))
+                return set_error(MBREC_ERR_ARG, "mbrec:args", "rec_nn_with_L2_reg: lambdaL2[%d] must be >= 0", b);
+            srv[b] = (float)std::sqrt(lambda_b[b]);
+            reg |= lambda_b[b] > 0.0;
+        }
+    }
     if ((rc = ws.alloc(&ws.x, np * B)) || (rc = ws.alloc(&ws.vm, np * B)) ||
         (rc = ws.alloc(&ws.d, np * B)) || (rc = ws.alloc(&ws.v, np * B)) ||
         (rc = ws.alloc(&ws.uc, np * B)) || (rc = ws.alloc(&ws.tp, np * B)) ||
         (rc = ws.alloc(&ws.ur, reg ? np * B : 1)) || (rc = ws.alloc(&ws.us, ns * B)) ||
         (rc = ws.alloc(&ws.ts, ns * B)) || (rc = ws.alloc(&ws.part, (size_t)kMaxQ * B * kRedBlocks)) ||
-        (rc = ws.alloc(&ws.sc, G)) || (rc = ws.alloc(&ws.gs, 2 * (size_t)G)))
+        (rc = ws.alloc(&ws.sc, G)) || (rc = ws.alloc(&ws.gs, 2 * (size_t)G)) ||
+        (rc = ws.alloc(&ws.srv, lambda_b ? (size_t)B : 1)))
         return rc;
+    if (lambda_b)
+        MBREC_HIP(hipMemcpyAsync(ws.srv, srv.data(), sizeof(float) * B, hipMemcpyHostToDevice, s));
     ws.w = rec;   // the outer iterate lives in the output buffer
     Consts k;
     k.sl = 1.0f;                                   // lambda = 1 (rec_nn_with_L2_reg.m:33)
@@ -422,6 +441,7 @@ int nnls_l2(Plan *p, const float *y, int B, int coupled, int num_iter, double la
     k.reg = reg;
     k.tol = 1e-8;                                  // inner_tol (nnls_L2_less.m:33)
     k.coupled = coupled;
+    k.srv = lambda_b ? ws.srv : nullptr;
     const int nb = 1024;
     // w0 = init (warm start) or 0 (nnls_L2_less.m:28); v0 = 0 (:29)
     hipLaunchKernelGGL(k_copy_or_zero, dim3(nb), dim3(256), 0, s, init, ws.w, np * B);
@@ -462,4 +482,13 @@ extern "C" int mbrec_rec_nn_with_L2_reg_dev(mbrec_plan h, const float *data, int
     if (!h || !data || !rec || B <= 0 || num_iter < 0)
         return set_error(MBREC_ERR_ARG, "mbrec:args", "rec_nn_with_L2_reg: invalid arguments");
     return nnls_l2(&h->p, data, B, coupled, num_iter, lambdaL2, init, rec, lcurve, (hipStream_t)s);
+}
+
+extern "C" int mbrec_rec_nn_with_L2_reg_batch_dev(mbrec_plan h, const float *data, int B, int num_iter,
+                                                  const double *lambdaL2, const float *init, float *rec,
+                                                  double *lcurve, mbrec_stream s)
+{
+    if (!h || !data || !rec || !lambdaL2 || B <= 0 || num_iter < 0)
+        return set_error(MBREC_ERR_ARG, "mbrec:args", "rec_nn_with_L2_reg_batch: invalid arguments");
+    return nnls_l2(&h->p, data, B, 0, num_iter, 0.0, init, rec, lcurve, (hipStream_t)s, lambdaL2);
 }

@@ -54,6 +54,8 @@ SIGNATURES = [
     ("mbrec_rec_nn_with_L2_reg_dev", C.c_int,
      [c_void_p, c_void_p, C.c_int, C.c_int, C.c_int, C.c_double, c_void_p, c_void_p, c_void_p,
       c_void_p]),
+    ("mbrec_rec_nn_with_L2_reg_batch_dev", C.c_int,
+     [c_void_p, c_void_p, C.c_int, C.c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     ("mbrec_rec_nn_with_L1_eye_reg_dev", C.c_int,
      [c_void_p, c_void_p, C.c_int, C.c_int, C.c_int, C.c_double, c_void_p, c_void_p]),
     ("mbrec_rec_nn_with_TV_reg_dev", C.c_int,
@@ -67,6 +69,21 @@ SIGNATURES = [
     ("mbrec_bp_kernel_dev", C.c_int, [c_void_p, C.c_int, c_void_p, c_void_p]),
     ("mbrec_upsample_detectors_dev", C.c_int,
      [c_void_p, C.c_int, C.c_int, c_void_p, C.c_int, C.c_int, c_void_p, c_void_p]),
+    ("mbrec_unet_create", C.c_int, [C.POINTER(c_void_p), C.c_int, C.c_ulonglong]),
+    ("mbrec_unet_destroy", C.c_int, [c_void_p]),
+    ("mbrec_unet_num_params", C.c_size_t, [c_void_p]),
+    ("mbrec_unet_layout", C.c_int,
+     [c_void_p, C.c_int, c_int32_p, c_int32_p, c_int32_p, C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)]),
+    ("mbrec_unet_get_params", C.c_int, [c_void_p, c_void_p, C.c_size_t]),
+    ("mbrec_unet_set_params", C.c_int, [c_void_p, c_void_p, C.c_size_t]),
+    ("mbrec_unet_forward", C.c_int, [c_void_p, c_void_p, c_void_p, C.c_int, C.c_int, C.c_int, c_void_p]),
+    ("mbrec_unet_gradients", C.c_int,
+     [c_void_p, c_void_p, c_void_p, C.c_int, C.c_int, C.c_int, c_void_p, c_double_p, c_void_p]),
+    ("mbrec_unet_train_step", C.c_int,
+     [c_void_p, c_void_p, c_void_p, C.c_int, C.c_int, C.c_int, C.c_float, C.c_float, C.c_float, C.c_float,
+      c_double_p, c_void_p]),
+    ("mbrec_conv3x3_bf16", C.c_int,
+     [c_void_p, c_void_p, c_void_p, c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, c_void_p]),
 ]
 
 

@@ -84,6 +84,27 @@ def rec_nn_with_L2_reg(model, data_filt, num_iter_nn_outer, lambdaL2=0.0, RegL2=
     return rec, errimg, errreg, 0.0, err, t
 
 
+def rec_nn_with_L2_reg_batch(model, data_filt, num_iter_nn_outer, lambdaL2, init=None):
+    """B independent rec_nn_with_L2_reg.m solves (RegL2 = identity), each with its own
+    lambdaL2 (array [B]), in one batched device call.  data_filt (B, D, nT) device
+    tensor; returns the (B, Nx, Nz) device tensor of reconstructions."""
+    import torch
+    y, _ = _to_device(data_filt)
+    D, nT = model.shape_sinogram
+    B = y.numel() // (D * nT)
+    lam = np.ascontiguousarray(np.broadcast_to(np.asarray(lambdaL2, dtype=np.float64), (B,)))
+    rec = torch.empty((B, *model.shape_image), dtype=torch.float32, device=y.device)
+    initp = None
+    if init is not None:
+        initp, _ = _to_device(init)
+        if initp.numel() != rec.numel():
+            raise ValueError("init has the wrong number of pixels")
+    _lib.check(_lib.lib().mbrec_rec_nn_with_L2_reg_batch_dev(
+        model.plan.h, y.data_ptr(), int(B), int(num_iter_nn_outer), lam.ctypes.data,
+        initp.data_ptr() if initp is not None else None, rec.data_ptr(), None, _lib.stream_handle()))
+    return rec
+
+
 def rec_nn_with_L1_eye_reg(model, data_filt, num_iter, regParam, *, coupled=True):
     """rec_nn_with_L1_eye_reg.m:1-50: non-negative L1 (identity) reconstruction by split
     Bregman (nnls_L1_eye_matrix_less.m) with MATLAB-cgs inner solves, started from A'y.
