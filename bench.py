@@ -145,8 +145,7 @@ def main():
     if not a.no_cpu_baseline:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import oracle as O
-        sig_px = np.floor(h / 2 * 1e6 + 0.5) / 1e6
-        kz, kx = O.full_kernels(N, N, h, h, sig_px)
+        kz, kx = O.full_kernels(N, N, disc.dz, disc.dx, disc.pixelSigma)
         kw = dict(kz=kz, kx=kx, ir=model.Data.IR)
         if use_sir:
             kw.update(sidx=model.Data.SPMRIndices, sir=model.Data.SPMR,

@@ -112,6 +112,21 @@ size_t mbrec_plan_device_bytes(mbrec_plan plan);
 /* Fixed-point travel-time format: number of fractional bits. */
 int mbrec_plan_tau_frac_bits(mbrec_plan plan);
 
+/* Autotuner: times every valid launch variant of the transpose gather (sinograms
+ * per pass x detector unroll) and of the forward scatter (LDS batch chunk) for a
+ * batch of B on this plan (reps launches each, HIP events on stream s) and keeps
+ * the fastest for later calls with the same batch stride.  choice (may be
+ * NULL) receives {gather variant, scatter chunk}.  Results do not depend on the
+ * choice beyond FP32 summation order. */
+int mbrec_plan_autotune(mbrec_plan plan, int B, int reps, mbrec_stream s, int *choice);
+/* Set the launch choice for batches of B by hand: gather variant in
+ * [0, mbrec_num_gather_variants()) or -1 (automatic); scatter choice > 0: the
+ * record-lane kernel with that LDS batch chunk (a multiple of 4), < 0: the
+ * batch-lane kernel with lane groups of -value (16, 32 or 64) sinograms, 0:
+ * automatic.  Invalid choices are rejected (mbrec:args). */
+int mbrec_plan_set_launch(mbrec_plan plan, int B, int gather_variant, int scatter_choice);
+int mbrec_num_gather_variants(void);
+
 /* Profiling: average duration (ms, HIP events on stream s) of one operator
  * stage over reps launches for a batch of B: 0 = forward pair scatter into
  * bucketed bin windows, 1 = bins -> samples (window sum + hat filter),
@@ -128,6 +143,15 @@ int mbrec_mex_transpose_dev(mbrec_plan plan, const float *sino, float *img, int 
  * (MSOTmodel.m:285-286): prefilter, MEX loop, IR convolution, 1/sigma. */
 int mbrec_model_forward_dev(mbrec_plan plan, const float *img, float *sino, int B, mbrec_stream s);
 int mbrec_model_transpose_dev(mbrec_plan plan, const float *sino, float *img, int B, mbrec_stream s);
+/* The same on cropped acquisitions: sinogram b is the window (first, count) =
+ * windows[2b], windows[2b+1] (host) of the plan's sample axis, i.e. the model of
+ * MSOTmodel.m with numCroppedSamplesAtSinogramStart increased by first and
+ * count samples; sino keeps the plan's [B][D][nT] layout, zero (forward) or
+ * ignored (transpose) outside each window.  Synchronous. */
+int mbrec_model_forward_window_dev(mbrec_plan plan, const float *img, float *sino, int B, const int32_t *windows,
+                                   mbrec_stream s);
+int mbrec_model_transpose_window_dev(mbrec_plan plan, const float *sino, float *img, int B,
+                                     const int32_t *windows, mbrec_stream s);
 
 /* rec_nn_with_L2_reg (reconstruction/rec_nn_with_L2_reg.m:1-66) with RegL2 = identity
  * and nnls_L2_less (nnls_L2_less.m:1-101): 50-style outer splitting, 2 LSQR inner
@@ -193,6 +217,27 @@ int mbrec_bp_kernel_dev(const double *t, int Nt, double *K, mbrec_stream s);
  * (device, the not-a-knot spline map), for nbatch sinograms [nbatch][D][nT]. */
 int mbrec_upsample_detectors_dev(const double *U, int Dup, int D, const double *sig, int nT, int nbatch,
                                  double *out, mbrec_stream s);
+
+/* ---- batched solves over several geometries, one graph launch -------------
+ * No counterpart in the reference (it reconstructs one acquisition per call).
+ * ngroups groups, group g = a plan (its own grid and sample axis of nT_g
+ * samples) and group_B[g] sinograms: data[g] [B_g][D][nT_g], rec[g] [B_g][P_g],
+ * init[g] (may be NULL, or the whole array NULL) [B_g][P_g] -- device buffers
+ * owned by the caller, whose addresses are captured.  windows [sum B][2]:
+ * (first sample, sample count) of every sinogram on its group's axis -- a
+ * sinogram cropped at its start and/or end; samples outside are ignored.
+ * lambdaL2 [sum B] (host).  Each sinogram is an independent
+ * rec_nn_with_L2_reg.m solve (RegL2 = identity) of num_iter outer iterations.
+ * Groups must use distinct plans.  mbrec_batch_run(use_graph = 1) captures all
+ * groups' solves into one HIP graph on first use (one per use_init value) and
+ * launches it; use_graph = 0 launches the same kernels directly. */
+typedef struct mbrec_batch_s *mbrec_batch;
+int mbrec_batch_create(mbrec_batch *batch, int ngroups, const mbrec_plan *plans, const int *group_B,
+                       const float *const *data, float *const *rec, const float *const *init,
+                       const int32_t *windows, const double *lambdaL2, int num_iter);
+int mbrec_batch_set_lambda(mbrec_batch batch, const double *lambdaL2);
+int mbrec_batch_run(mbrec_batch batch, int use_init, int use_graph, mbrec_stream s);
+int mbrec_batch_destroy(mbrec_batch batch);
 
 /* ---- learned warm start: 4-level convolutional U-Net ----------------------
  * No counterpart in the reference: its NNLS loops start from zero

@@ -63,10 +63,14 @@ static void free_work(Plan *p)
 
 int ensure_work(Plan *p, int B)
 {
-    const size_t need_h = (size_t)p->D * p->nbk * p->WL;
+    const size_t need_h = std::max<size_t>((size_t)p->D * p->nbk * p->WL, 1);
     if (B <= p->bcap && p->d_hpart && p->work_bytes && need_h * batch_stride(p->bcap) * sizeof(float) <=
                                                              p->work_bytes)
         return MBREC_OK;
+    if (p->pins)
+        return set_error(MBREC_ERR_ARG, "mbrec:planPinned",
+                         "plan workspace is held by a batch graph (batch size %d > %d); destroy the batch first", B,
+                         p->bcap);
     if (p->bcap) {
         MBREC_HIP(hipDeviceSynchronize());
         free_work(p);
@@ -96,6 +100,78 @@ void host_decode(int32_t q, const BinGeom &g, int &nn, int &m)
     nn = base + g.K;
 }
 
+// Pull records of the forward operator without SIR: per detector, the pairs
+// whose sample nn lies in [0, nT + 2K) sorted by bin j = 3 nn + m (device tile
+// order inside a bin), each stored as (pixel, interpolation weight u), with CSR
+// offsets [D][NB + 1] into the record array.  Bin j of detector d then receives
+// (1 - u) c from its own pairs and u c from the pairs of bin j - 1.
+static int build_pull_records(Plan *p, const std::vector<int32_t> &slot_pix, const std::vector<int32_t> &tq)
+{
+    const BinGeom &g = p->geom;
+    const int span = g.nT + 2 * g.K;
+    const int NB = g.NB;
+    std::vector<int32_t> off((size_t)p->D * (NB + 1), 0);
+    std::vector<int32_t> jbuf((size_t)p->D * p->Pd, -1);
+    for (int d = 0; d < p->D; ++d)
+        for (int sl = 0; sl < p->Pd; ++sl) {
+            if (slot_pix[sl] < 0) continue;
+            const size_t at = (size_t)d * p->Pd + sl;
+            int nn, m;
+            host_decode(tq[at], g, nn, m);
+            if ((unsigned)nn >= (unsigned)span) continue;
+            const int j = 3 * nn + m;
+            jbuf[at] = j;
+            off[(size_t)d * (NB + 1) + j + 1]++;
+        }
+    size_t total = 0;
+    for (int d = 0; d < p->D; ++d) {
+        int32_t *o = &off[(size_t)d * (NB + 1)];
+        if (total + (size_t)p->Pd > (size_t)INT32_MAX)
+            return set_error(MBREC_ERR_ARG, "mbrec:size", "too many (pixel, detector) pairs for int32 offsets");
+        o[0] = (int32_t)total;
+        for (int j = 0; j < NB; ++j) o[j + 1] += o[j];
+        total = o[NB];
+    }
+    std::vector<int2> rec(std::max<size_t>(total, 1));
+    std::vector<int32_t> fill(off);
+    const float scale = std::ldexp(1.0f, -g.F);
+    for (int d = 0; d < p->D; ++d)
+        for (int sl = 0; sl < p->Pd; ++sl) {
+            const size_t at = (size_t)d * p->Pd + sl;
+            const int j = jbuf[at];
+            if (j < 0) continue;
+            // u exactly as decode_tau computes it on the device
+            const int32_t q = tq[at];
+            const float fr = (float)(q & ((1 << g.F) - 1)) * scale;
+            const int m = j - 3 * (j / 3);
+            const float ph = m == 0 ? 0.0f : (m == 1 ? g.phi1 : g.phi2);
+            const float iw = m == 0 ? g.iw0 : (m == 1 ? g.iw1 : g.iw2);
+            const float u = (fr - ph) * iw;
+            int32_t ub;
+            std::memcpy(&ub, &u, sizeof(ub));
+            rec[fill[(size_t)d * (NB + 1) + j]++] = make_int2(slot_pix[sl], ub);
+        }
+    void *old[] = {p->d_rec, p->d_rec_off, p->d_rec_k};
+    for (void *q : old)
+        if (q) (void)hipFree(q);
+    p->table_bytes -= p->rec_bytes;
+    p->rec_bytes = 0;
+    p->d_rec = nullptr;
+    p->d_rec_off = p->d_rec_k = nullptr;
+    p->pull = 1;
+    p->nbk = 0;
+    p->nrec = total;
+    p->lo_off = 0;
+    p->hi_off = 1;
+    p->WL = p->Wb + 1;
+    int rc;
+    if ((rc = dalloc(&p->d_rec, rec.size(), &p->rec_bytes)) || (rc = dalloc(&p->d_rec_off, off.size(), &p->rec_bytes)) ||
+        (rc = upload(p->d_rec, rec.data(), rec.size())) || (rc = upload(p->d_rec_off, off.data(), off.size())))
+        return rc;
+    p->table_bytes += p->rec_bytes;
+    return MBREC_OK;
+}
+
 // Bucket the (pixel, travel time) pairs of every detector by bin j = 3 nn + m
 // (bucket floor(j / Wb), clamped to the first / last bucket), keeping the device
 // tile order inside a bucket.  Without SIR a pair contributes iff its sample
@@ -105,6 +181,7 @@ void host_decode(int32_t q, const BinGeom &g, int &nn, int &m)
 int build_records(Plan *p, const std::vector<int32_t> &slot_pix, const std::vector<int32_t> &tq,
                   const int32_t *sidx, const int32_t *win, int Ls)
 {
+    if (!sidx) return build_pull_records(p, slot_pix, tq);
     const BinGeom &g = p->geom;
     const int span = g.nT + 2 * g.K;
     const int half = Ls / 2;
@@ -158,8 +235,10 @@ int build_records(Plan *p, const std::vector<int32_t> &slot_pix, const std::vect
     p->rec_bytes = 0;
     p->d_rec = nullptr;
     p->d_rec_off = p->d_rec_k = nullptr;
+    p->pull = 0;
     p->nbk = nbk;
     p->nrec = total;
+    p->WL = p->Wb + p->hi_off - p->lo_off;
     int rc;
     if ((rc = dalloc(&p->d_rec, rec.size(), &p->rec_bytes)) || (rc = dalloc(&p->d_rec_off, off.size(), &p->rec_bytes)) ||
         (rc = upload(p->d_rec, rec.data(), rec.size())) || (rc = upload(p->d_rec_off, off.data(), off.size())))
@@ -319,6 +398,7 @@ int mbrec_plan_set_sir(mbrec_plan h, const int32_t *sirIdx, const double *sir, i
 {
     if (!h || !sirIdx || !sir || !se || Ls <= 0 || Nsir <= 0)
         return set_error(MBREC_ERR_ARG, "mbrec:args", "plan_set_sir: invalid arguments");
+    if (h->p.pins) return set_error(MBREC_ERR_ARG, "mbrec:planPinned", "plan is held by a batch graph");
     Plan *p = &h->p;
     const size_t tot = (size_t)Ls * Nsir;
     std::vector<int32_t> win(2 * (size_t)Nsir);
@@ -384,6 +464,7 @@ int mbrec_plan_set_ir(mbrec_plan h, const double *ir, int Lir, int per_det)
 {
     if (!h || !ir || Lir <= 0 || (Lir % 2) == 0)
         return set_error(MBREC_ERR_ARG, "mbrec:args", "plan_set_ir: IR length must be odd and positive");
+    if (h->p.pins) return set_error(MBREC_ERR_ARG, "mbrec:planPinned", "plan is held by a batch graph");
     Plan *p = &h->p;
     const size_t n = (size_t)Lir * (per_det ? p->D : 1);
     std::vector<float> f(n);
@@ -407,6 +488,7 @@ int mbrec_plan_set_prefilter(mbrec_plan h, const double *kz, int Kz, const doubl
 {
     if (!h || !kz || !kx || Kz < 0 || Kx < 0)
         return set_error(MBREC_ERR_ARG, "mbrec:args", "plan_set_prefilter: invalid arguments");
+    if (h->p.pins) return set_error(MBREC_ERR_ARG, "mbrec:planPinned", "plan is held by a batch graph");
     Plan *p = &h->p;
     if (2 * Kz + 1 > p->Nz || 2 * Kx + 1 > p->Nx)
         return set_error(MBREC_ERR_ARG, "mbrec:prefilter",
@@ -430,6 +512,7 @@ int mbrec_plan_set_sigma(mbrec_plan h, double sigma)
 {
     if (!h || !(sigma > 0.0) || !std::isfinite(sigma))
         return set_error(MBREC_ERR_ARG, "mbrec:sigma", "normalisation factor must be positive");
+    if (h->p.pins) return set_error(MBREC_ERR_ARG, "mbrec:planPinned", "plan is held by a batch graph");
     h->p.sigma = sigma;
     return MBREC_OK;
 }
@@ -437,6 +520,7 @@ int mbrec_plan_set_sigma(mbrec_plan h, double sigma)
 int mbrec_plan_destroy(mbrec_plan h)
 {
     if (!h) return MBREC_OK;
+    if (h->p.pins) return set_error(MBREC_ERR_ARG, "mbrec:planPinned", "plan is held by a batch graph");
     (void)hipDeviceSynchronize();
     free_plan(&h->p);
     delete h;
@@ -453,6 +537,29 @@ size_t mbrec_plan_device_bytes(mbrec_plan h)
            sizeof(float) * ((size_t)p->D * p->geom.NB * Bp + (size_t)p->P * Bp + (size_t)2 * p->bcap * p->P +
                             (size_t)p->bcap * p->D * p->nT);
 }
+
+int mbrec_plan_autotune(mbrec_plan h, int B, int reps, mbrec_stream s, int *choice)
+{
+    if (!h || B <= 0 || reps <= 0) return set_error(MBREC_ERR_ARG, "mbrec:args", "plan_autotune: invalid arguments");
+    if (h->p.pins) return set_error(MBREC_ERR_ARG, "mbrec:planPinned", "plan is held by a batch graph");
+    int g = 0, bc = 0;
+    int rc = autotune(&h->p, B, reps, (hipStream_t)s, &g, &bc);
+    if (rc) return rc;
+    if (choice) {
+        choice[0] = g;
+        choice[1] = bc;
+    }
+    return MBREC_OK;
+}
+
+int mbrec_plan_set_launch(mbrec_plan h, int B, int gather_variant, int scatter_choice)
+{
+    if (!h || B <= 0) return set_error(MBREC_ERR_ARG, "mbrec:args", "plan_set_launch: invalid arguments");
+    if (h->p.pins) return set_error(MBREC_ERR_ARG, "mbrec:planPinned", "plan is held by a batch graph");
+    return set_launch(&h->p, B, gather_variant, scatter_choice);
+}
+
+int mbrec_num_gather_variants(void) { return num_gather_variants(); }
 
 int mbrec_plan_tau_frac_bits(mbrec_plan h) { return h ? h->p.geom.F : -1; }
 
@@ -478,6 +585,45 @@ int mbrec_model_transpose_dev(mbrec_plan h, const float *sino, float *img, int B
 {
     if (!h || !img || !sino || B <= 0) return set_error(MBREC_ERR_ARG, "mbrec:args", "invalid arguments");
     return op_transpose(&h->p, sino, img, B, true, (hipStream_t)s);
+}
+
+// model operators on sinograms cropped to per-sinogram windows of the plan's sample axis
+static int window_op(mbrec_plan h, const float *in, float *out, int B, const int32_t *windows, hipStream_t s,
+                     bool fwd)
+{
+    if (!h || !in || !out || !windows || B <= 0)
+        return set_error(MBREC_ERR_ARG, "mbrec:args", "invalid arguments");
+    Plan *p = &h->p;
+    std::vector<int2> w(B);
+    for (int b = 0; b < B; ++b) {
+        const long long f = windows[2 * b], c = windows[2 * b + 1];
+        if (f < 0 || c <= 0 || f + c > p->nT)
+            return set_error(MBREC_ERR_ARG, "mbrec:window", "window %d [%lld, %lld) outside [0, %d)", b, f, f + c,
+                             p->nT);
+        w[b] = make_int2((int)f, (int)(f + c));
+    }
+    int2 *dw = nullptr;
+    MBREC_HIP(hipMalloc((void **)&dw, sizeof(int2) * B));
+    hipError_t e = hipMemcpyAsync(dw, w.data(), sizeof(int2) * B, hipMemcpyHostToDevice, s);
+    int rc = e == hipSuccess ? (fwd ? op_forward(p, in, out, B, true, s, dw) : op_transpose(p, in, out, B, true, s, dw))
+                             : hip_check(e, "hipMemcpyAsync");
+    const hipError_t e2 = hipStreamSynchronize(s);
+    (void)hipFree(dw);
+    if (rc) return rc;
+    MBREC_HIP(e2);
+    return MBREC_OK;
+}
+
+int mbrec_model_forward_window_dev(mbrec_plan h, const float *img, float *sino, int B, const int32_t *windows,
+                                   mbrec_stream s)
+{
+    return window_op(h, img, sino, B, windows, (hipStream_t)s, true);
+}
+
+int mbrec_model_transpose_window_dev(mbrec_plan h, const float *sino, float *img, int B, const int32_t *windows,
+                                     mbrec_stream s)
+{
+    return window_op(h, sino, img, B, windows, (hipStream_t)s, false);
 }
 
 int mbrec_plan_time_stage(mbrec_plan h, int stage, int B, int reps, mbrec_stream s, float *ms)

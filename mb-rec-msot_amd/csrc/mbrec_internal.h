@@ -44,6 +44,13 @@ struct BinGeom {
     int F;                       // fractional bits of the travel-time format
 };
 
+// launch choices picked by the autotuner for one batch stride
+struct TuneChoice {
+    int Bp = 0;          // batch stride the choices were timed for (0 = none)
+    int gather = -1;     // gather variant index
+    int scatter_bc = 0;  // scatter LDS batch chunk
+};
+
 struct Plan {
     int Nz = 0, Nx = 0, P = 0, D = 0, nT = 0;
     double R = 0.0;
@@ -56,9 +63,13 @@ struct Plan {
     // forward scatter: per detector, (pixel, travel time) records grouped into buckets
     // of Wb consecutive bins; bucket k of detector d deposits into bins
     // [k*Wb + lo_off, k*Wb + lo_off + WL)
+    // Without SIR (pull = 1) the forward is gather-only instead: d_rec holds
+    // (pixel, weight u as float bits) sorted by bin and d_rec_off [D][NB + 1]
+    // CSR offsets (see build_pull_records).
+    int pull = 0;
     int Wb = 384, nbk = 0, lo_off = 0, hi_off = 1, WL = 0;
-    int2 *d_rec = nullptr;       // [records] (pixel, fixed-point travel time)
-    int32_t *d_rec_off = nullptr;   // [D][nbk + 1] record offsets
+    int2 *d_rec = nullptr;       // [records] (pixel, fixed-point travel time) / (pixel, u)
+    int32_t *d_rec_off = nullptr;   // [D][nbk + 1] record offsets / [D][NB + 1]
     int32_t *d_rec_k = nullptr;  // [records] 0-based SIR index (SIR plans)
     size_t nrec = 0;
     size_t rec_bytes = 0;              // device bytes of d_rec / d_rec_off / d_rec_k
@@ -85,6 +96,8 @@ struct Plan {
     float *d_sino_tmp = nullptr; // [B][D][nT]       samples before/after the IR filter
     size_t table_bytes = 0;
     size_t work_bytes = 0;
+    int pins = 0;                // batches whose captured graphs use this plan's buffers
+    TuneChoice tune;
 };
 
 static inline int batch_stride(int B) { return B <= 1 ? 1 : (B + 3) / 4 * 4; }
@@ -105,9 +118,17 @@ int build_records(Plan *p, const std::vector<int32_t> &slot_pix, const std::vect
 void host_decode(int32_t q, const BinGeom &g, int &nn, int &m);
 
 int time_stage(Plan *p, int which, int B, int reps, hipStream_t s, float *ms);
+int autotune(Plan *p, int B, int reps, hipStream_t s, int *gather_out, int *scatter_bc_out);
+int set_launch(Plan *p, int B, int gather, int scatter);
+int num_gather_variants();
 
-int op_forward(Plan *p, const float *img, float *sino, int B, bool model, hipStream_t s);
-int op_transpose(Plan *p, const float *sino, float *img, int B, bool model, hipStream_t s);
+// win (device, [B], optional): per-sinogram sample window [x, y); samples outside
+// it are zero in the output (forward) and ignored in the input (transpose), both
+// before and after the IR filter -- a sinogram cropped to that window.
+int op_forward(Plan *p, const float *img, float *sino, int B, bool model, hipStream_t s,
+               const int2 *win = nullptr);
+int op_transpose(Plan *p, const float *sino, float *img, int B, bool model, hipStream_t s,
+                 const int2 *win = nullptr);
 
 }  // namespace mbrec
 

@@ -178,9 +178,12 @@ __global__ __launch_bounds__(256) void k_scatter_bk(
 // ---------------------------------------------------------------------------
 constexpr int kYTile = 128;
 
+// DIRECT: read the bins [D][NB][Bp] themselves (pull forward) instead of the
+// overlapping bucket windows
+template <bool DIRECT>
 __global__ __launch_bounds__(256) void k_bins_to_y(
     const float *__restrict__ hpart, float *__restrict__ y, HatTaps taps, BinGeom g, int Wb, int nbk,
-    int lo_off, int WL, int Bp, int D, int B, int Bc, float scale)
+    int lo_off, int WL, int Bp, int D, int B, int Bc, float scale, const int2 *__restrict__ win)
 {
     extern __shared__ float sm[];
     const int d = blockIdx.y, b0 = blockIdx.z * Bc;
@@ -195,12 +198,16 @@ __global__ __launch_bounds__(256) void k_bins_to_y(
     // load summed bins for the tile: bs[b][j - jlo]; window k covers bins [k*Wb + lo_off, k*Wb + lo_off + WL)
     for (int i = threadIdx.x; i < nj * nb; i += blockDim.x) {
         const int jj = i / nb, b = i - jj * nb;
-        const int rel = jlo + jj - lo_off;          // >= 0: jlo >= 3 and lo_off <= 0
-        const int ka = rel >= WL - 1 ? (rel - (WL - 1) + Wb - 1) / Wb : 0;
-        const int kz = min(nbk - 1, rel / Wb);
         float acc = 0.0f;
-        for (int k = ka; k <= kz; ++k)
-            acc += hpart[(((size_t)d * nbk + k) * WL + (rel - k * Wb)) * Bp + b0 + b];
+        if (DIRECT) {
+            acc = hpart[((size_t)d * g.NB + jlo + jj) * Bp + b0 + b];
+        } else {
+            const int rel = jlo + jj - lo_off;          // >= 0: jlo >= 3 and lo_off <= 0
+            const int ka = rel >= WL - 1 ? (rel - (WL - 1) + Wb - 1) / Wb : 0;
+            const int kz = min(nbk - 1, rel / Wb);
+            for (int k = ka; k <= kz; ++k)
+                acc += hpart[(((size_t)d * nbk + k) * WL + (rel - k * Wb)) * Bp + b0 + b];
+        }
         sm[b * njp + jj] = acc;
     }
     __syncthreads();
@@ -212,6 +219,10 @@ __global__ __launch_bounds__(256) void k_bins_to_y(
         for (int m = 0; m < 3; ++m) {
             const int jb = 3 * (r - taps.kmin + g.K) + m - jlo;
             for (int q = 0; q < taps.nw; ++q) acc = fmaf(taps.w[m][q], bs[jb - 3 * q], acc);
+        }
+        if (win) {
+            const int2 wb = win[b0 + b];
+            if (r < wb.x || r >= wb.y) acc = 0.0f;
         }
         y[((size_t)(b0 + b) * D + d) * nT + r] = scale * acc;
     }
@@ -226,7 +237,7 @@ constexpr int kFirTile = 1024;   // outputs per workgroup (256 threads x 4)
 
 __global__ __launch_bounds__(256) void k_fir(const float *__restrict__ x, float *__restrict__ out,
                                              const float *__restrict__ taps, int L, int per_det, int off,
-                                             int D, int nT, float scale)
+                                             int D, int nT, float scale, const int2 *__restrict__ win)
 {
     extern __shared__ float sm[];
     const int row = blockIdx.y;                   // b * D + d
@@ -237,9 +248,15 @@ __global__ __launch_bounds__(256) void k_fir(const float *__restrict__ x, float 
     float *xs = sm;
     float *ts = sm + nx;
     const float *xr = x + (size_t)row * nT;
+    int lo = 0, hi = nT;   // samples outside [lo, hi) are zero on input and output
+    if (win) {
+        const int2 wb = win[row / D];
+        lo = max(0, wb.x);
+        hi = min(nT, wb.y);
+    }
     for (int i = threadIdx.x; i < nx; i += blockDim.x) {
         const int t = xbase + i;
-        xs[i] = (t >= 0 && t < nT) ? xr[t] : 0.0f;
+        xs[i] = (t >= lo && t < hi) ? xr[t] : 0.0f;
     }
     const float *tp = taps + (per_det ? (size_t)d * L : 0);
     for (int i = threadIdx.x; i < L; i += blockDim.x) ts[i] = tp[i];
@@ -259,10 +276,10 @@ __global__ __launch_bounds__(256) void k_fir(const float *__restrict__ x, float 
     }
     float *orow = out + (size_t)row * nT;
     const int t = t0 + o;
-    if (t < nT) orow[t] = scale * (float)a0;
-    if (t + 1 < nT) orow[t + 1] = scale * (float)a1;
-    if (t + 2 < nT) orow[t + 2] = scale * (float)a2;
-    if (t + 3 < nT) orow[t + 3] = scale * (float)a3;
+    if (t < nT) orow[t] = (t >= lo && t < hi) ? scale * (float)a0 : 0.0f;
+    if (t + 1 < nT) orow[t + 1] = (t + 1 >= lo && t + 1 < hi) ? scale * (float)a1 : 0.0f;
+    if (t + 2 < nT) orow[t + 2] = (t + 2 >= lo && t + 2 < hi) ? scale * (float)a2 : 0.0f;
+    if (t + 3 < nT) orow[t + 3] = (t + 3 >= lo && t + 3 < hi) ? scale * (float)a3 : 0.0f;
 }
 
 // ---------------------------------------------------------------------------
@@ -272,7 +289,8 @@ __global__ __launch_bounds__(256) void k_fir(const float *__restrict__ x, float 
 constexpr int kBinTile = 384;   // 128 samples x 3 phases
 
 __global__ __launch_bounds__(256) void k_z_to_bins(const float *__restrict__ z, float *__restrict__ bins,
-                                                   HatTaps taps, BinGeom g, int D, int B, int Bp, int Bc)
+                                                   HatTaps taps, BinGeom g, int D, int B, int Bp, int Bc,
+                                                   const int2 *__restrict__ win)
 {
     extern __shared__ float sm[];
     const int d = blockIdx.y, b0 = blockIdx.z * Bc;
@@ -286,7 +304,13 @@ __global__ __launch_bounds__(256) void k_z_to_bins(const float *__restrict__ z, 
     for (int i = threadIdx.x; i < nz * nb; i += blockDim.x) {
         const int b = i / nz, q = i - b * nz;
         const int t = zlo + q;
-        sm[b * nzp + q] = (t >= 0 && t < nT) ? z[((size_t)(b0 + b) * D + d) * nT + t] : 0.0f;
+        int lo = 0, hi = nT;
+        if (win) {
+            const int2 wb = win[b0 + b];
+            lo = max(0, wb.x);
+            hi = min(nT, wb.y);
+        }
+        sm[b * nzp + q] = (t >= lo && t < hi) ? z[((size_t)(b0 + b) * D + d) * nT + t] : 0.0f;
     }
     __syncthreads();
     float *brow = bins + (size_t)d * NB * Bp + b0;
@@ -400,12 +424,251 @@ __global__ __launch_bounds__(256) void k_gather(
 }
 
 // ---------------------------------------------------------------------------
+// batch-lane variants for batches of >= 16 sinograms: lane l of a wave owns
+// sinogram b0 + l % LB, and the wave's 64 / LB lane groups own consecutive
+// records (scatter) or pixel slots (gather).  Every image / bins access is then
+// a contiguous run of LB floats, the travel time and SIR taps of a pair are
+// loaded once for its LB sinograms, and the LDS window atomics of one wave hit
+// LB distinct rows (odd row pitch: distinct banks).
+// ---------------------------------------------------------------------------
+template <bool SIR, int LB>
+__global__ __launch_bounds__(256) void k_scatter_bl(
+    const int2 *__restrict__ rec, const int32_t *__restrict__ rec_off, const int32_t *__restrict__ rec_k,
+    const float *__restrict__ imgI, float *__restrict__ hpart, const float *__restrict__ sir,
+    const int32_t *__restrict__ swin, int Ls, BinGeom g, int Wb, int nbk, int lo_off, int WL, int WLp, int Bp,
+    int B)
+{
+    extern __shared__ float L[];   // [LB][WLp]
+    constexpr int RPW = 64 / LB;
+    const int d = blockIdx.x, kb = blockIdx.y, b0 = blockIdx.z * LB;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int bl = lane % LB, rq = lane / LB;
+    const int b = b0 + bl;
+    const bool bvalid = b < B;
+    const int nb = min(LB, B - b0);
+    for (int i = threadIdx.x; i < LB * WLp; i += blockDim.x) L[i] = 0.0f;
+    __syncthreads();
+    const int base = kb * Wb + lo_off;
+    const int r0 = rec_off[(size_t)d * (nbk + 1) + kb], r1 = rec_off[(size_t)d * (nbk + 1) + kb + 1];
+    const int span = g.nT + 2 * g.K;
+    float *Lb = L + bl * WLp;
+    for (int r = r0 + wv * RPW + rq; r < r1; r += 4 * RPW) {
+        const int2 e = rec[r];
+        const Decoded dc = decode_tau(e.y, g);
+        const int jl = 3 * dc.nn + dc.m - base;
+        const float cv = bvalid ? imgI[(size_t)e.x * Bp + b] : 0.0f;
+        const float u1 = dc.u, u0 = 1.0f - dc.u;
+        if (!SIR) {
+            if (!bvalid || (unsigned)jl >= (unsigned)(WL - 1)) continue;   // never for a consistent plan
+            atomicAdd(&Lb[jl], cv * u0);
+            atomicAdd(&Lb[jl + 1], cv * u1);
+        } else {
+            const int k = rec_k[r];
+            const int s0 = swin[2 * k], len = swin[2 * k + 1];
+            const float *hk = sir + (size_t)k * Ls;
+            const int half = Ls / 2;
+            for (int s = s0; s < s0 + len; ++s) {
+                const int nn = dc.nn + s - half;
+                if ((unsigned)nn >= (unsigned)span) continue;
+                const int jj = jl + 3 * (s - half);
+                if (!bvalid || (unsigned)jj >= (unsigned)(WL - 1)) continue;
+                const float w = hk[s] * cv;
+                atomicAdd(&Lb[jj], w * u0);
+                atomicAdd(&Lb[jj + 1], w * u1);
+            }
+        }
+    }
+    __syncthreads();
+    float *dst = hpart + (((size_t)d * nbk + kb) * WL) * Bp + b0;
+    for (int i = threadIdx.x; i < WL * nb; i += blockDim.x) {
+        const int jl = i / nb, bb = i - jl * nb;
+        dst[(size_t)jl * Bp + bb] = L[bb * WLp + jl];
+    }
+}
+
+template <bool SIR, int LB, int U>
+__global__ __launch_bounds__(256) void k_gather_bl(
+    const int32_t *__restrict__ tq, const int32_t *__restrict__ pix, const float *__restrict__ bins,
+    float *__restrict__ out, const int32_t *__restrict__ sidx, const float *__restrict__ sir,
+    const int32_t *__restrict__ swin, int Ls, BinGeom g, int Pd, int P, int D, int B, int Bp, float scale)
+{
+    constexpr int RPW = 64 / LB;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int bl = lane % LB, rq = lane / LB;
+    const int slot = (blockIdx.x * 4 + wv) * RPW + rq;   // grid covers Pd exactly
+    const int b = blockIdx.y * LB + bl;
+    const bool bv = b < Bp;
+    const int span = g.nT + 2 * g.K;
+    const size_t rowstride = (size_t)g.NB * Bp;
+    float acc = 0.0f, cmp = 0.0f;
+    for (int d0 = 0; d0 < D; d0 += U) {
+        int32_t q[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int d = d0 + u;
+            q[u] = d < D ? tq[(size_t)d * Pd + slot] : INT32_MIN;
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int d = d0 + u;
+            const Decoded dc = decode_tau(q[u], g);
+            const float *gb = bins + (size_t)d * rowstride + b;
+            if (!SIR) {
+                if (bv && (unsigned)dc.nn < (unsigned)span) {
+                    const int j = 3 * dc.nn + dc.m;
+                    const float g0 = gb[(size_t)j * Bp], g1 = gb[(size_t)(j + 1) * Bp];
+                    kahan_add(acc, cmp, fmaf(dc.u, g1 - g0, g0));
+                }
+            } else if (d < D) {
+                const int k = sidx[(size_t)d * Pd + slot];
+                const int s0 = swin[2 * k], len = swin[2 * k + 1];
+                const float *hk = sir + (size_t)k * Ls;
+                const int shift = dc.nn - Ls / 2;
+                float part = 0.0f;
+                for (int s = s0; s < s0 + len; ++s) {
+                    const int nn = shift + s;
+                    if ((unsigned)nn >= (unsigned)span) continue;
+                    const float w = hk[s];
+                    if (bv) {
+                        const int j = 3 * nn + dc.m;
+                        const float g0 = gb[(size_t)j * Bp], g1 = gb[(size_t)(j + 1) * Bp];
+                        part = fmaf(w, fmaf(dc.u, g1 - g0, g0), part);
+                    }
+                }
+                kahan_add(acc, cmp, part);
+            }
+        }
+    }
+    const int px = pix[slot];
+    if (px >= 0 && b < B) out[(size_t)b * P + px] = scale * (acc - cmp);
+}
+
+// ---------------------------------------------------------------------------
+// forward without SIR, gather form ("pull"): pairs sorted by bin (CSR), so
+// bins[d][j][b] = sum_{pairs of bin j} (1 - u) c[px][b] + sum_{pairs of bin j-1} u c[px][b]
+// is computed without atomics, each pair read once, in a fixed order.
+// k_fwd_pull: one thread per bin (256 consecutive bins of one detector per
+// workgroup), BC sinograms per pass; the u-part of bin j - 1 comes from the
+// neighbouring thread through LDS (thread 0 computes it for the first bin).
+// ---------------------------------------------------------------------------
+template <int BC>
+__device__ __forceinline__ void load_c(const float *p, float (&v)[BC])
+{
+    if constexpr (BC % 4 == 0) {
+#pragma unroll
+        for (int q = 0; q < BC; q += 4) {
+            const float4 t = *reinterpret_cast<const float4 *>(p + q);
+            v[q] = t.x; v[q + 1] = t.y; v[q + 2] = t.z; v[q + 3] = t.w;
+        }
+    } else {
+#pragma unroll
+        for (int q = 0; q < BC; ++q) v[q] = p[q];
+    }
+}
+
+template <int BC>
+__global__ __launch_bounds__(256) void k_fwd_pull(const int2 *__restrict__ rec, const int32_t *__restrict__ off,
+                                                  const float *__restrict__ imgI, float *__restrict__ bins, int NB,
+                                                  int Bp)
+{
+    __shared__ float s1s[BC][257];
+    const int d = blockIdx.y;
+    const int j = blockIdx.x * 256 + threadIdx.x;
+    const int b0 = blockIdx.z * BC;
+    const int32_t *o = off + (size_t)d * (NB + 1);
+    float s0[BC], s1[BC], c[BC];
+#pragma unroll
+    for (int q = 0; q < BC; ++q) { s0[q] = 0.0f; s1[q] = 0.0f; }
+    if (j < NB) {
+        const int re = o[j + 1];
+        for (int r = o[j]; r < re; ++r) {
+            const int2 e = rec[r];
+            const float u = __int_as_float(e.y);
+            load_c<BC>(imgI + (size_t)e.x * Bp + b0, c);
+#pragma unroll
+            for (int q = 0; q < BC; ++q) {
+                s0[q] = fmaf(1.0f - u, c[q], s0[q]);
+                s1[q] = fmaf(u, c[q], s1[q]);
+            }
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < BC; ++q) s1s[q][threadIdx.x + 1] = s1[q];
+    if (threadIdx.x == 0) {
+        float p1[BC];
+#pragma unroll
+        for (int q = 0; q < BC; ++q) p1[q] = 0.0f;
+        if (j > 0 && j - 1 < NB) {
+            const int re = o[j];
+            for (int r = o[j - 1]; r < re; ++r) {
+                const int2 e = rec[r];
+                const float u = __int_as_float(e.y);
+                load_c<BC>(imgI + (size_t)e.x * Bp + b0, c);
+#pragma unroll
+                for (int q = 0; q < BC; ++q) p1[q] = fmaf(u, c[q], p1[q]);
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < BC; ++q) s1s[q][0] = p1[q];
+    }
+    __syncthreads();
+    if (j < NB) {
+        float *dst = bins + ((size_t)d * NB + j) * Bp + b0;
+#pragma unroll
+        for (int q = 0; q < BC; ++q) dst[q] = s0[q] + s1s[q][threadIdx.x];
+    }
+}
+
+// batch-lane form: lane l owns sinogram b0 + l % LB; each lane group walks JR
+// consecutive bins of one detector, carrying the u-part into the next bin.
+template <int LB>
+__global__ __launch_bounds__(256) void k_fwd_pull_bl(const int2 *__restrict__ rec, const int32_t *__restrict__ off,
+                                                     const float *__restrict__ imgI, float *__restrict__ bins, int NB,
+                                                     int Bp, int JR)
+{
+    constexpr int RPW = 64 / LB;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int bl = lane % LB, rq = lane / LB;
+    const int d = blockIdx.y;
+    const int ja = ((blockIdx.x * 4 + wv) * RPW + rq) * JR;
+    if (ja >= NB) return;   // no barriers in this kernel
+    const int jb = min(NB, ja + JR);
+    const int b = blockIdx.z * LB + bl;
+    const bool bv = b < Bp;
+    const int32_t *o = off + (size_t)d * (NB + 1);
+    const float *cb = imgI + b;
+    float prev = 0.0f;
+    if (ja > 0) {
+        const int re = o[ja];
+        for (int r = o[ja - 1]; r < re; ++r) {
+            const int2 e = rec[r];
+            const float cv = bv ? cb[(size_t)e.x * Bp] : 0.0f;
+            prev = fmaf(__int_as_float(e.y), cv, prev);
+        }
+    }
+    int r = o[ja];
+    for (int j = ja; j < jb; ++j) {
+        const int re = o[j + 1];
+        float s0 = 0.0f, s1 = 0.0f;
+        for (; r < re; ++r) {
+            const int2 e = rec[r];
+            const float u = __int_as_float(e.y);
+            const float cv = bv ? cb[(size_t)e.x * Bp] : 0.0f;
+            s0 = fmaf(1.0f - u, cv, s0);
+            s1 = fmaf(u, cv, s1);
+        }
+        if (bv) bins[((size_t)d * NB + j) * Bp + b] = s0 + prev;
+        prev = s1;
+    }
+}
+
+// ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
 static inline unsigned nblocks(size_t n, int t = 256) { return (unsigned)((n + t - 1) / t); }
 constexpr size_t kLdsBudget = 160 * 1024;   // LDS per workgroup on gfx950
 
-static int scatter_chunk(const Plan *p, int Bp)
+static int scatter_chunk_max(const Plan *p, int Bp)
 {
     const int WLp = p->WL | 1;
     if (Bp == 1) return 1;
@@ -414,10 +677,118 @@ static int scatter_chunk(const Plan *p, int Bp)
     return std::max(bc, 4);
 }
 
-static int launch_scatter(Plan *p, const float *imgI, int B, int Bp, hipStream_t s)
+// batch-lane scatter lane-group width for this plan and batch stride (0 = not usable)
+static int scatter_lb_max(const Plan *p, int Bp)
+{
+    if (Bp < 16) return 0;
+    const size_t WLp = (size_t)(p->WL | 1);
+    for (int lb = 64; lb >= 16; lb /= 2)
+        if (lb <= std::max(16, Bp) && lb * WLp * sizeof(float) <= kLdsBudget) return lb;
+    return 0;
+}
+
+template <int LB>
+static void scatter_bl(Plan *p, const float *imgI, int B, int Bp, hipStream_t s)
 {
     const int WLp = p->WL | 1;
-    const int Bc = scatter_chunk(p, Bp);
+    dim3 grid(p->D, p->nbk, (B + LB - 1) / LB);
+    const size_t lds = sizeof(float) * (size_t)LB * WLp;
+    if (p->use_sir)
+        hipLaunchKernelGGL((k_scatter_bl<true, LB>), grid, dim3(256), lds, s, p->d_rec, p->d_rec_off, p->d_rec_k,
+                           imgI, p->d_hpart, p->d_sir, p->d_swin, p->Ls, p->geom, p->Wb, p->nbk, p->lo_off, p->WL,
+                           WLp, Bp, B);
+    else
+        hipLaunchKernelGGL((k_scatter_bl<false, LB>), grid, dim3(256), lds, s, p->d_rec, p->d_rec_off, p->d_rec_k,
+                           imgI, p->d_hpart, p->d_sir, p->d_swin, p->Ls, p->geom, p->Wb, p->nbk, p->lo_off, p->WL,
+                           WLp, Bp, B);
+}
+
+// scatter launch choice: > 0 record-lane kernel with that LDS batch chunk, < 0
+// batch-lane kernel with lane groups of -value sinograms
+static int scatter_choice(const Plan *p, int Bp)
+{
+    if (p->tune.Bp == Bp && p->tune.scatter_bc != 0) {
+        const int c = p->tune.scatter_bc;
+        if (c > 0) return std::min(c, scatter_chunk_max(p, Bp));
+        if (-c <= scatter_lb_max(p, Bp)) return c;
+    }
+    const int lb = scatter_lb_max(p, Bp);
+    return lb ? -lb : scatter_chunk_max(p, Bp);
+}
+
+// pull forward choice: > 0 bins per thread pass BC (1, 4, 8, 16), < 0 batch-lane LB
+static bool pull_choice_ok(int c, int Bp)
+{
+    if (c > 0) return c == 1 || ((c == 4 || c == 8 || c == 16) && Bp % c == 0);
+    if (c < 0) {
+        int cap = 16;
+        while (cap < Bp) cap *= 2;
+        return Bp >= 16 && (c == -16 || c == -32 || c == -64) && -c <= cap;
+    }
+    return false;
+}
+
+static int default_pull_choice(int Bp)
+{
+    if (Bp >= 64) return -64;
+    if (Bp >= 32) return -32;
+    if (Bp >= 16) return -16;
+    if (Bp % 8 == 0) return 8;
+    if (Bp % 4 == 0) return 4;
+    return 1;
+}
+
+constexpr int kPullRange = 32;   // bins per lane group (batch-lane pull)
+
+template <int BC>
+static void fwd_pull(Plan *p, const float *imgI, int Bp, hipStream_t s)
+{
+    dim3 grid((p->geom.NB + 255) / 256, p->D, Bp / BC);
+    hipLaunchKernelGGL(k_fwd_pull<BC>, grid, dim3(256), 0, s, p->d_rec, p->d_rec_off, imgI, p->d_bins, p->geom.NB, Bp);
+}
+
+template <int LB>
+static void fwd_pull_bl(Plan *p, const float *imgI, int Bp, hipStream_t s)
+{
+    constexpr int RPW = 64 / LB;
+    const int ngrp = (p->geom.NB + kPullRange - 1) / kPullRange;
+    dim3 grid((ngrp + 4 * RPW - 1) / (4 * RPW), p->D, (Bp + LB - 1) / LB);
+    hipLaunchKernelGGL(k_fwd_pull_bl<LB>, grid, dim3(256), 0, s, p->d_rec, p->d_rec_off, imgI, p->d_bins,
+                       p->geom.NB, Bp, kPullRange);
+}
+
+static int launch_pull(Plan *p, const float *imgI, int Bp, hipStream_t s)
+{
+    int c = (p->tune.Bp == Bp && p->tune.scatter_bc != 0) ? p->tune.scatter_bc : default_pull_choice(Bp);
+    if (!pull_choice_ok(c, Bp)) c = default_pull_choice(Bp);
+    switch (c) {
+    case 1: fwd_pull<1>(p, imgI, Bp, s); break;
+    case 4: fwd_pull<4>(p, imgI, Bp, s); break;
+    case 8: fwd_pull<8>(p, imgI, Bp, s); break;
+    case 16: fwd_pull<16>(p, imgI, Bp, s); break;
+    case -16: fwd_pull_bl<16>(p, imgI, Bp, s); break;
+    case -32: fwd_pull_bl<32>(p, imgI, Bp, s); break;
+    default: fwd_pull_bl<64>(p, imgI, Bp, s); break;
+    }
+    MBREC_HIP(hipGetLastError());
+    return MBREC_OK;
+}
+
+static int launch_scatter(Plan *p, const float *imgI, int B, int Bp, hipStream_t s)
+{
+    if (p->pull) return launch_pull(p, imgI, Bp, s);
+    const int WLp = p->WL | 1;
+    if (p->WL < p->Wb + 1 || p->lo_off > 0 || p->hi_off < 1 || p->WL != p->Wb + p->hi_off - p->lo_off)
+        return set_error(MBREC_ERR_ARG, "mbrec:internal", "scatter window not set up (WL=%d)", p->WL);
+    const int ch = scatter_choice(p, Bp);
+    if (ch < 0) {
+        if (ch == -64) scatter_bl<64>(p, imgI, B, Bp, s);
+        else if (ch == -32) scatter_bl<32>(p, imgI, B, Bp, s);
+        else scatter_bl<16>(p, imgI, B, Bp, s);
+        MBREC_HIP(hipGetLastError());
+        return MBREC_OK;
+    }
+    const int Bc = ch;
     if ((size_t)Bc * WLp * sizeof(float) > kLdsBudget)
         return set_error(MBREC_ERR_ARG, "mbrec:size", "bucket window too long for LDS (WL=%d)", p->WL);
     dim3 grid(p->D, p->nbk, (B + Bc - 1) / Bc);
@@ -438,36 +809,42 @@ static int launch_scatter(Plan *p, const float *imgI, int B, int Bp, hipStream_t
     return MBREC_OK;
 }
 
-static int launch_bins_to_y(Plan *p, float *y, int B, int Bp, float scale, hipStream_t s)
+static int launch_bins_to_y(Plan *p, float *y, int B, int Bp, float scale, hipStream_t s,
+                            const int2 *win = nullptr)
 {
     const int nj = 3 * (kYTile + p->taps.nw) + 3;
     const int Bc = std::max(1, std::min(B, (int)((96 * 1024) / (sizeof(float) * (size_t)(nj | 1)))));
     dim3 grid((p->nT + kYTile - 1) / kYTile, p->D, (B + Bc - 1) / Bc);
-    hipLaunchKernelGGL(k_bins_to_y, grid, dim3(256), sizeof(float) * (size_t)Bc * (nj | 1), s, p->d_hpart, y,
-                       p->taps, p->geom, p->Wb, p->nbk, p->lo_off, p->WL, Bp, p->D, B, Bc, scale);
+    if (p->pull)
+        hipLaunchKernelGGL(k_bins_to_y<true>, grid, dim3(256), sizeof(float) * (size_t)Bc * (nj | 1), s, p->d_bins, y,
+                           p->taps, p->geom, p->Wb, p->nbk, p->lo_off, p->WL, Bp, p->D, B, Bc, scale, win);
+    else
+        hipLaunchKernelGGL(k_bins_to_y<false>, grid, dim3(256), sizeof(float) * (size_t)Bc * (nj | 1), s, p->d_hpart,
+                           y, p->taps, p->geom, p->Wb, p->nbk, p->lo_off, p->WL, Bp, p->D, B, Bc, scale, win);
     MBREC_HIP(hipGetLastError());
     return MBREC_OK;
 }
 
-static int launch_fir(Plan *p, const float *x, float *out, int B, bool corr, float scale, hipStream_t s)
+static int launch_fir(Plan *p, const float *x, float *out, int B, bool corr, float scale, hipStream_t s,
+                      const int2 *win = nullptr)
 {
     const int L = p->Lir;
     const int off = L / 2;            // odd L: the same offset serves the convolution and the flipped-IR correlation
     dim3 grid((p->nT + kFirTile - 1) / kFirTile, B * p->D);
     const size_t lds = sizeof(float) * (size_t)(kFirTile + 2 * L + 2);
     hipLaunchKernelGGL(k_fir, grid, dim3(256), lds, s, x, out, corr ? p->d_irf : p->d_ir, L, p->ir_per_det, off,
-                       p->D, p->nT, scale);
+                       p->D, p->nT, scale, win);
     MBREC_HIP(hipGetLastError());
     return MBREC_OK;
 }
 
-static int launch_z_to_bins(Plan *p, const float *z, int B, int Bp, hipStream_t s)
+static int launch_z_to_bins(Plan *p, const float *z, int B, int Bp, hipStream_t s, const int2 *win = nullptr)
 {
     const int nz = kBinTile / 3 + p->taps.nw + 2;
     const int Bc = std::max(1, std::min(B, (int)((96 * 1024) / (sizeof(float) * (size_t)(nz | 1)))));
     dim3 grid((p->geom.NB + kBinTile - 1) / kBinTile, p->D, (B + Bc - 1) / Bc);
     hipLaunchKernelGGL(k_z_to_bins, grid, dim3(256), sizeof(float) * (size_t)Bc * (nz | 1), s, z, p->d_bins, p->taps,
-                       p->geom, p->D, B, Bp, Bc);
+                       p->geom, p->D, B, Bp, Bc, win);
     MBREC_HIP(hipGetLastError());
     return MBREC_OK;
 }
@@ -484,12 +861,68 @@ static void gather_bc(Plan *p, float *img, int B, int Bp, float scale, hipStream
                            p->d_sidx, p->d_sir, p->d_swin, p->Ls, p->geom, p->Pd, p->P, p->D, B, Bp, scale);
 }
 
+template <int LB, int U>
+static void gather_bl(Plan *p, float *img, int B, int Bp, float scale, hipStream_t s)
+{
+    constexpr int RPW = 64 / LB;
+    dim3 grid(p->Pd / (4 * RPW), (Bp + LB - 1) / LB);
+    if (p->use_sir)
+        hipLaunchKernelGGL((k_gather_bl<true, LB, 1>), grid, dim3(256), 0, s, p->d_tq, p->d_pix, p->d_bins, img,
+                           p->d_sidx, p->d_sir, p->d_swin, p->Ls, p->geom, p->Pd, p->P, p->D, B, Bp, scale);
+    else
+        hipLaunchKernelGGL((k_gather_bl<false, LB, U>), grid, dim3(256), 0, s, p->d_tq, p->d_pix, p->d_bins, img,
+                           p->d_sidx, p->d_sir, p->d_swin, p->Ls, p->geom, p->Pd, p->P, p->D, B, Bp, scale);
+}
+
+// gather variants: record-lane (lb = 0; sinograms per pass BC, detector unroll U)
+// and batch-lane (lane groups of lb sinograms, detector unroll U)
+struct GatherVariant {
+    int bc, u, lb;
+};
+static const GatherVariant kGatherVariants[] = {{1, 8, 0},  {1, 4, 0},  {4, 4, 0},  {4, 2, 0},
+                                                {8, 2, 0},  {8, 1, 0},  {16, 1, 0}, {0, 2, 16},
+                                                {0, 2, 32}, {0, 1, 64}, {0, 2, 64}, {0, 4, 64}};
+constexpr int kNumGatherVariants = sizeof(kGatherVariants) / sizeof(kGatherVariants[0]);
+
+static bool gather_variant_ok(int v, int Bp)
+{
+    const GatherVariant &g = kGatherVariants[v];
+    if (g.lb) {
+        int cap = 16;
+        while (cap < Bp) cap *= 2;
+        return Bp >= 16 && g.lb <= cap;
+    }
+    return g.bc == 1 ? Bp == 1 : Bp % g.bc == 0;
+}
+
+static int default_gather_variant(int Bp)
+{
+    if (Bp == 1) return 0;
+    if (Bp >= 64) return 10;
+    if (Bp >= 32) return 8;
+    if (Bp >= 16) return 7;
+    if (Bp % 8 == 0) return 4;
+    return 2;
+}
+
 static int launch_gather(Plan *p, float *img, int B, int Bp, float scale, hipStream_t s)
 {
-    if (Bp == 1) gather_bc<1, 8>(p, img, B, Bp, scale, s);
-    else if (Bp % 16 == 0) gather_bc<16, 1>(p, img, B, Bp, scale, s);
-    else if (Bp % 8 == 0) gather_bc<8, 2>(p, img, B, Bp, scale, s);
-    else gather_bc<4, 4>(p, img, B, Bp, scale, s);
+    int v = (p->tune.Bp == Bp && p->tune.gather >= 0) ? p->tune.gather : default_gather_variant(Bp);
+    if (!gather_variant_ok(v, Bp)) v = default_gather_variant(Bp);
+    switch (v) {
+    case 0: gather_bc<1, 8>(p, img, B, Bp, scale, s); break;
+    case 1: gather_bc<1, 4>(p, img, B, Bp, scale, s); break;
+    case 2: gather_bc<4, 4>(p, img, B, Bp, scale, s); break;
+    case 3: gather_bc<4, 2>(p, img, B, Bp, scale, s); break;
+    case 4: gather_bc<8, 2>(p, img, B, Bp, scale, s); break;
+    case 5: gather_bc<8, 1>(p, img, B, Bp, scale, s); break;
+    case 6: gather_bc<16, 1>(p, img, B, Bp, scale, s); break;
+    case 7: gather_bl<16, 2>(p, img, B, Bp, scale, s); break;
+    case 8: gather_bl<32, 2>(p, img, B, Bp, scale, s); break;
+    case 9: gather_bl<64, 1>(p, img, B, Bp, scale, s); break;
+    case 10: gather_bl<64, 2>(p, img, B, Bp, scale, s); break;
+    default: gather_bl<64, 4>(p, img, B, Bp, scale, s); break;
+    }
     MBREC_HIP(hipGetLastError());
     return MBREC_OK;
 }
@@ -509,7 +942,7 @@ static int launch_prefilter(Plan *p, const float *in, float *out, float *tmp, in
 // ---------------------------------------------------------------------------
 // composite operators
 // ---------------------------------------------------------------------------
-int op_forward(Plan *p, const float *img, float *sino, int B, bool model, hipStream_t s)
+int op_forward(Plan *p, const float *img, float *sino, int B, bool model, hipStream_t s, const int2 *win)
 {
     int rc = ensure_work(p, B);
     if (rc) return rc;
@@ -524,23 +957,23 @@ int op_forward(Plan *p, const float *img, float *sino, int B, bool model, hipStr
     if ((rc = launch_scatter(p, p->d_imgI, B, Bp, s))) return rc;
     const float scale = model ? (float)(1.0 / p->sigma) : 1.0f;
     if (model && p->d_ir) {
-        if ((rc = launch_bins_to_y(p, p->d_sino_tmp, B, Bp, 1.0f, s))) return rc;
-        return launch_fir(p, p->d_sino_tmp, sino, B, false, scale, s);
+        if ((rc = launch_bins_to_y(p, p->d_sino_tmp, B, Bp, 1.0f, s, win))) return rc;
+        return launch_fir(p, p->d_sino_tmp, sino, B, false, scale, s, win);
     }
-    return launch_bins_to_y(p, sino, B, Bp, scale, s);
+    return launch_bins_to_y(p, sino, B, Bp, scale, s, win);
 }
 
-int op_transpose(Plan *p, const float *sino, float *img, int B, bool model, hipStream_t s)
+int op_transpose(Plan *p, const float *sino, float *img, int B, bool model, hipStream_t s, const int2 *win)
 {
     int rc = ensure_work(p, B);
     if (rc) return rc;
     const int Bp = batch_stride(B);
     const float *z = sino;
     if (model && p->d_ir) {
-        if ((rc = launch_fir(p, sino, p->d_sino_tmp, B, true, 1.0f, s))) return rc;
+        if ((rc = launch_fir(p, sino, p->d_sino_tmp, B, true, 1.0f, s, win))) return rc;
         z = p->d_sino_tmp;
     }
-    if ((rc = launch_z_to_bins(p, z, B, Bp, s))) return rc;
+    if ((rc = launch_z_to_bins(p, z, B, Bp, s, win))) return rc;
     const float scale = model ? (float)(1.0 / p->sigma) : 1.0f;
     if (model && p->Kz >= 0) {
         float *t1 = p->d_img_tmp, *t2 = p->d_img_tmp + (size_t)p->bcap * p->P;
@@ -591,5 +1024,102 @@ int time_stage(Plan *p, int which, int B, int reps, hipStream_t s, float *ms)
     MBREC_HIP(hipFree(sino));
     return MBREC_OK;
 }
+
+// ---------------------------------------------------------------------------
+// autotuner: time every valid gather variant and scatter batch chunk for a
+// batch of B on this plan (HIP events, reps launches each after one warm-up)
+// and keep the fastest; later launches with the same batch stride use them.
+// ---------------------------------------------------------------------------
+int autotune(Plan *p, int B, int reps, hipStream_t s, int *gather_out, int *scatter_bc_out)
+{
+    int rc = ensure_work(p, B);
+    if (rc) return rc;
+    const int Bp = batch_stride(B);
+    MBREC_HIP(hipMemsetAsync(p->d_imgI, 0, sizeof(float) * (size_t)p->P * Bp, s));
+    MBREC_HIP(hipMemsetAsync(p->d_bins, 0, sizeof(float) * (size_t)p->D * p->geom.NB * Bp, s));
+    hipEvent_t a, b;
+    MBREC_HIP(hipEventCreate(&a));
+    MBREC_HIP(hipEventCreate(&b));
+    auto timeit = [&](auto &&fn, float &ms) -> int {
+        int r = fn();
+        if (r) return r;
+        MBREC_HIP(hipEventRecord(a, s));
+        for (int i = 0; i < reps; ++i)
+            if ((r = fn())) return r;
+        MBREC_HIP(hipEventRecord(b, s));
+        MBREC_HIP(hipEventSynchronize(b));
+        MBREC_HIP(hipEventElapsedTime(&ms, a, b));
+        ms /= reps;
+        return MBREC_OK;
+    };
+    const TuneChoice saved = p->tune;
+    p->tune.Bp = Bp;
+    // gather
+    float best = 3.4e38f;
+    int best_v = default_gather_variant(Bp);
+    for (int v = 0; v < kNumGatherVariants; ++v) {
+        if (!gather_variant_ok(v, Bp)) continue;
+        p->tune.gather = v;
+        float ms = 0.0f;
+        if ((rc = timeit([&]() { return launch_gather(p, p->d_img_tmp, B, Bp, 1.0f, s); }, ms))) {
+            p->tune = saved;
+            return rc;
+        }
+        if (ms < best) { best = ms; best_v = v; }
+    }
+    p->tune.gather = best_v;
+    // scatter: record-lane LDS batch chunks, then batch-lane lane groups
+    std::vector<int> cand;
+    if (p->pull) {
+        for (int c : {1, 4, 8, 16, -16, -32, -64})
+            if (pull_choice_ok(c, Bp) && !(c == 1 && Bp > 16)) cand.push_back(c);
+    } else {
+        const int mx = scatter_chunk_max(p, Bp);
+        for (int bc = (Bp == 1 ? 1 : 4); bc < mx; bc *= 2) cand.push_back(bc);
+        cand.push_back(mx);
+        for (int lb = 16; lb <= scatter_lb_max(p, Bp); lb *= 2) cand.push_back(-lb);
+    }
+    best = 3.4e38f;
+    int best_bc = cand.back();
+    for (int c : cand) {
+        p->tune.scatter_bc = c;
+        float ms = 0.0f;
+        if ((rc = timeit([&]() { return launch_scatter(p, p->d_imgI, B, Bp, s); }, ms))) {
+            p->tune = saved;
+            return rc;
+        }
+        if (ms < best) { best = ms; best_bc = c; }
+    }
+    p->tune.scatter_bc = best_bc;
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    if (gather_out) *gather_out = p->tune.gather;
+    if (scatter_bc_out) *scatter_bc_out = p->tune.scatter_bc;
+    return MBREC_OK;
+}
+
+// explicit launch choice (validated): gather variant index, scatter choice (> 0
+// record-lane chunk, < 0 batch-lane lane group); -1 / 0 keep the automatic choice
+int set_launch(Plan *p, int B, int gather, int scatter)
+{
+    const int Bp = batch_stride(B);
+    if (gather >= kNumGatherVariants || (gather >= 0 && !gather_variant_ok(gather, Bp)))
+        return set_error(MBREC_ERR_ARG, "mbrec:args", "gather variant %d is not valid for a batch of %d", gather, B);
+    if (p->pull) {
+        if (scatter != 0 && !pull_choice_ok(scatter, Bp))
+            return set_error(MBREC_ERR_ARG, "mbrec:args", "forward choice %d is not valid for a batch of %d", scatter,
+                             B);
+    } else if (scatter > 0 && (scatter > scatter_chunk_max(p, Bp) || (Bp > 1 && scatter % 4)))
+        return set_error(MBREC_ERR_ARG, "mbrec:args", "scatter chunk %d is not valid for a batch of %d", scatter, B);
+    else if (scatter < 0 && (-scatter > scatter_lb_max(p, Bp) || (-scatter != 16 && -scatter != 32 && -scatter != 64)))
+        return set_error(MBREC_ERR_ARG, "mbrec:args", "scatter lane group %d is not valid for a batch of %d", -scatter,
+                         B);
+    p->tune.Bp = Bp;
+    p->tune.gather = gather;
+    p->tune.scatter_bc = scatter;
+    return MBREC_OK;
+}
+
+int num_gather_variants() { return kNumGatherVariants; }
 
 }  // namespace mbrec

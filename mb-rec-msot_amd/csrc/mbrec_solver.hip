@@ -369,31 +369,31 @@ struct Workspace {
 };
 
 static int lsqr_solve(Plan *p, Workspace &ws, const float *y, int B, int G, int maxit, Consts k,
-                      hipStream_t s)
+                      hipStream_t s, const int2 *win)
 {
     const size_t np = (size_t)p->P, ns = (size_t)p->D * p->nT;
     int rc;
     // u = b - A_aug w
-    if ((rc = op_forward(p, ws.w, ws.ts, B, true, s))) return rc;
+    if ((rc = op_forward(p, ws.w, ws.ts, B, true, s, win))) return rc;
     hipLaunchKernelGGL(k_init_sino, vgrid(B), dim3(256), 0, s, y, ws.ts, ws.us, ns, B, k, ws.part);
     hipLaunchKernelGGL(k_init_pix, vgrid(B), dim3(256), 0, s, ws.w, ws.vm, ws.ur, ws.uc, ws.x, ws.d, np, B, k,
                        ws.part);
     hipLaunchKernelGGL(k_scal_init, dim3(1), dim3(256), 0, s, ws.sc, ws.part, G, B, k);
     hipLaunchKernelGGL(k_scale_u, vgrid(B), dim3(256), 0, s, ws.us, ns, ws.ur, ws.uc, np, B, k, ws.sc);
     // v = A_aug' u
-    if ((rc = op_transpose(p, ws.us, ws.tp, B, true, s))) return rc;
+    if ((rc = op_transpose(p, ws.us, ws.tp, B, true, s, win))) return rc;
     hipLaunchKernelGGL(k_v_update, vgrid(B), dim3(256), 0, s, ws.tp, ws.ur, ws.uc, ws.v, np, B, k, ws.sc, 1,
                        ws.part);
     hipLaunchKernelGGL(k_scal_v0, dim3(1), dim3(256), 0, s, ws.sc, ws.part, G, B, k);
     hipLaunchKernelGGL(k_scale_v, vgrid(B), dim3(256), 0, s, ws.v, np, B, k, ws.sc);
     for (int ii = 1; ii <= maxit; ++ii) {
-        if ((rc = op_forward(p, ws.v, ws.ts, B, true, s))) return rc;
+        if ((rc = op_forward(p, ws.v, ws.ts, B, true, s, win))) return rc;
         hipLaunchKernelGGL(k_u_sino, vgrid(B), dim3(256), 0, s, ws.ts, ws.us, ns, B, k, ws.sc, ws.part);
         hipLaunchKernelGGL(k_u_pix, vgrid(B), dim3(256), 0, s, ws.v, ws.ur, ws.uc, np, B, k, ws.sc, ws.part);
         hipLaunchKernelGGL(k_scal_u, dim3(1), dim3(256), 0, s, ws.sc, ws.part, G, B, k);
         hipLaunchKernelGGL(k_scale_u, vgrid(B), dim3(256), 0, s, ws.us, ns, ws.ur, ws.uc, np, B, k, ws.sc);
         hipLaunchKernelGGL(k_dx, vgrid(B), dim3(256), 0, s, ws.v, ws.d, ws.x, np, B, k, ws.sc, ws.part);
-        if ((rc = op_transpose(p, ws.us, ws.tp, B, true, s))) return rc;
+        if ((rc = op_transpose(p, ws.us, ws.tp, B, true, s, win))) return rc;
         hipLaunchKernelGGL(k_v_update, vgrid(B), dim3(256), 0, s, ws.tp, ws.ur, ws.uc, ws.v, np, B, k, ws.sc, 0,
                            ws.part);
         hipLaunchKernelGGL(k_scal_v, dim3(1), dim3(256), 0, s, ws.sc, ws.part, G, B, k, ii);
@@ -403,13 +403,20 @@ static int lsqr_solve(Plan *p, Workspace &ws, const float *y, int B, int G, int 
     return MBREC_OK;
 }
 
-// lambda_b (host, [B], may be null): per-sinogram lambdaL2 of independent solves (coupled = 0)
-int nnls_l2(Plan *p, const float *y, int B, int coupled, int num_iter, double lambdaL2,
-            const float *init, float *rec, double *lcurve, hipStream_t s, const double *lambda_b = nullptr)
+// A prepared NNLS-L2 solve: workspace and constants for B sinograms on one plan.
+struct L2Solve {
+    Workspace ws;
+    Consts k{};
+    int B = 0, G = 0, num_iter = 0, reg = 0;
+};
+
+// Allocate the workspace, size the plan's operator workspace for B and upload the
+// per-sinogram sqrt(lambdaL2) (lambda_b, host [B], may be null: one lambdaL2 for all).
+static int l2_prepare(Plan *p, L2Solve &c, int B, int coupled, int num_iter, double lambdaL2,
+                      const double *lambda_b, hipStream_t s)
 {
     const size_t np = (size_t)p->P, ns = (size_t)p->D * p->nT;
-    const int G = coupled ? 1 : B;
-    Workspace ws;
+    Workspace &ws = c.ws;
     int rc;
     int reg = lambdaL2 > 0.0;
     std::vector<float> srv;
@@ -423,6 +430,7 @@ int nnls_l2(Plan *p, const float *y, int B, int coupled, int num_iter, double la
             reg |= lambda_b[b] > 0.0;
         }
     }
+    const int G = coupled ? 1 : B;
     if ((rc = ws.alloc(&ws.x, np * B)) || (rc = ws.alloc(&ws.vm, np * B)) ||
         (rc = ws.alloc(&ws.d, np * B)) || (rc = ws.alloc(&ws.v, np * B)) ||
         (rc = ws.alloc(&ws.uc, np * B)) || (rc = ws.alloc(&ws.tp, np * B)) ||
@@ -431,26 +439,59 @@ int nnls_l2(Plan *p, const float *y, int B, int coupled, int num_iter, double la
         (rc = ws.alloc(&ws.sc, G)) || (rc = ws.alloc(&ws.gs, 2 * (size_t)G)) ||
         (rc = ws.alloc(&ws.srv, lambda_b ? (size_t)B : 1)))
         return rc;
-    if (lambda_b)
+    if (lambda_b) {
         MBREC_HIP(hipMemcpyAsync(ws.srv, srv.data(), sizeof(float) * B, hipMemcpyHostToDevice, s));
-    ws.w = rec;   // the outer iterate lives in the output buffer
-    Consts k;
+        MBREC_HIP(hipStreamSynchronize(s));   // srv is a host temporary
+    }
+    if ((rc = ensure_work(p, B))) return rc;
+    Consts &k = c.k;
     k.sl = 1.0f;                                   // lambda = 1 (rec_nn_with_L2_reg.m:33)
     k.sb = (float)std::sqrt(1e-3);                 // beta = 1e-3 (:34)
-    k.sr = (float)(reg ? std::sqrt(lambdaL2) : 0.0);
+    k.sr = (float)(reg && !lambda_b ? std::sqrt(lambdaL2) : 0.0);
     k.reg = reg;
     k.tol = 1e-8;                                  // inner_tol (nnls_L2_less.m:33)
     k.coupled = coupled;
     k.srv = lambda_b ? ws.srv : nullptr;
+    c.B = B;
+    c.G = G;
+    c.num_iter = num_iter;
+    c.reg = reg;
+    return MBREC_OK;
+}
+
+// Launch the whole solve on stream s (kernels only: no allocation, copy or
+// synchronisation, so the sequence can be captured into a graph).
+static int l2_enqueue(Plan *p, L2Solve &c, const float *y, const float *init, float *rec, const int2 *win,
+                      hipStream_t s)
+{
+    const size_t np = (size_t)p->P;
+    Workspace &ws = c.ws;
+    const int B = c.B;
+    int rc;
+    ws.w = rec;   // the outer iterate lives in the output buffer
     const int nb = 1024;
     // w0 = init (warm start) or 0 (nnls_L2_less.m:28); v0 = 0 (:29)
     hipLaunchKernelGGL(k_copy_or_zero, dim3(nb), dim3(256), 0, s, init, ws.w, np * B);
     hipLaunchKernelGGL(k_copy_or_zero, dim3(nb), dim3(256), 0, s, (const float *)nullptr, ws.vm, np * B);
-    for (int it = 0; it < num_iter; ++it) {
-        if ((rc = lsqr_solve(p, ws, y, B, G, 2, k, s))) return rc;
-        hipLaunchKernelGGL(k_nnls_update, vgrid(B), dim3(256), 0, s, ws.x, ws.w, ws.vm, np, B, k, ws.sc);
+    for (int it = 0; it < c.num_iter; ++it) {
+        if ((rc = lsqr_solve(p, ws, y, B, c.G, 2, c.k, s, win))) return rc;
+        hipLaunchKernelGGL(k_nnls_update, vgrid(B), dim3(256), 0, s, ws.x, ws.w, ws.vm, np, B, c.k, ws.sc);
     }
     MBREC_HIP(hipGetLastError());
+    return MBREC_OK;
+}
+
+// lambda_b (host, [B], may be null): per-sinogram lambdaL2 of independent solves (coupled = 0)
+int nnls_l2(Plan *p, const float *y, int B, int coupled, int num_iter, double lambdaL2,
+            const float *init, float *rec, double *lcurve, hipStream_t s, const double *lambda_b = nullptr)
+{
+    const size_t np = (size_t)p->P, ns = (size_t)p->D * p->nT;
+    L2Solve c;
+    int rc;
+    if ((rc = l2_prepare(p, c, B, coupled, num_iter, lambdaL2, lambda_b, s))) return rc;
+    if ((rc = l2_enqueue(p, c, y, init, rec, nullptr, s))) return rc;
+    Workspace &ws = c.ws;
+    const int G = c.G;
     if (lcurve) {
         // rec_nn_with_L2_reg.m:51,54: ||A rec - y||^2 and ||RegL2 rec||^2 per group
         if ((rc = op_forward(p, ws.w, ws.ts, B, true, s))) return rc;
@@ -463,7 +504,7 @@ int nnls_l2(Plan *p, const float *y, int B, int coupled, int num_iter, double la
         MBREC_HIP(hipStreamSynchronize(s));
         for (int g = 0; g < G; ++g) {
             lcurve[2 * g] = h[g];
-            lcurve[2 * g + 1] = reg ? h[G + g] : 0.0;
+            lcurve[2 * g + 1] = c.reg ? h[G + g] : 0.0;
         }
     } else {
         MBREC_HIP(hipStreamSynchronize(s));   // workspace is freed on return
@@ -491,4 +532,212 @@ extern "C" int mbrec_rec_nn_with_L2_reg_batch_dev(mbrec_plan h, const float *dat
     if (!h || !data || !rec || !lambdaL2 || B <= 0 || num_iter < 0)
         return set_error(MBREC_ERR_ARG, "mbrec:args", "rec_nn_with_L2_reg_batch: invalid arguments");
     return nnls_l2(&h->p, data, B, 0, num_iter, 0.0, init, rec, lcurve, (hipStream_t)s, lambdaL2);
+}
+
+// ---------------------------------------------------------------------------
+// Batched solves over several geometries in one graph launch.
+//
+// A batch holds G groups; group g is a plan (grid + travel times on its own
+// sample axis of nT_g samples) and B_g sinograms, each with a sample window
+// [first, first + count) of that axis (a sinogram cropped at its start and/or
+// end) and its own lambdaL2.  All groups' complete 50-iteration-style solves
+// are captured once into a HIP graph -- one branch per group, so groups of
+// different sizes run concurrently -- and every later run is a single graph
+// launch.  Buffers are the caller's (their addresses are part of the graph).
+// ---------------------------------------------------------------------------
+namespace mbrec {
+
+__global__ __launch_bounds__(256) void k_mask_sino(const float *__restrict__ in, float *__restrict__ out,
+                                                   const int2 *__restrict__ win, int D, int nT, size_t n)
+{
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const int t = (int)(i % nT);
+        const int b = (int)(i / ((size_t)D * nT));
+        const int2 w = win[b];
+        out[i] = (t >= w.x && t < w.y) ? in[i] : 0.0f;
+    }
+}
+
+struct BatchGroup {
+    Plan *p = nullptr;
+    int B = 0;
+    const float *data = nullptr;
+    float *rec = nullptr;
+    const float *init = nullptr;
+    float *ys = nullptr;       // masked copy of the data
+    int2 *win = nullptr;       // [B] sample windows
+    L2Solve solve;
+    hipStream_t st = nullptr;
+    hipEvent_t done = nullptr;
+    bool pinned = false;
+};
+
+struct Batch {
+    std::vector<BatchGroup> g;
+    int num_iter = 0;
+    hipStream_t cap = nullptr;
+    hipEvent_t fork = nullptr;
+    hipGraph_t graph[2] = {nullptr, nullptr};
+    hipGraphExec_t exec[2] = {nullptr, nullptr};
+};
+
+static void batch_free(Batch *b)
+{
+    for (int i = 0; i < 2; ++i) {
+        if (b->exec[i]) (void)hipGraphExecDestroy(b->exec[i]);
+        if (b->graph[i]) (void)hipGraphDestroy(b->graph[i]);
+    }
+    for (BatchGroup &g : b->g) {
+        if (g.pinned) --g.p->pins;
+        if (g.ys) (void)hipFree(g.ys);
+        if (g.win) (void)hipFree(g.win);
+        if (g.st) (void)hipStreamDestroy(g.st);
+        if (g.done) (void)hipEventDestroy(g.done);
+    }
+    if (b->fork) (void)hipEventDestroy(b->fork);
+    if (b->cap) (void)hipStreamDestroy(b->cap);
+}
+
+// all groups' solves, group g on stream st_g, forked from / joined into stream s
+static int batch_enqueue(Batch *b, int use_init, hipStream_t s)
+{
+    MBREC_HIP(hipEventRecord(b->fork, s));
+    for (BatchGroup &g : b->g) {
+        MBREC_HIP(hipStreamWaitEvent(g.st, b->fork, 0));
+        const size_t n = (size_t)g.B * g.p->D * g.p->nT;
+        hipLaunchKernelGGL(k_mask_sino, dim3(1024), dim3(256), 0, g.st, g.data, g.ys, g.win, g.p->D, g.p->nT, n);
+        int rc = l2_enqueue(g.p, g.solve, g.ys, use_init ? g.init : nullptr, g.rec, g.win, g.st);
+        if (rc) return rc;
+        MBREC_HIP(hipEventRecord(g.done, g.st));
+        MBREC_HIP(hipStreamWaitEvent(s, g.done, 0));
+    }
+    return MBREC_OK;
+}
+
+}  // namespace mbrec
+
+struct mbrec_batch_s {
+    mbrec::Batch b;
+};
+
+extern "C" int mbrec_batch_destroy(mbrec_batch h)
+{
+    if (!h) return MBREC_OK;
+    (void)hipDeviceSynchronize();
+    batch_free(&h->b);
+    delete h;
+    return MBREC_OK;
+}
+
+extern "C" int mbrec_batch_create(mbrec_batch *out, int ngroups, const mbrec_plan *plans, const int *group_B,
+                                  const float *const *data, float *const *rec, const float *const *init,
+                                  const int32_t *windows, const double *lambdaL2, int num_iter)
+{
+    if (!out || ngroups <= 0 || !plans || !group_B || !data || !rec || !windows || !lambdaL2 || num_iter < 0)
+        return set_error(MBREC_ERR_ARG, "mbrec:args", "batch_create: invalid arguments");
+    *out = nullptr;
+    for (int i = 0; i < ngroups; ++i) {
+        if (!plans[i] || group_B[i] <= 0 || !data[i] || !rec[i])
+            return set_error(MBREC_ERR_ARG, "mbrec:args", "batch_create: group %d: invalid plan, size or buffer", i);
+        for (int j = 0; j < i; ++j)
+            if (plans[j] == plans[i])
+                return set_error(MBREC_ERR_ARG, "mbrec:args",
+                                 "batch_create: groups %d and %d share a plan (merge them into one group)", j, i);
+    }
+    mbrec_batch h = new mbrec_batch_s;
+    Batch *b = &h->b;
+    b->num_iter = num_iter;
+    b->g.resize(ngroups);
+    auto fail = [&](int rc) {
+        mbrec_batch_destroy(h);
+        return rc;
+    };
+    if (hipStreamCreateWithFlags(&b->cap, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&b->fork, hipEventDisableTiming) != hipSuccess)
+        return fail(set_error(MBREC_ERR_HIP, "mbrec:hip", "batch_create: stream / event creation failed"));
+    size_t off = 0;
+    for (int i = 0; i < ngroups; ++i) {
+        BatchGroup &g = b->g[i];
+        g.p = &plans[i]->p;
+        g.B = group_B[i];
+        g.data = data[i];
+        g.rec = rec[i];
+        g.init = init ? init[i] : nullptr;
+        std::vector<int2> w(g.B);
+        for (int k = 0; k < g.B; ++k) {
+            const long long first = windows[2 * (off + k)], count = windows[2 * (off + k) + 1];
+            if (first < 0 || count <= 0 || first + count > g.p->nT)
+                return fail(set_error(MBREC_ERR_ARG, "mbrec:window",
+                                      "batch_create: sinogram %zu window [%lld, %lld) outside [0, %d)", off + k,
+                                      first, first + count, g.p->nT));
+            w[k] = make_int2((int)first, (int)(first + count));
+        }
+        if (hipStreamCreateWithFlags(&g.st, hipStreamNonBlocking) != hipSuccess ||
+            hipEventCreateWithFlags(&g.done, hipEventDisableTiming) != hipSuccess ||
+            hipMalloc((void **)&g.win, sizeof(int2) * g.B) != hipSuccess ||
+            hipMalloc((void **)&g.ys, sizeof(float) * (size_t)g.B * g.p->D * g.p->nT) != hipSuccess ||
+            hipMemcpy(g.win, w.data(), sizeof(int2) * g.B, hipMemcpyHostToDevice) != hipSuccess) {
+            (void)hipGetLastError();
+            return fail(set_error(MBREC_ERR_HIP, "mbrec:hip", "batch_create: group %d allocation failed", i));
+        }
+        int rc = l2_prepare(g.p, g.solve, g.B, 0, num_iter, 0.0, lambdaL2 + off, g.st);
+        if (rc) return fail(rc);
+        ++g.p->pins;   // its workspace and tables must stay where the graph saw them
+        g.pinned = true;
+        off += g.B;
+    }
+    *out = h;
+    return MBREC_OK;
+}
+
+extern "C" int mbrec_batch_set_lambda(mbrec_batch h, const double *lambdaL2)
+{
+    if (!h || !lambdaL2) return set_error(MBREC_ERR_ARG, "mbrec:args", "batch_set_lambda: invalid arguments");
+    Batch *b = &h->b;
+    size_t off = 0;
+    for (BatchGroup &g : b->g) {
+        if (!g.solve.k.reg) {
+            for (int k = 0; k < g.B; ++k)
+                if (lambdaL2[off + k] > 0.0)
+                    return set_error(MBREC_ERR_ARG, "mbrec:args",
+                                     "batch_set_lambda: the batch was created without an L2 term in group %d",
+                                     (int)(&g - b->g.data()));
+        }
+        std::vector<float> srv(g.B);
+        for (int k = 0; k < g.B; ++k) {
+            if (!(lambdaL2[off + k] >= 0.0))
+                return set_error(MBREC_ERR_ARG, "mbrec:args", "batch_set_lambda: lambdaL2 must be >= 0");
+            srv[k] = (float)std::sqrt(lambdaL2[off + k]);
+        }
+        MBREC_HIP(hipMemcpy(g.solve.ws.srv, srv.data(), sizeof(float) * g.B, hipMemcpyHostToDevice));
+        off += g.B;
+    }
+    return MBREC_OK;
+}
+
+extern "C" int mbrec_batch_run(mbrec_batch h, int use_init, int use_graph, mbrec_stream stream)
+{
+    if (!h) return set_error(MBREC_ERR_ARG, "mbrec:args", "batch_run: null batch");
+    Batch *b = &h->b;
+    use_init = use_init ? 1 : 0;
+    if (use_init)
+        for (BatchGroup &g : b->g)
+            if (!g.init) return set_error(MBREC_ERR_ARG, "mbrec:args", "batch_run: use_init without init buffers");
+    hipStream_t s = (hipStream_t)stream;
+    if (!use_graph) return batch_enqueue(b, use_init, s);
+    if (!b->exec[use_init]) {
+        MBREC_HIP(hipStreamBeginCapture(b->cap, hipStreamCaptureModeThreadLocal));
+        int rc = batch_enqueue(b, use_init, b->cap);
+        hipGraph_t graph = nullptr;
+        const hipError_t e = hipStreamEndCapture(b->cap, &graph);
+        if (rc) {
+            if (graph) (void)hipGraphDestroy(graph);
+            return rc;
+        }
+        MBREC_HIP(e);
+        b->graph[use_init] = graph;
+        MBREC_HIP(hipGraphInstantiate(&b->exec[use_init], graph, nullptr, nullptr, 0));
+    }
+    MBREC_HIP(hipGraphLaunch(b->exec[use_init], s));
+    return MBREC_OK;
 }

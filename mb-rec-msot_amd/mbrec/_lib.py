@@ -46,11 +46,16 @@ SIGNATURES = [
     ("mbrec_plan_destroy", C.c_int, [c_void_p]),
     ("mbrec_plan_device_bytes", C.c_size_t, [c_void_p]),
     ("mbrec_plan_tau_frac_bits", C.c_int, [c_void_p]),
+    ("mbrec_plan_autotune", C.c_int, [c_void_p, C.c_int, C.c_int, c_void_p, c_int32_p]),
+    ("mbrec_plan_set_launch", C.c_int, [c_void_p, C.c_int, C.c_int, C.c_int]),
+    ("mbrec_num_gather_variants", C.c_int, []),
     ("mbrec_plan_time_stage", C.c_int, [c_void_p, C.c_int, C.c_int, C.c_int, c_void_p, c_float_p]),
     ("mbrec_mex_forward_dev", C.c_int, [c_void_p, c_void_p, c_void_p, C.c_int, c_void_p]),
     ("mbrec_mex_transpose_dev", C.c_int, [c_void_p, c_void_p, c_void_p, C.c_int, c_void_p]),
     ("mbrec_model_forward_dev", C.c_int, [c_void_p, c_void_p, c_void_p, C.c_int, c_void_p]),
     ("mbrec_model_transpose_dev", C.c_int, [c_void_p, c_void_p, c_void_p, C.c_int, c_void_p]),
+    ("mbrec_model_forward_window_dev", C.c_int, [c_void_p, c_void_p, c_void_p, C.c_int, c_void_p, c_void_p]),
+    ("mbrec_model_transpose_window_dev", C.c_int, [c_void_p, c_void_p, c_void_p, C.c_int, c_void_p, c_void_p]),
     ("mbrec_rec_nn_with_L2_reg_dev", C.c_int,
      [c_void_p, c_void_p, C.c_int, C.c_int, C.c_int, C.c_double, c_void_p, c_void_p, c_void_p,
       c_void_p]),
@@ -69,6 +74,12 @@ SIGNATURES = [
     ("mbrec_bp_kernel_dev", C.c_int, [c_void_p, C.c_int, c_void_p, c_void_p]),
     ("mbrec_upsample_detectors_dev", C.c_int,
      [c_void_p, C.c_int, C.c_int, c_void_p, C.c_int, C.c_int, c_void_p, c_void_p]),
+    ("mbrec_batch_create", C.c_int,
+     [C.POINTER(c_void_p), C.c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+      C.c_int]),
+    ("mbrec_batch_set_lambda", C.c_int, [c_void_p, c_void_p]),
+    ("mbrec_batch_run", C.c_int, [c_void_p, C.c_int, C.c_int, c_void_p]),
+    ("mbrec_batch_destroy", C.c_int, [c_void_p]),
     ("mbrec_unet_create", C.c_int, [C.POINTER(c_void_p), C.c_int, C.c_ulonglong]),
     ("mbrec_unet_destroy", C.c_int, [c_void_p]),
     ("mbrec_unet_num_params", C.c_size_t, [c_void_p]),

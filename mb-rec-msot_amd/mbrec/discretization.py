@@ -39,9 +39,13 @@ class Discretization:
             if not callable(custom_pixel_model):
                 raise ValueError("Custom pixel model must be function handle.")
             self.pixelModel = custom_pixel_model
+            self.pixelSigma = None
         else:
+            # Discretization.m:43.  At h = 125 um, h/2 = 62.5 um is a rounding tie that the
+            # grid spacing's last bit decides (min(dx, dz) may be 124.99.. um -> 62 um).
             res = min(self.dx, self.dz)
-            sigma = matlab_round(res / 2 * 1e6) / 1e6     # Discretization.m:43
+            sigma = matlab_round(res / 2 * 1e6) / 1e6
+            self.pixelSigma = sigma
             self.pixelModel = lambda r, s=sigma: gaussian_pixel_model(r, s)
         self.prefilter = self._prefilter()
         self.prefilterConj = np.conj(self.prefilter)

@@ -88,6 +88,17 @@ class DevicePlan:
     def tau_frac_bits(self):
         return int(_lib.lib().mbrec_plan_tau_frac_bits(self.h))
 
+    def autotune(self, B=1, reps=10):
+        """Pick the fastest gather / scatter launch variants for a batch of B
+        (project autotuner in libmbrec); returns (gather variant, scatter chunk)."""
+        ch = (C.c_int32 * 2)()
+        _lib.check(_lib.lib().mbrec_plan_autotune(self.h, int(B), int(reps), _lib.stream_handle(), ch))
+        return int(ch[0]), int(ch[1])
+
+    def set_launch(self, B, gather=-1, scatter=0):
+        """Force a launch variant for batches of B (see mbrec_plan_set_launch)."""
+        _lib.check(_lib.lib().mbrec_plan_set_launch(self.h, int(B), int(gather), int(scatter)))
+
     STAGES = {"scatter": 0, "bins_to_y": 1, "fir": 2, "z_to_bins": 3, "gather": 4}
 
     def time_stage(self, stage, B=1, reps=20):

@@ -91,7 +91,7 @@ def _plan_problem(Nz, Nx, D, nT, R, seed, sir=False):
 
 
 @pytest.mark.parametrize("sir", [False, True])
-@pytest.mark.parametrize("B", [1, 3, 6])
+@pytest.mark.parametrize("B", [1, 3, 6, 20, 64])
 def test_device_mex_ops_batched(sir, B):
     Nz, Nx, D, nT, R = 37, 29, 12, 220, 2.6666666666666665
     _log(f"device_mex sir={sir} B={B}")
@@ -111,6 +111,43 @@ def test_device_mex_ops_batched(sir, B):
     sgm = np.transpose(sg, (0, 2, 1))
     xto = O.transpose_sir(sgm, sidx, tt, s, se, R) if sir else O.transpose_das(sgm, tt, R)
     assert rel_l2(xt.reshape(B, -1), xto) < TOL_OP
+
+
+@pytest.mark.parametrize("sir", [False, True])
+@pytest.mark.parametrize("B", [1, 8, 20, 64])
+def test_every_launch_variant_matches_oracle(sir, B):
+    """Each gather variant and scatter choice valid for B gives the oracle's result."""
+    from mbrec import _lib
+    Nz, Nx, D, nT, R = 24, 20, 10, 200, 2.6666666666666665
+    _log(f"launch variants sir={sir} B={B}")
+    plan, tt, kw, torch = _plan_problem(Nz, Nx, D, nT, R, seed=40 + B, sir=sir)
+    x = rng(15).uniform(0, 1, (B, Nx, Nz))
+    sg = rng(16).standard_normal((B, D, nT))
+    xo = x.reshape(B, -1)
+    sgm = np.transpose(sg, (0, 2, 1))
+    if sir:
+        sidx, s, se = (a.T for a in kw["sir"])
+        yo = np.transpose(O.forward_sir(xo, sidx, tt, s, nT, se, R), (0, 2, 1))
+        xto = O.transpose_sir(sgm, sidx, tt, s, se, R)
+    else:
+        yo = np.transpose(O.forward_das(xo, tt, nT, R), (0, 2, 1))
+        xto = O.transpose_das(sgm, tt, R)
+    xd = torch.tensor(x, dtype=torch.float32).cuda()
+    sd = torch.tensor(sg, dtype=torch.float32).cuda()
+    n_ok = 0
+    for gv in range(_lib.lib().mbrec_num_gather_variants()):
+        for sc in (1, 4, 8, 16, -16, -32, -64):
+            try:
+                plan.set_launch(B, gv, sc)
+            except Exception:
+                continue
+            n_ok += 1
+            y = plan.forward(xd, model=False).cpu().numpy()
+            xt = plan.transpose(sd, model=False).cpu().numpy()
+            assert rel_l2(y, yo) < TOL_OP, (gv, sc)
+            assert rel_l2(xt.reshape(B, -1), xto) < TOL_OP, (gv, sc)
+    assert n_ok >= 1
+    plan.set_launch(B, -1, 0)
 
 
 def _geometry_model(Nz, Nx, D, nT, h, sir=False, sigma=None):
@@ -133,7 +170,7 @@ def _oracle_of(model):
     d = model.Discretization
     Nz, Nx = d.Nz, d.Nx
     h = d.dz
-    sig_px = np.floor(h / 2 * 1e6 + 0.5) / 1e6
+    sig_px = d.pixelSigma   # the model's own sigma = round(min(dx, dz) / 2 um)
     kz, kx = O.full_kernels(Nz, Nx, d.dz, d.dx, sig_px)
     kw = dict(kz=kz, kx=kx, ir=model.Data.IR)
     if model.includeSir:

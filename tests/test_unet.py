@@ -136,7 +136,11 @@ def test_conv3x3_bf16_matches_torch_fp32(require_gpu, N, H, W, Cin, Cout, relu):
 
 
 def _net_case():
-    c, N, H, W = 32, 2, 16, 16
+    # 32 x 32 (4 x 4 at the bottleneck): at 16 x 16 the bottleneck sums run over so
+    # few pixels that recomputing the oracle with FP32 instead of FP64 sums (which
+    # flips a few bfloat16 roundings) already moves deep-layer gradients by ~3 %;
+    # at 32 x 32 that noise floor is <= 0.25 % per layer.
+    c, N, H, W = 32, 2, 32, 32
     g = rng(21)
     th = U.he_init(c, 22).astype(np.float32).astype(np.float64)
     x = g.uniform(0.0, 1.0, (N, H, W)).astype(np.float32)
@@ -181,8 +185,8 @@ def test_unet_gradients_match_oracle(require_gpu):
     for (ci, co, k, wo, bo), name in zip(lay, U.LAYER_NAMES):
         gw_l, gg_l = gw[wo:bo + co], grad[wo:bo + co]
         assert np.abs(gw_l).max() > 0, name
-        assert rel_l2(gg_l, gw_l) < 2e-2, (name, rel_l2(gg_l, gw_l))
-    assert rel_l2(grad, gw) < 1e-2
+        assert rel_l2(gg_l, gw_l) < 1.5e-2, (name, rel_l2(gg_l, gw_l))
+    assert rel_l2(grad, gw) < 5e-3
 
 
 @pytest.mark.gpu
