@@ -4,9 +4,10 @@ Workload (one "step"): rec_nn_with_L2_reg, 50 outer iterations (2 LSQR inner
 iterations each, lambda = 1, beta = 1e-3) for a batch of B = 64 independent
 seeded sinograms on the 400 x 400 / 100 um grid without SIR (--sir adds the
 SIR model), 256-element 40 mm / 174 deg arc, fs 40 MHz, 2030 samples, c 1500 m/s, delay 0, synthetic gauspuls EIR
-(fc 4 MHz, 100 %), band 0.1-12 MHz, explicit sigma_max.  Phantoms: 20 seeded
-disks per sinogram, 1 % Gaussian noise, lambdaL2 = 1e-2 * ||A' y_0|| with
-RegL2 = identity.  All inputs are resident in HBM before the timed region.
+(fc 4 MHz, 100 %), band 0.1-12 MHz, sigma_max by power iteration.  Phantoms:
+seeded disks / ellipses / vessel trees, 1 % Gaussian noise, lambdaL2 =
+1e-2 * ||A'y|| per sinogram, RegL2 = identity.  The batch runs as one captured
+HIP graph (BatchSolver); all inputs are resident in HBM before the timed region.
 
 Prints one JSON line (rank 0).  value = reconstructions / s over the timed
 steps.  roofline: the dominant operator kernel (by time per solve), its
@@ -57,11 +58,11 @@ def main():
     import numpy as np
     import torch
 
+    from mbrec.batch import BatchSolver
     from mbrec.discretization import Discretization
     from mbrec.model import MSOTmodel
-    from mbrec.phantoms import disks
     from mbrec.probe import Probe
-    from mbrec.solvers import rec_nn_with_L2_reg
+    from mbrec.warmstart import phantom_batch
 
     use_sir = a.sir
     N, h = a.grid, 1e-4
@@ -70,21 +71,25 @@ def main():
     disc = Discretization([-half, half], N)
     probe = Probe("arc256_r40_174", True, False)
     model = MSOTmodel(probe, disc, 1500.0, use_sir, True, 0, 1e5, 12e6, None, sigma_iterations=40)
-    log(f"setup {time.time() - t0:.1f} s, sigma {model.normalizationFactor:.6g}, "
-        f"device tables {model.plan.device_bytes / 1e6:.1f} MB")
     D, nT = model.shape_sinogram
     B = a.batch
-    ph = np.stack([disks(disc, n=20, seed=1000 + i) for i in range(B)]).astype(np.float32)
+    tune = model.plan.autotune(B, 5)
+    log(f"setup {time.time() - t0:.1f} s, sigma {model.normalizationFactor:.6g}, "
+        f"device tables {model.plan.device_bytes / 1e6:.1f} MB, launch choice {tune}")
+    ph = phantom_batch(disc, B, 1000)
     x_true = torch.tensor(ph).cuda()
     y = model.applyForward(x_true)
     g = torch.Generator(device="cuda").manual_seed(7)
     noise = torch.randn(y.shape, generator=g, device="cuda")
     y = (y + 0.01 * y.abs().amax(dim=(1, 2), keepdim=True) * noise).contiguous()
-    lam = 1e-2 * torch.linalg.vector_norm(model.applyTranspose(y[:1]).double()).item()
+    # lambdaL2 = 1e-2 ||A'y|| per sinogram (reconstruction_overview_script.m:47-48)
+    lam = (1e-2 * torch.linalg.vector_norm(model.applyTranspose(y).reshape(B, -1).double(), dim=1)).cpu().numpy()
+    bs = BatchSolver([(model, [(0, nT)] * B)], a.iters, lam)
+    bs.data[0].copy_(y)
     torch.cuda.synchronize()
 
     def step():
-        return rec_nn_with_L2_reg(model, y, a.iters, lam, coupled=False)
+        return bs.run(graph=True)
 
     for _ in range(a.warmup):
         step()
@@ -176,7 +181,7 @@ def main():
                                   f"{'SIR' if use_sir else 'no SIR'}, {D} ch x {nT} samples, batch {B} "
                                   "independent seeded sinograms (plain cold start)",
                       "batch": B, "grid": N, "detectors": D, "samples": nT, "sir": use_sir,
-                      "outer_iterations": a.iters, "lambdaL2": lam},
+                      "outer_iterations": a.iters, "lambdaL2_median": float(np.median(lam))},
            "rel_err_vs_phantom": round(rel_err_truth, 5),
            "roofline": roofline, "cpu_baseline": cpu}
     print(json.dumps(out), flush=True)

@@ -167,10 +167,12 @@ int mbrec_rec_nn_with_L2_reg_dev(mbrec_plan plan, const float *data, int B, int 
 
 /* Independent solves of B sinograms with a lambdaL2 of their own (host array
  * [B]), as B separate rec_nn_with_L2_reg.m calls would use; otherwise as
- * mbrec_rec_nn_with_L2_reg_dev with coupled = 0 (lcurve: 2 values per sinogram). */
+ * mbrec_rec_nn_with_L2_reg_dev with coupled = 0 (lcurve: 2 values per sinogram).
+ * init / init_dual (may be NULL): starting outer iterate w and scaled dual v of
+ * nnls_L2_less.m:28-29 (zero there) -- the warm start of a learned estimate. */
 int mbrec_rec_nn_with_L2_reg_batch_dev(mbrec_plan plan, const float *data, int B, int num_iter,
-                                       const double *lambdaL2, const float *init, float *rec, double *lcurve,
-                                       mbrec_stream s);
+                                       const double *lambdaL2, const float *init, const float *init_dual,
+                                       float *rec, double *lcurve, mbrec_stream s);
 /* rec_nn_with_L1_eye_reg (reconstruction/rec_nn_with_L1_eye_reg.m:1-50) and
  * nnls_L1_eye_matrix_less (nnls_L1_eye_matrix_less.m:1-67): split Bregman with
  * 2 CGS iterations of (A'A + I) per outer iteration, lambdaParam = 1, etaReg =
@@ -222,7 +224,8 @@ int mbrec_upsample_detectors_dev(const double *U, int Dup, int D, const double *
  * No counterpart in the reference (it reconstructs one acquisition per call).
  * ngroups groups, group g = a plan (its own grid and sample axis of nT_g
  * samples) and group_B[g] sinograms: data[g] [B_g][D][nT_g], rec[g] [B_g][P_g],
- * init[g] (may be NULL, or the whole array NULL) [B_g][P_g] -- device buffers
+ * init[g] / init_dual[g] (may be NULL, or the whole array NULL) [B_g][P_g]: warm
+ * start of the outer iterate and of the scaled dual -- device buffers
  * owned by the caller, whose addresses are captured.  windows [sum B][2]:
  * (first sample, sample count) of every sinogram on its group's axis -- a
  * sinogram cropped at its start and/or end; samples outside are ignored.
@@ -234,7 +237,8 @@ int mbrec_upsample_detectors_dev(const double *U, int Dup, int D, const double *
 typedef struct mbrec_batch_s *mbrec_batch;
 int mbrec_batch_create(mbrec_batch *batch, int ngroups, const mbrec_plan *plans, const int *group_B,
                        const float *const *data, float *const *rec, const float *const *init,
-                       const int32_t *windows, const double *lambdaL2, int num_iter);
+                       const float *const *init_dual, const int32_t *windows, const double *lambdaL2,
+                       int num_iter);
 int mbrec_batch_set_lambda(mbrec_batch batch, const double *lambdaL2);
 int mbrec_batch_run(mbrec_batch batch, int use_init, int use_graph, mbrec_stream s);
 int mbrec_batch_destroy(mbrec_batch batch);

@@ -667,6 +667,7 @@ __global__ __launch_bounds__(256) void k_fwd_pull_bl(const int2 *__restrict__ re
 // ---------------------------------------------------------------------------
 static inline unsigned nblocks(size_t n, int t = 256) { return (unsigned)((n + t - 1) / t); }
 constexpr size_t kLdsBudget = 160 * 1024;   // LDS per workgroup on gfx950
+constexpr size_t kHatLds = 32 * 1024;       // LDS per workgroup of the hat-filter kernels
 
 static int scatter_chunk_max(const Plan *p, int Bp)
 {
@@ -813,7 +814,8 @@ static int launch_bins_to_y(Plan *p, float *y, int B, int Bp, float scale, hipSt
                             const int2 *win = nullptr)
 {
     const int nj = 3 * (kYTile + p->taps.nw) + 3;
-    const int Bc = std::max(1, std::min(B, (int)((96 * 1024) / (sizeof(float) * (size_t)(nj | 1)))));
+    // LDS chunk of sinograms: at most kHatLds bytes, so several workgroups share a CU
+    const int Bc = std::max(1, std::min(B, (int)(kHatLds / (sizeof(float) * (size_t)(nj | 1)))));
     dim3 grid((p->nT + kYTile - 1) / kYTile, p->D, (B + Bc - 1) / Bc);
     if (p->pull)
         hipLaunchKernelGGL(k_bins_to_y<true>, grid, dim3(256), sizeof(float) * (size_t)Bc * (nj | 1), s, p->d_bins, y,
@@ -841,7 +843,7 @@ static int launch_fir(Plan *p, const float *x, float *out, int B, bool corr, flo
 static int launch_z_to_bins(Plan *p, const float *z, int B, int Bp, hipStream_t s, const int2 *win = nullptr)
 {
     const int nz = kBinTile / 3 + p->taps.nw + 2;
-    const int Bc = std::max(1, std::min(B, (int)((96 * 1024) / (sizeof(float) * (size_t)(nz | 1)))));
+    const int Bc = std::max(1, std::min(B, (int)(kHatLds / (sizeof(float) * (size_t)(nz | 1)))));
     dim3 grid((p->geom.NB + kBinTile - 1) / kBinTile, p->D, (B + Bc - 1) / Bc);
     hipLaunchKernelGGL(k_z_to_bins, grid, dim3(256), sizeof(float) * (size_t)Bc * (nz | 1), s, z, p->d_bins, p->taps,
                        p->geom, p->D, B, Bp, Bc, win);

@@ -462,7 +462,7 @@ static int l2_prepare(Plan *p, L2Solve &c, int B, int coupled, int num_iter, dou
 // Launch the whole solve on stream s (kernels only: no allocation, copy or
 // synchronisation, so the sequence can be captured into a graph).
 static int l2_enqueue(Plan *p, L2Solve &c, const float *y, const float *init, float *rec, const int2 *win,
-                      hipStream_t s)
+                      hipStream_t s, const float *init_dual = nullptr)
 {
     const size_t np = (size_t)p->P;
     Workspace &ws = c.ws;
@@ -470,9 +470,9 @@ static int l2_enqueue(Plan *p, L2Solve &c, const float *y, const float *init, fl
     int rc;
     ws.w = rec;   // the outer iterate lives in the output buffer
     const int nb = 1024;
-    // w0 = init (warm start) or 0 (nnls_L2_less.m:28); v0 = 0 (:29)
+    // w0 = init (warm start) or 0 (nnls_L2_less.m:28); v0 = init_dual (warm start) or 0 (:29)
     hipLaunchKernelGGL(k_copy_or_zero, dim3(nb), dim3(256), 0, s, init, ws.w, np * B);
-    hipLaunchKernelGGL(k_copy_or_zero, dim3(nb), dim3(256), 0, s, (const float *)nullptr, ws.vm, np * B);
+    hipLaunchKernelGGL(k_copy_or_zero, dim3(nb), dim3(256), 0, s, init_dual, ws.vm, np * B);
     for (int it = 0; it < c.num_iter; ++it) {
         if ((rc = lsqr_solve(p, ws, y, B, c.G, 2, c.k, s, win))) return rc;
         hipLaunchKernelGGL(k_nnls_update, vgrid(B), dim3(256), 0, s, ws.x, ws.w, ws.vm, np, B, c.k, ws.sc);
@@ -483,13 +483,14 @@ static int l2_enqueue(Plan *p, L2Solve &c, const float *y, const float *init, fl
 
 // lambda_b (host, [B], may be null): per-sinogram lambdaL2 of independent solves (coupled = 0)
 int nnls_l2(Plan *p, const float *y, int B, int coupled, int num_iter, double lambdaL2,
-            const float *init, float *rec, double *lcurve, hipStream_t s, const double *lambda_b = nullptr)
+            const float *init, float *rec, double *lcurve, hipStream_t s, const double *lambda_b = nullptr,
+            const float *init_dual = nullptr)
 {
     const size_t np = (size_t)p->P, ns = (size_t)p->D * p->nT;
     L2Solve c;
     int rc;
     if ((rc = l2_prepare(p, c, B, coupled, num_iter, lambdaL2, lambda_b, s))) return rc;
-    if ((rc = l2_enqueue(p, c, y, init, rec, nullptr, s))) return rc;
+    if ((rc = l2_enqueue(p, c, y, init, rec, nullptr, s, init_dual))) return rc;
     Workspace &ws = c.ws;
     const int G = c.G;
     if (lcurve) {
@@ -526,12 +527,12 @@ extern "C" int mbrec_rec_nn_with_L2_reg_dev(mbrec_plan h, const float *data, int
 }
 
 extern "C" int mbrec_rec_nn_with_L2_reg_batch_dev(mbrec_plan h, const float *data, int B, int num_iter,
-                                                  const double *lambdaL2, const float *init, float *rec,
-                                                  double *lcurve, mbrec_stream s)
+                                                  const double *lambdaL2, const float *init, const float *init_dual,
+                                                  float *rec, double *lcurve, mbrec_stream s)
 {
     if (!h || !data || !rec || !lambdaL2 || B <= 0 || num_iter < 0)
         return set_error(MBREC_ERR_ARG, "mbrec:args", "rec_nn_with_L2_reg_batch: invalid arguments");
-    return nnls_l2(&h->p, data, B, 0, num_iter, 0.0, init, rec, lcurve, (hipStream_t)s, lambdaL2);
+    return nnls_l2(&h->p, data, B, 0, num_iter, 0.0, init, rec, lcurve, (hipStream_t)s, lambdaL2, init_dual);
 }
 
 // ---------------------------------------------------------------------------
@@ -564,6 +565,7 @@ struct BatchGroup {
     const float *data = nullptr;
     float *rec = nullptr;
     const float *init = nullptr;
+    const float *init_dual = nullptr;
     float *ys = nullptr;       // masked copy of the data
     int2 *win = nullptr;       // [B] sample windows
     L2Solve solve;
@@ -606,7 +608,8 @@ static int batch_enqueue(Batch *b, int use_init, hipStream_t s)
         MBREC_HIP(hipStreamWaitEvent(g.st, b->fork, 0));
         const size_t n = (size_t)g.B * g.p->D * g.p->nT;
         hipLaunchKernelGGL(k_mask_sino, dim3(1024), dim3(256), 0, g.st, g.data, g.ys, g.win, g.p->D, g.p->nT, n);
-        int rc = l2_enqueue(g.p, g.solve, g.ys, use_init ? g.init : nullptr, g.rec, g.win, g.st);
+        int rc = l2_enqueue(g.p, g.solve, g.ys, use_init ? g.init : nullptr, g.rec, g.win, g.st,
+                            use_init ? g.init_dual : nullptr);
         if (rc) return rc;
         MBREC_HIP(hipEventRecord(g.done, g.st));
         MBREC_HIP(hipStreamWaitEvent(s, g.done, 0));
@@ -631,7 +634,8 @@ extern "C" int mbrec_batch_destroy(mbrec_batch h)
 
 extern "C" int mbrec_batch_create(mbrec_batch *out, int ngroups, const mbrec_plan *plans, const int *group_B,
                                   const float *const *data, float *const *rec, const float *const *init,
-                                  const int32_t *windows, const double *lambdaL2, int num_iter)
+                                  const float *const *init_dual, const int32_t *windows, const double *lambdaL2,
+                                  int num_iter)
 {
     if (!out || ngroups <= 0 || !plans || !group_B || !data || !rec || !windows || !lambdaL2 || num_iter < 0)
         return set_error(MBREC_ERR_ARG, "mbrec:args", "batch_create: invalid arguments");
@@ -663,6 +667,7 @@ extern "C" int mbrec_batch_create(mbrec_batch *out, int ngroups, const mbrec_pla
         g.data = data[i];
         g.rec = rec[i];
         g.init = init ? init[i] : nullptr;
+        g.init_dual = init_dual ? init_dual[i] : nullptr;
         std::vector<int2> w(g.B);
         for (int k = 0; k < g.B; ++k) {
             const long long first = windows[2 * (off + k)], count = windows[2 * (off + k) + 1];

@@ -60,7 +60,7 @@ SIGNATURES = [
      [c_void_p, c_void_p, C.c_int, C.c_int, C.c_int, C.c_double, c_void_p, c_void_p, c_void_p,
       c_void_p]),
     ("mbrec_rec_nn_with_L2_reg_batch_dev", C.c_int,
-     [c_void_p, c_void_p, C.c_int, C.c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+     [c_void_p, c_void_p, C.c_int, C.c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     ("mbrec_rec_nn_with_L1_eye_reg_dev", C.c_int,
      [c_void_p, c_void_p, C.c_int, C.c_int, C.c_int, C.c_double, c_void_p, c_void_p]),
     ("mbrec_rec_nn_with_TV_reg_dev", C.c_int,
@@ -76,7 +76,7 @@ SIGNATURES = [
      [c_void_p, C.c_int, C.c_int, c_void_p, C.c_int, C.c_int, c_void_p, c_void_p]),
     ("mbrec_batch_create", C.c_int,
      [C.POINTER(c_void_p), C.c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
-      C.c_int]),
+      c_void_p, C.c_int]),
     ("mbrec_batch_set_lambda", C.c_int, [c_void_p, c_void_p]),
     ("mbrec_batch_run", C.c_int, [c_void_p, C.c_int, C.c_int, c_void_p]),
     ("mbrec_batch_destroy", C.c_int, [c_void_p]),

@@ -19,15 +19,16 @@ from . import _lib
 
 class BatchSolver:
     """groups: [(model, windows)] with windows a list of (first_sample, count)
-    per sinogram on that model's sample axis.  ``data[g]`` (B_g, D, nT_g) and
-    ``init[g]`` (B_g, Nx, Nz) are device tensors to fill before ``run``;
-    ``rec[g]`` receives the reconstructions."""
+    per sinogram on that model's sample axis.  ``data[g]`` (B_g, D, nT_g) and,
+    with_init=True, ``init[g]`` / ``init_dual[g]`` (B_g, Nx, Nz: warm start of the
+    outer iterate and of the scaled dual) are device tensors to fill before
+    ``run``; ``rec[g]`` receives the reconstructions."""
 
     def __init__(self, groups, num_iter, lambdaL2, with_init=False):
         import torch
         plans, Bs, wins = [], [], []
         self.models = [m for m, _ in groups]
-        self.data, self.rec, self.init = [], [], []
+        self.data, self.rec, self.init, self.init_dual = [], [], [], []
         for m, w in groups:
             w = np.asarray(w, dtype=np.int32).reshape(-1, 2)
             D, nT = m.shape_sinogram
@@ -39,6 +40,7 @@ class BatchSolver:
             self.rec.append(torch.zeros((B, *m.shape_image), dtype=torch.float32, device="cuda"))
             if with_init:
                 self.init.append(torch.zeros((B, *m.shape_image), dtype=torch.float32, device="cuda"))
+                self.init_dual.append(torch.zeros((B, *m.shape_image), dtype=torch.float32, device="cuda"))
         self.windows = np.ascontiguousarray(np.concatenate(wins), dtype=np.int32)
         n = self.windows.shape[0]
         lam = np.ascontiguousarray(np.broadcast_to(np.asarray(lambdaL2, dtype=np.float64), (n,)))
@@ -48,9 +50,10 @@ class BatchSolver:
         Dp = (C.c_void_p * G)(*[t.data_ptr() for t in self.data])
         Rp = (C.c_void_p * G)(*[t.data_ptr() for t in self.rec])
         Ip = (C.c_void_p * G)(*[t.data_ptr() for t in self.init]) if with_init else None
+        Vp = (C.c_void_p * G)(*[t.data_ptr() for t in self.init_dual]) if with_init else None
         h = C.c_void_p()
         torch.cuda.synchronize()
-        _lib.check(_lib.lib().mbrec_batch_create(C.byref(h), G, P, BB, Dp, Rp, Ip, self.windows.ctypes.data,
+        _lib.check(_lib.lib().mbrec_batch_create(C.byref(h), G, P, BB, Dp, Rp, Ip, Vp, self.windows.ctypes.data,
                                                  lam.ctypes.data, int(num_iter)))
         self.h = h
         self.sizes = Bs

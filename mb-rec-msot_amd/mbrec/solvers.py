@@ -84,24 +84,30 @@ def rec_nn_with_L2_reg(model, data_filt, num_iter_nn_outer, lambdaL2=0.0, RegL2=
     return rec, errimg, errreg, 0.0, err, t
 
 
-def rec_nn_with_L2_reg_batch(model, data_filt, num_iter_nn_outer, lambdaL2, init=None):
+def rec_nn_with_L2_reg_batch(model, data_filt, num_iter_nn_outer, lambdaL2, init=None, init_dual=None):
     """B independent rec_nn_with_L2_reg.m solves (RegL2 = identity), each with its own
     lambdaL2 (array [B]), in one batched device call.  data_filt (B, D, nT) device
-    tensor; returns the (B, Nx, Nz) device tensor of reconstructions."""
+    tensor; init / init_dual (optional (B, Nx, Nz)) seed the outer iterate w and the
+    scaled dual v of nnls_L2_less.m:28-29.  Returns the (B, Nx, Nz) device tensor."""
     import torch
     y, _ = _to_device(data_filt)
     D, nT = model.shape_sinogram
     B = y.numel() // (D * nT)
     lam = np.ascontiguousarray(np.broadcast_to(np.asarray(lambdaL2, dtype=np.float64), (B,)))
     rec = torch.empty((B, *model.shape_image), dtype=torch.float32, device=y.device)
-    initp = None
-    if init is not None:
-        initp, _ = _to_device(init)
-        if initp.numel() != rec.numel():
-            raise ValueError("init has the wrong number of pixels")
+    ptr = []
+    for t in (init, init_dual):
+        if t is None:
+            ptr.append(None)
+            continue
+        td, _ = _to_device(t)
+        if td.numel() != rec.numel():
+            raise ValueError("init / init_dual have the wrong number of pixels")
+        ptr.append(td)
     _lib.check(_lib.lib().mbrec_rec_nn_with_L2_reg_batch_dev(
         model.plan.h, y.data_ptr(), int(B), int(num_iter_nn_outer), lam.ctypes.data,
-        initp.data_ptr() if initp is not None else None, rec.data_ptr(), None, _lib.stream_handle()))
+        ptr[0].data_ptr() if ptr[0] is not None else None, ptr[1].data_ptr() if ptr[1] is not None else None,
+        rec.data_ptr(), None, _lib.stream_handle()))
     return rec
 
 

@@ -48,15 +48,16 @@ def test_batch(model, n, seed, noise=0.01):
 
 
 def iterations_to_tol(model, y, lam, ref, init, tol, grid):
-    """Smallest k in grid whose solve is within tol of ref for every sinogram; the
-    per-k worst relative error."""
+    """Smallest k in grid whose solve is within tol of ref for every sinogram (init =
+    None or (w0, v0)); the per-k worst relative error."""
     import torch
     from .solvers import rec_nn_with_L2_reg_batch
     B = y.shape[0]
     errs = {}
     nref = torch.linalg.vector_norm(ref.reshape(B, -1).double(), dim=1)
+    w0, v0 = init if init is not None else (None, None)
     for k in grid:
-        r = rec_nn_with_L2_reg_batch(model, y, k, lam, init=init)
+        r = rec_nn_with_L2_reg_batch(model, y, k, lam, init=w0, init_dual=v0)
         e = (torch.linalg.vector_norm((r - ref).reshape(B, -1).double(), dim=1) / nref).max().item()
         errs[k] = e
         if e <= tol:
@@ -67,11 +68,12 @@ def iterations_to_tol(model, y, lam, ref, init, tol, grid):
 def time_solve(model, y, lam, k, init=None, reps=2):
     import torch
     from .solvers import rec_nn_with_L2_reg_batch
-    rec_nn_with_L2_reg_batch(model, y, k, lam, init=init)
+    w0, v0 = init if init is not None else (None, None)
+    rec_nn_with_L2_reg_batch(model, y, k, lam, init=w0, init_dual=v0)
     torch.cuda.synchronize()
     t = time.perf_counter()
     for _ in range(reps):
-        rec_nn_with_L2_reg_batch(model, y, k, lam, init=init)
+        rec_nn_with_L2_reg_batch(model, y, k, lam, init=w0, init_dual=v0)
     torch.cuda.synchronize()
     return (time.perf_counter() - t) / reps
 
@@ -90,6 +92,7 @@ def main(argv=None):
     ap.add_argument("--tol", type=float, default=1e-3)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--report", default=None)
+    ap.add_argument("--grid-iters", default="10,20,30,50,75,100,150,200,300,400,600,800,1200,1600")
     a = ap.parse_args(argv)
     import torch
     from .warmstart import UNet, make_training_set, train, warm_start_estimate
@@ -114,18 +117,20 @@ def main(argv=None):
     from .solvers import rec_nn_with_L2_reg_batch
     ref = rec_nn_with_L2_reg_batch(model, y, a.ref_iters, lam)
     t0 = time.perf_counter()
-    est = warm_start_estimate(net, model, y)
+    est = warm_start_estimate(net, model, y, lam)
     torch.cuda.synchronize()
     t_net = time.perf_counter() - t0
-    e_net = (torch.linalg.vector_norm((est.reshape(a.test, -1) - ref.reshape(a.test, -1)).double(), dim=1) /
+    e_net = (torch.linalg.vector_norm((est[0].reshape(a.test, -1) - ref.reshape(a.test, -1)).double(), dim=1) /
              torch.linalg.vector_norm(ref.reshape(a.test, -1).double(), dim=1))
-    grid = [5, 10, 15, 20, 30, 40, 50, 60, 80, 100, 125, 150, 200, 250, 300, 350]
+    grid = [int(k) for k in a.grid_iters.split(",")]
     k_cold, errs_cold = iterations_to_tol(model, y, lam, ref, None, a.tol, grid)
+    k_primal, errs_primal = iterations_to_tol(model, y, lam, ref, (est[0], None), a.tol, grid)
     k_warm, errs_warm = iterations_to_tol(model, y, lam, ref, est, a.tol, grid)
     rep = {"train": a.train, "target_iters": a.target_iters, "epochs": a.epochs, "loss": hist,
            "t_data_s": t_data, "t_train_s": t_train, "ref_iters": a.ref_iters, "tol": a.tol,
            "net_err_vs_ref_median": float(e_net.median()), "net_err_vs_ref_max": float(e_net.max()),
-           "k_cold": k_cold, "k_warm": k_warm, "errs_cold": errs_cold, "errs_warm": errs_warm,
+           "k_cold": k_cold, "k_warm": k_warm, "k_primal_only": k_primal, "errs_cold": errs_cold,
+           "errs_warm": errs_warm, "errs_primal_only": errs_primal,
            "t_net_s": t_net}
     if k_cold and k_warm:
         tc = time_solve(model, y, lam, k_cold)

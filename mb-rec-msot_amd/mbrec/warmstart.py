@@ -185,11 +185,25 @@ def train(net, inputs, targets, epochs=10, batch=8, lr=1e-3, seed=0, log=None):
     return hist
 
 
-def warm_start_estimate(net, model, y):
-    """Network estimate from sinograms y (B, D, nT): max(net(A^T y), 0) as (B, P)."""
+BETA = 1e-3   # nnls_L2_less.m:34
+
+
+def warm_start_estimate(net, model, y, lambdaL2):
+    """Warm start (w0, v0) for nnls_L2_less from sinograms y (B, D, nT).
+
+    w0 = max(net(A'y), 0).  v0 is the scaled dual that makes w0 a fixed point of
+    the outer iteration if w0 were the solution: with g = A'(A w0 - y) + lambdaL2 w0
+    (the gradient of the subproblem's data + L2 term, halved), the u-step optimality
+    at u = w = x gives v = g / beta, and the w-step requires v = 0 where x > 0 and
+    v >= 0 where x = 0; so v0 = max(g, 0) / beta on the zero set of w0, 0 elsewhere.
+    Returns (w0, v0) as (B, Nx, Nz) device tensors."""
     import torch
     B = y.shape[0]
-    Nx, Nz = model.shape_image[-2:]
+    Nx, Nz = model.shape_image
     aty = model.applyTranspose(y).reshape(B, Nx, Nz)
-    est = net.forward(aty)
-    return torch.clamp(est, min=0.0).reshape(B, -1)
+    w0 = torch.clamp(net.forward(aty), min=0.0)
+    lam = torch.as_tensor(np.asarray(lambdaL2, dtype=np.float32), device=w0.device).reshape(-1, 1, 1)
+    r = model.applyForward(w0) - y
+    g = model.applyTranspose(r).reshape(B, Nx, Nz) + lam * w0
+    v0 = torch.where(w0 > 0, torch.zeros_like(g), torch.clamp(g, min=0.0) / BETA)
+    return w0.contiguous(), v0.contiguous()
