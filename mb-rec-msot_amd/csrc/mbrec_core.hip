@@ -298,26 +298,73 @@ static int make_geometry(Plan *p, double maxabs_tau)
 }
 
 // Device pixel order: TZ x TX tiles, tiles column-major, lz fastest in a tile.
+// Device pixel order: 16 x 16 tiles of 256 slots (tile-major, z fastest), each
+// made of four 8 x 8 sub-tiles of 64 slots, so every aligned run of 64 slots is
+// an 8 x 8 square and every run of 256 a 16 x 16 square.  Single-column grids
+// (Nx = 1) use runs of 256 consecutive pixels.  Slots beyond the grid map to -1.
 static void make_pixel_order(Plan *p, std::vector<int32_t> &slot_pix)
 {
-    if (p->Nx == 1) { p->TZ = 64; p->TX = 1; } else { p->TZ = 8; p->TX = 8; }
-    const int ntz = (p->Nz + p->TZ - 1) / p->TZ, ntx = (p->Nx + p->TX - 1) / p->TX;
-    p->Pd = ntz * ntx * p->TZ * p->TX;
+    if (p->Nx == 1) {
+        p->TZ = 256;
+        p->TX = 1;
+        p->Pd = (p->Nz + 255) / 256 * 256;
+        slot_pix.assign(p->Pd, -1);
+        for (int iz = 0; iz < p->Nz; ++iz) slot_pix[iz] = iz;
+        return;
+    }
+    p->TZ = 16;
+    p->TX = 16;
+    const int ntz = (p->Nz + 15) / 16, ntx = (p->Nx + 15) / 16;
+    p->Pd = ntz * ntx * 256;
     slot_pix.assign(p->Pd, -1);
     int slot = 0;
     for (int tx = 0; tx < ntx; ++tx)
         for (int tz = 0; tz < ntz; ++tz)
-            for (int lx = 0; lx < p->TX; ++lx)
-                for (int lz = 0; lz < p->TZ; ++lz, ++slot) {
-                    const int iz = tz * p->TZ + lz, ix = tx * p->TX + lx;
-                    if (iz < p->Nz && ix < p->Nx) slot_pix[slot] = iz + p->Nz * ix;
-                }
+            for (int sx = 0; sx < 2; ++sx)
+                for (int sz = 0; sz < 2; ++sz)
+                    for (int lx = 0; lx < 8; ++lx)
+                        for (int lz = 0; lz < 8; ++lz, ++slot) {
+                            const int iz = tz * 16 + sz * 8 + lz, ix = tx * 16 + sx * 8 + lx;
+                            if (iz < p->Nz && ix < p->Nx) slot_pix[slot] = iz + p->Nz * ix;
+                        }
+}
+
+// Bin window of every (256-slot tile, detector): first bin and length covering
+// bins j, j + 1 of all its pairs (length 0: no pair in range); kTileWinMax bounds
+// what the tile gather stages in LDS.
+static int build_tile_windows(Plan *p, const std::vector<int32_t> &tq)
+{
+    const BinGeom &g = p->geom;
+    const int span = g.nT + 2 * g.K;
+    const int ntile = p->Pd / 256;
+    std::vector<int2> w((size_t)ntile * p->D);
+    int wmax = 0;
+    for (int t = 0; t < ntile; ++t)
+        for (int d = 0; d < p->D; ++d) {
+            int lo = INT32_MAX, hi = INT32_MIN;
+            const int32_t *q = tq.data() + (size_t)d * p->Pd + (size_t)t * 256;
+            for (int k = 0; k < 256; ++k) {
+                int nn, m;
+                host_decode(q[k], g, nn, m);
+                if ((unsigned)nn >= (unsigned)span) continue;
+                const int j = 3 * nn + m;
+                lo = std::min(lo, j);
+                hi = std::max(hi, j + 1);
+            }
+            w[(size_t)t * p->D + d] = lo <= hi ? make_int2(lo, hi - lo + 1) : make_int2(0, 0);
+            if (lo <= hi) wmax = std::max(wmax, hi - lo + 1);
+        }
+    p->tile_wmax = wmax;
+    int rc;
+    if ((rc = dalloc(&p->d_twin, w.size(), &p->table_bytes)) || (rc = upload(p->d_twin, w.data(), w.size())))
+        return rc;
+    return MBREC_OK;
 }
 
 static void free_plan(Plan *p)
 {
     void *ptrs[] = {p->d_pix, p->d_tq, p->d_sidx, p->d_sir, p->d_swin, p->d_ir, p->d_irf,
-                    p->d_kz,  p->d_kx, p->d_rec, p->d_rec_off, p->d_rec_k};
+                    p->d_kz,  p->d_kx, p->d_rec, p->d_rec_off, p->d_rec_k, p->d_twin};
     for (void *q : ptrs)
         if (q) (void)hipFree(q);
     free_work(p);
@@ -383,7 +430,7 @@ int mbrec_plan_create(mbrec_plan *out, int Nz, int Nx, int D, int nT, double R,
     if ((rc = dalloc(&p->d_pix, p->Pd, &p->table_bytes)) ||
         (rc = dalloc(&p->d_tq, (size_t)D * p->Pd, &p->table_bytes)) ||
         (rc = upload(p->d_pix, slot_pix.data(), slot_pix.size())) ||
-        (rc = upload(p->d_tq, tq.data(), tq.size())) ||
+        (rc = upload(p->d_tq, tq.data(), tq.size())) || (rc = build_tile_windows(p, tq)) ||
         (rc = build_records(p, slot_pix, tq, nullptr, nullptr, 0))) {
         free_plan(p);
         delete h;

@@ -54,10 +54,12 @@ struct TuneChoice {
 struct Plan {
     int Nz = 0, Nx = 0, P = 0, D = 0, nT = 0;
     double R = 0.0;
-    int TZ = 8, TX = 8;          // device pixel tile
+    int TZ = 16, TX = 16;        // device pixel tile (256 slots; 64-slot runs are 8 x 8 squares)
     int Pd = 0;                  // device pixel slots (tile-padded)
     int32_t *d_pix = nullptr;    // [Pd] slot -> pixel index or -1
     int32_t *d_tq = nullptr;     // [D][Pd] fixed-point travel times (transpose gather)
+    int2 *d_twin = nullptr;      // [Pd / 256][D] bin window (first bin, length) of each 256-slot tile
+    int tile_wmax = 0;           // longest tile window
     BinGeom geom{};
     HatTaps taps{};
     // forward scatter: per detector, (pixel, travel time) records grouped into buckets

@@ -579,16 +579,32 @@ __global__ __launch_bounds__(256) void k_fwd_pull(const int2 *__restrict__ rec, 
     float s0[BC], s1[BC], c[BC];
 #pragma unroll
     for (int q = 0; q < BC; ++q) { s0[q] = 0.0f; s1[q] = 0.0f; }
+    (void)c;
     if (j < NB) {
         const int re = o[j + 1];
-        for (int r = o[j]; r < re; ++r) {
-            const int2 e = rec[r];
-            const float u = __int_as_float(e.y);
-            load_c<BC>(imgI + (size_t)e.x * Bp + b0, c);
+        constexpr int U = BC <= 4 ? 4 : (BC <= 8 ? 2 : 1);   // pairs in flight per thread
+        for (int r = o[j]; r < re; r += U) {
+            int2 e[U];
+            float cu[U][BC];
 #pragma unroll
-            for (int q = 0; q < BC; ++q) {
-                s0[q] = fmaf(1.0f - u, c[q], s0[q]);
-                s1[q] = fmaf(u, c[q], s1[q]);
+            for (int u = 0; u < U; ++u) e[u] = r + u < re ? rec[r + u] : make_int2(0, 0);
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                if (r + u < re) {
+                    load_c<BC>(imgI + (size_t)e[u].x * Bp + b0, cu[u]);
+                } else {
+#pragma unroll
+                    for (int q = 0; q < BC; ++q) cu[u][q] = 0.0f;
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const float uu = r + u < re ? __int_as_float(e[u].y) : 0.0f;
+#pragma unroll
+                for (int q = 0; q < BC; ++q) {
+                    s0[q] = fmaf(1.0f - uu, cu[u][q], s0[q]);
+                    s1[q] = fmaf(uu, cu[u][q], s1[q]);
+                }
             }
         }
     }
@@ -620,8 +636,10 @@ __global__ __launch_bounds__(256) void k_fwd_pull(const int2 *__restrict__ rec, 
 }
 
 // batch-lane form: lane l owns sinogram b0 + l % LB; each lane group walks JR
-// consecutive bins of one detector, carrying the u-part into the next bin.
-template <int LB>
+// consecutive bins of one detector, carrying the u-part into the next bin.  The
+// pairs are consumed in runs of U whose records and image rows are all loaded
+// before any is accumulated, so U loads are in flight per lane.
+template <int LB, int U>
 __global__ __launch_bounds__(256) void k_fwd_pull_bl(const int2 *__restrict__ rec, const int32_t *__restrict__ off,
                                                      const float *__restrict__ imgI, float *__restrict__ bins, int NB,
                                                      int Bp, int JR)
@@ -637,6 +655,7 @@ __global__ __launch_bounds__(256) void k_fwd_pull_bl(const int2 *__restrict__ re
     const bool bv = b < Bp;
     const int32_t *o = off + (size_t)d * (NB + 1);
     const float *cb = imgI + b;
+    float *dst = bins + (size_t)d * NB * Bp + b;
     float prev = 0.0f;
     if (ja > 0) {
         const int re = o[ja];
@@ -647,18 +666,99 @@ __global__ __launch_bounds__(256) void k_fwd_pull_bl(const int2 *__restrict__ re
         }
     }
     int r = o[ja];
-    for (int j = ja; j < jb; ++j) {
-        const int re = o[j + 1];
-        float s0 = 0.0f, s1 = 0.0f;
-        for (; r < re; ++r) {
-            const int2 e = rec[r];
-            const float u = __int_as_float(e.y);
-            const float cv = bv ? cb[(size_t)e.x * Bp] : 0.0f;
-            s0 = fmaf(1.0f - u, cv, s0);
-            s1 = fmaf(u, cv, s1);
+    const int rend = o[jb];
+    int j = ja;
+    int jend = o[ja + 1];   // first record past bin j
+    float s0 = 0.0f, s1 = 0.0f;
+    while (r < rend) {
+        int2 e[U];
+        float cv[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) e[u] = r + u < rend ? rec[r + u] : make_int2(0, 0);
+#pragma unroll
+        for (int u = 0; u < U; ++u) cv[u] = (bv && r + u < rend) ? cb[(size_t)e[u].x * Bp] : 0.0f;
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            if (r + u >= rend) break;
+            while (r + u >= jend) {   // bin j is complete
+                if (bv) dst[(size_t)j * Bp] = s0 + prev;
+                prev = s1;
+                s0 = s1 = 0.0f;
+                ++j;
+                jend = o[j + 1];
+            }
+            const float uu = __int_as_float(e[u].y);
+            s0 = fmaf(1.0f - uu, cv[u], s0);
+            s1 = fmaf(uu, cv[u], s1);
         }
-        if (bv) bins[((size_t)d * NB + j) * Bp + b] = s0 + prev;
+        r += U;
+    }
+    for (; j < jb; ++j) {   // the last bin with pairs and any empty bins after it
+        if (bv) dst[(size_t)j * Bp] = s0 + prev;
         prev = s1;
+        s0 = s1 = 0.0f;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// tile gather (batch-lane, Bp >= 16): one workgroup per 16 x 16 pixel tile and
+// chunk of LB sinograms.  For every detector the tile's bin window (all bins its
+// pairs read, at most a few hundred) is staged in LDS with 16-byte loads, the
+// tile's travel times next to it; each wave then interpolates 64 pixels from LDS
+// (lane l: sinogram b0 + l % LB, its lane group a run of LB pixels), keeping
+// Kahan-compensated sums in registers across all detectors.  Every bin row is
+// read once per (tile, detector) instead of twice per pair.
+// ---------------------------------------------------------------------------
+template <int LB>
+__global__ __launch_bounds__(256) void k_gather_tile(
+    const int32_t *__restrict__ tq, const int2 *__restrict__ twin, const int32_t *__restrict__ pix,
+    const float *__restrict__ bins, float *__restrict__ out, BinGeom g, int Pd, int P, int D, int B, int Bp,
+    int WMAX, float scale)
+{
+    extern __shared__ float sm[];                 // [WMAX][LB] window, then [256] travel times
+    float *W = sm;
+    int32_t *T = reinterpret_cast<int32_t *>(sm + (size_t)WMAX * LB);
+    constexpr int NG = 64 / LB;                   // lane groups per wave
+    constexpr int PPG = 64 / NG;                  // pixels per lane group
+    const int tile = blockIdx.x, b0 = blockIdx.y * LB;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int bl = lane % LB, gi = lane / LB;
+    const int pbase = wv * 64 + gi * PPG;
+    const int span = g.nT + 2 * g.K;
+    const size_t rowstride = (size_t)g.NB * Bp;
+    const int c4n = LB / 4;                       // float4 per window row
+    float acc[PPG], cmp[PPG];
+#pragma unroll
+    for (int k = 0; k < PPG; ++k) { acc[k] = 0.0f; cmp[k] = 0.0f; }
+    for (int d = 0; d < D; ++d) {
+        const int2 wd = twin[(size_t)tile * D + d];
+        __syncthreads();                          // previous detector's window no longer read
+        T[threadIdx.x] = tq[(size_t)d * Pd + (size_t)tile * 256 + threadIdx.x];
+        const float *src = bins + (size_t)d * rowstride + (size_t)wd.x * Bp + b0;
+        for (int i = threadIdx.x; i < wd.y * c4n; i += blockDim.x) {
+            const int row = i / c4n, c4 = i - row * c4n;
+            float4 v = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            if (b0 + 4 * c4 < Bp) v = *reinterpret_cast<const float4 *>(src + (size_t)row * Bp + 4 * c4);
+            *reinterpret_cast<float4 *>(W + row * LB + 4 * c4) = v;
+        }
+        __syncthreads();
+        if (wd.y == 0) continue;
+#pragma unroll
+        for (int k = 0; k < PPG; ++k) {
+            const Decoded dc = decode_tau(T[pbase + k], g);
+            if ((unsigned)dc.nn < (unsigned)span) {
+                const int jl = 3 * dc.nn + dc.m - wd.x;
+                const float g0 = W[jl * LB + bl], g1 = W[(jl + 1) * LB + bl];
+                kahan_add(acc[k], cmp[k], fmaf(dc.u, g1 - g0, g0));
+            }
+        }
+    }
+    const int b = b0 + bl;
+    if (b >= B) return;
+#pragma unroll
+    for (int k = 0; k < PPG; ++k) {
+        const int px = pix[(size_t)tile * 256 + pbase + k];
+        if (px >= 0) out[(size_t)b * P + px] = scale * (acc[k] - cmp[k]);
     }
 }
 
@@ -754,7 +854,7 @@ static void fwd_pull_bl(Plan *p, const float *imgI, int Bp, hipStream_t s)
     constexpr int RPW = 64 / LB;
     const int ngrp = (p->geom.NB + kPullRange - 1) / kPullRange;
     dim3 grid((ngrp + 4 * RPW - 1) / (4 * RPW), p->D, (Bp + LB - 1) / LB);
-    hipLaunchKernelGGL(k_fwd_pull_bl<LB>, grid, dim3(256), 0, s, p->d_rec, p->d_rec_off, imgI, p->d_bins,
+    hipLaunchKernelGGL((k_fwd_pull_bl<LB, 8>), grid, dim3(256), 0, s, p->d_rec, p->d_rec_off, imgI, p->d_bins,
                        p->geom.NB, Bp, kPullRange);
 }
 
@@ -876,19 +976,39 @@ static void gather_bl(Plan *p, float *img, int B, int Bp, float scale, hipStream
                            p->d_sidx, p->d_sir, p->d_swin, p->Ls, p->geom, p->Pd, p->P, p->D, B, Bp, scale);
 }
 
-// gather variants: record-lane (lb = 0; sinograms per pass BC, detector unroll U)
-// and batch-lane (lane groups of lb sinograms, detector unroll U)
+template <int LB>
+static size_t tile_gather_lds(const Plan *p)
+{
+    return sizeof(float) * (size_t)p->tile_wmax * LB + sizeof(int32_t) * 256;
+}
+
+template <int LB>
+static void gather_tile(Plan *p, float *img, int B, int Bp, float scale, hipStream_t s)
+{
+    dim3 grid(p->Pd / 256, (Bp + LB - 1) / LB);
+    hipLaunchKernelGGL(k_gather_tile<LB>, grid, dim3(256), tile_gather_lds<LB>(p), s, p->d_tq, p->d_twin, p->d_pix,
+                       p->d_bins, img, p->geom, p->Pd, p->P, p->D, B, Bp, p->tile_wmax, scale);
+}
+
+// gather variants: record-lane (lb = 0; sinograms per pass BC, detector unroll U),
+// batch-lane (lane groups of lb sinograms, detector unroll U) and tile (u = -1:
+// LDS-staged bin windows, lane groups of lb sinograms)
 struct GatherVariant {
     int bc, u, lb;
 };
-static const GatherVariant kGatherVariants[] = {{1, 8, 0},  {1, 4, 0},  {4, 4, 0},  {4, 2, 0},
-                                                {8, 2, 0},  {8, 1, 0},  {16, 1, 0}, {0, 2, 16},
-                                                {0, 2, 32}, {0, 1, 64}, {0, 2, 64}, {0, 4, 64}};
+static const GatherVariant kGatherVariants[] = {{1, 8, 0},   {1, 4, 0},   {4, 4, 0},   {4, 2, 0},  {8, 2, 0},
+                                                {8, 1, 0},   {16, 1, 0},  {0, 2, 16},  {0, 2, 32}, {0, 1, 64},
+                                                {0, 2, 64},  {0, 4, 64},  {0, -1, 16}, {0, -1, 32}, {0, -1, 64}};
 constexpr int kNumGatherVariants = sizeof(kGatherVariants) / sizeof(kGatherVariants[0]);
 
-static bool gather_variant_ok(int v, int Bp)
+static bool gather_variant_ok(const Plan *p, int v, int Bp)
 {
     const GatherVariant &g = kGatherVariants[v];
+    if (g.u < 0) {   // tile gather: SIR-free plans, window fits LDS
+        if (p->use_sir || !p->d_twin || Bp < 16 || Bp % 4) return false;
+        const size_t lds = sizeof(float) * (size_t)p->tile_wmax * g.lb + 1024;
+        return lds <= kLdsBudget && g.lb <= std::max(16, Bp);
+    }
     if (g.lb) {
         int cap = 16;
         while (cap < Bp) cap *= 2;
@@ -897,9 +1017,11 @@ static bool gather_variant_ok(int v, int Bp)
     return g.bc == 1 ? Bp == 1 : Bp % g.bc == 0;
 }
 
-static int default_gather_variant(int Bp)
+static int default_gather_variant(const Plan *p, int Bp)
 {
     if (Bp == 1) return 0;
+    for (int v : {14, 13, 12})   // tile gather with the widest lane group that fits
+        if (gather_variant_ok(p, v, Bp) && kGatherVariants[v].lb <= Bp) return v;
     if (Bp >= 64) return 10;
     if (Bp >= 32) return 8;
     if (Bp >= 16) return 7;
@@ -909,8 +1031,8 @@ static int default_gather_variant(int Bp)
 
 static int launch_gather(Plan *p, float *img, int B, int Bp, float scale, hipStream_t s)
 {
-    int v = (p->tune.Bp == Bp && p->tune.gather >= 0) ? p->tune.gather : default_gather_variant(Bp);
-    if (!gather_variant_ok(v, Bp)) v = default_gather_variant(Bp);
+    int v = (p->tune.Bp == Bp && p->tune.gather >= 0) ? p->tune.gather : default_gather_variant(p, Bp);
+    if (!gather_variant_ok(p, v, Bp)) v = default_gather_variant(p, Bp);
     switch (v) {
     case 0: gather_bc<1, 8>(p, img, B, Bp, scale, s); break;
     case 1: gather_bc<1, 4>(p, img, B, Bp, scale, s); break;
@@ -923,7 +1045,10 @@ static int launch_gather(Plan *p, float *img, int B, int Bp, float scale, hipStr
     case 8: gather_bl<32, 2>(p, img, B, Bp, scale, s); break;
     case 9: gather_bl<64, 1>(p, img, B, Bp, scale, s); break;
     case 10: gather_bl<64, 2>(p, img, B, Bp, scale, s); break;
-    default: gather_bl<64, 4>(p, img, B, Bp, scale, s); break;
+    case 11: gather_bl<64, 4>(p, img, B, Bp, scale, s); break;
+    case 12: gather_tile<16>(p, img, B, Bp, scale, s); break;
+    case 13: gather_tile<32>(p, img, B, Bp, scale, s); break;
+    default: gather_tile<64>(p, img, B, Bp, scale, s); break;
     }
     MBREC_HIP(hipGetLastError());
     return MBREC_OK;
@@ -1058,9 +1183,9 @@ int autotune(Plan *p, int B, int reps, hipStream_t s, int *gather_out, int *scat
     p->tune.Bp = Bp;
     // gather
     float best = 3.4e38f;
-    int best_v = default_gather_variant(Bp);
+    int best_v = default_gather_variant(p, Bp);
     for (int v = 0; v < kNumGatherVariants; ++v) {
-        if (!gather_variant_ok(v, Bp)) continue;
+        if (!gather_variant_ok(p, v, Bp)) continue;
         p->tune.gather = v;
         float ms = 0.0f;
         if ((rc = timeit([&]() { return launch_gather(p, p->d_img_tmp, B, Bp, 1.0f, s); }, ms))) {
@@ -1105,7 +1230,7 @@ int autotune(Plan *p, int B, int reps, hipStream_t s, int *gather_out, int *scat
 int set_launch(Plan *p, int B, int gather, int scatter)
 {
     const int Bp = batch_stride(B);
-    if (gather >= kNumGatherVariants || (gather >= 0 && !gather_variant_ok(gather, Bp)))
+    if (gather >= kNumGatherVariants || (gather >= 0 && !gather_variant_ok(p, gather, Bp)))
         return set_error(MBREC_ERR_ARG, "mbrec:args", "gather variant %d is not valid for a batch of %d", gather, B);
     if (p->pull) {
         if (scatter != 0 && !pull_choice_ok(scatter, Bp))

@@ -142,7 +142,7 @@ __global__ __launch_bounds__(256) void k_v_update(const float *__restrict__ atu,
         for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
             float t = k.sl * atu[off + i] + k.sb * uc[off + i];
             if (k.reg) t += sr_of(k, b) * ur[off + i];
-            const float vn = t - beta * v[off + i];
+            const float vn = first ? t : t - beta * v[off + i];   // v is not yet defined when first
             v[off + i] = vn;
             vv += (double)vn * vn;
         }

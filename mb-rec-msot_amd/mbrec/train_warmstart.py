@@ -60,6 +60,8 @@ def iterations_to_tol(model, y, lam, ref, init, tol, grid):
         r = rec_nn_with_L2_reg_batch(model, y, k, lam, init=w0, init_dual=v0)
         e = (torch.linalg.vector_norm((r - ref).reshape(B, -1).double(), dim=1) / nref).max().item()
         errs[k] = e
+        log(f"  {'cold' if init is None else ('warm' if w0 is not None and v0 is not None else 'primal')} "
+            f"{k} iterations: worst rel. error {e:.3e}")
         if e <= tol:
             return k, errs
     return None, errs
@@ -115,7 +117,11 @@ def main(argv=None):
     # evaluation on a held-out batch
     y, lam, ph = test_batch(model, a.test, 90_000 + a.seed)
     from .solvers import rec_nn_with_L2_reg_batch
+    log(f"reference: {a.ref_iters} iterations of the plain solve ...")
+    t0 = time.perf_counter()
     ref = rec_nn_with_L2_reg_batch(model, y, a.ref_iters, lam)
+    torch.cuda.synchronize()
+    log(f"reference done in {time.perf_counter() - t0:.1f} s")
     t0 = time.perf_counter()
     est = warm_start_estimate(net, model, y, lam)
     torch.cuda.synchronize()
