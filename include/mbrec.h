@@ -120,10 +120,13 @@ int mbrec_plan_tau_frac_bits(mbrec_plan plan);
  * choice beyond FP32 summation order. */
 int mbrec_plan_autotune(mbrec_plan plan, int B, int reps, mbrec_stream s, int *choice);
 /* Set the launch choice for batches of B by hand: gather variant in
- * [0, mbrec_num_gather_variants()) or -1 (automatic); scatter choice > 0: the
- * record-lane kernel with that LDS batch chunk (a multiple of 4), < 0: the
- * batch-lane kernel with lane groups of -value (16, 32 or 64) sinograms, 0:
- * automatic.  Invalid choices are rejected (mbrec:args). */
+ * [0, mbrec_num_gather_variants()) or -1 (automatic); forward choice 0:
+ * automatic.  Plans with SIR (scatter form): > 0 the record-lane kernel with that
+ * LDS batch chunk (a multiple of 4), < 0 the batch-lane kernel with lane groups
+ * of -value (16, 32 or 64) sinograms.  Plans without SIR (pull form): 1, 4, 8,
+ * 16 bins per thread pass of that many sinograms; -(LB + 100 mode) batch-lane
+ * with lane groups LB and mode 0 (plain order), 1 (XCD-aware order) or 2
+ * (XCD-aware, 16 pairs in flight).  Invalid choices are rejected (mbrec:args). */
 int mbrec_plan_set_launch(mbrec_plan plan, int B, int gather_variant, int scatter_choice);
 int mbrec_num_gather_variants(void);
 

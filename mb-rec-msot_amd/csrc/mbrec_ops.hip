@@ -642,13 +642,26 @@ __global__ __launch_bounds__(256) void k_fwd_pull(const int2 *__restrict__ rec, 
 template <int LB, int U>
 __global__ __launch_bounds__(256) void k_fwd_pull_bl(const int2 *__restrict__ rec, const int32_t *__restrict__ off,
                                                      const float *__restrict__ imgI, float *__restrict__ bins, int NB,
-                                                     int Bp, int JR)
+                                                     int Bp, int JR, int xcd_map /* D / 8, or 0 */)
 {
     constexpr int RPW = 64 / LB;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int bl = lane % LB, rq = lane / LB;
-    const int d = blockIdx.y;
-    const int ja = ((blockIdx.x * 4 + wv) * RPW + rq) * JR;
+    // XCD-aware order: workgroups are dealt round-robin to the 8 XCDs, so XCD x
+    // (= blockIdx.x % 8) is given the detectors [x * D/8, (x + 1) * D/8), walked
+    // detector-fastest: its concurrent workgroups read the same arc of image rows
+    // (neighbouring detectors, same bins) out of that XCD's L2.
+    int d, jblk;
+    if (xcd_map) {
+        const int dpx = xcd_map;                   // detectors per XCD
+        const int x = blockIdx.x % 8, sidx = blockIdx.x / 8;
+        d = x * dpx + sidx % dpx;
+        jblk = sidx / dpx;
+    } else {
+        d = blockIdx.y;
+        jblk = blockIdx.x;
+    }
+    const int ja = ((jblk * 4 + wv) * RPW + rq) * JR;
     if (ja >= NB) return;   // no barriers in this kernel
     const int jb = min(NB, ja + JR);
     const int b = blockIdx.z * LB + bl;
@@ -762,6 +775,82 @@ __global__ __launch_bounds__(256) void k_gather_tile(
     }
 }
 
+// Pipelined tile gather: while the waves interpolate detector d from LDS, every
+// thread already holds its share of detector d+1's bin window (up to PF 16-byte
+// loads) and travel time in registers; each pixel's (bin, weight) is decoded
+// once per detector by one thread and shared through LDS.
+template <int LB, int PF>
+__global__ __launch_bounds__(256) void k_gather_tile2(
+    const int32_t *__restrict__ tq, const int2 *__restrict__ twin, const int32_t *__restrict__ pix,
+    const float *__restrict__ bins, float *__restrict__ out, BinGeom g, int Pd, int P, int D, int B, int Bp,
+    int WMAX, float scale)
+{
+    extern __shared__ float sm[];                 // [WMAX][LB] window, [256] bin, [256] weight
+    float *W = sm;
+    int32_t *J = reinterpret_cast<int32_t *>(sm + (size_t)WMAX * LB);
+    float *Uw = sm + (size_t)WMAX * LB + 256;
+    constexpr int NG = 64 / LB, PPG = 64 / NG, C4N = LB / 4;
+    const int tile = blockIdx.x, b0 = blockIdx.y * LB;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int bl = lane % LB, gi = lane / LB;
+    const int pbase = wv * 64 + gi * PPG;
+    const int span = g.nT + 2 * g.K;
+    const size_t rowstride = (size_t)g.NB * Bp;
+    float acc[PPG], cmp[PPG];
+#pragma unroll
+    for (int k = 0; k < PPG; ++k) { acc[k] = 0.0f; cmp[k] = 0.0f; }
+    float4 pf[PF];
+    int32_t qn;
+    int2 wn;
+    auto prefetch = [&](int d) {
+        wn = twin[(size_t)tile * D + d];
+        qn = tq[(size_t)d * Pd + (size_t)tile * 256 + threadIdx.x];
+        const float *src = bins + (size_t)d * rowstride + (size_t)wn.x * Bp + b0;
+#pragma unroll
+        for (int k = 0; k < PF; ++k) {
+            const int i = threadIdx.x + k * 256;
+            const int row = i / C4N, c4 = i - row * C4N;
+            pf[k] = (row < wn.y && b0 + 4 * c4 < Bp)
+                        ? *reinterpret_cast<const float4 *>(src + (size_t)row * Bp + 4 * c4)
+                        : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        }
+    };
+    prefetch(0);
+    for (int d = 0; d < D; ++d) {
+        const int2 wd = wn;
+        __syncthreads();                          // the previous detector is no longer read
+#pragma unroll
+        for (int k = 0; k < PF; ++k) {
+            const int i = threadIdx.x + k * 256;
+            if (i < wd.y * C4N) *reinterpret_cast<float4 *>(W + 4 * i) = pf[k];
+        }
+        {
+            const Decoded dc = decode_tau(qn, g);
+            const bool ok = (unsigned)dc.nn < (unsigned)span;
+            J[threadIdx.x] = ok ? 3 * dc.nn + dc.m - wd.x : -1;
+            Uw[threadIdx.x] = dc.u;
+        }
+        __syncthreads();
+        if (d + 1 < D) prefetch(d + 1);
+#pragma unroll
+        for (int k = 0; k < PPG; ++k) {
+            const int jl = J[pbase + k];
+            if (jl >= 0) {
+                const float u = Uw[pbase + k];
+                const float g0 = W[jl * LB + bl], g1 = W[(jl + 1) * LB + bl];
+                kahan_add(acc[k], cmp[k], fmaf(u, g1 - g0, g0));
+            }
+        }
+    }
+    const int b = b0 + bl;
+    if (b >= B) return;
+#pragma unroll
+    for (int k = 0; k < PPG; ++k) {
+        const int px = pix[(size_t)tile * 256 + pbase + k];
+        if (px >= 0) out[(size_t)b * P + px] = scale * (acc[k] - cmp[k]);
+    }
+}
+
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
@@ -817,14 +906,17 @@ static int scatter_choice(const Plan *p, int Bp)
     return lb ? -lb : scatter_chunk_max(p, Bp);
 }
 
-// pull forward choice: > 0 bins per thread pass BC (1, 4, 8, 16), < 0 batch-lane LB
+// pull forward choice: > 0 bins per thread pass BC (1, 4, 8, 16); < 0 batch-lane,
+// -(LB + 100 * mode) with lane groups LB (16, 32, 64) and mode 0: plain order,
+// 8 pairs in flight; 1: XCD-aware order, 8 in flight; 2: XCD-aware, 16 in flight
 static bool pull_choice_ok(int c, int Bp)
 {
     if (c > 0) return c == 1 || ((c == 4 || c == 8 || c == 16) && Bp % c == 0);
     if (c < 0) {
+        const int lb = (-c) % 100, mode = (-c) / 100;
         int cap = 16;
         while (cap < Bp) cap *= 2;
-        return Bp >= 16 && (c == -16 || c == -32 || c == -64) && -c <= cap;
+        return Bp >= 16 && (lb == 16 || lb == 32 || lb == 64) && lb <= cap && mode <= 2;
     }
     return false;
 }
@@ -848,14 +940,23 @@ static void fwd_pull(Plan *p, const float *imgI, int Bp, hipStream_t s)
     hipLaunchKernelGGL(k_fwd_pull<BC>, grid, dim3(256), 0, s, p->d_rec, p->d_rec_off, imgI, p->d_bins, p->geom.NB, Bp);
 }
 
-template <int LB>
-static void fwd_pull_bl(Plan *p, const float *imgI, int Bp, hipStream_t s)
+// xcd: XCD-aware workgroup order (needs D % 8 == 0): a 1-D grid of (detector,
+// bin block) pairs, D / 8 detectors per XCD passed as xcd_map (0 = plain order)
+template <int LB, int U>
+static void fwd_pull_bl(Plan *p, const float *imgI, int Bp, bool xcd, hipStream_t s)
 {
     constexpr int RPW = 64 / LB;
     const int ngrp = (p->geom.NB + kPullRange - 1) / kPullRange;
-    dim3 grid((ngrp + 4 * RPW - 1) / (4 * RPW), p->D, (Bp + LB - 1) / LB);
-    hipLaunchKernelGGL((k_fwd_pull_bl<LB, 8>), grid, dim3(256), 0, s, p->d_rec, p->d_rec_off, imgI, p->d_bins,
-                       p->geom.NB, Bp, kPullRange);
+    const int nblk = (ngrp + 4 * RPW - 1) / (4 * RPW);
+    if (xcd && p->D % 8 == 0) {
+        dim3 grid(nblk * p->D, 1, (Bp + LB - 1) / LB);
+        hipLaunchKernelGGL((k_fwd_pull_bl<LB, U>), grid, dim3(256), 0, s, p->d_rec, p->d_rec_off, imgI, p->d_bins,
+                           p->geom.NB, Bp, kPullRange, p->D / 8);
+        return;
+    }
+    dim3 grid(nblk, p->D, (Bp + LB - 1) / LB);
+    hipLaunchKernelGGL((k_fwd_pull_bl<LB, U>), grid, dim3(256), 0, s, p->d_rec, p->d_rec_off, imgI, p->d_bins,
+                       p->geom.NB, Bp, kPullRange, 0);
 }
 
 static int launch_pull(Plan *p, const float *imgI, int Bp, hipStream_t s)
@@ -867,9 +968,15 @@ static int launch_pull(Plan *p, const float *imgI, int Bp, hipStream_t s)
     case 4: fwd_pull<4>(p, imgI, Bp, s); break;
     case 8: fwd_pull<8>(p, imgI, Bp, s); break;
     case 16: fwd_pull<16>(p, imgI, Bp, s); break;
-    case -16: fwd_pull_bl<16>(p, imgI, Bp, s); break;
-    case -32: fwd_pull_bl<32>(p, imgI, Bp, s); break;
-    default: fwd_pull_bl<64>(p, imgI, Bp, s); break;
+    case -16: fwd_pull_bl<16, 8>(p, imgI, Bp, false, s); break;
+    case -32: fwd_pull_bl<32, 8>(p, imgI, Bp, false, s); break;
+    case -64: fwd_pull_bl<64, 8>(p, imgI, Bp, false, s); break;
+    case -116: fwd_pull_bl<16, 8>(p, imgI, Bp, true, s); break;
+    case -132: fwd_pull_bl<32, 8>(p, imgI, Bp, true, s); break;
+    case -164: fwd_pull_bl<64, 8>(p, imgI, Bp, true, s); break;
+    case -216: fwd_pull_bl<16, 16>(p, imgI, Bp, true, s); break;
+    case -232: fwd_pull_bl<32, 16>(p, imgI, Bp, true, s); break;
+    default: fwd_pull_bl<64, 16>(p, imgI, Bp, true, s); break;
     }
     MBREC_HIP(hipGetLastError());
     return MBREC_OK;
@@ -990,6 +1097,17 @@ static void gather_tile(Plan *p, float *img, int B, int Bp, float scale, hipStre
                        p->d_bins, img, p->geom, p->Pd, p->P, p->D, B, Bp, p->tile_wmax, scale);
 }
 
+constexpr int kTilePF = 16;   // 16-byte prefetch registers per thread (pipelined tile gather)
+
+template <int LB>
+static void gather_tile2(Plan *p, float *img, int B, int Bp, float scale, hipStream_t s)
+{
+    dim3 grid(p->Pd / 256, (Bp + LB - 1) / LB);
+    const size_t lds = sizeof(float) * (size_t)p->tile_wmax * LB + 2 * sizeof(int32_t) * 256;
+    hipLaunchKernelGGL((k_gather_tile2<LB, kTilePF>), grid, dim3(256), lds, s, p->d_tq, p->d_twin, p->d_pix,
+                       p->d_bins, img, p->geom, p->Pd, p->P, p->D, B, Bp, p->tile_wmax, scale);
+}
+
 // gather variants: record-lane (lb = 0; sinograms per pass BC, detector unroll U),
 // batch-lane (lane groups of lb sinograms, detector unroll U) and tile (u = -1:
 // LDS-staged bin windows, lane groups of lb sinograms)
@@ -998,15 +1116,17 @@ struct GatherVariant {
 };
 static const GatherVariant kGatherVariants[] = {{1, 8, 0},   {1, 4, 0},   {4, 4, 0},   {4, 2, 0},  {8, 2, 0},
                                                 {8, 1, 0},   {16, 1, 0},  {0, 2, 16},  {0, 2, 32}, {0, 1, 64},
-                                                {0, 2, 64},  {0, 4, 64},  {0, -1, 16}, {0, -1, 32}, {0, -1, 64}};
+                                                {0, 2, 64},  {0, 4, 64},  {0, -1, 16}, {0, -1, 32}, {0, -1, 64},
+                                                {0, -2, 16}, {0, -2, 32}, {0, -2, 64}};
 constexpr int kNumGatherVariants = sizeof(kGatherVariants) / sizeof(kGatherVariants[0]);
 
 static bool gather_variant_ok(const Plan *p, int v, int Bp)
 {
     const GatherVariant &g = kGatherVariants[v];
-    if (g.u < 0) {   // tile gather: SIR-free plans, window fits LDS
+    if (g.u < 0) {   // tile gather: SIR-free plans, window fits LDS (and the prefetch registers)
         if (p->use_sir || !p->d_twin || Bp < 16 || Bp % 4) return false;
-        const size_t lds = sizeof(float) * (size_t)p->tile_wmax * g.lb + 1024;
+        const size_t lds = sizeof(float) * (size_t)p->tile_wmax * g.lb + 2048;
+        if (g.u == -2 && (size_t)p->tile_wmax * (g.lb / 4) > (size_t)256 * kTilePF) return false;
         return lds <= kLdsBudget && g.lb <= std::max(16, Bp);
     }
     if (g.lb) {
@@ -1020,9 +1140,7 @@ static bool gather_variant_ok(const Plan *p, int v, int Bp)
 static int default_gather_variant(const Plan *p, int Bp)
 {
     if (Bp == 1) return 0;
-    for (int v : {14, 13, 12})   // tile gather with the widest lane group that fits
-        if (gather_variant_ok(p, v, Bp) && kGatherVariants[v].lb <= Bp) return v;
-    if (Bp >= 64) return 10;
+    if (Bp >= 64) return 11;
     if (Bp >= 32) return 8;
     if (Bp >= 16) return 7;
     if (Bp % 8 == 0) return 4;
@@ -1048,7 +1166,10 @@ static int launch_gather(Plan *p, float *img, int B, int Bp, float scale, hipStr
     case 11: gather_bl<64, 4>(p, img, B, Bp, scale, s); break;
     case 12: gather_tile<16>(p, img, B, Bp, scale, s); break;
     case 13: gather_tile<32>(p, img, B, Bp, scale, s); break;
-    default: gather_tile<64>(p, img, B, Bp, scale, s); break;
+    case 14: gather_tile<64>(p, img, B, Bp, scale, s); break;
+    case 15: gather_tile2<16>(p, img, B, Bp, scale, s); break;
+    case 16: gather_tile2<32>(p, img, B, Bp, scale, s); break;
+    default: gather_tile2<64>(p, img, B, Bp, scale, s); break;
     }
     MBREC_HIP(hipGetLastError());
     return MBREC_OK;
@@ -1198,7 +1319,7 @@ int autotune(Plan *p, int B, int reps, hipStream_t s, int *gather_out, int *scat
     // scatter: record-lane LDS batch chunks, then batch-lane lane groups
     std::vector<int> cand;
     if (p->pull) {
-        for (int c : {1, 4, 8, 16, -16, -32, -64})
+        for (int c : {1, 4, 8, 16, -16, -32, -64, -116, -132, -164, -216, -232, -264})
             if (pull_choice_ok(c, Bp) && !(c == 1 && Bp > 16)) cand.push_back(c);
     } else {
         const int mx = scatter_chunk_max(p, Bp);
