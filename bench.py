@@ -28,6 +28,7 @@ sys.path[:0] = [os.path.join(ROOT, "mb-rec-msot_amd"), ROOT]
 
 METRIC = "NNLS recons/sec 400²px×256ch at rel.err≤1e-3 (learned vs plain); op HBM GB/s"
 HBM_PEAK_GBS = 8000.0
+FP32_PEAK_TFLOPS = 157.3
 
 
 def parse():
@@ -40,7 +41,12 @@ def parse():
     ap.add_argument("--grid", type=int, default=400)
     ap.add_argument("--sir", action="store_true", help="400x400 with SIR (the SIR parity configuration)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-sample-reps", type=int, default=1)
+    ap.add_argument("--cpu-sample-reps", type=int, default=100)
+    ap.add_argument("--traffic-calibration", action="store_true",
+                    help="end with a plain 256 MiB device copy (reference for the FETCH/WRITE_SIZE counters)")
+    ap.add_argument("--traffic-from", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
+                    help="no GPU: turn two rocprofv3 --pmc counter_collection.csv files of a "
+                         "--traffic-calibration run into profiles/traffic.json and exit")
     return ap.parse_args()
 
 
@@ -48,8 +54,50 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
+CALIB_BYTES = 256 * 1024 * 1024
+
+
+def traffic_from(fetch_csv, write_csv, N=400, B=64):
+    """HBM bytes per launch of the dominant kernels from rocprofv3 FETCH_SIZE / WRITE_SIZE
+    (KiB per dispatch), each scaled by what the counter reads for the 256 MiB copy."""
+    import csv
+    import statistics
+
+    def per_kernel(path):
+        acc = {}
+        for r in csv.DictReader(open(path)):
+            k = (r["Kernel_Name"], r["Dispatch_Id"])
+            acc[k] = acc.get(k, 0.0) + float(r["Counter_Value"]) * 1024.0
+        by = {}
+        for (name, _), v in acc.items():
+            by.setdefault(name, []).append(v)
+        return by
+
+    f, w = per_kernel(fetch_csv), per_kernel(write_csv)
+    copy = [n for n in f if "copyBuffer" in n]
+    cal_f = CALIB_BYTES / max(f[copy[0]]) if copy else 1.0
+    cal_w = CALIB_BYTES / max(w[copy[0]]) if copy else 1.0
+    out = {"calibration": {"copy_bytes": CALIB_BYTES, "fetch_factor": round(cal_f, 4), "write_factor": round(cal_w, 4)}}
+    # the kernel the solve launches most often of each kind (the tuned variant)
+    for kind, pat in (("scatter", "k_fwd_pull"), ("gather", "k_gather")):
+        names = [n for n in f if pat in n and n in w]
+        if not names:
+            continue
+        name = max(names, key=lambda n: len(f[n]))
+        fb, wb = statistics.median(f[name]) * cal_f, statistics.median(w[name]) * cal_w
+        out[f"das_{kind}_B{B}_N{N}"] = {"kernel": name.split("(")[0], "dispatches": len(f[name]),
+                                        "fetch_bytes": fb, "write_bytes": wb, "hbm_bytes_per_launch": fb + wb}
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    with open(path, "w") as fh:
+        json.dump(out, fh, indent=1)
+    print(json.dumps(out, indent=1))
+
+
 def main():
     a = parse()
+    if a.traffic_from:
+        traffic_from(*a.traffic_from, N=a.grid, B=a.batch)
+        return
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only; --gpus {a.gpus} is not supported in this build "
               "(batches of independent solves shard across GPUs with no exchange; see DESIGN.md)",
@@ -110,17 +158,24 @@ def main():
     log(f"{a.steps} steps: {ms_step:.1f} ms/step (wall {wall / a.steps * 1e3:.1f}), {value:.2f} recon/s")
 
     # ---- roofline of the operator kernels (HIP events on the launch stream) ----
+    # Algorithmic bytes per operator application (one geometry, B sinograms, FP32):
+    # 4 P D travel times (+ 4 P D SIR indices + referenced SIR entries) + 4 B P image
+    # + 4 B D nT sinogram.  Algorithmic flops: 2 B per tap of every pair, about 2R taps
+    # per pair (x the SIR window length with SIR).
     P = N * N
     sir_bytes = 0
+    taps = 2.0 * model.Data.circularInterpolationRadiusInSamples
     if use_sir:
         se = model.Data.SPMRTimestepIndexStartAndEnd
         ref = np.unique(model.Data.SPMRIndices) - 1
         sir_bytes = 4 * int(np.sum(np.maximum(se[ref, 1] - se[ref, 0] + 1, 0)))
+        taps *= float(np.mean(np.maximum(se[ref, 1] - se[ref, 0] + 1, 0)))
     alg_bytes = 4 * P * D * (2 if use_sir else 1) + sir_bytes + 4 * B * P + 4 * B * D * nT
+    alg_flops = 2.0 * B * P * D * taps
     reps = 10
-    st = {k: model.plan.time_stage(k, B, reps) for k in ("scatter", "bins_to_y", "fir", "z_to_bins", "gather")}
-    # launches per 50-iteration solve: 151 forward (scatter, bins_to_y, fir) and 150 transpose
-    # (fir, z_to_bins, gather) applications
+    stages = ("scatter", "bins_to_y", "fir", "z_to_bins", "gather")
+    st = {k: model.plan.time_stage(k, B, reps) for k in stages}
+    # launches per solve: 3 iters + 1 forward and 3 iters transpose applications
     n_forward, n_transpose = 3 * a.iters + 1, 3 * a.iters
     per_solve = {"scatter": n_forward * st["scatter"], "bins_to_y": n_forward * st["bins_to_y"],
                  "fir": (n_forward + n_transpose) * st["fir"], "z_to_bins": n_transpose * st["z_to_bins"],
@@ -128,22 +183,37 @@ def main():
     dom = "gather" if per_solve["gather"] >= per_solve["scatter"] else "scatter"
     t_dom = st[dom]
     achieved = alg_bytes / (t_dom * 1e-3) / 1e9
+    flops_rate = alg_flops / (t_dom * 1e-3) / 1e12
     traffic = None
     tfile = os.path.join(ROOT, "profiles", "traffic.json")
+    key = f"{'sir' if use_sir else 'das'}_{dom}_B{B}_N{N}"
     if os.path.exists(tfile):
         try:
-            tr = json.load(open(tfile))
-            key = f"{'sir' if use_sir else 'das'}_{dom}_B{B}_N{N}"
-            traffic = tr.get(key)
+            traffic = json.load(open(tfile)).get(key, {}).get("hbm_bytes_per_launch")
         except Exception:
             traffic = None
-    kname = {"gather": "k_gather", "scatter": "k_scatter_bk"}[dom]
+    kname = {"gather": "transpose gather", "scatter": "forward pairs (pull without SIR, scatter with SIR)"}[dom]
+    t_bytes, t_flops = alg_bytes / (HBM_PEAK_GBS * 1e9), alg_flops / (FP32_PEAK_TFLOPS * 1e12)
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                "kernel": f"{kname}<{'SIR' if use_sir else 'DAS'}>", "algorithmic_bytes": int(alg_bytes),
-                "kernel_ms": round(t_dom, 4), "stage_ms": {k: round(v, 4) for k, v in st.items()},
+                "kernel": f"{kname}, {'SIR' if use_sir else 'DAS'}, B={B}", "algorithmic_bytes": int(alg_bytes),
+                "kernel_ms": round(t_dom, 4),
+                "flops_term": {"algorithmic_flops": alg_flops, "achieved_tflops": round(flops_rate, 2),
+                               "peak_tflops_fp32": FP32_PEAK_TFLOPS,
+                               "frac": round(flops_rate / FP32_PEAK_TFLOPS, 4),
+                               "binding_term": "flops" if t_flops > t_bytes else "bytes",
+                               "frac_of_roofline_time": round(max(t_bytes, t_flops) / (t_dom * 1e-3), 4)},
+                "stage_ms": {k: round(v, 4) for k, v in st.items()},
                 "ms_per_solve_by_stage": {k: round(v, 2) for k, v in per_solve.items()}}
-    log("stages (ms, B=%d): %s; %s %.0f GB/s" % (B, " ".join(f"{k} {v:.3f}" for k, v in st.items()), dom, achieved))
+    # the operator pair at B = 1 (travel-time stream per application)
+    t1 = {k: model.plan.time_stage(k, 1, reps) for k in ("scatter", "gather")}
+    b1 = 4 * P * D * (2 if use_sir else 1) + sir_bytes + 4 * P + 4 * D * nT
+    roofline["operator_pair_B1"] = {k: {"ms": round(v, 4), "GB/s": round(b1 / (v * 1e-3) / 1e9, 1),
+                                        "frac": round(b1 / (v * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+                                    for k, v in t1.items()}
+    log("stages (ms, B=%d): %s; %s %.0f GB/s, %.2f TFLOP/s; B=1 %s" % (
+        B, " ".join(f"{k} {v:.3f}" for k, v in st.items()), dom, achieved, flops_rate,
+        " ".join(f"{k} {v:.3f} ms" for k, v in t1.items())))
 
     # ---- CPU baseline: FP64 oracle on a bounded sample --------------------------
     cpu = None
@@ -174,6 +244,12 @@ def main():
                          f"solve = 151 A + 150 A'"}
         log(f"cpu baseline: A {t_f:.2f} s, A' {t_a:.2f} s -> {1 / t_solve:.5f} recon/s on {O.num_threads()} threads")
 
+    if a.traffic_calibration:
+        xc = torch.empty(CALIB_BYTES // 4, dtype=torch.float32, device="cuda").fill_(1.0)
+        torch.cuda.synchronize()
+        yc = xc.clone()
+        torch.cuda.synchronize()
+        del xc, yc
     out = {"metric": METRIC, "value": round(value, 4), "unit": "recon/s", "n_gpus": 1, "steps": a.steps,
            "warmup": a.warmup, "ms_per_step": round(ms_step, 3), "higher_is_better": True, "scaling": "weak",
            "vs_baseline": None, "dtype": "f32", "data": "synthetic",

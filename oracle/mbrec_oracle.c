@@ -515,12 +515,25 @@ static void augop_apply(void *ctx, const double *in, double *out, int transp)
     }
 }
 
+/* w0 / v0 (may be NULL = zero, as at :28-29): starting outer iterate and scaled
+ * dual, the warm-start form used with a learned estimate. */
+static void nnls_L2_less_from(orc_op *model, const double *target, double lambda, double beta, int nIter,
+                              int nIter_inner, const double *w0, const double *v0, double *rec);
+
 ORC_API void orc_nnls_L2_less(orc_op *model, const double *target, double lambda,
                               double beta, int nIter, int nIter_inner, double *rec)
+{
+    nnls_L2_less_from(model, target, lambda, beta, nIter, nIter_inner, NULL, NULL, rec);
+}
+
+static void nnls_L2_less_from(orc_op *model, const double *target, double lambda, double beta, int nIter,
+                              int nIter_inner, const double *w0, const double *v0, double *rec)
 {
     const size_t m = model->m, n = model->n;
     double *w = (double *)calloc(n, sizeof(double));     /* img_out_iter, :28 */
     double *v = (double *)calloc(n, sizeof(double));     /* lagrange multiplier, :29 */
+    if (w0) memcpy(w, w0, sizeof(double) * n);
+    if (v0) memcpy(v, v0, sizeof(double) * n);
     double *u = (double *)malloc(sizeof(double) * n);
     double *baug = (double *)malloc(sizeof(double) * (m + n));
     orc_augop aug = {model, sqrt(lambda), sqrt(beta)};
@@ -543,6 +556,27 @@ ORC_API void orc_nnls_L2_less(orc_op *model, const double *target, double lambda
  * [P x W] and the L-curve data term ||A rec - y||^2 (:51) and reg term
  * ||rec||^2 (:54).  counts[0..1] receive the model fwd/adj application counts
  * (including the discarded start value A'y of :41). */
+/* rec_nn_with_L2_reg.m:31-51 for W independent sinograms (one call each, as
+ * separate calls of the reference would make), warm-started from w0 / v0
+ * ([W][P] each, may be NULL). */
+ORC_API void orc_rec_nn_with_L2_reg_from(const orc_model *mdl, const double *data, int W, int num_iter,
+                                         const double *lambdaL2, const double *w0, const double *v0,
+                                         double *rec)
+{
+    const size_t P1 = (size_t)mdl->Nz * mdl->Nx, S1 = (size_t)mdl->nT * mdl->D;
+    for (int b = 0; b < W; ++b) {
+        const int reg = lambdaL2[b] > 0.0;
+        orc_regop rop = {mdl, 1, reg ? sqrt(lambdaL2[b]) : 0.0};
+        const size_t m = S1 + (reg ? P1 : 0);
+        orc_op op = {regop_apply, &rop, m, P1, 0, 0};
+        double *target = (double *)calloc(m, sizeof(double));
+        memcpy(target, data + b * S1, sizeof(double) * S1);
+        nnls_L2_less_from(&op, target, 1.0, 1e-3, num_iter, 2, w0 ? w0 + b * P1 : NULL, v0 ? v0 + b * P1 : NULL,
+                          rec + b * P1);
+        free(target);
+    }
+}
+
 ORC_API void orc_rec_nn_with_L2_reg(const orc_model *mdl, const double *data, int W,
                                     int num_iter, double lambdaL2, double *rec,
                                     double *lcurve, int *counts)

@@ -59,6 +59,7 @@ def lib():
         L.orc_model_transpose.argtypes = [C.POINTER(OrcModel), _p, _i, _p]
         L.orc_rec_nn_with_L2_reg.argtypes = [C.POINTER(OrcModel), _p, _i, _i, _d, _p, _p, _p]
         L.orc_rec_nn_with_L1_eye_reg.argtypes = [C.POINTER(OrcModel), _p, _i, _i, _d, _p, _p]
+        L.orc_rec_nn_with_L2_reg_from.argtypes = [C.POINTER(OrcModel), _p, _i, _i, _p, _p, _p, _p]
         L.orc_num_threads.restype = _i
         _lib = L
     return _lib
@@ -329,6 +330,24 @@ class Model:
         lib().orc_rec_nn_with_L2_reg(C.byref(self.m), data.ctypes.data, W, int(num_iter), float(lambdaL2),
                                      rec.ctypes.data, lc.ctypes.data, cnt.ctypes.data)
         return rec, lc, cnt
+
+    def rec_nn_with_L2_reg_from(self, data, num_iter, lambdaL2, w0=None, v0=None):
+        """Independent solves (one per sinogram, own lambdaL2) from a warm start (w0, v0)
+        of nnls_L2_less's outer iterate and scaled dual (None = zero).  (W, Nx, Nz)."""
+        data = np.ascontiguousarray(data, dtype=np.float64)
+        W = data.size // (self.D * self.nT)
+        lam = np.ascontiguousarray(np.broadcast_to(np.asarray(lambdaL2, dtype=np.float64), (W,)))
+        rec = np.empty((W, self.Nx, self.Nz))
+        ptr = []
+        for a in (w0, v0):
+            if a is None:
+                ptr.append(None)
+            else:
+                a = self._k(a, np.float64)
+                ptr.append(a.ctypes.data)
+        lib().orc_rec_nn_with_L2_reg_from(C.byref(self.m), data.ctypes.data, W, int(num_iter), lam.ctypes.data,
+                                          ptr[0], ptr[1], rec.ctypes.data)
+        return rec
 
     def rec_nn_with_L1_eye_reg(self, data, num_iter, reg):
         data = np.ascontiguousarray(data, dtype=np.float64)

@@ -257,3 +257,61 @@ def test_nnls_L1_matches_oracle():
             for w in range(2):
                 ro, _ = orc.rec_nn_with_L1_eye_reg(y[w:w + 1], 15, eta)
                 assert rel_l2(rec[w], ro) < TOL_REC
+
+
+def test_nnls_warm_start_dual_matches_oracle():
+    """The warm-start form (outer iterate w0 and scaled dual v0 of nnls_L2_less) against the
+    oracle started from the same pair, independent solves with their own lambdaL2."""
+    import torch
+    from mbrec.solvers import rec_nn_with_L2_reg_batch
+    from mbrec.phantoms import disks, add_noise
+    _log("nnls_L2 warm start (w0, v0)")
+    m = _geometry_model(24, 24, 16, 400, 1.25e-4, sir=False, sigma=3.0)
+    orc = _oracle_of(m)
+    ph = np.stack([disks(m.Discretization, n=4, seed=s) for s in (5, 6)])
+    y = add_noise(orc.forward(ph), 0.01, seed=7)
+    lam = [1e-2 * np.linalg.norm(orc.transpose(y[b:b + 1])) for b in range(2)]
+    g = rng(8)
+    w0 = np.maximum(g.standard_normal((2, 24, 24)), 0) * np.abs(ph).max()
+    v0 = g.standard_normal((2, 24, 24)) * np.abs(ph).max()
+    got = rec_nn_with_L2_reg_batch(m, torch.as_tensor(y, dtype=torch.float32).cuda(), 10, lam,
+                                   init=w0.astype(np.float32), init_dual=v0.astype(np.float32)).cpu().numpy()
+    want = orc.rec_nn_with_L2_reg_from(y, 10, lam, w0, v0)
+    cold = orc.rec_nn_with_L2_reg_from(y, 10, lam)
+    assert rel_l2(want, cold) > 1e-2          # the start matters
+    assert rel_l2(got, want) < TOL_REC
+
+
+def test_learned_warm_start_gains_over_cold_start():
+    """A U-Net trained on the device (own kernels) on plain solves, used as warm start
+    (w0, KKT dual v0): at equal outer iterations its error to a long plain solve is lower
+    than the cold start's."""
+    import torch
+    from mbrec.solvers import rec_nn_with_L2_reg_batch
+    from mbrec.warmstart import UNet, make_training_set, train, warm_start_estimate
+    _log("learned warm start gain")
+    m = _geometry_model(32, 32, 16, 400, 1.25e-4, sir=False, sigma=3.0)
+    xin, xt = make_training_set(m, 64, 500, 300, batch=32)
+    net = UNet(32, seed=1)
+    hist = train(net, xin, xt, epochs=25, batch=8, lr=2e-3, seed=1)
+    assert hist[-1] < 0.5 * hist[0]
+    from mbrec.warmstart import phantom_batch
+    ph = torch.as_tensor(phantom_batch(m.Discretization, 8, 9000)).cuda()
+    y = m.applyForward(ph)
+    gen = torch.Generator(device="cuda").manual_seed(3)
+    y = (y + 0.01 * y.abs().amax(dim=(1, 2), keepdim=True) * torch.randn(y.shape, generator=gen,
+                                                                           device="cuda")).contiguous()
+    lam = (1e-2 * torch.linalg.vector_norm(m.applyTranspose(y).reshape(8, -1).double(), dim=1)).cpu().numpy()
+    ref = rec_nn_with_L2_reg_batch(m, y, 1500, lam)
+    w0, v0 = warm_start_estimate(net, m, y, lam)
+    nref = torch.linalg.vector_norm(ref.reshape(8, -1).double(), dim=1)
+
+    def err(r):
+        return float((torch.linalg.vector_norm((r - ref).reshape(8, -1).double(), dim=1) / nref).median())
+
+    k = 20
+    e_cold = err(rec_nn_with_L2_reg_batch(m, y, k, lam))
+    e_warm = err(rec_nn_with_L2_reg_batch(m, y, k, lam, init=w0, init_dual=v0))
+    print(f"median error after {k} iterations: cold {e_cold:.4f}, warm {e_warm:.4f}, "
+          f"network alone {err(w0):.4f}", flush=True)
+    assert e_warm < e_cold

@@ -294,6 +294,42 @@ def test_nnls_L2_fixed_point_is_nonnegative_least_squares():
     assert rel_l2(rec.ravel(), ref) < 1e-3
 
 
+def test_nnls_L2_warm_start_oracle():
+    """Warm-start form (w0, v0) of nnls_L2_less: zero start = the cold solve; the KKT pair
+    (x*, g(x*)/beta on the zero set) of the bound-constrained minimiser is a fixed point;
+    a general start equals the exact-Krylov restatement started from it."""
+    from scipy.optimize import lsq_linear
+    m = _tiny_model(seed=80, Nz=5, Nx=4, D=4, nT=40)
+    A = _dense_of(m)
+    n = A.shape[1]
+    g = rng(81)
+    y = A @ np.maximum(g.standard_normal(n), 0) + 0.05 * g.standard_normal(A.shape[0])
+    lam = 0.02
+    cold, _, _ = m.rec_nn_with_L2_reg(y.reshape(1, m.D, m.nT), 7, lam)
+    np.testing.assert_allclose(m.rec_nn_with_L2_reg_from(y.reshape(1, m.D, m.nT), 7, [lam]), cold,
+                               rtol=1e-12, atol=1e-14)
+    Aa = np.vstack([A, np.sqrt(lam) * np.eye(n)])
+    ya = np.concatenate([y, np.zeros(n)])
+    xs = lsq_linear(Aa, ya, bounds=(0, np.inf), tol=1e-15, lsmr_tol="auto", max_iter=20000).x
+    xs[xs < 1e-12 * xs.max()] = 0.0
+    gr = A.T @ (A @ xs - y) + lam * xs
+    vs = np.where(xs > 0, 0.0, np.maximum(gr, 0.0) / 1e-3)
+    out = m.rec_nn_with_L2_reg_from(y.reshape(1, m.D, m.nT), 5, [lam], xs.reshape(1, -1), vs.reshape(1, -1))
+    assert rel_l2(out.ravel(), xs) < 1e-6
+    # general start: the restatement with the same (w0, v0)
+    w0 = np.maximum(g.standard_normal(n), 0)
+    v0 = g.standard_normal(n)
+    sb = np.sqrt(1e-3)
+    Aaug = np.vstack([Aa, sb * np.eye(n)])
+    w, v = w0.copy(), v0.copy()
+    for _ in range(6):
+        u = O.lsqr_krylov(Aaug, np.concatenate([ya, sb * (w + v)]), 2, w)
+        w = np.maximum(u - v, 0)
+        v = v + w - u
+    out = m.rec_nn_with_L2_reg_from(y.reshape(1, m.D, m.nT), 6, [lam], w0.reshape(1, -1), v0.reshape(1, -1))
+    assert rel_l2(out.ravel(), w) < 1e-8
+
+
 # --- L1 split Bregman with MATLAB-cgs inner solves ---------------------------------
 def _cgs_by_cg_polynomial(M, b, k):
     """CGS iterates for symmetric M with shadow residual r0 = b, x0 = 0, from the identity
