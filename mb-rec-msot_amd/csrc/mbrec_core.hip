@@ -52,10 +52,10 @@ static int upload(T *dst, const T *src, size_t n)
 
 static void free_work(Plan *p)
 {
-    float *bufs[] = {p->d_bins, p->d_hpart, p->d_imgI, p->d_img_tmp, p->d_sino_tmp};
+    float *bufs[] = {p->d_bins, p->d_hpart, p->d_imgI, p->d_img_tmp, p->d_sino_tmp, p->d_gpart};
     for (float *q : bufs)
         if (q) (void)hipFree(q);
-    p->d_bins = p->d_hpart = p->d_imgI = p->d_img_tmp = p->d_sino_tmp = nullptr;
+    p->d_bins = p->d_hpart = p->d_imgI = p->d_img_tmp = p->d_sino_tmp = p->d_gpart = nullptr;
     p->bcap = 0;
     p->work_bytes = 0;
 }
@@ -82,7 +82,8 @@ int ensure_work(Plan *p, int B)
         (rc = dalloc(&p->d_hpart, need_h * Bp, nullptr)) ||
         (rc = dalloc(&p->d_imgI, (size_t)p->P * Bp, &acc)) ||
         (rc = dalloc(&p->d_img_tmp, (size_t)2 * B * p->P, &acc)) ||
-        (rc = dalloc(&p->d_sino_tmp, (size_t)B * p->D * p->nT, &acc))) {
+        (rc = dalloc(&p->d_sino_tmp, (size_t)B * p->D * p->nT, &acc)) ||
+        (B >= 16 && (rc = dalloc(&p->d_gpart, (size_t)8 * B * p->P, &acc)))) {
         free_work(p);
         return rc;
     }
@@ -164,6 +165,7 @@ static int build_pull_records(Plan *p, const std::vector<int32_t> &slot_pix, con
     p->lo_off = 0;
     p->hi_off = 1;
     p->WL = p->Wb + 1;
+    rec.resize(total + kRecPad, make_int2(0, 0));   // zero pad: runs of records may read past the last bin
     int rc;
     if ((rc = dalloc(&p->d_rec, rec.size(), &p->rec_bytes)) || (rc = dalloc(&p->d_rec_off, off.size(), &p->rec_bytes)) ||
         (rc = upload(p->d_rec, rec.data(), rec.size())) || (rc = upload(p->d_rec_off, off.data(), off.size())))

@@ -35,6 +35,8 @@ struct HatTaps {
 };
 
 // Decoding of a fixed-point travel time into (bin index, interpolation weight).
+constexpr int kRecPad = 64;   // zero records after the pull table (runs of up to 64 read past the end)
+
 struct BinGeom {
     float phi1, phi2;            // sorted interior breakpoint phases in [0,1)
     float iw0, iw1, iw2;         // 1 / segment widths (0 for empty segments)
@@ -96,6 +98,7 @@ struct Plan {
     float *d_imgI = nullptr;     // [Pd..P][Bp]      interleaved image (scatter input)
     float *d_img_tmp = nullptr;  // 2 x [B][P]       prefilter temporaries
     float *d_sino_tmp = nullptr; // [B][D][nT]       samples before/after the IR filter
+    float *d_gpart = nullptr;    // [8][B][P]        partial images of the XCD-grouped gather (B >= 16; DAS only)
     size_t table_bytes = 0;
     size_t work_bytes = 0;
     int pins = 0;                // batches whose captured graphs use this plan's buffers

@@ -36,11 +36,13 @@ __device__ __forceinline__ Decoded decode_tau(int32_t q, const BinGeom &g)
     const int base = q >> g.F;   // arithmetic shift = floor
     const float fr = (float)(q & ((1 << g.F) - 1)) * __builtin_ldexpf(1.0f, -g.F);
     const int m = (fr >= g.phi1) + (fr >= g.phi2);
-    const float ph = (m == 0) ? 0.0f : ((m == 1) ? g.phi1 : g.phi2);
-    const float iw = (m == 0) ? g.iw0 : ((m == 1) ? g.iw1 : g.iw2);
+    // the three segment positions, then a select of values: selecting among the
+    // parameters themselves lets the compiler turn the choice into a memory load
+    // from the kernel-argument segment, a dependent load on every pair
+    const float u0 = fr * g.iw0, u1 = (fr - g.phi1) * g.iw1, u2 = (fr - g.phi2) * g.iw2;
     r.nn = base + g.K;
     r.m = m;
-    r.u = (fr - ph) * iw;
+    r.u = (m == 0) ? u0 : ((m == 1) ? u1 : u2);
     return r;
 }
 
@@ -50,6 +52,36 @@ __device__ __forceinline__ void kahan_add(float &s, float &c, float v)
     const float t = s + y;
     c = (t - s) - y;
     s = t;
+}
+
+// Staging of n elements into LDS with U global loads in flight per thread
+// before any is stored (a plain strided loop waits out one load latency per
+// element).  ld(e) returns element e, st(e, v) stores it; NT threads take part.
+template <int U, int NT, class Ld, class St>
+__device__ __forceinline__ void stage_batched(int t, int n, Ld &&ld, St &&st)
+{
+    for (int e0 = t; e0 < n; e0 += NT * U) {
+        float v[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int e = e0 + NT * u;
+            v[u] = e < n ? ld(e) : 0.0f;
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int e = e0 + NT * u;
+            if (e < n) st(e, v[u]);
+        }
+    }
+}
+
+// q = e / d, r = e % d for 0 <= e < 2^24, 0 < d < 2^16 (float reciprocal, one correction step)
+__device__ __forceinline__ void divmod_small(int e, int d, float inv, int &q, int &r)
+{
+    q = (int)((float)e * inv);
+    r = e - q * d;
+    if (r < 0) { --q; r += d; }
+    else if (r >= d) { ++q; r -= d; }
 }
 
 // ---------------------------------------------------------------------------
@@ -84,14 +116,130 @@ __global__ __launch_bounds__(256) void k_filt_axis(
     const int sgn = transp ? 1 : -1;
     double acc = 0.0;
     for (int a = -K; a <= K; ++a) {
-        int j = i0 + sgn * a;
-        j %= N;
-        if (j < 0) j += N;
+        int j = i0 + sgn * a;   // |a| <= K < N: one wrap at most
+        j = j < 0 ? j + N : (j >= N ? j - N : j);
         const float v = axis == 0 ? src[(size_t)ix * Nz + j] : src[(size_t)j * Nz + iz];
         acc = fma((double)k[a + K], (double)v, acc);
     }
     if (interleaved_out) out[(size_t)pxl * Bp + b] = (float)acc;
     else out[idx] = (float)acc;
+}
+
+// ---------------------------------------------------------------------------
+// separable circular prefilter, LDS-staged passes (FP64 sums as above).
+// z pass: 16 columns of Nz samples per workgroup, each staged once with K
+// wrapped samples on either side.  Planar output [B][P]; interleaved output
+// [P][Bp] (IL) takes the 16 columns as 16 consecutive sinograms of one x
+// column, so its stores are 64-byte runs (columns b >= B are written as zero).
+// x pass: one sinogram's 32 consecutive z rows over all Nx columns per
+// workgroup, planar in and out.
+// forward: out[i] = sum_a k[a] in[(i - a) mod N];  transpose: in[(i + a) mod N].
+// ---------------------------------------------------------------------------
+constexpr int kFzCols = 16;
+constexpr int kFxRows = 32;
+
+template <bool IL>
+__global__ __launch_bounds__(256) void k_filt_z(const float *__restrict__ in, float *__restrict__ out,
+                                                const float *__restrict__ k, int K, int Nz, int Nx, int B, int Bp,
+                                                int transp)
+{
+    extern __shared__ float sm[];
+    const int S = (Nz + 2 * K) | 1;   // column stride in LDS
+    double *tk = reinterpret_cast<double *>(sm);
+    float *xs = sm + 2 * (2 * K + 1 + 1);
+    const size_t P = (size_t)Nz * Nx;
+    // columns of this workgroup: IL: (ix, b0 + c); planar: (b, ix0 + c) over the flat b * Nx + ix index
+    int ixc = 0, b0 = 0;
+    size_t col0 = 0;
+    if (IL) {
+        ixc = blockIdx.x;
+        b0 = blockIdx.y * kFzCols;
+    } else {
+        col0 = (size_t)blockIdx.x * kFzCols;
+    }
+    for (int i = threadIdx.x; i < 2 * K + 1; i += blockDim.x) tk[i] = (double)k[i];
+    const int W = Nz + 2 * K;
+    const float invW = 1.0f / (float)W;
+    stage_batched<8, 256>(
+        threadIdx.x, kFzCols * W,
+        [&](int e) {
+            int c, i;
+            divmod_small(e, W, invW, c, i);
+            int j = i - K;
+            j = j < 0 ? j + Nz : (j >= Nz ? j - Nz : j);
+            if (IL) return b0 + c < B ? in[(size_t)(b0 + c) * P + (size_t)ixc * Nz + j] : 0.0f;
+            return col0 + c < (size_t)B * Nx ? in[(col0 + c) * Nz + j] : 0.0f;
+        },
+        [&](int e, float v) {
+            int c, i;
+            divmod_small(e, W, invW, c, i);
+            xs[c * S + i] = v;
+        });
+    __syncthreads();
+    // out[i] = sum_a k[a + K] xs[i + K - sgn a]: forward reads xs[i + 2K - t], transpose xs[i + t] (t = a + K)
+    if (IL) {
+        const int c = threadIdx.x % kFzCols;
+        const int b = b0 + c;
+        if (b >= Bp) return;
+        for (int iz = threadIdx.x / kFzCols; iz < Nz; iz += 256 / kFzCols) {
+            const float *x = xs + c * S + iz;
+            double acc = 0.0;
+            if (transp)
+                for (int t = 0; t <= 2 * K; ++t) acc = fma(tk[t], (double)x[t], acc);
+            else
+                for (int t = 0; t <= 2 * K; ++t) acc = fma(tk[t], (double)x[2 * K - t], acc);
+            out[((size_t)ixc * Nz + iz) * Bp + b] = b < B ? (float)acc : 0.0f;
+        }
+    } else {
+        for (int c = 0; c < kFzCols; ++c) {
+            if (col0 + c >= (size_t)B * Nx) break;
+            for (int iz = threadIdx.x; iz < Nz; iz += blockDim.x) {
+                const float *x = xs + c * S + iz;
+                double acc = 0.0;
+                if (transp)
+                    for (int t = 0; t <= 2 * K; ++t) acc = fma(tk[t], (double)x[t], acc);
+                else
+                    for (int t = 0; t <= 2 * K; ++t) acc = fma(tk[t], (double)x[2 * K - t], acc);
+                out[(col0 + c) * Nz + iz] = (float)acc;
+            }
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void k_filt_x(const float *__restrict__ in, float *__restrict__ out,
+                                                const float *__restrict__ k, int K, int Nz, int Nx, int transp)
+{
+    extern __shared__ float sm[];
+    double *tk = reinterpret_cast<double *>(sm);
+    float *xs = sm + 2 * (2 * K + 1 + 1);   // [Nx + 2K][kFxRows]
+    const int iz0 = blockIdx.x * kFxRows, b = blockIdx.y;
+    const size_t P = (size_t)Nz * Nx;
+    const float *src = in + (size_t)b * P;
+    for (int i = threadIdx.x; i < 2 * K + 1; i += blockDim.x) tk[i] = (double)k[i];
+    const int r = threadIdx.x % kFxRows;
+    const bool rv = iz0 + r < Nz;
+    // element e = i * kFxRows + r: row i of the wrapped x axis, z row iz0 + r
+    stage_batched<8, 256>(
+        threadIdx.x, (Nx + 2 * K) * kFxRows,
+        [&](int e) {
+            const int i = e / kFxRows, rr = e % kFxRows;
+            int j = i - K;
+            j = j < 0 ? j + Nx : (j >= Nx ? j - Nx : j);
+            return iz0 + rr < Nz ? src[(size_t)j * Nz + iz0 + rr] : 0.0f;
+        },
+        [&](int e, float v) { xs[e] = v; });
+    __syncthreads();
+    if (!rv) return;
+    float *dst = out + (size_t)b * P + iz0 + r;
+    for (int ix = threadIdx.x / kFxRows; ix < Nx; ix += 256 / kFxRows) {
+        const float *x = xs + ix * kFxRows + r;
+        double acc = 0.0;
+        if (transp)
+            for (int t = 0; t <= 2 * K; ++t) acc = fma(tk[t], (double)x[t * kFxRows], acc);
+        else
+            for (int t = 0; t <= 2 * K; ++t) acc = fma(tk[t], (double)x[(2 * K - t) * kFxRows], acc);
+        dst[(size_t)ix * Nz] = (float)acc;
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -282,6 +430,91 @@ __global__ __launch_bounds__(256) void k_fir(const float *__restrict__ x, float 
     if (t + 3 < nT) orow[t + 3] = (t + 3 >= lo && t + 3 < hi) ? scale * (float)a3 : 0.0f;
 }
 
+// FIR, register-blocked: each thread owns 16 consecutive outputs; taps are
+// consumed in groups of 16 (zero-padded to Lp = 16 * ceil(L / 16)) against a
+// 31-sample register window, so every staged sample feeds 16 FMAs.  A half
+// workgroup (128 threads) covers a tile of kFirSpan outputs of one row; the
+// two halves take consecutive (row, tile) pairs.  Outputs go back through LDS
+// for contiguous stores.  Same sums as k_fir (FP64, taps ascending).
+constexpr int kFirOut = 16;
+constexpr int kFirSpan = 128 * kFirOut;
+
+__host__ __device__ constexpr int fir16_padded_taps(int L) { return (L + 15) & ~15; }
+// LDS window per half workgroup: 16 x NBLK floats, element e at (e % 16) * NBLK + e / 16
+// (NBLK odd), so 16-float runs owned by consecutive threads sit in distinct banks
+__host__ __device__ constexpr int fir16_nblk(int Lp) { return ((kFirSpan + Lp + 32) / 16) | 1; }
+__host__ __device__ constexpr int fir16_window(int Lp) { return 16 * fir16_nblk(Lp); }
+
+__global__ __launch_bounds__(256) void k_fir16(const float *__restrict__ x, float *__restrict__ out,
+                                               const float *__restrict__ taps, int L, int per_det, int off, int D,
+                                               int nT, int rows, float scale, const int2 *__restrict__ win)
+{
+    extern __shared__ float sm[];
+    const int Lp = fir16_padded_taps(L), nx = fir16_window(Lp), NBLK = fir16_nblk(Lp);
+    const int hf = threadIdx.x >> 7, lt = threadIdx.x & 127;
+    double *tk = reinterpret_cast<double *>(sm) + (size_t)hf * Lp;
+    float *xs = sm + 2 * 2 * Lp + (size_t)hf * nx;
+    const int ntile = (nT + kFirSpan - 1) / kFirSpan;
+    const int tile = blockIdx.x * 2 + hf;
+    const bool live = tile < rows * ntile;
+    const int row = live ? tile / ntile : 0;
+    const int t0 = live ? (tile - row * ntile) * kFirSpan : 0;
+    int lo = 0, hi = nT;   // samples outside [lo, hi) are zero on input and output
+    if (win) {
+        const int2 wb = win[row / D];
+        lo = max(0, wb.x);
+        hi = min(nT, wb.y);
+    }
+    // xs[m] = x[xbase + m], xbase = t0 + off - Lp - 15
+    const int xbase = t0 + off - Lp - 15;
+    const float *xr = x + (size_t)row * nT;
+    stage_batched<8, 128>(
+        lt, nx,
+        [&](int i) {
+            const int t = xbase + i;
+            return (live && t >= lo && t < hi) ? xr[t] : 0.0f;
+        },
+        [&](int i, float v) { xs[(i & 15) * NBLK + (i >> 4)] = v; });
+    const float *tp = taps + (per_det ? (size_t)(row % D) * L : 0);
+    for (int i = lt; i < Lp; i += 128) tk[i] = i < L ? (double)tp[i] : 0.0;
+    __syncthreads();
+    // output t0 + 16 lt + k at tap i reads element 16 lt + k - i + Lp + 15; the window
+    // W[w] = element 16 (lt + (Lp - i0) / 16) + w, i.e. block lt + (Lp - i0) / 16, lane w
+    const int blk = lt + Lp / 16;
+    double W[31], acc[kFirOut];
+#pragma unroll
+    for (int w = 0; w < 16; ++w) W[w] = (double)xs[w * NBLK + blk];
+#pragma unroll
+    for (int w = 16; w < 31; ++w) W[w] = (double)xs[(w - 16) * NBLK + blk + 1];
+#pragma unroll
+    for (int k = 0; k < kFirOut; ++k) acc[k] = 0.0;
+    for (int i0 = 0; i0 < Lp; i0 += 16) {
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+            const double w = tk[i0 + q];
+#pragma unroll
+            for (int k = 0; k < kFirOut; ++k) acc[k] = fma(w, W[k - q + 15], acc[k]);
+        }
+        if (i0 + 16 < Lp) {
+#pragma unroll
+            for (int w = 30; w >= 16; --w) W[w] = W[w - 16];
+            const int bn = blk - (i0 + 16) / 16;
+#pragma unroll
+            for (int w = 0; w < 16; ++w) W[w] = (double)xs[w * NBLK + bn];
+        }
+    }
+    __syncthreads();   // xs is reused for the outputs (same layout)
+#pragma unroll
+    for (int k = 0; k < kFirOut; ++k) {
+        const int t = t0 + 16 * lt + k;
+        xs[k * NBLK + lt] = (t >= lo && t < hi) ? scale * (float)acc[k] : 0.0f;
+    }
+    __syncthreads();
+    if (!live) return;
+    float *orow = out + (size_t)row * nT;
+    for (int i = lt; i < kFirSpan && t0 + i < nT; i += 128) orow[t0 + i] = xs[(i & 15) * NBLK + (i >> 4)];
+}
+
 // ---------------------------------------------------------------------------
 // samples -> bins (M): bins[d][j][b] = sum_i w[m][i] z[b][d][n + kmin + i], z = 0 outside [0, nT).
 // One workgroup per (bin tile, detector, batch chunk).
@@ -323,6 +556,134 @@ __global__ __launch_bounds__(256) void k_z_to_bins(const float *__restrict__ z, 
         float acc = 0.0f;
         for (int q = 0; q < taps.nw; ++q) acc = fmaf(taps.w[m][q], zs[q], acc);
         brow[(size_t)j * Bp + b] = acc;
+    }
+}
+
+// batch-chunk forms of the two hat-filter kernels (Bp >= 16): a workgroup
+// takes BC consecutive sinograms (lane b of the chunk = sinogram b0 + b), so
+// bin-row stores and loads are BC-float runs; integer work per element is
+// compile-time (no runtime divisions), and the taps sit in LDS.  Same sums in
+// the same order as k_z_to_bins / k_bins_to_y.
+template <int BC>
+__global__ __launch_bounds__(256) void k_z_to_bins_bc(const float *__restrict__ z, float *__restrict__ bins,
+                                                      HatTaps taps, BinGeom g, int D, int B, int Bp,
+                                                      const int2 *__restrict__ win)
+{
+    extern __shared__ float sm[];
+    float *tw = sm;                        // [3][nw]
+    float *zsm = sm + 3 * MBREC_MAX_TAPS;  // [BC][nzp]
+    const int d = blockIdx.y, b0 = blockIdx.z * BC;
+    const int nT = g.nT, NB = g.NB, nw = taps.nw;
+    const int j0 = blockIdx.x * kBinTile, j1 = min(NB, j0 + kBinTile);
+    const int nlo = j0 / 3 - g.K, nhi = (j1 - 1) / 3 - g.K;
+    const int zlo = nlo + taps.kmin;
+    const int nz = nhi - nlo + nw, nzp = nz | 1;
+    for (int i = threadIdx.x; i < 3 * nw; i += 256) tw[i] = taps.w[i / nw][i % nw];
+    int2 *wl = reinterpret_cast<int2 *>(zsm + BC * nzp);   // [BC] sample window per sinogram
+    for (int b = threadIdx.x; b < BC; b += 256) {
+        int2 w = make_int2(0, b0 + b < B ? nT : 0);
+        if (win && b0 + b < B) w = make_int2(max(0, win[b0 + b].x), min(nT, win[b0 + b].y));
+        wl[b] = w;
+    }
+    __syncthreads();
+    const float invz = 1.0f / (float)nz;
+    stage_batched<8, 256>(
+        threadIdx.x, BC * nz,
+        [&](int e) {
+            int b, q;
+            divmod_small(e, nz, invz, b, q);
+            const int t = zlo + q;
+            const int2 w = wl[b];
+            return (t >= w.x && t < w.y) ? z[((size_t)(b0 + b) * D + d) * nT + t] : 0.0f;
+        },
+        [&](int e, float v) {
+            int b, q;
+            divmod_small(e, nz, invz, b, q);
+            zsm[b * nzp + q] = v;
+        });
+    __syncthreads();
+    const int b = threadIdx.x % BC, bg = b0 + b;
+    if (bg >= Bp) return;
+    float *brow = bins + (size_t)d * NB * Bp + bg;
+    const float *zb = zsm + b * nzp;
+    const int ns = nhi - nlo + 1;
+    for (int k = threadIdx.x / BC; k < ns; k += 256 / BC) {
+        const float *zs = zb + k;   // z[n + kmin + q] for n = nlo + k
+        float a0 = 0.0f, a1 = 0.0f, a2 = 0.0f;
+        for (int q = 0; q < nw; ++q) {
+            const float v = zs[q];
+            a0 = fmaf(tw[q], v, a0);
+            a1 = fmaf(tw[nw + q], v, a1);
+            a2 = fmaf(tw[2 * nw + q], v, a2);
+        }
+        const int j = j0 + 3 * k;
+        brow[(size_t)j * Bp] = a0;
+        if (j + 1 < j1) brow[(size_t)(j + 1) * Bp] = a1;
+        if (j + 2 < j1) brow[(size_t)(j + 2) * Bp] = a2;
+    }
+}
+
+template <int BC, int TY, bool DIRECT>
+__global__ __launch_bounds__(256) void k_bins_to_y_bc(
+    const float *__restrict__ hpart, float *__restrict__ y, HatTaps taps, BinGeom g, int Wb, int nbk, int lo_off,
+    int WL, int Bp, int D, int B, float scale, const int2 *__restrict__ win)
+{
+    extern __shared__ float sm[];
+    float *tw = sm;
+    float *bsm = sm + 3 * MBREC_MAX_TAPS;   // [BC][njp]
+    const int d = blockIdx.y, b0 = blockIdx.z * BC;
+    const int nT = g.nT, nw = taps.nw;
+    const int t0 = blockIdx.x * TY, t1 = min(nT, t0 + TY);
+    const int kmax = taps.kmin + nw - 1;
+    const int jlo = 3 * (t0 - kmax + g.K);
+    const int jhi = 3 * (t1 - 1 - taps.kmin + g.K) + 2;
+    const int nj = jhi - jlo + 1, njp = nj | 1;
+    for (int i = threadIdx.x; i < 3 * nw; i += 256) tw[i] = taps.w[i / nw][i % nw];
+    if (DIRECT) {   // element e = jj * BC + b
+        stage_batched<8, 256>(
+            threadIdx.x, nj * BC,
+            [&](int e) {
+                const int jj = e / BC, b = e % BC;
+                return b0 + b < Bp ? hpart[((size_t)d * g.NB + jlo + jj) * Bp + b0 + b] : 0.0f;
+            },
+            [&](int e, float v) { bsm[(e % BC) * njp + e / BC] = v; });
+    } else {
+        const int b = threadIdx.x % BC;
+        for (int jj = threadIdx.x / BC; jj < nj; jj += 256 / BC) {
+            float acc = 0.0f;
+            if (b0 + b < Bp) {
+                const int rel = jlo + jj - lo_off;
+                const int ka = rel >= WL - 1 ? (rel - (WL - 1) + Wb - 1) / Wb : 0;
+                const int kz = min(nbk - 1, rel / Wb);
+                for (int k = ka; k <= kz; ++k)
+                    acc += hpart[(((size_t)d * nbk + k) * WL + (rel - k * Wb)) * Bp + b0 + b];
+            }
+            bsm[b * njp + jj] = acc;
+        }
+    }
+    __syncthreads();
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    for (int b = wv; b < BC; b += 4) {
+        const int bg = b0 + b;
+        if (bg >= B) break;
+        const float *bs = bsm + b * njp;
+        int lo = 0, hi = nT;
+        if (win) {
+            const int2 wb = win[bg];
+            lo = wb.x;
+            hi = wb.y;
+        }
+        float *yr = y + ((size_t)bg * D + d) * nT;
+        for (int r = t0 + lane; r < t1; r += 64) {
+            float acc = 0.0f;
+#pragma unroll
+            for (int m = 0; m < 3; ++m) {
+                const int jb = 3 * (r - taps.kmin + g.K) + m - jlo;
+                for (int q = 0; q < nw; ++q) acc = fmaf(tw[m * nw + q], bs[jb - 3 * q], acc);
+            }
+            if (r < lo || r >= hi) acc = 0.0f;
+            yr[r] = scale * acc;
+        }
     }
 }
 
@@ -713,6 +1074,65 @@ __global__ __launch_bounds__(256) void k_fwd_pull_bl(const int2 *__restrict__ re
     }
 }
 
+// batch-lane form with lane groups of 64 sinograms: each wave owns JR bins of
+// one detector, so bin offsets and records are uniform over the wave and live
+// in scalar registers.  Per bin, its records are consumed in runs of U whose
+// image rows are all loaded before any is accumulated (records past the bin's
+// end read the zero padding after the table with weight 0).
+template <int U>
+__global__ __launch_bounds__(256) void k_fwd_pull_wu(const int2 *__restrict__ rec, const int32_t *__restrict__ off,
+                                                     const float *__restrict__ imgI, float *__restrict__ bins, int NB,
+                                                     int Bp, int JR, int xcd_map /* D / 8, or 0 */)
+{
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    int d, jblk;
+    if (xcd_map) {   // XCD x (= blockIdx.x % 8) takes detectors [x * D/8, (x + 1) * D/8)
+        const int x = blockIdx.x % 8, sidx = blockIdx.x / 8;
+        d = x * xcd_map + sidx % xcd_map;
+        jblk = sidx / xcd_map;
+    } else {
+        d = blockIdx.y;
+        jblk = blockIdx.x;
+    }
+    const int ja = __builtin_amdgcn_readfirstlane((jblk * 4 + wv) * JR);
+    if (ja >= NB) return;   // no barriers in this kernel
+    const int jb = min(NB, ja + JR);
+    const int b = blockIdx.z * 64 + lane;
+    const int bo = min(b, Bp - 1);   // lanes past Bp compute a valid column and store nothing
+    const int32_t *o = off + (size_t)d * (NB + 1);
+    const float *cb = imgI + bo;
+    float *dst = bins + (size_t)d * NB * Bp + b;
+    float prev = 0.0f;
+    if (ja > 0)   // u-part of bin ja - 1
+        for (int r = o[ja - 1], re = o[ja]; r < re; ++r) {
+            const int2 e = rec[r];
+            prev = fmaf(__int_as_float(e.y), cb[(size_t)e.x * Bp], prev);
+        }
+    int r0 = o[ja];
+    for (int j = ja; j < jb; ++j) {
+        const int r1 = o[j + 1];
+        float s0 = 0.0f, s1 = 0.0f;
+        for (int r = r0; r < r1; r += U) {
+            int2 e[U];
+            float cv[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) e[u] = rec[r + u];
+#pragma unroll
+            for (int u = 0; u < U; ++u) cv[u] = cb[(size_t)e[u].x * Bp];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const bool in = r + u < r1;
+                const float uu = in ? __int_as_float(e[u].y) : 0.0f;
+                s0 = fmaf(in ? 1.0f - uu : 0.0f, cv[u], s0);
+                s1 = fmaf(uu, cv[u], s1);
+            }
+        }
+        if (b < Bp) dst[(size_t)j * Bp] = s0 + prev;
+        prev = s1;
+        r0 = r1;
+    }
+}
+
 // ---------------------------------------------------------------------------
 // tile gather (batch-lane, Bp >= 16): one workgroup per 16 x 16 pixel tile and
 // chunk of LB sinograms.  For every detector the tile's bin window (all bins its
@@ -851,6 +1271,128 @@ __global__ __launch_bounds__(256) void k_gather_tile2(
     }
 }
 
+// XCD-grouped batch-lane gather: workgroups are dealt round-robin to the 8 XCDs;
+// XCD g (= blockIdx.x % 8) takes detectors [g*DG, (g+1)*DG), DG = ceil(D/8), for every pixel slot
+// and writes a partial image, so the bin rows its concurrent workgroups read
+// (DG detectors x one small pixel region) stay in that XCD's L2.  The 8
+// partials are summed in a fixed order by k_sum_parts.
+template <int LB, int U>
+__global__ __launch_bounds__(256) void k_gather_bl_xcd(
+    const int32_t *__restrict__ tq, const int32_t *__restrict__ pix, const float *__restrict__ bins,
+    float *__restrict__ part, BinGeom g, int Pd, int P, int D, int B, int Bp, int nsb)
+{
+    constexpr int RPW = 64 / LB;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int bl = lane % LB, rq = lane / LB;
+    const int grp = blockIdx.x % 8, s = blockIdx.x / 8;
+    const int sb = s % nsb, bc = s / nsb;
+    const int slot = (sb * 4 + wv) * RPW + rq;
+    const int b = bc * LB + bl;
+    const bool bv = b < Bp;
+    const int DG = (D + 7) / 8, d0 = grp * DG, dn = min(DG, D - d0);   // dn <= 0: empty group
+    const int span = g.nT + 2 * g.K;
+    const size_t rowstride = (size_t)g.NB * Bp;
+    float acc = 0.0f, cmp = 0.0f;
+    for (int dd = 0; dd < dn; dd += U) {
+        int32_t q[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) q[u] = dd + u < dn ? tq[(size_t)(d0 + dd + u) * Pd + slot] : INT32_MIN;
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const Decoded dc = decode_tau(q[u], g);
+            if (bv && (unsigned)dc.nn < (unsigned)span) {
+                const float *gb = bins + (size_t)(d0 + dd + u) * rowstride + b;
+                const int j = 3 * dc.nn + dc.m;
+                const float g0 = gb[(size_t)j * Bp], g1 = gb[(size_t)(j + 1) * Bp];
+                kahan_add(acc, cmp, fmaf(dc.u, g1 - g0, g0));
+            }
+        }
+    }
+    const int px = pix[slot];
+    if (px >= 0 && b < B) part[((size_t)grp * B + b) * P + px] = acc - cmp;
+}
+
+// Batch-lane gather with lane groups of 64 sinograms: each wave owns one pixel
+// slot, so the travel time of every (slot, detector) pair is uniform over the
+// wave.  Lane i decodes the pair of detector d0 + i (one strided load of 64
+// travel times and one decode for 64 pairs); the wave then walks the 64 pairs
+// with the bin offset and hat weight of pair k read out of lane k into scalar
+// registers, issuing the two bin-row loads of U pairs before it accumulates
+// them (plain sums of U pairs, compensated across groups).  XG: workgroups are
+// dealt round-robin to the 8 XCDs and XCD x sums detectors
+// [x*ceil(D/8), (x+1)*ceil(D/8)) into partial image x (k_sum_parts adds them).
+// Bin offsets are 32-bit: D * NB * Bp < 2^31 (checked by the launcher).
+template <int U, bool XG>
+__global__ __launch_bounds__(256) void k_gather_wu(
+    const int32_t *__restrict__ tq, const int32_t *__restrict__ pix, const float *__restrict__ bins,
+    float *__restrict__ out, BinGeom g, int Pd, int P, int D, int B, int Bp, int nsb, float scale)
+{
+    const int lane = threadIdx.x & 63;
+    int grp = 0, sb, bc;
+    if (XG) {
+        grp = blockIdx.x % 8;
+        const int s = blockIdx.x / 8;
+        sb = s % nsb;
+        bc = s / nsb;
+    } else {
+        sb = blockIdx.x;
+        bc = blockIdx.y;
+    }
+    const int slot = __builtin_amdgcn_readfirstlane(sb * 4 + (threadIdx.x >> 6));
+    const int b = bc * 64 + lane;
+    const int bo = min(b, Bp - 1);   // lanes past Bp read a valid column; their sums are dropped
+    const int bo1 = bo + Bp;
+    const int DG = XG ? (D + 7) / 8 : D;
+    const int dbeg = grp * DG, dend = min(D, dbeg + DG);
+    const int span = g.nT + 2 * g.K;
+    float acc = 0.0f, cmp = 0.0f;
+    for (int d0 = dbeg; d0 < dend; d0 += 64) {
+        const int d = d0 + lane;
+        const int32_t q = d < dend ? tq[(size_t)d * Pd + slot] : INT32_MIN;
+        const Decoded dc = decode_tau(q, g);
+        const bool ok = (unsigned)dc.nn < (unsigned)span;
+        const int off = ok ? (d * g.NB + 3 * dc.nn + dc.m) * Bp : 0;
+        const float wu = ok ? dc.u : 0.0f;
+        const float wv = ok ? 1.0f : 0.0f;
+        const int n = min(64, dend - d0);
+        for (int k = 0; k < n; k += U) {
+            float g0[U], g1[U], uu[U], vv[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int o = __builtin_amdgcn_readlane(off, k + u);
+                uu[u] = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, wu), k + u));
+                vv[u] = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, wv), k + u));
+                const float *row = bins + o;
+                g0[u] = row[bo];
+                g1[u] = row[bo1];
+            }
+            float part = 0.0f;
+#pragma unroll
+            for (int u = 0; u < U; ++u) part += vv[u] * fmaf(uu[u], g1[u] - g0[u], g0[u]);
+            kahan_add(acc, cmp, part);
+        }
+    }
+    const int px = pix[slot];
+    if (px >= 0 && b < B) {
+        if (XG)
+            out[((size_t)grp * B + b) * P + px] = acc - cmp;
+        else
+            out[(size_t)b * P + px] = scale * (acc - cmp);
+    }
+}
+
+// out[b][x] = scale * sum of the 8 partial images (fixed order)
+__global__ __launch_bounds__(256) void k_sum_parts(const float *__restrict__ part, float *__restrict__ out,
+                                                   size_t n, float scale)
+{
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        float s = 0.0f;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) s += part[(size_t)k * n + i];
+        out[i] = scale * s;
+    }
+}
+
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
@@ -916,6 +1458,8 @@ static bool pull_choice_ok(int c, int Bp)
         const int lb = (-c) % 100, mode = (-c) / 100;
         int cap = 16;
         while (cap < Bp) cap *= 2;
+        if (mode >= 3)   // wave-uniform runs: lane groups of 64 only
+            return Bp >= 16 && lb == 64 && mode <= 5;
         return Bp >= 16 && (lb == 16 || lb == 32 || lb == 64) && lb <= cap && mode <= 2;
     }
     return false;
@@ -923,7 +1467,7 @@ static bool pull_choice_ok(int c, int Bp)
 
 static int default_pull_choice(int Bp)
 {
-    if (Bp >= 64) return -64;
+    if (Bp >= 64) return -564;
     if (Bp >= 32) return -32;
     if (Bp >= 16) return -16;
     if (Bp % 8 == 0) return 8;
@@ -959,6 +1503,22 @@ static void fwd_pull_bl(Plan *p, const float *imgI, int Bp, bool xcd, hipStream_
                        p->geom.NB, Bp, kPullRange, 0);
 }
 
+template <int U>
+static void fwd_pull_wu(Plan *p, const float *imgI, int Bp, bool xcd, hipStream_t s)
+{
+    const int ngrp = (p->geom.NB + kPullRange - 1) / kPullRange;
+    const int nblk = (ngrp + 3) / 4;
+    if (xcd && p->D % 8 == 0) {
+        dim3 grid(nblk * p->D, 1, (Bp + 63) / 64);
+        hipLaunchKernelGGL((k_fwd_pull_wu<U>), grid, dim3(256), 0, s, p->d_rec, p->d_rec_off, imgI, p->d_bins,
+                           p->geom.NB, Bp, kPullRange, p->D / 8);
+        return;
+    }
+    dim3 grid(nblk, p->D, (Bp + 63) / 64);
+    hipLaunchKernelGGL((k_fwd_pull_wu<U>), grid, dim3(256), 0, s, p->d_rec, p->d_rec_off, imgI, p->d_bins,
+                       p->geom.NB, Bp, kPullRange, 0);
+}
+
 static int launch_pull(Plan *p, const float *imgI, int Bp, hipStream_t s)
 {
     int c = (p->tune.Bp == Bp && p->tune.scatter_bc != 0) ? p->tune.scatter_bc : default_pull_choice(Bp);
@@ -976,7 +1536,10 @@ static int launch_pull(Plan *p, const float *imgI, int Bp, hipStream_t s)
     case -164: fwd_pull_bl<64, 8>(p, imgI, Bp, true, s); break;
     case -216: fwd_pull_bl<16, 16>(p, imgI, Bp, true, s); break;
     case -232: fwd_pull_bl<32, 16>(p, imgI, Bp, true, s); break;
-    default: fwd_pull_bl<64, 16>(p, imgI, Bp, true, s); break;
+    case -264: fwd_pull_bl<64, 16>(p, imgI, Bp, true, s); break;
+    case -364: fwd_pull_wu<8>(p, imgI, Bp, false, s); break;
+    case -464: fwd_pull_wu<8>(p, imgI, Bp, true, s); break;
+    default: fwd_pull_wu<16>(p, imgI, Bp, true, s); break;
     }
     MBREC_HIP(hipGetLastError());
     return MBREC_OK;
@@ -1017,9 +1580,30 @@ static int launch_scatter(Plan *p, const float *imgI, int B, int Bp, hipStream_t
     return MBREC_OK;
 }
 
+template <int BC, int TY>
+static void bins_to_y_bc(Plan *p, float *y, int B, int Bp, float scale, hipStream_t s, const int2 *win)
+{
+    const int nj = 3 * (TY + p->taps.nw) + 3;
+    const size_t lds = sizeof(float) * (3 * MBREC_MAX_TAPS + (size_t)BC * (nj | 1));
+    dim3 grid((p->nT + TY - 1) / TY, p->D, (Bp + BC - 1) / BC);
+    if (p->pull)
+        hipLaunchKernelGGL((k_bins_to_y_bc<BC, TY, true>), grid, dim3(256), lds, s, p->d_bins, y, p->taps, p->geom,
+                           p->Wb, p->nbk, p->lo_off, p->WL, Bp, p->D, B, scale, win);
+    else
+        hipLaunchKernelGGL((k_bins_to_y_bc<BC, TY, false>), grid, dim3(256), lds, s, p->d_hpart, y, p->taps,
+                           p->geom, p->Wb, p->nbk, p->lo_off, p->WL, Bp, p->D, B, scale, win);
+}
+
 static int launch_bins_to_y(Plan *p, float *y, int B, int Bp, float scale, hipStream_t s,
                             const int2 *win = nullptr)
 {
+    if (Bp >= 16) {
+        if (Bp >= 64) bins_to_y_bc<64, 64>(p, y, B, Bp, scale, s, win);
+        else if (Bp >= 32) bins_to_y_bc<32, 128>(p, y, B, Bp, scale, s, win);
+        else bins_to_y_bc<16, 128>(p, y, B, Bp, scale, s, win);
+        MBREC_HIP(hipGetLastError());
+        return MBREC_OK;
+    }
     const int nj = 3 * (kYTile + p->taps.nw) + 3;
     // LDS chunk of sinograms: at most kHatLds bytes, so several workgroups share a CU
     const int Bc = std::max(1, std::min(B, (int)(kHatLds / (sizeof(float) * (size_t)(nj | 1)))));
@@ -1039,6 +1623,16 @@ static int launch_fir(Plan *p, const float *x, float *out, int B, bool corr, flo
 {
     const int L = p->Lir;
     const int off = L / 2;            // odd L: the same offset serves the convolution and the flipped-IR correlation
+    const int Lp = fir16_padded_taps(L);
+    const size_t lds16 = 2 * (sizeof(double) * (size_t)Lp + sizeof(float) * (size_t)fir16_window(Lp));
+    if (lds16 <= kLdsBudget) {
+        const int rows = B * p->D, ntile = (p->nT + kFirSpan - 1) / kFirSpan;
+        const size_t nblk = ((size_t)rows * ntile + 1) / 2;
+        hipLaunchKernelGGL(k_fir16, dim3((unsigned)nblk), dim3(256), lds16, s, x, out, corr ? p->d_irf : p->d_ir, L,
+                           p->ir_per_det, off, p->D, p->nT, rows, scale, win);
+        MBREC_HIP(hipGetLastError());
+        return MBREC_OK;
+    }
     dim3 grid((p->nT + kFirTile - 1) / kFirTile, B * p->D);
     const size_t lds = sizeof(float) * (size_t)(kFirTile + 2 * L + 2);
     hipLaunchKernelGGL(k_fir, grid, dim3(256), lds, s, x, out, corr ? p->d_irf : p->d_ir, L, p->ir_per_det, off,
@@ -1047,8 +1641,24 @@ static int launch_fir(Plan *p, const float *x, float *out, int B, bool corr, flo
     return MBREC_OK;
 }
 
+template <int BC>
+static void z_to_bins_bc(Plan *p, const float *z, int B, int Bp, hipStream_t s, const int2 *win)
+{
+    const int nz = kBinTile / 3 + p->taps.nw + 2;
+    const size_t lds = sizeof(float) * (3 * MBREC_MAX_TAPS + (size_t)BC * (nz | 1) + 2 * BC + 2);
+    dim3 grid((p->geom.NB + kBinTile - 1) / kBinTile, p->D, (Bp + BC - 1) / BC);
+    hipLaunchKernelGGL(k_z_to_bins_bc<BC>, grid, dim3(256), lds, s, z, p->d_bins, p->taps, p->geom, p->D, B, Bp, win);
+}
+
 static int launch_z_to_bins(Plan *p, const float *z, int B, int Bp, hipStream_t s, const int2 *win = nullptr)
 {
+    if (Bp >= 16) {
+        if (Bp >= 64) z_to_bins_bc<64>(p, z, B, Bp, s, win);
+        else if (Bp >= 32) z_to_bins_bc<32>(p, z, B, Bp, s, win);
+        else z_to_bins_bc<16>(p, z, B, Bp, s, win);
+        MBREC_HIP(hipGetLastError());
+        return MBREC_OK;
+    }
     const int nz = kBinTile / 3 + p->taps.nw + 2;
     const int Bc = std::max(1, std::min(B, (int)(kHatLds / (sizeof(float) * (size_t)(nz | 1)))));
     dim3 grid((p->geom.NB + kBinTile - 1) / kBinTile, p->D, (B + Bc - 1) / Bc);
@@ -1097,6 +1707,33 @@ static void gather_tile(Plan *p, float *img, int B, int Bp, float scale, hipStre
                        p->d_bins, img, p->geom, p->Pd, p->P, p->D, B, Bp, p->tile_wmax, scale);
 }
 
+template <int LB, int U>
+static void gather_bl_xcd(Plan *p, float *img, int B, int Bp, float scale, hipStream_t s)
+{
+    constexpr int RPW = 64 / LB;
+    const int nsb = p->Pd / (4 * RPW);
+    dim3 grid(nsb * 8 * ((Bp + LB - 1) / LB));
+    hipLaunchKernelGGL((k_gather_bl_xcd<LB, U>), grid, dim3(256), 0, s, p->d_tq, p->d_pix, p->d_bins, p->d_gpart,
+                       p->geom, p->Pd, p->P, p->D, B, Bp, nsb);
+    const size_t n = (size_t)B * p->P;
+    hipLaunchKernelGGL(k_sum_parts, dim3(std::min<size_t>(4096, (n + 255) / 256)), dim3(256), 0, s,
+                       (const float *)p->d_gpart, img, n, scale);
+}
+
+template <int U, bool XG>
+static void gather_wu(Plan *p, float *img, int B, int Bp, float scale, hipStream_t s)
+{
+    const int nsb = p->Pd / 4, nbc = (Bp + 63) / 64;
+    const dim3 grid = XG ? dim3(8 * nsb * nbc) : dim3(nsb, nbc);
+    hipLaunchKernelGGL((k_gather_wu<U, XG>), grid, dim3(256), 0, s, p->d_tq, p->d_pix, p->d_bins,
+                       XG ? p->d_gpart : img, p->geom, p->Pd, p->P, p->D, B, Bp, nsb, scale);
+    if (XG) {
+        const size_t n = (size_t)B * p->P;
+        hipLaunchKernelGGL(k_sum_parts, dim3(std::min<size_t>(4096, (n + 255) / 256)), dim3(256), 0, s,
+                           (const float *)p->d_gpart, img, n, scale);
+    }
+}
+
 constexpr int kTilePF = 16;   // 16-byte prefetch registers per thread (pipelined tile gather)
 
 template <int LB>
@@ -1117,12 +1754,24 @@ struct GatherVariant {
 static const GatherVariant kGatherVariants[] = {{1, 8, 0},   {1, 4, 0},   {4, 4, 0},   {4, 2, 0},  {8, 2, 0},
                                                 {8, 1, 0},   {16, 1, 0},  {0, 2, 16},  {0, 2, 32}, {0, 1, 64},
                                                 {0, 2, 64},  {0, 4, 64},  {0, -1, 16}, {0, -1, 32}, {0, -1, 64},
-                                                {0, -2, 16}, {0, -2, 32}, {0, -2, 64}};
+                                                {0, -2, 16}, {0, -2, 32}, {0, -2, 64}, {0, -3, 16},
+                                                {0, -3, 32}, {0, -3, 64}, {0, -4, 64}, {0, -5, 8},
+                                                {0, -5, 16}, {0, -6, 8},  {0, -6, 16}};
 constexpr int kNumGatherVariants = sizeof(kGatherVariants) / sizeof(kGatherVariants[0]);
 
 static bool gather_variant_ok(const Plan *p, int v, int Bp)
 {
     const GatherVariant &g = kGatherVariants[v];
+    if (g.u <= -5) {   // wave-uniform gather (u = -6: XCD-grouped): SIR-free, 32-bit bin offsets
+        if (p->use_sir || Bp < 16 || (size_t)p->D * p->geom.NB * Bp >= ((size_t)1 << 31)) return false;
+        return g.u == -5 || p->d_gpart;
+    }
+    if (g.u <= -3) {   // XCD-grouped gather: SIR-free plans, partial images
+        if (p->use_sir || Bp < 16 || !p->d_gpart) return false;
+        int cap = 16;
+        while (cap < Bp) cap *= 2;
+        return g.lb <= cap;
+    }
     if (g.u < 0) {   // tile gather: SIR-free plans, window fits LDS (and the prefetch registers)
         if (p->use_sir || !p->d_twin || Bp < 16 || Bp % 4) return false;
         const size_t lds = sizeof(float) * (size_t)p->tile_wmax * g.lb + 2048;
@@ -1140,7 +1789,7 @@ static bool gather_variant_ok(const Plan *p, int v, int Bp)
 static int default_gather_variant(const Plan *p, int Bp)
 {
     if (Bp == 1) return 0;
-    if (Bp >= 64) return 11;
+    if (Bp >= 64) return p->d_gpart && gather_variant_ok(p, 24, Bp) ? 24 : 11;
     if (Bp >= 32) return 8;
     if (Bp >= 16) return 7;
     if (Bp % 8 == 0) return 4;
@@ -1169,7 +1818,15 @@ static int launch_gather(Plan *p, float *img, int B, int Bp, float scale, hipStr
     case 14: gather_tile<64>(p, img, B, Bp, scale, s); break;
     case 15: gather_tile2<16>(p, img, B, Bp, scale, s); break;
     case 16: gather_tile2<32>(p, img, B, Bp, scale, s); break;
-    default: gather_tile2<64>(p, img, B, Bp, scale, s); break;
+    case 17: gather_tile2<64>(p, img, B, Bp, scale, s); break;
+    case 18: gather_bl_xcd<16, 2>(p, img, B, Bp, scale, s); break;
+    case 19: gather_bl_xcd<32, 2>(p, img, B, Bp, scale, s); break;
+    case 20: gather_bl_xcd<64, 2>(p, img, B, Bp, scale, s); break;
+    case 21: gather_bl_xcd<64, 4>(p, img, B, Bp, scale, s); break;
+    case 22: gather_wu<8, false>(p, img, B, Bp, scale, s); break;
+    case 23: gather_wu<16, false>(p, img, B, Bp, scale, s); break;
+    case 24: gather_wu<8, true>(p, img, B, Bp, scale, s); break;
+    default: gather_wu<16, true>(p, img, B, Bp, scale, s); break;
     }
     MBREC_HIP(hipGetLastError());
     return MBREC_OK;
@@ -1178,6 +1835,33 @@ static int launch_gather(Plan *p, float *img, int B, int Bp, float scale, hipStr
 static int launch_prefilter(Plan *p, const float *in, float *out, float *tmp, int B, bool transp,
                             bool interleaved_out, int Bp, hipStream_t s)
 {
+    const size_t tapb = sizeof(float) * 2 * (size_t)(2 * std::max(p->Kz, p->Kx) + 2);
+    const size_t ldz = tapb + sizeof(float) * (size_t)kFzCols * ((p->Nz + 2 * p->Kz) | 1);
+    const size_t ldx = tapb + sizeof(float) * (size_t)kFxRows * (p->Nx + 2 * p->Kx);
+    if (ldz <= kLdsBudget && ldx <= kLdsBudget) {
+        // forward: x pass then z pass (the z pass writes the interleaved layout);
+        // transpose: z^T then x^T
+        const dim3 gx((p->Nz + kFxRows - 1) / kFxRows, B);
+        const int tr = transp ? 1 : 0;
+        if (!transp) {
+            hipLaunchKernelGGL(k_filt_x, gx, dim3(256), ldx, s, in, tmp, p->d_kx, p->Kx, p->Nz, p->Nx, 0);
+            if (interleaved_out)
+                hipLaunchKernelGGL(k_filt_z<true>, dim3(p->Nx, (Bp + kFzCols - 1) / kFzCols), dim3(256), ldz, s,
+                                   (const float *)tmp, out, p->d_kz, p->Kz, p->Nz, p->Nx, B, Bp, 0);
+            else
+                hipLaunchKernelGGL(k_filt_z<false>, dim3(((size_t)B * p->Nx + kFzCols - 1) / kFzCols), dim3(256), ldz,
+                                   s, (const float *)tmp, out, p->d_kz, p->Kz, p->Nz, p->Nx, B, Bp, 0);
+        } else {
+            if (interleaved_out)
+                return set_error(MBREC_ERR_ARG, "mbrec:internal", "interleaved output of the transposed prefilter");
+            hipLaunchKernelGGL(k_filt_z<false>, dim3(((size_t)B * p->Nx + kFzCols - 1) / kFzCols), dim3(256), ldz, s,
+                               in, tmp, p->d_kz, p->Kz, p->Nz, p->Nx, B, Bp, tr);
+            hipLaunchKernelGGL(k_filt_x, gx, dim3(256), ldx, s, (const float *)tmp, out, p->d_kx, p->Kx, p->Nz,
+                               p->Nx, tr);
+        }
+        MBREC_HIP(hipGetLastError());
+        return MBREC_OK;
+    }
     const size_t n = (size_t)p->P * B;
     hipLaunchKernelGGL(k_filt_axis, dim3(nblocks(n)), dim3(256), 0, s, in, tmp, p->d_kz, p->Kz, p->Nz, p->Nx, B, 0,
                        transp ? 1 : 0, 0, Bp);
@@ -1319,7 +2003,7 @@ int autotune(Plan *p, int B, int reps, hipStream_t s, int *gather_out, int *scat
     // scatter: record-lane LDS batch chunks, then batch-lane lane groups
     std::vector<int> cand;
     if (p->pull) {
-        for (int c : {1, 4, 8, 16, -16, -32, -64, -116, -132, -164, -216, -232, -264})
+        for (int c : {1, 4, 8, 16, -16, -32, -64, -116, -132, -164, -216, -232, -264, -364, -464, -564})
             if (pull_choice_ok(c, Bp) && !(c == 1 && Bp > 16)) cand.push_back(c);
     } else {
         const int mx = scatter_chunk_max(p, Bp);
@@ -1351,6 +2035,10 @@ int autotune(Plan *p, int B, int reps, hipStream_t s, int *gather_out, int *scat
 int set_launch(Plan *p, int B, int gather, int scatter)
 {
     const int Bp = batch_stride(B);
+    if (gather >= 0) {   // variant validity can depend on the workspace (partial images)
+        const int rc = ensure_work(p, B);
+        if (rc) return rc;
+    }
     if (gather >= kNumGatherVariants || (gather >= 0 && !gather_variant_ok(p, gather, Bp)))
         return set_error(MBREC_ERR_ARG, "mbrec:args", "gather variant %d is not valid for a batch of %d", gather, B);
     if (p->pull) {
