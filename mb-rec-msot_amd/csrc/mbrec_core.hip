@@ -78,7 +78,7 @@ int ensure_work(Plan *p, int B)
     const int Bp = batch_stride(B);
     size_t acc = 0;
     int rc;
-    if ((rc = dalloc(&p->d_bins, (size_t)p->D * p->geom.NB * Bp, &acc)) ||
+    if ((rc = dalloc(&p->d_bins, ((size_t)p->D * p->geom.NB + 2) * Bp, &acc)) ||
         (rc = dalloc(&p->d_hpart, need_h * Bp, nullptr)) ||
         (rc = dalloc(&p->d_imgI, (size_t)p->P * Bp, &acc)) ||
         (rc = dalloc(&p->d_img_tmp, (size_t)2 * B * p->P, &acc)) ||
@@ -87,6 +87,8 @@ int ensure_work(Plan *p, int B)
         free_work(p);
         return rc;
     }
+    // the two zero rows after the last detector's bins (pairs outside the bin range read them)
+    MBREC_HIP(hipMemset(p->d_bins + (size_t)p->D * p->geom.NB * Bp, 0, sizeof(float) * 2 * Bp));
     p->bcap = B;
     p->work_bytes = need_h * Bp * sizeof(float);   // hpart size (reallocated when the window grows)
     return MBREC_OK;
@@ -331,6 +333,37 @@ static void make_pixel_order(Plan *p, std::vector<int32_t> &slot_pix)
                         }
 }
 
+// Decoded pairs, slot-major: entry (slot, d) = (bin row d * NB + j, u) of the
+// pair's first bin j, u exactly as decode_tau computes it; pairs outside the
+// bin range point at the zero row D * NB (the bins buffer carries two zero rows
+// after the last detector), so a gather reads them without a test.
+static int build_pair_table(Plan *p, const std::vector<int32_t> &tq)
+{
+    const BinGeom &g = p->geom;
+    const int span = g.nT + 2 * g.K;
+    const float scale = std::ldexp(1.0f, -g.F);
+    std::vector<int2> t((size_t)p->Pd * p->D + kRecPad, make_int2(0, 0));   // zero pad for run prefetches
+    for (int d = 0; d < p->D; ++d)
+        for (int sl = 0; sl < p->Pd; ++sl) {
+            const int32_t q = tq[(size_t)d * p->Pd + sl];
+            int nn, m;
+            host_decode(q, g, nn, m);
+            int2 e = make_int2(p->D * g.NB, 0);
+            if ((unsigned)nn < (unsigned)span) {
+                const float fr = (float)(q & ((1 << g.F) - 1)) * scale;
+                const float u = m == 0 ? fr * g.iw0 : (m == 1 ? (fr - g.phi1) * g.iw1 : (fr - g.phi2) * g.iw2);
+                int32_t ub;
+                std::memcpy(&ub, &u, sizeof(ub));
+                e = make_int2(d * g.NB + 3 * nn + m, ub);
+            }
+            t[(size_t)sl * p->D + d] = e;
+        }
+    int rc;
+    if ((rc = dalloc(&p->d_ptab, t.size(), &p->table_bytes)) || (rc = upload(p->d_ptab, t.data(), t.size())))
+        return rc;
+    return MBREC_OK;
+}
+
 // Bin window of every (256-slot tile, detector): first bin and length covering
 // bins j, j + 1 of all its pairs (length 0: no pair in range); kTileWinMax bounds
 // what the tile gather stages in LDS.
@@ -357,8 +390,30 @@ static int build_tile_windows(Plan *p, const std::vector<int32_t> &tq)
             if (lo <= hi) wmax = std::max(wmax, hi - lo + 1);
         }
     p->tile_wmax = wmax;
+    // decoded pairs per (tile, detector): row inside the tile's window, or the
+    // zero row kTileZeroRow for pairs outside the bin range; u exactly as decode_tau
+    const float scale = std::ldexp(1.0f, -g.F);
+    std::vector<int2> tt((size_t)ntile * p->D * 256);
+    for (int t = 0; t < ntile; ++t)
+        for (int d = 0; d < p->D; ++d) {
+            const int lo = w[(size_t)t * p->D + d].x;
+            const int32_t *q = tq.data() + (size_t)d * p->Pd + (size_t)t * 256;
+            int2 *e = tt.data() + ((size_t)t * p->D + d) * 256;
+            for (int k = 0; k < 256; ++k) {
+                int nn, m;
+                host_decode(q[k], g, nn, m);
+                e[k] = make_int2(kTileZeroRow, 0);
+                if ((unsigned)nn >= (unsigned)span) continue;
+                const float fr = (float)(q[k] & ((1 << g.F) - 1)) * scale;
+                const float u = m == 0 ? fr * g.iw0 : (m == 1 ? (fr - g.phi1) * g.iw1 : (fr - g.phi2) * g.iw2);
+                int32_t ub;
+                std::memcpy(&ub, &u, sizeof(ub));
+                e[k] = make_int2(3 * nn + m - lo, ub);
+            }
+        }
     int rc;
-    if ((rc = dalloc(&p->d_twin, w.size(), &p->table_bytes)) || (rc = upload(p->d_twin, w.data(), w.size())))
+    if ((rc = dalloc(&p->d_twin, w.size(), &p->table_bytes)) || (rc = upload(p->d_twin, w.data(), w.size())) ||
+        (rc = dalloc(&p->d_ttab, tt.size(), &p->table_bytes)) || (rc = upload(p->d_ttab, tt.data(), tt.size())))
         return rc;
     return MBREC_OK;
 }
@@ -366,7 +421,7 @@ static int build_tile_windows(Plan *p, const std::vector<int32_t> &tq)
 static void free_plan(Plan *p)
 {
     void *ptrs[] = {p->d_pix, p->d_tq, p->d_sidx, p->d_sir, p->d_swin, p->d_ir, p->d_irf,
-                    p->d_kz,  p->d_kx, p->d_rec, p->d_rec_off, p->d_rec_k, p->d_twin};
+                    p->d_kz,  p->d_kx, p->d_rec, p->d_rec_off, p->d_rec_k, p->d_twin, p->d_ptab, p->d_ttab};
     for (void *q : ptrs)
         if (q) (void)hipFree(q);
     free_work(p);
@@ -433,6 +488,7 @@ int mbrec_plan_create(mbrec_plan *out, int Nz, int Nx, int D, int nT, double R,
         (rc = dalloc(&p->d_tq, (size_t)D * p->Pd, &p->table_bytes)) ||
         (rc = upload(p->d_pix, slot_pix.data(), slot_pix.size())) ||
         (rc = upload(p->d_tq, tq.data(), tq.size())) || (rc = build_tile_windows(p, tq)) ||
+        (rc = build_pair_table(p, tq)) ||
         (rc = build_records(p, slot_pix, tq, nullptr, nullptr, 0))) {
         free_plan(p);
         delete h;

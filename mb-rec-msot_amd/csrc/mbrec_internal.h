@@ -35,6 +35,7 @@ struct HatTaps {
 };
 
 // Decoding of a fixed-point travel time into (bin index, interpolation weight).
+constexpr int kTileZeroRow = 256;   // tile gather: LDS window rows [0, 256), then two zero rows
 constexpr int kRecPad = 64;   // zero records after the pull table (runs of up to 64 read past the end)
 
 struct BinGeom {
@@ -60,8 +61,10 @@ struct Plan {
     int Pd = 0;                  // device pixel slots (tile-padded)
     int32_t *d_pix = nullptr;    // [Pd] slot -> pixel index or -1
     int32_t *d_tq = nullptr;     // [D][Pd] fixed-point travel times (transpose gather)
+    int2 *d_ptab = nullptr;      // [Pd][D] decoded pairs (bin row d*NB + j or the zero row D*NB, u bits)
     int2 *d_twin = nullptr;      // [Pd / 256][D] bin window (first bin, length) of each 256-slot tile
     int tile_wmax = 0;           // longest tile window
+    int2 *d_ttab = nullptr;      // [Pd / 256][D][256] (bin row inside the tile window, or kTileZeroRow; u bits)
     BinGeom geom{};
     HatTaps taps{};
     // forward scatter: per detector, (pixel, travel time) records grouped into buckets

@@ -54,6 +54,10 @@ __device__ __forceinline__ void kahan_add(float &s, float &c, float v)
     s = t;
 }
 
+// Word 3 of a raw buffer resource on CDNA (32-bit data format, no swizzle);
+// reads past num_records return 0.
+constexpr int kBufferWord3 = 0x00020000;
+
 // Staging of n elements into LDS with U global loads in flight per thread
 // before any is stored (a plain strided loop waits out one load latency per
 // element).  ld(e) returns element e, st(e, v) stores it; NT threads take part.
@@ -160,7 +164,7 @@ __global__ __launch_bounds__(256) void k_filt_z(const float *__restrict__ in, fl
     for (int i = threadIdx.x; i < 2 * K + 1; i += blockDim.x) tk[i] = (double)k[i];
     const int W = Nz + 2 * K;
     const float invW = 1.0f / (float)W;
-    stage_batched<8, 256>(
+    stage_batched<16, 256>(
         threadIdx.x, kFzCols * W,
         [&](int e) {
             int c, i;
@@ -219,7 +223,7 @@ __global__ __launch_bounds__(256) void k_filt_x(const float *__restrict__ in, fl
     const int r = threadIdx.x % kFxRows;
     const bool rv = iz0 + r < Nz;
     // element e = i * kFxRows + r: row i of the wrapped x axis, z row iz0 + r
-    stage_batched<8, 256>(
+    stage_batched<16, 256>(
         threadIdx.x, (Nx + 2 * K) * kFxRows,
         [&](int e) {
             const int i = e / kFxRows, rr = e % kFxRows;
@@ -564,7 +568,7 @@ __global__ __launch_bounds__(256) void k_z_to_bins(const float *__restrict__ z, 
 // bin-row stores and loads are BC-float runs; integer work per element is
 // compile-time (no runtime divisions), and the taps sit in LDS.  Same sums in
 // the same order as k_z_to_bins / k_bins_to_y.
-template <int BC>
+template <int BC, int NW>
 __global__ __launch_bounds__(256) void k_z_to_bins_bc(const float *__restrict__ z, float *__restrict__ bins,
                                                       HatTaps taps, BinGeom g, int D, int B, int Bp,
                                                       const int2 *__restrict__ win)
@@ -573,7 +577,7 @@ __global__ __launch_bounds__(256) void k_z_to_bins_bc(const float *__restrict__ 
     float *tw = sm;                        // [3][nw]
     float *zsm = sm + 3 * MBREC_MAX_TAPS;  // [BC][nzp]
     const int d = blockIdx.y, b0 = blockIdx.z * BC;
-    const int nT = g.nT, NB = g.NB, nw = taps.nw;
+    const int nT = g.nT, NB = g.NB, nw = NW ? NW : taps.nw;   // NW > 0: taps per phase known at compile time
     const int j0 = blockIdx.x * kBinTile, j1 = min(NB, j0 + kBinTile);
     const int nlo = j0 / 3 - g.K, nhi = (j1 - 1) / 3 - g.K;
     const int zlo = nlo + taps.kmin;
@@ -587,7 +591,7 @@ __global__ __launch_bounds__(256) void k_z_to_bins_bc(const float *__restrict__ 
     }
     __syncthreads();
     const float invz = 1.0f / (float)nz;
-    stage_batched<8, 256>(
+    stage_batched<16, 256>(
         threadIdx.x, BC * nz,
         [&](int e) {
             int b, q;
@@ -607,14 +611,29 @@ __global__ __launch_bounds__(256) void k_z_to_bins_bc(const float *__restrict__ 
     float *brow = bins + (size_t)d * NB * Bp + bg;
     const float *zb = zsm + b * nzp;
     const int ns = nhi - nlo + 1;
+    float wr[3 * (NW ? NW : 1)];   // NW > 0: the taps in registers
+    if (NW) {
+#pragma unroll
+        for (int i = 0; i < 3 * NW; ++i) wr[i] = tw[i];
+    }
     for (int k = threadIdx.x / BC; k < ns; k += 256 / BC) {
         const float *zs = zb + k;   // z[n + kmin + q] for n = nlo + k
         float a0 = 0.0f, a1 = 0.0f, a2 = 0.0f;
-        for (int q = 0; q < nw; ++q) {
-            const float v = zs[q];
-            a0 = fmaf(tw[q], v, a0);
-            a1 = fmaf(tw[nw + q], v, a1);
-            a2 = fmaf(tw[2 * nw + q], v, a2);
+        if (NW) {
+#pragma unroll
+            for (int q = 0; q < (NW ? NW : 1); ++q) {
+                const float v = zs[q];
+                a0 = fmaf(wr[q], v, a0);
+                a1 = fmaf(wr[NW + q], v, a1);
+                a2 = fmaf(wr[2 * NW + q], v, a2);
+            }
+        } else {
+            for (int q = 0; q < nw; ++q) {
+                const float v = zs[q];
+                a0 = fmaf(tw[q], v, a0);
+                a1 = fmaf(tw[nw + q], v, a1);
+                a2 = fmaf(tw[2 * nw + q], v, a2);
+            }
         }
         const int j = j0 + 3 * k;
         brow[(size_t)j * Bp] = a0;
@@ -623,7 +642,7 @@ __global__ __launch_bounds__(256) void k_z_to_bins_bc(const float *__restrict__ 
     }
 }
 
-template <int BC, int TY, bool DIRECT>
+template <int BC, int TY, bool DIRECT, int NW>
 __global__ __launch_bounds__(256) void k_bins_to_y_bc(
     const float *__restrict__ hpart, float *__restrict__ y, HatTaps taps, BinGeom g, int Wb, int nbk, int lo_off,
     int WL, int Bp, int D, int B, float scale, const int2 *__restrict__ win)
@@ -632,21 +651,40 @@ __global__ __launch_bounds__(256) void k_bins_to_y_bc(
     float *tw = sm;
     float *bsm = sm + 3 * MBREC_MAX_TAPS;   // [BC][njp]
     const int d = blockIdx.y, b0 = blockIdx.z * BC;
-    const int nT = g.nT, nw = taps.nw;
+    const int nT = g.nT, nw = NW ? NW : taps.nw;   // NW > 0: taps per phase known at compile time
     const int t0 = blockIdx.x * TY, t1 = min(nT, t0 + TY);
     const int kmax = taps.kmin + nw - 1;
     const int jlo = 3 * (t0 - kmax + g.K);
     const int jhi = 3 * (t1 - 1 - taps.kmin + g.K) + 2;
     const int nj = jhi - jlo + 1, njp = nj | 1;
     for (int i = threadIdx.x; i < 3 * nw; i += 256) tw[i] = taps.w[i / nw][i % nw];
-    if (DIRECT) {   // element e = jj * BC + b
-        stage_batched<8, 256>(
-            threadIdx.x, nj * BC,
-            [&](int e) {
-                const int jj = e / BC, b = e % BC;
-                return b0 + b < Bp ? hpart[((size_t)d * g.NB + jlo + jj) * Bp + b0 + b] : 0.0f;
-            },
-            [&](int e, float v) { bsm[(e % BC) * njp + e / BC] = v; });
+    int2 *wl = reinterpret_cast<int2 *>(bsm + BC * njp);   // [BC] sample window per sinogram
+    for (int b = threadIdx.x; b < BC; b += 256)
+        wl[b] = (win && b0 + b < B) ? win[b0 + b] : make_int2(0, nT);
+    if (DIRECT) {   // 16-byte element e = jj * (BC / 4) + c: sinograms b0 + 4c .. b0 + 4c + 3 of bin row jlo + jj
+        constexpr int C4 = BC / 4, U = 8;
+        const int n4 = nj * C4;
+        for (int e0 = threadIdx.x; e0 < n4; e0 += 256 * U) {
+            float4 v[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int e = e0 + 256 * u, jj = e / C4, c = e % C4;
+                v[u] = (e < n4 && b0 + 4 * c < Bp)
+                           ? *reinterpret_cast<const float4 *>(hpart + ((size_t)d * g.NB + jlo + jj) * Bp + b0 + 4 * c)
+                           : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int e = e0 + 256 * u, jj = e / C4, c = e % C4;
+                if (e < n4) {
+                    float *q = bsm + 4 * c * njp + jj;
+                    q[0] = v[u].x;
+                    q[njp] = v[u].y;
+                    q[2 * njp] = v[u].z;
+                    q[3 * njp] = v[u].w;
+                }
+            }
+        }
     } else {
         const int b = threadIdx.x % BC;
         for (int jj = threadIdx.x / BC; jj < nj; jj += 256 / BC) {
@@ -663,23 +701,29 @@ __global__ __launch_bounds__(256) void k_bins_to_y_bc(
     }
     __syncthreads();
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    float wr[3 * (NW ? NW : 1)];   // NW > 0: the taps in registers
+    if (NW) {
+#pragma unroll
+        for (int i = 0; i < 3 * NW; ++i) wr[i] = tw[i];
+    }
     for (int b = wv; b < BC; b += 4) {
         const int bg = b0 + b;
         if (bg >= B) break;
         const float *bs = bsm + b * njp;
-        int lo = 0, hi = nT;
-        if (win) {
-            const int2 wb = win[bg];
-            lo = wb.x;
-            hi = wb.y;
-        }
+        const int lo = wl[b].x, hi = wl[b].y;
         float *yr = y + ((size_t)bg * D + d) * nT;
         for (int r = t0 + lane; r < t1; r += 64) {
             float acc = 0.0f;
+            const float *bj = bs + 3 * (r - taps.kmin + g.K) - jlo;
+            if (NW) {
 #pragma unroll
-            for (int m = 0; m < 3; ++m) {
-                const int jb = 3 * (r - taps.kmin + g.K) + m - jlo;
-                for (int q = 0; q < nw; ++q) acc = fmaf(tw[m * nw + q], bs[jb - 3 * q], acc);
+                for (int m = 0; m < 3; ++m)
+#pragma unroll
+                    for (int q = 0; q < (NW ? NW : 1); ++q) acc = fmaf(wr[m * NW + q], bj[m - 3 * q], acc);
+            } else {
+#pragma unroll
+                for (int m = 0; m < 3; ++m)
+                    for (int q = 0; q < nw; ++q) acc = fmaf(tw[m * nw + q], bj[m - 3 * q], acc);
             }
             if (r < lo || r >= hi) acc = 0.0f;
             yr[r] = scale * acc;
@@ -1133,6 +1177,86 @@ __global__ __launch_bounds__(256) void k_fwd_pull_wu(const int2 *__restrict__ re
     }
 }
 
+// Wave-uniform pull with raw-buffer image-row loads: records are read with
+// scalar loads, each image row at a scalar byte offset (no per-pair address
+// arithmetic in vector registers); full runs of U records carry no tests and
+// the tail of a bin runs once more with weights zeroed past its end.  The image
+// is read as a raw buffer with 32-bit offsets: P * Bp floats below 2^31 bytes
+// (checked by the launcher).
+template <int U>
+__global__ __launch_bounds__(256) void k_fwd_pull_wt(const int2 *__restrict__ rec, const int32_t *__restrict__ off,
+                                                     const float *__restrict__ imgI, float *__restrict__ bins, int NB,
+                                                     int Bp, int JR, int xcd_map /* D / 8, or 0 */, int img_bytes)
+{
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    int d, jblk;
+    if (xcd_map) {   // XCD x (= blockIdx.x % 8) takes detectors [x * D/8, (x + 1) * D/8)
+        const int x = blockIdx.x % 8, sidx = blockIdx.x / 8;
+        d = x * xcd_map + sidx % xcd_map;
+        jblk = sidx / xcd_map;
+    } else {
+        d = blockIdx.y;
+        jblk = blockIdx.x;
+    }
+    const int ja = __builtin_amdgcn_readfirstlane((jblk * 4 + wv) * JR);
+    if (ja >= NB) return;   // no barriers in this kernel
+    const int jb = min(NB, ja + JR);
+    const int b = blockIdx.z * 64 + lane;
+    const int vo = 4 * min(b, Bp - 1);   // lanes past Bp compute a valid column and store nothing
+    const int rowb = 4 * Bp;
+    const __amdgpu_buffer_rsrc_t rs =
+        __builtin_amdgcn_make_buffer_rsrc(const_cast<float *>(imgI), (short)0, img_bytes, kBufferWord3);
+    const int32_t *o = off + (size_t)d * (NB + 1);
+    float *dst = bins + (size_t)d * NB * Bp + b;
+    auto ld = [&](int px) {
+        return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rs, vo, px * rowb, 0));
+    };
+    float prev = 0.0f;
+    if (ja > 0)   // u-part of bin ja - 1
+        for (int r = o[ja - 1], re = o[ja]; r < re; ++r) {
+            const int2 e = rec[r];
+            prev = fmaf(__int_as_float(e.y), ld(e.x), prev);
+        }
+    int r0 = o[ja];
+    for (int j = ja; j < jb; ++j) {
+        const int r1 = o[j + 1];
+        float s0 = 0.0f, s1 = 0.0f;
+        int r = r0;
+        for (; r + U <= r1; r += U) {
+            int2 e[U];
+            float cv[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) e[u] = rec[r + u];
+#pragma unroll
+            for (int u = 0; u < U; ++u) cv[u] = ld(e[u].x);
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const float uu = __int_as_float(e[u].y);
+                s0 = fmaf(1.0f - uu, cv[u], s0);
+                s1 = fmaf(uu, cv[u], s1);
+            }
+        }
+        if (r < r1) {   // tail: records past r1 (next bin's, or the zero pad) get weight 0
+            int2 e[U];
+            float cv[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) e[u] = rec[r + u];
+#pragma unroll
+            for (int u = 0; u < U; ++u) cv[u] = ld(e[u].x);
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const bool in = r + u < r1;
+                const float uu = in ? __int_as_float(e[u].y) : 0.0f;
+                s0 = fmaf(in ? 1.0f - uu : 0.0f, cv[u], s0);
+                s1 = fmaf(uu, cv[u], s1);
+            }
+        }
+        if (b < Bp) dst[(size_t)j * Bp] = s0 + prev;
+        prev = s1;
+        r0 = r1;
+    }
+}
+
 // ---------------------------------------------------------------------------
 // tile gather (batch-lane, Bp >= 16): one workgroup per 16 x 16 pixel tile and
 // chunk of LB sinograms.  For every detector the tile's bin window (all bins its
@@ -1381,14 +1505,192 @@ __global__ __launch_bounds__(256) void k_gather_wu(
     }
 }
 
-// out[b][x] = scale * sum of the 8 partial images (fixed order)
+// Batch-lane gather from the decoded pair table (lane groups of 64 sinograms,
+// one pixel slot per wave): the (bin row, u) of every pair of the wave's slot
+// is read with scalar loads, so a pair costs two bin-row loads at a scalar base
+// address and three vector operations; pairs outside the bin range read the
+// zero rows.  XG: detector groups per XCD and partial images as k_gather_wu.
+// Bin rows are read as a raw buffer with 32-bit byte offsets: the bins buffer,
+// (D * NB + 2) * Bp floats, stays below 2^31 bytes (checked by the launcher).
+template <int U, bool XG>
+__global__ __launch_bounds__(256) void k_gather_wt(
+    const int2 *__restrict__ ptab, const int32_t *__restrict__ pix, const float *__restrict__ bins,
+    float *__restrict__ out, int Pd, int P, int D, int B, int Bp, int nsb, float scale, int bins_bytes)
+{
+    const int lane = threadIdx.x & 63;
+    int grp = 0, sb, bc;
+    if (XG) {
+        grp = blockIdx.x % 8;
+        const int s = blockIdx.x / 8;
+        sb = s % nsb;
+        bc = s / nsb;
+    } else {
+        sb = blockIdx.x;
+        bc = blockIdx.y;
+    }
+    const int slot = __builtin_amdgcn_readfirstlane(sb * 4 + (threadIdx.x >> 6));
+    const int b = bc * 64 + lane;
+    // byte offsets of the lane's column in a bin row and the row after it (lanes
+    // past Bp read a valid column; their sums are dropped)
+    const int vo0 = 4 * min(b, Bp - 1), vo1 = vo0 + 4 * Bp;
+    const int rowb = 4 * Bp;
+    const __amdgpu_buffer_rsrc_t rs =
+        __builtin_amdgcn_make_buffer_rsrc(const_cast<float *>(bins), (short)0, bins_bytes, kBufferWord3);
+    const int DG = XG ? (D + 7) / 8 : D;
+    const int dbeg = grp * DG, dend = min(D, dbeg + DG);
+    const int2 *tr = ptab + (size_t)slot * D;
+    float acc = 0.0f, cmp = 0.0f;
+    int d0 = dbeg;
+    int2 e[U];   // table entries of the current run; the next run's are read while its rows load
+#pragma unroll
+    for (int u = 0; u < U; ++u) e[u] = tr[d0 + u];   // the table is padded: reads past a row stay inside
+    for (; d0 + U <= dend; d0 += U) {
+        float g0[U], g1[U], uu[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int so = e[u].x * rowb;
+            g0[u] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rs, vo0, so, 0));
+            g1[u] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rs, vo1, so, 0));
+            uu[u] = __int_as_float(e[u].y);
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) e[u] = tr[d0 + U + u];
+        float part = 0.0f;
+#pragma unroll
+        for (int u = 0; u < U; ++u) part += fmaf(uu[u], g1[u] - g0[u], g0[u]);
+        kahan_add(acc, cmp, part);
+    }
+    for (; d0 < dend; ++d0) {   // detectors past the last full run
+        const int2 e = tr[d0];
+        const int so = e.x * rowb;
+        const float g0 = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rs, vo0, so, 0));
+        const float g1 = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rs, vo1, so, 0));
+        kahan_add(acc, cmp, fmaf(__int_as_float(e.y), g1 - g0, g0));
+    }
+    const int px = pix[slot];
+    if (px >= 0 && b < B) {
+        if (XG)
+            out[((size_t)grp * B + b) * P + px] = acc - cmp;
+        else
+            out[(size_t)b * P + px] = scale * (acc - cmp);
+    }
+}
+
+// Tile gather from LDS, table-driven: one workgroup of 16 waves per 256-slot
+// tile, chunk of 64 sinograms (lane = sinogram) and detector group.  For each
+// detector the tile's bin window (rows [lo, lo + len) of 64 sinograms) is
+// staged in one of two LDS buffers while the other is read: every thread holds
+// its share of the next detector's window in registers across the current
+// detector's work, so there is one barrier per detector.  Wave w interpolates
+// the tile's pixels 16w .. 16w + 15: its (row, u) pairs come from the tile
+// table with scalar loads; out-of-range pairs read two zero rows at index WMAX.
+// Sums: over runs of 8 detectors, then of the runs (pairwise-like, two levels).  G > 1: the
+// detector groups write partial images [G][B][P], summed by k_sum_parts.
+template <int G>
+__global__ __launch_bounds__(1024) void k_gather_tt(
+    const int2 *__restrict__ ttab, const int2 *__restrict__ twin, const int32_t *__restrict__ pix,
+    const float *__restrict__ bins, float *__restrict__ out, int D, int NB, int P, int B, int Bp, float scale,
+    int bins_bytes)
+{
+    extern __shared__ float sm[];   // 2 buffers x (kTileZeroRow + 2) rows x 64 floats
+    constexpr int RS = kTileZeroRow + 2;
+    const int tile = blockIdx.x, grp = blockIdx.y % G, b0 = (blockIdx.y / G) * 64;
+    const int lane = threadIdx.x & 63, wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int DG = (D + G - 1) / G, dbeg = grp * DG, dend = min(D, dbeg + DG);
+    for (int i = threadIdx.x; i < 2 * 2 * 64; i += 1024)   // the zero rows of both buffers
+        sm[(i / 128) * RS * 64 + (kTileZeroRow + (i / 64) % 2) * 64 + i % 64] = 0.0f;
+    // window staging: thread t moves 16-byte pieces f = t + 1024 k (row f / 16, column
+    // 4 (f % 16)) of the window's rows [lo, lo + len); columns past Bp carry the next
+    // row's values and are never used (raw-buffer reads past the end return 0)
+    const __amdgpu_buffer_rsrc_t rs =
+        __builtin_amdgcn_make_buffer_rsrc(const_cast<float *>(bins), (short)0, bins_bytes, kBufferWord3);
+    const int rowb = 4 * Bp;
+    int voff[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const int f = threadIdx.x + 1024 * k;
+        voff[k] = (f >> 4) * rowb + 4 * (b0 + 4 * (f & 15));
+    }
+    float4 pf[4];
+    int plen = 0;   // rows of the window held in pf
+    auto fetch = [&](int d) {
+        const int2 w = twin[(size_t)tile * D + d];
+        const int so = (d * NB + w.x) * rowb;
+        plen = w.y;
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+            if (((threadIdx.x + 1024 * k) >> 4) < plen)
+                pf[k] = __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(rs, voff[k], so, 0));
+    };
+    auto store = [&](int buf) {
+        float *dstb = sm + buf * RS * 64;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const int f = threadIdx.x + 1024 * k;
+            if ((f >> 4) < plen) *reinterpret_cast<float4 *>(dstb + (f >> 4) * 64 + 4 * (f & 15)) = pf[k];
+        }
+    };
+    float acc[16], part[16];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) acc[k] = part[k] = 0.0f;
+    const int2 *tt = ttab + (size_t)tile * D * 256 + 16 * wv;
+    if (dbeg < dend) {
+        fetch(dbeg);
+        store(0);
+        __syncthreads();
+        if (dbeg + 1 < dend) fetch(dbeg + 1);
+        int2 ec[16];   // this wave's table entries for the current detector (scalar registers)
+#pragma unroll
+        for (int k = 0; k < 16; ++k) ec[k] = tt[(size_t)dbeg * 256 + k];
+        for (int d = dbeg; d < dend; ++d) {
+            const int cur = (d - dbeg) & 1;
+            const float *wb = sm + cur * RS * 64 + lane;
+            int2 en[16];   // next detector's entries, read while this one is interpolated
+            const int dn = min(d + 1, dend - 1);
+#pragma unroll
+            for (int k = 0; k < 16; ++k) en[k] = tt[(size_t)dn * 256 + k];
+#pragma unroll
+            for (int k = 0; k < 16; ++k) {
+                const float g0 = wb[ec[k].x * 64], g1 = wb[ec[k].x * 64 + 64];
+                part[k] += fmaf(__int_as_float(ec[k].y), g1 - g0, g0);
+            }
+#pragma unroll
+            for (int k = 0; k < 16; ++k) ec[k] = en[k];
+            if (((d - dbeg) & 7) == 7) {
+#pragma unroll
+                for (int k = 0; k < 16; ++k) {
+                    acc[k] += part[k];
+                    part[k] = 0.0f;
+                }
+            }
+            if (d + 1 < dend) store(cur ^ 1);
+            __syncthreads();
+            if (d + 2 < dend) fetch(d + 2);
+        }
+    }
+    const int b = b0 + lane;
+    if (b >= B) return;
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+        const float v = acc[k] + part[k];
+        const int px = pix[(size_t)tile * 256 + 16 * wv + k];
+        if (px < 0) continue;
+        if (G > 1)
+            out[((size_t)grp * B + b) * P + px] = v;
+        else
+            out[(size_t)b * P + px] = scale * v;
+    }
+}
+
+// out[b][x] = scale * sum of the NP partial images (fixed order)
+template <int NP>
 __global__ __launch_bounds__(256) void k_sum_parts(const float *__restrict__ part, float *__restrict__ out,
                                                    size_t n, float scale)
 {
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
         float s = 0.0f;
 #pragma unroll
-        for (int k = 0; k < 8; ++k) s += part[(size_t)k * n + i];
+        for (int k = 0; k < NP; ++k) s += part[(size_t)k * n + i];
         out[i] = scale * s;
     }
 }
@@ -1451,15 +1753,17 @@ static int scatter_choice(const Plan *p, int Bp)
 // pull forward choice: > 0 bins per thread pass BC (1, 4, 8, 16); < 0 batch-lane,
 // -(LB + 100 * mode) with lane groups LB (16, 32, 64) and mode 0: plain order,
 // 8 pairs in flight; 1: XCD-aware order, 8 in flight; 2: XCD-aware, 16 in flight
-static bool pull_choice_ok(int c, int Bp)
+static bool pull_choice_ok(const Plan *p, int c, int Bp)
 {
     if (c > 0) return c == 1 || ((c == 4 || c == 8 || c == 16) && Bp % c == 0);
     if (c < 0) {
         const int lb = (-c) % 100, mode = (-c) / 100;
         int cap = 16;
         while (cap < Bp) cap *= 2;
+        if (mode >= 9)   // raw-buffer image rows: 32-bit byte offsets
+            return Bp >= 16 && lb == 64 && mode <= 11 && sizeof(float) * (size_t)p->P * Bp < ((size_t)1 << 31);
         if (mode >= 3)   // wave-uniform runs: lane groups of 64 only
-            return Bp >= 16 && lb == 64 && mode <= 5;
+            return Bp >= 16 && lb == 64 && mode <= 8;
         return Bp >= 16 && (lb == 16 || lb == 32 || lb == 64) && lb <= cap && mode <= 2;
     }
     return false;
@@ -1467,7 +1771,7 @@ static bool pull_choice_ok(int c, int Bp)
 
 static int default_pull_choice(int Bp)
 {
-    if (Bp >= 64) return -564;
+    if (Bp >= 64) return -964;
     if (Bp >= 32) return -32;
     if (Bp >= 16) return -16;
     if (Bp % 8 == 0) return 8;
@@ -1504,25 +1808,37 @@ static void fwd_pull_bl(Plan *p, const float *imgI, int Bp, bool xcd, hipStream_
 }
 
 template <int U>
-static void fwd_pull_wu(Plan *p, const float *imgI, int Bp, bool xcd, hipStream_t s)
+static void fwd_pull_wu(Plan *p, const float *imgI, int Bp, bool xcd, hipStream_t s, int JR = kPullRange)
 {
-    const int ngrp = (p->geom.NB + kPullRange - 1) / kPullRange;
+    const int ngrp = (p->geom.NB + JR - 1) / JR;
     const int nblk = (ngrp + 3) / 4;
     if (xcd && p->D % 8 == 0) {
         dim3 grid(nblk * p->D, 1, (Bp + 63) / 64);
         hipLaunchKernelGGL((k_fwd_pull_wu<U>), grid, dim3(256), 0, s, p->d_rec, p->d_rec_off, imgI, p->d_bins,
-                           p->geom.NB, Bp, kPullRange, p->D / 8);
+                           p->geom.NB, Bp, JR, p->D / 8);
         return;
     }
     dim3 grid(nblk, p->D, (Bp + 63) / 64);
     hipLaunchKernelGGL((k_fwd_pull_wu<U>), grid, dim3(256), 0, s, p->d_rec, p->d_rec_off, imgI, p->d_bins,
-                       p->geom.NB, Bp, kPullRange, 0);
+                       p->geom.NB, Bp, JR, 0);
+}
+
+template <int U>
+static void fwd_pull_wt(Plan *p, const float *imgI, int Bp, hipStream_t s, int JR)
+{
+    const int ngrp = (p->geom.NB + JR - 1) / JR;
+    const int nblk = (ngrp + 3) / 4;
+    const int bytes = (int)(sizeof(float) * (size_t)p->P * Bp);
+    const int xm = p->D % 8 == 0 ? p->D / 8 : 0;
+    const dim3 grid = xm ? dim3(nblk * p->D, 1, (Bp + 63) / 64) : dim3(nblk, p->D, (Bp + 63) / 64);
+    hipLaunchKernelGGL((k_fwd_pull_wt<U>), grid, dim3(256), 0, s, p->d_rec, p->d_rec_off, imgI, p->d_bins,
+                       p->geom.NB, Bp, JR, xm, bytes);
 }
 
 static int launch_pull(Plan *p, const float *imgI, int Bp, hipStream_t s)
 {
     int c = (p->tune.Bp == Bp && p->tune.scatter_bc != 0) ? p->tune.scatter_bc : default_pull_choice(Bp);
-    if (!pull_choice_ok(c, Bp)) c = default_pull_choice(Bp);
+    if (!pull_choice_ok(p, c, Bp)) c = default_pull_choice(Bp);
     switch (c) {
     case 1: fwd_pull<1>(p, imgI, Bp, s); break;
     case 4: fwd_pull<4>(p, imgI, Bp, s); break;
@@ -1539,7 +1855,13 @@ static int launch_pull(Plan *p, const float *imgI, int Bp, hipStream_t s)
     case -264: fwd_pull_bl<64, 16>(p, imgI, Bp, true, s); break;
     case -364: fwd_pull_wu<8>(p, imgI, Bp, false, s); break;
     case -464: fwd_pull_wu<8>(p, imgI, Bp, true, s); break;
-    default: fwd_pull_wu<16>(p, imgI, Bp, true, s); break;
+    case -564: fwd_pull_wu<16>(p, imgI, Bp, true, s); break;
+    case -664: fwd_pull_wu<16>(p, imgI, Bp, true, s, 16); break;
+    case -764: fwd_pull_wu<16>(p, imgI, Bp, true, s, 64); break;
+    case -864: fwd_pull_wu<8>(p, imgI, Bp, true, s, 16); break;
+    case -964: fwd_pull_wt<16>(p, imgI, Bp, s, 16); break;
+    case -1064: fwd_pull_wt<8>(p, imgI, Bp, s, 16); break;
+    default: fwd_pull_wt<16>(p, imgI, Bp, s, 32); break;
     }
     MBREC_HIP(hipGetLastError());
     return MBREC_OK;
@@ -1580,18 +1902,29 @@ static int launch_scatter(Plan *p, const float *imgI, int B, int Bp, hipStream_t
     return MBREC_OK;
 }
 
+template <int BC, int TY, int NW>
+static void bins_to_y_bc_nw(Plan *p, float *y, int B, int Bp, float scale, hipStream_t s, const int2 *win)
+{
+    const int nj = 3 * (TY + p->taps.nw) + 3;
+    const size_t lds = sizeof(float) * (3 * MBREC_MAX_TAPS + (size_t)BC * (nj | 1) + 2 * BC + 2);
+    dim3 grid((p->nT + TY - 1) / TY, p->D, (Bp + BC - 1) / BC);
+    if (p->pull)
+        hipLaunchKernelGGL((k_bins_to_y_bc<BC, TY, true, NW>), grid, dim3(256), lds, s, p->d_bins, y, p->taps,
+                           p->geom, p->Wb, p->nbk, p->lo_off, p->WL, Bp, p->D, B, scale, win);
+    else
+        hipLaunchKernelGGL((k_bins_to_y_bc<BC, TY, false, NW>), grid, dim3(256), lds, s, p->d_hpart, y, p->taps,
+                           p->geom, p->Wb, p->nbk, p->lo_off, p->WL, Bp, p->D, B, scale, win);
+}
+
 template <int BC, int TY>
 static void bins_to_y_bc(Plan *p, float *y, int B, int Bp, float scale, hipStream_t s, const int2 *win)
 {
-    const int nj = 3 * (TY + p->taps.nw) + 3;
-    const size_t lds = sizeof(float) * (3 * MBREC_MAX_TAPS + (size_t)BC * (nj | 1));
-    dim3 grid((p->nT + TY - 1) / TY, p->D, (Bp + BC - 1) / BC);
-    if (p->pull)
-        hipLaunchKernelGGL((k_bins_to_y_bc<BC, TY, true>), grid, dim3(256), lds, s, p->d_bins, y, p->taps, p->geom,
-                           p->Wb, p->nbk, p->lo_off, p->WL, Bp, p->D, B, scale, win);
-    else
-        hipLaunchKernelGGL((k_bins_to_y_bc<BC, TY, false>), grid, dim3(256), lds, s, p->d_hpart, y, p->taps,
-                           p->geom, p->Wb, p->nbk, p->lo_off, p->WL, Bp, p->D, B, scale, win);
+    switch (p->taps.nw) {   // 2 floor(R) + 3 taps per phase
+    case 5: bins_to_y_bc_nw<BC, TY, 5>(p, y, B, Bp, scale, s, win); break;
+    case 7: bins_to_y_bc_nw<BC, TY, 7>(p, y, B, Bp, scale, s, win); break;
+    case 9: bins_to_y_bc_nw<BC, TY, 9>(p, y, B, Bp, scale, s, win); break;
+    default: bins_to_y_bc_nw<BC, TY, 0>(p, y, B, Bp, scale, s, win); break;
+    }
 }
 
 static int launch_bins_to_y(Plan *p, float *y, int B, int Bp, float scale, hipStream_t s,
@@ -1647,7 +1980,24 @@ static void z_to_bins_bc(Plan *p, const float *z, int B, int Bp, hipStream_t s, 
     const int nz = kBinTile / 3 + p->taps.nw + 2;
     const size_t lds = sizeof(float) * (3 * MBREC_MAX_TAPS + (size_t)BC * (nz | 1) + 2 * BC + 2);
     dim3 grid((p->geom.NB + kBinTile - 1) / kBinTile, p->D, (Bp + BC - 1) / BC);
-    hipLaunchKernelGGL(k_z_to_bins_bc<BC>, grid, dim3(256), lds, s, z, p->d_bins, p->taps, p->geom, p->D, B, Bp, win);
+    switch (p->taps.nw) {   // 2 floor(R) + 3 taps per phase
+    case 5:
+        hipLaunchKernelGGL((k_z_to_bins_bc<BC, 5>), grid, dim3(256), lds, s, z, p->d_bins, p->taps, p->geom, p->D, B,
+                           Bp, win);
+        break;
+    case 7:
+        hipLaunchKernelGGL((k_z_to_bins_bc<BC, 7>), grid, dim3(256), lds, s, z, p->d_bins, p->taps, p->geom, p->D, B,
+                           Bp, win);
+        break;
+    case 9:
+        hipLaunchKernelGGL((k_z_to_bins_bc<BC, 9>), grid, dim3(256), lds, s, z, p->d_bins, p->taps, p->geom, p->D, B,
+                           Bp, win);
+        break;
+    default:
+        hipLaunchKernelGGL((k_z_to_bins_bc<BC, 0>), grid, dim3(256), lds, s, z, p->d_bins, p->taps, p->geom, p->D, B,
+                           Bp, win);
+        break;
+    }
 }
 
 static int launch_z_to_bins(Plan *p, const float *z, int B, int Bp, hipStream_t s, const int2 *win = nullptr)
@@ -1716,7 +2066,7 @@ static void gather_bl_xcd(Plan *p, float *img, int B, int Bp, float scale, hipSt
     hipLaunchKernelGGL((k_gather_bl_xcd<LB, U>), grid, dim3(256), 0, s, p->d_tq, p->d_pix, p->d_bins, p->d_gpart,
                        p->geom, p->Pd, p->P, p->D, B, Bp, nsb);
     const size_t n = (size_t)B * p->P;
-    hipLaunchKernelGGL(k_sum_parts, dim3(std::min<size_t>(4096, (n + 255) / 256)), dim3(256), 0, s,
+    hipLaunchKernelGGL(k_sum_parts<8>, dim3(std::min<size_t>(4096, (n + 255) / 256)), dim3(256), 0, s,
                        (const float *)p->d_gpart, img, n, scale);
 }
 
@@ -1729,7 +2079,37 @@ static void gather_wu(Plan *p, float *img, int B, int Bp, float scale, hipStream
                        XG ? p->d_gpart : img, p->geom, p->Pd, p->P, p->D, B, Bp, nsb, scale);
     if (XG) {
         const size_t n = (size_t)B * p->P;
-        hipLaunchKernelGGL(k_sum_parts, dim3(std::min<size_t>(4096, (n + 255) / 256)), dim3(256), 0, s,
+        hipLaunchKernelGGL(k_sum_parts<8>, dim3(std::min<size_t>(4096, (n + 255) / 256)), dim3(256), 0, s,
+                           (const float *)p->d_gpart, img, n, scale);
+    }
+}
+
+template <int U, bool XG>
+static void gather_wt(Plan *p, float *img, int B, int Bp, float scale, hipStream_t s)
+{
+    const int nsb = p->Pd / 4, nbc = (Bp + 63) / 64;
+    const dim3 grid = XG ? dim3(8 * nsb * nbc) : dim3(nsb, nbc);
+    const int bytes = (int)(sizeof(float) * ((size_t)p->D * p->geom.NB + 2) * Bp);
+    hipLaunchKernelGGL((k_gather_wt<U, XG>), grid, dim3(256), 0, s, p->d_ptab, p->d_pix, p->d_bins,
+                       XG ? p->d_gpart : img, p->Pd, p->P, p->D, B, Bp, nsb, scale, bytes);
+    if (XG) {
+        const size_t n = (size_t)B * p->P;
+        hipLaunchKernelGGL(k_sum_parts<8>, dim3(std::min<size_t>(4096, (n + 255) / 256)), dim3(256), 0, s,
+                           (const float *)p->d_gpart, img, n, scale);
+    }
+}
+
+template <int G>
+static void gather_tt(Plan *p, float *img, int B, int Bp, float scale, hipStream_t s)
+{
+    const int nbc = (Bp + 63) / 64;
+    const size_t lds = sizeof(float) * 2 * (size_t)(kTileZeroRow + 2) * 64;
+    const int bytes = (int)(sizeof(float) * ((size_t)p->D * p->geom.NB + 2) * Bp);
+    hipLaunchKernelGGL((k_gather_tt<G>), dim3(p->Pd / 256, G * nbc), dim3(1024), lds, s, p->d_ttab, p->d_twin,
+                       p->d_pix, p->d_bins, G > 1 ? p->d_gpart : img, p->D, p->geom.NB, p->P, B, Bp, scale, bytes);
+    if (G > 1) {
+        const size_t n = (size_t)B * p->P;
+        hipLaunchKernelGGL(k_sum_parts<G>, dim3(std::min<size_t>(4096, (n + 255) / 256)), dim3(256), 0, s,
                            (const float *)p->d_gpart, img, n, scale);
     }
 }
@@ -1756,12 +2136,24 @@ static const GatherVariant kGatherVariants[] = {{1, 8, 0},   {1, 4, 0},   {4, 4,
                                                 {0, 2, 64},  {0, 4, 64},  {0, -1, 16}, {0, -1, 32}, {0, -1, 64},
                                                 {0, -2, 16}, {0, -2, 32}, {0, -2, 64}, {0, -3, 16},
                                                 {0, -3, 32}, {0, -3, 64}, {0, -4, 64}, {0, -5, 8},
-                                                {0, -5, 16}, {0, -6, 8},  {0, -6, 16}};
+                                                {0, -5, 16}, {0, -6, 8},  {0, -6, 16}, {0, -7, 8},
+                                                {0, -7, 16}, {0, -8, 8},  {0, -8, 16}, {0, -9, 1},
+                                                {0, -9, 2},  {0, -9, 4},  {0, -9, 8}};
 constexpr int kNumGatherVariants = sizeof(kGatherVariants) / sizeof(kGatherVariants[0]);
 
 static bool gather_variant_ok(const Plan *p, int v, int Bp)
 {
     const GatherVariant &g = kGatherVariants[v];
+    if (g.u == -9) {   // LDS tile gather from the tile table: SIR-free, window fits the registers and LDS
+        if (p->use_sir || !p->d_ttab || Bp < 16 || p->tile_wmax > kTileZeroRow) return false;
+        if (sizeof(float) * ((size_t)p->D * p->geom.NB + 2) * Bp >= ((size_t)1 << 31)) return false;
+        return g.lb == 1 || p->d_gpart;
+    }
+    if (g.u <= -7) {   // pair-table gather (u = -8: XCD-grouped): SIR-free, 32-bit bin offsets
+        if (p->use_sir || !p->d_ptab || Bp < 16 || sizeof(float) * ((size_t)p->D * p->geom.NB + 2) * Bp >= ((size_t)1 << 31))
+            return false;
+        return g.u == -7 || p->d_gpart;
+    }
     if (g.u <= -5) {   // wave-uniform gather (u = -6: XCD-grouped): SIR-free, 32-bit bin offsets
         if (p->use_sir || Bp < 16 || (size_t)p->D * p->geom.NB * Bp >= ((size_t)1 << 31)) return false;
         return g.u == -5 || p->d_gpart;
@@ -1789,7 +2181,11 @@ static bool gather_variant_ok(const Plan *p, int v, int Bp)
 static int default_gather_variant(const Plan *p, int Bp)
 {
     if (Bp == 1) return 0;
-    if (Bp >= 64) return p->d_gpart && gather_variant_ok(p, 24, Bp) ? 24 : 11;
+    if (Bp >= 64) {
+        for (int v : {31, 24})   // LDS tile gather in two detector groups, else the wave-uniform XCD gather
+            if (gather_variant_ok(p, v, Bp)) return v;
+        return 11;
+    }
     if (Bp >= 32) return 8;
     if (Bp >= 16) return 7;
     if (Bp % 8 == 0) return 4;
@@ -1826,7 +2222,15 @@ static int launch_gather(Plan *p, float *img, int B, int Bp, float scale, hipStr
     case 22: gather_wu<8, false>(p, img, B, Bp, scale, s); break;
     case 23: gather_wu<16, false>(p, img, B, Bp, scale, s); break;
     case 24: gather_wu<8, true>(p, img, B, Bp, scale, s); break;
-    default: gather_wu<16, true>(p, img, B, Bp, scale, s); break;
+    case 25: gather_wu<16, true>(p, img, B, Bp, scale, s); break;
+    case 26: gather_wt<8, false>(p, img, B, Bp, scale, s); break;
+    case 27: gather_wt<16, false>(p, img, B, Bp, scale, s); break;
+    case 28: gather_wt<8, true>(p, img, B, Bp, scale, s); break;
+    case 29: gather_wt<16, true>(p, img, B, Bp, scale, s); break;
+    case 30: gather_tt<1>(p, img, B, Bp, scale, s); break;
+    case 31: gather_tt<2>(p, img, B, Bp, scale, s); break;
+    case 32: gather_tt<4>(p, img, B, Bp, scale, s); break;
+    default: gather_tt<8>(p, img, B, Bp, scale, s); break;
     }
     MBREC_HIP(hipGetLastError());
     return MBREC_OK;
@@ -2003,8 +2407,9 @@ int autotune(Plan *p, int B, int reps, hipStream_t s, int *gather_out, int *scat
     // scatter: record-lane LDS batch chunks, then batch-lane lane groups
     std::vector<int> cand;
     if (p->pull) {
-        for (int c : {1, 4, 8, 16, -16, -32, -64, -116, -132, -164, -216, -232, -264, -364, -464, -564})
-            if (pull_choice_ok(c, Bp) && !(c == 1 && Bp > 16)) cand.push_back(c);
+        for (int c : {1, 4, 8, 16, -16, -32, -64, -116, -132, -164, -216, -232, -264, -364, -464, -564, -664, -764,
+                      -864, -964, -1064, -1164})
+            if (pull_choice_ok(p, c, Bp) && !(c == 1 && Bp > 16)) cand.push_back(c);
     } else {
         const int mx = scatter_chunk_max(p, Bp);
         for (int bc = (Bp == 1 ? 1 : 4); bc < mx; bc *= 2) cand.push_back(bc);
@@ -2042,7 +2447,7 @@ int set_launch(Plan *p, int B, int gather, int scatter)
     if (gather >= kNumGatherVariants || (gather >= 0 && !gather_variant_ok(p, gather, Bp)))
         return set_error(MBREC_ERR_ARG, "mbrec:args", "gather variant %d is not valid for a batch of %d", gather, B);
     if (p->pull) {
-        if (scatter != 0 && !pull_choice_ok(scatter, Bp))
+        if (scatter != 0 && !pull_choice_ok(p, scatter, Bp))
             return set_error(MBREC_ERR_ARG, "mbrec:args", "forward choice %d is not valid for a batch of %d", scatter,
                              B);
     } else if (scatter > 0 && (scatter > scatter_chunk_max(p, Bp) || (Bp > 1 && scatter % 4)))

@@ -391,13 +391,19 @@ static int lsqr_solve(Plan *p, Workspace &ws, const float *y, int B, int G, int 
         hipLaunchKernelGGL(k_u_sino, vgrid(B), dim3(256), 0, s, ws.ts, ws.us, ns, B, k, ws.sc, ws.part);
         hipLaunchKernelGGL(k_u_pix, vgrid(B), dim3(256), 0, s, ws.v, ws.ur, ws.uc, np, B, k, ws.sc, ws.part);
         hipLaunchKernelGGL(k_scal_u, dim3(1), dim3(256), 0, s, ws.sc, ws.part, G, B, k);
-        hipLaunchKernelGGL(k_scale_u, vgrid(B), dim3(256), 0, s, ws.us, ns, ws.ur, ws.uc, np, B, k, ws.sc);
         hipLaunchKernelGGL(k_dx, vgrid(B), dim3(256), 0, s, ws.v, ws.d, ws.x, np, B, k, ws.sc, ws.part);
+        // The transpose product that closes the last iteration (with the
+        // normalisation of u before it) feeds only the stopping tests and the
+        // next direction; x is final here, and on non-convergence the
+        // minimal-residual iterate is this last one (the residual estimate
+        // |s| normr never increases), so that work is dead.
+        if (ii == maxit) break;
+        hipLaunchKernelGGL(k_scale_u, vgrid(B), dim3(256), 0, s, ws.us, ns, ws.ur, ws.uc, np, B, k, ws.sc);
         if ((rc = op_transpose(p, ws.us, ws.tp, B, true, s, win))) return rc;
         hipLaunchKernelGGL(k_v_update, vgrid(B), dim3(256), 0, s, ws.tp, ws.ur, ws.uc, ws.v, np, B, k, ws.sc, 0,
                            ws.part);
         hipLaunchKernelGGL(k_scal_v, dim3(1), dim3(256), 0, s, ws.sc, ws.part, G, B, k, ii);
-        if (ii < maxit) hipLaunchKernelGGL(k_scale_v, vgrid(B), dim3(256), 0, s, ws.v, np, B, k, ws.sc);
+        hipLaunchKernelGGL(k_scale_v, vgrid(B), dim3(256), 0, s, ws.v, np, B, k, ws.sc);
     }
     MBREC_HIP(hipGetLastError());
     return MBREC_OK;
