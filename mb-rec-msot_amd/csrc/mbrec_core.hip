@@ -390,8 +390,9 @@ static int build_tile_windows(Plan *p, const std::vector<int32_t> &tq)
             if (lo <= hi) wmax = std::max(wmax, hi - lo + 1);
         }
     p->tile_wmax = wmax;
+    p->tile_zr = 64 * ((wmax + 63) / 64);   // staged windows are whole 64-row rounds
     // decoded pairs per (tile, detector): row inside the tile's window, or the
-    // zero row kTileZeroRow for pairs outside the bin range; u exactly as decode_tau
+    // zero row tile_zr for pairs outside the bin range; u exactly as decode_tau
     const float scale = std::ldexp(1.0f, -g.F);
     std::vector<int2> tt((size_t)ntile * p->D * 256);
     for (int t = 0; t < ntile; ++t)
@@ -402,7 +403,7 @@ static int build_tile_windows(Plan *p, const std::vector<int32_t> &tq)
             for (int k = 0; k < 256; ++k) {
                 int nn, m;
                 host_decode(q[k], g, nn, m);
-                e[k] = make_int2(kTileZeroRow, 0);
+                e[k] = make_int2(p->tile_zr, 0);
                 if ((unsigned)nn >= (unsigned)span) continue;
                 const float fr = (float)(q[k] & ((1 << g.F) - 1)) * scale;
                 const float u = m == 0 ? fr * g.iw0 : (m == 1 ? (fr - g.phi1) * g.iw1 : (fr - g.phi2) * g.iw2);

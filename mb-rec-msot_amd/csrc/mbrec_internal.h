@@ -35,8 +35,8 @@ struct HatTaps {
 };
 
 // Decoding of a fixed-point travel time into (bin index, interpolation weight).
-constexpr int kTileZeroRow = 256;   // tile gather: LDS window rows [0, 256), then two zero rows
-constexpr int kRecPad = 64;   // zero records after the pull table (runs of up to 64 read past the end)
+constexpr int kTileRowsMax = 256;   // tile gathers: longest bin window they stage (rows)
+constexpr int kRecPad = 512;   // zero records after the pull table (runs of up to 512 read past the end)
 
 struct BinGeom {
     float phi1, phi2;            // sorted interior breakpoint phases in [0,1)
@@ -64,7 +64,8 @@ struct Plan {
     int2 *d_ptab = nullptr;      // [Pd][D] decoded pairs (bin row d*NB + j or the zero row D*NB, u bits)
     int2 *d_twin = nullptr;      // [Pd / 256][D] bin window (first bin, length) of each 256-slot tile
     int tile_wmax = 0;           // longest tile window
-    int2 *d_ttab = nullptr;      // [Pd / 256][D][256] (bin row inside the tile window, or kTileZeroRow; u bits)
+    int2 *d_ttab = nullptr;      // [Pd / 256][D][256] (bin row inside the tile window, or tile_zr; u bits)
+    int tile_zr = 0;             // first of the two zero rows after a staged tile window (64 * ceil(wmax / 64))
     BinGeom geom{};
     HatTaps taps{};
     // forward scatter: per detector, (pixel, travel time) records grouped into buckets

@@ -58,6 +58,11 @@ __device__ __forceinline__ void kahan_add(float &s, float &c, float v)
 // reads past num_records return 0.
 constexpr int kBufferWord3 = 0x00020000;
 
+// s_waitcnt immediate that waits until at most n vector-memory operations of
+// the wave are outstanding (gfx9 encoding: vmcnt in bits 3:0 and 15:14,
+// expcnt 6:4 and lgkmcnt 11:8 left at their maxima, i.e. not waited for)
+__host__ __device__ constexpr int kWaitVm(int n) { return (n & 15) | (((n >> 4) & 3) << 14) | 0x70 | 0xF00; }
+
 // Staging of n elements into LDS with U global loads in flight per thread
 // before any is stored (a plain strided loop waits out one load latency per
 // element).  ld(e) returns element e, st(e, v) stores it; NT threads take part.
@@ -1257,6 +1262,131 @@ __global__ __launch_bounds__(256) void k_fwd_pull_wt(const int2 *__restrict__ re
     }
 }
 
+// forward pull for a single sinogram (Bp == 1), record-streaming form: a wave
+// owns 64 consecutive bins of one detector (lane i: bin ja + i) and streams the
+// records of bins ja - 1 .. ja + 63 (contiguous in the CSR table) in coalesced
+// chunks of 64, lane l taking record r + l.  The per-record terms (1 - u) c and
+// u c are prefix-summed over the wave; lane i then takes the sum of its
+// bin's run of records (the (1 - u) parts) and of the previous bin's run (the u
+// parts) as differences of the prefix sums at the run ends.
+// v plus its DPP-moved copy; lanes without a source (or rows outside ROWS) add 0
+template <int CTRL, int ROWS>
+__device__ __forceinline__ float dpp_add(float v)
+{
+    return v + __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), CTRL, ROWS,
+                                                                      0xf, false));
+}
+
+// inclusive prefix sum over the 64 lanes: shifts by 1, 2, 4, 8 within each row of
+// 16 lanes, then rows 1 and 3 add the last lane of the row before (row_bcast:15),
+// rows 2 and 3 add lane 31 (row_bcast:31)
+__device__ __forceinline__ float wave_scan_add(float v)
+{
+    v = dpp_add<0x111, 0xf>(v);   // row_shr:1
+    v = dpp_add<0x112, 0xf>(v);   // row_shr:2
+    v = dpp_add<0x114, 0xf>(v);   // row_shr:4
+    v = dpp_add<0x118, 0xf>(v);   // row_shr:8
+    v = dpp_add<0x142, 0xa>(v);   // row_bcast:15 into rows 1, 3
+    v = dpp_add<0x143, 0xc>(v);   // row_bcast:31 into rows 2, 3
+    return v;
+}
+
+// inclusive prefix s (over the wave) summed over lanes [lo, hi) of the chunk
+__device__ __forceinline__ float run_sum(float s, int lo, int hi)
+{
+    const float a = __shfl(s, max(hi - 1, 0), 64), b = __shfl(s, max(lo - 1, 0), 64);
+    return hi > lo ? a - (lo > 0 ? b : 0.0f) : 0.0f;
+}
+
+template <int NG>
+__global__ __launch_bounds__(256) void k_fwd_pull_b1(const int2 *__restrict__ rec, const int32_t *__restrict__ off,
+                                                     const float *__restrict__ img, float *__restrict__ bins, int NB,
+                                                     int xcd_map /* D / 8, or 0 */)
+{
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    int d, jblk;
+    if (xcd_map) {   // XCD x (= blockIdx.x % 8) takes detectors [x * D/8, (x + 1) * D/8)
+        const int x = blockIdx.x % 8, sidx = blockIdx.x / 8;
+        d = x * xcd_map + sidx % xcd_map;
+        jblk = sidx / xcd_map;
+    } else {
+        d = blockIdx.y;
+        jblk = blockIdx.x;
+    }
+    // the wave owns NG groups of 64 bins, [ja, ja + 64 NG); lane i of group g: bin ja + 64 g + i
+    const int ja = (jblk * 4 + wv) * 64 * NG;
+    if (ja >= NB) return;   // no barriers in this kernel
+    const int32_t *o = off + (size_t)d * (NB + 1);
+    // record runs: bin J is [o[J], o[J + 1]); bin J - 1 is [o[J - 1], o[J])
+    int oJ[NG], oJ1[NG], oJm[NG], gs[NG], ge[NG];
+#pragma unroll
+    for (int g = 0; g < NG; ++g) {
+        const int J = ja + 64 * g + lane;
+        oJ[g] = o[min(J, NB)];
+        oJ1[g] = o[min(J + 1, NB)];
+        oJm[g] = J > 0 ? o[min(J - 1, NB)] : oJ[g];
+        const int j0 = ja + 64 * g;   // the group's records: bins j0 - 1 .. j0 + 63
+        gs[g] = __builtin_amdgcn_readfirstlane(o[min(max(j0 - 1, 0), NB)]);
+        ge[g] = __builtin_amdgcn_readfirstlane(o[min(j0 + 64, NB)]);
+    }
+    const int rs = gs[0], re = ge[NG - 1];
+    float a0[NG], a1[NG];
+#pragma unroll
+    for (int g = 0; g < NG; ++g) a0[g] = a1[g] = 0.0f;
+    // NC chunks per step, software-pipelined: the records of step k + 2 and the
+    // pixel values of step k + 1 are in flight while step k is summed (records past
+    // re, the next bins' or the zero pad, carry weight 0)
+    constexpr int NC = 8;
+    int2 e[NC], en[NC];
+    float c[NC];
+    auto ld_e = [&](int r0, int2(&q)[NC]) {
+#pragma unroll
+        for (int k = 0; k < NC; ++k) q[k] = rec[r0 + 64 * k + lane];
+    };
+    auto ld_c = [&](const int2(&q)[NC], float(&v)[NC]) {
+#pragma unroll
+        for (int k = 0; k < NC; ++k) v[k] = img[q[k].x];
+    };
+    int r0 = rs;
+    if (r0 < re) {
+        ld_e(r0, e);
+        ld_c(e, c);
+        if (r0 + 64 * NC < re) ld_e(r0 + 64 * NC, en);
+    }
+    for (; r0 < re; r0 += 64 * NC) {
+        float cn[NC];
+        int2 enn[NC];
+        if (r0 + 64 * NC < re) ld_c(en, cn);
+        if (r0 + 128 * NC < re) ld_e(r0 + 128 * NC, enn);
+#pragma unroll
+        for (int q = 0; q < NC; ++q) {
+            const int r = r0 + 64 * q;
+            if (r >= re) break;
+            const bool in = r + lane < re;
+            const float uq = __int_as_float(e[q].y);
+            const float w0 = in ? (1.0f - uq) * c[q] : 0.0f, w1 = in ? uq * c[q] : 0.0f;
+            const float s0 = wave_scan_add(w0), s1 = wave_scan_add(w1);
+#pragma unroll
+            for (int g = 0; g < NG; ++g) {
+                if (r >= ge[g] || r + 64 <= gs[g]) continue;   // the chunk misses the group's records
+                a0[g] += run_sum(s0, min(max(oJ[g] - r, 0), 64), min(max(oJ1[g] - r, 0), 64));
+                a1[g] += run_sum(s1, min(max(oJm[g] - r, 0), 64), min(max(oJ[g] - r, 0), 64));
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < NC; ++q) {
+            e[q] = en[q];
+            c[q] = cn[q];
+            en[q] = enn[q];
+        }
+    }
+#pragma unroll
+    for (int g = 0; g < NG; ++g) {
+        const int J = ja + 64 * g + lane;
+        if (J < NB) bins[(size_t)d * NB + J] = a0[g] + a1[g];
+    }
+}
+
 // ---------------------------------------------------------------------------
 // tile gather (batch-lane, Bp >= 16): one workgroup per 16 x 16 pixel tile and
 // chunk of LB sinograms.  For every detector the tile's bin window (all bins its
@@ -1590,15 +1720,15 @@ template <int G>
 __global__ __launch_bounds__(1024) void k_gather_tt(
     const int2 *__restrict__ ttab, const int2 *__restrict__ twin, const int32_t *__restrict__ pix,
     const float *__restrict__ bins, float *__restrict__ out, int D, int NB, int P, int B, int Bp, float scale,
-    int bins_bytes)
+    int bins_bytes, int ZR)
 {
-    extern __shared__ float sm[];   // 2 buffers x (kTileZeroRow + 2) rows x 64 floats
-    constexpr int RS = kTileZeroRow + 2;
+    extern __shared__ float sm[];   // 2 buffers x (ZR + 2) rows x 64 floats
+    const int RS = ZR + 2;
     const int tile = blockIdx.x, grp = blockIdx.y % G, b0 = (blockIdx.y / G) * 64;
     const int lane = threadIdx.x & 63, wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int DG = (D + G - 1) / G, dbeg = grp * DG, dend = min(D, dbeg + DG);
     for (int i = threadIdx.x; i < 2 * 2 * 64; i += 1024)   // the zero rows of both buffers
-        sm[(i / 128) * RS * 64 + (kTileZeroRow + (i / 64) % 2) * 64 + i % 64] = 0.0f;
+        sm[(i / 128) * RS * 64 + (ZR + (i / 64) % 2) * 64 + i % 64] = 0.0f;
     // window staging: thread t moves 16-byte pieces f = t + 1024 k (row f / 16, column
     // 4 (f % 16)) of the window's rows [lo, lo + len); columns past Bp carry the next
     // row's values and are never used (raw-buffer reads past the end return 0)
@@ -1666,6 +1796,132 @@ __global__ __launch_bounds__(1024) void k_gather_tt(
             if (d + 1 < dend) store(cur ^ 1);
             __syncthreads();
             if (d + 2 < dend) fetch(d + 2);
+        }
+    }
+    const int b = b0 + lane;
+    if (b >= B) return;
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+        const float v = acc[k] + part[k];
+        const int px = pix[(size_t)tile * 256 + 16 * wv + k];
+        if (px < 0) continue;
+        if (G > 1)
+            out[((size_t)grp * B + b) * P + px] = v;
+        else
+            out[(size_t)b * P + px] = scale * v;
+    }
+}
+
+// Tile gather with asynchronous staging: as k_gather_tt, but each detector's
+// bin window and the tile's pair-table entries go from memory straight into a
+// ring of three LDS buffers (buffer loads with the LDS destination), two
+// detectors ahead of the one being interpolated, without passing through
+// registers.  Each wave issues a fixed number of loads per detector (the window
+// rounded up to NCH rounds of 64 rows, ZR = 64 NCH; waves 0 and 1 also move the
+// 256 table entries), so a wait for all but the latest detector's loads covers
+// the detector about to be read.  The three buffers are distinct arrays and the
+// detector loop is unrolled by three, so the compiler can tell the buffer being
+// read from the one being filled (and does not wait for the fill before reading).
+constexpr int kTaMaxD = 512;   // detectors whose window origins the async tile gather keeps in LDS
+
+template <int NCH>
+struct TaLayout {
+    static constexpr int ZR = 64 * NCH, RS = ZR + 2;    // window rows, then two zero rows
+    static constexpr int BUF = RS * 64 + 2 * 256;       // floats per buffer: rows, then 256 table entries
+};
+
+template <int G, int NCH>
+__global__ __launch_bounds__(1024) void k_gather_ta(
+    const int2 *__restrict__ ttab, const int2 *__restrict__ twin, const int32_t *__restrict__ pix,
+    const float *__restrict__ bins, float *__restrict__ out, int D, int NB, int P, int B, int Bp, float scale,
+    int bins_bytes, int ttab_bytes)
+{
+    using L = TaLayout<NCH>;
+    __shared__ __attribute__((aligned(16))) float s0[L::BUF], s1[L::BUF], s2[L::BUF];
+    __shared__ int2 tw[kTaMaxD];   // the tile's window origins
+    const int tile = blockIdx.x, grp = blockIdx.y % G, b0 = (blockIdx.y / G) * 64;
+    const int lane = threadIdx.x & 63, wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int DG = (D + G - 1) / G, dbeg = grp * DG, dend = min(D, dbeg + DG);
+    for (int i = threadIdx.x; i < 2 * 64; i += 1024) {   // the zero rows of the three buffers
+        s0[L::ZR * 64 + i] = 0.0f;
+        s1[L::ZR * 64 + i] = 0.0f;
+        s2[L::ZR * 64 + i] = 0.0f;
+    }
+    for (int i = threadIdx.x; i < D; i += 1024) tw[i] = twin[(size_t)tile * D + i];
+    __syncthreads();
+    const __amdgpu_buffer_rsrc_t rs =
+        __builtin_amdgcn_make_buffer_rsrc(const_cast<float *>(bins), (short)0, bins_bytes, kBufferWord3);
+    const __amdgpu_buffer_rsrc_t rt =
+        __builtin_amdgcn_make_buffer_rsrc(const_cast<int2 *>(ttab), (short)0, ttab_bytes, kBufferWord3);
+    const int rowb = 4 * Bp;
+    // load c of wave w moves rows 4 (16 c + w) .. + 3: lane l takes row + l / 16, columns b0 + 4 (l % 16) ..
+    const int vo = (lane >> 4) * rowb + 4 * (b0 + 4 * (lane & 15));
+    auto stage = [&](int d, float *bb) __attribute__((always_inline)) {
+        const int so = (d * NB + __builtin_amdgcn_readfirstlane(tw[d].x)) * rowb;
+#pragma unroll
+        for (int c = 0; c < NCH; ++c) {
+            const int r0 = 4 * (16 * c + wv);
+            __builtin_amdgcn_raw_ptr_buffer_load_lds(rs, (__attribute__((address_space(3))) void *)(bb + r0 * 64),
+                                                     16, vo, so + r0 * rowb, 0, 0);
+        }
+        if (wv < 2)   // table entries 128 w .. 128 w + 127 of (tile, d): 16 bytes (2 entries) per lane
+            __builtin_amdgcn_raw_ptr_buffer_load_lds(
+                rt, (__attribute__((address_space(3))) void *)(bb + L::RS * 64 + 256 * wv), 16, 16 * lane,
+                (int)((((size_t)tile * D + d) * 256 + 128 * wv) * sizeof(int2)), 0, 0);
+    };
+    float acc[16], part[16];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) acc[k] = part[k] = 0.0f;
+    // one detector: wait for its loads, barrier, start the fill two ahead, interpolate
+    auto step = [&](int d, const float *rd, float *wr) __attribute__((always_inline)) {
+        if (d + 1 < dend) {   // this wave's loads for d are done (those for d + 1 may be in flight)
+            if (wv < 2)
+                __builtin_amdgcn_s_waitcnt(kWaitVm(NCH + 1));
+            else
+                __builtin_amdgcn_s_waitcnt(kWaitVm(NCH));
+        } else {
+            __builtin_amdgcn_s_waitcnt(kWaitVm(0));
+        }
+        // everyone's loads for d done; the buffer of d - 1 no longer read (a bare
+        // barrier: a full fence here would also wait for the loads of d + 1)
+        __builtin_amdgcn_s_barrier();
+        if (d + 2 < dend) stage(d + 2, wr);
+        const int4 *te = reinterpret_cast<const int4 *>(rd + L::RS * 64) + 8 * wv;   // this wave's 16 entries
+        const float *wb = rd + lane;
+        float g0[16], g1[16], uu[16];
+#pragma unroll
+        for (int k2 = 0; k2 < 8; ++k2) {
+            const int4 e = te[k2];   // entries 2 k2, 2 k2 + 1 (same address in every lane)
+            g0[2 * k2] = wb[e.x * 64];
+            g1[2 * k2] = wb[e.x * 64 + 64];
+            uu[2 * k2] = __int_as_float(e.y);
+            g0[2 * k2 + 1] = wb[e.z * 64];
+            g1[2 * k2 + 1] = wb[e.z * 64 + 64];
+            uu[2 * k2 + 1] = __int_as_float(e.w);
+        }
+#pragma unroll
+        for (int k = 0; k < 16; ++k) part[k] += fmaf(uu[k], g1[k] - g0[k], g0[k]);
+        if (((d - dbeg) & 7) == 7) {
+#pragma unroll
+            for (int k = 0; k < 16; ++k) {
+                acc[k] += part[k];
+                part[k] = 0.0f;
+            }
+        }
+    };
+    if (dbeg < dend) {
+        stage(dbeg, s0);
+        if (dbeg + 1 < dend) stage(dbeg + 1, s1);
+        // unrolled by six: the compiler's wait before the first step of an iteration
+        // (where it loses track of which buffer the pending fills target) is paid
+        // once per six detectors
+        for (int d = dbeg; d < dend; d += 6) {
+            step(d, s0, s2);
+            if (d + 1 < dend) step(d + 1, s1, s0);
+            if (d + 2 < dend) step(d + 2, s2, s1);
+            if (d + 3 < dend) step(d + 3, s0, s2);
+            if (d + 4 < dend) step(d + 4, s1, s0);
+            if (d + 5 < dend) step(d + 5, s2, s1);
         }
     }
     const int b = b0 + lane;
@@ -1755,7 +2011,7 @@ static int scatter_choice(const Plan *p, int Bp)
 // 8 pairs in flight; 1: XCD-aware order, 8 in flight; 2: XCD-aware, 16 in flight
 static bool pull_choice_ok(const Plan *p, int c, int Bp)
 {
-    if (c > 0) return c == 1 || ((c == 4 || c == 8 || c == 16) && Bp % c == 0);
+    if (c > 0) return c == 1 || ((c == 2 || c == 3) && Bp == 1) || ((c == 4 || c == 8 || c == 16) && Bp % c == 0);
     if (c < 0) {
         const int lb = (-c) % 100, mode = (-c) / 100;
         int cap = 16;
@@ -1776,10 +2032,20 @@ static int default_pull_choice(int Bp)
     if (Bp >= 16) return -16;
     if (Bp % 8 == 0) return 8;
     if (Bp % 4 == 0) return 4;
-    return 1;
+    return Bp == 1 ? 2 : 1;
 }
 
 constexpr int kPullRange = 32;   // bins per lane group (batch-lane pull)
+
+template <int NG>
+static void fwd_pull_b1(Plan *p, const float *imgI, hipStream_t s)
+{
+    const int nblk = (p->geom.NB + 256 * NG - 1) / (256 * NG);   // 4 waves x 64 NG bins per workgroup
+    const int xm = p->D % 8 == 0 ? p->D / 8 : 0;
+    const dim3 grid = xm ? dim3(nblk * p->D) : dim3(nblk, p->D);
+    hipLaunchKernelGGL(k_fwd_pull_b1<NG>, grid, dim3(256), 0, s, p->d_rec, p->d_rec_off, imgI, p->d_bins,
+                       p->geom.NB, xm);
+}
 
 template <int BC>
 static void fwd_pull(Plan *p, const float *imgI, int Bp, hipStream_t s)
@@ -1841,6 +2107,8 @@ static int launch_pull(Plan *p, const float *imgI, int Bp, hipStream_t s)
     if (!pull_choice_ok(p, c, Bp)) c = default_pull_choice(Bp);
     switch (c) {
     case 1: fwd_pull<1>(p, imgI, Bp, s); break;
+    case 2: fwd_pull_b1<1>(p, imgI, s); break;
+    case 3: fwd_pull_b1<4>(p, imgI, s); break;
     case 4: fwd_pull<4>(p, imgI, Bp, s); break;
     case 8: fwd_pull<8>(p, imgI, Bp, s); break;
     case 16: fwd_pull<16>(p, imgI, Bp, s); break;
@@ -2099,14 +2367,42 @@ static void gather_wt(Plan *p, float *img, int B, int Bp, float scale, hipStream
     }
 }
 
+template <int G, int NCH>
+static void gather_ta_n(Plan *p, float *img, int B, int Bp, float scale, hipStream_t s)
+{
+    const int nbc = (Bp + 63) / 64;
+    const size_t lds = 0;   // static LDS
+    const int bytes = (int)(sizeof(float) * ((size_t)p->D * p->geom.NB + 2) * Bp);
+    const int tbytes = (int)(sizeof(int2) * (size_t)(p->Pd / 256) * p->D * 256);
+    hipLaunchKernelGGL((k_gather_ta<G, NCH>), dim3(p->Pd / 256, G * nbc), dim3(1024), lds, s, p->d_ttab, p->d_twin,
+                       p->d_pix, p->d_bins, G > 1 ? p->d_gpart : img, p->D, p->geom.NB, p->P, B, Bp, scale, bytes,
+                       tbytes);
+    if (G > 1) {
+        const size_t n = (size_t)B * p->P;
+        hipLaunchKernelGGL(k_sum_parts<G>, dim3(std::min<size_t>(4096, (n + 255) / 256)), dim3(256), 0, s,
+                           (const float *)p->d_gpart, img, n, scale);
+    }
+}
+
+template <int G>
+static void gather_ta(Plan *p, float *img, int B, int Bp, float scale, hipStream_t s)
+{
+    switch (p->tile_zr / 64) {   // 64-row rounds of the staged window (validity keeps it in 1..3)
+    case 1: gather_ta_n<G, 1>(p, img, B, Bp, scale, s); break;
+    case 2: gather_ta_n<G, 2>(p, img, B, Bp, scale, s); break;
+    default: gather_ta_n<G, 3>(p, img, B, Bp, scale, s); break;
+    }
+}
+
 template <int G>
 static void gather_tt(Plan *p, float *img, int B, int Bp, float scale, hipStream_t s)
 {
     const int nbc = (Bp + 63) / 64;
-    const size_t lds = sizeof(float) * 2 * (size_t)(kTileZeroRow + 2) * 64;
+    const size_t lds = sizeof(float) * 2 * (size_t)(p->tile_zr + 2) * 64;
     const int bytes = (int)(sizeof(float) * ((size_t)p->D * p->geom.NB + 2) * Bp);
     hipLaunchKernelGGL((k_gather_tt<G>), dim3(p->Pd / 256, G * nbc), dim3(1024), lds, s, p->d_ttab, p->d_twin,
-                       p->d_pix, p->d_bins, G > 1 ? p->d_gpart : img, p->D, p->geom.NB, p->P, B, Bp, scale, bytes);
+                       p->d_pix, p->d_bins, G > 1 ? p->d_gpart : img, p->D, p->geom.NB, p->P, B, Bp, scale, bytes,
+                       p->tile_zr);
     if (G > 1) {
         const size_t n = (size_t)B * p->P;
         hipLaunchKernelGGL(k_sum_parts<G>, dim3(std::min<size_t>(4096, (n + 255) / 256)), dim3(256), 0, s,
@@ -2138,14 +2434,22 @@ static const GatherVariant kGatherVariants[] = {{1, 8, 0},   {1, 4, 0},   {4, 4,
                                                 {0, -3, 32}, {0, -3, 64}, {0, -4, 64}, {0, -5, 8},
                                                 {0, -5, 16}, {0, -6, 8},  {0, -6, 16}, {0, -7, 8},
                                                 {0, -7, 16}, {0, -8, 8},  {0, -8, 16}, {0, -9, 1},
-                                                {0, -9, 2},  {0, -9, 4},  {0, -9, 8}};
+                                                {0, -9, 2},  {0, -9, 4},  {0, -9, 8},  {0, -10, 1},
+                                                {0, -10, 2}, {0, -10, 4}};
 constexpr int kNumGatherVariants = sizeof(kGatherVariants) / sizeof(kGatherVariants[0]);
 
 static bool gather_variant_ok(const Plan *p, int v, int Bp)
 {
     const GatherVariant &g = kGatherVariants[v];
+    if (g.u == -10) {   // tile gather with asynchronous staging: three buffers of at most 3 rounds of 64 rows
+        if (p->use_sir || !p->d_ttab || Bp < 16 || p->tile_zr > 192) return false;
+        if (sizeof(float) * ((size_t)p->D * p->geom.NB + 2) * Bp >= ((size_t)1 << 31)) return false;
+        if (sizeof(int2) * (size_t)(p->Pd / 256) * p->D * 256 >= ((size_t)1 << 31)) return false;
+        if (p->D > kTaMaxD) return false;
+        return g.lb == 1 || p->d_gpart;
+    }
     if (g.u == -9) {   // LDS tile gather from the tile table: SIR-free, window fits the registers and LDS
-        if (p->use_sir || !p->d_ttab || Bp < 16 || p->tile_wmax > kTileZeroRow) return false;
+        if (p->use_sir || !p->d_ttab || Bp < 16 || p->tile_zr > kTileRowsMax) return false;
         if (sizeof(float) * ((size_t)p->D * p->geom.NB + 2) * Bp >= ((size_t)1 << 31)) return false;
         return g.lb == 1 || p->d_gpart;
     }
@@ -2230,7 +2534,10 @@ static int launch_gather(Plan *p, float *img, int B, int Bp, float scale, hipStr
     case 30: gather_tt<1>(p, img, B, Bp, scale, s); break;
     case 31: gather_tt<2>(p, img, B, Bp, scale, s); break;
     case 32: gather_tt<4>(p, img, B, Bp, scale, s); break;
-    default: gather_tt<8>(p, img, B, Bp, scale, s); break;
+    case 33: gather_tt<8>(p, img, B, Bp, scale, s); break;
+    case 34: gather_ta<1>(p, img, B, Bp, scale, s); break;
+    case 35: gather_ta<2>(p, img, B, Bp, scale, s); break;
+    default: gather_ta<4>(p, img, B, Bp, scale, s); break;
     }
     MBREC_HIP(hipGetLastError());
     return MBREC_OK;
@@ -2407,8 +2714,8 @@ int autotune(Plan *p, int B, int reps, hipStream_t s, int *gather_out, int *scat
     // scatter: record-lane LDS batch chunks, then batch-lane lane groups
     std::vector<int> cand;
     if (p->pull) {
-        for (int c : {1, 4, 8, 16, -16, -32, -64, -116, -132, -164, -216, -232, -264, -364, -464, -564, -664, -764,
-                      -864, -964, -1064, -1164})
+        for (int c : {1, 2, 3, 4, 8, 16, -16, -32, -64, -116, -132, -164, -216, -232, -264, -364, -464, -564, -664,
+                      -764, -864, -964, -1064, -1164})
             if (pull_choice_ok(p, c, Bp) && !(c == 1 && Bp > 16)) cand.push_back(c);
     } else {
         const int mx = scatter_chunk_max(p, Bp);
