@@ -140,20 +140,25 @@ __global__ __launch_bounds__(256) void k_filt_axis(
 // wrapped samples on either side.  Planar output [B][P]; interleaved output
 // [P][Bp] (IL) takes the 16 columns as 16 consecutive sinograms of one x
 // column, so its stores are 64-byte runs (columns b >= B are written as zero).
-// x pass: one sinogram's 32 consecutive z rows over all Nx columns per
+// x pass: one sinogram's kFxRows consecutive z rows over all Nx columns per
 // workgroup, planar in and out.
 // forward: out[i] = sum_a k[a] in[(i - a) mod N];  transpose: in[(i + a) mod N].
 // ---------------------------------------------------------------------------
 constexpr int kFzCols = 16;
-constexpr int kFxRows = 32;
+constexpr int kFxRows = 16;
 
-template <bool IL>
+constexpr int kFiltR = 8;   // outputs per thread of the register-blocked prefilter passes
+
+// KT > 0: K = KT known at compile time, each thread forms kFiltR consecutive
+// outputs of a column from one register window of kFiltR + 2 KT samples
+// (staged columns carry kFiltR extra samples so the last window stays inside).
+template <bool IL, int KT>
 __global__ __launch_bounds__(256) void k_filt_z(const float *__restrict__ in, float *__restrict__ out,
                                                 const float *__restrict__ k, int K, int Nz, int Nx, int B, int Bp,
                                                 int transp)
 {
     extern __shared__ float sm[];
-    const int S = (Nz + 2 * K) | 1;   // column stride in LDS
+    const int S = (Nz + 2 * K + (KT ? kFiltR : 0)) | 1;   // column stride in LDS
     double *tk = reinterpret_cast<double *>(sm);
     float *xs = sm + 2 * (2 * K + 1 + 1);
     const size_t P = (size_t)Nz * Nx;
@@ -167,13 +172,14 @@ __global__ __launch_bounds__(256) void k_filt_z(const float *__restrict__ in, fl
         col0 = (size_t)blockIdx.x * kFzCols;
     }
     for (int i = threadIdx.x; i < 2 * K + 1; i += blockDim.x) tk[i] = (double)k[i];
-    const int W = Nz + 2 * K;
+    const int W = Nz + 2 * K + (KT ? kFiltR : 0);
     const float invW = 1.0f / (float)W;
     stage_batched<16, 256>(
         threadIdx.x, kFzCols * W,
         [&](int e) {
             int c, i;
             divmod_small(e, W, invW, c, i);
+            if (i >= Nz + 2 * K) return 0.0f;   // window overrun of the last block (outputs not stored)
             int j = i - K;
             j = j < 0 ? j + Nz : (j >= Nz ? j - Nz : j);
             if (IL) return b0 + c < B ? in[(size_t)(b0 + c) * P + (size_t)ixc * Nz + j] : 0.0f;
@@ -186,6 +192,42 @@ __global__ __launch_bounds__(256) void k_filt_z(const float *__restrict__ in, fl
         });
     __syncthreads();
     // out[i] = sum_a k[a + K] xs[i + K - sgn a]: forward reads xs[i + 2K - t], transpose xs[i + t] (t = a + K)
+    if (KT) {   // 16 threads per column, blocks of kFiltR outputs
+        constexpr int NT = 2 * (KT ? KT : 1) + 1, NW = kFiltR + NT - 1;
+        double tr[NT];
+#pragma unroll
+        for (int t = 0; t < NT; ++t) tr[t] = tk[t];
+        const int c = IL ? threadIdx.x % kFzCols : threadIdx.x / 16, ib = IL ? threadIdx.x / kFzCols : threadIdx.x % 16;
+        const bool live = IL ? b0 + c < Bp : col0 + c < (size_t)B * Nx;
+        if (!live) return;
+        for (int iz0 = ib * kFiltR; iz0 < Nz; iz0 += 16 * kFiltR) {
+            const float *x = xs + c * S + iz0;
+            double w[NW], acc[kFiltR];
+#pragma unroll
+            for (int i = 0; i < NW; ++i) w[i] = (double)x[i];
+#pragma unroll
+            for (int q = 0; q < kFiltR; ++q) {
+                acc[q] = 0.0;
+                if (transp) {
+#pragma unroll
+                    for (int t = 0; t < NT; ++t) acc[q] = fma(tr[t], w[q + t], acc[q]);
+                } else {
+#pragma unroll
+                    for (int t = 0; t < NT; ++t) acc[q] = fma(tr[t], w[q + NT - 1 - t], acc[q]);
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < kFiltR; ++q) {
+                const int iz = iz0 + q;
+                if (iz >= Nz) break;
+                if (IL)
+                    out[((size_t)ixc * Nz + iz) * Bp + b0 + c] = b0 + c < B ? (float)acc[q] : 0.0f;
+                else
+                    out[(col0 + c) * Nz + iz] = (float)acc[q];
+            }
+        }
+        return;
+    }
     if (IL) {
         const int c = threadIdx.x % kFzCols;
         const int b = b0 + c;
@@ -215,12 +257,13 @@ __global__ __launch_bounds__(256) void k_filt_z(const float *__restrict__ in, fl
     }
 }
 
+template <int KT>
 __global__ __launch_bounds__(256) void k_filt_x(const float *__restrict__ in, float *__restrict__ out,
                                                 const float *__restrict__ k, int K, int Nz, int Nx, int transp)
 {
     extern __shared__ float sm[];
     double *tk = reinterpret_cast<double *>(sm);
-    float *xs = sm + 2 * (2 * K + 1 + 1);   // [Nx + 2K][kFxRows]
+    float *xs = sm + 2 * (2 * K + 1 + 1);   // [Nx + 2K (+ kFiltR)][kFxRows]
     const int iz0 = blockIdx.x * kFxRows, b = blockIdx.y;
     const size_t P = (size_t)Nz * Nx;
     const float *src = in + (size_t)b * P;
@@ -229,9 +272,10 @@ __global__ __launch_bounds__(256) void k_filt_x(const float *__restrict__ in, fl
     const bool rv = iz0 + r < Nz;
     // element e = i * kFxRows + r: row i of the wrapped x axis, z row iz0 + r
     stage_batched<16, 256>(
-        threadIdx.x, (Nx + 2 * K) * kFxRows,
+        threadIdx.x, (Nx + 2 * K + (KT ? kFiltR : 0)) * kFxRows,
         [&](int e) {
             const int i = e / kFxRows, rr = e % kFxRows;
+            if (i >= Nx + 2 * K) return 0.0f;   // window overrun of the last block (outputs not stored)
             int j = i - K;
             j = j < 0 ? j + Nx : (j >= Nx ? j - Nx : j);
             return iz0 + rr < Nz ? src[(size_t)j * Nz + iz0 + rr] : 0.0f;
@@ -240,6 +284,35 @@ __global__ __launch_bounds__(256) void k_filt_x(const float *__restrict__ in, fl
     __syncthreads();
     if (!rv) return;
     float *dst = out + (size_t)b * P + iz0 + r;
+    if (KT) {   // blocks of kFiltR consecutive x outputs per thread
+        constexpr int NT = 2 * (KT ? KT : 1) + 1, NW = kFiltR + NT - 1;
+        double tr[NT];
+#pragma unroll
+        for (int t = 0; t < NT; ++t) tr[t] = tk[t];
+        for (int ix0 = (threadIdx.x / kFxRows) * kFiltR; ix0 < Nx; ix0 += (256 / kFxRows) * kFiltR) {
+            const float *x = xs + ix0 * kFxRows + r;
+            double w[NW], acc[kFiltR];
+#pragma unroll
+            for (int i = 0; i < NW; ++i) w[i] = (double)x[i * kFxRows];
+#pragma unroll
+            for (int q = 0; q < kFiltR; ++q) {
+                acc[q] = 0.0;
+                if (transp) {
+#pragma unroll
+                    for (int t = 0; t < NT; ++t) acc[q] = fma(tr[t], w[q + t], acc[q]);
+                } else {
+#pragma unroll
+                    for (int t = 0; t < NT; ++t) acc[q] = fma(tr[t], w[q + NT - 1 - t], acc[q]);
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < kFiltR; ++q) {
+                if (ix0 + q >= Nx) break;
+                dst[(size_t)(ix0 + q) * Nz] = (float)acc[q];
+            }
+        }
+        return;
+    }
     for (int ix = threadIdx.x / kFxRows; ix < Nx; ix += 256 / kFxRows) {
         const float *x = xs + ix * kFxRows + r;
         double acc = 0.0;
@@ -2486,7 +2559,9 @@ static int default_gather_variant(const Plan *p, int Bp)
 {
     if (Bp == 1) return 0;
     if (Bp >= 64) {
-        for (int v : {31, 24})   // LDS tile gather in two detector groups, else the wave-uniform XCD gather
+        // LDS tile gathers in two detector groups (asynchronous, then register staging),
+        // else the wave-uniform XCD gather
+        for (int v : {35, 31, 24})
             if (gather_variant_ok(p, v, Bp)) return v;
         return 11;
     }
@@ -2543,33 +2618,47 @@ static int launch_gather(Plan *p, float *img, int B, int Bp, float scale, hipStr
     return MBREC_OK;
 }
 
+template <int KZ, int KX>
+static void prefilter_passes(Plan *p, const float *in, float *out, float *tmp, int B, bool transp,
+                             bool interleaved_out, int Bp, hipStream_t s)
+{
+    const size_t tapb = sizeof(float) * 2 * (size_t)(2 * std::max(p->Kz, p->Kx) + 2);
+    const size_t ldz = tapb + sizeof(float) * (size_t)kFzCols * ((p->Nz + 2 * p->Kz + (KZ ? kFiltR : 0)) | 1);
+    const size_t ldz0 = tapb + sizeof(float) * (size_t)kFzCols * ((p->Nz + 2 * p->Kz) | 1);
+    const size_t ldx = tapb + sizeof(float) * (size_t)kFxRows * (p->Nx + 2 * p->Kx + (KX ? kFiltR : 0));
+    const dim3 gx((p->Nz + kFxRows - 1) / kFxRows, B);
+    const int tr = transp ? 1 : 0;
+    const dim3 gzp(((size_t)B * p->Nx + kFzCols - 1) / kFzCols);
+    if (!transp) {   // forward: x pass then z pass (the z pass writes the interleaved layout)
+        hipLaunchKernelGGL((k_filt_x<KX>), gx, dim3(256), ldx, s, in, tmp, p->d_kx, p->Kx, p->Nz, p->Nx, 0);
+        if (interleaved_out)   // (measured: the register-blocked form is slower for this layout)
+            hipLaunchKernelGGL((k_filt_z<true, 0>), dim3(p->Nx, (Bp + kFzCols - 1) / kFzCols), dim3(256), ldz0, s,
+                               (const float *)tmp, out, p->d_kz, p->Kz, p->Nz, p->Nx, B, Bp, 0);
+        else
+            hipLaunchKernelGGL((k_filt_z<false, KZ>), gzp, dim3(256), ldz, s, (const float *)tmp, out, p->d_kz, p->Kz,
+                               p->Nz, p->Nx, B, Bp, 0);
+    } else {   // transpose: z^T then x^T
+        hipLaunchKernelGGL((k_filt_z<false, KZ>), gzp, dim3(256), ldz, s, in, tmp, p->d_kz, p->Kz, p->Nz, p->Nx, B,
+                           Bp, tr);
+        hipLaunchKernelGGL((k_filt_x<KX>), gx, dim3(256), ldx, s, (const float *)tmp, out, p->d_kx, p->Kx, p->Nz,
+                           p->Nx, tr);
+    }
+}
+
 static int launch_prefilter(Plan *p, const float *in, float *out, float *tmp, int B, bool transp,
                             bool interleaved_out, int Bp, hipStream_t s)
 {
     const size_t tapb = sizeof(float) * 2 * (size_t)(2 * std::max(p->Kz, p->Kx) + 2);
-    const size_t ldz = tapb + sizeof(float) * (size_t)kFzCols * ((p->Nz + 2 * p->Kz) | 1);
-    const size_t ldx = tapb + sizeof(float) * (size_t)kFxRows * (p->Nx + 2 * p->Kx);
+    const size_t ldz = tapb + sizeof(float) * (size_t)kFzCols * ((p->Nz + 2 * p->Kz + kFiltR) | 1);
+    const size_t ldx = tapb + sizeof(float) * (size_t)kFxRows * (p->Nx + 2 * p->Kx + kFiltR);
     if (ldz <= kLdsBudget && ldx <= kLdsBudget) {
-        // forward: x pass then z pass (the z pass writes the interleaved layout);
-        // transpose: z^T then x^T
-        const dim3 gx((p->Nz + kFxRows - 1) / kFxRows, B);
-        const int tr = transp ? 1 : 0;
-        if (!transp) {
-            hipLaunchKernelGGL(k_filt_x, gx, dim3(256), ldx, s, in, tmp, p->d_kx, p->Kx, p->Nz, p->Nx, 0);
-            if (interleaved_out)
-                hipLaunchKernelGGL(k_filt_z<true>, dim3(p->Nx, (Bp + kFzCols - 1) / kFzCols), dim3(256), ldz, s,
-                                   (const float *)tmp, out, p->d_kz, p->Kz, p->Nz, p->Nx, B, Bp, 0);
-            else
-                hipLaunchKernelGGL(k_filt_z<false>, dim3(((size_t)B * p->Nx + kFzCols - 1) / kFzCols), dim3(256), ldz,
-                                   s, (const float *)tmp, out, p->d_kz, p->Kz, p->Nz, p->Nx, B, Bp, 0);
-        } else {
-            if (interleaved_out)
-                return set_error(MBREC_ERR_ARG, "mbrec:internal", "interleaved output of the transposed prefilter");
-            hipLaunchKernelGGL(k_filt_z<false>, dim3(((size_t)B * p->Nx + kFzCols - 1) / kFzCols), dim3(256), ldz, s,
-                               in, tmp, p->d_kz, p->Kz, p->Nz, p->Nx, B, Bp, tr);
-            hipLaunchKernelGGL(k_filt_x, gx, dim3(256), ldx, s, (const float *)tmp, out, p->d_kx, p->Kx, p->Nz,
-                               p->Nx, tr);
-        }
+        if (transp && interleaved_out)
+            return set_error(MBREC_ERR_ARG, "mbrec:internal", "interleaved output of the transposed prefilter");
+        // taps per axis known at compile time for the common half-widths (register-blocked passes)
+        if (p->Kz == 11 && p->Kx == 11) prefilter_passes<11, 11>(p, in, out, tmp, B, transp, interleaved_out, Bp, s);
+        else if (p->Kz == 10 && p->Kx == 10) prefilter_passes<10, 10>(p, in, out, tmp, B, transp, interleaved_out, Bp, s);
+        else if (p->Kz == 12 && p->Kx == 12) prefilter_passes<12, 12>(p, in, out, tmp, B, transp, interleaved_out, Bp, s);
+        else prefilter_passes<0, 0>(p, in, out, tmp, B, transp, interleaved_out, Bp, s);
         MBREC_HIP(hipGetLastError());
         return MBREC_OK;
     }

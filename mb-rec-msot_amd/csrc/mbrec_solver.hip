@@ -110,24 +110,12 @@ __global__ void k_scal_init(Scal *sc, const double *part, int G, int B, Consts k
     }
 }
 
-// scale the three u blocks by inv_u (pending normalisation) of their group
-__global__ __launch_bounds__(256) void k_scale_u(float *__restrict__ us, size_t ns, float *__restrict__ ur,
-                                                 float *__restrict__ uc, size_t np, int B, Consts k,
-                                                 const Scal *sc)
-{
-    const int b = blockIdx.y;
-    const Scal &S = sc[k.coupled ? 0 : b];
-    if (!S.active) return;
-    const float f = (float)S.inv_u;
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < ns; i += (size_t)gridDim.x * blockDim.x)
-        us[(size_t)b * ns + i] *= f;
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < np; i += (size_t)gridDim.x * blockDim.x) {
-        if (k.reg) ur[(size_t)b * np + i] *= f;
-        uc[(size_t)b * np + i] *= f;
-    }
-}
-
-// v = sl*A'u_s + sr*u_r + sb*u_c - beta*v (first=1: without the beta term); ||v||^2 -> Q_E
+// u and v are kept unnormalised: the stored vectors times the pending factors
+// inv_u / inv_v of their group are LSQR's unit vectors, and every consumer
+// applies the factor as it reads (no separate scaling pass over the vectors).
+//
+// v = inv_u*(sl*A'u_s + sr*u_r + sb*u_c) - beta*inv_v*v (first=1: without the beta
+// term); ||v||^2 -> Q_E.  atu = A' applied to the stored u_s.
 __global__ __launch_bounds__(256) void k_v_update(const float *__restrict__ atu, const float *__restrict__ ur,
                                                   const float *__restrict__ uc, float *__restrict__ v,
                                                   size_t n, int B, Consts k, const Scal *sc, int first,
@@ -138,11 +126,12 @@ __global__ __launch_bounds__(256) void k_v_update(const float *__restrict__ atu,
     const size_t off = (size_t)b * n;
     double vv = 0.0;
     if (S.active) {
-        const float beta = first ? 0.0f : (float)S.beta;
+        const float bv = first ? 0.0f : (float)(S.beta * S.inv_v), fu = (float)S.inv_u;
         for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
             float t = k.sl * atu[off + i] + k.sb * uc[off + i];
             if (k.reg) t += sr_of(k, b) * ur[off + i];
-            const float vn = first ? t : t - beta * v[off + i];   // v is not yet defined when first
+            t *= fu;
+            const float vn = first ? t : t - bv * v[off + i];   // v is not yet defined when first
             v[off + i] = vn;
             vv += (double)vn * vn;
         }
@@ -163,18 +152,7 @@ __global__ void k_scal_v0(Scal *sc, const double *part, int G, int B, Consts k)
     }
 }
 
-__global__ __launch_bounds__(256) void k_scale_v(float *__restrict__ v, size_t n, int B, Consts k,
-                                                 const Scal *sc)
-{
-    const int b = blockIdx.y;
-    const Scal &S = sc[k.coupled ? 0 : b];
-    if (!S.active) return;
-    const float f = (float)S.inv_v;
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
-        v[(size_t)b * n + i] *= f;
-}
-
-// u_s = sl*(A v) - alpha*u_s; ||.||^2 -> Q_A
+// u_s = sl*inv_v*(A v) - alpha*inv_u*u_s (stored vectors); ||.||^2 -> Q_A
 __global__ __launch_bounds__(256) void k_u_sino(const float *__restrict__ av, float *__restrict__ us, size_t n,
                                                 int B, Consts k, const Scal *sc, double *part)
 {
@@ -183,9 +161,9 @@ __global__ __launch_bounds__(256) void k_u_sino(const float *__restrict__ av, fl
     const size_t off = (size_t)b * n;
     double uu = 0.0;
     if (S.active) {
-        const float a = (float)S.alpha;
+        const float a = (float)(S.alpha * S.inv_u), fv = (float)(k.sl * S.inv_v);
         for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
-            const float u = k.sl * av[off + i] - a * us[off + i];
+            const float u = fv * av[off + i] - a * us[off + i];
             us[off + i] = u;
             uu += (double)u * u;
         }
@@ -193,7 +171,7 @@ __global__ __launch_bounds__(256) void k_u_sino(const float *__restrict__ av, fl
     block_sum_store(uu, part, Q_A, b, B);
 }
 
-// u_r = sr*v - alpha*u_r, u_c = sb*v - alpha*u_c; ||.||^2 -> Q_C
+// u_r = sr*inv_v*v - alpha*inv_u*u_r, u_c = sb*inv_v*v - alpha*inv_u*u_c; ||.||^2 -> Q_C
 __global__ __launch_bounds__(256) void k_u_pix(const float *__restrict__ v, float *__restrict__ ur,
                                                float *__restrict__ uc, size_t n, int B, Consts k,
                                                const Scal *sc, double *part)
@@ -203,9 +181,9 @@ __global__ __launch_bounds__(256) void k_u_pix(const float *__restrict__ v, floa
     const size_t off = (size_t)b * n;
     double uu = 0.0;
     if (S.active) {
-        const float a = (float)S.alpha;
+        const float a = (float)(S.alpha * S.inv_u), fv = (float)S.inv_v;
         for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
-            const float vi = v[off + i];
+            const float vi = fv * v[off + i];
             if (k.reg) {
                 const float r = sr_of(k, b) * vi - a * ur[off + i];
                 ur[off + i] = r;
@@ -239,7 +217,7 @@ __global__ void k_scal_u(Scal *sc, const double *part, int G, int B, Consts k)
     }
 }
 
-// ||x_old||^2 -> Q_B, d = (v - thet*d)/rho, ||d||^2 -> Q_D, x += phi*d
+// ||x_old||^2 -> Q_B, d = (inv_v*v - thet*d)/rho, ||d||^2 -> Q_D, x += phi*d
 __global__ __launch_bounds__(256) void k_dx(const float *__restrict__ v, float *__restrict__ dv,
                                             float *__restrict__ x, size_t n, int B, Consts k,
                                             const Scal *sc, double *part)
@@ -249,11 +227,11 @@ __global__ __launch_bounds__(256) void k_dx(const float *__restrict__ v, float *
     const size_t off = (size_t)b * n;
     double xx = 0.0, dd = 0.0;
     if (S.active) {
-        const float thet = (float)S.thet, irho = (float)(1.0 / S.rho), phi = (float)S.phi;
+        const float thet = (float)S.thet, irho = (float)(1.0 / S.rho), phi = (float)S.phi, fv = (float)S.inv_v;
         for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
             const float xo = x[off + i];
             xx += (double)xo * xo;
-            const float dn = (v[off + i] - thet * dv[off + i]) * irho;
+            const float dn = (fv * v[off + i] - thet * dv[off + i]) * irho;
             dv[off + i] = dn;
             dd += (double)dn * dn;
             x[off + i] = xo + phi * dn;
@@ -379,31 +357,26 @@ static int lsqr_solve(Plan *p, Workspace &ws, const float *y, int B, int G, int 
     hipLaunchKernelGGL(k_init_pix, vgrid(B), dim3(256), 0, s, ws.w, ws.vm, ws.ur, ws.uc, ws.x, ws.d, np, B, k,
                        ws.part);
     hipLaunchKernelGGL(k_scal_init, dim3(1), dim3(256), 0, s, ws.sc, ws.part, G, B, k);
-    hipLaunchKernelGGL(k_scale_u, vgrid(B), dim3(256), 0, s, ws.us, ns, ws.ur, ws.uc, np, B, k, ws.sc);
     // v = A_aug' u
     if ((rc = op_transpose(p, ws.us, ws.tp, B, true, s, win))) return rc;
     hipLaunchKernelGGL(k_v_update, vgrid(B), dim3(256), 0, s, ws.tp, ws.ur, ws.uc, ws.v, np, B, k, ws.sc, 1,
                        ws.part);
     hipLaunchKernelGGL(k_scal_v0, dim3(1), dim3(256), 0, s, ws.sc, ws.part, G, B, k);
-    hipLaunchKernelGGL(k_scale_v, vgrid(B), dim3(256), 0, s, ws.v, np, B, k, ws.sc);
     for (int ii = 1; ii <= maxit; ++ii) {
         if ((rc = op_forward(p, ws.v, ws.ts, B, true, s, win))) return rc;
         hipLaunchKernelGGL(k_u_sino, vgrid(B), dim3(256), 0, s, ws.ts, ws.us, ns, B, k, ws.sc, ws.part);
         hipLaunchKernelGGL(k_u_pix, vgrid(B), dim3(256), 0, s, ws.v, ws.ur, ws.uc, np, B, k, ws.sc, ws.part);
         hipLaunchKernelGGL(k_scal_u, dim3(1), dim3(256), 0, s, ws.sc, ws.part, G, B, k);
         hipLaunchKernelGGL(k_dx, vgrid(B), dim3(256), 0, s, ws.v, ws.d, ws.x, np, B, k, ws.sc, ws.part);
-        // The transpose product that closes the last iteration (with the
-        // normalisation of u before it) feeds only the stopping tests and the
-        // next direction; x is final here, and on non-convergence the
-        // minimal-residual iterate is this last one (the residual estimate
-        // |s| normr never increases), so that work is dead.
+        // The transpose product that closes the last iteration feeds only the
+        // stopping tests and the next direction; x is final here, and on
+        // non-convergence the minimal-residual iterate is this last one (the
+        // residual estimate |s| normr never increases), so that work is dead.
         if (ii == maxit) break;
-        hipLaunchKernelGGL(k_scale_u, vgrid(B), dim3(256), 0, s, ws.us, ns, ws.ur, ws.uc, np, B, k, ws.sc);
         if ((rc = op_transpose(p, ws.us, ws.tp, B, true, s, win))) return rc;
         hipLaunchKernelGGL(k_v_update, vgrid(B), dim3(256), 0, s, ws.tp, ws.ur, ws.uc, ws.v, np, B, k, ws.sc, 0,
                            ws.part);
         hipLaunchKernelGGL(k_scal_v, dim3(1), dim3(256), 0, s, ws.sc, ws.part, G, B, k, ii);
-        hipLaunchKernelGGL(k_scale_v, vgrid(B), dim3(256), 0, s, ws.v, np, B, k, ws.sc);
     }
     MBREC_HIP(hipGetLastError());
     return MBREC_OK;
