@@ -41,6 +41,8 @@ def parse():
     ap.add_argument("--grid", type=int, default=400)
     ap.add_argument("--sir", action="store_true", help="400x400 with SIR (the SIR parity configuration)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--launch", default=None, metavar="GATHER,FORWARD",
+                    help="pin the gather variant and forward choice instead of autotuning (profiling runs)")
     ap.add_argument("--cpu-sample-reps", type=int, default=100)
     ap.add_argument("--traffic-calibration", action="store_true",
                     help="end with a plain 256 MiB device copy (reference for the FETCH/WRITE_SIZE counters)")
@@ -121,7 +123,12 @@ def main():
     model = MSOTmodel(probe, disc, 1500.0, use_sir, True, 0, 1e5, 12e6, None, sigma_iterations=40)
     D, nT = model.shape_sinogram
     B = a.batch
-    tune = model.plan.autotune(B, 5)
+    if a.launch:
+        gv, fc = (int(v) for v in a.launch.split(","))
+        model.plan.set_launch(B, gv, fc)
+        tune = (gv, fc)
+    else:
+        tune = model.plan.autotune(B, 5)
     log(f"setup {time.time() - t0:.1f} s, sigma {model.normalizationFactor:.6g}, "
         f"device tables {model.plan.device_bytes / 1e6:.1f} MB, launch choice {tune}")
     ph = phantom_batch(disc, B, 1000)
@@ -175,8 +182,9 @@ def main():
     reps = 10
     stages = ("scatter", "bins_to_y", "fir", "z_to_bins", "gather")
     st = {k: model.plan.time_stage(k, B, reps) for k in stages}
-    # launches per solve: 3 iters + 1 forward and 3 iters transpose applications
-    n_forward, n_transpose = 3 * a.iters + 1, 3 * a.iters
+    # launches per solve: per outer iteration 3 forward and 2 transpose applications
+    # (LSQR's closing transpose of its last inner iteration is dead and not launched)
+    n_forward, n_transpose = 3 * a.iters, 2 * a.iters
     per_solve = {"scatter": n_forward * st["scatter"], "bins_to_y": n_forward * st["bins_to_y"],
                  "fir": (n_forward + n_transpose) * st["fir"], "z_to_bins": n_transpose * st["z_to_bins"],
                  "gather": n_transpose * st["gather"]}

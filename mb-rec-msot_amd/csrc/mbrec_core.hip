@@ -421,7 +421,7 @@ static int build_tile_windows(Plan *p, const std::vector<int32_t> &tq)
 
 static void free_plan(Plan *p)
 {
-    void *ptrs[] = {p->d_pix, p->d_tq, p->d_sidx, p->d_sir, p->d_swin, p->d_ir, p->d_irf,
+    void *ptrs[] = {p->d_pix, p->d_tq, p->d_sidx, p->d_sir, p->d_swin, p->d_ir, p->d_irf, p->d_ir64, p->d_irf64,
                     p->d_kz,  p->d_kx, p->d_rec, p->d_rec_off, p->d_rec_k, p->d_twin, p->d_ptab, p->d_ttab};
     for (void *q : ptrs)
         if (q) (void)hipFree(q);
@@ -580,10 +580,21 @@ int mbrec_plan_set_ir(mbrec_plan h, const double *ir, int Lir, int per_det)
     const int rows = per_det ? p->D : 1;
     for (int r = 0; r < rows; ++r)
         for (int i = 0; i < Lir; ++i) ff[(size_t)r * Lir + i] = f[(size_t)r * Lir + (Lir - 1 - i)];
-    if (p->d_ir) { (void)hipFree(p->d_ir); (void)hipFree(p->d_irf); }
+    // FP64 copies zero-padded to whole groups of 16 taps (exact: the FP32 taps widened)
+    const int Lp = (Lir + 15) & ~15;
+    std::vector<double> d64((size_t)rows * Lp, 0.0), df64((size_t)rows * Lp, 0.0);
+    for (int r = 0; r < rows; ++r)
+        for (int i = 0; i < Lir; ++i) {
+            d64[(size_t)r * Lp + i] = (double)f[(size_t)r * Lir + i];
+            df64[(size_t)r * Lp + i] = (double)ff[(size_t)r * Lir + i];
+        }
+    if (p->d_ir) { (void)hipFree(p->d_ir); (void)hipFree(p->d_irf); (void)hipFree(p->d_ir64); (void)hipFree(p->d_irf64); }
     int rc;
     if ((rc = dalloc(&p->d_ir, n, &p->table_bytes)) || (rc = upload(p->d_ir, f.data(), n)) ||
-        (rc = dalloc(&p->d_irf, n, &p->table_bytes)) || (rc = upload(p->d_irf, ff.data(), n)))
+        (rc = dalloc(&p->d_irf, n, &p->table_bytes)) || (rc = upload(p->d_irf, ff.data(), n)) ||
+        (rc = dalloc(&p->d_ir64, d64.size(), &p->table_bytes)) || (rc = upload(p->d_ir64, d64.data(), d64.size())) ||
+        (rc = dalloc(&p->d_irf64, df64.size(), &p->table_bytes)) ||
+        (rc = upload(p->d_irf64, df64.data(), df64.size())))
         return rc;
     p->Lir = Lir;
     p->ir_per_det = per_det ? 1 : 0;

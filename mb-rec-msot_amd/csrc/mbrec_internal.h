@@ -90,6 +90,8 @@ struct Plan {
     // impulse response (taps of the convolution and of the correlation = flipped IR)
     float *d_ir = nullptr;       // [Lir] or [D][Lir]
     float *d_irf = nullptr;      // flipped
+    double *d_ir64 = nullptr;    // [rows][16 ceil(Lir / 16)] the same taps in FP64, zero-padded (FIR kernel)
+    double *d_irf64 = nullptr;   // flipped
     int Lir = 0, ir_per_det = 0;
     // prefilter
     float *d_kz = nullptr, *d_kx = nullptr;

@@ -517,7 +517,8 @@ __global__ __launch_bounds__(256) void k_fir(const float *__restrict__ x, float 
 // 31-sample register window, so every staged sample feeds 16 FMAs.  A half
 // workgroup (128 threads) covers a tile of kFirSpan outputs of one row; the
 // two halves take consecutive (row, tile) pairs.  Outputs go back through LDS
-// for contiguous stores.  Same sums as k_fir (FP64, taps ascending).
+// for contiguous stores; the taps are FP64 scalar operands.  Same sums as k_fir
+// (FP64, taps ascending).
 constexpr int kFirOut = 16;
 constexpr int kFirSpan = 128 * kFirOut;
 
@@ -528,14 +529,13 @@ __host__ __device__ constexpr int fir16_nblk(int Lp) { return ((kFirSpan + Lp + 
 __host__ __device__ constexpr int fir16_window(int Lp) { return 16 * fir16_nblk(Lp); }
 
 __global__ __launch_bounds__(256) void k_fir16(const float *__restrict__ x, float *__restrict__ out,
-                                               const float *__restrict__ taps, int L, int per_det, int off, int D,
+                                               const double *__restrict__ taps64, int L, int per_det, int off, int D,
                                                int nT, int rows, float scale, const int2 *__restrict__ win)
 {
     extern __shared__ float sm[];
     const int Lp = fir16_padded_taps(L), nx = fir16_window(Lp), NBLK = fir16_nblk(Lp);
-    const int hf = threadIdx.x >> 7, lt = threadIdx.x & 127;
-    double *tk = reinterpret_cast<double *>(sm) + (size_t)hf * Lp;
-    float *xs = sm + 2 * 2 * Lp + (size_t)hf * nx;
+    const int hf = __builtin_amdgcn_readfirstlane(threadIdx.x >> 7), lt = threadIdx.x & 127;
+    float *xs = sm + (size_t)hf * nx;
     const int ntile = (nT + kFirSpan - 1) / kFirSpan;
     const int tile = blockIdx.x * 2 + hf;
     const bool live = tile < rows * ntile;
@@ -557,33 +557,41 @@ __global__ __launch_bounds__(256) void k_fir16(const float *__restrict__ x, floa
             return (live && t >= lo && t < hi) ? xr[t] : 0.0f;
         },
         [&](int i, float v) { xs[(i & 15) * NBLK + (i >> 4)] = v; });
-    const float *tp = taps + (per_det ? (size_t)(row % D) * L : 0);
-    for (int i = lt; i < Lp; i += 128) tk[i] = i < L ? (double)tp[i] : 0.0;
+    // the row's taps, FP64 and zero-padded to Lp, read as scalars (uniform per wave)
+    const double *tk = taps64 + (per_det ? (size_t)(row % D) * Lp : 0);
     __syncthreads();
-    // output t0 + 16 lt + k at tap i reads element 16 lt + k - i + Lp + 15; the window
-    // W[w] = element 16 (lt + (Lp - i0) / 16) + w, i.e. block lt + (Lp - i0) / 16, lane w
+    // output t0 + 16 lt + k at tap i reads element 16 lt + k - i + Lp + 15.  Taps go in
+    // groups of 8: for the group starting at tap g the window is W[w] = element
+    // 16 lt + Lp + 8 - g + w (w < 23), read as W[k - q + 7]; the next group's window is
+    // this one moved up by 8 with 8 new elements below (lanes 0..7 of a block after a
+    // group at an even multiple of 8, lanes 8..15 of the block before after an odd one).
     const int blk = lt + Lp / 16;
-    double W[31], acc[kFirOut];
+    double W[23], acc[kFirOut];
 #pragma unroll
-    for (int w = 0; w < 16; ++w) W[w] = (double)xs[w * NBLK + blk];
+    for (int w = 0; w < 8; ++w) W[w] = (double)xs[(w + 8) * NBLK + blk];
 #pragma unroll
-    for (int w = 16; w < 31; ++w) W[w] = (double)xs[(w - 16) * NBLK + blk + 1];
+    for (int w = 8; w < 23; ++w) W[w] = (double)xs[(w - 8) * NBLK + blk + 1];
 #pragma unroll
     for (int k = 0; k < kFirOut; ++k) acc[k] = 0.0;
-    for (int i0 = 0; i0 < Lp; i0 += 16) {
+    auto group = [&](int g) {
 #pragma unroll
-        for (int q = 0; q < 16; ++q) {
-            const double w = tk[i0 + q];
+        for (int q = 0; q < 8; ++q) {
+            const double w = tk[g + q];
 #pragma unroll
-            for (int k = 0; k < kFirOut; ++k) acc[k] = fma(w, W[k - q + 15], acc[k]);
+            for (int k = 0; k < kFirOut; ++k) acc[k] = fma(w, W[k - q + 7], acc[k]);
         }
-        if (i0 + 16 < Lp) {
+    };
+    auto shift = [&](int bn, int lane0) {
 #pragma unroll
-            for (int w = 30; w >= 16; --w) W[w] = W[w - 16];
-            const int bn = blk - (i0 + 16) / 16;
+        for (int w = 22; w >= 8; --w) W[w] = W[w - 8];
 #pragma unroll
-            for (int w = 0; w < 16; ++w) W[w] = (double)xs[w * NBLK + bn];
-        }
+        for (int w = 0; w < 8; ++w) W[w] = (double)xs[(lane0 + w) * NBLK + bn];
+    };
+    for (int i0 = 0; i0 < Lp; i0 += 16) {   // Lp is a multiple of 16
+        group(i0);
+        shift(blk - i0 / 16, 0);
+        group(i0 + 8);
+        if (i0 + 16 < Lp) shift(blk - i0 / 16 - 1, 8);
     }
     __syncthreads();   // xs is reused for the outputs (same layout)
 #pragma unroll
@@ -2298,12 +2306,12 @@ static int launch_fir(Plan *p, const float *x, float *out, int B, bool corr, flo
     const int L = p->Lir;
     const int off = L / 2;            // odd L: the same offset serves the convolution and the flipped-IR correlation
     const int Lp = fir16_padded_taps(L);
-    const size_t lds16 = 2 * (sizeof(double) * (size_t)Lp + sizeof(float) * (size_t)fir16_window(Lp));
-    if (lds16 <= kLdsBudget) {
+    const size_t lds16 = 2 * sizeof(float) * (size_t)fir16_window(Lp);
+    if (lds16 <= kLdsBudget && p->d_ir64) {
         const int rows = B * p->D, ntile = (p->nT + kFirSpan - 1) / kFirSpan;
         const size_t nblk = ((size_t)rows * ntile + 1) / 2;
-        hipLaunchKernelGGL(k_fir16, dim3((unsigned)nblk), dim3(256), lds16, s, x, out, corr ? p->d_irf : p->d_ir, L,
-                           p->ir_per_det, off, p->D, p->nT, rows, scale, win);
+        hipLaunchKernelGGL(k_fir16, dim3((unsigned)nblk), dim3(256), lds16, s, x, out, corr ? p->d_irf64 : p->d_ir64,
+                           L, p->ir_per_det, off, p->D, p->nT, rows, scale, win);
         MBREC_HIP(hipGetLastError());
         return MBREC_OK;
     }
