@@ -182,6 +182,80 @@ static int build_pull_records(Plan *p, const std::vector<int32_t> &slot_pix, con
 // nn lies in [0, nT + 2K); with SIR (sidx, win given) iff one of its taps
 // nn + s - Ls/2, s in its window, does, and its SIR index is stored with it.
 // The window margins lo_off / hi_off must already cover the tap displacements.
+// SIR forward records in pull form (see the Plan fields): a pair's taps reach samples
+// [nf, nf + len) with nf = nn + s0 - Ls/2 (bin-space sample index nn = n + K); pairs
+// none of whose taps land in [0, nT + 2K) are left out.  u exactly as decode_tau.
+static int build_sir_pull(Plan *p, const std::vector<int32_t> &slot_pix, const std::vector<int32_t> &tq,
+                          const int32_t *sidx, const int32_t *win, int Ls)
+{
+    const BinGeom &g = p->geom;
+    const int span = g.nT + 2 * g.K;
+    const int half = Ls / 2;
+    int maxlen = 0;
+    for (int k = 0; k < p->Nsir_build; ++k) maxlen = std::max(maxlen, (int)win[2 * (size_t)k + 1]);
+    // nf ranges over [-(maxlen - 1), span - 1] for pairs with a tap in range
+    const int lo = -std::max(maxlen - 1, 0), n_nf = span - lo + 1;
+    if ((size_t)p->Nsir_build * Ls >= ((size_t)1 << 26) || maxlen >= 64 || p->P >= (1 << 28))
+        return MBREC_OK;   // the packed record does not fit: the SIR forward keeps the bucketed form only
+    std::vector<int32_t> off((size_t)p->D * (n_nf + 1), 0);
+    std::vector<int32_t> nfb((size_t)p->D * p->Pd, INT32_MIN);
+    for (int d = 0; d < p->D; ++d)
+        for (int sl = 0; sl < p->Pd; ++sl) {
+            if (slot_pix[sl] < 0) continue;
+            const size_t at = (size_t)d * p->Pd + sl;
+            const int k = sidx[at];
+            const int s0 = win[2 * (size_t)k], len = win[2 * (size_t)k + 1];
+            int nn, m;
+            host_decode(tq[at], g, nn, m);
+            if (tq[at] == INT32_MIN || len <= 0) continue;
+            const long long nf = (long long)nn + s0 - half;
+            if (nf + len - 1 < 0 || nf >= span) continue;   // no tap in range
+            nfb[at] = (int32_t)nf;
+            off[(size_t)d * (n_nf + 1) + (nf - lo) + 1]++;
+        }
+    size_t total = 0;
+    for (int d = 0; d < p->D; ++d) {
+        int32_t *o = &off[(size_t)d * (n_nf + 1)];
+        if (total + (size_t)p->Pd > (size_t)INT32_MAX) return MBREC_OK;
+        o[0] = (int32_t)total;
+        for (int i = 0; i < n_nf; ++i) o[i + 1] += o[i];
+        total = o[n_nf];
+    }
+    std::vector<int4> rec(total + kRecPad, make_int4(0, 0, 0, 0));
+    std::vector<int32_t> fill(off);
+    const float scale = std::ldexp(1.0f, -g.F);
+    for (int d = 0; d < p->D; ++d)
+        for (int sl = 0; sl < p->Pd; ++sl) {
+            const size_t at = (size_t)d * p->Pd + sl;
+            if (nfb[at] == INT32_MIN) continue;
+            const int k = sidx[at];
+            const int s0 = win[2 * (size_t)k], len = win[2 * (size_t)k + 1];
+            int nn, m;
+            host_decode(tq[at], g, nn, m);
+            const float fr = (float)(tq[at] & ((1 << g.F) - 1)) * scale;
+            const float u = m == 0 ? fr * g.iw0 : (m == 1 ? (fr - g.phi1) * g.iw1 : (fr - g.phi2) * g.iw2);
+            int32_t ub;
+            std::memcpy(&ub, &u, sizeof(ub));
+            const int base = k * Ls + s0;
+            rec[fill[(size_t)d * (n_nf + 1) + (nfb[at] - lo)]++] =
+                make_int4(slot_pix[sl] | (m << 28), nfb[at], ub, base | (len << 26));
+        }
+    if (p->d_srec) { (void)hipFree(p->d_srec); (void)hipFree(p->d_srec_off); }
+    p->table_bytes -= p->srec_bytes;
+    p->srec_bytes = 0;
+    p->d_srec = nullptr;
+    p->d_srec_off = nullptr;
+    int rc;
+    if ((rc = dalloc(&p->d_srec, rec.size(), &p->srec_bytes)) ||
+        (rc = dalloc(&p->d_srec_off, off.size(), &p->srec_bytes)) || (rc = upload(p->d_srec, rec.data(), rec.size())) ||
+        (rc = upload(p->d_srec_off, off.data(), off.size())))
+        return rc;
+    p->table_bytes += p->srec_bytes;
+    p->sp_lo = lo;
+    p->sp_n = n_nf;
+    return MBREC_OK;
+}
+
 int build_records(Plan *p, const std::vector<int32_t> &slot_pix, const std::vector<int32_t> &tq,
                   const int32_t *sidx, const int32_t *win, int Ls)
 {
@@ -422,7 +496,7 @@ static int build_tile_windows(Plan *p, const std::vector<int32_t> &tq)
 static void free_plan(Plan *p)
 {
     void *ptrs[] = {p->d_pix, p->d_tq, p->d_sidx, p->d_sir, p->d_swin, p->d_ir, p->d_irf, p->d_ir64, p->d_irf64,
-                    p->d_kz,  p->d_kx, p->d_rec, p->d_rec_off, p->d_rec_k, p->d_twin, p->d_ptab, p->d_ttab};
+                    p->d_kz,  p->d_kx, p->d_rec, p->d_rec_off, p->d_rec_k, p->d_twin, p->d_ptab, p->d_ttab, p->d_srec, p->d_srec_off};
     for (void *q : ptrs)
         if (q) (void)hipFree(q);
     free_work(p);
@@ -558,6 +632,8 @@ int mbrec_plan_set_sir(mbrec_plan h, const int32_t *sirIdx, const double *sir, i
         std::vector<int32_t> tq((size_t)p->D * p->Pd);
         MBREC_HIP(hipMemcpy(tq.data(), p->d_tq, sizeof(int32_t) * tq.size(), hipMemcpyDeviceToHost));
         if ((rc = build_records(p, slot_pix, tq, sidx.data(), win.data(), Ls))) return rc;
+        p->Nsir_build = Nsir;
+        if ((rc = build_sir_pull(p, slot_pix, tq, sidx.data(), win.data(), Ls))) return rc;
     }
     p->use_sir = 1;
     p->Ls = Ls;

@@ -79,6 +79,14 @@ struct Plan {
     int2 *d_rec = nullptr;       // [records] (pixel, fixed-point travel time) / (pixel, u)
     int32_t *d_rec_off = nullptr;   // [D][nbk + 1] record offsets / [D][NB + 1]
     int32_t *d_rec_k = nullptr;  // [records] 0-based SIR index (SIR plans)
+    // SIR forward in pull form: one record per (pixel, detector) pair with a tap in the
+    // bin range, int4 (pixel | phase << 28, first affected sample nf = n + s0 - Ls/2 + K,
+    // u bits, SIR table offset of the first tap | tap count << 26), sorted by (detector, nf);
+    // CSR over nf in [sp_lo, sp_lo + sp_n) per detector
+    int4 *d_srec = nullptr;
+    int32_t *d_srec_off = nullptr;
+    int sp_lo = 0, sp_n = 0, Nsir_build = 0;
+    size_t srec_bytes = 0;
     size_t nrec = 0;
     size_t rec_bytes = 0;              // device bytes of d_rec / d_rec_off / d_rec_k
     // SIR

@@ -379,17 +379,34 @@ __global__ __launch_bounds__(256) void k_scatter_bk(
             const int s0 = swin[2 * k], len = swin[2 * k + 1];
             const float *hk = sir + (size_t)k * Ls;
             const int half = Ls / 2;
-            for (int s = s0; s < s0 + len; ++s) {
-                const int nn = dc.nn + s - half;
-                if ((unsigned)nn >= (unsigned)span) continue;
-                const int jj = jl + 3 * (s - half);
-                if ((unsigned)jj >= (unsigned)(WL - 1)) continue;   // cannot happen for a consistent plan
-                const float wv = hk[s];
-                const float w0 = wv * u0, w1 = wv * u1;
-                for (int b = 0; b < nb; ++b) {
-                    const float cv = c[b];
-                    atomicAdd(&L[b * WLp + jj], cv * w0);
-                    atomicAdd(&L[b * WLp + jj + 1], cv * w1);
+            const float c0 = c[0];   // the first sinogram's value, kept for the common single-sinogram case
+            // taps in runs of 8: the run's SIR weights are loaded before any is deposited
+            for (int cs = 0; cs < len; cs += 8) {
+                float wt[8];
+                int jj[8];
+#pragma unroll
+                for (int t = 0; t < 8; ++t) {
+                    const int s = s0 + cs + t;
+                    const int nn = dc.nn + s - half;
+                    jj[t] = jl + 3 * (s - half);
+                    // taps past the window, outside the bin range or (cannot happen for a
+                    // consistent plan) outside the LDS window deposit nothing
+                    const bool ok = cs + t < len && (unsigned)nn < (unsigned)span &&
+                                    (unsigned)jj[t] < (unsigned)(WL - 1);
+                    wt[t] = ok ? hk[s] : 0.0f;
+                    if (!ok) jj[t] = -1;
+                }
+#pragma unroll
+                for (int t = 0; t < 8; ++t) {
+                    if (jj[t] < 0) continue;
+                    const float w0 = wt[t] * u0, w1 = wt[t] * u1;
+                    atomicAdd(&L[jj[t]], c0 * w0);
+                    atomicAdd(&L[jj[t] + 1], c0 * w1);
+                    for (int b = 1; b < nb; ++b) {
+                        const float cv = c[b];
+                        atomicAdd(&L[b * WLp + jj[t]], cv * w0);
+                        atomicAdd(&L[b * WLp + jj[t] + 1], cv * w1);
+                    }
                 }
             }
         }
@@ -881,16 +898,30 @@ __global__ __launch_bounds__(256) void k_gather(
                 float part[BC];
 #pragma unroll
                 for (int b = 0; b < BC; ++b) part[b] = 0.0f;
-                for (int s = s0; s < s0 + len; ++s) {
-                    const int nn = shift + s;
-                    if ((unsigned)nn >= (unsigned)span) continue;
-                    const float wvl = hk[s];
-                    const int j = 3 * nn + dc.m;
-                    float g0[BC], g1[BC];
-                    load_bins<BC>(gb + (size_t)j * Bp, g0);
-                    load_bins<BC>(gb + (size_t)(j + 1) * Bp, g1);
+                // taps in runs of ST whose loads are all issued before any is used; taps
+                // outside the window or the bin range read nothing and add 0 (same order)
+                constexpr int ST = BC <= 1 ? 8 : (BC <= 4 ? 4 : 2);
+                for (int c = 0; c < len; c += ST) {
+                    float wt[ST], g0[ST][BC], g1[ST][BC];
 #pragma unroll
-                    for (int b = 0; b < BC; ++b) part[b] = fmaf(wvl, fmaf(dc.u, g1[b] - g0[b], g0[b]), part[b]);
+                    for (int t = 0; t < ST; ++t) {
+                        const int s = s0 + c + t, nn = shift + s;
+                        const bool ok = c + t < len && (unsigned)nn < (unsigned)span;
+                        const int j = ok ? 3 * nn + dc.m : 0;
+                        wt[t] = ok ? hk[s] : 0.0f;
+                        if (ok) {
+                            load_bins<BC>(gb + (size_t)j * Bp, g0[t]);
+                            load_bins<BC>(gb + (size_t)(j + 1) * Bp, g1[t]);
+                        } else {
+#pragma unroll
+                            for (int b = 0; b < BC; ++b) g0[t][b] = g1[t][b] = 0.0f;
+                        }
+                    }
+#pragma unroll
+                    for (int t = 0; t < ST; ++t)
+#pragma unroll
+                        for (int b = 0; b < BC; ++b)
+                            part[b] = fmaf(wt[t], fmaf(dc.u, g1[t][b] - g0[t][b], g0[t][b]), part[b]);
                 }
 #pragma unroll
                 for (int b = 0; b < BC; ++b) kahan_add(acc[b], cmp[b], part[b]);
@@ -1016,15 +1047,21 @@ __global__ __launch_bounds__(256) void k_gather_bl(
                 const float *hk = sir + (size_t)k * Ls;
                 const int shift = dc.nn - Ls / 2;
                 float part = 0.0f;
-                for (int s = s0; s < s0 + len; ++s) {
-                    const int nn = shift + s;
-                    if ((unsigned)nn >= (unsigned)span) continue;
-                    const float w = hk[s];
-                    if (bv) {
-                        const int j = 3 * nn + dc.m;
-                        const float g0 = gb[(size_t)j * Bp], g1 = gb[(size_t)(j + 1) * Bp];
-                        part = fmaf(w, fmaf(dc.u, g1 - g0, g0), part);
+                // taps in runs of 8 whose loads are all issued before any is used; taps
+                // outside the window or the bin range read nothing and add 0 (same order)
+                for (int c = 0; c < len; c += 8) {
+                    float wt[8], g0[8], g1[8];
+#pragma unroll
+                    for (int t = 0; t < 8; ++t) {
+                        const int s = s0 + c + t, nn = shift + s;
+                        const bool ok = c + t < len && (unsigned)nn < (unsigned)span;
+                        const int j = ok ? 3 * nn + dc.m : 0;
+                        wt[t] = ok ? hk[s] : 0.0f;
+                        g0[t] = ok && bv ? gb[(size_t)j * Bp] : 0.0f;
+                        g1[t] = ok && bv ? gb[(size_t)(j + 1) * Bp] : 0.0f;
                     }
+#pragma unroll
+                    for (int t = 0; t < 8; ++t) part = fmaf(wt[t], fmaf(dc.u, g1[t] - g0[t], g0[t]), part);
                 }
                 kahan_add(acc, cmp, part);
             }
@@ -1465,6 +1502,74 @@ __global__ __launch_bounds__(256) void k_fwd_pull_b1(const int2 *__restrict__ re
     for (int g = 0; g < NG; ++g) {
         const int J = ja + 64 * g + lane;
         if (J < NB) bins[(size_t)d * NB + J] = a0[g] + a1[g];
+    }
+}
+
+// SIR forward for a single sinogram, pull form (records of build_sir_pull): a wave owns
+// samples a - 1 .. a + 14 of one detector in four lane groups of 16 (lane i of a group:
+// sample a - 1 + i).  Group g takes every fourth record of the records whose taps can
+// reach those samples; per record each lane adds the tap landing on its sample (if any)
+// to its phase accumulators (u-part of phase 2 into a carry for the next sample's phase
+// 0).  No atomics: the four groups are added at the end, and lane 0 (sample a - 1) only
+// hands its carry on.  Same terms as k_scatter_bk (c (h (1 - u)), c (h u)), summed in
+// record order instead of atomic order.
+__global__ __launch_bounds__(256) void k_fwd_sir_pull1(const int4 *__restrict__ rec, const int32_t *__restrict__ off,
+                                                       const float *__restrict__ img, const float *__restrict__ sir,
+                                                       float *__restrict__ bins, int NB, int span, int lo, int n_nf,
+                                                       int maxlen)
+{
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int g = lane >> 4, i = lane & 15;
+    const int d = blockIdx.y;
+    const int a = (blockIdx.x * 4 + wv) * 15;   // first output sample of the wave
+    if (a > span) return;                       // samples 0 .. span (span: only bin 3 span = NB - 1); no barriers
+    const int n = a - 1 + i;
+    const bool live = n >= 0 && n < span;       // taps land on samples in [0, span) only
+    const int32_t *o = off + (size_t)d * (n_nf + 1);
+    // records with nf in [a - maxlen, a + 14] can reach samples a - 1 .. a + 14
+    const int ia = min(max(a - maxlen - lo, 0), n_nf), ib = min(max(a + 15 - lo, 0), n_nf);
+    const int rs = o[ia], re = o[ib];
+    float acc0 = 0.0f, acc1 = 0.0f, acc2 = 0.0f, carry = 0.0f;
+    constexpr int U = 4;   // records per group whose loads are in flight together
+    for (int r0 = rs + g; r0 < re; r0 += 4 * U) {
+        int4 e[U];
+        float c[U], h[U];
+#pragma unroll
+        for (int q = 0; q < U; ++q) e[q] = rec[min(r0 + 4 * q, re - 1)];   // clamped: duplicates are masked below
+#pragma unroll
+        for (int q = 0; q < U; ++q) {
+            c[q] = img[e[q].x & 0x0fffffff];
+            const int t = n - e[q].y, len = (int)((unsigned)e[q].w >> 26);
+            const bool ok = live && r0 + 4 * q < re && t >= 0 && t < len;
+            h[q] = ok ? sir[(e[q].w & 0x03ffffff) + t] : 0.0f;
+        }
+#pragma unroll
+        for (int q = 0; q < U; ++q) {
+            const int m = (int)((unsigned)e[q].x >> 28);
+            const float u = __int_as_float(e[q].z);
+            const float w0 = c[q] * (h[q] * (1.0f - u)), w1 = c[q] * (h[q] * u);
+            acc0 += m == 0 ? w0 : 0.0f;
+            acc1 += m == 0 ? w1 : (m == 1 ? w0 : 0.0f);
+            acc2 += m == 1 ? w1 : (m == 2 ? w0 : 0.0f);
+            carry += m == 2 ? w1 : 0.0f;
+        }
+    }
+    // the four groups' sums (lanes i, i + 16, i + 32, i + 48), then the carry of sample n - 1
+    acc0 += __shfl_xor(acc0, 32, 64);
+    acc1 += __shfl_xor(acc1, 32, 64);
+    acc2 += __shfl_xor(acc2, 32, 64);
+    carry += __shfl_xor(carry, 32, 64);
+    acc0 += __shfl_xor(acc0, 16, 64);
+    acc1 += __shfl_xor(acc1, 16, 64);
+    acc2 += __shfl_xor(acc2, 16, 64);
+    carry += __shfl_xor(carry, 16, 64);
+    const float cin = __shfl_up(carry, 1, 64);
+    if (g != 0 || i == 0 || n > span) return;
+    float *row = bins + (size_t)d * NB;
+    if (3 * n < NB) row[3 * n] = acc0 + cin;
+    if (n < span) {
+        row[3 * n + 1] = acc1;
+        row[3 * n + 2] = acc2;
     }
 }
 
@@ -2076,9 +2181,17 @@ static void scatter_bl(Plan *p, const float *imgI, int B, int Bp, hipStream_t s)
 
 // scatter launch choice: > 0 record-lane kernel with that LDS batch chunk, < 0
 // batch-lane kernel with lane groups of -value sinograms
+// SIR forward in pull form (records of build_sir_pull; single sinogram): a scatter choice
+// of its own, the bins written directly (no bucket windows)
+constexpr int kSirPull = 1000;
+
+static bool sir_pull_ok(const Plan *p, int Bp) { return p->use_sir && p->d_srec && Bp == 1; }
+
 static int scatter_choice(const Plan *p, int Bp)
 {
-    if (p->tune.Bp == Bp && p->tune.scatter_bc != 0) {
+    if (p->tune.Bp == Bp && p->tune.scatter_bc == kSirPull && sir_pull_ok(p, Bp)) return kSirPull;
+    if (!(p->tune.Bp == Bp && p->tune.scatter_bc != 0) && sir_pull_ok(p, Bp)) return kSirPull;
+    if (p->tune.Bp == Bp && p->tune.scatter_bc != 0 && p->tune.scatter_bc != kSirPull) {
         const int c = p->tune.scatter_bc;
         if (c > 0) return std::min(c, scatter_chunk_max(p, Bp));
         if (-c <= scatter_lb_max(p, Bp)) return c;
@@ -2216,9 +2329,21 @@ static int launch_pull(Plan *p, const float *imgI, int Bp, hipStream_t s)
     return MBREC_OK;
 }
 
+// whether the forward pairs stage writes the bins themselves (pull forms) rather than
+// overlapping bucket windows
+static bool forward_writes_bins(const Plan *p, int Bp) { return p->pull || scatter_choice(p, Bp) == kSirPull; }
+
 static int launch_scatter(Plan *p, const float *imgI, int B, int Bp, hipStream_t s)
 {
     if (p->pull) return launch_pull(p, imgI, Bp, s);
+    if (scatter_choice(p, Bp) == kSirPull) {
+        const int nblk = (p->geom.nT + 2 * p->geom.K + 1 + 59) / 60;   // 4 waves x 15 samples per workgroup
+        hipLaunchKernelGGL(k_fwd_sir_pull1, dim3(nblk, p->D), dim3(256), 0, s, p->d_srec, p->d_srec_off, imgI,
+                           p->d_sir, p->d_bins, p->geom.NB, p->geom.nT + 2 * p->geom.K, p->sp_lo, p->sp_n,
+                           p->sir_maxlen);
+        MBREC_HIP(hipGetLastError());
+        return MBREC_OK;
+    }
     const int WLp = p->WL | 1;
     if (p->WL < p->Wb + 1 || p->lo_off > 0 || p->hi_off < 1 || p->WL != p->Wb + p->hi_off - p->lo_off)
         return set_error(MBREC_ERR_ARG, "mbrec:internal", "scatter window not set up (WL=%d)", p->WL);
@@ -2257,7 +2382,7 @@ static void bins_to_y_bc_nw(Plan *p, float *y, int B, int Bp, float scale, hipSt
     const int nj = 3 * (TY + p->taps.nw) + 3;
     const size_t lds = sizeof(float) * (3 * MBREC_MAX_TAPS + (size_t)BC * (nj | 1) + 2 * BC + 2);
     dim3 grid((p->nT + TY - 1) / TY, p->D, (Bp + BC - 1) / BC);
-    if (p->pull)
+    if (forward_writes_bins(p, Bp))
         hipLaunchKernelGGL((k_bins_to_y_bc<BC, TY, true, NW>), grid, dim3(256), lds, s, p->d_bins, y, p->taps,
                            p->geom, p->Wb, p->nbk, p->lo_off, p->WL, Bp, p->D, B, scale, win);
     else
@@ -2290,7 +2415,7 @@ static int launch_bins_to_y(Plan *p, float *y, int B, int Bp, float scale, hipSt
     // LDS chunk of sinograms: at most kHatLds bytes, so several workgroups share a CU
     const int Bc = std::max(1, std::min(B, (int)(kHatLds / (sizeof(float) * (size_t)(nj | 1)))));
     dim3 grid((p->nT + kYTile - 1) / kYTile, p->D, (B + Bc - 1) / Bc);
-    if (p->pull)
+    if (forward_writes_bins(p, Bp))
         hipLaunchKernelGGL(k_bins_to_y<true>, grid, dim3(256), sizeof(float) * (size_t)Bc * (nj | 1), s, p->d_bins, y,
                            p->taps, p->geom, p->Wb, p->nbk, p->lo_off, p->WL, Bp, p->D, B, Bc, scale, win);
     else
@@ -2819,6 +2944,7 @@ int autotune(Plan *p, int B, int reps, hipStream_t s, int *gather_out, int *scat
         for (int bc = (Bp == 1 ? 1 : 4); bc < mx; bc *= 2) cand.push_back(bc);
         cand.push_back(mx);
         for (int lb = 16; lb <= scatter_lb_max(p, Bp); lb *= 2) cand.push_back(-lb);
+        if (sir_pull_ok(p, Bp)) cand.push_back(kSirPull);
     }
     best = 3.4e38f;
     int best_bc = cand.back();
@@ -2854,6 +2980,9 @@ int set_launch(Plan *p, int B, int gather, int scatter)
         if (scatter != 0 && !pull_choice_ok(p, scatter, Bp))
             return set_error(MBREC_ERR_ARG, "mbrec:args", "forward choice %d is not valid for a batch of %d", scatter,
                              B);
+    } else if (scatter == kSirPull) {
+        if (!sir_pull_ok(p, Bp))
+            return set_error(MBREC_ERR_ARG, "mbrec:args", "the SIR pull forward is not available for a batch of %d", B);
     } else if (scatter > 0 && (scatter > scatter_chunk_max(p, Bp) || (Bp > 1 && scatter % 4)))
         return set_error(MBREC_ERR_ARG, "mbrec:args", "scatter chunk %d is not valid for a batch of %d", scatter, B);
     else if (scatter < 0 && (-scatter > scatter_lb_max(p, Bp) || (-scatter != 16 && -scatter != 32 && -scatter != 64)))
