@@ -185,8 +185,11 @@ static int build_pull_records(Plan *p, const std::vector<int32_t> &slot_pix, con
 // SIR forward records in pull form (see the Plan fields): a pair's taps reach samples
 // [nf, nf + len) with nf = nn + s0 - Ls/2 (bin-space sample index nn = n + K); pairs
 // none of whose taps land in [0, nT + 2K) are left out.  u exactly as decode_tau.
+// Records are grouped by (detector, phase m) and sorted by nf inside a group; the
+// weights of window k sit between zero margins in their own table (d_sirw), and a
+// record's base points at its window's first weight there.
 static int build_sir_pull(Plan *p, const std::vector<int32_t> &slot_pix, const std::vector<int32_t> &tq,
-                          const int32_t *sidx, const int32_t *win, int Ls)
+                          const int32_t *sidx, const int32_t *win, int Ls, const float *sirf)
 {
     const BinGeom &g = p->geom;
     const int span = g.nT + 2 * g.K;
@@ -195,9 +198,16 @@ static int build_sir_pull(Plan *p, const std::vector<int32_t> &slot_pix, const s
     for (int k = 0; k < p->Nsir_build; ++k) maxlen = std::max(maxlen, (int)win[2 * (size_t)k + 1]);
     // nf ranges over [-(maxlen - 1), span - 1] for pairs with a tap in range
     const int lo = -std::max(maxlen - 1, 0), n_nf = span - lo + 1;
-    if ((size_t)p->Nsir_build * Ls >= ((size_t)1 << 26) || maxlen >= 64 || p->P >= (1 << 28))
+    // weights table: window k at k * stride + kSirWinPad, kSirWinPad zeros before it and at least as many after
+    const int stride = maxlen + 2 * kSirWinPad;
+    if ((size_t)p->Nsir_build * stride >= ((size_t)1 << 26) || maxlen >= 64 || p->P >= (1 << 28))
         return MBREC_OK;   // the packed record does not fit: the SIR forward keeps the bucketed form only
-    std::vector<int32_t> off((size_t)p->D * (n_nf + 1), 0);
+    std::vector<float> sw((size_t)p->Nsir_build * stride, 0.0f);
+    for (int k = 0; k < p->Nsir_build; ++k)
+        for (int i = 0; i < win[2 * (size_t)k + 1]; ++i)
+            sw[(size_t)k * stride + kSirWinPad + i] = sirf[(size_t)k * Ls + win[2 * (size_t)k] + i];
+    // CSR over (detector, phase m, nf): list (d, m) starts at off[(3 d + m) (n_nf + 1)]
+    std::vector<int32_t> off((size_t)p->D * 3 * (n_nf + 1), 0);
     std::vector<int32_t> nfb((size_t)p->D * p->Pd, INT32_MIN);
     for (int d = 0; d < p->D; ++d)
         for (int sl = 0; sl < p->Pd; ++sl) {
@@ -211,11 +221,11 @@ static int build_sir_pull(Plan *p, const std::vector<int32_t> &slot_pix, const s
             const long long nf = (long long)nn + s0 - half;
             if (nf + len - 1 < 0 || nf >= span) continue;   // no tap in range
             nfb[at] = (int32_t)nf;
-            off[(size_t)d * (n_nf + 1) + (nf - lo) + 1]++;
+            off[((size_t)d * 3 + m) * (n_nf + 1) + (nf - lo) + 1]++;
         }
     size_t total = 0;
-    for (int d = 0; d < p->D; ++d) {
-        int32_t *o = &off[(size_t)d * (n_nf + 1)];
+    for (int dm = 0; dm < 3 * p->D; ++dm) {
+        int32_t *o = &off[(size_t)dm * (n_nf + 1)];
         if (total + (size_t)p->Pd > (size_t)INT32_MAX) return MBREC_OK;
         o[0] = (int32_t)total;
         for (int i = 0; i < n_nf; ++i) o[i + 1] += o[i];
@@ -236,19 +246,21 @@ static int build_sir_pull(Plan *p, const std::vector<int32_t> &slot_pix, const s
             const float u = m == 0 ? fr * g.iw0 : (m == 1 ? (fr - g.phi1) * g.iw1 : (fr - g.phi2) * g.iw2);
             int32_t ub;
             std::memcpy(&ub, &u, sizeof(ub));
-            const int base = k * Ls + s0;
-            rec[fill[(size_t)d * (n_nf + 1) + (nfb[at] - lo)]++] =
+            const int base = k * stride + kSirWinPad;
+            rec[fill[((size_t)d * 3 + m) * (n_nf + 1) + (nfb[at] - lo)]++] =
                 make_int4(slot_pix[sl] | (m << 28), nfb[at], ub, base | (len << 26));
         }
-    if (p->d_srec) { (void)hipFree(p->d_srec); (void)hipFree(p->d_srec_off); }
+    if (p->d_srec) { (void)hipFree(p->d_srec); (void)hipFree(p->d_srec_off); (void)hipFree(p->d_sirw); }
     p->table_bytes -= p->srec_bytes;
     p->srec_bytes = 0;
     p->d_srec = nullptr;
     p->d_srec_off = nullptr;
+    p->d_sirw = nullptr;
     int rc;
     if ((rc = dalloc(&p->d_srec, rec.size(), &p->srec_bytes)) ||
-        (rc = dalloc(&p->d_srec_off, off.size(), &p->srec_bytes)) || (rc = upload(p->d_srec, rec.data(), rec.size())) ||
-        (rc = upload(p->d_srec_off, off.data(), off.size())))
+        (rc = dalloc(&p->d_srec_off, off.size(), &p->srec_bytes)) ||
+        (rc = dalloc(&p->d_sirw, sw.size(), &p->srec_bytes)) || (rc = upload(p->d_srec, rec.data(), rec.size())) ||
+        (rc = upload(p->d_srec_off, off.data(), off.size())) || (rc = upload(p->d_sirw, sw.data(), sw.size())))
         return rc;
     p->table_bytes += p->srec_bytes;
     p->sp_lo = lo;
@@ -496,7 +508,7 @@ static int build_tile_windows(Plan *p, const std::vector<int32_t> &tq)
 static void free_plan(Plan *p)
 {
     void *ptrs[] = {p->d_pix, p->d_tq, p->d_sidx, p->d_sir, p->d_swin, p->d_ir, p->d_irf, p->d_ir64, p->d_irf64,
-                    p->d_kz,  p->d_kx, p->d_rec, p->d_rec_off, p->d_rec_k, p->d_twin, p->d_ptab, p->d_ttab, p->d_srec, p->d_srec_off};
+                    p->d_kz,  p->d_kx, p->d_rec, p->d_rec_off, p->d_rec_k, p->d_twin, p->d_ptab, p->d_ttab, p->d_srec, p->d_srec_off, p->d_sirw};
     for (void *q : ptrs)
         if (q) (void)hipFree(q);
     free_work(p);
@@ -633,7 +645,7 @@ int mbrec_plan_set_sir(mbrec_plan h, const int32_t *sirIdx, const double *sir, i
         MBREC_HIP(hipMemcpy(tq.data(), p->d_tq, sizeof(int32_t) * tq.size(), hipMemcpyDeviceToHost));
         if ((rc = build_records(p, slot_pix, tq, sidx.data(), win.data(), Ls))) return rc;
         p->Nsir_build = Nsir;
-        if ((rc = build_sir_pull(p, slot_pix, tq, sidx.data(), win.data(), Ls))) return rc;
+        if ((rc = build_sir_pull(p, slot_pix, tq, sidx.data(), win.data(), Ls, sirf.data()))) return rc;
     }
     p->use_sir = 1;
     p->Ls = Ls;

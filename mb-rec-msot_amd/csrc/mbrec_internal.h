@@ -37,6 +37,7 @@ struct HatTaps {
 // Decoding of a fixed-point travel time into (bin index, interpolation weight).
 constexpr int kTileRowsMax = 256;   // tile gathers: longest bin window they stage (rows)
 constexpr int kRecPad = 512;   // zero records after the pull table (runs of up to 512 read past the end)
+constexpr int kSirWinPad = 16;   // zero weights on each side of a window in the pull form's SIR table
 
 struct BinGeom {
     float phi1, phi2;            // sorted interior breakpoint phases in [0,1)
@@ -81,9 +82,10 @@ struct Plan {
     int32_t *d_rec_k = nullptr;  // [records] 0-based SIR index (SIR plans)
     // SIR forward in pull form: one record per (pixel, detector) pair with a tap in the
     // bin range, int4 (pixel | phase << 28, first affected sample nf = n + s0 - Ls/2 + K,
-    // u bits, SIR table offset of the first tap | tap count << 26), sorted by (detector, nf);
-    // CSR over nf in [sp_lo, sp_lo + sp_n) per detector
+    // u bits, d_sirw offset of the first tap | tap count << 26), sorted by (detector, phase, nf);
+    // CSR over nf in [sp_lo, sp_lo + sp_n) per (detector, phase): d_srec_off[(3 d + m) (sp_n + 1) + i]
     int4 *d_srec = nullptr;
+    float *d_sirw = nullptr;     // [Nsir][maxlen + 2 kSirWinPad]: each SIR window between zero margins
     int32_t *d_srec_off = nullptr;
     int sp_lo = 0, sp_n = 0, Nsir_build = 0;
     size_t srec_bytes = 0;

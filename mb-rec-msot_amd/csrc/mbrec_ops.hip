@@ -1525,33 +1525,35 @@ __global__ __launch_bounds__(256) void k_fwd_sir_pull1(const int4 *__restrict__ 
     if (a > span) return;                       // samples 0 .. span (span: only bin 3 span = NB - 1); no barriers
     const int n = a - 1 + i;
     const bool live = n >= 0 && n < span;       // taps land on samples in [0, span) only
-    const int32_t *o = off + (size_t)d * (n_nf + 1);
     // records with nf in [a - maxlen, a + 14] can reach samples a - 1 .. a + 14
     const int ia = min(max(a - maxlen - lo, 0), n_nf), ib = min(max(a + 15 - lo, 0), n_nf);
-    const int rs = o[ia], re = o[ib];
     float acc0 = 0.0f, acc1 = 0.0f, acc2 = 0.0f, carry = 0.0f;
     constexpr int U = 4;   // records per group whose loads are in flight together
-    for (int r0 = rs + g; r0 < re; r0 += 4 * U) {
-        int4 e[U];
-        float c[U], h[U];
+    for (int mp = 0; mp < 3; ++mp) {   // the record lists of the three phases
+        const int32_t *o = off + ((size_t)d * 3 + mp) * (n_nf + 1);
+        const int rs = o[ia], re = o[ib];
+        for (int r0 = rs + g; r0 < re; r0 += 4 * U) {
+            int4 e[U];
+            float c[U], h[U];
 #pragma unroll
-        for (int q = 0; q < U; ++q) e[q] = rec[min(r0 + 4 * q, re - 1)];   // clamped: duplicates are masked below
+            for (int q = 0; q < U; ++q) e[q] = rec[min(r0 + 4 * q, re - 1)];   // clamped: duplicates are masked below
 #pragma unroll
-        for (int q = 0; q < U; ++q) {
-            c[q] = img[e[q].x & 0x0fffffff];
-            const int t = n - e[q].y, len = (int)((unsigned)e[q].w >> 26);
-            const bool ok = live && r0 + 4 * q < re && t >= 0 && t < len;
-            h[q] = ok ? sir[(e[q].w & 0x03ffffff) + t] : 0.0f;
-        }
+            for (int q = 0; q < U; ++q) {
+                c[q] = img[e[q].x & 0x0fffffff];
+                const int t = n - e[q].y, len = (int)((unsigned)e[q].w >> 26);
+                const bool ok = live && r0 + 4 * q < re && t >= 0 && t < len;
+                h[q] = ok ? sir[(e[q].w & 0x03ffffff) + t] : 0.0f;
+            }
 #pragma unroll
-        for (int q = 0; q < U; ++q) {
-            const int m = (int)((unsigned)e[q].x >> 28);
-            const float u = __int_as_float(e[q].z);
-            const float w0 = c[q] * (h[q] * (1.0f - u)), w1 = c[q] * (h[q] * u);
-            acc0 += m == 0 ? w0 : 0.0f;
-            acc1 += m == 0 ? w1 : (m == 1 ? w0 : 0.0f);
-            acc2 += m == 1 ? w1 : (m == 2 ? w0 : 0.0f);
-            carry += m == 2 ? w1 : 0.0f;
+            for (int q = 0; q < U; ++q) {
+                const int m = (int)((unsigned)e[q].x >> 28);
+                const float u = __int_as_float(e[q].z);
+                const float w0 = c[q] * (h[q] * (1.0f - u)), w1 = c[q] * (h[q] * u);
+                acc0 += m == 0 ? w0 : 0.0f;
+                acc1 += m == 0 ? w1 : (m == 1 ? w0 : 0.0f);
+                acc2 += m == 1 ? w1 : (m == 2 ? w0 : 0.0f);
+                carry += m == 2 ? w1 : 0.0f;
+            }
         }
     }
     // the four groups' sums (lanes i, i + 16, i + 32, i + 48), then the carry of sample n - 1
@@ -1570,6 +1572,106 @@ __global__ __launch_bounds__(256) void k_fwd_sir_pull1(const int4 *__restrict__ 
     if (n < span) {
         row[3 * n + 1] = acc1;
         row[3 * n + 2] = acc2;
+    }
+}
+
+// SIR forward for batches (lane = sinogram), pull form: a wave owns TS samples
+// a .. a + TS - 1 of one detector for 64 sinograms and keeps their 3 TS + 1 bins
+// (the last one: phase 0 of sample a + TS, never stored) in registers.  Records whose
+// taps reach samples a - 1 .. a + TS - 1 are read as wave-uniform scalars, one phase
+// at a time, with their TS + 1 weights at those samples: the weights table holds every
+// window between kSirWinPad zeros on each side, so a record needs no per-tap test.
+// Taps on samples outside [0, span) are dropped where they would reach a stored bin:
+// the phase-2 u-part from sample -1 (bin 0) and the phase-0 part at sample span (bin
+// 3 span, which only takes the u-part of sample span - 1).
+template <int TS, int U>
+__device__ __forceinline__ void sir_pull_phase(float (&acc)[3 * TS + 1], const int4 *__restrict__ rec, int rs, int re,
+                                               const float *__restrict__ imgI, const float *__restrict__ sirw,
+                                               int a, int maxlen, int Bp, int bo, int m, float f0)
+{
+    for (int r0 = rs; r0 < re; r0 += U) {
+        int4 e[U];
+        float c[U];
+#pragma unroll
+        for (int q = 0; q < U; ++q) e[q] = rec[r0 + q];   // the table is padded past its end
+#pragma unroll
+        for (int q = 0; q < U; ++q) c[q] = imgI[(size_t)(e[q].x & 0x0fffffff) * Bp + bo];
+#pragma unroll
+        for (int q = 0; q < U; ++q) {
+            if (r0 + q >= re) break;
+            // weight j is tap a - 1 + j - nf of the window: offset a - 1 - nf in [-TS, maxlen - 1]
+            const int sh = min(max(a - 1 - e[q].y, -TS), maxlen - 1);
+            const float *w = sirw + (e[q].w & 0x03ffffff) + sh;
+            float h[TS + 1];
+#pragma unroll
+            for (int j = 0; j <= TS; ++j) h[j] = w[j];
+            const float u = __int_as_float(e[q].z);
+            const float c0 = c[q] * (1.0f - u), c1 = c[q] * u;
+            if (m == 0) {
+#pragma unroll
+                for (int t = 0; t < TS; ++t) {
+                    acc[3 * t] += c0 * h[t + 1];
+                    acc[3 * t + 1] += c1 * h[t + 1];
+                }
+            } else if (m == 1) {
+#pragma unroll
+                for (int t = 0; t < TS; ++t) {
+                    acc[3 * t + 1] += c0 * h[t + 1];
+                    acc[3 * t + 2] += c1 * h[t + 1];
+                }
+            } else {
+                acc[0] += (c1 * f0) * h[0];
+#pragma unroll
+                for (int t = 0; t < TS; ++t) {
+                    acc[3 * t + 2] += c0 * h[t + 1];
+                    acc[3 * t + 3] += c1 * h[t + 1];
+                }
+            }
+        }
+    }
+}
+
+template <int TS>
+__global__ __launch_bounds__(256) void k_fwd_sir_pull_bl(const int4 *__restrict__ rec,
+                                                         const int32_t *__restrict__ off,
+                                                         const float *__restrict__ imgI, const float *__restrict__ sirw,
+                                                         float *__restrict__ bins, int NB, int span, int lo, int n_nf,
+                                                         int maxlen, int Bp)
+{
+    static_assert(TS <= kSirWinPad, "window margins must cover the samples of a wave");
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int d = blockIdx.y;
+    const int a = __builtin_amdgcn_readfirstlane((blockIdx.x * 4 + wv) * TS);
+    if (a > span) return;   // samples 0 .. span; no barriers in this kernel
+    const int b = blockIdx.z * 64 + lane;
+    const int bo = min(b, Bp - 1);
+    // records with nf in [a - maxlen, a + TS - 1] can reach samples a - 1 .. a + TS - 1
+    const int ia = min(max(a - maxlen - lo, 0), n_nf), ib = min(max(a + TS - lo, 0), n_nf);
+    float acc[3 * TS + 1];
+#pragma unroll
+    for (int k = 0; k < 3 * TS + 1; ++k) acc[k] = 0.0f;
+    constexpr int U = 2;
+    const int32_t *o = off + (size_t)d * 3 * (n_nf + 1);
+    sir_pull_phase<TS, U>(acc, rec, o[ia], o[ib], imgI, sirw, a, maxlen, Bp, bo, 0, 1.0f);
+    // bin 3 span keeps only the u-part of sample span - 1 (added below)
+#pragma unroll
+    for (int t = 0; t < TS; ++t)
+        if (a + t == span) acc[3 * t] = 0.0f;
+    o += n_nf + 1;
+    sir_pull_phase<TS, U>(acc, rec, o[ia], o[ib], imgI, sirw, a, maxlen, Bp, bo, 1, 1.0f);
+    o += n_nf + 1;
+    sir_pull_phase<TS, U>(acc, rec, o[ia], o[ib], imgI, sirw, a, maxlen, Bp, bo, 2, a > 0 ? 1.0f : 0.0f);
+    if (b >= Bp) return;
+    float *dst = bins + (size_t)d * NB * Bp + b;
+#pragma unroll
+    for (int t = 0; t < TS; ++t) {
+        const int n = a + t;
+        if (n > span) break;
+        if (3 * n < NB) dst[(size_t)(3 * n) * Bp] = acc[3 * t];
+        if (n < span) {
+            dst[(size_t)(3 * n + 1) * Bp] = acc[3 * t + 1];
+            dst[(size_t)(3 * n + 2) * Bp] = acc[3 * t + 2];
+        }
     }
 }
 
@@ -2185,7 +2287,7 @@ static void scatter_bl(Plan *p, const float *imgI, int B, int Bp, hipStream_t s)
 // of its own, the bins written directly (no bucket windows)
 constexpr int kSirPull = 1000;
 
-static bool sir_pull_ok(const Plan *p, int Bp) { return p->use_sir && p->d_srec && Bp == 1; }
+static bool sir_pull_ok(const Plan *p, int) { return p->use_sir && p->d_srec && p->d_sirw; }
 
 static int scatter_choice(const Plan *p, int Bp)
 {
@@ -2337,10 +2439,18 @@ static int launch_scatter(Plan *p, const float *imgI, int B, int Bp, hipStream_t
 {
     if (p->pull) return launch_pull(p, imgI, Bp, s);
     if (scatter_choice(p, Bp) == kSirPull) {
-        const int nblk = (p->geom.nT + 2 * p->geom.K + 1 + 59) / 60;   // 4 waves x 15 samples per workgroup
-        hipLaunchKernelGGL(k_fwd_sir_pull1, dim3(nblk, p->D), dim3(256), 0, s, p->d_srec, p->d_srec_off, imgI,
-                           p->d_sir, p->d_bins, p->geom.NB, p->geom.nT + 2 * p->geom.K, p->sp_lo, p->sp_n,
-                           p->sir_maxlen);
+        const int span = p->geom.nT + 2 * p->geom.K;
+        if (Bp == 1) {
+            const int nblk = (span + 1 + 59) / 60;   // 4 waves x 15 samples per workgroup
+            hipLaunchKernelGGL(k_fwd_sir_pull1, dim3(nblk, p->D), dim3(256), 0, s, p->d_srec, p->d_srec_off, imgI,
+                               p->d_sirw, p->d_bins, p->geom.NB, span, p->sp_lo, p->sp_n, p->sir_maxlen);
+        } else {
+            constexpr int TS = 16;
+            const int nblk = (span + 1 + 4 * TS - 1) / (4 * TS);   // 4 waves x TS samples per workgroup
+            hipLaunchKernelGGL((k_fwd_sir_pull_bl<TS>), dim3(nblk, p->D, (Bp + 63) / 64), dim3(256), 0, s, p->d_srec,
+                               p->d_srec_off, imgI, p->d_sirw, p->d_bins, p->geom.NB, span, p->sp_lo, p->sp_n,
+                               p->sir_maxlen, Bp);
+        }
         MBREC_HIP(hipGetLastError());
         return MBREC_OK;
     }
