@@ -37,7 +37,7 @@ struct HatTaps {
 // Decoding of a fixed-point travel time into (bin index, interpolation weight).
 constexpr int kTileRowsMax = 256;   // tile gathers: longest bin window they stage (rows)
 constexpr int kRecPad = 512;   // zero records after the pull table (runs of up to 512 read past the end)
-constexpr int kSirWinPad = 16;   // zero weights on each side of a window in the pull form's SIR table
+constexpr int kSirWinPad = 32;   // zero weights on each side of a window in the pull form's SIR table
 
 struct BinGeom {
     float phi1, phi2;            // sorted interior breakpoint phases in [0,1)

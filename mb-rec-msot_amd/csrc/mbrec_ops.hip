@@ -1631,7 +1631,7 @@ __device__ __forceinline__ void sir_pull_phase(float (&acc)[3 * TS + 1], const i
     }
 }
 
-template <int TS>
+template <int TS, int U>
 __global__ __launch_bounds__(256) void k_fwd_sir_pull_bl(const int4 *__restrict__ rec,
                                                          const int32_t *__restrict__ off,
                                                          const float *__restrict__ imgI, const float *__restrict__ sirw,
@@ -1650,7 +1650,6 @@ __global__ __launch_bounds__(256) void k_fwd_sir_pull_bl(const int4 *__restrict_
     float acc[3 * TS + 1];
 #pragma unroll
     for (int k = 0; k < 3 * TS + 1; ++k) acc[k] = 0.0f;
-    constexpr int U = 2;
     const int32_t *o = off + (size_t)d * 3 * (n_nf + 1);
     sir_pull_phase<TS, U>(acc, rec, o[ia], o[ib], imgI, sirw, a, maxlen, Bp, bo, 0, 1.0f);
     // bin 3 span keeps only the u-part of sample span - 1 (added below)
@@ -2283,17 +2282,25 @@ static void scatter_bl(Plan *p, const float *imgI, int B, int Bp, hipStream_t s)
 
 // scatter launch choice: > 0 record-lane kernel with that LDS batch chunk, < 0
 // batch-lane kernel with lane groups of -value sinograms
-// SIR forward in pull form (records of build_sir_pull; single sinogram): a scatter choice
-// of its own, the bins written directly (no bucket windows)
-constexpr int kSirPull = 1000;
+// SIR forward in pull form (records of build_sir_pull): scatter choices of their own, the
+// bins written directly (no bucket windows).  kSirPull: the default (for one sinogram
+// the lane-group kernel, else the batch-lane kernel with 24 samples and 2 records in
+// flight per wave); kSirPull + 1 .. 3: batch-lane kernels with (16 samples, 4 records),
+// (16, 2), (32, 1)
+constexpr int kSirPull = 1000, kSirPullVariants = 4;
 
-static bool sir_pull_ok(const Plan *p, int) { return p->use_sir && p->d_srec && p->d_sirw; }
+static bool is_sir_pull(int c) { return c >= kSirPull && c < kSirPull + kSirPullVariants; }
+
+static bool sir_pull_ok(const Plan *p, int Bp, int c = kSirPull)
+{
+    return p->use_sir && p->d_srec && p->d_sirw && is_sir_pull(c) && (c == kSirPull || Bp > 1);
+}
 
 static int scatter_choice(const Plan *p, int Bp)
 {
-    if (p->tune.Bp == Bp && p->tune.scatter_bc == kSirPull && sir_pull_ok(p, Bp)) return kSirPull;
+    if (p->tune.Bp == Bp && sir_pull_ok(p, Bp, p->tune.scatter_bc)) return p->tune.scatter_bc;
     if (!(p->tune.Bp == Bp && p->tune.scatter_bc != 0) && sir_pull_ok(p, Bp)) return kSirPull;
-    if (p->tune.Bp == Bp && p->tune.scatter_bc != 0 && p->tune.scatter_bc != kSirPull) {
+    if (p->tune.Bp == Bp && p->tune.scatter_bc != 0 && !is_sir_pull(p->tune.scatter_bc)) {
         const int c = p->tune.scatter_bc;
         if (c > 0) return std::min(c, scatter_chunk_max(p, Bp));
         if (-c <= scatter_lb_max(p, Bp)) return c;
@@ -2433,24 +2440,31 @@ static int launch_pull(Plan *p, const float *imgI, int Bp, hipStream_t s)
 
 // whether the forward pairs stage writes the bins themselves (pull forms) rather than
 // overlapping bucket windows
-static bool forward_writes_bins(const Plan *p, int Bp) { return p->pull || scatter_choice(p, Bp) == kSirPull; }
+static bool forward_writes_bins(const Plan *p, int Bp) { return p->pull || is_sir_pull(scatter_choice(p, Bp)); }
+
+template <int TS, int U>
+static void sir_pull_bl(Plan *p, const float *imgI, int Bp, hipStream_t s)
+{
+    const int span = p->geom.nT + 2 * p->geom.K;
+    const int nblk = (span + 1 + 4 * TS - 1) / (4 * TS);   // 4 waves x TS samples per workgroup
+    hipLaunchKernelGGL((k_fwd_sir_pull_bl<TS, U>), dim3(nblk, p->D, (Bp + 63) / 64), dim3(256), 0, s, p->d_srec,
+                       p->d_srec_off, imgI, p->d_sirw, p->d_bins, p->geom.NB, span, p->sp_lo, p->sp_n, p->sir_maxlen,
+                       Bp);
+}
 
 static int launch_scatter(Plan *p, const float *imgI, int B, int Bp, hipStream_t s)
 {
     if (p->pull) return launch_pull(p, imgI, Bp, s);
-    if (scatter_choice(p, Bp) == kSirPull) {
-        const int span = p->geom.nT + 2 * p->geom.K;
+    if (const int sc = scatter_choice(p, Bp); is_sir_pull(sc)) {
         if (Bp == 1) {
+            const int span = p->geom.nT + 2 * p->geom.K;
             const int nblk = (span + 1 + 59) / 60;   // 4 waves x 15 samples per workgroup
             hipLaunchKernelGGL(k_fwd_sir_pull1, dim3(nblk, p->D), dim3(256), 0, s, p->d_srec, p->d_srec_off, imgI,
                                p->d_sirw, p->d_bins, p->geom.NB, span, p->sp_lo, p->sp_n, p->sir_maxlen);
-        } else {
-            constexpr int TS = 16;
-            const int nblk = (span + 1 + 4 * TS - 1) / (4 * TS);   // 4 waves x TS samples per workgroup
-            hipLaunchKernelGGL((k_fwd_sir_pull_bl<TS>), dim3(nblk, p->D, (Bp + 63) / 64), dim3(256), 0, s, p->d_srec,
-                               p->d_srec_off, imgI, p->d_sirw, p->d_bins, p->geom.NB, span, p->sp_lo, p->sp_n,
-                               p->sir_maxlen, Bp);
-        }
+        } else if (sc == kSirPull + 1) sir_pull_bl<16, 4>(p, imgI, Bp, s);
+        else if (sc == kSirPull + 2) sir_pull_bl<16, 2>(p, imgI, Bp, s);
+        else if (sc == kSirPull + 3) sir_pull_bl<32, 1>(p, imgI, Bp, s);
+        else sir_pull_bl<24, 2>(p, imgI, Bp, s);
         MBREC_HIP(hipGetLastError());
         return MBREC_OK;
     }
@@ -3054,7 +3068,8 @@ int autotune(Plan *p, int B, int reps, hipStream_t s, int *gather_out, int *scat
         for (int bc = (Bp == 1 ? 1 : 4); bc < mx; bc *= 2) cand.push_back(bc);
         cand.push_back(mx);
         for (int lb = 16; lb <= scatter_lb_max(p, Bp); lb *= 2) cand.push_back(-lb);
-        if (sir_pull_ok(p, Bp)) cand.push_back(kSirPull);
+        for (int c = kSirPull; c < kSirPull + kSirPullVariants; ++c)
+            if (sir_pull_ok(p, Bp, c)) cand.push_back(c);
     }
     best = 3.4e38f;
     int best_bc = cand.back();
@@ -3090,9 +3105,10 @@ int set_launch(Plan *p, int B, int gather, int scatter)
         if (scatter != 0 && !pull_choice_ok(p, scatter, Bp))
             return set_error(MBREC_ERR_ARG, "mbrec:args", "forward choice %d is not valid for a batch of %d", scatter,
                              B);
-    } else if (scatter == kSirPull) {
-        if (!sir_pull_ok(p, Bp))
-            return set_error(MBREC_ERR_ARG, "mbrec:args", "the SIR pull forward is not available for a batch of %d", B);
+    } else if (is_sir_pull(scatter)) {
+        if (!sir_pull_ok(p, Bp, scatter))
+            return set_error(MBREC_ERR_ARG, "mbrec:args", "SIR pull forward %d is not available for a batch of %d",
+                             scatter, B);
     } else if (scatter > 0 && (scatter > scatter_chunk_max(p, Bp) || (Bp > 1 && scatter % 4)))
         return set_error(MBREC_ERR_ARG, "mbrec:args", "scatter chunk %d is not valid for a batch of %d", scatter, B);
     else if (scatter < 0 && (-scatter > scatter_lb_max(p, Bp) || (-scatter != 16 && -scatter != 32 && -scatter != 64)))
