@@ -268,6 +268,74 @@ static int build_sir_pull(Plan *p, const std::vector<int32_t> &slot_pix, const s
     return MBREC_OK;
 }
 
+// SIR transpose tables for the LDS tile form (d_stwin, d_stab): for pair (slot, d) with
+// decoded (nn, m, u) and window [s0, s0 + len) of its SIR index, the taps s whose sample
+// nn + s - Ls/2 lies in [0, nT + 2K) form [s_lo, s_hi); they read bin rows
+// 3 (nn + s - Ls/2) + m and the next.  The tile's window starts at the lowest such row
+// of its 256 pairs.  Weights come from d_sirw (layout of build_sir_pull).
+static int build_sir_tiles(Plan *p, const std::vector<int32_t> &slot_pix, const std::vector<int32_t> &tq,
+                           const int32_t *sidx, const int32_t *win, int Ls)
+{
+    const BinGeom &g = p->geom;
+    const int span = g.nT + 2 * g.K;
+    const int half = Ls / 2;
+    const int ntile = p->Pd / 256;
+    int maxlen = 0;
+    for (int k = 0; k < p->Nsir_build; ++k) maxlen = std::max(maxlen, (int)win[2 * (size_t)k + 1]);
+    const int stride = maxlen + 2 * kSirWinPad;
+    if (p->d_stab) { (void)hipFree(p->d_stab); (void)hipFree(p->d_stwin); }
+    p->table_bytes -= p->stab_bytes;
+    p->stab_bytes = 0;
+    p->d_stab = nullptr;
+    p->d_stwin = nullptr;
+    p->stile_zr = 0;
+    if (!p->d_sirw || maxlen > 32) return MBREC_OK;   // the kernel's weight registers hold 32 taps per pair
+    std::vector<int32_t> w((size_t)ntile * p->D, 0);
+    std::vector<int4> tab((size_t)ntile * p->D * 256, make_int4(0, 0, 0, 0));
+    const float scale = std::ldexp(1.0f, -g.F);
+    int wmax = 0;
+    for (int t = 0; t < ntile; ++t)
+        for (int d = 0; d < p->D; ++d) {
+            int lo = INT32_MAX, hi = INT32_MIN;
+            int4 *e = tab.data() + ((size_t)t * p->D + d) * 256;
+            for (int k = 0; k < 256; ++k) {
+                const size_t at = (size_t)d * p->Pd + (size_t)t * 256 + k;
+                if (slot_pix[(size_t)t * 256 + k] < 0) continue;
+                const int sk = sidx[at];
+                const int s0 = win[2 * (size_t)sk], len = win[2 * (size_t)sk + 1];
+                if (tq[at] == INT32_MIN || len <= 0) continue;
+                int nn, m;
+                host_decode(tq[at], g, nn, m);
+                const long long sl = std::max<long long>(s0, (long long)half - nn);
+                const long long sh = std::min<long long>((long long)s0 + len, (long long)span + half - nn);
+                if (sl >= sh) continue;
+                const int jl = (int)(3 * (nn + sl - half) + m), jh = (int)(3 * (nn + sh - 1 - half) + m + 1);
+                lo = std::min(lo, jl);
+                hi = std::max(hi, jh);
+                const float fr = (float)(tq[at] & ((1 << g.F) - 1)) * scale;
+                const float u = m == 0 ? fr * g.iw0 : (m == 1 ? (fr - g.phi1) * g.iw1 : (fr - g.phi2) * g.iw2);
+                int32_t ub;
+                std::memcpy(&ub, &u, sizeof(ub));
+                e[k] = make_int4(jl, ub, (int)((size_t)sk * stride + kSirWinPad + (sl - s0)), (int)(sh - sl));
+            }
+            if (lo > hi) continue;
+            w[(size_t)t * p->D + d] = lo;
+            wmax = std::max(wmax, hi - lo + 1);
+            for (int k = 0; k < 256; ++k)
+                if (e[k].w > 0) e[k].x -= lo;
+        }
+    const int zr = 64 * ((wmax + 63) / 64);
+    p->stile_wmax = wmax;
+    if (zr > 256 || (size_t)p->Nsir_build * stride >= ((size_t)1 << 31)) return MBREC_OK;
+    int rc;
+    if ((rc = dalloc(&p->d_stwin, w.size(), &p->stab_bytes)) || (rc = dalloc(&p->d_stab, tab.size(), &p->stab_bytes)) ||
+        (rc = upload(p->d_stwin, w.data(), w.size())) || (rc = upload(p->d_stab, tab.data(), tab.size())))
+        return rc;
+    p->table_bytes += p->stab_bytes;
+    p->stile_zr = zr;
+    return MBREC_OK;
+}
+
 int build_records(Plan *p, const std::vector<int32_t> &slot_pix, const std::vector<int32_t> &tq,
                   const int32_t *sidx, const int32_t *win, int Ls)
 {
@@ -508,7 +576,7 @@ static int build_tile_windows(Plan *p, const std::vector<int32_t> &tq)
 static void free_plan(Plan *p)
 {
     void *ptrs[] = {p->d_pix, p->d_tq, p->d_sidx, p->d_sir, p->d_swin, p->d_ir, p->d_irf, p->d_ir64, p->d_irf64,
-                    p->d_kz,  p->d_kx, p->d_rec, p->d_rec_off, p->d_rec_k, p->d_twin, p->d_ptab, p->d_ttab, p->d_srec, p->d_srec_off, p->d_sirw};
+                    p->d_kz,  p->d_kx, p->d_rec, p->d_rec_off, p->d_rec_k, p->d_twin, p->d_ptab, p->d_ttab, p->d_srec, p->d_srec_off, p->d_sirw, p->d_stwin, p->d_stab};
     for (void *q : ptrs)
         if (q) (void)hipFree(q);
     free_work(p);
@@ -646,6 +714,7 @@ int mbrec_plan_set_sir(mbrec_plan h, const int32_t *sirIdx, const double *sir, i
         if ((rc = build_records(p, slot_pix, tq, sidx.data(), win.data(), Ls))) return rc;
         p->Nsir_build = Nsir;
         if ((rc = build_sir_pull(p, slot_pix, tq, sidx.data(), win.data(), Ls, sirf.data()))) return rc;
+        if ((rc = build_sir_tiles(p, slot_pix, tq, sidx.data(), win.data(), Ls))) return rc;
     }
     p->use_sir = 1;
     p->Ls = Ls;

@@ -86,6 +86,14 @@ struct Plan {
     // CSR over nf in [sp_lo, sp_lo + sp_n) per (detector, phase): d_srec_off[(3 d + m) (sp_n + 1) + i]
     int4 *d_srec = nullptr;
     float *d_sirw = nullptr;     // [Nsir][maxlen + 2 kSirWinPad]: each SIR window between zero margins
+    // SIR transpose in LDS tile form: per (256-slot tile, detector) the first bin row of the
+    // staged window and per pair int4 (row of its first in-range tap inside the window,
+    // u bits, d_sirw offset of that tap's weight, in-range tap count)
+    int32_t *d_stwin = nullptr;  // [Pd / 256][D]
+    int4 *d_stab = nullptr;      // [Pd / 256][D][256]
+    int stile_zr = 0;            // staged rows: 64 * ceil(longest window / 64)
+    int stile_wmax = 0;          // longest window
+    size_t stab_bytes = 0;
     int32_t *d_srec_off = nullptr;
     int sp_lo = 0, sp_n = 0, Nsir_build = 0;
     size_t srec_bytes = 0;

@@ -2225,6 +2225,134 @@ __global__ __launch_bounds__(1024) void k_gather_ta(
     }
 }
 
+// SIR transpose for batches, LDS tile form: WG = 16 waves, one 256-slot tile x 64
+// sinograms x one of G detector groups.  Per detector the tile's window of bin rows
+// (every in-range tap of its pairs) and its 256 pair entries are copied
+// memory -> LDS by buffer loads with the LDS destination, one detector ahead into the
+// other of two buffers.  Wave w takes pairs 16 w .. 16 w + 15: their weights (32 taps per
+// pair, zero past the pair's in-range count) are loaded into 8 registers, two pairs per
+// register, and each tap's weight is taken from its lane with readlane.  Per pair
+// A = sum w g0 and C = sum w g1 over its taps (g0, g1: the tap's two bin rows), and the
+// pair adds A + u (C - A) = sum w ((1 - u) g0 + u g1).
+template <int G, int NCH>
+__global__ __launch_bounds__(1024) void k_gather_tas(const int4 *__restrict__ stab, const int32_t *__restrict__ stwin,
+                                                     const int32_t *__restrict__ pix, const float *__restrict__ bins,
+                                                     const float *__restrict__ sirw, float *__restrict__ out, int D,
+                                                     int NB, int P, int B, int Bp, float scale, int bins_bytes,
+                                                     int stab_bytes)
+{
+    constexpr int ZR = 64 * NCH;
+    constexpr int BUF = ZR * 64 + 4 * 256;   // floats per buffer: rows, then 256 int4 entries
+    __shared__ __attribute__((aligned(16))) float s0[BUF], s1[BUF];
+    __shared__ int32_t tw[kTaMaxD];   // the tile's window origins
+    const int tile = blockIdx.x, grp = blockIdx.y % G, b0 = (blockIdx.y / G) * 64;
+    const int lane = threadIdx.x & 63, wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int DG = (D + G - 1) / G, dbeg = grp * DG, dend = min(D, dbeg + DG);
+    for (int i = threadIdx.x; i < D; i += 1024) tw[i] = stwin[(size_t)tile * D + i];
+    __syncthreads();
+    const __amdgpu_buffer_rsrc_t rs =
+        __builtin_amdgcn_make_buffer_rsrc(const_cast<float *>(bins), (short)0, bins_bytes, kBufferWord3);
+    const __amdgpu_buffer_rsrc_t rt =
+        __builtin_amdgcn_make_buffer_rsrc(const_cast<int4 *>(stab), (short)0, stab_bytes, kBufferWord3);
+    const int rowb = 4 * Bp;
+    // load c of wave w moves rows 4 (16 c + w) .. + 3: lane l takes row + l / 16, columns b0 + 4 (l % 16) ..
+    const int vo = (lane >> 4) * rowb + 4 * (b0 + 4 * (lane & 15));
+    auto stage = [&](int d, float *bb) __attribute__((always_inline)) {
+        const int so = (d * NB + __builtin_amdgcn_readfirstlane(tw[d])) * rowb;
+#pragma unroll
+        for (int c = 0; c < NCH; ++c) {
+            const int r0 = 4 * (16 * c + wv);
+            __builtin_amdgcn_raw_ptr_buffer_load_lds(rs, (__attribute__((address_space(3))) void *)(bb + r0 * 64),
+                                                     16, vo, so + r0 * rowb, 0, 0);
+        }
+        // entries 16 w .. 16 w + 15 of (tile, d), 4 bytes per lane: every wave issues the same
+        // number of loads, so one wait count fits all
+        __builtin_amdgcn_raw_ptr_buffer_load_lds(
+            rt, (__attribute__((address_space(3))) void *)(bb + ZR * 64 + 64 * wv), 4, 4 * lane,
+            __builtin_amdgcn_readfirstlane((int)((((size_t)tile * D + d) * 256 + 16 * wv) * sizeof(int4))), 0, 0);
+    };
+    float acc[16], part[16];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) acc[k] = part[k] = 0.0f;
+    auto step = [&](int d, const float *rd, float *wr) __attribute__((always_inline)) {
+        // this wave's loads for d are done, then everyone's (and the buffer of d - 1 is free)
+        __builtin_amdgcn_s_waitcnt(kWaitVm(0));
+        __builtin_amdgcn_s_barrier();
+        const int4 *te = reinterpret_cast<const int4 *>(rd + ZR * 64) + 16 * wv;
+        // the weights are loaded with explicit instructions and waited for by hand: the
+        // compiler's own wait for them would also wait for the staging loads issued after
+        // them (its count cannot tell the two kinds of load apart); sirw[0] is a zero margin
+        float W[8];
+#pragma unroll
+        for (int r = 0; r < 8; ++r) {
+            const int4 e = te[2 * r + (lane >> 5)];
+            const int t = lane & 31;
+            const float *wp = sirw + (t < e.w ? e.z + t : 0);
+            asm volatile("global_load_dword %0, %1, off" : "=v"(W[r]) : "v"(wp) : "memory");
+        }
+        if (d + 1 < dend) {
+            stage(d + 1, wr);
+            __builtin_amdgcn_s_waitcnt(kWaitVm(NCH + 1));   // the weights (older than the staging loads)
+        } else {
+            __builtin_amdgcn_s_waitcnt(kWaitVm(0));
+        }
+#pragma unroll
+        for (int r = 0; r < 8; ++r) asm volatile("" : "+v"(W[r]));   // uses stay after the wait
+        const int4 ev = te[lane & 15];   // lane k < 16: entry k (taken with readlane below)
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+            const int row0 = __builtin_amdgcn_readlane(ev.x, k), cnt = __builtin_amdgcn_readlane(ev.w, k);
+            const float u = __int_as_float(__builtin_amdgcn_readlane(ev.y, k));
+            const float *g = rd + row0 * 64 + lane;
+            const int wk = __float_as_int(W[k >> 1]);
+            float A = 0.0f, C = 0.0f;
+            int c = 0;
+            for (; c + 8 <= cnt; c += 8) {
+#pragma unroll
+                for (int i = 0; i < 8; ++i) {
+                    const float w = __int_as_float(__builtin_amdgcn_readlane(wk, (k & 1) * 32 + c + i));
+                    A = fmaf(w, g[(3 * (c + i)) * 64], A);
+                    C = fmaf(w, g[(3 * (c + i) + 1) * 64], C);
+                }
+            }
+#pragma unroll
+            for (int i = 0; i < 7; ++i) {   // the last cnt % 8 taps (no reads past the pair's rows)
+                if (c + i >= cnt) break;
+                const float w = __int_as_float(__builtin_amdgcn_readlane(wk, (k & 1) * 32 + c + i));
+                A = fmaf(w, g[(3 * (c + i)) * 64], A);
+                C = fmaf(w, g[(3 * (c + i) + 1) * 64], C);
+            }
+            part[k] += fmaf(u, C - A, A);
+        }
+        if (((d - dbeg) & 7) == 7) {
+#pragma unroll
+            for (int k = 0; k < 16; ++k) {
+                acc[k] += part[k];
+                part[k] = 0.0f;
+            }
+        }
+    };
+    if (dbeg < dend) {
+        stage(dbeg, s0);
+        for (int d = dbeg; d < dend; d += 2) {
+            step(d, s0, s1);
+            if (d + 1 < dend) step(d + 1, s1, s0);
+        }
+    }
+    const int b = b0 + lane;
+    if (b >= B) return;
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+        const float v = acc[k] + part[k];
+        const int px = pix[(size_t)tile * 256 + 16 * wv + k];
+        if (px < 0) continue;
+        if (G > 1)
+            out[((size_t)grp * B + b) * P + px] = v;
+        else
+            out[(size_t)b * P + px] = scale * v;
+    }
+}
+
 // out[b][x] = scale * sum of the NP partial images (fixed order)
 template <int NP>
 __global__ __launch_bounds__(256) void k_sum_parts(const float *__restrict__ part, float *__restrict__ out,
@@ -2724,6 +2852,33 @@ static void gather_ta(Plan *p, float *img, int B, int Bp, float scale, hipStream
     }
 }
 
+template <int G, int NCH>
+static void gather_tas_n(Plan *p, float *img, int B, int Bp, float scale, hipStream_t s)
+{
+    const int nbc = (Bp + 63) / 64;
+    const int bytes = (int)(sizeof(float) * ((size_t)p->D * p->geom.NB + 2) * Bp);
+    const int tbytes = (int)(sizeof(int4) * (size_t)(p->Pd / 256) * p->D * 256);
+    hipLaunchKernelGGL((k_gather_tas<G, NCH>), dim3(p->Pd / 256, G * nbc), dim3(1024), 0, s, p->d_stab, p->d_stwin,
+                       p->d_pix, p->d_bins, p->d_sirw, G > 1 ? p->d_gpart : img, p->D, p->geom.NB, p->P, B, Bp,
+                       scale, bytes, tbytes);
+    if (G > 1) {
+        const size_t n = (size_t)B * p->P;
+        hipLaunchKernelGGL(k_sum_parts<G>, dim3(std::min<size_t>(4096, (n + 255) / 256)), dim3(256), 0, s,
+                           (const float *)p->d_gpart, img, n, scale);
+    }
+}
+
+template <int G>
+static void gather_tas(Plan *p, float *img, int B, int Bp, float scale, hipStream_t s)
+{
+    switch (p->stile_zr / 64) {   // 64-row rounds of the staged window (validity keeps it in 1..4)
+    case 1: gather_tas_n<G, 1>(p, img, B, Bp, scale, s); break;
+    case 2: gather_tas_n<G, 2>(p, img, B, Bp, scale, s); break;
+    case 3: gather_tas_n<G, 3>(p, img, B, Bp, scale, s); break;
+    default: gather_tas_n<G, 4>(p, img, B, Bp, scale, s); break;
+    }
+}
+
 template <int G>
 static void gather_tt(Plan *p, float *img, int B, int Bp, float scale, hipStream_t s)
 {
@@ -2765,12 +2920,20 @@ static const GatherVariant kGatherVariants[] = {{1, 8, 0},   {1, 4, 0},   {4, 4,
                                                 {0, -5, 16}, {0, -6, 8},  {0, -6, 16}, {0, -7, 8},
                                                 {0, -7, 16}, {0, -8, 8},  {0, -8, 16}, {0, -9, 1},
                                                 {0, -9, 2},  {0, -9, 4},  {0, -9, 8},  {0, -10, 1},
-                                                {0, -10, 2}, {0, -10, 4}};
+                                                {0, -10, 2}, {0, -10, 4}, {0, -11, 1}, {0, -11, 2},
+                                                {0, -11, 4}};
 constexpr int kNumGatherVariants = sizeof(kGatherVariants) / sizeof(kGatherVariants[0]);
 
 static bool gather_variant_ok(const Plan *p, int v, int Bp)
 {
     const GatherVariant &g = kGatherVariants[v];
+    if (g.u == -11) {   // SIR tile gather: tables built, window within 4 rounds of 64 rows, 32-bit offsets
+        if (!p->use_sir || !p->d_stab || p->stile_zr < 64 || p->stile_zr > 256 || Bp < 16) return false;
+        if (sizeof(float) * ((size_t)p->D * p->geom.NB + 2) * Bp >= ((size_t)1 << 31)) return false;
+        if (sizeof(int4) * (size_t)(p->Pd / 256) * p->D * 256 >= ((size_t)1 << 31)) return false;
+        if (p->D > kTaMaxD) return false;
+        return g.lb == 1 || p->d_gpart;
+    }
     if (g.u == -10) {   // tile gather with asynchronous staging: three buffers of at most 3 rounds of 64 rows
         if (p->use_sir || !p->d_ttab || Bp < 16 || p->tile_zr > 192) return false;
         if (sizeof(float) * ((size_t)p->D * p->geom.NB + 2) * Bp >= ((size_t)1 << 31)) return false;
@@ -2815,6 +2978,9 @@ static bool gather_variant_ok(const Plan *p, int v, int Bp)
 static int default_gather_variant(const Plan *p, int Bp)
 {
     if (Bp == 1) return 0;
+    if (p->use_sir && Bp >= 16)
+        for (int v : {39, 38, 37})   // SIR tile gather in four detector groups, two, one
+            if (gather_variant_ok(p, v, Bp)) return v;
     if (Bp >= 64) {
         // LDS tile gathers in two detector groups (asynchronous, then register staging),
         // else the wave-uniform XCD gather
@@ -2869,7 +3035,10 @@ static int launch_gather(Plan *p, float *img, int B, int Bp, float scale, hipStr
     case 33: gather_tt<8>(p, img, B, Bp, scale, s); break;
     case 34: gather_ta<1>(p, img, B, Bp, scale, s); break;
     case 35: gather_ta<2>(p, img, B, Bp, scale, s); break;
-    default: gather_ta<4>(p, img, B, Bp, scale, s); break;
+    case 36: gather_ta<4>(p, img, B, Bp, scale, s); break;
+    case 37: gather_tas<1>(p, img, B, Bp, scale, s); break;
+    case 38: gather_tas<2>(p, img, B, Bp, scale, s); break;
+    default: gather_tas<4>(p, img, B, Bp, scale, s); break;
     }
     MBREC_HIP(hipGetLastError());
     return MBREC_OK;
