@@ -121,12 +121,18 @@ int mbrec_plan_tau_frac_bits(mbrec_plan plan);
 int mbrec_plan_autotune(mbrec_plan plan, int B, int reps, mbrec_stream s, int *choice);
 /* Set the launch choice for batches of B by hand: gather variant in
  * [0, mbrec_num_gather_variants()) or -1 (automatic); forward choice 0:
- * automatic.  Plans with SIR (scatter form): > 0 the record-lane kernel with that
- * LDS batch chunk (a multiple of 4), < 0 the batch-lane kernel with lane groups
- * of -value (16, 32 or 64) sinograms.  Plans without SIR (pull form): 1, 4, 8,
- * 16 bins per thread pass of that many sinograms; -(LB + 100 mode) batch-lane
- * with lane groups LB and mode 0 (plain order), 1 (XCD-aware order) or 2
- * (XCD-aware, 16 pairs in flight).  Invalid choices are rejected (mbrec:args). */
+ * automatic.  Plans with SIR: 1000 the pull form (one sinogram: lane groups of
+ * samples; batches: 24 samples x 2 records in flight per wave), 1001 / 1002 / 1003
+ * batch pull forms with (16, 4), (16, 2), (32, 1); bucketed scatter: > 0 the
+ * record-lane kernel with that LDS batch chunk (a multiple of 4), < 0 the
+ * batch-lane kernel with lane groups of -value (16, 32 or 64) sinograms.  Plans
+ * without SIR (pull form): 1, 4, 8, 16 bins per thread pass of that many
+ * sinograms; 2, 3 record streaming for one sinogram (1 or 4 bins per lane);
+ * -(LB + 100 mode) batch-lane with lane groups LB and mode 0 (plain order), 1
+ * (XCD-aware order), 2 (XCD-aware, 16 pairs in flight); -364 / -464 / -564 /
+ * -664 / -764 / -864 wave-uniform, -964 / -1064 / -1164 wave-uniform with raw
+ * buffer loads (bins per wave and rows in flight differ).  Invalid choices are
+ * rejected (mbrec:args). */
 int mbrec_plan_set_launch(mbrec_plan plan, int B, int gather_variant, int scatter_choice);
 int mbrec_num_gather_variants(void);
 

@@ -3198,9 +3198,16 @@ int autotune(Plan *p, int B, int reps, hipStream_t s, int *gather_out, int *scat
     hipEvent_t a, b;
     MBREC_HIP(hipEventCreate(&a));
     MBREC_HIP(hipEventCreate(&b));
-    auto timeit = [&](auto &&fn, float &ms) -> int {
-        int r = fn();
-        if (r) return r;
+    // one timed launch first (it also warms up); a candidate already 4x slower than
+    // the best so far is not timed further
+    auto timeit = [&](auto &&fn, float &ms, float best_ms) -> int {
+        int r;
+        MBREC_HIP(hipEventRecord(a, s));
+        if ((r = fn())) return r;
+        MBREC_HIP(hipEventRecord(b, s));
+        MBREC_HIP(hipEventSynchronize(b));
+        MBREC_HIP(hipEventElapsedTime(&ms, a, b));
+        if (ms > 4.0f * best_ms) return MBREC_OK;
         MBREC_HIP(hipEventRecord(a, s));
         for (int i = 0; i < reps; ++i)
             if ((r = fn())) return r;
@@ -3213,13 +3220,16 @@ int autotune(Plan *p, int B, int reps, hipStream_t s, int *gather_out, int *scat
     const TuneChoice saved = p->tune;
     p->tune.Bp = Bp;
     // gather
-    float best = 3.4e38f;
+    float best = 1e30f;
     int best_v = default_gather_variant(p, Bp);
-    for (int v = 0; v < kNumGatherVariants; ++v) {
-        if (!gather_variant_ok(p, v, Bp)) continue;
+    std::vector<int> order;   // the default first: slow candidates are then cut short
+    if (gather_variant_ok(p, best_v, Bp)) order.push_back(best_v);
+    for (int v = 0; v < kNumGatherVariants; ++v)
+        if (v != best_v && gather_variant_ok(p, v, Bp)) order.push_back(v);
+    for (int v : order) {
         p->tune.gather = v;
         float ms = 0.0f;
-        if ((rc = timeit([&]() { return launch_gather(p, p->d_img_tmp, B, Bp, 1.0f, s); }, ms))) {
+        if ((rc = timeit([&]() { return launch_gather(p, p->d_img_tmp, B, Bp, 1.0f, s); }, ms, best))) {
             p->tune = saved;
             return rc;
         }
@@ -3232,20 +3242,20 @@ int autotune(Plan *p, int B, int reps, hipStream_t s, int *gather_out, int *scat
         for (int c : {1, 2, 3, 4, 8, 16, -16, -32, -64, -116, -132, -164, -216, -232, -264, -364, -464, -564, -664,
                       -764, -864, -964, -1064, -1164})
             if (pull_choice_ok(p, c, Bp) && !(c == 1 && Bp > 16)) cand.push_back(c);
-    } else {
+    } else {   // SIR: the pull forms first
+        for (int c = kSirPull; c < kSirPull + kSirPullVariants; ++c)
+            if (sir_pull_ok(p, Bp, c)) cand.push_back(c);
         const int mx = scatter_chunk_max(p, Bp);
         for (int bc = (Bp == 1 ? 1 : 4); bc < mx; bc *= 2) cand.push_back(bc);
         cand.push_back(mx);
         for (int lb = 16; lb <= scatter_lb_max(p, Bp); lb *= 2) cand.push_back(-lb);
-        for (int c = kSirPull; c < kSirPull + kSirPullVariants; ++c)
-            if (sir_pull_ok(p, Bp, c)) cand.push_back(c);
     }
-    best = 3.4e38f;
-    int best_bc = cand.back();
+    best = 1e30f;
+    int best_bc = cand.front();
     for (int c : cand) {
         p->tune.scatter_bc = c;
         float ms = 0.0f;
-        if ((rc = timeit([&]() { return launch_scatter(p, p->d_imgI, B, Bp, s); }, ms))) {
+        if ((rc = timeit([&]() { return launch_scatter(p, p->d_imgI, B, Bp, s); }, ms, best))) {
             p->tune = saved;
             return rc;
         }
