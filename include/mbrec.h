@@ -134,6 +134,12 @@ int mbrec_plan_autotune(mbrec_plan plan, int B, int reps, mbrec_stream s, int *c
  * buffer loads (bins per wave and rows in flight differ).  Invalid choices are
  * rejected (mbrec:args). */
 int mbrec_plan_set_launch(mbrec_plan plan, int B, int gather_variant, int scatter_choice);
+/* Kernel of the IR convolution / correlation stage (MSOTmodel.m:259-269, FFTs in FP64
+ * there): mfma = 0 (default) the register-blocked vector kernel, mfma = 1 Toeplitz
+ * products on the FP64 matrix cores when one IR serves every detector.  Both sum the
+ * exact products of the FP32 samples and taps in FP64; the results differ by summation
+ * order only.  No counterpart in the reference: a speed choice. */
+int mbrec_plan_set_fir_kernel(mbrec_plan plan, int mfma);
 int mbrec_num_gather_variants(void);
 
 /* Profiling: average duration (ms, HIP events on stream s) of one operator

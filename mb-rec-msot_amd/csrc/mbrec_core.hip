@@ -833,6 +833,15 @@ int mbrec_plan_set_launch(mbrec_plan h, int B, int gather_variant, int scatter_c
     return set_launch(&h->p, B, gather_variant, scatter_choice);
 }
 
+int mbrec_plan_set_fir_kernel(mbrec_plan h, int mfma)
+{
+    if (!h || (mfma != 0 && mfma != 1))
+        return set_error(MBREC_ERR_ARG, "mbrec:args", "plan_set_fir_kernel: invalid arguments");
+    if (h->p.pins) return set_error(MBREC_ERR_ARG, "mbrec:planPinned", "plan is held by a batch graph");
+    h->p.fir_mfma = mfma;
+    return MBREC_OK;
+}
+
 int mbrec_num_gather_variants(void) { return num_gather_variants(); }
 
 int mbrec_plan_tau_frac_bits(mbrec_plan h) { return h ? h->p.geom.F : -1; }

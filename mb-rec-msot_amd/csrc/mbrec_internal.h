@@ -111,6 +111,7 @@ struct Plan {
     double *d_ir64 = nullptr;    // [rows][16 ceil(Lir / 16)] the same taps in FP64, zero-padded (FIR kernel)
     double *d_irf64 = nullptr;   // flipped
     int Lir = 0, ir_per_det = 0;
+    int fir_mfma = 0;            // IR filter: 1 Toeplitz products on the FP64 matrix cores (one IR for all detectors), 0 VALU
     // prefilter
     float *d_kz = nullptr, *d_kx = nullptr;
     int Kz = -1, Kx = -1;        // -1 = no prefilter

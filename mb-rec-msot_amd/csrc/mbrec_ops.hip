@@ -66,6 +66,8 @@ __host__ __device__ constexpr int kWaitVm(int n) { return (n & 15) | (((n >> 4) 
 // Staging of n elements into LDS with U global loads in flight per thread
 // before any is stored (a plain strided loop waits out one load latency per
 // element).  ld(e) returns element e, st(e, v) stores it; NT threads take part.
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
 template <int U, int NT, class Ld, class St>
 __device__ __forceinline__ void stage_batched(int t, int n, Ld &&ld, St &&st)
 {
@@ -620,6 +622,97 @@ __global__ __launch_bounds__(256) void k_fir16(const float *__restrict__ x, floa
     if (!live) return;
     float *orow = out + (size_t)row * nT;
     for (int i = lt; i < kFirSpan && t0 + i < nT; i += 128) orow[t0 + i] = xs[(i & 15) * NBLK + (i >> 4)];
+}
+
+// FIR on the matrix cores (shared IR, FP64 sums of the exact FP32 products): for a block
+// of 16 outputs t = tb .. tb + 15 of 16 rows, y[t] = sum_i taps[i] x[t + off - i] is the
+// product of the 16 x K Toeplitz matrix T[a][j] = taps[a + L - 1 - j] (zero outside [0, L))
+// with the K x 16 window X[j][r] = x_r[tb + off - (L - 1) + j], K = 4 kFmNS >= L + 15, done
+// as kFmNS v_mfma_f64_16x16x4_f64 steps.  T's fragments are the same for every block and
+// stay in registers (lane l: row l % 16, k-column l / 16 of each step); a workgroup stages
+// 16 rows x kFmW samples in LDS (zero outside the row's window and [0, nT)), each of its 4
+// waves computes kFmBlk blocks, and the outputs go back through LDS for row-contiguous
+// stores.
+constexpr int kFmNS = 54;                  // k steps: IRs of up to 4 * 54 - 15 = 201 taps
+constexpr int kFmBlk = 4;                  // 16-output blocks per wave
+constexpr int kFmTT = 4 * kFmBlk * 16;     // outputs per row and workgroup
+constexpr int kFmW = kFmTT + 4 * kFmNS;    // staged samples per row
+constexpr int kFmWp = 516;                 // LDS row stride: = 4 mod 64, >= kFmW
+static_assert(kFmWp >= kFmW && kFmWp % 64 == 4, "LDS row stride");
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+
+__global__ __launch_bounds__(256) void k_fir_mfma(const float *__restrict__ x, float *__restrict__ out,
+                                                  const double *__restrict__ taps, int L, int off, int D, int nT,
+                                                  int rows, float scale, const int2 *__restrict__ win)
+{
+    __shared__ float xs[16 * kFmWp];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int r0 = blockIdx.y * 16, t0 = blockIdx.x * kFmTT;
+    const int sbase = t0 + off - (L - 1);   // sample at staged index 0
+    constexpr float kInvW = 1.0f / (float)kFmW;
+    stage_batched<8, 256>(
+        threadIdx.x, 16 * kFmW,
+        [&](int e) {
+            int r, m;
+            divmod_small(e, kFmW, kInvW, r, m);
+            const int row = r0 + r;
+            if (row >= rows) return 0.0f;
+            int lo = 0, hi = nT;   // samples outside [lo, hi) are zero on input and output
+            if (win) {
+                const int2 wb = win[row / D];
+                lo = max(0, wb.x);
+                hi = min(nT, wb.y);
+            }
+            const int t = sbase + m;
+            return (t >= lo && t < hi) ? x[(size_t)row * nT + t] : 0.0f;
+        },
+        [&](int e, float v) {
+            int r, m;
+            divmod_small(e, kFmW, kInvW, r, m);
+            xs[r * kFmWp + m] = v;
+        });
+    const int a = lane & 15, q = lane >> 4;
+    double A[kFmNS];
+#pragma unroll
+    for (int s = 0; s < kFmNS; ++s) {
+        const int i = a + L - 1 - (4 * s + q);
+        A[s] = (i >= 0 && i < L) ? taps[i] : 0.0;
+    }
+    __syncthreads();
+    f64x4 acc[kFmBlk];
+#pragma unroll
+    for (int b = 0; b < kFmBlk; ++b) acc[b] = f64x4{0.0, 0.0, 0.0, 0.0};
+    // lane l supplies X[4 s + l / 16][row l % 16] of block b: staged index 16 (kFmBlk w + b) + 4 s + l / 16
+    const float *xb = xs + a * kFmWp + q + wv * (kFmBlk * 16);
+#pragma unroll
+    for (int s = 0; s < kFmNS; ++s) {
+#pragma unroll
+        for (int b = 0; b < kFmBlk; ++b)
+            acc[b] = __builtin_amdgcn_mfma_f64_16x16x4f64(A[s], (double)xb[16 * b + 4 * s], acc[b], 0, 0, 0);
+    }
+    // xs is reused for the outputs; the f64 16x16 result layout: lane l, register v holds
+    // D[4 v + l / 16][l % 16], i.e. y[row l % 16][tb + 4 v + l / 16]
+    __syncthreads();
+#pragma unroll
+    for (int b = 0; b < kFmBlk; ++b)
+#pragma unroll
+        for (int v = 0; v < 4; ++v)
+            xs[a * kFmWp + wv * (kFmBlk * 16) + 16 * b + 4 * v + q] = scale * (float)acc[b][v];
+    __syncthreads();
+    constexpr float kInvT = 1.0f / (float)kFmTT;
+    for (int e = threadIdx.x; e < 16 * kFmTT; e += 256) {
+        int r, m;
+        divmod_small(e, kFmTT, kInvT, r, m);
+        const int row = r0 + r, t = t0 + m;
+        if (row >= rows || t >= nT) continue;
+        int lo = 0, hi = nT;
+        if (win) {
+            const int2 wb = win[row / D];
+            lo = max(0, wb.x);
+            hi = min(nT, wb.y);
+        }
+        out[(size_t)row * nT + t] = (t >= lo && t < hi) ? xs[r * kFmWp + m] : 0.0f;
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -2684,6 +2777,14 @@ static int launch_fir(Plan *p, const float *x, float *out, int B, bool corr, flo
     const int off = L / 2;            // odd L: the same offset serves the convolution and the flipped-IR correlation
     const int Lp = fir16_padded_taps(L);
     const size_t lds16 = 2 * sizeof(float) * (size_t)fir16_window(Lp);
+    if (!p->ir_per_det && L <= 4 * kFmNS - 15 && p->d_ir64 && p->fir_mfma) {
+        const int rows = B * p->D;
+        const dim3 grid((p->nT + kFmTT - 1) / kFmTT, (rows + 15) / 16);
+        hipLaunchKernelGGL(k_fir_mfma, grid, dim3(256), 0, s, x, out, corr ? p->d_irf64 : p->d_ir64, L, off, p->D,
+                           p->nT, rows, scale, win);
+        MBREC_HIP(hipGetLastError());
+        return MBREC_OK;
+    }
     if (lds16 <= kLdsBudget && p->d_ir64) {
         const int rows = B * p->D, ntile = (p->nT + kFirSpan - 1) / kFirSpan;
         const size_t nblk = ((size_t)rows * ntile + 1) / 2;

@@ -48,6 +48,7 @@ SIGNATURES = [
     ("mbrec_plan_tau_frac_bits", C.c_int, [c_void_p]),
     ("mbrec_plan_autotune", C.c_int, [c_void_p, C.c_int, C.c_int, c_void_p, c_int32_p]),
     ("mbrec_plan_set_launch", C.c_int, [c_void_p, C.c_int, C.c_int, C.c_int]),
+    ("mbrec_plan_set_fir_kernel", C.c_int, [c_void_p, C.c_int]),
     ("mbrec_num_gather_variants", C.c_int, []),
     ("mbrec_plan_time_stage", C.c_int, [c_void_p, C.c_int, C.c_int, C.c_int, c_void_p, c_float_p]),
     ("mbrec_mex_forward_dev", C.c_int, [c_void_p, c_void_p, c_void_p, C.c_int, c_void_p]),

@@ -99,6 +99,11 @@ class DevicePlan:
         """Force a launch variant for batches of B (see mbrec_plan_set_launch)."""
         _lib.check(_lib.lib().mbrec_plan_set_launch(self.h, int(B), int(gather), int(scatter)))
 
+    def set_fir_kernel(self, mfma):
+        """IR filter kernel: FP64 matrix-core Toeplitz products (True) or the vector
+        kernel (False, default); both sum in FP64."""
+        _lib.check(_lib.lib().mbrec_plan_set_fir_kernel(self.h, 1 if mfma else 0))
+
     STAGES = {"scatter": 0, "bins_to_y": 1, "fir": 2, "z_to_bins": 3, "gather": 4}
 
     def time_stage(self, stage, B=1, reps=20):

@@ -182,11 +182,15 @@ def _oracle_of(model):
                    sigma=model.normalizationFactor, **kw)
 
 
+@pytest.mark.parametrize("fir_mfma", [True, False])
 @pytest.mark.parametrize("sir", [False, True])
-def test_model_operators_match_oracle(sir):
+def test_model_operators_match_oracle(sir, fir_mfma):
+    """Both IR filter kernels (FP64 matrix cores, vector) against the oracle, with the
+    adjoint identity of the device pair."""
     import torch
-    _log(f"model_ops sir={sir}")
+    _log(f"model_ops sir={sir} fir_mfma={fir_mfma}")
     m = _geometry_model(48, 40, 32, 700, 1e-4, sir=sir, sigma=2.5)
+    m.plan.set_fir_kernel(fir_mfma)
     orc = _oracle_of(m)
     x = np.maximum(rng(9).standard_normal((2, 40, 48)), 0)
     y = m.applyForward(x)
