@@ -46,6 +46,10 @@ def parse():
     ap.add_argument("--cpu-sample-reps", type=int, default=100)
     ap.add_argument("--traffic-calibration", action="store_true",
                     help="end with a plain 256 MiB device copy (reference for the FETCH/WRITE_SIZE counters)")
+    ap.add_argument("--secondary", action="store_true",
+                    help="time the secondary paths (dmas_cf, bp_kernel, backproject_waveeq, detector "
+                         "upsampling, TV and L1 solves) on the device and their oracle on the host; "
+                         "prints one JSON line")
     ap.add_argument("--traffic-from", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
                     help="no GPU: turn two rocprofv3 --pmc counter_collection.csv files of a "
                          "--traffic-calibration run into profiles/traffic.json and exit")
@@ -95,10 +99,139 @@ def traffic_from(fetch_csv, write_csv, N=400, B=64):
     print(json.dumps(out, indent=1))
 
 
+def secondary(a):
+    """Secondary paths at the 400^2 / 256 x 2030 sizes: device time (HIP events on the
+    launch stream, inputs resident) next to the FP64 oracle on the host cores."""
+    import numpy as np
+    import torch
+
+    from mbrec import _lib
+    from mbrec.discretization import Discretization
+    from mbrec.model import MSOTmodel
+    from mbrec.phantoms import add_noise, disks
+    from mbrec.preprocess import spline_upsampling_matrix
+    from mbrec.probe import Probe
+    from mbrec.solvers import rec_nn_with_L1_eye_reg, rec_nn_with_TV_reg
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+
+    L = _lib.lib()
+    st = _lib.stream_handle
+    C_, FS, COV, RAD = 1500.0, 4e7, 174.0, 0.040
+    N, h, D, nT = a.grid, 1e-4, 256, 2030
+    v = (np.arange(N) - (N - 1) / 2) * h
+    Xm, Zm = np.meshgrid(v, v)
+    X, Z = Xm.T.copy(), Zm.T.copy()   # (Nx, Nz)
+    sig = np.random.default_rng(11).standard_normal((1, D, nT)).astype(np.float32)
+    res = {}
+
+    def dev_time(fn, reps=5):
+        fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / reps
+
+    def host_time(fn):
+        t = time.perf_counter()
+        out = fn()
+        return (time.perf_counter() - t) * 1e3, out
+
+    def entry(name, gpu_ms, cpu_ms, note, err=None):
+        res[name] = {"gpu_ms": round(gpu_ms, 4), "cpu_ms": round(cpu_ms, 2), "cpu_cores": O.num_threads(),
+                     "cpu_kind": "port", "speedup": round(cpu_ms / gpu_ms, 1), "workload": note}
+        if err is not None:
+            res[name]["rel_l2_vs_oracle"] = float(err)
+        log(f"{name}: device {gpu_ms:.3f} ms, oracle {cpu_ms:.1f} ms ({note})")
+
+    rel = lambda x, y: float(np.linalg.norm(np.ravel(x - y)) / max(np.linalg.norm(np.ravel(y)), 1e-300))
+    # dmas_cf (C2 sinogram, W = 1)
+    dx = torch.tensor(X.ravel()).cuda()
+    dz = torch.tensor(Z.ravel()).cuda()
+    ds = torch.tensor(sig).cuda()
+    out = torch.empty((4, 1, N * N), dtype=torch.float64, device="cuda")
+    g = dev_time(lambda: _lib.check(L.mbrec_dmas_cf_dev(dx.data_ptr(), dz.data_ptr(), N * N, ds.data_ptr(), 1, D,
+                                                        nT, C_, FS, 0, COV, RAD, out.data_ptr(), st())))
+    c, want = host_time(lambda: O.dmas_cf(sig.astype(np.float64), X, Z, C_, FS, 0, COV, RAD))
+    got = out.cpu().numpy().reshape(4, 1, N, N)
+    entry("dmas_cf", g, c, f"{N}x{N} px, {D} x {nT} sinogram, 4 outputs",
+          max(rel(got[k], want[k]) for k in range(4)))
+    # bp_kernel (Nt = nT)
+    t = np.arange(1, nT + 1) / FS
+    K = torch.empty((nT, nT), dtype=torch.float64, device="cuda")
+    g = dev_time(lambda: _lib.check(L.mbrec_bp_kernel_dev(t.ctypes.data, nT, K.data_ptr(), st())), reps=2)
+    c, Ko = host_time(lambda: O.bp_kernel(t))
+    entry("bp_kernel", g, c, f"Nt = {nT}", rel(K.cpu().numpy().T, Ko))
+    # backproject_waveeq (C2 sinogram, K above)
+    psf = np.outer(np.hanning(5), np.hanning(3))
+    psf_f = np.asfortranarray(psf).ravel(order="F").copy()
+    xv, zv = np.ascontiguousarray(X[:, 0]), np.ascontiguousarray(Z[0, :])
+    ds64 = torch.tensor(sig.astype(np.float64)).cuda()
+    ob = torch.empty((1, N, N), dtype=torch.float64, device="cuda")
+    g = dev_time(lambda: _lib.check(L.mbrec_backproject_waveeq_dev(
+        xv.ctypes.data, zv.ctypes.data, N, N, ds64.data_ptr(), 1, D, nT, C_, FS, 0, COV, RAD, psf_f.ctypes.data,
+        psf.shape[0], psf.shape[1], K.data_ptr(), nT, ob.data_ptr(), st())))
+    c, want = host_time(lambda: O.backproject_waveeq(sig.astype(np.float64), X, Z, C_, FS, 0, COV, RAD, psf, Ko))
+    entry("backproject_waveeq", g, c, f"{N}x{N} px, {D} x {nT} sinogram, K {nT}^2", rel(ob.cpu().numpy(), want))
+    # detector upsampling 256 -> 512, 32 sinograms
+    nb = 32
+    U = spline_upsampling_matrix(D, 2)
+    Ud = torch.tensor(np.asfortranarray(U).ravel(order="F").copy()).cuda()
+    s32 = np.random.default_rng(12).standard_normal((nb, D, nT))
+    sd = torch.tensor(s32).cuda()
+    ou = torch.empty((nb, 2 * D, nT), dtype=torch.float64, device="cuda")
+    g = dev_time(lambda: _lib.check(L.mbrec_upsample_detectors_dev(Ud.data_ptr(), 2 * D, D, sd.data_ptr(), nT, nb,
+                                                                    ou.data_ptr(), st())))
+    Uo = O.not_a_knot_spline_matrix(D, 2)
+    c, want = host_time(lambda: np.einsum("ud,bdt->but", Uo, s32))
+    entry("upsample_detectors", g, c, f"{nb} sinograms {D} -> {2 * D} detectors x {nT}", rel(ou.cpu().numpy(), want))
+    # TV and L1 solves: 400^2 DAS model, one sinogram; the oracle on a bounded number of
+    # outer iterations, scaled to the device's
+    half = (N - 1) / 2 * h
+    disc = Discretization([-half, half], N)
+    model = MSOTmodel(Probe("arc256_r40_174", True, False), disc, 1500.0, False, True, 0, 1e5, 12e6, None,
+                      sigma_iterations=40)
+    kz, kx = O.full_kernels(N, N, disc.dz, disc.dx, disc.pixelSigma)
+    orc = O.Model(N, N, model.Data.absoluteTravelTimeIndices.reshape(D, -1), nT,
+                  model.Data.circularInterpolationRadiusInSamples, sigma=model.normalizationFactor, kz=kz, kx=kx,
+                  ir=model.Data.IR)
+    ph = disks(disc, n=20, seed=3)[None]
+    y = add_noise(orc.forward(ph), 0.01, seed=4)
+    it_gpu, it_cpu = 10, 2
+    tau = 1e-3
+    eta = 0.05 * float(np.abs(orc.transpose(y)).max())
+    rec_nn_with_TV_reg(model, y, 1, tau)             # first calls: workspace and library set-up
+    rec_nn_with_L1_eye_reg(model, y, 1, eta)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    rec_nn_with_TV_reg(model, y, it_gpu, tau)
+    torch.cuda.synchronize()
+    g = (time.perf_counter() - t0) * 1e3 / it_gpu
+    c, _ = host_time(lambda: O.rec_nn_with_TV_reg(orc, y, it_cpu, tau))
+    entry("rec_nn_with_TV_reg", g, c / it_cpu, f"{N}x{N} DAS, 1 sinogram, per outer iteration "
+          f"(device {it_gpu}, oracle {it_cpu} iterations, wall incl. host calls)")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    rec_nn_with_L1_eye_reg(model, y, it_gpu, eta)
+    torch.cuda.synchronize()
+    g = (time.perf_counter() - t0) * 1e3 / it_gpu
+    c, _ = host_time(lambda: orc.rec_nn_with_L1_eye_reg(y, it_cpu, eta))
+    entry("rec_nn_with_L1_eye_reg", g, c / it_cpu, f"{N}x{N} DAS, 1 sinogram, per outer iteration "
+          f"(device {it_gpu}, oracle {it_cpu} iterations, wall incl. host calls)")
+    print(json.dumps({"secondary": res}), flush=True)
+
+
 def main():
     a = parse()
     if a.traffic_from:
         traffic_from(*a.traffic_from, N=a.grid, B=a.batch)
+        return
+    if a.secondary:
+        secondary(a)
         return
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only; --gpus {a.gpus} is not supported in this build "
