@@ -2446,6 +2446,100 @@ __global__ __launch_bounds__(1024) void k_gather_tas(const int4 *__restrict__ st
     }
 }
 
+// SIR transpose for one sinogram, LDS tile form: WG = one 256-slot tile (thread k: slot k)
+// x one of G detector groups.  Per detector the tile's window of bin rows (the SIR tile
+// table's, at most 256 rows) is copied to LDS (one row per thread, double buffered); the
+// weights of a wave's 64 pairs are loaded cooperatively (each load instruction reads two
+// pairs' contiguous windows, 32 taps each, instead of 64 scattered words) one detector
+// ahead and passed through a per-wave LDS area; each thread then sums its pair's in-range
+// taps w (g0 + u (g1 - g0)) from LDS.
+constexpr int kTas1WS = 65;   // LDS floats per pair couple: pair 2i at 65 i, pair 2i + 1 at 65 i + 32
+
+template <int G>
+__global__ __launch_bounds__(256) void k_gather_tas1(const int4 *__restrict__ stab, const int32_t *__restrict__ stwin,
+                                                     const int32_t *__restrict__ pix, const float *__restrict__ bins,
+                                                     const float *__restrict__ sirw, float *__restrict__ out, int D,
+                                                     int NB, int P, float scale, int ZR)
+{
+    __shared__ float win[2][256];
+    __shared__ float wts[4][32 * kTas1WS];
+    const int tile = blockIdx.x, grp = blockIdx.y, k = threadIdx.x, lane = k & 63;
+    const int DG = (D + G - 1) / G, dbeg = grp * DG, dend = min(D, dbeg + DG);
+    const size_t nrows = (size_t)D * NB + 2;   // bins rows incl. the two zero rows
+    float *ws = wts[k >> 6];
+    auto row_of = [&](int d) -> float {
+        const size_t r = (size_t)d * NB + stwin[(size_t)tile * D + d] + k;
+        return (k < ZR && r < nrows) ? bins[r] : 0.0f;
+    };
+    // instruction i: lanes 0..31 load taps 0..31 of the wave's pair 2i, lanes 32..63 of pair
+    // 2i + 1 (zero past the pair's count: sirw[0] is a zero margin)
+    auto wload = [&](const int4 &ee, float (&wr)[32]) {
+#pragma unroll
+        for (int i = 0; i < 32; ++i) {
+            const int q = 2 * i + (lane >> 5), t = lane & 31;
+            const int cq = __shfl(ee.w, q, 64), zq = __shfl(ee.z, q, 64);
+            wr[i] = sirw[t < cq ? zq + t : 0];
+        }
+    };
+    float wr[32];
+    float acc = 0.0f, part = 0.0f;
+    if (dbeg < dend) {
+        int4 e = stab[((size_t)tile * D + dbeg) * 256 + k];
+        float wv = row_of(dbeg);
+        wload(e, wr);
+#pragma unroll
+        for (int i = 0; i < 32; ++i) ws[kTas1WS * i + lane] = wr[i];
+        for (int d = dbeg; d < dend; ++d) {
+            float *wb = win[(d - dbeg) & 1];
+            wb[k] = wv;
+            // the window buffer written here was last read two detectors ago: every thread
+            // has passed the previous detector's barrier since
+            __syncthreads();
+            int4 en = e;
+            if (d + 1 < dend) {
+                en = stab[((size_t)tile * D + d + 1) * 256 + k];
+                wv = row_of(d + 1);
+                wload(en, wr);
+            }
+            const int cnt = e.w;
+            const float *wp = ws + kTas1WS * (lane >> 1) + 32 * (lane & 1);
+            const float *g = wb + e.x;
+            float A = 0.0f, C = 0.0f;
+            int t = 0;
+            for (; t + 4 <= cnt; t += 4) {
+                A = fmaf(wp[t], g[3 * t], A);
+                C = fmaf(wp[t], g[3 * t + 1], C);
+                A = fmaf(wp[t + 1], g[3 * t + 3], A);
+                C = fmaf(wp[t + 1], g[3 * t + 4], C);
+                A = fmaf(wp[t + 2], g[3 * t + 6], A);
+                C = fmaf(wp[t + 2], g[3 * t + 7], C);
+                A = fmaf(wp[t + 3], g[3 * t + 9], A);
+                C = fmaf(wp[t + 3], g[3 * t + 10], C);
+            }
+            for (; t < cnt; ++t) {
+                A = fmaf(wp[t], g[3 * t], A);
+                C = fmaf(wp[t], g[3 * t + 1], C);
+            }
+            part += fmaf(__int_as_float(e.y), C - A, A);
+            if (((d - dbeg) & 7) == 7) {
+                acc += part;
+                part = 0.0f;
+            }
+            if (d + 1 < dend) {   // the wave's own area: its reads above come first (LDS order)
+#pragma unroll
+                for (int i = 0; i < 32; ++i) ws[kTas1WS * i + lane] = wr[i];
+            }
+            e = en;
+        }
+    }
+    const int px = pix[(size_t)tile * 256 + k];
+    if (px < 0) return;
+    if (G > 1)
+        out[(size_t)grp * P + px] = acc + part;
+    else
+        out[px] = scale * (acc + part);
+}
+
 // out[b][x] = scale * sum of the NP partial images (fixed order)
 template <int NP>
 __global__ __launch_bounds__(256) void k_sum_parts(const float *__restrict__ part, float *__restrict__ out,
@@ -2981,6 +3075,18 @@ static void gather_tas(Plan *p, float *img, int B, int Bp, float scale, hipStrea
 }
 
 template <int G>
+static void gather_tas1(Plan *p, float *img, float scale, hipStream_t s)
+{
+    hipLaunchKernelGGL((k_gather_tas1<G>), dim3(p->Pd / 256, G), dim3(256), 0, s, p->d_stab, p->d_stwin, p->d_pix,
+                       p->d_bins, p->d_sirw, G > 1 ? p->d_gpart : img, p->D, p->geom.NB, p->P, scale, p->stile_zr);
+    if (G > 1) {
+        const size_t n = (size_t)p->P;
+        hipLaunchKernelGGL(k_sum_parts<G>, dim3(std::min<size_t>(4096, (n + 255) / 256)), dim3(256), 0, s,
+                           (const float *)p->d_gpart, img, n, scale);
+    }
+}
+
+template <int G>
 static void gather_tt(Plan *p, float *img, int B, int Bp, float scale, hipStream_t s)
 {
     const int nbc = (Bp + 63) / 64;
@@ -3022,12 +3128,16 @@ static const GatherVariant kGatherVariants[] = {{1, 8, 0},   {1, 4, 0},   {4, 4,
                                                 {0, -7, 16}, {0, -8, 8},  {0, -8, 16}, {0, -9, 1},
                                                 {0, -9, 2},  {0, -9, 4},  {0, -9, 8},  {0, -10, 1},
                                                 {0, -10, 2}, {0, -10, 4}, {0, -11, 1}, {0, -11, 2},
-                                                {0, -11, 4}};
+                                                {0, -11, 4}, {0, -12, 1}, {0, -12, 2}, {0, -12, 4},
+                                                {0, -12, 8}};
 constexpr int kNumGatherVariants = sizeof(kGatherVariants) / sizeof(kGatherVariants[0]);
 
 static bool gather_variant_ok(const Plan *p, int v, int Bp)
 {
     const GatherVariant &g = kGatherVariants[v];
+    if (g.u == -12)   // SIR tile gather for one sinogram: the SIR tile tables, G partial images
+        return p->use_sir && p->d_stab && p->stile_zr >= 64 && p->stile_zr <= 256 && Bp == 1 &&
+               (g.lb == 1 || p->d_gpart);
     if (g.u == -11) {   // SIR tile gather: tables built, window within 4 rounds of 64 rows, 32-bit offsets
         if (!p->use_sir || !p->d_stab || p->stile_zr < 64 || p->stile_zr > 256 || Bp < 16) return false;
         if (sizeof(float) * ((size_t)p->D * p->geom.NB + 2) * Bp >= ((size_t)1 << 31)) return false;
@@ -3078,7 +3188,7 @@ static bool gather_variant_ok(const Plan *p, int v, int Bp)
 
 static int default_gather_variant(const Plan *p, int Bp)
 {
-    if (Bp == 1) return 0;
+    if (Bp == 1) return gather_variant_ok(p, 43, Bp) ? 43 : 0;   // SIR: the tile form in 8 detector groups
     if (p->use_sir && Bp >= 16)
         for (int v : {39, 38, 37})   // SIR tile gather in four detector groups, two, one
             if (gather_variant_ok(p, v, Bp)) return v;
@@ -3139,7 +3249,11 @@ static int launch_gather(Plan *p, float *img, int B, int Bp, float scale, hipStr
     case 36: gather_ta<4>(p, img, B, Bp, scale, s); break;
     case 37: gather_tas<1>(p, img, B, Bp, scale, s); break;
     case 38: gather_tas<2>(p, img, B, Bp, scale, s); break;
-    default: gather_tas<4>(p, img, B, Bp, scale, s); break;
+    case 39: gather_tas<4>(p, img, B, Bp, scale, s); break;
+    case 40: gather_tas1<1>(p, img, scale, s); break;
+    case 41: gather_tas1<2>(p, img, scale, s); break;
+    case 42: gather_tas1<4>(p, img, scale, s); break;
+    default: gather_tas1<8>(p, img, scale, s); break;
     }
     MBREC_HIP(hipGetLastError());
     return MBREC_OK;
