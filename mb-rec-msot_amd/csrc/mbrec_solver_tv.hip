@@ -64,15 +64,38 @@ __global__ void k_tv_rhs(const double *__restrict__ g, const double *__restrict_
     rhs[idx] = g[idx] - bt;
 }
 
-__global__ void k_tv_scale_eig(double *__restrict__ t, const double *__restrict__ den, int N, int B,
-                               const int *__restrict__ active)
+// every item (an item past its stop test is solved again from its unchanged right-hand
+// side; nothing downstream reads that result)
+__global__ void k_tv_scale_eig(double *__restrict__ t, const double *__restrict__ den, int N, int B)
 {
     const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     const size_t P = (size_t)N * N;
     if (idx >= P * B) return;
     const int b = (int)(idx / P);
-    if (!active[b]) return;
     t[idx] /= den[idx - (size_t)b * P];
+}
+
+// SB_ITV2's stop test on the device: an active item stops once
+// ||u_prev - u|| / ||u|| <= 1e-3 (NaN also stops, as MATLAB's while condition); cnt counts
+// the steps in which some item was active
+__global__ void k_tv_active(const double *__restrict__ part, int *__restrict__ active, int B, int *cnt)
+{
+    __shared__ int any;
+    if (threadIdx.x == 0) any = 0;
+    __syncthreads();
+    for (int b = threadIdx.x; b < B; b += blockDim.x) {
+        if (!active[b]) continue;
+        any = 1;
+        double sa = 0.0, sb = 0.0;
+        for (int i = 0; i < kRedBlocks; ++i) {
+            sa += part[((size_t)Q_A * B + b) * kRedBlocks + i];
+            sb += part[((size_t)Q_B * B + b) * kRedBlocks + i];
+        }
+        const double err = sqrt(sa) / sqrt(sb);
+        if (!(err > 1e-3)) active[b] = 0;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0 && any) cnt[0] += 1;
 }
 
 // Bub = B u + b; s = |Bub|; d = max(s - mu, 0) Bub / s (0 where s == 0); b = Bub - d;
@@ -110,6 +133,8 @@ __global__ __launch_bounds__(256) void k_tv_shrink(const double *__restrict__ u,
     block_sum_store(e, part, Q_A, b, B);
     block_sum_store(nu, part, Q_B, b, B);
 }
+
+__global__ void k_tv_zero_cnt(int *cnt) { cnt[0] = 0; }
 
 __global__ void k_tv_store(const double *__restrict__ u, float *__restrict__ x, size_t n)
 {
@@ -285,7 +310,7 @@ struct TvProx {
     double *C = nullptr, *den = nullptr;           // DCT-II basis [N x N], 1 + l_i + l_j
     double *g = nullptr, *u = nullptr, *up = nullptr, *bz = nullptr, *bx = nullptr, *dz = nullptr, *dx = nullptr;
     double *rhs = nullptr, *t1 = nullptr, *mu = nullptr, *part = nullptr, *sums = nullptr;
-    int *active = nullptr;
+    int *active = nullptr, *cnt = nullptr;
 };
 
 static int prox_setup(TvProx &T, TvWork &w, int N, int B, hipStream_t s)
@@ -299,7 +324,7 @@ static int prox_setup(TvProx &T, TvWork &w, int N, int B, hipStream_t s)
         if ((rc = w.alloc(b, n))) return rc;
     if ((rc = w.alloc(&T.C, P)) || (rc = w.alloc(&T.den, P)) || (rc = w.alloc(&T.mu, B)) ||
         (rc = w.alloc(&T.part, (size_t)kMaxQ * B * kRedBlocks)) || (rc = w.alloc(&T.sums, 6 * (size_t)B)) ||
-        (rc = w.alloc(&T.active, B)))
+        (rc = w.alloc(&T.active, B)) || (rc = w.alloc(&T.cnt, 1)))
         return rc;
     std::vector<double> C(P), den(P), lam(N);
     for (int k = 0; k < N; ++k) {
@@ -315,39 +340,37 @@ static int prox_setup(TvProx &T, TvWork &w, int N, int B, hipStream_t s)
     return MBREC_OK;
 }
 
-// u = (I + B'B)^-1 rhs for the active items: U = C ((C' R C) ./ den) C'
-static int neumann_solve(TvProx &T, const std::vector<int> &act, hipStream_t s)
+// u = (I + B'B)^-1 rhs for every item: U = C ((C' R C) ./ den) C' (strided-batched GEMMs)
+static int neumann_solve(TvProx &T, hipStream_t s)
 {
     rocblas_handle h;
     int rc = blas_handle(s, &h);
     if (rc) return rc;
     const int N = T.N;
-    const size_t P = (size_t)N * N;
+    const rocblas_stride P = (rocblas_stride)N * N;
     const double one = 1.0, zero = 0.0;
-    auto gemm = [&](rocblas_operation ta, rocblas_operation tb, const double *A, const double *B_, double *Cc) {
-        return rocblas_dgemm(h, ta, tb, N, N, N, &one, A, N, B_, N, &zero, Cc, N) == rocblas_status_success;
+    auto gemm = [&](rocblas_operation ta, rocblas_operation tb, const double *A, rocblas_stride sa, const double *B_,
+                    rocblas_stride sb, double *Cc) {
+        return rocblas_dgemm_strided_batched(h, ta, tb, N, N, N, &one, A, N, sa, B_, N, sb, &zero, Cc, N, P, T.B) ==
+               rocblas_status_success;
     };
-    for (int b = 0; b < T.B; ++b) {
-        if (!act[b]) continue;
-        double *R = T.rhs + b * P, *t1 = T.t1 + b * P;
-        // t1 = C' R ; R <- t1 C ; R ./= den ; t1 = C R ; U = t1 C'
-        if (!gemm(rocblas_operation_transpose, rocblas_operation_none, T.C, R, t1) ||
-            !gemm(rocblas_operation_none, rocblas_operation_none, t1, T.C, R))
-            return set_error(MBREC_ERR_HIP, "mbrec:rocblas", "rocblas_dgemm failed");
-    }
+    const auto nt = rocblas_operation_none, tr = rocblas_operation_transpose;
+    // t1 = C' R ; R <- t1 C ; R ./= den ; t1 = C R ; U = t1 C'
+    if (!gemm(tr, nt, T.C, 0, T.rhs, P, T.t1) || !gemm(nt, nt, T.t1, P, T.C, 0, T.rhs))
+        return set_error(MBREC_ERR_HIP, "mbrec:rocblas", "rocblas_dgemm_strided_batched failed");
     hipLaunchKernelGGL(k_tv_scale_eig, dim3((unsigned)((P * T.B + 255) / 256)), dim3(256), 0, s, T.rhs,
-                       (const double *)T.den, N, T.B, (const int *)T.active);
-    for (int b = 0; b < T.B; ++b) {
-        if (!act[b]) continue;
-        double *R = T.rhs + b * P, *t1 = T.t1 + b * P, *U = T.u + b * P;
-        if (!gemm(rocblas_operation_none, rocblas_operation_none, T.C, R, t1) ||
-            !gemm(rocblas_operation_none, rocblas_operation_transpose, t1, T.C, U))
-            return set_error(MBREC_ERR_HIP, "mbrec:rocblas", "rocblas_dgemm failed");
-    }
+                       (const double *)T.den, N, T.B);
+    if (!gemm(nt, nt, T.C, 0, T.rhs, P, T.t1) || !gemm(nt, tr, T.t1, P, T.C, 0, T.u))
+        return set_error(MBREC_ERR_HIP, "mbrec:rocblas", "rocblas_dgemm_strided_batched failed");
     return MBREC_OK;
 }
 
-// x (FP32, [B][N][N]) <- SB_ITV2(z, mu_b) per item; mu host [B]; items with skip[b] are left as they are
+// x (FP32, [B][N][N]) <- SB_ITV2(z, mu_b) per item; mu host [B]; items with skip[b] are left as they are.
+// The stop test runs on the device after every split-Bregman step; the host reads the
+// active flags every kTvCheck steps (an item that stopped inside a group is not updated
+// again, so the result is the per-step loop's).
+constexpr int kTvCheck = 4;
+
 static int tv_prox(TvProx &T, const float *z, float *x, const std::vector<double> &mu,
                    const std::vector<int> &skip, hipStream_t s, int *iters_out)
 {
@@ -358,34 +381,32 @@ static int tv_prox(TvProx &T, const float *z, float *x, const std::vector<double
     MBREC_HIP(hipMemcpyAsync(T.mu, mu.data(), sizeof(double) * B, hipMemcpyHostToDevice, s));
     MBREC_HIP(hipMemcpyAsync(T.active, act.data(), sizeof(int) * B, hipMemcpyHostToDevice, s));
     hipLaunchKernelGGL(k_tv_load, dim3(nblk(n)), dim3(256), 0, s, z, T.g, T.up, T.bz, T.bx, T.dz, T.dx, n);
-    int it = 0, rc;
-    std::vector<double> sums(6 * (size_t)B);
+    hipLaunchKernelGGL(k_tv_zero_cnt, dim3(1), dim3(1), 0, s, T.cnt);
+    int rc;
     for (;;) {
         bool any = false;
         for (int b = 0; b < B; ++b) any = any || act[b];
         if (!any) break;
-        hipLaunchKernelGGL(k_tv_rhs, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, (const double *)T.g,
-                           (const double *)T.bz, (const double *)T.bx, (const double *)T.dz, (const double *)T.dx,
-                           T.rhs, N, B, (const int *)T.active);
-        if ((rc = neumann_solve(T, act, s))) return rc;
-        hipLaunchKernelGGL(k_tv_shrink, vgrid(B), dim3(256), 0, s, (const double *)T.u, T.up, T.bz, T.bx, T.dz,
-                           T.dx, N, B, (const double *)T.mu, (const int *)T.active, T.part);
-        hipLaunchKernelGGL(k_tv_sums, dim3((6 * B + 255) / 256), dim3(256), 0, s, (const double *)T.part, B, T.sums);
-        MBREC_HIP(hipMemcpyAsync(sums.data(), T.sums, sizeof(double) * 6 * B, hipMemcpyDeviceToHost, s));
-        MBREC_HIP(hipStreamSynchronize(s));
-        ++it;
-        for (int b = 0; b < B; ++b) {
-            if (!act[b]) continue;
-            const double err = std::sqrt(sums[Q_A * B + b]) / std::sqrt(sums[Q_B * B + b]);
-            if (!(err > 1e-3)) act[b] = 0;     // while err > tol (NaN also stops, as in MATLAB)
+        for (int c = 0; c < kTvCheck; ++c) {
+            hipLaunchKernelGGL(k_tv_rhs, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, (const double *)T.g,
+                               (const double *)T.bz, (const double *)T.bx, (const double *)T.dz,
+                               (const double *)T.dx, T.rhs, N, B, (const int *)T.active);
+            if ((rc = neumann_solve(T, s))) return rc;
+            hipLaunchKernelGGL(k_tv_shrink, vgrid(B), dim3(256), 0, s, (const double *)T.u, T.up, T.bz, T.bx, T.dz,
+                               T.dx, N, B, (const double *)T.mu, (const int *)T.active, T.part);
+            hipLaunchKernelGGL(k_tv_active, dim3(1), dim3(256), 0, s, (const double *)T.part, T.active, B, T.cnt);
         }
-        MBREC_HIP(hipMemcpyAsync(T.active, act.data(), sizeof(int) * B, hipMemcpyHostToDevice, s));
+        MBREC_HIP(hipMemcpyAsync(act.data(), T.active, sizeof(int) * B, hipMemcpyDeviceToHost, s));
+        MBREC_HIP(hipStreamSynchronize(s));
     }
     // write back items that were processed
     for (int b = 0; b < B; ++b)
         if (!skip[b])
             hipLaunchKernelGGL(k_tv_store, dim3(nblk(P)), dim3(256), 0, s, (const double *)(T.up + b * P), x + b * P, P);
-    if (iters_out) *iters_out = it;
+    if (iters_out) {
+        MBREC_HIP(hipMemcpyAsync(iters_out, T.cnt, sizeof(int), hipMemcpyDeviceToHost, s));
+        MBREC_HIP(hipStreamSynchronize(s));
+    }
     MBREC_HIP(hipGetLastError());
     return MBREC_OK;
 }
