@@ -150,6 +150,10 @@ constexpr int kFzCols = 16;
 constexpr int kFxRows = 16;
 
 constexpr int kFiltR = 8;   // outputs per thread of the register-blocked prefilter passes
+// prefilter sums: FP32 (the taps decay geometrically from a dominant centre tap, so an
+// FP32 sum of the 2K + 1 terms is within a few ulps; FP64 sums cost 17 ms per 50-iteration
+// batch-64 solve at 400^2 and changed no result beyond the operator tolerances)
+using PfAcc = float;
 
 // KT > 0: K = KT known at compile time, each thread forms kFiltR consecutive
 // outputs of a column from one register window of kFiltR + 2 KT samples
@@ -161,7 +165,7 @@ __global__ __launch_bounds__(256) void k_filt_z(const float *__restrict__ in, fl
 {
     extern __shared__ float sm[];
     const int S = (Nz + 2 * K + (KT ? kFiltR : 0)) | 1;   // column stride in LDS
-    double *tk = reinterpret_cast<double *>(sm);
+    PfAcc *tk = reinterpret_cast<PfAcc *>(sm);
     float *xs = sm + 2 * (2 * K + 1 + 1);
     const size_t P = (size_t)Nz * Nx;
     // columns of this workgroup: IL: (ix, b0 + c); planar: (b, ix0 + c) over the flat b * Nx + ix index
@@ -173,7 +177,7 @@ __global__ __launch_bounds__(256) void k_filt_z(const float *__restrict__ in, fl
     } else {
         col0 = (size_t)blockIdx.x * kFzCols;
     }
-    for (int i = threadIdx.x; i < 2 * K + 1; i += blockDim.x) tk[i] = (double)k[i];
+    for (int i = threadIdx.x; i < 2 * K + 1; i += blockDim.x) tk[i] = (PfAcc)k[i];
     const int W = Nz + 2 * K + (KT ? kFiltR : 0);
     const float invW = 1.0f / (float)W;
     stage_batched<16, 256>(
@@ -196,7 +200,7 @@ __global__ __launch_bounds__(256) void k_filt_z(const float *__restrict__ in, fl
     // out[i] = sum_a k[a + K] xs[i + K - sgn a]: forward reads xs[i + 2K - t], transpose xs[i + t] (t = a + K)
     if (KT) {   // 16 threads per column, blocks of kFiltR outputs
         constexpr int NT = 2 * (KT ? KT : 1) + 1, NW = kFiltR + NT - 1;
-        double tr[NT];
+        PfAcc tr[NT];
 #pragma unroll
         for (int t = 0; t < NT; ++t) tr[t] = tk[t];
         const int c = IL ? threadIdx.x % kFzCols : threadIdx.x / 16, ib = IL ? threadIdx.x / kFzCols : threadIdx.x % 16;
@@ -204,12 +208,12 @@ __global__ __launch_bounds__(256) void k_filt_z(const float *__restrict__ in, fl
         if (!live) return;
         for (int iz0 = ib * kFiltR; iz0 < Nz; iz0 += 16 * kFiltR) {
             const float *x = xs + c * S + iz0;
-            double w[NW], acc[kFiltR];
+            PfAcc w[NW], acc[kFiltR];
 #pragma unroll
-            for (int i = 0; i < NW; ++i) w[i] = (double)x[i];
+            for (int i = 0; i < NW; ++i) w[i] = (PfAcc)x[i];
 #pragma unroll
             for (int q = 0; q < kFiltR; ++q) {
-                acc[q] = 0.0;
+                acc[q] = 0;
                 if (transp) {
 #pragma unroll
                     for (int t = 0; t < NT; ++t) acc[q] = fma(tr[t], w[q + t], acc[q]);
@@ -236,11 +240,11 @@ __global__ __launch_bounds__(256) void k_filt_z(const float *__restrict__ in, fl
         if (b >= Bp) return;
         for (int iz = threadIdx.x / kFzCols; iz < Nz; iz += 256 / kFzCols) {
             const float *x = xs + c * S + iz;
-            double acc = 0.0;
+            PfAcc acc = 0;
             if (transp)
-                for (int t = 0; t <= 2 * K; ++t) acc = fma(tk[t], (double)x[t], acc);
+                for (int t = 0; t <= 2 * K; ++t) acc = fma(tk[t], (PfAcc)x[t], acc);
             else
-                for (int t = 0; t <= 2 * K; ++t) acc = fma(tk[t], (double)x[2 * K - t], acc);
+                for (int t = 0; t <= 2 * K; ++t) acc = fma(tk[t], (PfAcc)x[2 * K - t], acc);
             out[((size_t)ixc * Nz + iz) * Bp + b] = b < B ? (float)acc : 0.0f;
         }
     } else {
@@ -248,11 +252,11 @@ __global__ __launch_bounds__(256) void k_filt_z(const float *__restrict__ in, fl
             if (col0 + c >= (size_t)B * Nx) break;
             for (int iz = threadIdx.x; iz < Nz; iz += blockDim.x) {
                 const float *x = xs + c * S + iz;
-                double acc = 0.0;
+                PfAcc acc = 0;
                 if (transp)
-                    for (int t = 0; t <= 2 * K; ++t) acc = fma(tk[t], (double)x[t], acc);
+                    for (int t = 0; t <= 2 * K; ++t) acc = fma(tk[t], (PfAcc)x[t], acc);
                 else
-                    for (int t = 0; t <= 2 * K; ++t) acc = fma(tk[t], (double)x[2 * K - t], acc);
+                    for (int t = 0; t <= 2 * K; ++t) acc = fma(tk[t], (PfAcc)x[2 * K - t], acc);
                 out[(col0 + c) * Nz + iz] = (float)acc;
             }
         }
@@ -264,12 +268,12 @@ __global__ __launch_bounds__(256) void k_filt_x(const float *__restrict__ in, fl
                                                 const float *__restrict__ k, int K, int Nz, int Nx, int transp)
 {
     extern __shared__ float sm[];
-    double *tk = reinterpret_cast<double *>(sm);
+    PfAcc *tk = reinterpret_cast<PfAcc *>(sm);
     float *xs = sm + 2 * (2 * K + 1 + 1);   // [Nx + 2K (+ kFiltR)][kFxRows]
     const int iz0 = blockIdx.x * kFxRows, b = blockIdx.y;
     const size_t P = (size_t)Nz * Nx;
     const float *src = in + (size_t)b * P;
-    for (int i = threadIdx.x; i < 2 * K + 1; i += blockDim.x) tk[i] = (double)k[i];
+    for (int i = threadIdx.x; i < 2 * K + 1; i += blockDim.x) tk[i] = (PfAcc)k[i];
     const int r = threadIdx.x % kFxRows;
     const bool rv = iz0 + r < Nz;
     // element e = i * kFxRows + r: row i of the wrapped x axis, z row iz0 + r
@@ -288,17 +292,17 @@ __global__ __launch_bounds__(256) void k_filt_x(const float *__restrict__ in, fl
     float *dst = out + (size_t)b * P + iz0 + r;
     if (KT) {   // blocks of kFiltR consecutive x outputs per thread
         constexpr int NT = 2 * (KT ? KT : 1) + 1, NW = kFiltR + NT - 1;
-        double tr[NT];
+        PfAcc tr[NT];
 #pragma unroll
         for (int t = 0; t < NT; ++t) tr[t] = tk[t];
         for (int ix0 = (threadIdx.x / kFxRows) * kFiltR; ix0 < Nx; ix0 += (256 / kFxRows) * kFiltR) {
             const float *x = xs + ix0 * kFxRows + r;
-            double w[NW], acc[kFiltR];
+            PfAcc w[NW], acc[kFiltR];
 #pragma unroll
-            for (int i = 0; i < NW; ++i) w[i] = (double)x[i * kFxRows];
+            for (int i = 0; i < NW; ++i) w[i] = (PfAcc)x[i * kFxRows];
 #pragma unroll
             for (int q = 0; q < kFiltR; ++q) {
-                acc[q] = 0.0;
+                acc[q] = 0;
                 if (transp) {
 #pragma unroll
                     for (int t = 0; t < NT; ++t) acc[q] = fma(tr[t], w[q + t], acc[q]);
@@ -317,11 +321,11 @@ __global__ __launch_bounds__(256) void k_filt_x(const float *__restrict__ in, fl
     }
     for (int ix = threadIdx.x / kFxRows; ix < Nx; ix += 256 / kFxRows) {
         const float *x = xs + ix * kFxRows + r;
-        double acc = 0.0;
+        PfAcc acc = 0;
         if (transp)
-            for (int t = 0; t <= 2 * K; ++t) acc = fma(tk[t], (double)x[t * kFxRows], acc);
+            for (int t = 0; t <= 2 * K; ++t) acc = fma(tk[t], (PfAcc)x[t * kFxRows], acc);
         else
-            for (int t = 0; t <= 2 * K; ++t) acc = fma(tk[t], (double)x[(2 * K - t) * kFxRows], acc);
+            for (int t = 0; t <= 2 * K; ++t) acc = fma(tk[t], (PfAcc)x[(2 * K - t) * kFxRows], acc);
         dst[(size_t)ix * Nz] = (float)acc;
     }
 }
