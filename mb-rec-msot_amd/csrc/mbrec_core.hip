@@ -83,7 +83,7 @@ int ensure_work(Plan *p, int B)
         (rc = dalloc(&p->d_imgI, (size_t)p->P * Bp, &acc)) ||
         (rc = dalloc(&p->d_img_tmp, (size_t)2 * B * p->P, &acc)) ||
         (rc = dalloc(&p->d_sino_tmp, (size_t)B * p->D * p->nT, &acc)) ||
-        (rc = dalloc(&p->d_gpart, (size_t)8 * B * p->P, &acc))) {
+        (rc = dalloc(&p->d_gpart, (size_t)std::max(8 * B, 16) * p->P, &acc))) {
         free_work(p);
         return rc;
     }

@@ -123,7 +123,7 @@ struct Plan {
     float *d_imgI = nullptr;     // [Pd..P][Bp]      interleaved image (scatter input)
     float *d_img_tmp = nullptr;  // 2 x [B][P]       prefilter temporaries
     float *d_sino_tmp = nullptr; // [B][D][nT]       samples before/after the IR filter
-    float *d_gpart = nullptr;    // [8][B][P]        partial images of the XCD-grouped gather (B >= 16; DAS only)
+    float *d_gpart = nullptr;    // [max(8 B, 16)][P]  partial images of the detector-grouped gathers
     size_t table_bytes = 0;
     size_t work_bytes = 0;
     int pins = 0;                // batches whose captured graphs use this plan's buffers

@@ -2450,6 +2450,69 @@ __global__ __launch_bounds__(1024) void k_gather_tas(const int4 *__restrict__ st
     }
 }
 
+// Transpose without SIR for one sinogram, LDS tile form: WG = one 256-slot tile (thread k:
+// slot k) x one of G detector groups.  The group's window origins go to LDS first; per
+// detector the tile's bin window (at most tile_zr <= 256 rows; one row per thread) is
+// staged in LDS, double buffered, with the window rows and travel times of the next two
+// detectors in flight; each thread decodes its travel time and interpolates its two bins
+// from LDS (Kahan sum over detectors, as the record-lane gather).
+template <int G>
+__global__ __launch_bounds__(256) void k_gather_ta1(const int32_t *__restrict__ tq, const int2 *__restrict__ twin,
+                                                    const int32_t *__restrict__ pix, const float *__restrict__ bins,
+                                                    float *__restrict__ out, BinGeom g, int D, int Pd, int P,
+                                                    float scale, int ZR)
+{
+    __shared__ float win[2][256];
+    __shared__ int los[256];
+    const int tile = blockIdx.x, grp = blockIdx.y, k = threadIdx.x;
+    const int DG = (D + G - 1) / G, dbeg = grp * DG, dend = min(D, dbeg + DG);
+    const int span = g.nT + 2 * g.K, NB = g.NB;
+    const size_t nrows = (size_t)D * NB + 2;   // bins rows incl. the two zero rows
+    const size_t slot = (size_t)tile * 256 + k;
+    if (k < dend - dbeg) los[k] = twin[(size_t)tile * D + dbeg + k].x;   // DG <= 256
+    __syncthreads();
+    auto row_of = [&](int d) -> float {
+        const size_t r = (size_t)d * NB + los[d - dbeg] + k;
+        return (k < ZR && r < nrows) ? bins[r] : 0.0f;
+    };
+    auto tq_of = [&](int d) -> int32_t { return tq[(size_t)d * Pd + slot]; };
+    float acc = 0.0f, cmp = 0.0f;
+    if (dbeg < dend) {
+        int32_t q0 = tq_of(dbeg), q1 = 0;
+        float w0 = row_of(dbeg), w1 = 0.0f;
+        if (dbeg + 1 < dend) {
+            q1 = tq_of(dbeg + 1);
+            w1 = row_of(dbeg + 1);
+        }
+        for (int d = dbeg; d < dend; ++d) {
+            float *wb = win[(d - dbeg) & 1];
+            wb[k] = w0;
+            // the buffer written here was last read two detectors ago: every thread has
+            // passed the previous detector's barrier since
+            __syncthreads();
+            const int32_t qd = q0;
+            q0 = q1;
+            w0 = w1;
+            if (d + 2 < dend) {
+                q1 = tq_of(d + 2);
+                w1 = row_of(d + 2);
+            }
+            const Decoded dc = decode_tau(qd, g);
+            if ((unsigned)dc.nn < (unsigned)span) {
+                const int r = 3 * dc.nn + dc.m - los[d - dbeg];
+                const float g0 = wb[r], g1 = wb[r + 1];
+                kahan_add(acc, cmp, fmaf(dc.u, g1 - g0, g0));
+            }
+        }
+    }
+    const int px = pix[slot];
+    if (px < 0) return;
+    if (G > 1)
+        out[(size_t)grp * P + px] = acc - cmp;
+    else
+        out[px] = scale * (acc - cmp);
+}
+
 // SIR transpose for one sinogram, LDS tile form: WG = one 256-slot tile (thread k: slot k)
 // x one of G detector groups.  Per detector the tile's window of bin rows (the SIR tile
 // table's, at most 256 rows) is copied to LDS (one row per thread, double buffered); the
@@ -3079,6 +3142,18 @@ static void gather_tas(Plan *p, float *img, int B, int Bp, float scale, hipStrea
 }
 
 template <int G>
+static void gather_ta1(Plan *p, float *img, float scale, hipStream_t s)
+{
+    hipLaunchKernelGGL((k_gather_ta1<G>), dim3(p->Pd / 256, G), dim3(256), 0, s, p->d_tq, p->d_twin, p->d_pix,
+                       p->d_bins, G > 1 ? p->d_gpart : img, p->geom, p->D, p->Pd, p->P, scale, p->tile_zr);
+    if (G > 1) {
+        const size_t n = (size_t)p->P;
+        hipLaunchKernelGGL(k_sum_parts<G>, dim3(std::min<size_t>(4096, (n + 255) / 256)), dim3(256), 0, s,
+                           (const float *)p->d_gpart, img, n, scale);
+    }
+}
+
+template <int G>
 static void gather_tas1(Plan *p, float *img, float scale, hipStream_t s)
 {
     hipLaunchKernelGGL((k_gather_tas1<G>), dim3(p->Pd / 256, G), dim3(256), 0, s, p->d_stab, p->d_stwin, p->d_pix,
@@ -3133,12 +3208,16 @@ static const GatherVariant kGatherVariants[] = {{1, 8, 0},   {1, 4, 0},   {4, 4,
                                                 {0, -9, 2},  {0, -9, 4},  {0, -9, 8},  {0, -10, 1},
                                                 {0, -10, 2}, {0, -10, 4}, {0, -11, 1}, {0, -11, 2},
                                                 {0, -11, 4}, {0, -12, 1}, {0, -12, 2}, {0, -12, 4},
-                                                {0, -12, 8}};
+                                                {0, -12, 8}, {0, -13, 1}, {0, -13, 2}, {0, -13, 4},
+                                                {0, -13, 8}, {0, -13, 16}};
 constexpr int kNumGatherVariants = sizeof(kGatherVariants) / sizeof(kGatherVariants[0]);
 
 static bool gather_variant_ok(const Plan *p, int v, int Bp)
 {
     const GatherVariant &g = kGatherVariants[v];
+    if (g.u == -13)   // tile gather for one sinogram without SIR: tile windows of at most 256 rows
+        return !p->use_sir && p->d_twin && p->tile_zr >= 64 && p->tile_zr <= 256 && Bp == 1 &&
+               (g.lb == 1 || p->d_gpart);
     if (g.u == -12)   // SIR tile gather for one sinogram: the SIR tile tables, G partial images
         return p->use_sir && p->d_stab && p->stile_zr >= 64 && p->stile_zr <= 256 && Bp == 1 &&
                (g.lb == 1 || p->d_gpart);
@@ -3192,7 +3271,9 @@ static bool gather_variant_ok(const Plan *p, int v, int Bp)
 
 static int default_gather_variant(const Plan *p, int Bp)
 {
-    if (Bp == 1) return gather_variant_ok(p, 43, Bp) ? 43 : 0;   // SIR: the tile form in 8 detector groups
+    if (Bp == 1)   // the tile forms in 8 detector groups (SIR, then without), else record-lane
+        for (int v : {43, 47, 0})
+            if (gather_variant_ok(p, v, Bp)) return v;
     if (p->use_sir && Bp >= 16)
         for (int v : {39, 38, 37})   // SIR tile gather in four detector groups, two, one
             if (gather_variant_ok(p, v, Bp)) return v;
@@ -3257,7 +3338,12 @@ static int launch_gather(Plan *p, float *img, int B, int Bp, float scale, hipStr
     case 40: gather_tas1<1>(p, img, scale, s); break;
     case 41: gather_tas1<2>(p, img, scale, s); break;
     case 42: gather_tas1<4>(p, img, scale, s); break;
-    default: gather_tas1<8>(p, img, scale, s); break;
+    case 43: gather_tas1<8>(p, img, scale, s); break;
+    case 44: gather_ta1<1>(p, img, scale, s); break;
+    case 45: gather_ta1<2>(p, img, scale, s); break;
+    case 46: gather_ta1<4>(p, img, scale, s); break;
+    case 47: gather_ta1<8>(p, img, scale, s); break;
+    default: gather_ta1<16>(p, img, scale, s); break;
     }
     MBREC_HIP(hipGetLastError());
     return MBREC_OK;
