@@ -50,6 +50,9 @@ def parse():
                     help="time the secondary paths (dmas_cf, bp_kernel, backproject_waveeq, detector "
                          "upsampling, TV and L1 solves) on the device and their oracle on the host; "
                          "prints one JSON line")
+    ap.add_argument("--mixed", action="store_true",
+                    help="the mixed batch: 224 sinograms (28 wavelengths x 8 frames) over 200^2 / 300^2 / "
+                         "400^2 grids with crop lengths U{1500..2030}, one graph; prints one JSON line")
     ap.add_argument("--traffic-from", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
                     help="no GPU: turn two rocprofv3 --pmc counter_collection.csv files of a "
                          "--traffic-calibration run into profiles/traffic.json and exit")
@@ -225,8 +228,87 @@ def secondary(a):
     print(json.dumps({"secondary": res}), flush=True)
 
 
+def mixed(a):
+    """224 independent NNLS-L2 solves (50 outer iterations) in one graph launch: grids drawn
+    from {200^2, 300^2, 400^2} at 100 um, each sinogram a window of U{1500..2030} samples
+    of its grid's 2030-sample axis, per-sinogram lambdaL2 = 1e-2 ||A'y||, seeded phantoms
+    scaled by a smooth per-wavelength spectrum, 1 % noise.  Also times each grid's group
+    alone (one graph each, run one after the other) for comparison."""
+    import numpy as np
+    import torch
+
+    from mbrec.batch import BatchSolver
+    from mbrec.discretization import Discretization
+    from mbrec.model import MSOTmodel
+    from mbrec.probe import Probe
+    from mbrec.warmstart import phantom_batch
+
+    g = np.random.default_rng(2024)
+    n_wl, n_fr = 28, 8
+    n = n_wl * n_fr
+    grids = g.choice([200, 300, 400], size=n)
+    counts = g.integers(1500, 2031, size=n)
+    firsts = np.array([g.integers(0, 2030 - c + 1) for c in counts])
+    spectrum = 0.6 + 0.4 * np.sin(np.linspace(0, np.pi, n_wl)) ** 2
+    h = 1e-4
+    groups, tensors = [], []
+    t0 = time.time()
+    for N in (200, 300, 400):
+        idx = np.nonzero(grids == N)[0]
+        half = (N - 1) / 2 * h
+        disc = Discretization([-half, half], N)
+        model = MSOTmodel(Probe("arc256_r40_174", True, False), disc, 1500.0, False, True, 0, 1e5, 12e6, None,
+                          sigma_iterations=40)
+        model.plan.autotune(len(idx), 3)
+        ph = phantom_batch(disc, len(idx), 7000 + N) * spectrum[idx // n_fr][:, None, None]
+        x = torch.tensor(ph, dtype=torch.float32).cuda()
+        y = model.applyForward(x)
+        noise = torch.randn(y.shape, generator=torch.Generator(device="cuda").manual_seed(N), device="cuda")
+        y = (y + 0.01 * y.abs().amax(dim=(1, 2), keepdim=True) * noise).contiguous()
+        lam = (1e-2 * torch.linalg.vector_norm(model.applyTranspose(y).reshape(len(idx), -1).double(), dim=1)).cpu()
+        groups.append((model, [(int(firsts[i]), int(counts[i])) for i in idx]))
+        tensors.append((y, lam.numpy()))
+    log(f"mixed batch set-up {time.time() - t0:.1f} s: " +
+        ", ".join(f"{N}^2 x {int(np.sum(grids == N))}" for N in (200, 300, 400)))
+    lam_all = np.concatenate([t[1] for t in tensors])
+
+    def timed(bs, steps):
+        bs.run(graph=True)
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(steps):
+            bs.run(graph=True)
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / steps
+
+    bs = BatchSolver(groups, a.iters, lam_all)
+    for k, (y, _) in enumerate(tensors):
+        bs.data[k].copy_(y)
+    ms = timed(bs, a.steps)
+    bs.close()
+    alone = []
+    for (model, w), (y, lam) in zip(groups, tensors):
+        b1 = BatchSolver([(model, w)], a.iters, lam)
+        b1.data[0].copy_(y)
+        alone.append(timed(b1, a.steps))
+        b1.close()
+    out = {"mixed_batch": {"sinograms": n, "grids": {f"{N}^2": int(np.sum(grids == N)) for N in (200, 300, 400)},
+                           "crop_lengths": [int(counts.min()), int(counts.max())], "outer_iterations": a.iters,
+                           "ms_per_batch": round(ms, 2), "recon_per_s": round(n / (ms / 1e3), 2),
+                           "ms_groups_one_after_another": round(sum(alone), 2),
+                           "ms_per_group_alone": [round(v, 2) for v in alone]}}
+    log(f"mixed batch: {ms:.1f} ms per batch of {n} ({n / (ms / 1e3):.1f} recon/s); groups one after another: "
+        f"{sum(alone):.1f} ms")
+    print(json.dumps(out), flush=True)
+
+
 def main():
     a = parse()
+    if a.mixed:
+        mixed(a)
+        return
     if a.traffic_from:
         traffic_from(*a.traffic_from, N=a.grid, B=a.batch)
         return
