@@ -53,6 +53,9 @@ def parse():
     ap.add_argument("--mixed", action="store_true",
                     help="the mixed batch: 224 sinograms (28 wavelengths x 8 frames) over 200^2 / 300^2 / "
                          "400^2 grids with crop lengths U{1500..2030}, one graph; prints one JSON line")
+    ap.add_argument("--tv600", action="store_true",
+                    help="rec_nn_with_TV_reg at 600^2 / 60 um with SIR: 32 vessel-tree sinograms of 256 "
+                         "detectors upsampled to 512 by spline; prints one JSON line")
     ap.add_argument("--traffic-from", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
                     help="no GPU: turn two rocprofv3 --pmc counter_collection.csv files of a "
                          "--traffic-calibration run into profiles/traffic.json and exit")
@@ -228,6 +231,58 @@ def secondary(a):
     print(json.dumps({"secondary": res}), flush=True)
 
 
+def tv600(a):
+    """The TV configuration: 600^2 grid at 60 um, SIR on, 512 detectors (the 256-detector
+    arc upsampled by the not-a-knot spline across detectors), a batch of 32 seeded vessel
+    trees, 1 % noise; rec_nn_with_TV_reg (inner SpaRSA 2 steps, TV prox by split Bregman)
+    for 5 outer iterations after a 1-iteration warm-up.  The 512-detector data are the
+    model's sinograms at the even detectors (256 of them), upsampled back to 512."""
+    import numpy as np
+    import torch
+
+    from mbrec.discretization import Discretization
+    from mbrec.model import MSOTmodel
+    from mbrec.phantoms import vessel_tree
+    from mbrec.preprocess import upsample_detectors_in_sinogram
+    from mbrec.probe import Probe
+    from mbrec.solvers import rec_nn_with_TV_reg
+
+    N, h, B, iters = 600, 60e-6, 32, 5
+    half = (N - 1) / 2 * h
+    disc = Discretization([-half, half], N)
+    t0 = time.time()
+    model = MSOTmodel(Probe("arc256_r40_174", True, False, num_detectors=512), disc, 1500.0, True, True, 0, 1e5,
+                      12e6, None, sigma_iterations=20, progress=log)
+    D, nT = model.shape_sinogram
+    t_setup = time.time() - t0
+    tune = model.plan.autotune(B, 3)
+    log(f"tv600: set-up {t_setup:.1f} s ({D} detectors, {nT} samples, tables {model.plan.device_bytes / 1e9:.2f} GB), "
+        f"launch choice {tune}")
+    ph = np.stack([vessel_tree(disc, seed=100 + b) for b in range(B)]).astype(np.float32)
+    y = model.applyForward(torch.tensor(ph).cuda())
+    y = y + 0.01 * y.abs().amax(dim=(1, 2), keepdim=True) * torch.randn(y.shape, device="cuda")
+    y256 = y[:, 0::2, :].double().cpu().numpy()
+    torch.cuda.synchronize()
+    tu = time.perf_counter()
+    y512 = upsample_detectors_in_sinogram(y256, 2)
+    t_up = time.perf_counter() - tu
+    yd = torch.tensor(y512, dtype=torch.float32).cuda()
+    tau = 1e-3
+    rec_nn_with_TV_reg(model, yd, 1, tau, coupled=False)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    rec, e_img, e_reg, *_ = rec_nn_with_TV_reg(model, yd, iters, tau, coupled=False)
+    torch.cuda.synchronize()
+    t_solve = time.perf_counter() - t1
+    out = {"tv600": {"grid": N, "pixel_um": 60, "detectors": D, "samples": nT, "sir": True, "batch": B,
+                     "outer_iterations": iters, "s_per_solve_batch": round(t_solve, 3),
+                     "ms_per_outer_iteration": round(t_solve / iters * 1e3, 1),
+                     "recon_iterations_per_s": round(B * iters / t_solve, 2),
+                     "upsample_host_to_host_s": round(t_up, 3), "setup_s": round(t_setup, 1)}}
+    log(f"tv600: {t_solve / iters * 1e3:.1f} ms per outer iteration for {B} sinograms")
+    print(json.dumps(out), flush=True)
+
+
 def mixed(a):
     """224 independent NNLS-L2 solves (50 outer iterations) in one graph launch: grids drawn
     from {200^2, 300^2, 400^2} at 100 um, each sinogram a window of U{1500..2030} samples
@@ -308,6 +363,9 @@ def main():
     a = parse()
     if a.mixed:
         mixed(a)
+        return
+    if a.tv600:
+        tv600(a)
         return
     if a.traffic_from:
         traffic_from(*a.traffic_from, N=a.grid, B=a.batch)

@@ -10,6 +10,7 @@ reversed axes: images (W, Nx, Nz), sinograms (W, D, nT), travel times (D, Nx, Nz
 from __future__ import annotations
 
 import ctypes as C
+import time
 from types import SimpleNamespace
 
 import numpy as np
@@ -147,7 +148,10 @@ class MSOTmodel:
                  include_sir: bool, use_single_speed_of_sound: bool = True,
                  num_cropped_samples_at_sinogram_start: int = 0, filt_cutoff_min: float = 1e5,
                  filt_cutoff_max: float = 12e6, model_normalization_factor=None,
-                 sigma_iterations: int = 60):
+                 sigma_iterations: int = 60, progress=None):
+        """progress: optional callable receiving a line after each set-up stage."""
+        say = progress or (lambda msg: None)
+        t_start = time.time()
         if not use_single_speed_of_sound:
             raise NotImplementedError("dual speed of sound travel times are not provided")
         self.Probe = probe
@@ -167,6 +171,7 @@ class MSOTmodel:
         R = (Z[1, 0] - Z[0, 0]) / (c / fs)                      # MSOTmodel.m:36-38
         self.Data = SimpleNamespace(numTimeSamplesPerTransducer=nT, absoluteTravelTimeIndices=tt,
                                     circularInterpolationRadiusInSamples=R)
+        say(f"model: travel times ({time.time() - t_start:.1f} s)")
         if self.includeSir:
             ir = impulse_response_with_sir(probe.detector.EIR, fs, filt_cutoff_min, filt_cutoff_max)
             X = discretization.region["xPositionsFov"].T.ravel()
@@ -181,6 +186,7 @@ class MSOTmodel:
             self.Data.SPMRIndices = sidx
             self.Data.SPMRGrid = info
             sir = (sidx, table, se)
+            say(f"model: SIR tables {table.shape} ({time.time() - t_start:.1f} s)")
         else:
             ir = impulse_response_without_sir(probe.detector.EIR, fs, c, discretization.pixelModel,
                                               filt_cutoff_min, filt_cutoff_max)
@@ -190,8 +196,10 @@ class MSOTmodel:
         self.plan = DevicePlan(discretization.Nz, discretization.Nx, D, nT, R,
                                tt.reshape(D, -1), sir=sir, ir=ir,
                                prefilter=(discretization.kz, discretization.kx), sigma=1.0)
+        say(f"model: device plan ({time.time() - t_start:.1f} s)")
         if model_normalization_factor is None:
             model_normalization_factor = self.estimate_sigma(sigma_iterations)
+            say(f"model: sigma_max ({time.time() - t_start:.1f} s)")
         self.normalizationFactor = float(model_normalization_factor)
         self.plan.set_sigma(self.normalizationFactor)
         self.Funcs = SimpleNamespace(applyForward=self.applyForward, applyTranspose=self.applyTranspose)
