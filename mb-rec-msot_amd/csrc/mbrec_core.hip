@@ -200,7 +200,8 @@ static int build_sir_pull(Plan *p, const std::vector<int32_t> &slot_pix, const s
     const int lo = -std::max(maxlen - 1, 0), n_nf = span - lo + 1;
     // weights table: window k at k * stride + kSirWinPad, kSirWinPad zeros before it and at least as many after
     const int stride = maxlen + 2 * kSirWinPad;
-    if ((size_t)p->Nsir_build * stride >= ((size_t)1 << 26) || maxlen >= 64 || p->P >= (1 << 28))
+    if ((size_t)p->Nsir_build * stride >= ((size_t)1 << 31) || maxlen >= (1 << 15) || p->P >= (1 << 28) ||
+        span >= (1 << 15))
         return MBREC_OK;   // the packed record does not fit: the SIR forward keeps the bucketed form only
     std::vector<float> sw((size_t)p->Nsir_build * stride, 0.0f);
     for (int k = 0; k < p->Nsir_build; ++k)
@@ -248,7 +249,7 @@ static int build_sir_pull(Plan *p, const std::vector<int32_t> &slot_pix, const s
             std::memcpy(&ub, &u, sizeof(ub));
             const int base = k * stride + kSirWinPad;
             rec[fill[((size_t)d * 3 + m) * (n_nf + 1) + (nfb[at] - lo)]++] =
-                make_int4(slot_pix[sl] | (m << 28), nfb[at], ub, base | (len << 26));
+                make_int4(slot_pix[sl] | (m << 28), (nfb[at] & 0xffff) | (len << 16), ub, base);
         }
     if (p->d_srec) { (void)hipFree(p->d_srec); (void)hipFree(p->d_srec_off); (void)hipFree(p->d_sirw); }
     p->table_bytes -= p->srec_bytes;

@@ -81,8 +81,9 @@ struct Plan {
     int32_t *d_rec_off = nullptr;   // [D][nbk + 1] record offsets / [D][NB + 1]
     int32_t *d_rec_k = nullptr;  // [records] 0-based SIR index (SIR plans)
     // SIR forward in pull form: one record per (pixel, detector) pair with a tap in the
-    // bin range, int4 (pixel | phase << 28, first affected sample nf = n + s0 - Ls/2 + K,
-    // u bits, d_sirw offset of the first tap | tap count << 26), sorted by (detector, phase, nf);
+    // bin range, int4 (pixel | phase << 28, first affected sample nf = n + s0 - Ls/2 + K as
+    // 16-bit two's complement | tap count << 16, u bits, d_sirw offset of the first tap),
+    // sorted by (detector, phase, nf);
     // CSR over nf in [sp_lo, sp_lo + sp_n) per (detector, phase): d_srec_off[(3 d + m) (sp_n + 1) + i]
     int4 *d_srec = nullptr;
     float *d_sirw = nullptr;     // [Nsir][maxlen + 2 kSirWinPad]: each SIR window between zero margins

@@ -1637,9 +1637,9 @@ __global__ __launch_bounds__(256) void k_fwd_sir_pull1(const int4 *__restrict__ 
 #pragma unroll
             for (int q = 0; q < U; ++q) {
                 c[q] = img[e[q].x & 0x0fffffff];
-                const int t = n - e[q].y, len = (int)((unsigned)e[q].w >> 26);
+                const int t = n - ((e[q].y << 16) >> 16), len = (int)((unsigned)e[q].y >> 16);
                 const bool ok = live && r0 + 4 * q < re && t >= 0 && t < len;
-                h[q] = ok ? sir[(e[q].w & 0x03ffffff) + t] : 0.0f;
+                h[q] = ok ? sir[e[q].w + t] : 0.0f;
             }
 #pragma unroll
             for (int q = 0; q < U; ++q) {
@@ -1697,8 +1697,8 @@ __device__ __forceinline__ void sir_pull_phase(float (&acc)[3 * TS + 1], const i
         for (int q = 0; q < U; ++q) {
             if (r0 + q >= re) break;
             // weight j is tap a - 1 + j - nf of the window: offset a - 1 - nf in [-TS, maxlen - 1]
-            const int sh = min(max(a - 1 - e[q].y, -TS), maxlen - 1);
-            const float *w = sirw + (e[q].w & 0x03ffffff) + sh;
+            const int sh = min(max(a - 1 - ((e[q].y << 16) >> 16), -TS), maxlen - 1);
+            const float *w = sirw + (size_t)e[q].w + sh;
             float h[TS + 1];
 #pragma unroll
             for (int j = 0; j <= TS; ++j) h[j] = w[j];
