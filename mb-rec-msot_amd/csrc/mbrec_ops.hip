@@ -2349,8 +2349,10 @@ __global__ __launch_bounds__(1024) void k_gather_tas(const int4 *__restrict__ st
     __syncthreads();
     const __amdgpu_buffer_rsrc_t rs =
         __builtin_amdgcn_make_buffer_rsrc(const_cast<float *>(bins), (short)0, bins_bytes, kBufferWord3);
-    const __amdgpu_buffer_rsrc_t rt =
-        __builtin_amdgcn_make_buffer_rsrc(const_cast<int4 *>(stab), (short)0, stab_bytes, kBufferWord3);
+    // the tile's own table rows: D x 256 entries from its base (32-bit offsets whatever
+    // the table's size)
+    const __amdgpu_buffer_rsrc_t rt = __builtin_amdgcn_make_buffer_rsrc(
+        const_cast<int4 *>(stab + (size_t)tile * D * 256), (short)0, stab_bytes, kBufferWord3);
     const int rowb = 4 * Bp;
     // load c of wave w moves rows 4 (16 c + w) .. + 3: lane l takes row + l / 16, columns b0 + 4 (l % 16) ..
     const int vo = (lane >> 4) * rowb + 4 * (b0 + 4 * (lane & 15));
@@ -2366,7 +2368,7 @@ __global__ __launch_bounds__(1024) void k_gather_tas(const int4 *__restrict__ st
         // number of loads, so one wait count fits all
         __builtin_amdgcn_raw_ptr_buffer_load_lds(
             rt, (__attribute__((address_space(3))) void *)(bb + ZR * 64 + 64 * wv), 4, 4 * lane,
-            __builtin_amdgcn_readfirstlane((int)((((size_t)tile * D + d) * 256 + 16 * wv) * sizeof(int4))), 0, 0);
+            __builtin_amdgcn_readfirstlane((d * 256 + 16 * wv) * (int)sizeof(int4)), 0, 0);
     };
     float acc[16], part[16];
 #pragma unroll
@@ -3119,7 +3121,7 @@ static void gather_tas_n(Plan *p, float *img, int B, int Bp, float scale, hipStr
 {
     const int nbc = (Bp + 63) / 64;
     const int bytes = (int)(sizeof(float) * ((size_t)p->D * p->geom.NB + 2) * Bp);
-    const int tbytes = (int)(sizeof(int4) * (size_t)(p->Pd / 256) * p->D * 256);
+    const int tbytes = (int)(sizeof(int4) * (size_t)p->D * 256);   // one tile's rows
     hipLaunchKernelGGL((k_gather_tas<G, NCH>), dim3(p->Pd / 256, G * nbc), dim3(1024), 0, s, p->d_stab, p->d_stwin,
                        p->d_pix, p->d_bins, p->d_sirw, G > 1 ? p->d_gpart : img, p->D, p->geom.NB, p->P, B, Bp,
                        scale, bytes, tbytes);
@@ -3224,7 +3226,6 @@ static bool gather_variant_ok(const Plan *p, int v, int Bp)
     if (g.u == -11) {   // SIR tile gather: tables built, window within 4 rounds of 64 rows, 32-bit offsets
         if (!p->use_sir || !p->d_stab || p->stile_zr < 64 || p->stile_zr > 256 || Bp < 16) return false;
         if (sizeof(float) * ((size_t)p->D * p->geom.NB + 2) * Bp >= ((size_t)1 << 31)) return false;
-        if (sizeof(int4) * (size_t)(p->Pd / 256) * p->D * 256 >= ((size_t)1 << 31)) return false;
         if (p->D > kTaMaxD) return false;
         return g.lb == 1 || p->d_gpart;
     }
