@@ -122,8 +122,10 @@ int mbrec_plan_autotune(mbrec_plan plan, int B, int reps, mbrec_stream s, int *c
 /* Set the launch choice for batches of B by hand: gather variant in
  * [0, mbrec_num_gather_variants()) or -1 (automatic); forward choice 0:
  * automatic.  Plans with SIR: 1000 the pull form (one sinogram: lane groups of
- * samples; batches: 24 samples x 2 records in flight per wave), 1001 / 1002 / 1003
- * batch pull forms with (16, 4), (16, 2), (32, 1); bucketed scatter: > 0 the
+ * samples, the default there; batches: 24 samples x 2 records in flight per wave), 1001 / 1002 / 1003
+ * batch pull forms with (16, 4), (16, 2), (32, 1), 1004 the band form for batches
+ * (each record once, overlapping rows summed in a second pass; the default for batches);
+ * bucketed scatter: > 0 the
  * record-lane kernel with that LDS batch chunk (a multiple of 4), < 0 the
  * batch-lane kernel with lane groups of -value (16, 32 or 64) sinograms.  Plans
  * without SIR (pull form): 1, 4, 8, 16 bins per thread pass of that many

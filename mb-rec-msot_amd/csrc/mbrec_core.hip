@@ -52,10 +52,11 @@ static int upload(T *dst, const T *src, size_t n)
 
 static void free_work(Plan *p)
 {
-    float *bufs[] = {p->d_bins, p->d_hpart, p->d_imgI, p->d_img_tmp, p->d_sino_tmp, p->d_gpart};
+    float *bufs[] = {p->d_bins, p->d_hpart, p->d_imgI, p->d_img_tmp, p->d_sino_tmp, p->d_gpart, p->d_sband};
     for (float *q : bufs)
         if (q) (void)hipFree(q);
-    p->d_bins = p->d_hpart = p->d_imgI = p->d_img_tmp = p->d_sino_tmp = p->d_gpart = nullptr;
+    p->d_bins = p->d_hpart = p->d_imgI = p->d_img_tmp = p->d_sino_tmp = p->d_gpart = p->d_sband = nullptr;
+    p->sband_nwave = p->sband_nfb = 0;
     p->bcap = 0;
     p->work_bytes = 0;
 }
@@ -86,6 +87,17 @@ int ensure_work(Plan *p, int B)
         (rc = dalloc(&p->d_gpart, (size_t)std::max(8 * B, 16) * p->P, &acc))) {
         free_work(p);
         return rc;
+    }
+    // band-form SIR forward rows (batches): waves of nfb = TS + 1 - maxlen first samples each
+    if (B > 1 && p->use_sir && p->d_srec && p->sir_maxlen <= kSirBandTS) {
+        const int nfb = kSirBandTS + 1 - std::max(p->sir_maxlen, 1);
+        const int nwave = (p->sp_n + nfb - 1) / nfb;
+        if ((rc = dalloc(&p->d_sband, (size_t)p->D * nwave * (3 * kSirBandTS + 1) * Bp, &acc))) {
+            free_work(p);
+            return rc;
+        }
+        p->sband_nwave = nwave;
+        p->sband_nfb = nfb;
     }
     // the two zero rows after the last detector's bins (pairs outside the bin range read them)
     MBREC_HIP(hipMemset(p->d_bins + (size_t)p->D * p->geom.NB * Bp, 0, sizeof(float) * 2 * Bp));

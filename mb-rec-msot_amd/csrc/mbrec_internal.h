@@ -38,6 +38,7 @@ struct HatTaps {
 constexpr int kTileRowsMax = 256;   // tile gathers: longest bin window they stage (rows)
 constexpr int kRecPad = 512;   // zero records after the pull table (runs of up to 512 read past the end)
 constexpr int kSirWinPad = 32;   // zero weights on each side of a window in the pull form's SIR table
+constexpr int kSirBandTS = 32;   // samples per wave of the band-form SIR forward
 
 struct BinGeom {
     float phi1, phi2;            // sorted interior breakpoint phases in [0,1)
@@ -125,6 +126,8 @@ struct Plan {
     float *d_img_tmp = nullptr;  // 2 x [B][P]       prefilter temporaries
     float *d_sino_tmp = nullptr; // [B][D][nT]       samples before/after the IR filter
     float *d_gpart = nullptr;    // [max(8 B, 16)][P]  partial images of the detector-grouped gathers
+    float *d_sband = nullptr;    // [D][band waves][3 kSirBandTS + 1][Bp]  band-form SIR forward rows (B > 1)
+    int sband_nwave = 0, sband_nfb = 0;
     size_t table_bytes = 0;
     size_t work_bytes = 0;
     int pins = 0;                // batches whose captured graphs use this plan's buffers

@@ -1771,6 +1771,97 @@ __global__ __launch_bounds__(256) void k_fwd_sir_pull_bl(const int4 *__restrict_
     }
 }
 
+// SIR forward for batches, band form, pass 1 (lane = sinogram): wave w takes the records
+// whose first affected sample nf lies in its band [a, a + nfb), a = lo + w nfb, so every
+// record is visited once; its taps land on samples a .. a + TS - 1 (TS >= nfb - 1 + maxlen)
+// and the wave keeps their 3 TS + 1 bins in registers.  A record's TS weights from sample
+// a on come from the zero-margin weights table (offset nf - a before its first tap).  The
+// bin of sample -1's phase-2 u-part and the phase-0 part at sample span are dropped
+// where they fall in the wave's range (the reference's range check).  The wave's rows go
+// to its slot of the band scratch; pass 2 adds the overlapping slots.
+template <int TS>
+__device__ __forceinline__ void sir_band_phase(float (&acc)[3 * TS + 1], const int4 *__restrict__ rec, int rs,
+                                               int re, const float *__restrict__ imgI,
+                                               const float *__restrict__ sirw, int a, int Bp, int bo, int m)
+{
+    for (int r = rs; r < re; ++r) {
+        const int4 e = rec[r];
+        const float c = imgI[(size_t)(e.x & 0x0fffffff) * Bp + bo];
+        const int t0 = ((e.y << 16) >> 16) - a;   // in [0, nfb)
+        const float *w = sirw + (size_t)e.w - t0;   // w[t]: weight of the tap on sample a + t
+        float h[TS];
+#pragma unroll
+        for (int t = 0; t < TS; ++t) h[t] = w[t];
+        const float u = __int_as_float(e.z);
+        const float c0 = c * (1.0f - u), c1 = c * u;
+#pragma unroll
+        for (int t = 0; t < TS; ++t) {
+            acc[3 * t + m] += c0 * h[t];
+            acc[3 * t + m + 1] += c1 * h[t];
+        }
+    }
+}
+
+template <int TS>
+__global__ __launch_bounds__(256) void k_fwd_sir_band(const int4 *__restrict__ rec, const int32_t *__restrict__ off,
+                                                      const float *__restrict__ imgI, const float *__restrict__ sirw,
+                                                      float *__restrict__ scratch, int span, int lo, int n_nf,
+                                                      int nfb, int nwave, int Bp)
+{
+    const int lane = threadIdx.x & 63;
+    const int w = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + (threadIdx.x >> 6));
+    if (w >= nwave) return;   // no barriers in this kernel
+    const int d = blockIdx.y;
+    const int b = blockIdx.z * 64 + lane;
+    const int bo = min(b, Bp - 1);
+    const int a = lo + w * nfb;
+    const int ia = w * nfb, ib = min(ia + nfb, n_nf);
+    float acc[3 * TS + 1];
+#pragma unroll
+    for (int k = 0; k < 3 * TS + 1; ++k) acc[k] = 0.0f;
+    const int32_t *o = off + (size_t)d * 3 * (n_nf + 1);
+    // phase 2 first: the u-part of sample -1 (row 0) is dropped before other phases add there
+    sir_band_phase<TS>(acc, rec, o[2 * (n_nf + 1) + ia], o[2 * (n_nf + 1) + ib], imgI, sirw, a, Bp, bo, 2);
+#pragma unroll
+    for (int t = 0; t < TS; ++t)
+        if (a + t == -1) acc[3 * t + 3] = 0.0f;
+    // phase 0: the part at sample span (row 3 span) is dropped, the u-part of span - 1 kept
+    float keep = 0.0f;
+#pragma unroll
+    for (int t = 0; t < TS; ++t)
+        if (a + t == span) keep = acc[3 * t];
+    sir_band_phase<TS>(acc, rec, o[ia], o[ib], imgI, sirw, a, Bp, bo, 0);
+#pragma unroll
+    for (int t = 0; t < TS; ++t)
+        if (a + t == span) acc[3 * t] = keep;
+    sir_band_phase<TS>(acc, rec, o[(n_nf + 1) + ia], o[(n_nf + 1) + ib], imgI, sirw, a, Bp, bo, 1);
+    if (b >= Bp) return;
+    float *dst = scratch + (((size_t)d * nwave + w) * (3 * TS + 1)) * Bp + b;
+#pragma unroll
+    for (int k = 0; k < 3 * TS + 1; ++k) dst[(size_t)k * Bp] = acc[k];
+}
+
+// pass 2: bins[d][j][b] = sum over the waves whose rows [3 a_w, 3 (a_w + TS)] hold j, in
+// wave order (rows of samples outside [0, span] are never asked for)
+template <int TS>
+__global__ __launch_bounds__(256) void k_fwd_sir_band_sum(const float *__restrict__ scratch,
+                                                          float *__restrict__ bins, int NB, int lo, int nfb,
+                                                          int nwave, int Bp)
+{
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;   // (j, b) of detector blockIdx.y
+    if (i >= (size_t)NB * Bp) return;
+    const int d = blockIdx.y;
+    const int j = (int)(i / Bp), b = (int)(i - (size_t)j * Bp);
+    // wave w covers rows 3 (lo + w nfb) .. 3 (lo + w nfb + TS)
+    const int wlo = max(0, (j - 3 * (lo + TS) + 3 * nfb - 1) / (3 * nfb)), whi = min(nwave - 1, (j - 3 * lo) / (3 * nfb));
+    float acc = 0.0f;
+    for (int w = wlo; w <= whi; ++w) {
+        const int r = j - 3 * (lo + w * nfb);
+        if (r >= 0 && r <= 3 * TS) acc += scratch[(((size_t)d * nwave + w) * (3 * TS + 1) + r) * Bp + b];
+    }
+    bins[((size_t)d * NB + j) * Bp + b] = acc;
+}
+
 // ---------------------------------------------------------------------------
 // tile gather (batch-lane, Bp >= 16): one workgroup per 16 x 16 pixel tile and
 // chunk of LB sinograms.  For every detector the tile's bin window (all bins its
@@ -2667,23 +2758,28 @@ static void scatter_bl(Plan *p, const float *imgI, int B, int Bp, hipStream_t s)
 // scatter launch choice: > 0 record-lane kernel with that LDS batch chunk, < 0
 // batch-lane kernel with lane groups of -value sinograms
 // SIR forward in pull form (records of build_sir_pull): scatter choices of their own, the
-// bins written directly (no bucket windows).  kSirPull: the default (for one sinogram
-// the lane-group kernel, else the batch-lane kernel with 24 samples and 2 records in
-// flight per wave); kSirPull + 1 .. 3: batch-lane kernels with (16 samples, 4 records),
-// (16, 2), (32, 1)
-constexpr int kSirPull = 1000, kSirPullVariants = 4;
+// bins written directly (no bucket windows).  kSirPull: for one sinogram the lane-group
+// kernel (the default there), else the batch-lane kernel with 24 samples and 2 records in
+// flight per wave; kSirPull + 1 .. 3: batch-lane kernels with (16 samples, 4 records),
+// (16, 2), (32, 1); kSirBand (batches; the default there): every record visited once by
+// the wave of its first-sample band, the overlapping wave rows summed in a second pass
+constexpr int kSirPull = 1000, kSirPullVariants = 5, kSirBand = kSirPull + 4;
 
 static bool is_sir_pull(int c) { return c >= kSirPull && c < kSirPull + kSirPullVariants; }
 
 static bool sir_pull_ok(const Plan *p, int Bp, int c = kSirPull)
 {
+    if (c == kSirBand)
+        return p->use_sir && p->d_srec && p->d_sirw && Bp > 1 && p->d_sband && p->sband_nfb >= 1 &&
+               p->bcap >= 1 && batch_stride(p->bcap) >= Bp;
     return p->use_sir && p->d_srec && p->d_sirw && is_sir_pull(c) && (c == kSirPull || Bp > 1);
 }
 
 static int scatter_choice(const Plan *p, int Bp)
 {
     if (p->tune.Bp == Bp && sir_pull_ok(p, Bp, p->tune.scatter_bc)) return p->tune.scatter_bc;
-    if (!(p->tune.Bp == Bp && p->tune.scatter_bc != 0) && sir_pull_ok(p, Bp)) return kSirPull;
+    if (!(p->tune.Bp == Bp && p->tune.scatter_bc != 0) && sir_pull_ok(p, Bp))
+        return sir_pull_ok(p, Bp, kSirBand) ? kSirBand : kSirPull;
     if (p->tune.Bp == Bp && p->tune.scatter_bc != 0 && !is_sir_pull(p->tune.scatter_bc)) {
         const int c = p->tune.scatter_bc;
         if (c > 0) return std::min(c, scatter_chunk_max(p, Bp));
@@ -2845,6 +2941,16 @@ static int launch_scatter(Plan *p, const float *imgI, int B, int Bp, hipStream_t
             const int nblk = (span + 1 + 59) / 60;   // 4 waves x 15 samples per workgroup
             hipLaunchKernelGGL(k_fwd_sir_pull1, dim3(nblk, p->D), dim3(256), 0, s, p->d_srec, p->d_srec_off, imgI,
                                p->d_sirw, p->d_bins, p->geom.NB, span, p->sp_lo, p->sp_n, p->sir_maxlen);
+        } else if (sc == kSirBand) {
+            constexpr int TS = kSirBandTS;
+            const int span = p->geom.nT + 2 * p->geom.K;
+            hipLaunchKernelGGL((k_fwd_sir_band<TS>), dim3((p->sband_nwave + 3) / 4, p->D, (Bp + 63) / 64), dim3(256),
+                               0, s, p->d_srec, p->d_srec_off, imgI, p->d_sirw, p->d_sband, span, p->sp_lo, p->sp_n,
+                               p->sband_nfb, p->sband_nwave, Bp);
+            hipLaunchKernelGGL((k_fwd_sir_band_sum<TS>),
+                               dim3((unsigned)(((size_t)p->geom.NB * Bp + 255) / 256), p->D), dim3(256), 0, s,
+                               (const float *)p->d_sband, p->d_bins, p->geom.NB, p->sp_lo, p->sband_nfb,
+                               p->sband_nwave, Bp);
         } else if (sc == kSirPull + 1) sir_pull_bl<16, 4>(p, imgI, Bp, s);
         else if (sc == kSirPull + 2) sir_pull_bl<16, 2>(p, imgI, Bp, s);
         else if (sc == kSirPull + 3) sir_pull_bl<32, 1>(p, imgI, Bp, s);
