@@ -130,8 +130,6 @@ int mbrec_plan_autotune(mbrec_plan plan, int B, int reps, mbrec_stream s, int *c
  * batch-lane kernel with lane groups of -value (16, 32 or 64) sinograms.  Plans
  * without SIR (pull form): 1, 4, 8, 16 bins per thread pass of that many
  * sinograms; 2, 3 record streaming for one sinogram (1 or 4 bins per lane);
- * 5 LDS histogram for one sinogram (one workgroup per detector, LDS float atomics:
- * the default there; the summation order of a bin varies from run to run);
  * -(LB + 100 mode) batch-lane with lane groups LB and mode 0 (plain order), 1
  * (XCD-aware order), 2 (XCD-aware, 16 pairs in flight); -364 / -464 / -564 /
  * -664 / -764 / -864 wave-uniform, -964 / -1064 / -1164 wave-uniform with raw

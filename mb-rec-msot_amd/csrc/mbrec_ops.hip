@@ -1602,50 +1602,6 @@ __global__ __launch_bounds__(256) void k_fwd_pull_b1(const int2 *__restrict__ re
     }
 }
 
-// forward for a single sinogram (Bp == 1), LDS histogram form: WG = one detector, 1024
-// threads.  The detector's NB bins live in LDS; the threads stream the detector's travel
-// times tq[d][slot] in slot order (coalesced) with the slot's pixel value, decode them and
-// add the two hat terms (1 - u) c to bin 3 n + m and u c to bin 3 n + m + 1 with LDS float
-// atomics (the reference's scatter, forwardMex.cpp; the summation order of a bin varies
-// from run to run).  The row is then written whole, as the pull writes it.
-constexpr int kHistNT = 1024, kHistU = 8;
-
-__global__ __launch_bounds__(kHistNT) void k_fwd_hist1(const int32_t *__restrict__ tq, const int32_t *__restrict__ pix,
-                                                       const float *__restrict__ img, float *__restrict__ bins,
-                                                       BinGeom g, int Pd)
-{
-    extern __shared__ float hist[];
-    const int d = blockIdx.x, k = threadIdx.x, NB = g.NB, span = g.nT + 2 * g.K;
-    for (int i = k; i < NB; i += kHistNT) hist[i] = 0.0f;
-    __syncthreads();
-    const int32_t *t = tq + (size_t)d * Pd;
-    for (int s0 = 0; s0 < Pd; s0 += kHistU * kHistNT) {
-        int32_t q[kHistU];
-        int px[kHistU];
-        float c[kHistU];
-#pragma unroll
-        for (int i = 0; i < kHistU; ++i) {
-            const int sl = s0 + i * kHistNT + k;
-            q[i] = sl < Pd ? t[sl] : 0;
-            px[i] = sl < Pd ? pix[sl] : -1;
-        }
-#pragma unroll
-        for (int i = 0; i < kHistU; ++i) c[i] = px[i] >= 0 ? img[px[i]] : 0.0f;
-#pragma unroll
-        for (int i = 0; i < kHistU; ++i) {
-            const Decoded dc = decode_tau(q[i], g);
-            if ((unsigned)dc.nn < (unsigned)span && c[i] != 0.0f) {
-                const int r = 3 * dc.nn + dc.m;   // r + 1 <= 3 span = NB - 1
-                atomicAdd(&hist[r], (1.0f - dc.u) * c[i]);
-                atomicAdd(&hist[r + 1], dc.u * c[i]);
-            }
-        }
-    }
-    __syncthreads();
-    float *o = bins + (size_t)d * NB;
-    for (int i = k; i < NB; i += kHistNT) o[i] = hist[i];
-}
-
 // SIR forward for a single sinogram, pull form (records of build_sir_pull): a wave owns
 // samples a - 1 .. a + 14 of one detector in four lane groups of 16 (lane i of a group:
 // sample a - 1 + i).  Group g takes every fourth record of the records whose taps can
@@ -2833,25 +2789,11 @@ static int scatter_choice(const Plan *p, int Bp)
     return lb ? -lb : scatter_chunk_max(p, Bp);
 }
 
-// LDS histogram forward (one sinogram): the travel-time table and one detector's bins in LDS
-static bool hist1_ok(const Plan *p)
-{
-    return !p->use_sir && p->d_tq && p->d_pix && p->Pd % 256 == 0 && sizeof(float) * (size_t)p->geom.NB <= 160 * 1024;
-}
-
-static void fwd_hist1(Plan *p, const float *imgI, hipStream_t s)
-{
-    hipLaunchKernelGGL(k_fwd_hist1, dim3(p->D), dim3(kHistNT), sizeof(float) * p->geom.NB, s, p->d_tq, p->d_pix, imgI,
-                       p->d_bins, p->geom, p->Pd);
-}
-
-// pull forward choice: > 0 bins per thread pass BC (1, 4, 8, 16), for one sinogram also
-// 2, 3 (record stream, 1 or 4 bin groups per wave) and 5 (LDS histogram, the default
-// there); < 0 batch-lane, -(LB + 100 * mode) with lane groups LB (16, 32, 64) and mode 0:
-// plain order, 8 pairs in flight; 1: XCD-aware order, 8 in flight; 2: XCD-aware, 16 in flight
+// pull forward choice: > 0 bins per thread pass BC (1, 4, 8, 16); < 0 batch-lane,
+// -(LB + 100 * mode) with lane groups LB (16, 32, 64) and mode 0: plain order,
+// 8 pairs in flight; 1: XCD-aware order, 8 in flight; 2: XCD-aware, 16 in flight
 static bool pull_choice_ok(const Plan *p, int c, int Bp)
 {
-    if (c == 5) return Bp == 1 && hist1_ok(p);
     if (c > 0) return c == 1 || ((c == 2 || c == 3) && Bp == 1) || ((c == 4 || c == 8 || c == 16) && Bp % c == 0);
     if (c < 0) {
         const int lb = (-c) % 100, mode = (-c) / 100;
@@ -2866,9 +2808,8 @@ static bool pull_choice_ok(const Plan *p, int c, int Bp)
     return false;
 }
 
-static int default_pull_choice(const Plan *p, int Bp)
+static int default_pull_choice(int Bp)
 {
-    if (Bp == 1 && hist1_ok(p)) return 5;
     if (Bp >= 64) return -964;
     if (Bp >= 32) return -32;
     if (Bp >= 16) return -16;
@@ -2945,13 +2886,12 @@ static void fwd_pull_wt(Plan *p, const float *imgI, int Bp, hipStream_t s, int J
 
 static int launch_pull(Plan *p, const float *imgI, int Bp, hipStream_t s)
 {
-    int c = (p->tune.Bp == Bp && p->tune.scatter_bc != 0) ? p->tune.scatter_bc : default_pull_choice(p, Bp);
-    if (!pull_choice_ok(p, c, Bp)) c = default_pull_choice(p, Bp);
+    int c = (p->tune.Bp == Bp && p->tune.scatter_bc != 0) ? p->tune.scatter_bc : default_pull_choice(Bp);
+    if (!pull_choice_ok(p, c, Bp)) c = default_pull_choice(Bp);
     switch (c) {
     case 1: fwd_pull<1>(p, imgI, Bp, s); break;
     case 2: fwd_pull_b1<1>(p, imgI, s); break;
     case 3: fwd_pull_b1<4>(p, imgI, s); break;
-    case 5: fwd_hist1(p, imgI, s); break;
     case 4: fwd_pull<4>(p, imgI, Bp, s); break;
     case 8: fwd_pull<8>(p, imgI, Bp, s); break;
     case 16: fwd_pull<16>(p, imgI, Bp, s); break;
@@ -3711,7 +3651,7 @@ int autotune(Plan *p, int B, int reps, hipStream_t s, int *gather_out, int *scat
     // scatter: record-lane LDS batch chunks, then batch-lane lane groups
     std::vector<int> cand;
     if (p->pull) {
-        for (int c : {1, 2, 3, 5, 4, 8, 16, -16, -32, -64, -116, -132, -164, -216, -232, -264, -364, -464, -564, -664,
+        for (int c : {1, 2, 3, 4, 8, 16, -16, -32, -64, -116, -132, -164, -216, -232, -264, -364, -464, -564, -664,
                       -764, -864, -964, -1064, -1164})
             if (pull_choice_ok(p, c, Bp) && !(c == 1 && Bp > 16)) cand.push_back(c);
     } else {   // SIR: the pull forms first
