@@ -3557,6 +3557,19 @@ int op_transpose(Plan *p, const float *sino, float *img, int B, bool model, hipS
 // profiling: average duration of one pipeline stage, HIP events on stream s
 // (0 scatter, 1 bins->samples, 2 IR filter, 3 samples->bins, 4 gather)
 // ---------------------------------------------------------------------------
+// Inputs of the stage timer and the autotuner: a dense pattern in [0.25, 1.25) (timings
+// of zero inputs would not count any work a kernel skips for zero values)
+__global__ __launch_bounds__(256) void k_timing_fill(float *__restrict__ a, size_t n)
+{
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256)
+        a[i] = 0.25f + (float)((i * 2654435761u) & 0xffff) * (1.0f / 65536.0f);
+}
+
+static void timing_fill(float *a, size_t n, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_timing_fill, dim3(2048), dim3(256), 0, s, a, n);
+}
+
 int time_stage(Plan *p, int which, int B, int reps, hipStream_t s, float *ms)
 {
     int rc = ensure_work(p, B);
@@ -3564,9 +3577,10 @@ int time_stage(Plan *p, int which, int B, int reps, hipStream_t s, float *ms)
     const int Bp = batch_stride(B);
     float *sino = nullptr;
     MBREC_HIP(hipMalloc((void **)&sino, sizeof(float) * (size_t)B * p->D * p->nT));
-    MBREC_HIP(hipMemsetAsync(sino, 0, sizeof(float) * (size_t)B * p->D * p->nT, s));
-    MBREC_HIP(hipMemsetAsync(p->d_imgI, 0, sizeof(float) * (size_t)p->P * Bp, s));
-    MBREC_HIP(hipMemsetAsync(p->d_img_tmp, 0, sizeof(float) * (size_t)p->P * B, s));
+    timing_fill(sino, (size_t)B * p->D * p->nT, s);
+    timing_fill(p->d_imgI, (size_t)p->P * Bp, s);
+    timing_fill(p->d_bins, (size_t)p->D * p->geom.NB * Bp, s);
+    MBREC_HIP(hipGetLastError());
     auto run = [&]() -> int {
         switch (which) {
         case 0: return launch_scatter(p, p->d_imgI, B, Bp, s);
@@ -3605,8 +3619,9 @@ int autotune(Plan *p, int B, int reps, hipStream_t s, int *gather_out, int *scat
     int rc = ensure_work(p, B);
     if (rc) return rc;
     const int Bp = batch_stride(B);
-    MBREC_HIP(hipMemsetAsync(p->d_imgI, 0, sizeof(float) * (size_t)p->P * Bp, s));
-    MBREC_HIP(hipMemsetAsync(p->d_bins, 0, sizeof(float) * (size_t)p->D * p->geom.NB * Bp, s));
+    timing_fill(p->d_imgI, (size_t)p->P * Bp, s);
+    timing_fill(p->d_bins, (size_t)p->D * p->geom.NB * Bp, s);
+    MBREC_HIP(hipGetLastError());
     hipEvent_t a, b;
     MBREC_HIP(hipEventCreate(&a));
     MBREC_HIP(hipEventCreate(&b));
