@@ -52,10 +52,11 @@ static int upload(T *dst, const T *src, size_t n)
 
 static void free_work(Plan *p)
 {
-    float *bufs[] = {p->d_bins, p->d_hpart, p->d_imgI, p->d_img_tmp, p->d_sino_tmp, p->d_gpart, p->d_sband};
+    float *bufs[] = {p->d_bins, p->d_hpart, p->d_imgI, p->d_img_tmp, p->d_sino_tmp, p->d_gpart, p->d_sband, p->d_img_slot};
     for (float *q : bufs)
         if (q) (void)hipFree(q);
     p->d_bins = p->d_hpart = p->d_imgI = p->d_img_tmp = p->d_sino_tmp = p->d_gpart = p->d_sband = nullptr;
+    p->d_img_slot = nullptr;
     p->sband_nwave = p->sband_nfb = 0;
     p->bcap = 0;
     p->work_bytes = 0;
@@ -84,7 +85,8 @@ int ensure_work(Plan *p, int B)
         (rc = dalloc(&p->d_imgI, (size_t)p->P * Bp, &acc)) ||
         (rc = dalloc(&p->d_img_tmp, (size_t)2 * B * p->P, &acc)) ||
         (rc = dalloc(&p->d_sino_tmp, (size_t)B * p->D * p->nT, &acc)) ||
-        (rc = dalloc(&p->d_gpart, (size_t)std::max(8 * B, 16) * p->P, &acc))) {
+        (rc = dalloc(&p->d_gpart, (size_t)std::max(8 * B, 16) * p->P, &acc)) ||
+        (p->d_rec1 && (rc = dalloc(&p->d_img_slot, (size_t)p->Pd, &acc)))) {
         free_work(p);
         return rc;
     }
@@ -147,7 +149,7 @@ static int build_pull_records(Plan *p, const std::vector<int32_t> &slot_pix, con
         for (int j = 0; j < NB; ++j) o[j + 1] += o[j];
         total = o[NB];
     }
-    std::vector<int2> rec(std::max<size_t>(total, 1));
+    std::vector<int2> rec(std::max<size_t>(total, 1)), rec1(rec.size());
     std::vector<int32_t> fill(off);
     const float scale = std::ldexp(1.0f, -g.F);
     for (int d = 0; d < p->D; ++d)
@@ -164,14 +166,16 @@ static int build_pull_records(Plan *p, const std::vector<int32_t> &slot_pix, con
             const float u = (fr - ph) * iw;
             int32_t ub;
             std::memcpy(&ub, &u, sizeof(ub));
-            rec[fill[(size_t)d * (NB + 1) + j]++] = make_int2(slot_pix[sl], ub);
+            const size_t r = fill[(size_t)d * (NB + 1) + j]++;
+            rec[r] = make_int2(slot_pix[sl], ub);
+            rec1[r] = make_int2(sl, ub);
         }
-    void *old[] = {p->d_rec, p->d_rec_off, p->d_rec_k};
+    void *old[] = {p->d_rec, p->d_rec_off, p->d_rec_k, p->d_rec1};
     for (void *q : old)
         if (q) (void)hipFree(q);
     p->table_bytes -= p->rec_bytes;
     p->rec_bytes = 0;
-    p->d_rec = nullptr;
+    p->d_rec = p->d_rec1 = nullptr;
     p->d_rec_off = p->d_rec_k = nullptr;
     p->pull = 1;
     p->nbk = 0;
@@ -180,9 +184,11 @@ static int build_pull_records(Plan *p, const std::vector<int32_t> &slot_pix, con
     p->hi_off = 1;
     p->WL = p->Wb + 1;
     rec.resize(total + kRecPad, make_int2(0, 0));   // zero pad: runs of records may read past the last bin
+    rec1.resize(total + kRecPad, make_int2(0, 0));
     int rc;
     if ((rc = dalloc(&p->d_rec, rec.size(), &p->rec_bytes)) || (rc = dalloc(&p->d_rec_off, off.size(), &p->rec_bytes)) ||
-        (rc = upload(p->d_rec, rec.data(), rec.size())) || (rc = upload(p->d_rec_off, off.data(), off.size())))
+        (rc = upload(p->d_rec, rec.data(), rec.size())) || (rc = upload(p->d_rec_off, off.data(), off.size())) ||
+        (rc = dalloc(&p->d_rec1, rec1.size(), &p->rec_bytes)) || (rc = upload(p->d_rec1, rec1.data(), rec1.size())))
         return rc;
     p->table_bytes += p->rec_bytes;
     return MBREC_OK;
@@ -399,12 +405,12 @@ int build_records(Plan *p, const std::vector<int32_t> &slot_pix, const std::vect
             if (sidx) rk[r] = sidx[at];
         }
     // replace the previous tables
-    void *old[] = {p->d_rec, p->d_rec_off, p->d_rec_k};
+    void *old[] = {p->d_rec, p->d_rec_off, p->d_rec_k, p->d_rec1};
     for (void *q : old)
         if (q) (void)hipFree(q);
     p->table_bytes -= p->rec_bytes;
     p->rec_bytes = 0;
-    p->d_rec = nullptr;
+    p->d_rec = p->d_rec1 = nullptr;
     p->d_rec_off = p->d_rec_k = nullptr;
     p->pull = 0;
     p->nbk = nbk;
@@ -589,7 +595,7 @@ static int build_tile_windows(Plan *p, const std::vector<int32_t> &tq)
 static void free_plan(Plan *p)
 {
     void *ptrs[] = {p->d_pix, p->d_tq, p->d_sidx, p->d_sir, p->d_swin, p->d_ir, p->d_irf, p->d_ir64, p->d_irf64,
-                    p->d_kz,  p->d_kx, p->d_rec, p->d_rec_off, p->d_rec_k, p->d_twin, p->d_ptab, p->d_ttab, p->d_srec, p->d_srec_off, p->d_sirw, p->d_stwin, p->d_stab};
+                    p->d_kz,  p->d_kx, p->d_rec, p->d_rec_off, p->d_rec_k, p->d_rec1, p->d_twin, p->d_ptab, p->d_ttab, p->d_srec, p->d_srec_off, p->d_sirw, p->d_stwin, p->d_stab};
     for (void *q : ptrs)
         if (q) (void)hipFree(q);
     free_work(p);

@@ -79,6 +79,7 @@ struct Plan {
     int pull = 0;
     int Wb = 384, nbk = 0, lo_off = 0, hi_off = 1, WL = 0;
     int2 *d_rec = nullptr;       // [records] (pixel, fixed-point travel time) / (pixel, u)
+    int2 *d_rec1 = nullptr;      // [records] (slot, u): the pull records with slot indices (one sinogram)
     int32_t *d_rec_off = nullptr;   // [D][nbk + 1] record offsets / [D][NB + 1]
     int32_t *d_rec_k = nullptr;  // [records] 0-based SIR index (SIR plans)
     // SIR forward in pull form: one record per (pixel, detector) pair with a tap in the
@@ -126,6 +127,7 @@ struct Plan {
     float *d_img_tmp = nullptr;  // 2 x [B][P]       prefilter temporaries
     float *d_sino_tmp = nullptr; // [B][D][nT]       samples before/after the IR filter
     float *d_gpart = nullptr;    // [max(8 B, 16)][P]  partial images of the detector-grouped gathers
+    float *d_img_slot = nullptr; // [Pd]  one sinogram's image in slot order (pull forward)
     float *d_sband = nullptr;    // [D][band waves][3 kSirBandTS + 1][Bp]  band-form SIR forward rows (B > 1)
     int sband_nwave = 0, sband_nfb = 0;
     size_t table_bytes = 0;

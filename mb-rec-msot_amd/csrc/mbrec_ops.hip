@@ -2820,14 +2820,31 @@ static int default_pull_choice(int Bp)
 
 constexpr int kPullRange = 32;   // bins per lane group (batch-lane pull)
 
+// one sinogram's image in slot order (0 in the pad slots): the record stream then reads
+// pixel values of the same device tile from the same few cache lines
+__global__ __launch_bounds__(256) void k_img_to_slots(const int32_t *__restrict__ pix, const float *__restrict__ img,
+                                                      float *__restrict__ out, int Pd)
+{
+    const int sl = blockIdx.x * 256 + threadIdx.x;
+    if (sl >= Pd) return;
+    const int px = pix[sl];
+    out[sl] = px >= 0 ? img[px] : 0.0f;
+}
+
 template <int NG>
 static void fwd_pull_b1(Plan *p, const float *imgI, hipStream_t s)
 {
     const int nblk = (p->geom.NB + 256 * NG - 1) / (256 * NG);   // 4 waves x 64 NG bins per workgroup
     const int xm = p->D % 8 == 0 ? p->D / 8 : 0;
     const dim3 grid = xm ? dim3(nblk * p->D) : dim3(nblk, p->D);
-    hipLaunchKernelGGL(k_fwd_pull_b1<NG>, grid, dim3(256), 0, s, p->d_rec, p->d_rec_off, imgI, p->d_bins,
-                       p->geom.NB, xm);
+    const int2 *rec = p->d_rec;
+    if (p->d_rec1 && p->d_img_slot) {   // slot-indexed records over the image in slot order
+        hipLaunchKernelGGL(k_img_to_slots, dim3((p->Pd + 255) / 256), dim3(256), 0, s, p->d_pix, imgI,
+                           p->d_img_slot, p->Pd);
+        rec = p->d_rec1;
+        imgI = p->d_img_slot;
+    }
+    hipLaunchKernelGGL(k_fwd_pull_b1<NG>, grid, dim3(256), 0, s, rec, p->d_rec_off, imgI, p->d_bins, p->geom.NB, xm);
 }
 
 template <int BC>
