@@ -113,6 +113,18 @@ def test_device_mex_ops_batched(sir, B):
     assert rel_l2(xt.reshape(B, -1), xto) < TOL_OP
 
 
+def test_one_sinogram_forward_larger_image():
+    """The one-sinogram pull (image gathered into device slot order, slot-indexed records)
+    on a 208 x 208 image of 169 device tiles gives the oracle's forward."""
+    Nz, Nx, D, nT, R = 208, 208, 12, 220, 2.6666666666666665
+    _log("one-sinogram forward 208x208")
+    plan, tt, kw, torch = _plan_problem(Nz, Nx, D, nT, R, seed=77)
+    x = rng(7).uniform(0, 1, (1, Nx, Nz))
+    y = plan.forward(torch.tensor(x, dtype=torch.float32).cuda(), model=False).cpu().numpy()
+    yo = np.transpose(O.forward_das(x.reshape(1, -1), tt, nT, R), (0, 2, 1))
+    assert rel_l2(y, yo) < TOL_OP
+
+
 @pytest.mark.parametrize("sir", [False, True])
 @pytest.mark.parametrize("B", [1, 8, 20, 64])
 def test_every_launch_variant_matches_oracle(sir, B):
