@@ -38,7 +38,7 @@ struct HatTaps {
 constexpr int kTileRowsMax = 256;   // tile gathers: longest bin window they stage (rows)
 constexpr int kRecPad = 512;   // zero records after the pull table (runs of up to 512 read past the end)
 constexpr int kSirWinPad = 32;   // zero weights on each side of a window in the pull form's SIR table
-constexpr int kSirBandTS = 28;   // samples per wave of the band-form SIR forward
+constexpr int kSirBandTS = 30;   // samples per wave of the band-form SIR forward
 
 struct BinGeom {
     float phi1, phi2;            // sorted interior breakpoint phases in [0,1)
