@@ -258,7 +258,7 @@ static int build_sir_pull(Plan *p, const std::vector<int32_t> &slot_pix, const s
             const size_t at = (size_t)d * p->Pd + sl;
             if (nfb[at] == INT32_MIN) continue;
             const int k = sidx[at];
-            const int s0 = win[2 * (size_t)k], len = win[2 * (size_t)k + 1];
+            const int len = win[2 * (size_t)k + 1];   // nfb[at] already holds the window's first sample
             int nn, m;
             host_decode(tq[at], g, nn, m);
             const float fr = (float)(tq[at] & ((1 << g.F) - 1)) * scale;
