@@ -473,7 +473,8 @@ def main():
             traffic = json.load(open(tfile)).get(key, {}).get("hbm_bytes_per_launch")
         except Exception:
             traffic = None
-    kname = {"gather": "transpose gather", "scatter": "forward pairs (pull form)"}[dom]
+    fwd_form = "band form" if use_sir and tune[1] == 1004 else "pull form"
+    kname = {"gather": "transpose gather", "scatter": f"forward pairs ({fwd_form})"}[dom]
     t_bytes, t_flops = alg_bytes / (HBM_PEAK_GBS * 1e9), alg_flops / (FP32_PEAK_TFLOPS * 1e12)
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
