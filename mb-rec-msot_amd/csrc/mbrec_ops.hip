@@ -1854,10 +1854,19 @@ __global__ __launch_bounds__(256) void k_fwd_sir_band_sum(const float *__restric
     const int j = (int)(i / Bp), b = (int)(i - (size_t)j * Bp);
     // wave w covers rows 3 (lo + w nfb) .. 3 (lo + w nfb + TS)
     const int wlo = max(0, (j - 3 * (lo + TS) + 3 * nfb - 1) / (3 * nfb)), whi = min(nwave - 1, (j - 3 * lo) / (3 * nfb));
+    // up to 8 rows in flight per step, added in wave order (absent rows add 0)
     float acc = 0.0f;
-    for (int w = wlo; w <= whi; ++w) {
-        const int r = j - 3 * (lo + w * nfb);
-        if (r >= 0 && r <= 3 * TS) acc += scratch[(((size_t)d * nwave + w) * (3 * TS + 1) + r) * Bp + b];
+    for (int w0 = wlo; w0 <= whi; w0 += 8) {
+        float v[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const int w = w0 + k, r = j - 3 * (lo + w * nfb);
+            v[k] = (w <= whi && r >= 0 && r <= 3 * TS)
+                       ? scratch[(((size_t)d * nwave + w) * (3 * TS + 1) + r) * Bp + b]
+                       : 0.0f;
+        }
+#pragma unroll
+        for (int k = 0; k < 8; ++k) acc += v[k];
     }
     bins[((size_t)d * NB + j) * Bp + b] = acc;
 }
