@@ -551,12 +551,15 @@ __host__ __device__ constexpr int fir16_padded_taps(int L) { return (L + 15) & ~
 __host__ __device__ constexpr int fir16_nblk(int Lp) { return ((kFirSpan + Lp + 32) / 16) | 1; }
 __host__ __device__ constexpr int fir16_window(int Lp) { return 16 * fir16_nblk(Lp); }
 
+// LPT > 0: the padded tap count Lp known at compile time (the tap loop unrolls fully, the
+// register window needs no moves and the LDS offsets are immediates); LPT = 0: any L.
+template <int LPT>
 __global__ __launch_bounds__(256) void k_fir16(const float *__restrict__ x, float *__restrict__ out,
                                                const double *__restrict__ taps64, int L, int per_det, int off, int D,
                                                int nT, int rows, float scale, const int2 *__restrict__ win)
 {
     extern __shared__ float sm[];
-    const int Lp = fir16_padded_taps(L), nx = fir16_window(Lp), NBLK = fir16_nblk(Lp);
+    const int Lp = LPT ? LPT : fir16_padded_taps(L), nx = fir16_window(Lp), NBLK = fir16_nblk(Lp);
     const int hf = __builtin_amdgcn_readfirstlane(threadIdx.x >> 7), lt = threadIdx.x & 127;
     float *xs = sm + (size_t)hf * nx;
     const int ntile = (nT + kFirSpan - 1) / kFirSpan;
@@ -610,11 +613,17 @@ __global__ __launch_bounds__(256) void k_fir16(const float *__restrict__ x, floa
 #pragma unroll
         for (int w = 0; w < 8; ++w) W[w] = (double)xs[(lane0 + w) * NBLK + bn];
     };
-    for (int i0 = 0; i0 < Lp; i0 += 16) {   // Lp is a multiple of 16
+    auto step = [&](int i0) {
         group(i0);
         shift(blk - i0 / 16, 0);
         group(i0 + 8);
         if (i0 + 16 < Lp) shift(blk - i0 / 16 - 1, 8);
+    };
+    if constexpr (LPT > 0) {
+#pragma unroll
+        for (int i0 = 0; i0 < LPT; i0 += 16) step(i0);
+    } else {
+        for (int i0 = 0; i0 < Lp; i0 += 16) step(i0);   // Lp is a multiple of 16
     }
     __syncthreads();   // xs is reused for the outputs (same layout)
 #pragma unroll
@@ -3083,8 +3092,12 @@ static int launch_fir(Plan *p, const float *x, float *out, int B, bool corr, flo
     if (lds16 <= kLdsBudget && p->d_ir64) {
         const int rows = B * p->D, ntile = (p->nT + kFirSpan - 1) / kFirSpan;
         const size_t nblk = ((size_t)rows * ntile + 1) / 2;
-        hipLaunchKernelGGL(k_fir16, dim3((unsigned)nblk), dim3(256), lds16, s, x, out, corr ? p->d_irf64 : p->d_ir64,
-                           L, p->ir_per_det, off, p->D, p->nT, rows, scale, win);
+        if (Lp == 208)   // the 201-tap IR of the default IR design
+            hipLaunchKernelGGL(k_fir16<208>, dim3((unsigned)nblk), dim3(256), lds16, s, x, out,
+                               corr ? p->d_irf64 : p->d_ir64, L, p->ir_per_det, off, p->D, p->nT, rows, scale, win);
+        else
+            hipLaunchKernelGGL(k_fir16<0>, dim3((unsigned)nblk), dim3(256), lds16, s, x, out,
+                               corr ? p->d_irf64 : p->d_ir64, L, p->ir_per_det, off, p->D, p->nT, rows, scale, win);
         MBREC_HIP(hipGetLastError());
         return MBREC_OK;
     }
