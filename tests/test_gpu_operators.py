@@ -194,6 +194,27 @@ def _oracle_of(model):
                    sigma=model.normalizationFactor, **kw)
 
 
+@pytest.mark.parametrize("per_det", [False, True])
+def test_model_operators_generic_ir_length(per_det):
+    """An IR of 57 taps (padded to 64, not the 201 taps the filter kernel is unrolled
+    for) takes the IR filter with its tap count read at run time: the model forward and
+    transpose against the oracle, one IR for all detectors or one per detector."""
+    import torch
+    from mbrec.model import DevicePlan
+    _log(f"generic IR length per_det={per_det}")
+    Nz, Nx, D, nT, R = 24, 20, 10, 300, 2.6666666666666665
+    tt = random_tt(Nz * Nx, D, nT, R, seed=91, spill=0.05)
+    ir = rng(92).standard_normal((D, 57) if per_det else 57)
+    plan = DevicePlan(Nz, Nx, D, nT, R, tt.T, ir=ir)
+    orc = O.Model(Nz, Nx, np.ascontiguousarray(tt.T), nT, R, ir=ir)
+    x = rng(93).uniform(0, 1, (2, Nx, Nz))
+    y = plan.forward(torch.tensor(x, dtype=torch.float32).cuda()).cpu().numpy()
+    assert rel_l2(y.reshape(2, -1), orc.forward(x).reshape(2, -1)) < TOL_OP
+    s = rng(94).standard_normal((2, D, nT))
+    xt = plan.transpose(torch.tensor(s, dtype=torch.float32).cuda()).cpu().numpy()
+    assert rel_l2(xt.reshape(2, -1), orc.transpose(s).reshape(2, -1)) < TOL_OP
+
+
 @pytest.mark.parametrize("fir_mfma", [True, False])
 @pytest.mark.parametrize("sir", [False, True])
 def test_model_operators_match_oracle(sir, fir_mfma):
