@@ -199,13 +199,15 @@ int mbrec_rec_nn_with_L1_eye_reg_dev(mbrec_plan plan, const float *data, int B, 
 
 /* rec_nn_with_TV_reg (reconstruction/rec_nn_with_TV_reg.m:1-55) and
  * nnls_sparsa_based_reg_matrix_less (nnls_sparsa_based_reg_matrix_less.m:1-86): the
- * outer splitting with SpaRSA inner solves (inner_iterations = MaxiterA = 2 in the
- * reference), Psi = split-Bregman isotropic-TV prox (SB_ITV2.m), Phi = isotropic TV
- * (ITV.m).  Square grids only.  lcurve (host, nullable) receives per group
- * {||A rec - y||^2, Phi(rec)}. */
+ * outer splitting with SpaRSA inner solves, Psi = split-Bregman isotropic-TV prox
+ * (SB_ITV2.m), Phi = isotropic TV (ITV.m).  inner_iterations = SpaRSA's MaxiterA (2 in
+ * the reference, :64), inner_min_iterations = its MiniterA (not passed by the reference,
+ * so SpaRSA's default 5 applies there): SpaRSA takes no fewer than MiniterA steps and,
+ * past MiniterA, stops after MaxiterA or when the relative objective change is <= 1e-8.
+ * Square grids only.  lcurve (host, nullable) receives per group {||A rec - y||^2, Phi(rec)}. */
 int mbrec_rec_nn_with_TV_reg_dev(mbrec_plan plan, const float *data, int B, int coupled, int num_iter,
-                                 double regParam, int inner_iterations, float *rec, double *lcurve,
-                                 mbrec_stream s);
+                                 double regParam, int inner_iterations, int inner_min_iterations, float *rec,
+                                 double *lcurve, mbrec_stream s);
 
 /* ---- 3. Secondary reconstruction paths (FP64) ------------------------------ */
 

@@ -218,6 +218,16 @@ static int build_sir_pull(Plan *p, const std::vector<int32_t> &slot_pix, const s
     const int lo = -std::max(maxlen - 1, 0), n_nf = span - lo + 1;
     // weights table: window k at k * stride + kSirWinPad, kSirWinPad zeros before it and at least as many after
     const int stride = maxlen + 2 * kSirWinPad;
+    // drop the previous tables first, so an early return below never leaves records of an
+    // earlier SIR table in use
+    if (p->d_srec) { (void)hipFree(p->d_srec); (void)hipFree(p->d_srec_off); (void)hipFree(p->d_sirw); }
+    p->table_bytes -= p->srec_bytes;
+    p->srec_bytes = 0;
+    p->d_srec = nullptr;
+    p->d_srec_off = nullptr;
+    p->d_sirw = nullptr;
+    p->sp_lo = 0;
+    p->sp_n = 0;
     if ((size_t)p->Nsir_build * stride >= ((size_t)1 << 31) || maxlen >= (1 << 15) || p->P >= (1 << 28) ||
         span >= (1 << 15))
         return MBREC_OK;   // the packed record does not fit: the SIR forward keeps the bucketed form only
@@ -269,12 +279,6 @@ static int build_sir_pull(Plan *p, const std::vector<int32_t> &slot_pix, const s
             rec[fill[((size_t)d * 3 + m) * (n_nf + 1) + (nfb[at] - lo)]++] =
                 make_int4(slot_pix[sl] | (m << 28), (nfb[at] & 0xffff) | (len << 16), ub, base);
         }
-    if (p->d_srec) { (void)hipFree(p->d_srec); (void)hipFree(p->d_srec_off); (void)hipFree(p->d_sirw); }
-    p->table_bytes -= p->srec_bytes;
-    p->srec_bytes = 0;
-    p->d_srec = nullptr;
-    p->d_srec_off = nullptr;
-    p->d_sirw = nullptr;
     int rc;
     if ((rc = dalloc(&p->d_srec, rec.size(), &p->srec_bytes)) ||
         (rc = dalloc(&p->d_srec_off, off.size(), &p->srec_bytes)) ||
@@ -707,6 +711,14 @@ int mbrec_plan_set_sir(mbrec_plan h, const int32_t *sirIdx, const double *sir, i
         }
     std::vector<float> sirf(tot);
     for (size_t i = 0; i < tot; ++i) sirf[i] = (float)sir[i];
+    // a second call replaces the tables: the workspace (band-form rows sized for the old
+    // window length and sample range) and the launch choices timed on the old tables go
+    if (p->bcap) {
+        MBREC_HIP(hipDeviceSynchronize());
+        free_work(p);
+    }
+    p->tune = TuneChoice();
+    p->use_sir = 0;
     if (p->d_sidx) { (void)hipFree(p->d_sidx); (void)hipFree(p->d_sir); (void)hipFree(p->d_swin); }
     int rc;
     if ((rc = dalloc(&p->d_sidx, sidx.size(), &p->table_bytes)) ||

@@ -2,7 +2,7 @@
 //
 // Operator formulation (exact, not an approximation):
 //   For one detector row y[0..nT) the transpose MEX loop
-//   (transposeDelayAndSumMex.cpp:171-200, transposeMex.cpp:277-320) evaluates
+//   (transposeDelayAndSumMex.cpp:67-96, transposeMex.cpp:115-156) evaluates
 //       g(tau) = sum_r y[r] * max(0, 1 - |r - tau| / R)
 //   at every pixel's travel time tau.  Each term is a hat in tau with kinks at
 //   r - R, r, r + R, so g is piecewise linear with kinks only at the three

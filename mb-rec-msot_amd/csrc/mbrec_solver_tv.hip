@@ -3,8 +3,9 @@
 // rec_nn_with_TV_reg.m:20-53 -> nnls_sparsa_based_reg_matrix_less.m:24-86: the same
 // outer splitting as the L2 loop, the inner problem solved by SpaRSA (published
 // algorithm: separable step x+ = Psi(x - grad/alpha, tau/alpha), every step accepted,
-// Barzilai-Borwein alpha, initial alpha 1, stop on relative objective change <= 1e-8 or
-// MaxiterA = 2 iterations) with
+// Barzilai-Borwein alpha, initial alpha 1; no fewer than MiniterA and, past MiniterA,
+// no more than MaxiterA steps, stopping early on a relative objective change <= 1e-8;
+// the reference passes MaxiterA = 2 and leaves MiniterA at SpaRSA's default of 5) with
 //   Psi = isotropic-TV prox by split Bregman (SB_ITV2.m:20-40, stop at
 //         ||u_prev - u|| / ||u|| <= 1e-3, shrinkage with d = 0 where |B u + b| = 0),
 //   Phi = sum of isotropic TV with central differences (ITV.m:5-6).
@@ -411,8 +412,8 @@ static int tv_prox(TvProx &T, const float *z, float *x, const std::vector<double
     return MBREC_OK;
 }
 
-int nnls_tv(Plan *p, const float *y, int B, int coupled, int num_iter, double tau, int inner, float *rec,
-            double *lcurve, hipStream_t s)
+int nnls_tv(Plan *p, const float *y, int B, int coupled, int num_iter, double tau, int inner, int inner_min,
+            float *rec, double *lcurve, hipStream_t s)
 {
     if (p->Nz != p->Nx)
         return set_error(MBREC_ERR_ARG, "rec_nn_with_TV_reg:square", "This TV implementation works only with square images!");
@@ -464,7 +465,11 @@ int nnls_tv(Plan *p, const float *y, int B, int coupled, int num_iter, double ta
         std::fill(f.begin(), f.end(), 0.0);
         for (int b = 0; b < B; ++b) f[group(b)] += tau * hs[Q_C * B + b];
         for (int g = 0; g < G; ++g) { f[g] += 0.5 * rr[g]; alpha[g] = 1.0; gactive[g] = 1; }
-        for (int kk = 0; kk < inner; ++kk) {
+        // SpaRSA's iteration count: its counter is k + 1 after step k; it continues while
+        // that is <= MiniterA, stops once it exceeds MaxiterA, and in between stops when the
+        // relative change of the objective is <= ToleranceA (1e-8)
+        const int steps = std::max(inner, inner_min);
+        for (int kk = 0; kk < steps; ++kk) {
             // grad = A_aug' resid; z = x - grad / alpha
             if ((rc = op_transpose(p, rs, atr, B, true, s))) return rc;
             std::vector<double> ia(G);
@@ -503,8 +508,13 @@ int nnls_tv(Plan *p, const float *y, int B, int coupled, int num_iter, double ta
                 f[g] = 0.5 * res[g] + tau * tv[g];
                 double a = adx[g] / (2.2250738585072014e-308 + dd[g]);
                 alpha[g] = std::min(1e30, std::max(1e-30, a));
-                if (std::fabs(f[g] - prev_f[g]) / prev_f[g] <= 1e-8) gactive[g] = 0;
+                const int counter = kk + 2;
+                if (counter <= inner_min) continue;
+                if (counter > inner || std::fabs(f[g] - prev_f[g]) / prev_f[g] <= 1e-8) gactive[g] = 0;
             }
+            bool any = false;
+            for (int g = 0; g < G; ++g) any = any || gactive[g];
+            if (!any) break;
         }
         hipLaunchKernelGGL(k_tv_nnls, dim3(nblk(np * B)), dim3(256), 0, s, (const float *)x, wv, v, np * B);
     }
@@ -530,10 +540,11 @@ int nnls_tv(Plan *p, const float *y, int B, int coupled, int num_iter, double ta
 using namespace mbrec;
 
 extern "C" int mbrec_rec_nn_with_TV_reg_dev(mbrec_plan h, const float *data, int B, int coupled, int num_iter,
-                                            double regParam, int inner_iterations, float *rec, double *lcurve,
-                                            mbrec_stream s)
+                                            double regParam, int inner_iterations, int inner_min_iterations,
+                                            float *rec, double *lcurve, mbrec_stream s)
 {
-    if (!h || !data || !rec || B <= 0 || num_iter < 0 || inner_iterations <= 0)
+    if (!h || !data || !rec || B <= 0 || num_iter < 0 || inner_iterations <= 0 || inner_min_iterations < 0)
         return set_error(MBREC_ERR_ARG, "mbrec:args", "rec_nn_with_TV_reg: invalid arguments");
-    return nnls_tv(&h->p, data, B, coupled, num_iter, regParam, inner_iterations, rec, lcurve, (hipStream_t)s);
+    return nnls_tv(&h->p, data, B, coupled, num_iter, regParam, inner_iterations, inner_min_iterations, rec, lcurve,
+                   (hipStream_t)s);
 }

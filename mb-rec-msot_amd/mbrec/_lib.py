@@ -65,7 +65,8 @@ SIGNATURES = [
     ("mbrec_rec_nn_with_L1_eye_reg_dev", C.c_int,
      [c_void_p, c_void_p, C.c_int, C.c_int, C.c_int, C.c_double, c_void_p, c_void_p]),
     ("mbrec_rec_nn_with_TV_reg_dev", C.c_int,
-     [c_void_p, c_void_p, C.c_int, C.c_int, C.c_int, C.c_double, C.c_int, c_void_p, c_void_p, c_void_p]),
+     [c_void_p, c_void_p, C.c_int, C.c_int, C.c_int, C.c_double, C.c_int, C.c_int, c_void_p, c_void_p,
+      c_void_p]),
     ("mbrec_dmas_cf_dev", C.c_int,
      [c_void_p, c_void_p, C.c_int, c_void_p, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_int,
       C.c_double, C.c_double, c_void_p, c_void_p]),

@@ -49,12 +49,7 @@ class DevicePlan:
         self.Nz, self.Nx, self.D, self.nT, self.R = Nz, Nx, D, nT, R
         self.P = Nz * Nx
         if sir is not None:
-            sidx, table, se = sir
-            sidx = np.ascontiguousarray(sidx, dtype=np.int32)
-            table = np.ascontiguousarray(table, dtype=np.float64)
-            se = np.ascontiguousarray(se, dtype=np.int32)
-            _lib.check(L.mbrec_plan_set_sir(h, sidx.ctypes.data, table.ctypes.data,
-                                            int(table.shape[1]), int(table.shape[0]), se.ctypes.data))
+            self.set_sir(*sir)
         if ir is not None:
             ir = np.ascontiguousarray(ir, dtype=np.float64)
             per_det = 1 if ir.ndim == 2 else 0
@@ -65,6 +60,15 @@ class DevicePlan:
             _lib.check(L.mbrec_plan_set_prefilter(h, kz.ctypes.data, (len(kz) - 1) // 2,
                                                   kx.ctypes.data, (len(kx) - 1) // 2))
         self.set_sigma(sigma)
+
+    def set_sir(self, sidx, table, se):
+        """SIR tables (D, P) int32 1-based indices, (Nsir, Ls) table, (Nsir, 2) windows; a
+        second call replaces the first (workspace and launch choices are rebuilt)."""
+        sidx = np.ascontiguousarray(sidx, dtype=np.int32)
+        table = np.ascontiguousarray(table, dtype=np.float64)
+        se = np.ascontiguousarray(se, dtype=np.int32)
+        _lib.check(_lib.lib().mbrec_plan_set_sir(self.h, sidx.ctypes.data, table.ctypes.data,
+                                                 int(table.shape[1]), int(table.shape[0]), se.ctypes.data))
 
     def set_sigma(self, sigma):
         self.sigma = float(sigma)

@@ -92,6 +92,15 @@ def spmr_table(dists, angles_deg, width, n_sub, c, fs, pixel_model, Ls=SPMR_LENG
     return out, se
 
 
+def spmr_indices(dist, ang, dmin, dd, da, n_angle):
+    """SPMR index of every pair (MSOTmodel.m:472-478), 1-based: distance slot
+    round(1 + (dist - dmin) / dd), angle slot 1 + round(angle_deg / da) (MATLAB round, half
+    away from zero), linearised angle-fastest over n_angle angles."""
+    i_d = matlab_round(1 + (np.asarray(dist) - dmin) / dd)
+    i_a = 1 + matlab_round(np.asarray(ang) * 180.0 / np.pi / da)
+    return ((i_d - 1) * n_angle + i_a).astype(np.int32)
+
+
 def sir_tables_for_model(xg, zg, xd, zd, resolution_fov, c, fs, width, n_sub, pixel_model,
                          Ls=SPMR_LENGTH):
     """get_SPMR_and_SPMR_indices (MSOTmodel.m:463-480) with a generated table.
@@ -101,8 +110,5 @@ def sir_tables_for_model(xg, zg, xd, zd, resolution_fov, c, fs, width, n_sub, pi
     dists = np.arange(dmin, dmax + 0.5 * dd, dd)
     angles = np.arange(0.0, amax + 0.5 * da, da)
     table, se = spmr_table(dists, angles, width, n_sub, c, fs, pixel_model, Ls)
-    i_d = matlab_round(1 + (dist - dmin) / dd)
-    i_a = 1 + matlab_round(ang * 180.0 / np.pi / da)
-    k = (i_d - 1) * len(angles) + i_a
-    return table, se, k.astype(np.int32), dict(dmin=dmin, dd=dd, da=da, n_angle=len(angles),
-                                                 n_dist=len(dists))
+    k = spmr_indices(dist, ang, dmin, dd, da, len(angles))
+    return table, se, k, dict(dmin=dmin, dd=dd, da=da, n_angle=len(angles), n_dist=len(dists))

@@ -143,12 +143,19 @@ def rec_nn_with_L1_eye_reg(model, data_filt, num_iter, regParam, *, coupled=True
     return rec, e_img, e_reg, err, t
 
 
-def rec_nn_with_TV_reg(model, data_filt, num_iter, regParam, *, coupled=True, inner_iterations=2):
+SPARSA_MINITER_DEFAULT = 5   # SpaRSA 2.0's default MiniterA; the reference does not pass one
+
+
+def rec_nn_with_TV_reg(model, data_filt, num_iter, regParam, *, coupled=True, inner_iterations=2,
+                       inner_min_iterations=SPARSA_MINITER_DEFAULT):
     """rec_nn_with_TV_reg.m:1-55: non-negative TV reconstruction (square grids only).
 
     Returns (rec_img, lCurveErrImg, lCurveErrReg, err, t) with lCurveErrImg = ||A rec - y||^2
     (:51) and lCurveErrReg = Phi(rec) = isotropic TV (:52), per sinogram when coupled=False.
-    The inner SpaRSA runs ``inner_iterations`` (MaxiterA, :27) iterations."""
+    The inner SpaRSA takes ``inner_iterations`` = MaxiterA (:27, passed at
+    nnls_sparsa_based_reg_matrix_less.m:64) and ``inner_min_iterations`` = MiniterA (not
+    passed by the reference, so SpaRSA's default applies): no fewer than MiniterA steps, and
+    past MiniterA no more than MaxiterA, stopping early on a relative objective change <= 1e-8."""
     import torch
     y, host = _to_device(data_filt)
     D, nT = model.shape_sinogram
@@ -160,7 +167,7 @@ def rec_nn_with_TV_reg(model, data_filt, num_iter, regParam, *, coupled=True, in
     t0 = time.perf_counter()
     _lib.check(_lib.lib().mbrec_rec_nn_with_TV_reg_dev(
         model.plan.h, y.data_ptr(), int(B), int(bool(coupled)), int(num_iter), float(regParam),
-        int(inner_iterations), rec.data_ptr(), lc.ctypes.data, _lib.stream_handle()))
+        int(inner_iterations), int(inner_min_iterations), rec.data_ptr(), lc.ctypes.data, _lib.stream_handle()))
     el = time.perf_counter() - t0
     e_img = lc[0] if coupled else lc[0::2]
     e_reg = lc[1] if coupled else lc[1::2]

@@ -28,7 +28,7 @@
  * (forwardDelayAndSumMex.cpp:80-99, transposeDelayAndSumMex.cpp:75-94) have
  * no sample-range check and read/write out of bounds for travel times near
  * the sinogram ends.  Here out-of-range samples are dropped, exactly as the
- * SIR loops do (forwardMex.cpp:139, transposeMex.cpp:298).
+ * SIR loops do (forwardMex.cpp:139, transposeMex.cpp:136).
  */
 #include <math.h>
 #include <stdint.h>
@@ -66,7 +66,7 @@ ORC_API void orc_forward_das(const double *image, int W, int P, const double *tt
     }
 }
 
-/* transposeDelayAndSumMex.cpp:171-200: serial detectors, parallel pixels */
+/* transposeDelayAndSumMex.cpp:67-96: serial detectors, parallel pixels */
 ORC_API void orc_transpose_das(const double *sig, int W, int nT, int D,
                                const double *tt, int P, double R, double *out)
 {
@@ -118,7 +118,7 @@ ORC_API void orc_forward_sir(const double *image, int W, int P, const int32_t *s
     }
 }
 
-/* transposeMex.cpp:277-320 */
+/* transposeMex.cpp:115-156 */
 ORC_API void orc_transpose_sir(const double *sig, int W, int nT, int D,
                                const int32_t *sirIdx, const double *tt, int P,
                                const double *sir, int Ls, const int32_t *se,

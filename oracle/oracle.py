@@ -85,7 +85,7 @@ def forward_das(image, tt, nT, R):
 
 
 def transpose_das(sig, tt, R):
-    """transposeDelayAndSumMex.cpp:171-200.  sig [W x nT x D] -> [W x P]."""
+    """transposeDelayAndSumMex.cpp:67-96.  sig [W x nT x D] -> [W x P]."""
     sig, tt = _F(sig, np.float64), _F(tt, np.float64)
     W, nT, D = sig.shape
     P = tt.shape[0]
@@ -107,7 +107,7 @@ def forward_sir(image, sidx, tt, sir, nT, se, R):
 
 
 def transpose_sir(sig, sidx, tt, sir, se, R):
-    """transposeMex.cpp:277-320."""
+    """transposeMex.cpp:115-156."""
     sig, tt, sir = _F(sig, np.float64), _F(tt, np.float64), _F(sir, np.float64)
     sidx, se = _F(sidx, np.int32), _F(se, np.int32)
     W, nT, D = sig.shape
@@ -433,6 +433,48 @@ def lsqr_krylov(A, b, maxit, x0):
 
 
 # ---------------------------------------------------------------------------
+# geometry tables: travel times, SPMR indices (restated from the reference's setup code)
+# ---------------------------------------------------------------------------
+def arc_detectors(D, radius, coverage_deg):
+    """Probe.m:112-115: detector k at (r cos a_k, -r sin a_k) with a_k running from
+    (180 - coverage)/2 to 180 - (180 - coverage)/2 degrees in D even steps."""
+    off = (180.0 - coverage_deg) / 2.0
+    a = (off + np.arange(D) * (coverage_deg / (D - 1))) * np.pi / 180.0
+    return radius * np.cos(a), -radius * np.sin(a)
+
+
+def travel_times(fov_x, fov_z, xd, zd, c, fs, delay, ncrop):
+    """Single speed of sound: refraction.m:20-24 (same side of the membrane: direct travel
+    |q - p| / c), then MSOTmodel.m:380-381 (tau = t fs - delayBeforeRecording -
+    numCroppedSamplesAtSinogramStart).  fov_x / fov_z (Nx, Nz) = MATLAB [Nz x Nx] read in
+    memory order; returns (D, P).  Pinned by hand-computed values (tests/test_oracle.py)."""
+    X = np.asarray(fov_x, dtype=np.float64).ravel()
+    Z = np.asarray(fov_z, dtype=np.float64).ravel()
+    t = np.sqrt((np.asarray(xd)[:, None] - X[None]) ** 2 + (np.asarray(zd)[:, None] - Z[None]) ** 2) / c
+    return t * fs - delay - ncrop
+
+
+def spmr_polar(fov_x, fov_z, xd, zd):
+    """MSOTmodel.m:573-597 (single speed of sound): distance from every detector to every
+    pixel and the azimuth angle between the detector's inward direction -q and q - p:
+    acos(((q - p) . q) / (|q| |q - p|)).  Returns (dist, angle) as (D, P)."""
+    X = np.asarray(fov_x, dtype=np.float64).ravel()[None]
+    Z = np.asarray(fov_z, dtype=np.float64).ravel()[None]
+    xq, zq = np.asarray(xd, dtype=np.float64)[:, None], np.asarray(zd, dtype=np.float64)[:, None]
+    dist = np.sqrt((X - xq) ** 2 + (Z - zq) ** 2)
+    ang = np.arccos(((xq - X) * xq + (zq - Z) * zq) / (np.sqrt(xq ** 2 + zq ** 2) * dist))
+    return dist, ang
+
+
+def spmr_indices(dist, ang, dmin, dd, da_deg, n_angle):
+    """MSOTmodel.m:472-478: k = (round(1 + (dist - dmin)/dd) - 1) n_angle + 1 + round(angle_deg/da),
+    1-based, MATLAB round (half away from zero).  Pinned by hand-computed pairs."""
+    i_d = matlab_round(1 + (np.asarray(dist) - dmin) / dd)
+    i_a = 1 + matlab_round(np.asarray(ang) * 180.0 / np.pi / da_deg)
+    return ((i_d - 1) * n_angle + i_a).astype(np.int64)
+
+
+# ---------------------------------------------------------------------------
 # secondary paths (numpy FP64, statement-by-statement restatements)
 # ---------------------------------------------------------------------------
 def matlab_round(v):
@@ -493,37 +535,62 @@ def divergence(xv, yv, U, V):
     return np.gradient(U, xv, axis=0, edge_order=1) + np.gradient(V, yv, axis=1, edge_order=1)
 
 
-def backproject_waveeq(sig, fov_x, fov_y, c, fs, cropped, coverage, radius, psf, K):
-    """backproject_waveeq.m:13-50 (FP64).  sig (W, D, nT); fov_x, fov_y (Nx, Nz); psf (pm, pn)."""
+def bp_radial_integrals(sig, tsec, c):
+    """backproject_waveeq.m:24-25: cumtrapz(t, S/c, 1) ./ t per detector (trapezoid rule
+    along time, first value 0).  sig (W, D, nT), tsec (nT,)."""
+    s = np.asarray(sig, dtype=np.float64) / c
+    W, D, _ = s.shape
+    cum = np.concatenate([np.zeros((W, D, 1)), np.cumsum(np.diff(tsec)[None, None] * (s[..., :-1] + s[..., 1:]) / 2,
+                                                         axis=-1)], axis=-1)
+    return cum / tsec[None, None, :]
+
+
+def bp_psf_same(S, psf):
+    """backproject_waveeq.m:28: convn(S, psf, 'same') / sum(psf(:)) on each (nT x D) slice:
+    the full 2-D convolution cropped to S's size starting at floor(size(psf) / 2) (MATLAB's
+    'same' centring, which differs from scipy's for even kernel sizes).  S (W, D, nT);
+    psf (pm, pn) = (time, detector)."""
     from scipy.signal import convolve
+    ps = np.atleast_2d(np.asarray(psf, dtype=np.float64))
+    W, D, nT = S.shape
+    out = np.empty_like(S)
+    r0, c0 = ps.shape[0] // 2, ps.shape[1] // 2
+    for w in range(W):
+        full = convolve(S[w].T, ps, mode="full")        # (nT + pm - 1, D + pn - 1)
+        out[w] = full[r0:r0 + nT, c0:c0 + D].T
+    return out / ps.sum()
+
+
+def bp_gather(Swl, fx, fy, ang, radius, c, fs, t1, dt):
+    """backproject_waveeq.m:35-46 for one wavelength: per detector p the nearest sample
+    T = round((|x - q_p| / c - t1) fs + 1) of Swl (nT, D) (1-based), accumulated into the
+    two components (cos, sin)(theta_p) dt c.  Returns (px, py) shaped like fx."""
+    px = np.zeros(fx.shape); py = np.zeros(fx.shape)
+    for p in range(Swl.shape[1]):
+        T = matlab_round((np.sqrt(np.abs(fx - np.cos(ang[p]) * radius) ** 2 +
+                                  np.abs(fy - np.sin(ang[p]) * radius) ** 2) / c - t1) * fs + 1).astype(np.int64)
+        px = px + Swl[T - 1, p] * np.cos(ang[p]) * dt * c
+        py = py + Swl[T - 1, p] * np.sin(ang[p]) * dt * c
+    return px, py
+
+
+def backproject_waveeq(sig, fov_x, fov_y, c, fs, cropped, coverage, radius, psf, K):
+    """backproject_waveeq.m:13-50 (FP64).  sig (W, D, nT); fov_x, fov_y (Nx, Nz); psf (pm, pn).
+    Composition of bp_radial_integrals, bp_psf_same, the K(t, t) S dt product, bp_gather and
+    fliplr(divergence(...)); each piece is pinned in tests/test_secondary.py."""
     sig = np.asarray(sig, dtype=np.float64)
     W, D, nT = sig.shape
     dt = 1.0 / fs
     ang = _bp_angles(D, coverage)
     tsam = np.arange(1, nT + 1) + cropped
     tsec = tsam * dt
-    s = sig / c
-    cum = np.concatenate([np.zeros((W, D, 1)), np.cumsum(np.diff(tsec)[None, None] * (s[..., :-1] + s[..., 1:]) / 2,
-                                                         axis=-1)], axis=-1)
-    S = cum / tsec[None, None, :]
-    ps = np.atleast_2d(np.asarray(psf, dtype=np.float64))
-    Sc = np.empty_like(S)
-    for w in range(W):
-        full = convolve(S[w].T, ps, mode="full")        # (nT, D) with psf (pm, pn)
-        r0, c0 = ps.shape[0] // 2, ps.shape[1] // 2
-        Sc[w] = full[r0:r0 + nT, c0:c0 + D].T
-    Sc = Sc / ps.sum()
+    Sc = bp_psf_same(bp_radial_integrals(sig, tsec, c), psf)
     fx, fy = np.asarray(fov_x, dtype=np.float64), np.asarray(fov_y, dtype=np.float64)
     out = np.zeros((W,) + fx.shape)
     Ksub = np.asarray(K)[np.ix_(tsam - 1, tsam - 1)]
     for w in range(W):
         Swl = Ksub @ Sc[w].T * dt                         # (nT, D)
-        px = np.zeros(fx.shape); py = np.zeros(fx.shape)
-        for p in range(D):
-            T = matlab_round((np.sqrt(np.abs(fx - np.cos(ang[p]) * radius) ** 2 +
-                                      np.abs(fy - np.sin(ang[p]) * radius) ** 2) / c - tsec[0]) * fs + 1).astype(np.int64)
-            px = px + Swl[T - 1, p] * np.cos(ang[p]) * dt * c
-            py = py + Swl[T - 1, p] * np.sin(ang[p]) * dt * c
+        px, py = bp_gather(Swl, fx, fy, ang, radius, c, fs, tsec[0], dt)
         div = divergence(fx[:, 0], fy[0, :], px, py)
         out[w] = div[::-1, :]                             # fliplr: reverse x (columns in MATLAB)
     return out
@@ -677,19 +744,28 @@ def itv(x):
     return np.sum(np.sqrt(dx ** 2 + dy ** 2))
 
 
-def sparsa(y, A, AT, tau, psi, phi, x0, maxiter=2, tolA=1e-8):
+SPARSA_MINITER = 5   # SpaRSA 2.0's default MiniterA (not passed at nnls_sparsa_based_reg_matrix_less.m:64)
+
+
+def sparsa(y, A, AT, tau, psi, phi, x0, maxiter=2, miniter=SPARSA_MINITER, tolA=1e-8):
     """SpaRSA 2.0 (Wright, Nowak, Figueiredo, IEEE TSP 57(7), 2009; called at
-    nnls_sparsa_based_reg_matrix_less.m:64) as published: step from the separable
+    nnls_sparsa_based_reg_matrix_less.m:64 with MaxiterA = maxiter, StopCriterion 1,
+    ToleranceA 1e-8, Initialization x0) as published: step from the separable
     approximation x+ = psi(x - grad/alpha, tau/alpha), every step accepted (no monotone
     or safeguard test, the SpaRSA defaults), Barzilai-Borwein alpha = ||A dx||^2/||dx||^2
-    clipped to [1e-30, 1e30], initial alpha = 1, stop criterion 1 (relative change of the
-    objective <= tolA) or maxiter iterations.  PARITY UNPINNED: SpaRSA is not vendored and
-    its exact defaults (MiniterA, initial alpha) cannot be checked here."""
+    clipped to [1e-30, 1e30], initial alpha = 1.  Iteration count: SpaRSA's counter is
+    k + 1 after step k; it continues while that is <= MiniterA, stops once it exceeds
+    MaxiterA, and otherwise stops when |f - f_prev| / f_prev <= ToleranceA (criterion 1).
+    The step, the BB rule and alpha_0 = 1 are pinned by a hand-derived trace and by
+    convergence to independently computed minimisers (tests/test_tv.py); SpaRSA's source
+    is not vendored, so MiniterA's default (5) and the counter convention are taken from
+    its published documentation and are PARITY UNPINNED."""
     x = x0.copy()
     resid = A(x) - y
     f = 0.5 * resid @ resid + tau * phi(x)
     alpha = 1.0
-    for _ in range(maxiter):
+    k = 0
+    while True:
         grad = AT(resid)
         prev_x, prev_resid, prev_f = x, resid, f
         x = psi(prev_x - grad / alpha, tau / alpha)
@@ -699,12 +775,16 @@ def sparsa(y, A, AT, tau, psi, phi, x0, maxiter=2, tolA=1e-8):
         f = 0.5 * resid @ resid + tau * phi(x)
         dd = dx @ dx
         alpha = min(1e30, max(1e-30, (Adx @ Adx) / (np.finfo(float).tiny + dd)))
-        if abs(f - prev_f) / prev_f <= tolA:
+        k += 1
+        counter = k + 1
+        if counter <= miniter:
+            continue
+        if counter > maxiter or abs(f - prev_f) / prev_f <= tolA:
             break
     return x
 
 
-def rec_nn_with_TV_reg(model, data, num_iter, regParam, exact_prox=False, inner=2):
+def rec_nn_with_TV_reg(model, data, num_iter, regParam, exact_prox=False, inner=2, inner_min=SPARSA_MINITER):
     """rec_nn_with_TV_reg.m:20-53 -> nnls_sparsa_based_reg_matrix_less.m:24-86 (FP64).
     model: an oracle.Model (square grid); data (W, D, nT).  Returns (rec (W, Nx, Nz), lcurve)."""
     data = np.asarray(data, dtype=np.float64)
@@ -731,7 +811,7 @@ def rec_nn_with_TV_reg(model, data, num_iter, regParam, exact_prox=False, inner=
     w_ = np.zeros(P * W); v_ = np.zeros(P * W)
     for _ in range(num_iter):
         y = np.concatenate([sl * tgt, sb * (w_ + v_)])
-        u = sparsa(y, A, AT, regParam, psi, phi, w_, maxiter=inner, tolA=1e-8)
+        u = sparsa(y, A, AT, regParam, psi, phi, w_, maxiter=inner, miniter=inner_min, tolA=1e-8)
         w_ = np.maximum(u - v_, 0)
         v_ = v_ + w_ - u
     rec = w_.reshape(W, N, N)

@@ -14,7 +14,7 @@ case sir_single_pixel (forwardMex.cpp:118-159):
     tau = 10, R = 1.5: hat taps r = 9, 10, 11 with weights 1/3, 1, 1/3;
     t = r + s - 2 for s = 1, 2, 3 (0-based): y[8] = 1/3, y[9] = 1 + 2/3,
     y[10] = 1/3 + 2 + 1, y[11] = 2/3 + 3, y[12] = 1.
-case das_transpose_single_detector (transposeDelayAndSumMex.cpp:171-200):
+case das_transpose_single_detector (transposeDelayAndSumMex.cpp:67-96):
     y = [0, 1, 2, ..., 9] (nT = 10), tau = 4.4, R = 2: taps r = 3..6, weights
     0.3, 0.8, 0.7, 0.2 -> x = 0.3*3 + 0.8*4 + 0.7*5 + 0.2*6 = 8.8.
 """

@@ -113,6 +113,36 @@ def test_device_mex_ops_batched(sir, B):
     assert rel_l2(xt.reshape(B, -1), xto) < TOL_OP
 
 
+def test_set_sir_twice_replaces_tables():
+    """A second mbrec_plan_set_sir with longer windows, more SIR entries and a different
+    table length replaces every SIR-dependent table and the workspace sized from them:
+    the batch (band form) and one-sinogram forwards and the transposes match the oracle of
+    the new tables."""
+    import torch
+    from mbrec.model import DevicePlan
+    Nz, Nx, D, nT, R = 33, 31, 10, 240, 2.6666666666666665
+    P = Nz * Nx
+    tt = random_tt(P, D, nT, R, seed=41, spill=0.05)
+    first = random_sir(P, D, Ls=7, Nsir=5, seed=42)
+    second = random_sir(P, D, Ls=29, Nsir=13, seed=43)
+    plan = DevicePlan(Nz, Nx, D, nT, R, tt.T, sir=tuple(a.T for a in first))
+    for B in (1, 20):           # build the workspace (and band rows) for the first tables
+        plan.forward(torch.rand((B, P), device="cuda"), model=False)
+    plan.set_sir(*(a.T for a in second))
+    sidx, s, se = second
+    for B in (1, 20):
+        _log(f"set_sir twice B={B}")
+        x = rng(44 + B).uniform(0, 1, (B, P))
+        y = plan.forward(torch.tensor(x, dtype=torch.float32).cuda(), model=False).cpu().numpy()
+        yo = np.transpose(O.forward_sir(x, sidx, tt, s, nT, se, R), (0, 2, 1))
+        assert rel_l2(y, yo) < TOL_OP
+        sg = rng(45 + B).standard_normal((B, D, nT))
+        xt = plan.transpose(torch.tensor(sg, dtype=torch.float32).cuda(), model=False).cpu().numpy()
+        xto = O.transpose_sir(np.transpose(sg, (0, 2, 1)), sidx, tt, s, se, R)
+        assert rel_l2(xt.reshape(B, -1), xto) < TOL_OP
+    plan.close()
+
+
 def test_one_sinogram_forward_larger_image():
     """The one-sinogram pull (image gathered into device slot order, slot-indexed records)
     on a 208 x 208 image of 169 device tiles gives the oracle's forward."""
@@ -178,20 +208,50 @@ def _geometry_model(Nz, Nx, D, nT, h, sir=False, sigma=None):
 
 
 def _oracle_of(model):
-    """Oracle model on the same tables; the prefilter kernels come from the oracle's own
-    direct DFT of the Gaussian pixel (sigma = round(h/2 um), Discretization.m:43)."""
+    """Oracle model of the same acquisition.  Travel times, the interpolation radius and the
+    SPMR indices are the oracle's own (restated from refraction.m:20-24, MSOTmodel.m:36-38,
+    380-381, 472-478, 573-597 and Probe.m:112-115, pinned by hand-computed values), computed
+    from the grid and probe parameters; the product's tables are only compared with them.
+    The SIR table values, the IR and sigma are inputs (data) shared with the model; the
+    prefilter kernels come from the oracle's own direct DFT of the Gaussian pixel
+    (sigma = round(h/2 um), Discretization.m:43)."""
     d = model.Discretization
     Nz, Nx = d.Nz, d.Nx
-    h = d.dz
+    probe = model.Probe
+    D = probe.detector.numOfTransducers
+    c, fs = model.Medium.speedOfSound, probe.DAC.frequency
+    X, Z = d.region["xPositionsFov"].T, d.region["zPositionsFov"].T
+    xd, zd = O.arc_detectors(D, probe.detector.radius, probe.detector.angularCoverage)
+    tt = O.travel_times(X, Z, xd, zd, c, fs, probe.DAC.delayBeforeRecording,
+                        model.DataPreprocessing.numCroppedSamplesAtSinogramStart)
+    np.testing.assert_allclose(model.Data.absoluteTravelTimeIndices.reshape(D, -1), tt, rtol=0, atol=1e-9)
+    R = (Z[0, 1] - Z[0, 0]) * fs / c
+    assert abs(R - model.Data.circularInterpolationRadiusInSamples) < 1e-12
     sig_px = d.pixelSigma   # the model's own sigma = round(min(dx, dz) / 2 um)
     kz, kx = O.full_kernels(Nz, Nx, d.dz, d.dx, sig_px)
     kw = dict(kz=kz, kx=kx, ir=model.Data.IR)
     if model.includeSir:
-        kw.update(sidx=model.Data.SPMRIndices, sir=model.Data.SPMR, se=model.Data.SPMRTimestepIndexStartAndEnd)
-    D = model.Probe.detector.numOfTransducers
-    return O.Model(Nz, Nx, model.Data.absoluteTravelTimeIndices.reshape(D, -1),
-                   model.Data.numTimeSamplesPerTransducer, model.Data.circularInterpolationRadiusInSamples,
-                   sigma=model.normalizationFactor, **kw)
+        g = model.Data.SPMRGrid
+        sidx = O.spmr_indices(*O.spmr_polar(X, Z, xd, zd), g["dmin"], g["dd"], g["da"], g["n_angle"])
+        np.testing.assert_array_equal(model.Data.SPMRIndices, sidx)
+        kw.update(sidx=sidx.astype(np.int32), sir=model.Data.SPMR, se=model.Data.SPMRTimestepIndexStartAndEnd)
+    return O.Model(Nz, Nx, tt, model.Data.numTimeSamplesPerTransducer, R, sigma=model.normalizationFactor, **kw)
+
+
+def test_sigma_matches_dense_svd():
+    """sigma_max of the unnormalised model (MSOTmodel.m:271-287, svds there) from the
+    product's device power iteration equals the largest singular value of the oracle's
+    dense FP64 operator (numpy SVD)."""
+    _log("sigma vs dense SVD")
+    m = _geometry_model(12, 10, 12, 160, 1.25e-4, sir=False, sigma=None)
+    orc = _oracle_of(m)
+    orc1 = O.Model(orc.Nz, orc.Nx, orc.tt, orc.nT, orc.R, sigma=1.0, kz=orc.kz, kx=orc.kx, ir=orc.ir)
+    P = 12 * 10
+    E = np.eye(P).reshape(P, 10, 12)
+    A = orc1.forward(E).reshape(P, -1).T
+    s_true = np.linalg.svd(A, compute_uv=False)[0]
+    s_dev = m.estimate_sigma(iters=500)
+    assert abs(s_dev - s_true) / s_true < 1e-5, (s_dev, s_true)
 
 
 @pytest.mark.parametrize("per_det", [False, True])

@@ -192,6 +192,66 @@ def test_gauspuls_definition():
         assert abs(20 * np.log10(v) + 6.0) < 0.2
 
 
+# --- geometry tables ---------------------------------------------------------
+def test_travel_times_hand_values():
+    """Three detectors on a 50 mm half circle (coverage 180 deg: at 0, 90, 180 deg, i.e.
+    (0.05, 0), (0, -0.05), (-0.05, 0)), c = 1500 m/s, fs = 30 MHz (1 m = 2e4 samples),
+    delay 10, 5 cropped samples.  Pixel (0, 0): 0.05 m -> 1000 - 15 = 985 for all three.
+    Pixel (x, z) = (0.03, -0.01): to (0, -0.05) a 3-4-5 triangle, 0.05 m -> 985; to
+    (0.05, 0) sqrt(0.0005) m -> 447.2135955 - 15; to (-0.05, 0) sqrt(0.0065) m -> 1612.4515497 - 15."""
+    xd, zd = O.arc_detectors(3, 0.05, 180.0)
+    np.testing.assert_allclose(xd, [0.05, 0.0, -0.05], atol=1e-15)
+    np.testing.assert_allclose(zd, [0.0, -0.05, 0.0], atol=1e-15)
+    tt = O.travel_times(np.array([[0.0], [0.03]]), np.array([[0.0], [-0.01]]), xd, zd, 1500.0, 3e7, 10, 5)
+    np.testing.assert_allclose(tt[:, 0], [985.0, 985.0, 985.0], atol=1e-9)
+    np.testing.assert_allclose(tt[:, 1], [447.2135955 - 15, 985.0, 1612.4515497 - 15], atol=1e-6)
+
+
+def test_travel_times_match_independent_geometry_and_product():
+    """The oracle's table on a 9 x 7 grid and 16 detectors equals the independent helper's
+    and the product's host table (MSOTmodel travel_time_indices)."""
+    from helpers import geometry_tt
+    from mbrec.discretization import Discretization
+    from mbrec.model import travel_time_indices
+    from mbrec.probe import Probe
+    Nz, Nx, h, D = 9, 7, 1e-4, 16
+    disc = Discretization([-(Nx - 1) / 2 * h, (Nx - 1) / 2 * h, -(Nz - 1) / 2 * h, (Nz - 1) / 2 * h], [Nx, Nz])
+    probe = Probe("arc256_r40_174", True, False, num_detectors=D)
+    probe.DAC.delayBeforeRecording = 7
+    X, Z = disc.region["xPositionsFov"].T, disc.region["zPositionsFov"].T
+    xd, zd = O.arc_detectors(D, 0.040, 174.0)
+    tt = O.travel_times(X, Z, xd, zd, 1500.0, 4e7, 7, 3)
+    want = geometry_tt(Nz, Nx, h, D, 0.040, 174.0, 1500.0, 4e7, delay=10).reshape(D, -1)
+    np.testing.assert_allclose(tt, want, rtol=0, atol=1e-9)
+    np.testing.assert_allclose(travel_time_indices(disc, probe, 1500.0, 3).reshape(D, -1), tt, rtol=0, atol=1e-9)
+
+
+def test_spmr_polar_hand_values():
+    """Detector at (0, -0.05): pixel (0.03, -0.01) is at 0.05 m and acos(0.8) (the 3-4-5
+    angle) off the detector's inward direction; pixel (0, 0) at 0.05 m and angle 0."""
+    dist, ang = O.spmr_polar(np.array([[0.03], [0.0]]), np.array([[-0.01], [0.0]]), np.array([0.0]),
+                             np.array([-0.05]))
+    np.testing.assert_allclose(dist[0], [0.05, 0.05], atol=1e-15)
+    np.testing.assert_allclose(ang[0], [np.arccos(0.8), 0.0], atol=1e-7)
+    from mbrec.sir import polar_coordinates
+    d2, a2 = polar_coordinates(np.array([0.03, 0.0]), np.array([-0.01, 0.0]), np.array([0.0]), np.array([-0.05]))
+    np.testing.assert_allclose(d2, dist, atol=1e-15)
+    np.testing.assert_allclose(a2, ang, atol=1e-7)
+
+
+def test_spmr_indices_hand_values():
+    """dmin = 1.0, dd = 0.25, da = 0.5 deg, 40 angles (MSOTmodel.m:472-478):
+    (dist 1.35, 3.6 deg):  round(1 + 1.4) = 2, 1 + round(7.2) = 8      -> (2-1)*40 + 8 = 48;
+    (dist 1.625, 0 deg):   round(1 + 2.5) = 4 (half away from zero), 1 -> 3*40 + 1 = 121;
+    (dist 0.975, 10.2 deg): round(1 - 0.1) = 1, 1 + round(20.4) = 21   -> 21."""
+    from mbrec.sir import spmr_indices
+    dist = np.array([1.35, 1.625, 0.975])
+    ang = np.array([3.6, 0.0, 10.2]) * np.pi / 180
+    want = [48, 121, 21]
+    np.testing.assert_array_equal(O.spmr_indices(dist, ang, 1.0, 0.25, 0.5, 40), want)
+    np.testing.assert_array_equal(spmr_indices(dist, ang, 1.0, 0.25, 0.5, 40), want)
+
+
 # --- solvers ---------------------------------------------------------------------
 def _tiny_model(seed=21, sir=False, Nz=8, Nx=7, D=5, nT=70):
     R = 2.6666666666666665

@@ -74,23 +74,99 @@ def test_spline_matrix_reproduces_cubics():
     np.testing.assert_allclose(U @ cub(knots), cub(q), rtol=1e-10, atol=1e-8)
 
 
-def test_cumtrapz_and_same_conv_pieces():
+def test_bp_radial_integrals_match_cumulative_trapezoid():
+    """The time integral of backproject_waveeq.m:24-25 is scipy's cumulative trapezoid
+    (initial value 0) divided by the acquisition time of each sample."""
     from scipy.integrate import cumulative_trapezoid
+    sig = rng(3).standard_normal((2, 5, 60))
+    tsec = (np.arange(1, 61) + 7) / FS
+    want = cumulative_trapezoid(sig / C, tsec, axis=-1, initial=0) / tsec
+    np.testing.assert_allclose(O.bp_radial_integrals(sig, tsec, C), want, rtol=1e-13, atol=0)
+
+
+def test_bp_psf_same_matches_convolve2d_for_odd_kernels():
     from scipy.signal import convolve2d
-    g = rng(3)
-    W, D, nT = 1, 5, 60
-    sig = g.standard_normal((W, D, nT))
-    psf = g.uniform(0.1, 1, (3, 5))
-    # numpy oracle with an identity K reduces to the pre-GEMM pipeline: check pieces directly
-    tsec = (np.arange(1, nT + 1) + 2) / FS
-    cum = cumulative_trapezoid(sig[0] / C, tsec, axis=-1, initial=0) / tsec
-    same = convolve2d(cum.T, psf, mode="same") / psf.sum()
-    K = np.eye(nT + 2)
-    X, Z = _grid(4, 1e-4)
-    # gather equals Sc at nearest samples; verify via the oracle's intermediate
-    out = O.backproject_waveeq(sig, X, Z, C, FS, 2, COV, RAD * 0 + (nT / 2 + 3) * C / FS, psf, K)
-    assert out.shape == (1, 4, 4) and np.all(np.isfinite(out))
-    assert same.shape == (nT, D)
+    S = rng(31).standard_normal((2, 7, 40))            # (W, D, nT)
+    psf = rng(32).uniform(0.1, 1, (3, 5))              # (time, detector)
+    got = O.bp_psf_same(S, psf)
+    for w in range(2):
+        want = convolve2d(S[w].T, psf, mode="same") / psf.sum()
+        np.testing.assert_allclose(got[w].T, want, rtol=1e-12, atol=1e-14)
+
+
+def test_bp_psf_same_matlab_centring_for_even_kernels():
+    """MATLAB's documented conv example: conv([-1 2 3 -2 0 1 2], [2 4 -1 1], 'same') =
+    [15 5 -9 7 6 7 -1] (the full result from index floor(4/2) + 1); scipy's 'same' starts
+    one sample earlier for even kernels.  Along time and along detectors."""
+    u = np.array([-1.0, 2, 3, -2, 0, 1, 2])
+    v = np.array([2.0, 4, -1, 1])
+    want = np.array([15.0, 5, -9, 7, 6, 7, -1]) / v.sum()
+    np.testing.assert_allclose(O.bp_psf_same(u[None, None, :], v[:, None])[0, 0], want, atol=1e-14)
+    np.testing.assert_allclose(O.bp_psf_same(u[None, :, None], v[None, :])[0, :, 0], want, atol=1e-14)
+
+
+def _brute_backproject(sig, fx, fy, cropped, K):
+    """backproject_waveeq.m with psf = 1, written as scalar loops: trapezoid sums, the
+    K(t, t) S dt product element by element, the nearest-sample gather per pixel and
+    detector, MATLAB gradient (central inside, one-sided at the edges), fliplr."""
+    import math
+    W, D, nT = sig.shape
+    dt = 1.0 / FS
+    off = (180.0 - COV) / 2
+    th = [-(off + p * COV / (D - 1)) * math.pi / 180 for p in range(D)]
+    t = [(n + 1 + cropped) * dt for n in range(nT)]
+    Nx, Nz = fx.shape
+    xs, zs = fx[:, 0], fy[0, :]
+    out = np.zeros((W, Nx, Nz))
+    for w in range(W):
+        S = np.zeros((nT, D))
+        for p in range(D):
+            acc = 0.0
+            for n in range(nT):
+                if n:
+                    acc += (t[n] - t[n - 1]) * (sig[w, p, n - 1] + sig[w, p, n]) / (2 * C)
+                S[n, p] = acc / t[n]
+        Swl = np.zeros((nT, D))
+        for i in range(nT):
+            for p in range(D):
+                Swl[i, p] = sum(K[i + cropped, j + cropped] * S[j, p] for j in range(nT)) * dt
+        px = np.zeros((Nx, Nz)); py = np.zeros((Nx, Nz))
+        for ix in range(Nx):
+            for iz in range(Nz):
+                for p in range(D):
+                    d = math.hypot(fx[ix, iz] - math.cos(th[p]) * RAD, fy[ix, iz] - math.sin(th[p]) * RAD)
+                    v = (d / C - t[0]) * FS + 1
+                    T = int(math.floor(v + 0.5)) if v >= 0 else -int(math.floor(-v + 0.5))
+                    px[ix, iz] += Swl[T - 1, p] * math.cos(th[p]) * dt * C
+                    py[ix, iz] += Swl[T - 1, p] * math.sin(th[p]) * dt * C
+
+        def grad(f, coord, i, n, get):
+            if i == 0:
+                return (get(1) - get(0)) / (coord[1] - coord[0])
+            if i == n - 1:
+                return (get(n - 1) - get(n - 2)) / (coord[n - 1] - coord[n - 2])
+            return (get(i + 1) - get(i - 1)) / (coord[i + 1] - coord[i - 1])
+        for ix in range(Nx):
+            for iz in range(Nz):
+                dU = grad(px, xs, ix, Nx, lambda k: px[k, iz])
+                dV = grad(py, zs, iz, Nz, lambda k: py[ix, k])
+                out[w, Nx - 1 - ix, iz] = dU + dV
+    return out
+
+
+def test_backproject_waveeq_matches_brute_force():
+    """Oracle composition (identity PSF, a random K of Nt > nT + cropped) against the scalar
+    loop form: pins the K(t, t) block, the gather, the divergence and the fliplr."""
+    D, nT, N = 7, 40, 5
+    cropped = int(round(RAD / C * FS)) - nT // 2
+    X, Z = _grid(N, 2e-4)
+    sig = rng(33).standard_normal((2, D, nT))
+    Nt = nT + cropped + 3
+    K = rng(34).standard_normal((Nt, Nt))
+    got = O.backproject_waveeq(sig, X, Z, C, FS, cropped, COV, RAD, np.ones((1, 1)), K)
+    want = _brute_backproject(sig, X, Z, cropped, K)
+    assert np.abs(want).max() > 0
+    np.testing.assert_allclose(got, want, rtol=1e-10, atol=1e-12 * np.abs(want).max())
 
 
 # --- GPU parity ----------------------------------------------------------------------
