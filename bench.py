@@ -1,21 +1,30 @@
-"""Benchmark: NNLS-L2 reconstructions per second at 400^2 px x 256 channels on one MI355X.
+"""Benchmark: NNLS-L2 reconstructions per second at 400^2 px x 256 channels on one MI355X,
+learned warm start vs plain cold start, each to 1e-3 relative L2 of the converged solution.
 
-Workload (one "step"): rec_nn_with_L2_reg, 50 outer iterations (2 LSQR inner
-iterations each, lambda = 1, beta = 1e-3) for a batch of B = 64 independent
-seeded sinograms on the 400 x 400 / 100 um grid without SIR (--sir adds the
-SIR model), 256-element 40 mm / 174 deg arc, fs 40 MHz, 2030 samples, c 1500 m/s, delay 0, synthetic gauspuls EIR
-(fc 4 MHz, 100 %), band 0.1-12 MHz, sigma_max by power iteration.  Phantoms:
-seeded disks / ellipses / vessel trees, 1 % Gaussian noise, lambdaL2 =
-1e-2 * ||A'y|| per sinogram, RegL2 = identity.  The batch runs as one captured
-HIP graph (BatchSolver); all inputs are resident in HBM before the timed region.
+Workload: a batch of B = 64 independent rec_nn_with_L2_reg solves (RegL2 = identity,
+nnls_L2_less with 2 LSQR inner iterations, lambda = 1, beta = 1e-3) on the 400 x 400 /
+100 um grid without SIR, 256-element 40 mm / 174 deg arc, fs 40 MHz, 2030 samples,
+c 1500 m/s, delay 0, synthetic gauspuls EIR (fc 4 MHz, 100 %), band 0.1-12 MHz,
+sigma_max by power iteration.  Phantoms: seeded disks / ellipses / vessel trees, 1 %
+Gaussian noise, lambdaL2 = 1e-2 ||A'y|| per sinogram.  All inputs are resident in HBM
+before the timed region.
 
-Prints one JSON line (rank 0).  value = reconstructions / s over the timed
-steps.  roofline: the dominant operator kernel (by time per solve), its
-algorithmic bytes per launch (travel-time table + SIR index + referenced SIR
-entries + image + sinogram, FP32) over its average duration measured with HIP
-events on the launch stream.  cpu_baseline: the FP64 C oracle (same loops and
-OpenMP decomposition as the reference MEX files) on a bounded sample of the
-same operator, scaled to one solve.
+The reference solution is the converged minimiser (the fixed point of the plain loop,
+KKT residual ~1e-7).  The outer iterations each form needs for every item of the batch
+to come within 1e-3 of it are found first (untimed, in chunks that continue from the
+loop's full state); then one "step" = one batch solve: for the learned form the
+network's state estimate from A'y (with its cold-start fallback) plus the warm-started
+loop, for the plain form the loop from zero.  value = B / (time per learned step).
+
+roofline: the dominant operator kernel (by time per outer iteration), its algorithmic
+bytes and flops per launch over its average duration (HIP events on the launch stream),
+labelled by the binding term.  cpu_baseline: the FP64 C oracle (same loops and OpenMP
+decomposition as the reference MEX files) on a bounded sample of operator applications,
+scaled to one plain solve of the same iteration count.
+
+Other modes: --fixed-iters N (and --sir) time a fixed-iteration plain solve; --mixed the
+224-sinogram mixed-grid batch; --tv600 the TV configuration; --secondary the secondary
+paths; --mex the per-call time of the MEX drop-ins.
 """
 import argparse
 import json
@@ -37,7 +46,13 @@ def parse():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--batch", type=int, default=64)
-    ap.add_argument("--iters", type=int, default=50)
+    ap.add_argument("--iters", type=int, default=50, help="outer iterations of --mixed")
+    ap.add_argument("--fixed-iters", type=int, default=0,
+                    help="time a fixed-iteration plain solve instead of the 1e-3 metric (--sir implies 50)")
+    ap.add_argument("--chunk", type=int, default=25, help="granularity of the iterations-to-1e-3 search")
+    ap.add_argument("--weights", default=None, help="warm-start network weights (.npz)")
+    ap.add_argument("--mex", action="store_true", help="per-call time of the four MEX drop-ins at 400^2")
+    ap.add_argument("--mex-reps", type=int, default=20)
     ap.add_argument("--grid", type=int, default=400)
     ap.add_argument("--sir", action="store_true", help="400x400 with SIR (the SIR parity configuration)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -359,42 +374,23 @@ def mixed(a):
     print(json.dumps(out), flush=True)
 
 
-def main():
-    a = parse()
-    if a.mixed:
-        mixed(a)
-        return
-    if a.tv600:
-        tv600(a)
-        return
-    if a.traffic_from:
-        traffic_from(*a.traffic_from, N=a.grid, B=a.batch)
-        return
-    if a.secondary:
-        secondary(a)
-        return
-    if a.gpus != 1:
-        print(f"bench.py runs on one GPU only; --gpus {a.gpus} is not supported in this build "
-              "(batches of independent solves shard across GPUs with no exchange; see DESIGN.md)",
-              flush=True)
-        sys.exit(2)
+def build_problem(a, use_sir, seed=1000):
+    """The 400^2 / 256-channel model, B seeded phantoms, their noisy sinograms (device,
+    resident) and the per-sinogram lambdaL2 = 1e-2 ||A'y||."""
     import numpy as np
     import torch
 
-    from mbrec.batch import BatchSolver
     from mbrec.discretization import Discretization
     from mbrec.model import MSOTmodel
     from mbrec.probe import Probe
     from mbrec.warmstart import phantom_batch
 
-    use_sir = a.sir
     N, h = a.grid, 1e-4
     half = (N - 1) / 2 * h
     t0 = time.time()
     disc = Discretization([-half, half], N)
-    probe = Probe("arc256_r40_174", True, False)
-    model = MSOTmodel(probe, disc, 1500.0, use_sir, True, 0, 1e5, 12e6, None, sigma_iterations=40)
-    D, nT = model.shape_sinogram
+    model = MSOTmodel(Probe("arc256_r40_174", True, False), disc, 1500.0, use_sir, True, 0, 1e5, 12e6, None,
+                      sigma_iterations=40)
     B = a.batch
     if a.launch:
         gv, fc = (int(v) for v in a.launch.split(","))
@@ -404,45 +400,28 @@ def main():
         tune = model.plan.autotune(B, 5)
     log(f"setup {time.time() - t0:.1f} s, sigma {model.normalizationFactor:.6g}, "
         f"device tables {model.plan.device_bytes / 1e6:.1f} MB, launch choice {tune}")
-    ph = phantom_batch(disc, B, 1000)
+    ph = phantom_batch(disc, B, seed)
     x_true = torch.tensor(ph).cuda()
     y = model.applyForward(x_true)
     g = torch.Generator(device="cuda").manual_seed(7)
     noise = torch.randn(y.shape, generator=g, device="cuda")
     y = (y + 0.01 * y.abs().amax(dim=(1, 2), keepdim=True) * noise).contiguous()
-    # lambdaL2 = 1e-2 ||A'y|| per sinogram (reconstruction_overview_script.m:47-48)
+    # lambdaL2 = 1e-2 ||A'y|| per sinogram: the configuration's choice (the reference's demo
+    # uses a fixed lambdaL2, reconstruction_overview_script.m:49)
     lam = (1e-2 * torch.linalg.vector_norm(model.applyTranspose(y).reshape(B, -1).double(), dim=1)).cpu().numpy()
-    bs = BatchSolver([(model, [(0, nT)] * B)], a.iters, lam)
-    bs.data[0].copy_(y)
-    torch.cuda.synchronize()
+    return model, ph, x_true, y, lam, tune
 
-    def step():
-        return bs.run(graph=True)
 
-    for _ in range(a.warmup):
-        step()
-    torch.cuda.synchronize()
-    s = torch.cuda.current_stream()
-    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    t_wall = time.perf_counter()
-    ev0.record(s)
-    for _ in range(a.steps):
-        rec = step()[0]
-    ev1.record(s)
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t_wall
-    ms_step = ev0.elapsed_time(ev1) / a.steps
-    value = B / (ms_step / 1e3)
-    rel_err_truth = (torch.linalg.vector_norm((rec - x_true).double()) /
-                     torch.linalg.vector_norm(x_true.double())).item()
-    log(f"{a.steps} steps: {ms_step:.1f} ms/step (wall {wall / a.steps * 1e3:.1f}), {value:.2f} recon/s")
-
-    # ---- roofline of the operator kernels (HIP events on the launch stream) ----
-    # Algorithmic bytes per operator application (one geometry, B sinograms, FP32):
-    # 4 P D travel times (+ 4 P D SIR indices + referenced SIR entries) + 4 B P image
-    # + 4 B D nT sinogram.  Algorithmic flops: 2 B per tap of every pair, about 2R taps
-    # per pair (x the SIR window length with SIR).
-    P = N * N
+def operator_roofline(model, B, use_sir, tune, reps=10):
+    """Roofline object of the dominant operator kernel (by time per solve) for batches of B.
+    Algorithmic bytes per application (FP32, one geometry): 4 P D travel times (+ 4 P D SIR
+    indices + referenced SIR entries) + 4 B P image + 4 B D nT sinogram; algorithmic flops
+    2 B per tap of every pair (about 2R taps per pair, x the SIR window with SIR).  The
+    kernel's time is the average of reps launches, HIP events on the launch stream; the
+    binding term is max(bytes / 8 TB/s, flops / 157.3 TFLOP/s)."""
+    import numpy as np
+    D, nT = model.shape_sinogram
+    P = model.Discretization.numberOfPixels
     sir_bytes = 0
     taps = 2.0 * model.Data.circularInterpolationRadiusInSamples
     if use_sir:
@@ -452,97 +431,300 @@ def main():
         taps *= float(np.mean(np.maximum(se[ref, 1] - se[ref, 0] + 1, 0)))
     alg_bytes = 4 * P * D * (2 if use_sir else 1) + sir_bytes + 4 * B * P + 4 * B * D * nT
     alg_flops = 2.0 * B * P * D * taps
-    reps = 10
     stages = ("scatter", "bins_to_y", "fir", "z_to_bins", "gather")
     st = {k: model.plan.time_stage(k, B, reps) for k in stages}
-    # launches per solve: per outer iteration 3 forward and 2 transpose applications
-    # (LSQR's closing transpose of its last inner iteration is dead and not launched)
-    n_forward, n_transpose = 3 * a.iters, 2 * a.iters
-    per_solve = {"scatter": n_forward * st["scatter"], "bins_to_y": n_forward * st["bins_to_y"],
-                 "fir": (n_forward + n_transpose) * st["fir"], "z_to_bins": n_transpose * st["z_to_bins"],
-                 "gather": n_transpose * st["gather"]}
-    dom = "gather" if per_solve["gather"] >= per_solve["scatter"] else "scatter"
-    t_dom = st[dom]
-    achieved = alg_bytes / (t_dom * 1e-3) / 1e9
-    flops_rate = alg_flops / (t_dom * 1e-3) / 1e12
+    # launches per outer iteration: 3 forward and 2 transpose applications (LSQR's closing
+    # transpose of its last inner iteration is dead and not launched)
+    per_it = {"scatter": 3 * st["scatter"], "bins_to_y": 3 * st["bins_to_y"], "fir": 5 * st["fir"],
+              "z_to_bins": 2 * st["z_to_bins"], "gather": 2 * st["gather"]}
+    dom = "gather" if per_it["gather"] >= per_it["scatter"] else "scatter"
+    t_dom = st[dom] * 1e-3
+    t_bytes, t_flops = alg_bytes / (HBM_PEAK_GBS * 1e9), alg_flops / (FP32_PEAK_TFLOPS * 1e12)
     traffic = None
     tfile = os.path.join(ROOT, "profiles", "traffic.json")
-    key = f"{'sir' if use_sir else 'das'}_{dom}_B{B}_N{N}"
+    key = f"{'sir' if use_sir else 'das'}_{dom}_B{B}_N{model.Discretization.Nx}"
     if os.path.exists(tfile):
-        try:
-            traffic = json.load(open(tfile)).get(key, {}).get("hbm_bytes_per_launch")
-        except Exception:
-            traffic = None
+        traffic = json.load(open(tfile)).get(key, {}).get("hbm_bytes_per_launch")
     fwd_form = "band form" if use_sir and tune[1] == 1004 else "pull form"
     kname = {"gather": "transpose gather", "scatter": f"forward pairs ({fwd_form})"}[dom]
-    t_bytes, t_flops = alg_bytes / (HBM_PEAK_GBS * 1e9), alg_flops / (FP32_PEAK_TFLOPS * 1e12)
-    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                "kernel": f"{kname}, {'SIR' if use_sir else 'DAS'}, B={B}", "algorithmic_bytes": int(alg_bytes),
-                "kernel_ms": round(t_dom, 4),
-                "flops_term": {"algorithmic_flops": alg_flops, "achieved_tflops": round(flops_rate, 2),
-                               "peak_tflops_fp32": FP32_PEAK_TFLOPS,
-                               "frac": round(flops_rate / FP32_PEAK_TFLOPS, 4),
-                               "binding_term": "flops" if t_flops > t_bytes else "bytes",
-                               "frac_of_roofline_time": round(max(t_bytes, t_flops) / (t_dom * 1e-3), 4)},
-                "stage_ms": {k: round(v, 4) for k, v in st.items()},
-                "ms_per_solve_by_stage": {k: round(v, 2) for k, v in per_solve.items()}}
-    # the operator pair at B = 1 (travel-time stream per application)
+    if t_flops >= t_bytes:
+        bound, achieved, peak, unit = "valu", alg_flops / t_dom / 1e12, FP32_PEAK_TFLOPS, "TFLOP/s"
+    else:
+        bound, achieved, peak, unit = "hbm", alg_bytes / t_dom / 1e9, HBM_PEAK_GBS, "GB/s"
+    roof = {"bound": bound, "achieved": round(achieved, 2), "peak": peak, "unit": unit,
+            "frac": round(achieved / peak, 4), "traffic": traffic,
+            "kernel": f"{kname}, {'SIR' if use_sir else 'DAS'}, B={B}", "kernel_ms": round(st[dom], 4),
+            "algorithmic_bytes": int(alg_bytes), "algorithmic_flops": alg_flops,
+            "bytes_term": {"GB/s": round(alg_bytes / t_dom / 1e9, 1),
+                           "frac": round(alg_bytes / t_dom / 1e9 / HBM_PEAK_GBS, 4)},
+            "flops_term": {"TFLOP/s": round(alg_flops / t_dom / 1e12, 2),
+                           "frac": round(alg_flops / t_dom / 1e12 / FP32_PEAK_TFLOPS, 4)},
+            "stage_ms": {k: round(v, 4) for k, v in st.items()},
+            "ms_per_outer_iteration_by_stage": {k: round(v, 3) for k, v in per_it.items()}}
+    # the operator pair for one sinogram (travel-time stream per application)
     t1 = {k: model.plan.time_stage(k, 1, reps) for k in ("scatter", "gather")}
     b1 = 4 * P * D * (2 if use_sir else 1) + sir_bytes + 4 * P + 4 * D * nT
-    roofline["operator_pair_B1"] = {k: {"ms": round(v, 4), "GB/s": round(b1 / (v * 1e-3) / 1e9, 1),
-                                        "frac": round(b1 / (v * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
-                                    for k, v in t1.items()}
-    log("stages (ms, B=%d): %s; %s %.0f GB/s, %.2f TFLOP/s; B=1 %s" % (
-        B, " ".join(f"{k} {v:.3f}" for k, v in st.items()), dom, achieved, flops_rate,
+    roof["operator_pair_B1"] = {k: {"ms": round(v, 4), "GB/s": round(b1 / (v * 1e-3) / 1e9, 1),
+                                    "frac": round(b1 / (v * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+                                for k, v in t1.items()}
+    log("stages (ms, B=%d): %s; %s %s %.2f %s (%.1f %%); B=1 %s" % (
+        B, " ".join(f"{k} {v:.3f}" for k, v in st.items()), dom, bound, achieved, unit, 100 * achieved / peak,
         " ".join(f"{k} {v:.3f} ms" for k, v in t1.items())))
+    return roof
 
-    # ---- CPU baseline: FP64 oracle on a bounded sample --------------------------
-    cpu = None
-    if not a.no_cpu_baseline:
-        sys.path.insert(0, os.path.join(ROOT, "oracle"))
-        import oracle as O
-        kz, kx = O.full_kernels(N, N, disc.dz, disc.dx, disc.pixelSigma)
-        kw = dict(kz=kz, kx=kx, ir=model.Data.IR)
-        if use_sir:
-            kw.update(sidx=model.Data.SPMRIndices, sir=model.Data.SPMR,
-                      se=model.Data.SPMRTimestepIndexStartAndEnd)
-        orc = O.Model(N, N, model.Data.absoluteTravelTimeIndices.reshape(D, -1), nT,
-                      model.Data.circularInterpolationRadiusInSamples, sigma=model.normalizationFactor, **kw)
-        x0 = ph[:1].astype(np.float64)
-        y0 = y[:1].double().cpu().numpy()
-        tc = time.perf_counter()
-        for _ in range(a.cpu_sample_reps):
-            orc.forward(x0)
-        t_f = (time.perf_counter() - tc) / a.cpu_sample_reps
-        tc = time.perf_counter()
-        for _ in range(a.cpu_sample_reps):
-            orc.transpose(y0)
-        t_a = (time.perf_counter() - tc) / a.cpu_sample_reps
-        t_solve = 151 * t_f + 150 * t_a
-        cpu = {"value": round(1.0 / t_solve, 6), "unit": "recon/s", "cores": O.num_threads(), "kind": "port",
-               "sample": f"{a.cpu_sample_reps} forward + {a.cpu_sample_reps} transpose FP64 applications of the "
-                         f"same {N}x{N} model (A: {t_f:.2f} s, A': {t_a:.2f} s), scaled to one 50-iteration "
-                         f"solve = 151 A + 150 A'"}
-        log(f"cpu baseline: A {t_f:.2f} s, A' {t_a:.2f} s -> {1 / t_solve:.5f} recon/s on {O.num_threads()} threads")
 
+def cpu_solve_rate(model, ph, y, use_sir, reps, iters):
+    """FP64 oracle (the MEX loops' order and OpenMP decomposition) on the host cores:
+    reps forward and reps transpose applications of one sinogram, scaled to one solve of
+    ``iters`` outer iterations at the reference's count of 3 A + 3 A' per iteration."""
+    import numpy as np
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    D, nT = model.shape_sinogram
+    disc = model.Discretization
+    kz, kx = O.full_kernels(disc.Nz, disc.Nx, disc.dz, disc.dx, disc.pixelSigma)
+    kw = dict(kz=kz, kx=kx, ir=model.Data.IR)
+    if use_sir:
+        kw.update(sidx=model.Data.SPMRIndices, sir=model.Data.SPMR, se=model.Data.SPMRTimestepIndexStartAndEnd)
+    orc = O.Model(disc.Nz, disc.Nx, model.Data.absoluteTravelTimeIndices.reshape(D, -1), nT,
+                  model.Data.circularInterpolationRadiusInSamples, sigma=model.normalizationFactor, **kw)
+    x0 = ph[:1].astype(np.float64)
+    y0 = y[:1].double().cpu().numpy()
+    tc = time.perf_counter()
+    for _ in range(reps):
+        orc.forward(x0)
+    t_f = (time.perf_counter() - tc) / reps
+    tc = time.perf_counter()
+    for _ in range(reps):
+        orc.transpose(y0)
+    t_a = (time.perf_counter() - tc) / reps
+    t_solve = iters * 3 * (t_f + t_a)
+    log(f"cpu baseline: A {t_f:.2f} s, A' {t_a:.2f} s -> {1 / t_solve:.3g} recon/s on {O.num_threads()} threads")
+    return {"value": 1.0 / t_solve, "unit": "recon/s", "cores": O.num_threads(), "kind": "port",
+            "sample": f"{reps} forward + {reps} transpose FP64 applications of the same {disc.Nx}x{disc.Nz} "
+                      f"model on one sinogram (A: {t_f:.3f} s, A': {t_a:.3f} s), scaled to one solve of "
+                      f"{iters} outer iterations at 3 A + 3 A' each"}
+
+
+def events_ms(fn, steps, warmup):
+    import torch
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    s = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(s)
+    out = None
+    for _ in range(steps):
+        out = fn()
+    e1.record(s)
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / steps, out
+
+
+def metric_bench(a):
+    """The headline metric: recon/s of batch-64 NNLS-L2 at 400^2 x 256 ch reaching 1e-3
+    relative L2 of the converged solution, learned warm start vs plain cold start."""
+    import numpy as np
+    import torch
+
+    from mbrec.solvers import rec_nn_with_L2_reg_batch
+    from mbrec.warmstart import UNet, iterations_to_tolerance, nnls_l2_minimiser, warm_start_estimate
+
+    weights = a.weights or os.path.join(ROOT, "mb-rec-msot_amd", "mbrec", "weights", f"unet_state_{a.grid}.npz")
+    if not os.path.exists(weights):
+        raise SystemExit(f"warm-start weights {weights} missing (python -m mbrec.train_warmstart --out ...)")
+    model, ph, x_true, y, lam, tune = build_problem(a, False)
+    B = a.batch
+    D, nT = model.shape_sinogram
+    tol = 1e-3
+    t0 = time.time()
+    ref, kkt = nnls_l2_minimiser(model, y, lam)
+    log(f"converged solution: KKT residual {kkt:.2e} ({time.time() - t0:.1f} s)")
+    nref = torch.linalg.vector_norm(ref.reshape(B, -1), dim=1)
+    worst = lambda r: (torch.linalg.vector_norm((r.double() - ref).reshape(B, -1), dim=1) / nref).max().item()
+    net = UNet.load(weights)
+    # iteration counts (untimed; chunks of 25 that continue from the full loop state)
+    k_plain, errs_plain = iterations_to_tolerance(model, y, lam, ref, tol, chunk=a.chunk)
+    log(f"plain cold start: {k_plain} outer iterations to {tol:g}")
+    w0, v0, used = warm_start_estimate(net, model, y, lam)
+    k_learned, errs_learned = iterations_to_tolerance(model, y, lam, ref, tol, w0, v0, chunk=a.chunk)
+    log(f"learned warm start: {k_learned} outer iterations to {tol:g} ({int(used.sum())}/{B} items from the network)")
+    if k_plain is None or k_learned is None:
+        raise SystemExit("a solve did not reach the tolerance")
+
+    def learned_step():
+        w, v, _ = warm_start_estimate(net, model, y, lam)
+        return rec_nn_with_L2_reg_batch(model, y, k_learned, lam, init=w, init_dual=v)
+
+    def plain_step():
+        return rec_nn_with_L2_reg_batch(model, y, k_plain, lam)
+
+    t_wall = time.perf_counter()
+    ms_learned, rec_l = events_ms(learned_step, a.steps, a.warmup)
+    wall = (time.perf_counter() - t_wall) * 1e3
+    ms_plain, rec_p = events_ms(plain_step, 1, 0)
+    e_learned, e_plain = worst(rec_l), worst(rec_p)
+    value = B / (ms_learned / 1e3)
+    plain_value = B / (ms_plain / 1e3)
+    log(f"learned {ms_learned:.1f} ms per batch ({value:.2f} recon/s, worst rel.err {e_learned:.2e}); plain "
+        f"{ms_plain:.1f} ms ({plain_value:.2f} recon/s, {e_plain:.2e}); speed-up {value / plain_value:.2f}x")
+    roof = operator_roofline(model, B, False, tune)
+    cpu = None if a.no_cpu_baseline else cpu_solve_rate(model, ph, y, False, a.cpu_sample_reps, k_plain)
+    out = {"metric": METRIC, "value": round(value, 4), "unit": "recon/s", "n_gpus": 1, "steps": a.steps,
+           "warmup": a.warmup, "ms_per_step": round(ms_learned, 3), "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+           "config": {"workload": f"batch {B} independent rec_nn_with_L2_reg solves (RegL2 = identity, "
+                                  f"lambdaL2 = 1e-2 ||A'y||), {a.grid}x{a.grid} px at 100 um, no SIR, {D} ch x {nT} "
+                                  f"samples, each to rel.err <= {tol:g} of the converged solution; learned "
+                                  f"warm start (U-Net state estimate) vs plain cold start",
+                      "batch": B, "grid": a.grid, "detectors": D, "samples": nT, "sir": False, "tol": tol},
+           "learned": {"recon_per_s": round(value, 4), "outer_iterations": k_learned, "worst_rel_err": e_learned,
+                       "items_from_network": int(used.sum()), "ms_per_batch": round(ms_learned, 2),
+                       "wall_ms_per_batch": round(wall / max(a.steps + a.warmup, 1), 2)},
+           "plain": {"recon_per_s": round(plain_value, 4), "outer_iterations": k_plain, "worst_rel_err": e_plain,
+                     "ms_per_batch": round(ms_plain, 2)},
+           "speedup_learned_vs_plain": round(value / plain_value, 3),
+           "reference_solution": {"method": "converged NNLS-L2 minimiser (accelerated projected gradient)",
+                                  "kkt_residual": kkt},
+           "convergence": {"plain": {str(k): v for k, v in errs_plain.items() if k % 100 == 0 or k == k_plain},
+                           "learned": {str(k): v for k, v in errs_learned.items()}},
+           "weights": os.path.relpath(weights, ROOT),
+           "roofline": roof, "cpu_baseline": cpu}
+    print(json.dumps(out), flush=True)
+
+
+def fixed_bench(a):
+    """Fixed-iteration plain solve (--fixed-iters N): one captured graph of N outer
+    iterations for the batch, recon/s = B / time per batch (the SIR configuration uses it)."""
+    import numpy as np
+    import torch
+
+    from mbrec.batch import BatchSolver
+
+    use_sir = a.sir
+    model, ph, x_true, y, lam, tune = build_problem(a, use_sir)
+    B = a.batch
+    D, nT = model.shape_sinogram
+    bs = BatchSolver([(model, [(0, nT)] * B)], a.fixed_iters, lam)
+    bs.data[0].copy_(y)
+    torch.cuda.synchronize()
+    ms_step, res = events_ms(lambda: bs.run(graph=True), a.steps, a.warmup)
+    rec = res[0]
+    value = B / (ms_step / 1e3)
+    log(f"{a.steps} steps: {ms_step:.1f} ms/step, {value:.2f} recon/s")
+    roof = operator_roofline(model, B, use_sir, tune)
+    cpu = None if a.no_cpu_baseline else cpu_solve_rate(model, ph, y, use_sir, a.cpu_sample_reps, a.fixed_iters)
     if a.traffic_calibration:
         xc = torch.empty(CALIB_BYTES // 4, dtype=torch.float32, device="cuda").fill_(1.0)
         torch.cuda.synchronize()
         yc = xc.clone()
         torch.cuda.synchronize()
         del xc, yc
-    out = {"metric": METRIC, "value": round(value, 4), "unit": "recon/s", "n_gpus": 1, "steps": a.steps,
+    out = {"metric": f"NNLS-L2 recons/sec, {a.fixed_iters} outer iterations (plain cold start)",
+           "value": round(value, 4), "unit": "recon/s", "n_gpus": 1, "steps": a.steps,
            "warmup": a.warmup, "ms_per_step": round(ms_step, 3), "higher_is_better": True, "scaling": "weak",
            "vs_baseline": None, "dtype": "f32", "data": "synthetic",
-           "config": {"workload": f"rec_nn_with_L2_reg {a.iters} it, {N}x{N} px at 100 um, "
+           "config": {"workload": f"rec_nn_with_L2_reg {a.fixed_iters} it, {a.grid}x{a.grid} px at 100 um, "
                                   f"{'SIR' if use_sir else 'no SIR'}, {D} ch x {nT} samples, batch {B} "
                                   "independent seeded sinograms (plain cold start)",
-                      "batch": B, "grid": N, "detectors": D, "samples": nT, "sir": use_sir,
-                      "outer_iterations": a.iters, "lambdaL2_median": float(np.median(lam))},
-           "rel_err_vs_phantom": round(rel_err_truth, 5),
-           "roofline": roofline, "cpu_baseline": cpu}
+                      "batch": B, "grid": a.grid, "detectors": D, "samples": nT, "sir": use_sir,
+                      "outer_iterations": a.fixed_iters, "lambdaL2_median": float(np.median(lam))},
+           "rel_err_vs_phantom": round((torch.linalg.vector_norm((rec - x_true).double()) /
+                                        torch.linalg.vector_norm(x_true.double())).item(), 5),
+           "roofline": roof, "cpu_baseline": cpu}
     print(json.dumps(out), flush=True)
+
+
+def mex_bench(a):
+    """Per-call wall time of the four MEX drop-ins at 400^2 / 256 x 2030 (one wavelength,
+    host FP64 in and out as MATLAB passes them), the tables cached after the first call,
+    next to the FP64 oracle's loops (the reference MEX loops' order and OpenMP
+    decomposition) on the host cores.  Prints one JSON line."""
+    import numpy as np
+    import torch
+
+    from mbrec import mex
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    from mbrec.discretization import Discretization
+    from mbrec.model import MSOTmodel
+    from mbrec.probe import Probe
+
+    N, h = a.grid, 1e-4
+    half = (N - 1) / 2 * h
+    model = MSOTmodel(Probe("arc256_r40_174", True, False), Discretization([-half, half], N), 1500.0, True, True,
+                      0, 1e5, 12e6, 1.0)
+    D, nT = model.shape_sinogram
+    P = N * N
+    R = float(model.Data.circularInterpolationRadiusInSamples)
+    # MATLAB layouts as F-ordered views: [P x D] tables, [Ls x Nsir] SIR table, [2 x Nsir] windows
+    tt = model.Data.absoluteTravelTimeIndices.reshape(D, P).T
+    sidx = np.ascontiguousarray(model.Data.SPMRIndices.reshape(D, P), dtype=np.int32).T
+    sir = np.ascontiguousarray(model.Data.SPMR, dtype=np.float64).T
+    se = np.ascontiguousarray(model.Data.SPMRTimestepIndexStartAndEnd, dtype=np.int32).T
+    g = np.random.default_rng(5)
+    img = np.asfortranarray(g.uniform(0, 1, (1, P)))
+    sig = np.asfortranarray(g.standard_normal((1, nT, D)))
+    nt = np.int32(nT)
+    calls = {
+        "forwardDelayAndSumMex": lambda: mex.forwardDelayAndSumMex(img, tt, nt, R),
+        "transposeDelayAndSumMex": lambda: mex.transposeDelayAndSumMex(sig, tt, R),
+        "forwardMex": lambda: mex.forwardMex(img, sidx, tt, sir, nt, se, R),
+        "transposeMex": lambda: mex.transposeMex(sig, sidx, tt, sir, se, R),
+    }
+    orc_calls = {
+        "forwardDelayAndSumMex": lambda: O.forward_das(img, tt, nT, R),
+        "transposeDelayAndSumMex": lambda: O.transpose_das(sig, tt, R),
+        "forwardMex": lambda: O.forward_sir(img, sidx, tt, sir, nT, se, R),
+        "transposeMex": lambda: O.transpose_sir(sig, sidx, tt, sir, se, R),
+    }
+    res = {}
+    for name, fn in calls.items():
+        t0 = time.perf_counter()
+        out = fn()                      # first call: plan and table upload
+        first = (time.perf_counter() - t0) * 1e3
+        ts = []
+        for _ in range(a.mex_reps):
+            t0 = time.perf_counter()
+            fn()
+            ts.append((time.perf_counter() - t0) * 1e3)
+        t0 = time.perf_counter()
+        want = orc_calls[name]()
+        t_orc = (time.perf_counter() - t0) * 1e3
+        err = float(np.linalg.norm((out - want).ravel()) / np.linalg.norm(want.ravel()))
+        res[name] = {"first_call_ms": round(first, 1), "per_call_ms_median": round(float(np.median(ts)), 3),
+                     "per_call_ms_min": round(float(np.min(ts)), 3), "oracle_ms": round(t_orc, 1),
+                     "oracle_cores": O.num_threads(), "rel_l2_vs_oracle": err}
+        log(f"{name}: first {first:.1f} ms, then {np.median(ts):.3f} ms per call (oracle {t_orc:.1f} ms), "
+            f"rel.err {err:.2e}")
+    print(json.dumps({"mex_drop_ins": {"grid": N, "detectors": D, "samples": nT, "wavelengths": 1,
+                                       "reps": a.mex_reps, "calls": res}}), flush=True)
+
+
+def main():
+    a = parse()
+    if a.traffic_from:
+        traffic_from(*a.traffic_from, N=a.grid, B=a.batch)
+        return
+    if a.gpus != 1:
+        print(f"bench.py runs on one GPU only; --gpus {a.gpus} is not supported in this build "
+              "(batches of independent solves shard across GPUs with no exchange; see DESIGN.md)",
+              flush=True)
+        sys.exit(2)
+    if a.mex:
+        mex_bench(a)
+    elif a.mixed:
+        mixed(a)
+    elif a.tv600:
+        tv600(a)
+    elif a.secondary:
+        secondary(a)
+    elif a.fixed_iters or a.sir:
+        a.fixed_iters = a.fixed_iters or 50
+        fixed_bench(a)
+    else:
+        metric_bench(a)
 
 
 if __name__ == "__main__":

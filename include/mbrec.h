@@ -171,6 +171,7 @@ int mbrec_model_transpose_window_dev(mbrec_plan plan, const float *sino, float *
                                      const int32_t *windows, mbrec_stream s);
 
 /* rec_nn_with_L2_reg (reconstruction/rec_nn_with_L2_reg.m:1-66) with RegL2 = identity
+ * (mbrec_rec_nn_with_L2_reg_terms_dev below takes other L2 terms)
  * and nnls_L2_less (nnls_L2_less.m:1-101): 50-style outer splitting, 2 LSQR inner
  * iterations, lambda = 1, beta = 1e-3.  data [B][D][nT], rec [B][P].
  * coupled != 0 solves the B sinograms as one stacked problem with shared LSQR
@@ -182,14 +183,49 @@ int mbrec_rec_nn_with_L2_reg_dev(mbrec_plan plan, const float *data, int B, int 
                                  int num_iter, double lambdaL2, const float *init,
                                  float *rec, double *lcurve, mbrec_stream s);
 
+/* rec_nn_with_L2_reg with general L2 terms (rec_nn_with_L2_reg.m:1-66,
+ * incorporate_l2_reg_into_model_functions.m:1-21, combine_forward_and_transpose_with_reg.m:6-14,
+ * combine_forward_and_transpose_with_two_regs.m:8-14): objective ||A x - y||^2 +
+ * lambdaL2 ||R1 x||^2 + lambdaL2_2 ||R2 x||^2 over x >= 0, the stacked operator
+ * [A; sqrt(lambdaL2) R1; sqrt(lambdaL2_2) R2] inside nnls_L2_less.  terms[0] = (lambdaL2,
+ * RegL2), terms[1] = (lambdaL2_2, RegL2_2); as in the reference the second term enters
+ * only when lambdaL2 > 0.  Each term's operator maps one image [P] to one image [P], per
+ * sinogram: MBREC_REG_IDENTITY (RegL2 = @(x) x), MBREC_REG_DEL2 (MATLAB del2(img, 1) per
+ * image, used as its own transpose like the reference's laplacian_per_wavelength,
+ * reconstruction_overview_script.m:50-51), or MBREC_REG_CALLBACK: apply(ctx, in, out, B, P,
+ * transp, s) writes out = R in (transp = 0) or R' in (transp = 1) for B images [B][P] on
+ * stream s, enqueuing work only (it may then be captured) and returning 0.  lcurve (host,
+ * nullable) receives per group {||A rec - y||^2, ||R1 rec||^2, ||R2 rec||^2} (0 for an
+ * absent term).  Other arguments as mbrec_rec_nn_with_L2_reg_dev. */
+#define MBREC_REG_IDENTITY 0
+#define MBREC_REG_DEL2 1
+#define MBREC_REG_CALLBACK 2
+typedef int (*mbrec_reg_apply)(void *ctx, const float *in, float *out, int B, int P, int transp, mbrec_stream s);
+typedef struct {
+    double lambda;
+    int kind;
+    mbrec_reg_apply apply;
+    void *ctx;
+} mbrec_l2_term;
+int mbrec_rec_nn_with_L2_reg_terms_dev(mbrec_plan plan, const float *data, int B, int coupled, int num_iter,
+                                       const mbrec_l2_term *terms, int nterms, const float *init, float *rec,
+                                       double *lcurve, mbrec_stream s);
+
+/* laplacian_per_wavelength (reconstruction/laplacian_per_wavelength.m:1-10): MATLAB
+ * del2(img, 1) of B device images [B][Nx][Nz] (z fastest), out = in not allowed. */
+int mbrec_del2_dev(const float *in, float *out, int Nz, int Nx, int B, mbrec_stream s);
+
 /* Independent solves of B sinograms with a lambdaL2 of their own (host array
  * [B]), as B separate rec_nn_with_L2_reg.m calls would use; otherwise as
  * mbrec_rec_nn_with_L2_reg_dev with coupled = 0 (lcurve: 2 values per sinogram).
  * init / init_dual (may be NULL): starting outer iterate w and scaled dual v of
- * nnls_L2_less.m:28-29 (zero there) -- the warm start of a learned estimate. */
+ * nnls_L2_less.m:28-29 (zero there) -- the warm start of a learned estimate.
+ * final_dual (may be NULL) receives the final scaled dual v [B][P]: (rec, final_dual)
+ * is the loop's whole state, so passing them back as init / init_dual continues the
+ * same iteration sequence. */
 int mbrec_rec_nn_with_L2_reg_batch_dev(mbrec_plan plan, const float *data, int B, int num_iter,
                                        const double *lambdaL2, const float *init, const float *init_dual,
-                                       float *rec, double *lcurve, mbrec_stream s);
+                                       float *rec, float *final_dual, double *lcurve, mbrec_stream s);
 /* rec_nn_with_L1_eye_reg (reconstruction/rec_nn_with_L1_eye_reg.m:1-50) and
  * nnls_L1_eye_matrix_less (nnls_L1_eye_matrix_less.m:1-67): split Bregman with
  * 2 CGS iterations of (A'A + I) per outer iteration, lambdaParam = 1, etaReg =
@@ -238,6 +274,27 @@ int mbrec_bp_kernel_dev(const double *t, int Nt, double *K, mbrec_stream s);
  * (device, the not-a-knot spline map), for nbatch sinograms [nbatch][D][nT]. */
 int mbrec_upsample_detectors_dev(const double *U, int Dup, int D, const double *sig, int nT, int nbatch,
                                  double *out, mbrec_stream s);
+
+/* ---- 4. Preprocessing (load_and_preprocess/, FP64) ------------------------- */
+
+/* filter_butter_zero_phase (load_and_preprocess/filter_butter_zero_phase.m:1-53, FFT
+ * branch): sig, out: device FP64 [ncols][nT] (time fastest, MATLAB [nT x cols]); demean
+ * along time when fmin > 0; spectrum times |H|^2 of the 8th-order Butterworth low-pass at
+ * fmax (if > 0) and 2nd-order high-pass at fmin (if > 0) at the nT bins of the whole
+ * circle, as a circular convolution.  out must not alias sig. */
+int mbrec_filter_butter_zero_phase_dev(const double *sig, int nT, int ncols, double fs, double fmin, double fmax,
+                                       double *out, mbrec_stream s);
+/* apply_butterworth_window_to_sinogram (apply_butterworth_window_to_sinogram.m:1-10), in
+ * place on device FP64 [ncols][nT]: s = mean + (s - mean) |h(pi t / nT)|, h the order
+ * n_butter digital Butterworth band-pass [low_cutoff, high_cutoff] / nT. */
+int mbrec_apply_butterworth_window_dev(double *sig, int nT, int ncols, int n_butter, double low_cutoff,
+                                       double high_cutoff, mbrec_stream s);
+/* interpolate_signals_of_broken_transducers (interpolate_signals_of_broken_transducers.m:1-20):
+ * sig, out device FP64 [nbatch][D][nT]; broken (host, 1-based, nbroken entries): each
+ * detector's signal becomes the not-a-knot spline across the working detectors evaluated
+ * at its index (interp2 'spline'), per time sample.  out must not alias sig. */
+int mbrec_interpolate_broken_transducers_dev(const double *sig, int D, int nT, int nbatch, const int32_t *broken,
+                                             int nbroken, double *out, mbrec_stream s);
 
 /* ---- batched solves over several geometries, one graph launch -------------
  * No counterpart in the reference (it reconstructs one acquisition per call).

@@ -165,6 +165,10 @@ int op_forward(Plan *p, const float *img, float *sino, int B, bool model, hipStr
                const int2 *win = nullptr);
 int op_transpose(Plan *p, const float *sino, float *img, int B, bool model, hipStream_t s,
                  const int2 *win = nullptr);
+// op_transpose on a workspace that is already in place (p->d_sino_tmp, d_bins with its two
+// zero rows, d_img_tmp of 2 * bcap * P, d_gpart): launches kernels only
+int op_transpose_run(Plan *p, const float *sino, float *img, int B, bool model, hipStream_t s,
+                     const int2 *win = nullptr);
 
 }  // namespace mbrec
 

@@ -2,17 +2,25 @@
 //
 // Each entry point has the argument order and layouts of its reference MEX
 // file.  Tables (travel times, SIR indices, SIR table, windows) are uploaded
-// once and cached, keyed by their dimensions and a 64-bit hash of their full
-// contents; images and sinograms are converted to FP32 planar layout, the
-// device operator runs, and the result is widened back to FP64 in the MEX
-// layout.  The per-call host<->device copies make this path PCIe-bound; the
-// device API (mbrec_*_dev) is the one that runs at HBM speed.
+// once per table and cached.  A cache entry is found by the tables' host
+// addresses, dimensions and R plus a fingerprint of each table: a hash of
+// kFpSamples evenly strided elements and of the first and last kFpEdge elements
+// (about 0.3 MB read per call whatever the table size, instead of the 328 + 164 MB
+// a full hash reads at 400^2).  MATLAB gives a modified array a new data address
+// unless it is edited in place while unshared; such an in-place edit of a cached
+// table is seen only where it touches a sampled element, so callers that edit a
+// table in place drop the cache (mbrec_mex_cache_clear, "clear mex").
+//
+// The image or sinogram is copied to the device as FP64 in the MEX layout and
+// permuted / narrowed to the device's planar FP32 layout there (and back), in
+// staging buffers that live with the cache entry and grow only when a call needs
+// more: after the first call with a given size a call allocates nothing.
 #include "mbrec_internal.h"
 
 #include <cstring>
+#include <list>
 #include <mutex>
 #include <vector>
-#include <list>
 
 namespace mbrec {
 
@@ -46,53 +54,144 @@ static uint64_t hash_bytes(const void *data, size_t nbytes, uint64_t seed)
     return r * M;
 }
 
+static const size_t kFpSamples = 32768;   // strided elements per table
+static const size_t kFpEdge = 1024;       // leading and trailing elements per table
+
+// fingerprint of an array of n elements of esize bytes: its address, size and a hash of
+// the strided sample and both ends
+static uint64_t fingerprint(const void *data, size_t n, size_t esize, uint64_t seed)
+{
+    const uint8_t *p = (const uint8_t *)data;
+    uint64_t h = hash_bytes(&data, sizeof(data), seed ^ (uint64_t)n * 0x100000001B3ull);
+    const size_t edge = std::min(n, kFpEdge);
+    h = hash_bytes(p, edge * esize, h);
+    h = hash_bytes(p + (n - edge) * esize, edge * esize, h);
+    if (n > 2 * kFpEdge) {
+        const size_t stride = std::max<size_t>(1, n / kFpSamples);
+        std::vector<uint8_t> buf;
+        buf.reserve((n / stride + 1) * esize);
+        for (size_t i = 0; i < n; i += stride) buf.insert(buf.end(), p + i * esize, p + (i + 1) * esize);
+        h = hash_bytes(buf.data(), buf.size(), h);
+    }
+    return h;
+}
+
 struct CacheEntry {
     uint64_t key;
     int P, D, nT;
     double R;
     mbrec_plan plan;
+    double *d_host_layout = nullptr;   // FP64 image / sinogram in the MEX layout
+    float *d_img = nullptr;            // planar FP32 [W][P]
+    float *d_sino = nullptr;           // planar FP32 [W][D][nT]
+    int wcap = 0;                      // staging capacity in wavelengths
 };
+
+static void free_entry(CacheEntry &e)
+{
+    if (e.d_host_layout) (void)hipFree(e.d_host_layout);
+    if (e.d_img) (void)hipFree(e.d_img);
+    if (e.d_sino) (void)hipFree(e.d_sino);
+    e.d_host_layout = nullptr;
+    e.d_img = e.d_sino = nullptr;
+    e.wcap = 0;
+    mbrec_plan_destroy(e.plan);
+}
 
 static std::mutex g_cache_mu;
 static std::list<CacheEntry> g_cache;   // most recently used first
 static const size_t kCacheMax = 4;
 
-static int get_plan(int P, int D, int nT, double R, const double *tt, const int32_t *sidx,
-                    const double *sir, int Ls, int Nsir, const int32_t *se, mbrec_plan *out)
+// the cache entry of these tables, with staging for W wavelengths; the returned entry
+// stays valid while g_cache_mu is held by the caller
+static int get_entry(int P, int D, int nT, double R, const double *tt, const int32_t *sidx, const double *sir,
+                     int Ls, int Nsir, const int32_t *se, int W, CacheEntry **out)
 {
-    uint64_t key = hash_bytes(tt, sizeof(double) * (size_t)P * D, 1469598103934665603ull);
+    uint64_t key = fingerprint(tt, (size_t)P * D, sizeof(double), 1469598103934665603ull);
     if (sidx) {
-        key = hash_bytes(sidx, sizeof(int32_t) * (size_t)P * D, key);
-        key = hash_bytes(sir, sizeof(double) * (size_t)Ls * Nsir, key ^ (uint64_t)Ls);
-        key = hash_bytes(se, sizeof(int32_t) * 2 * (size_t)Nsir, key ^ (uint64_t)Nsir);
+        key = fingerprint(sidx, (size_t)P * D, sizeof(int32_t), key);
+        key = fingerprint(sir, (size_t)Ls * Nsir, sizeof(double), key ^ (uint64_t)Ls);
+        key = fingerprint(se, 2 * (size_t)Nsir, sizeof(int32_t), key ^ (uint64_t)Nsir);
     }
-    std::lock_guard<std::mutex> lk(g_cache_mu);
+    CacheEntry *e = nullptr;
     for (auto it = g_cache.begin(); it != g_cache.end(); ++it)
         if (it->key == key && it->P == P && it->D == D && it->nT == nT && it->R == R) {
             g_cache.splice(g_cache.begin(), g_cache, it);
-            *out = it->plan;
-            return MBREC_OK;
+            e = &g_cache.front();
+            break;
         }
-    mbrec_plan plan = nullptr;
-    int rc = mbrec_plan_create(&plan, P, 1, D, nT, R, tt);
-    if (rc) return rc;
-    if (sidx && (rc = mbrec_plan_set_sir(plan, sidx, sir, Ls, Nsir, se))) {
-        mbrec_plan_destroy(plan);
-        return rc;
+    if (!e) {
+        mbrec_plan plan = nullptr;
+        int rc = mbrec_plan_create(&plan, P, 1, D, nT, R, tt);
+        if (rc) return rc;
+        if (sidx && (rc = mbrec_plan_set_sir(plan, sidx, sir, Ls, Nsir, se))) {
+            mbrec_plan_destroy(plan);
+            return rc;
+        }
+        g_cache.push_front(CacheEntry{key, P, D, nT, R, plan});
+        while (g_cache.size() > kCacheMax) {
+            free_entry(g_cache.back());
+            g_cache.pop_back();
+        }
+        e = &g_cache.front();
     }
-    g_cache.push_front(CacheEntry{key, P, D, nT, R, plan});
-    while (g_cache.size() > kCacheMax) {
-        mbrec_plan_destroy(g_cache.back().plan);
-        g_cache.pop_back();
+    if (W > e->wcap) {
+        if (e->d_host_layout) {
+            MBREC_HIP(hipDeviceSynchronize());
+            (void)hipFree(e->d_host_layout);
+            (void)hipFree(e->d_img);
+            (void)hipFree(e->d_sino);
+            e->d_host_layout = nullptr;
+            e->d_img = e->d_sino = nullptr;
+            e->wcap = 0;
+        }
+        const size_t nimg = (size_t)W * P, nsino = (size_t)W * D * nT;
+        MBREC_HIP(hipMalloc((void **)&e->d_host_layout, std::max(nimg, nsino) * sizeof(double)));
+        MBREC_HIP(hipMalloc((void **)&e->d_img, nimg * sizeof(float)));
+        MBREC_HIP(hipMalloc((void **)&e->d_sino, nsino * sizeof(float)));
+        e->wcap = W;
     }
-    *out = plan;
+    *out = e;
     return MBREC_OK;
 }
 
-struct DevBuf {
-    float *p = nullptr;
-    ~DevBuf() { if (p) (void)hipFree(p); }
-};
+// [P x W] FP64 (wavelength fastest) <-> planar FP32 [W][P]
+__global__ void k_mex_img_in(const double *__restrict__ a, float *__restrict__ b, int W, size_t n)
+{
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const size_t x = i / W, w = i - x * W;
+        b[w * (n / W) + x] = (float)a[i];
+    }
+}
+
+__global__ void k_mex_img_out(const float *__restrict__ b, double *__restrict__ a, int W, size_t n)
+{
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const size_t x = i / W, w = i - x * W;
+        a[i] = (double)b[w * (n / W) + x];
+    }
+}
+
+// [W x nT x D] FP64 (wavelength fastest) <-> planar FP32 [W][D][nT]
+__global__ void k_mex_sino_in(const double *__restrict__ a, float *__restrict__ b, int W, int nT, int D)
+{
+    const size_t n = (size_t)W * nT * D;
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const size_t w = i % W, r = i / W, t = r % nT, d = r / nT;
+        b[(w * D + d) * nT + t] = (float)a[i];
+    }
+}
+
+__global__ void k_mex_sino_out(const float *__restrict__ b, double *__restrict__ a, int W, int nT, int D)
+{
+    const size_t n = (size_t)W * nT * D;
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const size_t w = i % W, r = i / W, t = r % nT, d = r / nT;
+        a[i] = (double)b[(w * D + d) * nT + t];
+    }
+}
+
+static dim3 grid_of(size_t n) { return dim3((unsigned)std::min<size_t>((n + 255) / 256, 16384)); }
 
 static int run_forward(int W, int P, int D, int nT, double R, const double *image, const double *tt,
                        const int32_t *sidx, const double *sir, int Ls, int Nsir, const int32_t *se,
@@ -102,25 +201,18 @@ static int run_forward(int W, int P, int D, int nT, double R, const double *imag
         return set_error(MBREC_ERR_ARG, "forwardMex:args", "invalid dimensions or null buffer");
     int rc = mbrec_init(0);
     if (rc) return rc;
-    mbrec_plan plan;
-    if ((rc = get_plan(P, D, nT, R, tt, sidx, sir, Ls, Nsir, se, &plan))) return rc;
-    // image [W x P] -> planar [W][P]
-    std::vector<float> img((size_t)W * P);
-    for (int x = 0; x < P; ++x)
-        for (int w = 0; w < W; ++w) img[(size_t)w * P + x] = (float)image[(size_t)x * W + w];
-    DevBuf di, ds;
-    const size_t nS = (size_t)W * D * nT;
-    MBREC_HIP(hipMalloc((void **)&di.p, img.size() * sizeof(float)));
-    MBREC_HIP(hipMalloc((void **)&ds.p, nS * sizeof(float)));
-    MBREC_HIP(hipMemcpy(di.p, img.data(), img.size() * sizeof(float), hipMemcpyHostToDevice));
-    if ((rc = op_forward(&plan->p, di.p, ds.p, W, false, nullptr))) return rc;
-    std::vector<float> sino(nS);
-    MBREC_HIP(hipMemcpy(sino.data(), ds.p, nS * sizeof(float), hipMemcpyDeviceToHost));
-    // planar [W][D][nT] -> MEX [W x nT x D]
-    for (int w = 0; w < W; ++w)
-        for (int d = 0; d < D; ++d)
-            for (int t = 0; t < nT; ++t)
-                out[(size_t)d * W * nT + (size_t)t * W + w] = (double)sino[((size_t)w * D + d) * nT + t];
+    std::lock_guard<std::mutex> lk(g_cache_mu);
+    CacheEntry *e;
+    if ((rc = get_entry(P, D, nT, R, tt, sidx, sir, Ls, Nsir, se, W, &e))) return rc;
+    const size_t nimg = (size_t)W * P, nS = (size_t)W * D * nT;
+    MBREC_HIP(hipMemcpy(e->d_host_layout, image, nimg * sizeof(double), hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(k_mex_img_in, grid_of(nimg), dim3(256), 0, nullptr, (const double *)e->d_host_layout,
+                       e->d_img, W, nimg);
+    if ((rc = op_forward(&e->plan->p, e->d_img, e->d_sino, W, false, nullptr))) return rc;
+    hipLaunchKernelGGL(k_mex_sino_out, grid_of(nS), dim3(256), 0, nullptr, (const float *)e->d_sino,
+                       e->d_host_layout, W, nT, D);
+    MBREC_HIP(hipGetLastError());
+    MBREC_HIP(hipMemcpy(out, e->d_host_layout, nS * sizeof(double), hipMemcpyDeviceToHost));
     return MBREC_OK;
 }
 
@@ -132,23 +224,18 @@ static int run_transpose(int W, int nT, int D, int P, double R, const double *si
         return set_error(MBREC_ERR_ARG, "forwardMex:args", "invalid dimensions or null buffer");
     int rc = mbrec_init(0);
     if (rc) return rc;
-    mbrec_plan plan;
-    if ((rc = get_plan(P, D, nT, R, tt, sidx, sir, Ls, Nsir, se, &plan))) return rc;
-    const size_t nS = (size_t)W * D * nT;
-    std::vector<float> sino(nS);
-    for (int w = 0; w < W; ++w)
-        for (int d = 0; d < D; ++d)
-            for (int t = 0; t < nT; ++t)
-                sino[((size_t)w * D + d) * nT + t] = (float)sig[(size_t)d * W * nT + (size_t)t * W + w];
-    DevBuf di, ds;
-    MBREC_HIP(hipMalloc((void **)&di.p, (size_t)W * P * sizeof(float)));
-    MBREC_HIP(hipMalloc((void **)&ds.p, nS * sizeof(float)));
-    MBREC_HIP(hipMemcpy(ds.p, sino.data(), nS * sizeof(float), hipMemcpyHostToDevice));
-    if ((rc = op_transpose(&plan->p, ds.p, di.p, W, false, nullptr))) return rc;
-    std::vector<float> img((size_t)W * P);
-    MBREC_HIP(hipMemcpy(img.data(), di.p, img.size() * sizeof(float), hipMemcpyDeviceToHost));
-    for (int x = 0; x < P; ++x)
-        for (int w = 0; w < W; ++w) out[(size_t)x * W + w] = (double)img[(size_t)w * P + x];
+    std::lock_guard<std::mutex> lk(g_cache_mu);
+    CacheEntry *e;
+    if ((rc = get_entry(P, D, nT, R, tt, sidx, sir, Ls, Nsir, se, W, &e))) return rc;
+    const size_t nimg = (size_t)W * P, nS = (size_t)W * D * nT;
+    MBREC_HIP(hipMemcpy(e->d_host_layout, sig, nS * sizeof(double), hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(k_mex_sino_in, grid_of(nS), dim3(256), 0, nullptr, (const double *)e->d_host_layout,
+                       e->d_sino, W, nT, D);
+    if ((rc = op_transpose(&e->plan->p, e->d_sino, e->d_img, W, false, nullptr))) return rc;
+    hipLaunchKernelGGL(k_mex_img_out, grid_of(nimg), dim3(256), 0, nullptr, (const float *)e->d_img,
+                       e->d_host_layout, W, nimg);
+    MBREC_HIP(hipGetLastError());
+    MBREC_HIP(hipMemcpy(out, e->d_host_layout, nimg * sizeof(double), hipMemcpyDeviceToHost));
     return MBREC_OK;
 }
 
@@ -213,7 +300,8 @@ int mbrec_mex_type_error(const char *mex_name, int arg, int expect_int32)
 void mbrec_mex_cache_clear(void)
 {
     std::lock_guard<std::mutex> lk(g_cache_mu);
-    for (auto &e : g_cache) mbrec_plan_destroy(e.plan);
+    if (!g_cache.empty()) (void)hipDeviceSynchronize();
+    for (auto &e : g_cache) free_entry(e);
     g_cache.clear();
 }
 

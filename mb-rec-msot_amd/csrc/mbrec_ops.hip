@@ -3576,6 +3576,12 @@ int op_transpose(Plan *p, const float *sino, float *img, int B, bool model, hipS
 {
     int rc = ensure_work(p, B);
     if (rc) return rc;
+    return op_transpose_run(p, sino, img, B, model, s, win);
+}
+
+int op_transpose_run(Plan *p, const float *sino, float *img, int B, bool model, hipStream_t s, const int2 *win)
+{
+    int rc;
     const int Bp = batch_stride(B);
     const float *z = sino;
     if (model && p->d_ir) {

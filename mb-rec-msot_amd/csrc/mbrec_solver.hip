@@ -6,8 +6,11 @@
 // residual estimate normr = |s| normr, stopping tests normr <= tol ||b|| and
 // ||A'r|| / (||A|| ||r||) <= tol, stagnation counter).  The augmented operator
 // is A_aug = [sqrt(l) A_reg; sqrt(b) I] (add_nn_constraint_term_to_model_matrix,
-// nnls_L2_less.m:79-101) with A_reg = [A; sqrt(lambdaL2) I]
-// (combine_forward_and_transpose_with_reg.m:6-14, RegL2 = identity).
+// nnls_L2_less.m:79-101) with A_reg = [A; sqrt(lambdaL2) R1; sqrt(lambdaL2_2) R2]
+// (combine_forward_and_transpose_with_reg.m:6-14, _with_two_regs.m:8-14): each R is the
+// identity (fused into the vector kernels), MATLAB del2 applied per image with del2 also
+// as its "transpose" (laplacian_per_wavelength.m, reconstruction_overview_script.m:50-51),
+// or a caller-supplied operator pair.
 //
 // Vectors are FP32 on the device; every inner product, norm and recurrence
 // scalar is FP64 and lives on the device, one scalar set per group (all B
@@ -34,6 +37,16 @@ struct Consts {
     double tol;
     int coupled;
     const float *srv;   // per-sinogram sqrt(lambdaL2) (independent solves), or null: sr for all
+    float sr2;          // sqrt(lambda)*sqrt(lambdaL2_2)
+    int reg2;           // lambdaL2_2 > 0
+};
+
+// One L2 term's operator: kind 0 identity, 1 MATLAB del2 per image (its own transpose,
+// as the reference uses it), 2 the caller's pair (fn enqueues out = R in / R' in on s).
+struct RegOp {
+    int kind = 0;
+    mbrec_reg_apply fn = nullptr;
+    void *ctx = nullptr;
 };
 
 __device__ __forceinline__ float sr_of(const Consts &k, int b) { return k.srv ? k.srv[b] : k.sr; }
@@ -59,11 +72,13 @@ __global__ __launch_bounds__(256) void k_init_sino(const float *__restrict__ y, 
     block_sum_store(bb, part, Q_B, b, B);
 }
 
-// pixel blocks: u_r = -sr*w, u_c = sb*vm, x = w, d = 0; ||u||^2 -> Q_C, ||b||^2 -> Q_D
+// pixel blocks: u_r = -sr*R1 w, u_r2 = -sr2*R2 w, u_c = sb*vm, x = w, d = 0;
+// ||u||^2 -> Q_C, ||b||^2 -> Q_D.  r1w / r2w: R1 w and R2 w (w itself for an identity term)
 __global__ __launch_bounds__(256) void k_init_pix(const float *__restrict__ w, const float *__restrict__ vm,
-                                                  float *__restrict__ ur, float *__restrict__ uc,
-                                                  float *__restrict__ x, float *__restrict__ dv,
-                                                  size_t n, int B, Consts k, double *part)
+                                                  const float *__restrict__ r1w, const float *__restrict__ r2w,
+                                                  float *__restrict__ ur, float *__restrict__ ur2,
+                                                  float *__restrict__ uc, float *__restrict__ x,
+                                                  float *__restrict__ dv, size_t n, int B, Consts k, double *part)
 {
     const int b = blockIdx.y;
     const size_t off = (size_t)b * n;
@@ -71,8 +86,13 @@ __global__ __launch_bounds__(256) void k_init_pix(const float *__restrict__ w, c
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
         const float wi = w[off + i], vi = vm[off + i];
         if (k.reg) {
-            const float r = -sr_of(k, b) * wi;
+            const float r = -sr_of(k, b) * r1w[off + i];
             ur[off + i] = r;
+            uu += (double)r * r;
+        }
+        if (k.reg2) {
+            const float r = -k.sr2 * r2w[off + i];
+            ur2[off + i] = r;
             uu += (double)r * r;
         }
         const float bc = k.sb * (wi + vi);
@@ -114,12 +134,13 @@ __global__ void k_scal_init(Scal *sc, const double *part, int G, int B, Consts k
 // inv_u / inv_v of their group are LSQR's unit vectors, and every consumer
 // applies the factor as it reads (no separate scaling pass over the vectors).
 //
-// v = inv_u*(sl*A'u_s + sr*u_r + sb*u_c) - beta*inv_v*v (first=1: without the beta
-// term); ||v||^2 -> Q_E.  atu = A' applied to the stored u_s.
+// v = inv_u*(sl*A'u_s + sr*R1'u_r + sr2*R2'u_r2 + sb*u_c) - beta*inv_v*v (first=1: without
+// the beta term); ||v||^2 -> Q_E.  atu = A' applied to the stored u_s; rt1 / rt2 = R1'u_r /
+// R2'u_r2 (u_r / u_r2 themselves for an identity term).
 __global__ __launch_bounds__(256) void k_v_update(const float *__restrict__ atu, const float *__restrict__ ur,
-                                                  const float *__restrict__ uc, float *__restrict__ v,
-                                                  size_t n, int B, Consts k, const Scal *sc, int first,
-                                                  double *part)
+                                                  const float *__restrict__ ur2, const float *__restrict__ uc,
+                                                  float *__restrict__ v, size_t n, int B, Consts k, const Scal *sc,
+                                                  int first, double *part)
 {
     const int b = blockIdx.y;
     const Scal &S = sc[k.coupled ? 0 : b];
@@ -130,6 +151,7 @@ __global__ __launch_bounds__(256) void k_v_update(const float *__restrict__ atu,
         for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
             float t = k.sl * atu[off + i] + k.sb * uc[off + i];
             if (k.reg) t += sr_of(k, b) * ur[off + i];
+            if (k.reg2) t += k.sr2 * ur2[off + i];
             t *= fu;
             const float vn = first ? t : t - bv * v[off + i];   // v is not yet defined when first
             v[off + i] = vn;
@@ -171,10 +193,13 @@ __global__ __launch_bounds__(256) void k_u_sino(const float *__restrict__ av, fl
     block_sum_store(uu, part, Q_A, b, B);
 }
 
-// u_r = sr*inv_v*v - alpha*inv_u*u_r, u_c = sb*inv_v*v - alpha*inv_u*u_c; ||.||^2 -> Q_C
-__global__ __launch_bounds__(256) void k_u_pix(const float *__restrict__ v, float *__restrict__ ur,
-                                               float *__restrict__ uc, size_t n, int B, Consts k,
-                                               const Scal *sc, double *part)
+// u_r = sr*inv_v*R1 v - alpha*inv_u*u_r, u_r2 = sr2*inv_v*R2 v - alpha*inv_u*u_r2,
+// u_c = sb*inv_v*v - alpha*inv_u*u_c; ||.||^2 -> Q_C.  r1v / r2v: R1 v and R2 v (v itself for
+// an identity term)
+__global__ __launch_bounds__(256) void k_u_pix(const float *__restrict__ v, const float *__restrict__ r1v,
+                                               const float *__restrict__ r2v, float *__restrict__ ur,
+                                               float *__restrict__ ur2, float *__restrict__ uc, size_t n, int B,
+                                               Consts k, const Scal *sc, double *part)
 {
     const int b = blockIdx.y;
     const Scal &S = sc[k.coupled ? 0 : b];
@@ -185,8 +210,13 @@ __global__ __launch_bounds__(256) void k_u_pix(const float *__restrict__ v, floa
         for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
             const float vi = fv * v[off + i];
             if (k.reg) {
-                const float r = sr_of(k, b) * vi - a * ur[off + i];
+                const float r = sr_of(k, b) * (fv * r1v[off + i]) - a * ur[off + i];
                 ur[off + i] = r;
+                uu += (double)r * r;
+            }
+            if (k.reg2) {
+                const float r = k.sr2 * (fv * r2v[off + i]) - a * ur2[off + i];
+                ur2[off + i] = r;
                 uu += (double)r * r;
             }
             const float c = k.sb * vi - a * uc[off + i];
@@ -317,12 +347,42 @@ __global__ void k_group_sums(const double *part, int q, int G, int B, int couple
     for (int g = threadIdx.x; g < G; g += blockDim.x) out[g] = group_sum(part, q, g, B, coupled);
 }
 
+// MATLAB del2(U, 1) of images [B][Nx][Nz] (z fastest = MATLAB [Nz x Nx] columns): per
+// dimension the halved centred second difference inside, linear extrapolation of it at
+// both ends (n > 3), the middle value (n = 3) or zero (n < 3); (g_z + g_x) / 2.
+__device__ __forceinline__ float del2_axis(const float *__restrict__ f, int i, int n, size_t st)
+{
+    if (n < 3) return 0.0f;
+    auto g = [&](int j) { return 0.5f * (f[(size_t)(j + 1) * st] - 2.0f * f[(size_t)j * st] + f[(size_t)(j - 1) * st]); };
+    if (n == 3) return g(1);
+    if (i == 0) return 2.0f * g(1) - g(2);
+    if (i == n - 1) return 2.0f * g(n - 2) - g(n - 3);
+    return g(i);
+}
+
+__global__ __launch_bounds__(256) void k_del2(const float *__restrict__ in, float *__restrict__ out, int Nz, int Nx,
+                                              int B)
+{
+    const size_t P = (size_t)Nz * Nx, n = P * B;
+    for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x) {
+        const size_t b = t / P, r = t - b * P;
+        const int ix = (int)(r / Nz), iz = (int)(r - (size_t)ix * Nz);
+        const float *img = in + b * P;
+        const float gz = del2_axis(img + (size_t)ix * Nz, iz, Nz, 1);
+        const float gx = del2_axis(img + iz, ix, Nx, (size_t)Nz);
+        out[t] = 0.5f * (gz + gx);
+    }
+}
+
 // ---------------------------------------------------------------------------
 // host driver
 // ---------------------------------------------------------------------------
 struct Workspace {
     float *x = nullptr, *w = nullptr, *vm = nullptr, *d = nullptr, *v = nullptr;
     float *ur = nullptr, *uc = nullptr, *us = nullptr, *ts = nullptr, *tp = nullptr;
+    float *ur2 = nullptr;                  // second L2 term's block of u
+    float *r1 = nullptr, *r2 = nullptr;    // R1 / R2 (or their transposes) applied to a vector
+    RegOp op1, op2;
     double *part = nullptr;
     Scal *sc = nullptr;
     double *gs = nullptr;
@@ -346,26 +406,59 @@ struct Workspace {
     }
 };
 
+// out = R in (transp = 0) or R' in (transp = 1) for a non-identity term; returns the
+// pointer the fused kernels read (in itself for an identity term)
+static int reg_apply(Plan *p, const RegOp &op, const float *in, float *out, int B, int transp, hipStream_t s,
+                     const float **res)
+{
+    if (op.kind == 0) {
+        *res = in;
+        return MBREC_OK;
+    }
+    if (op.kind == 1) {
+        const size_t n = (size_t)p->P * B;
+        hipLaunchKernelGGL(k_del2, dim3((unsigned)std::min<size_t>((n + 255) / 256, 8192)), dim3(256), 0, s, in, out,
+                           p->Nz, p->Nx, B);
+        MBREC_HIP(hipGetLastError());
+    } else {
+        const int rc = op.fn(op.ctx, in, out, B, p->P, transp, (mbrec_stream)s);
+        if (rc) return set_error(MBREC_ERR_ARG, "mbrec:regOperator", "L2 term operator callback failed (%d)", rc);
+    }
+    *res = out;
+    return MBREC_OK;
+}
+
 static int lsqr_solve(Plan *p, Workspace &ws, const float *y, int B, int G, int maxit, Consts k,
                       hipStream_t s, const int2 *win)
 {
     const size_t np = (size_t)p->P, ns = (size_t)p->D * p->nT;
     int rc;
+    const float *q1 = nullptr, *q2 = nullptr;
+    auto apply_regs = [&](const float *in1, const float *in2, int transp) -> int {
+        int e = 0;
+        if (k.reg && (e = reg_apply(p, ws.op1, in1, ws.r1, B, transp, s, &q1))) return e;
+        if (k.reg2 && (e = reg_apply(p, ws.op2, in2, ws.r2, B, transp, s, &q2))) return e;
+        return MBREC_OK;
+    };
     // u = b - A_aug w
     if ((rc = op_forward(p, ws.w, ws.ts, B, true, s, win))) return rc;
     hipLaunchKernelGGL(k_init_sino, vgrid(B), dim3(256), 0, s, y, ws.ts, ws.us, ns, B, k, ws.part);
-    hipLaunchKernelGGL(k_init_pix, vgrid(B), dim3(256), 0, s, ws.w, ws.vm, ws.ur, ws.uc, ws.x, ws.d, np, B, k,
-                       ws.part);
+    if ((rc = apply_regs(ws.w, ws.w, 0))) return rc;
+    hipLaunchKernelGGL(k_init_pix, vgrid(B), dim3(256), 0, s, ws.w, ws.vm, q1, q2, ws.ur, ws.ur2, ws.uc, ws.x,
+                       ws.d, np, B, k, ws.part);
     hipLaunchKernelGGL(k_scal_init, dim3(1), dim3(256), 0, s, ws.sc, ws.part, G, B, k);
     // v = A_aug' u
     if ((rc = op_transpose(p, ws.us, ws.tp, B, true, s, win))) return rc;
-    hipLaunchKernelGGL(k_v_update, vgrid(B), dim3(256), 0, s, ws.tp, ws.ur, ws.uc, ws.v, np, B, k, ws.sc, 1,
+    if ((rc = apply_regs(ws.ur, ws.ur2, 1))) return rc;
+    hipLaunchKernelGGL(k_v_update, vgrid(B), dim3(256), 0, s, ws.tp, q1, q2, ws.uc, ws.v, np, B, k, ws.sc, 1,
                        ws.part);
     hipLaunchKernelGGL(k_scal_v0, dim3(1), dim3(256), 0, s, ws.sc, ws.part, G, B, k);
     for (int ii = 1; ii <= maxit; ++ii) {
         if ((rc = op_forward(p, ws.v, ws.ts, B, true, s, win))) return rc;
         hipLaunchKernelGGL(k_u_sino, vgrid(B), dim3(256), 0, s, ws.ts, ws.us, ns, B, k, ws.sc, ws.part);
-        hipLaunchKernelGGL(k_u_pix, vgrid(B), dim3(256), 0, s, ws.v, ws.ur, ws.uc, np, B, k, ws.sc, ws.part);
+        if ((rc = apply_regs(ws.v, ws.v, 0))) return rc;
+        hipLaunchKernelGGL(k_u_pix, vgrid(B), dim3(256), 0, s, ws.v, q1, q2, ws.ur, ws.ur2, ws.uc, np, B, k, ws.sc,
+                           ws.part);
         hipLaunchKernelGGL(k_scal_u, dim3(1), dim3(256), 0, s, ws.sc, ws.part, G, B, k);
         hipLaunchKernelGGL(k_dx, vgrid(B), dim3(256), 0, s, ws.v, ws.d, ws.x, np, B, k, ws.sc, ws.part);
         // The transpose product that closes the last iteration feeds only the
@@ -374,7 +467,8 @@ static int lsqr_solve(Plan *p, Workspace &ws, const float *y, int B, int G, int 
         // residual estimate |s| normr never increases), so that work is dead.
         if (ii == maxit) break;
         if ((rc = op_transpose(p, ws.us, ws.tp, B, true, s, win))) return rc;
-        hipLaunchKernelGGL(k_v_update, vgrid(B), dim3(256), 0, s, ws.tp, ws.ur, ws.uc, ws.v, np, B, k, ws.sc, 0,
+        if ((rc = apply_regs(ws.ur, ws.ur2, 1))) return rc;
+        hipLaunchKernelGGL(k_v_update, vgrid(B), dim3(256), 0, s, ws.tp, q1, q2, ws.uc, ws.v, np, B, k, ws.sc, 0,
                            ws.part);
         hipLaunchKernelGGL(k_scal_v, dim3(1), dim3(256), 0, s, ws.sc, ws.part, G, B, k, ii);
     }
@@ -391,8 +485,10 @@ struct L2Solve {
 
 // Allocate the workspace, size the plan's operator workspace for B and upload the
 // per-sinogram sqrt(lambdaL2) (lambda_b, host [B], may be null: one lambdaL2 for all).
+// op1: the first L2 term's operator (identity when null); lambda2 / op2: the second term.
 static int l2_prepare(Plan *p, L2Solve &c, int B, int coupled, int num_iter, double lambdaL2,
-                      const double *lambda_b, hipStream_t s)
+                      const double *lambda_b, hipStream_t s, const RegOp *op1 = nullptr, double lambda2 = 0.0,
+                      const RegOp *op2 = nullptr)
 {
     const size_t np = (size_t)p->P, ns = (size_t)p->D * p->nT;
     Workspace &ws = c.ws;
@@ -415,8 +511,14 @@ static int l2_prepare(Plan *p, L2Solve &c, int B, int coupled, int num_iter, dou
         (rc = ws.alloc(&ws.uc, np * B)) || (rc = ws.alloc(&ws.tp, np * B)) ||
         (rc = ws.alloc(&ws.ur, reg ? np * B : 1)) || (rc = ws.alloc(&ws.us, ns * B)) ||
         (rc = ws.alloc(&ws.ts, ns * B)) || (rc = ws.alloc(&ws.part, (size_t)kMaxQ * B * kRedBlocks)) ||
-        (rc = ws.alloc(&ws.sc, G)) || (rc = ws.alloc(&ws.gs, 2 * (size_t)G)) ||
+        (rc = ws.alloc(&ws.sc, G)) || (rc = ws.alloc(&ws.gs, 3 * (size_t)G)) ||
         (rc = ws.alloc(&ws.srv, lambda_b ? (size_t)B : 1)))
+        return rc;
+    const int reg2 = lambda2 > 0.0;
+    if (op1) ws.op1 = *op1;
+    if (op2) ws.op2 = *op2;
+    if ((rc = ws.alloc(&ws.ur2, reg2 ? np * B : 1)) || (rc = ws.alloc(&ws.r1, ws.op1.kind != 0 ? np * B : 1)) ||
+        (rc = ws.alloc(&ws.r2, ws.op2.kind != 0 ? np * B : 1)))
         return rc;
     if (lambda_b) {
         MBREC_HIP(hipMemcpyAsync(ws.srv, srv.data(), sizeof(float) * B, hipMemcpyHostToDevice, s));
@@ -431,6 +533,8 @@ static int l2_prepare(Plan *p, L2Solve &c, int B, int coupled, int num_iter, dou
     k.tol = 1e-8;                                  // inner_tol (nnls_L2_less.m:33)
     k.coupled = coupled;
     k.srv = lambda_b ? ws.srv : nullptr;
+    k.sr2 = (float)(reg2 ? std::sqrt(lambda2) : 0.0);
+    k.reg2 = reg2;
     c.B = B;
     c.G = G;
     c.num_iter = num_iter;
@@ -460,31 +564,57 @@ static int l2_enqueue(Plan *p, L2Solve &c, const float *y, const float *init, fl
     return MBREC_OK;
 }
 
-// lambda_b (host, [B], may be null): per-sinogram lambdaL2 of independent solves (coupled = 0)
+// lambda_b (host, [B], may be null): per-sinogram lambdaL2 of independent solves (coupled = 0).
+// With terms (host, nterms <= 2): the L2 terms' weights and operators (lambdaL2 and
+// lambda_b unused); lcurve then receives 3 values per group: ||A rec - y||^2, ||R1 rec||^2,
+// ||R2 rec||^2.
 int nnls_l2(Plan *p, const float *y, int B, int coupled, int num_iter, double lambdaL2,
             const float *init, float *rec, double *lcurve, hipStream_t s, const double *lambda_b = nullptr,
-            const float *init_dual = nullptr)
+            const float *init_dual = nullptr, float *final_dual = nullptr, const mbrec_l2_term *terms = nullptr,
+            int nterms = 0)
 {
     const size_t np = (size_t)p->P, ns = (size_t)p->D * p->nT;
     L2Solve c;
     int rc;
-    if ((rc = l2_prepare(p, c, B, coupled, num_iter, lambdaL2, lambda_b, s))) return rc;
+    RegOp ops[2];
+    double lam2 = 0.0;
+    if (terms) {
+        lambdaL2 = nterms > 0 ? terms[0].lambda : 0.0;
+        lam2 = nterms > 1 && lambdaL2 > 0.0 ? terms[1].lambda : 0.0;
+        for (int j = 0; j < nterms; ++j) {
+            ops[j].kind = terms[j].kind;
+            ops[j].fn = terms[j].apply;
+            ops[j].ctx = terms[j].ctx;
+        }
+    }
+    if ((rc = l2_prepare(p, c, B, coupled, num_iter, lambdaL2, lambda_b, s, &ops[0], lam2, &ops[1]))) return rc;
     if ((rc = l2_enqueue(p, c, y, init, rec, nullptr, s, init_dual))) return rc;
     Workspace &ws = c.ws;
+    if (final_dual)   // (rec, final_dual) is the complete state: a later call continues from it
+        MBREC_HIP(hipMemcpyAsync(final_dual, ws.vm, sizeof(float) * np * B, hipMemcpyDeviceToDevice, s));
     const int G = c.G;
     if (lcurve) {
         // rec_nn_with_L2_reg.m:51,54: ||A rec - y||^2 and ||RegL2 rec||^2 per group
         if ((rc = op_forward(p, ws.w, ws.ts, B, true, s))) return rc;
+        // rec_nn_with_L2_reg.m:51-64 with the terms' operators: ||R1 rec||^2, ||R2 rec||^2
+        const float *q1 = ws.w, *q2 = ws.w;
+        if ((rc = reg_apply(p, ws.op1, ws.w, ws.r1, B, 0, s, &q1)) ||
+            (rc = reg_apply(p, ws.op2, ws.w, ws.r2, B, 0, s, &q2)))
+            return rc;
         hipLaunchKernelGGL(k_resid, vgrid(B), dim3(256), 0, s, ws.ts, y, ns, B, ws.part);
-        hipLaunchKernelGGL(k_sumsq, vgrid(B), dim3(256), 0, s, ws.w, np, B, (int)Q_B, ws.part);
+        hipLaunchKernelGGL(k_sumsq, vgrid(B), dim3(256), 0, s, q1, np, B, (int)Q_B, ws.part);
+        hipLaunchKernelGGL(k_sumsq, vgrid(B), dim3(256), 0, s, q2, np, B, (int)Q_C, ws.part);
         hipLaunchKernelGGL(k_group_sums, dim3(1), dim3(256), 0, s, ws.part, (int)Q_A, G, B, coupled, ws.gs);
         hipLaunchKernelGGL(k_group_sums, dim3(1), dim3(256), 0, s, ws.part, (int)Q_B, G, B, coupled, ws.gs + G);
-        std::vector<double> h(2 * (size_t)G);
-        MBREC_HIP(hipMemcpyAsync(h.data(), ws.gs, sizeof(double) * 2 * G, hipMemcpyDeviceToHost, s));
+        hipLaunchKernelGGL(k_group_sums, dim3(1), dim3(256), 0, s, ws.part, (int)Q_C, G, B, coupled, ws.gs + 2 * G);
+        std::vector<double> h(3 * (size_t)G);
+        MBREC_HIP(hipMemcpyAsync(h.data(), ws.gs, sizeof(double) * 3 * G, hipMemcpyDeviceToHost, s));
         MBREC_HIP(hipStreamSynchronize(s));
+        const int per = terms ? 3 : 2;
         for (int g = 0; g < G; ++g) {
-            lcurve[2 * g] = h[g];
-            lcurve[2 * g + 1] = c.reg ? h[G + g] : 0.0;
+            lcurve[per * g] = h[g];
+            lcurve[per * g + 1] = (terms ? nterms > 0 : c.reg) ? h[G + g] : 0.0;
+            if (terms) lcurve[per * g + 2] = nterms > 1 ? h[2 * G + g] : 0.0;
         }
     } else {
         MBREC_HIP(hipStreamSynchronize(s));   // workspace is freed on return
@@ -505,13 +635,45 @@ extern "C" int mbrec_rec_nn_with_L2_reg_dev(mbrec_plan h, const float *data, int
     return nnls_l2(&h->p, data, B, coupled, num_iter, lambdaL2, init, rec, lcurve, (hipStream_t)s);
 }
 
+extern "C" int mbrec_del2_dev(const float *in, float *out, int Nz, int Nx, int B, mbrec_stream s)
+{
+    if (!in || !out || Nz <= 0 || Nx <= 0 || B <= 0) return set_error(MBREC_ERR_ARG, "mbrec:args", "del2: invalid arguments");
+    const size_t n = (size_t)Nz * Nx * B;
+    hipLaunchKernelGGL(k_del2, dim3((unsigned)std::min<size_t>((n + 255) / 256, 8192)), dim3(256), 0, (hipStream_t)s, in,
+                       out, Nz, Nx, B);
+    MBREC_HIP(hipGetLastError());
+    return MBREC_OK;
+}
+
+extern "C" int mbrec_rec_nn_with_L2_reg_terms_dev(mbrec_plan h, const float *data, int B, int coupled,
+                                                  int num_iter, const mbrec_l2_term *terms, int nterms,
+                                                  const float *init, float *rec, double *lcurve, mbrec_stream s)
+{
+    if (!h || !data || !rec || B <= 0 || num_iter < 0 || nterms < 0 || nterms > 2 || (nterms && !terms))
+        return set_error(MBREC_ERR_ARG, "mbrec:args", "rec_nn_with_L2_reg: invalid arguments");
+    for (int j = 0; j < nterms; ++j) {
+        if (!(terms[j].lambda >= 0.0))
+            return set_error(MBREC_ERR_ARG, "mbrec:args", "rec_nn_with_L2_reg: term %d weight must be >= 0", j + 1);
+        if (terms[j].kind < MBREC_REG_IDENTITY || terms[j].kind > MBREC_REG_CALLBACK ||
+            (terms[j].kind == MBREC_REG_CALLBACK && !terms[j].apply))
+            return set_error(MBREC_ERR_ARG, "mbrec:regOperator", "rec_nn_with_L2_reg: term %d has no valid operator",
+                             j + 1);
+    }
+    // incorporate_l2_reg_into_model_functions.m:4-21: the second term enters the operator
+    // only together with the first (lambdaL2 > 0); the L-curve norms are reported for every
+    // term given (rec_nn_with_L2_reg.m:55-64)
+    return nnls_l2(&h->p, data, B, coupled, num_iter, 0.0, init, rec, lcurve, (hipStream_t)s, nullptr, nullptr,
+                   nullptr, terms, nterms);
+}
+
 extern "C" int mbrec_rec_nn_with_L2_reg_batch_dev(mbrec_plan h, const float *data, int B, int num_iter,
                                                   const double *lambdaL2, const float *init, const float *init_dual,
-                                                  float *rec, double *lcurve, mbrec_stream s)
+                                                  float *rec, float *final_dual, double *lcurve, mbrec_stream s)
 {
     if (!h || !data || !rec || !lambdaL2 || B <= 0 || num_iter < 0)
         return set_error(MBREC_ERR_ARG, "mbrec:args", "rec_nn_with_L2_reg_batch: invalid arguments");
-    return nnls_l2(&h->p, data, B, 0, num_iter, 0.0, init, rec, lcurve, (hipStream_t)s, lambdaL2, init_dual);
+    return nnls_l2(&h->p, data, B, 0, num_iter, 0.0, init, rec, lcurve, (hipStream_t)s, lambdaL2, init_dual,
+                   final_dual);
 }
 
 // ---------------------------------------------------------------------------

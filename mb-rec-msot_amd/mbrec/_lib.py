@@ -60,8 +60,18 @@ SIGNATURES = [
     ("mbrec_rec_nn_with_L2_reg_dev", C.c_int,
      [c_void_p, c_void_p, C.c_int, C.c_int, C.c_int, C.c_double, c_void_p, c_void_p, c_void_p,
       c_void_p]),
+    ("mbrec_filter_butter_zero_phase_dev", C.c_int,
+     [c_void_p, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, c_void_p, c_void_p]),
+    ("mbrec_apply_butterworth_window_dev", C.c_int,
+     [c_void_p, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, c_void_p]),
+    ("mbrec_interpolate_broken_transducers_dev", C.c_int,
+     [c_void_p, C.c_int, C.c_int, C.c_int, c_void_p, C.c_int, c_void_p, c_void_p]),
+    ("mbrec_del2_dev", C.c_int, [c_void_p, c_void_p, C.c_int, C.c_int, C.c_int, c_void_p]),
+    ("mbrec_rec_nn_with_L2_reg_terms_dev", C.c_int,
+     [c_void_p, c_void_p, C.c_int, C.c_int, C.c_int, c_void_p, C.c_int, c_void_p, c_void_p, c_void_p, c_void_p]),
     ("mbrec_rec_nn_with_L2_reg_batch_dev", C.c_int,
-     [c_void_p, c_void_p, C.c_int, C.c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+     [c_void_p, c_void_p, C.c_int, C.c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+      c_void_p]),
     ("mbrec_rec_nn_with_L1_eye_reg_dev", C.c_int,
      [c_void_p, c_void_p, C.c_int, C.c_int, C.c_int, C.c_double, c_void_p, c_void_p]),
     ("mbrec_rec_nn_with_TV_reg_dev", C.c_int,
@@ -98,6 +108,14 @@ SIGNATURES = [
     ("mbrec_conv3x3_bf16", C.c_int,
      [c_void_p, c_void_p, c_void_p, c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, c_void_p]),
 ]
+
+
+REG_APPLY = C.CFUNCTYPE(C.c_int, c_void_p, c_void_p, c_void_p, C.c_int, C.c_int, C.c_int, c_void_p)
+
+
+class L2Term(C.Structure):
+    """mbrec_l2_term of include/mbrec.h."""
+    _fields_ = [("lambda_", C.c_double), ("kind", C.c_int), ("apply", REG_APPLY), ("ctx", c_void_p)]
 
 
 class MbrecError(RuntimeError):

@@ -152,7 +152,7 @@ class MSOTmodel:
                  include_sir: bool, use_single_speed_of_sound: bool = True,
                  num_cropped_samples_at_sinogram_start: int = 0, filt_cutoff_min: float = 1e5,
                  filt_cutoff_max: float = 12e6, model_normalization_factor=None,
-                 sigma_iterations: int = 60, progress=None):
+                 sigma_iterations: int = 50, progress=None):
         """progress: optional callable receiving a line after each set-up stage."""
         say = progress or (lambda msg: None)
         t_start = time.time()
@@ -217,25 +217,49 @@ class MSOTmodel:
     def shape_sinogram(self):
         return (self.Probe.detector.numOfTransducers, self.Data.numTimeSamplesPerTransducer)
 
-    def estimate_sigma(self, iters: int = 60, seed: int = 0) -> float:
-        """Largest singular value of the unnormalised operator by power iteration on A'A
-        (replaces svds(..., 1, 'largest') at MSOTmodel.m:278; FP64 norms)."""
+    def estimate_sigma(self, iters: int = 50, seed: int = 0, tol: float = 1e-10) -> float:
+        """Largest singular value of the unnormalised operator (replaces svds(A, 1,
+        'largest', MaxIterations 50, Tolerance 1e-8) at MSOTmodel.m:278) by Golub-Kahan-
+        Lanczos bidiagonalisation with full reorthogonalisation: after j steps
+        A V_j = U_j B_j with B_j upper bidiagonal, and the largest singular value of the
+        small B_j converges to sigma_max from below, fast even when the second singular
+        value is close (power iteration does not).  Stops when it changes by <= tol
+        relative, or after ``iters`` steps.  Vectors and scalars FP64, operators FP32."""
         import torch
         self.plan.set_sigma(1.0)
         g = torch.Generator(device="cpu").manual_seed(seed)
-        x = torch.rand((1, *self.shape_image), generator=g, dtype=torch.float32).cuda()
-        x /= torch.linalg.vector_norm(x.double()).float()
-        lam = 0.0
+        v = torch.rand((1, *self.shape_image), generator=g, dtype=torch.float64).cuda()
+        v /= torch.linalg.vector_norm(v)
+        fwd = lambda x: self.plan.forward(x.float().reshape(1, -1)).double()
+        adj = lambda y: self.plan.transpose(y.float().reshape(1, -1)).double().reshape(v.shape)
+        V, U, al, be = [v], [], [], []
+        u = fwd(v)
+        al.append(torch.linalg.vector_norm(u).item())
+        u = u / al[-1]
+        U.append(u)
+        sig = al[0]
         for _ in range(iters):
-            y = self.plan.transpose(self.plan.forward(x))
-            nrm = torch.linalg.vector_norm(y.double()).item()
-            lam_new = nrm
-            x = (y.double() / nrm).float()
-            if abs(lam_new - lam) <= 1e-9 * lam_new:
-                lam = lam_new
+            r = adj(U[-1]) - al[-1] * V[-1]
+            for q in V:
+                r -= (r * q).sum() * q
+            be.append(torch.linalg.vector_norm(r).item())
+            if be[-1] <= 1e-300:
                 break
-            lam = lam_new
-        return float(np.sqrt(lam))
+            V.append(r / be[-1])
+            pv = fwd(V[-1]) - be[-1] * U[-1]
+            for q in U:
+                pv -= (pv * q).sum() * q
+            al.append(torch.linalg.vector_norm(pv).item())
+            if al[-1] <= 1e-300:
+                break
+            U.append(pv / al[-1])
+            Bk = np.diag(al) + np.diag(be, 1)[: len(al), : len(al)]
+            s_new = float(np.linalg.svd(Bk, compute_uv=False)[0])
+            done = abs(s_new - sig) <= tol * s_new
+            sig = s_new
+            if done:
+                break
+        return sig
 
     # operators (MSOTmodel.m:285-286) --------------------------------------------
     def applyForward(self, p0):

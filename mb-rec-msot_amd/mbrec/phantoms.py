@@ -16,6 +16,15 @@ def _grid(disc):
     return disc.region["xPositionsFov"].T, disc.region["zPositionsFov"].T   # (Nx, Nz)
 
 
+def _box(X, Z, xlo, xhi, zlo, zhi):
+    """Index slices (along x = axis 0, z = axis 1) of the grid points inside a box: shapes
+    are evaluated there only (the same values as on the whole grid, zero outside)."""
+    xv, zv = X[:, 0], Z[0, :]
+    i0, i1 = np.searchsorted(xv, xlo, "left"), np.searchsorted(xv, xhi, "right")
+    j0, j1 = np.searchsorted(zv, zlo, "left"), np.searchsorted(zv, zhi, "right")
+    return slice(max(i0 - 1, 0), i1 + 1), slice(max(j0 - 1, 0), j1 + 1)
+
+
 def disks(disc, n: int = 20, rmin: float = 0.2e-3, rmax: float = 3e-3, amin: float = 0.1,
           amax: float = 1.0, seed: int = 0) -> np.ndarray:
     """Sum of n disks, centres uniform in the field of view, radii U(rmin, rmax),
@@ -28,7 +37,8 @@ def disks(disc, n: int = 20, rmin: float = 0.2e-3, rmax: float = 3e-3, amin: flo
         cx, cz = g.uniform(x0, x1), g.uniform(z0, z1)
         r = g.uniform(rmin, rmax)
         a = g.uniform(amin, amax)
-        img += a * (((X - cx) ** 2 + (Z - cz) ** 2) <= r * r)
+        bi, bj = _box(X, Z, cx - r, cx + r, cz - r, cz + r)
+        img[bi, bj] += a * (((X[bi, bj] - cx) ** 2 + (Z[bi, bj] - cz) ** 2) <= r * r)
     return img
 
 
@@ -42,9 +52,12 @@ def ellipses(disc, n: int = 10, seed: int = 0) -> np.ndarray:
         a, b = g.uniform(0.3e-3, 4e-3, size=2)
         th = g.uniform(0, np.pi)
         amp = g.uniform(0.1, 1.0)
-        xr = (X - cx) * np.cos(th) + (Z - cz) * np.sin(th)
-        zr = -(X - cx) * np.sin(th) + (Z - cz) * np.cos(th)
-        img += amp * ((xr / a) ** 2 + (zr / b) ** 2 <= 1.0)
+        m = max(a, b)
+        bi, bj = _box(X, Z, cx - m, cx + m, cz - m, cz + m)
+        Xb, Zb = X[bi, bj], Z[bi, bj]
+        xr = (Xb - cx) * np.cos(th) + (Zb - cz) * np.sin(th)
+        zr = -(Xb - cx) * np.sin(th) + (Zb - cz) * np.cos(th)
+        img[bi, bj] += amp * ((xr / a) ** 2 + (zr / b) ** 2 <= 1.0)
     return img
 
 
@@ -59,9 +72,11 @@ def vessel_tree(disc, depth: int = 5, rmin: float = 0.1e-3, rmax: float = 1e-3, 
     def seg(px, pz, qx, qz, r, amp):
         vx, vz = qx - px, qz - pz
         L2 = vx * vx + vz * vz + 1e-30
-        t = np.clip(((X - px) * vx + (Z - pz) * vz) / L2, 0.0, 1.0)
-        d2 = (X - px - t * vx) ** 2 + (Z - pz - t * vz) ** 2
-        np.maximum(img, amp * (d2 <= r * r), out=img)
+        bi, bj = _box(X, Z, min(px, qx) - r, max(px, qx) + r, min(pz, qz) - r, max(pz, qz) + r)
+        Xb, Zb = X[bi, bj], Z[bi, bj]
+        t = np.clip(((Xb - px) * vx + (Zb - pz) * vz) / L2, 0.0, 1.0)
+        d2 = (Xb - px - t * vx) ** 2 + (Zb - pz - t * vz) ** 2
+        np.maximum(img[bi, bj], amp * (d2 <= r * r), out=img[bi, bj])
 
     stack = [(g.uniform(x0, x1), z0 + 0.05 * span, g.uniform(-0.3, 0.3) + np.pi / 2, rmax, 0, 1.0)]
     while stack:

@@ -34,10 +34,71 @@ def _to_device(x):
     return x.contiguous(), False
 
 
+def laplacian_per_wavelength(x, shape=None):
+    """laplacian_per_wavelength.m:1-10: MATLAB del2(img, 1) of every image of x (a device
+    tensor (W, Nx, Nz), or (W, P) with ``shape`` = (Nx, Nz)) on the device.  Passed as
+    RegL2 / RegL2T (or RegL2_2 / RegL2T_2) of rec_nn_with_L2_reg it selects the device
+    solver's built-in del2 term, used as its own transpose as the reference does
+    (reconstruction_overview_script.m:50-51)."""
+    import torch
+    t = x if x.dtype == torch.float32 else x.float()
+    t = t.contiguous()
+    Nx, Nz = shape if shape is not None else t.shape[-2:]
+    B = t.numel() // (Nx * Nz)
+    out = torch.empty_like(t)
+    _lib.check(_lib.lib().mbrec_del2_dev(t.data_ptr(), out.data_ptr(), int(Nz), int(Nx), int(B),
+                                         _lib.stream_handle()))
+    return out
+
+
+laplacian_per_wavelength.is_del2 = True
+
+
+class _DeviceImages:
+    """A device buffer seen through __cuda_array_interface__ (no copy)."""
+
+    def __init__(self, ptr, shape):
+        self.__cuda_array_interface__ = {"shape": tuple(shape), "typestr": "<f4", "data": (int(ptr), False),
+                                         "version": 3, "strides": None}
+
+
+def _term(lam, R, RT, shape, keep):
+    """mbrec_l2_term for one L2 term (lambda, R, R'): identity, del2 or a Python callback
+    (R, RT map device tensors (B, Nx, Nz) -> (B, Nx, Nz))."""
+    import torch
+    t = _lib.L2Term()
+    t.lambda_ = float(lam or 0.0)
+    if _is_identity(R) and _is_identity(RT):
+        t.kind = 0
+    elif getattr(R, "is_del2", False) and getattr(RT, "is_del2", False):
+        t.kind = 1
+    else:
+        Nx, Nz = shape
+
+        def apply(ctx, pin, pout, B, P, transp, stream):
+            try:
+                src = torch.as_tensor(_DeviceImages(pin, (B, Nx, Nz)), device="cuda")
+                dst = torch.as_tensor(_DeviceImages(pout, (B, Nx, Nz)), device="cuda")
+                dst.copy_((RT if transp else R)(src))
+                return 0
+            except Exception:       # reported by the library as mbrec:regOperator
+                return 1
+        cb = _lib.REG_APPLY(apply)
+        keep.append(cb)
+        t.kind = 2
+        t.apply = cb
+    return t
+
+
 def rec_nn_with_L2_reg(model, data_filt, num_iter_nn_outer, lambdaL2=0.0, RegL2=None, RegL2T=None,
                        lambdaL2_2=None, RegL2_2=None, RegL2T_2=None, *, coupled=True, init=None):
-    """rec_nn_with_L2_reg.m:1-66 (RegL2 = identity).
+    """rec_nn_with_L2_reg.m:1-66.
 
+    RegL2 / RegL2T and RegL2_2 / RegL2T_2: None or ``identity`` (the identity),
+    ``laplacian_per_wavelength`` (MATLAB del2 per image, built into the device solver), or
+    any pair of callables mapping device image tensors (B, Nx, Nz) to (B, Nx, Nz) (run on
+    the current stream inside the solve).  As in incorporate_l2_reg_into_model_functions.m
+    the second term enters only with lambdaL2 > 0.
     Returns (rec_img, lCurveErrImg, lCurveErrReg, lCurveErrReg_2, err, t) like the
     reference; ``err`` stays zero (calculate_objective_per_iter = false, :35) and
     ``t`` holds the solve's wall time split evenly over the outer iterations.
@@ -48,10 +109,10 @@ def rec_nn_with_L2_reg(model, data_filt, num_iter_nn_outer, lambdaL2=0.0, RegL2=
     nnls_L2_less.m:28-29 and so is not computed here.
     """
     import torch
-    if lambdaL2 and lambdaL2 > 0 and not (_is_identity(RegL2) and _is_identity(RegL2T)):
-        raise NotImplementedError("only RegL2 = identity is provided on the device")
-    if lambdaL2_2 and lambdaL2_2 > 0:
-        raise NotImplementedError("the second (Laplacian) L2 term is not provided on the device")
+    plain = RegL2 is None and RegL2T is None and RegL2_2 is None and not (lambdaL2_2 and lambdaL2_2 > 0)
+    if not plain:
+        return _rec_nn_with_L2_reg_terms(model, data_filt, num_iter_nn_outer, lambdaL2, RegL2, RegL2T, lambdaL2_2,
+                                         RegL2_2, RegL2T_2, coupled, init)
     y, host = _to_device(data_filt)
     D, nT = model.shape_sinogram
     B = y.numel() // (D * nT)
@@ -84,11 +145,49 @@ def rec_nn_with_L2_reg(model, data_filt, num_iter_nn_outer, lambdaL2=0.0, RegL2=
     return rec, errimg, errreg, 0.0, err, t
 
 
-def rec_nn_with_L2_reg_batch(model, data_filt, num_iter_nn_outer, lambdaL2, init=None, init_dual=None):
+def _rec_nn_with_L2_reg_terms(model, data_filt, num_iter, lambdaL2, RegL2, RegL2T, lambdaL2_2, RegL2_2, RegL2T_2,
+                              coupled, init):
+    import torch
+    y, host = _to_device(data_filt)
+    D, nT = model.shape_sinogram
+    B = y.numel() // (D * nT)
+    y = y.view(B, D, nT)
+    rec = torch.empty((B, *model.shape_image), dtype=torch.float32, device=y.device)
+    keep = []
+    terms = [_term(lambdaL2, RegL2, RegL2T, model.shape_image, keep)]
+    if RegL2_2 is not None or (lambdaL2_2 and lambdaL2_2 > 0):
+        terms.append(_term(lambdaL2_2, RegL2_2, RegL2T_2, model.shape_image, keep))
+    arr = (_lib.L2Term * len(terms))(*terms)
+    initp = None
+    if init is not None:
+        it, _ = _to_device(init)
+        if it.numel() != rec.numel():
+            raise ValueError("init has the wrong number of pixels")
+        initp = it.contiguous()
+    G = 1 if coupled else B
+    lc = np.zeros(3 * G, dtype=np.float64)
+    t0 = time.perf_counter()
+    _lib.check(_lib.lib().mbrec_rec_nn_with_L2_reg_terms_dev(
+        model.plan.h, y.data_ptr(), int(B), int(bool(coupled)), int(num_iter), arr, len(terms),
+        initp.data_ptr() if initp is not None else None, rec.data_ptr(), lc.ctypes.data, _lib.stream_handle()))
+    el = time.perf_counter() - t0
+    pick = (lambda j: lc[j]) if coupled else (lambda j: lc[j::3])
+    err = np.zeros(int(num_iter))
+    t = np.full(int(num_iter), el / max(1, int(num_iter)))
+    if host:
+        out = rec.cpu().numpy().astype(np.float64)
+        rec = out[0] if np.ndim(data_filt) == 2 else out
+    return rec, pick(0), pick(1), pick(2), err, t
+
+
+def rec_nn_with_L2_reg_batch(model, data_filt, num_iter_nn_outer, lambdaL2, init=None, init_dual=None,
+                             return_dual=False):
     """B independent rec_nn_with_L2_reg.m solves (RegL2 = identity), each with its own
     lambdaL2 (array [B]), in one batched device call.  data_filt (B, D, nT) device
     tensor; init / init_dual (optional (B, Nx, Nz)) seed the outer iterate w and the
-    scaled dual v of nnls_L2_less.m:28-29.  Returns the (B, Nx, Nz) device tensor."""
+    scaled dual v of nnls_L2_less.m:28-29.  Returns the (B, Nx, Nz) device tensor, or
+    (rec, dual) with return_dual=True: passing them back as init / init_dual continues
+    the same iteration sequence."""
     import torch
     y, _ = _to_device(data_filt)
     D, nT = model.shape_sinogram
@@ -104,11 +203,12 @@ def rec_nn_with_L2_reg_batch(model, data_filt, num_iter_nn_outer, lambdaL2, init
         if td.numel() != rec.numel():
             raise ValueError("init / init_dual have the wrong number of pixels")
         ptr.append(td)
+    dual = torch.empty_like(rec) if return_dual else None
     _lib.check(_lib.lib().mbrec_rec_nn_with_L2_reg_batch_dev(
         model.plan.h, y.data_ptr(), int(B), int(num_iter_nn_outer), lam.ctypes.data,
         ptr[0].data_ptr() if ptr[0] is not None else None, ptr[1].data_ptr() if ptr[1] is not None else None,
-        rec.data_ptr(), None, _lib.stream_handle()))
-    return rec
+        rec.data_ptr(), dual.data_ptr() if dual is not None else None, None, _lib.stream_handle()))
+    return (rec, dual) if return_dual else rec
 
 
 def rec_nn_with_L1_eye_reg(model, data_filt, num_iter, regParam, *, coupled=True):

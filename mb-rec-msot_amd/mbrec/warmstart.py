@@ -137,73 +137,193 @@ def phantom_batch(disc, n, seed, kinds=("disks", "ellipses", "vessels")):
     return np.stack([gen[kinds[i % len(kinds)]](seed + i) for i in range(n)]).astype(np.float32)
 
 
-def make_training_set(model, n, seed, num_iter, noise=0.01, lambda_rel=1e-2, batch=16, kinds=None):
-    """(inputs A^T y, targets) as (n, Nx, Nz) float32 device tensors.
+BETA = 1e-3   # nnls_L2_less.m:34 (rec_nn_with_L2_reg.m:34)
 
-    Each phantom x gives y = A x + noise (Gaussian, ``noise`` * max|A x|); the
-    target is the plain cold-start NNLS-L2 solution after ``num_iter`` outer
-    iterations with lambda = lambda_rel * ||A^T y|| (per sinogram)."""
+
+def _lam_tensor(lambdaL2, B, device):
     import torch
-    from .solvers import rec_nn_with_L2_reg_batch
+    return torch.as_tensor(np.asarray(lambdaL2, dtype=np.float64), device=device).reshape(B, 1, 1)
+
+
+def objective_gradient(model, y, lam, x):
+    """grad of F(x) = 1/2||A x - y||^2 + lambdaL2/2 ||x||^2 (the objective of
+    rec_nn_with_L2_reg.m with RegL2 = identity) for images x (B, Nx, Nz), FP64 result."""
+    B = y.shape[0]
+    r = model.applyForward(x.float().contiguous()) - y
+    return model.applyTranspose(r).reshape(x.shape).double() + _lam_tensor(lam, B, x.device) * x.double()
+
+
+def objective(model, y, lam, x):
+    """F(x) per item (FP64), as above."""
+    import torch
+    B = y.shape[0]
+    r = (model.applyForward(x.float().contiguous()) - y).double()
+    lam_t = torch.as_tensor(np.asarray(lam, dtype=np.float64), device=x.device)
+    return 0.5 * (r.reshape(B, -1) ** 2).sum(dim=1) + 0.5 * lam_t * (x.double().reshape(B, -1) ** 2).sum(dim=1)
+
+
+def nnls_l2_minimiser(model, y, lam, max_iter=400, tol=2e-7, log=None):
+    """The minimiser of F over x >= 0 -- the fixed point nnls_L2_less converges to
+    (rec_nn_with_L2_reg.m:39-41, RegL2 = identity; F is strictly convex for lambdaL2 > 0,
+    so the minimiser is unique) -- by accelerated projected gradient (Beck & Teboulle's
+    FISTA with O'Donoghue & Candes' gradient restart), step 1/L with L = 1.02 + lambdaL2
+    >= ||A||^2 + lambdaL2 for the sigma_max-normalised model.  Vectors FP64, operators
+    FP32.  Runs until the KKT residual ||x - max(x - grad F, 0)|| / ||x|| <= tol for every
+    item (about 150 iterations at 400^2) or max_iter.  Returns (x (B, Nx, Nz) FP64, kkt)."""
+    import torch
+    B = y.shape[0]
+    Nx, Nz = model.shape_image
+    dev = y.device
+    L = 1.02 + _lam_tensor(lam, B, dev)
+    x = torch.zeros((B, Nx, Nz), dtype=torch.float64, device=dev)
+    z = x.clone()
+    t = torch.ones((B, 1, 1), dtype=torch.float64, device=dev)
+    kkt = float("inf")
+    for k in range(max_iter):
+        g = objective_gradient(model, y, lam, z)
+        xn = torch.clamp(z - g / L, min=0.0)
+        restart = ((z - xn) * (xn - x)).sum(dim=(1, 2), keepdim=True) > 0
+        tn = (1 + torch.sqrt(1 + 4 * t * t)) / 2
+        mom = torch.where(restart, torch.zeros_like(t), (t - 1) / tn)
+        t = torch.where(restart, torch.ones_like(t), tn)
+        z = xn + mom * (xn - x)
+        x = xn
+        if k % 10 == 9:
+            g = objective_gradient(model, y, lam, x)
+            pg = (x - torch.clamp(x - g, min=0.0)).reshape(B, -1)
+            kkt = (torch.linalg.vector_norm(pg, dim=1) /
+                   torch.linalg.vector_norm(x.reshape(B, -1), dim=1).clamp_min(1e-300)).max().item()
+            if log and k % 50 == 49:
+                log(f"  minimiser: {k + 1} iterations, KKT residual {kkt:.2e}")
+            if kkt <= tol:
+                break
+    return x, kkt
+
+
+def stationary_state(model, y, lam, x):
+    """The state s = w - v of nnls_L2_less at which it is stationary for the minimiser x:
+    after every outer iteration w = max(s, 0) and v = max(-s, 0) with s = u - v
+    (nnls_L2_less.m:68-72); at the fixed point u = w = x and the u-step's optimality
+    (grad F(x) = beta v, :56 with the stacked operator) gives v = grad F(x) / beta on
+    {x = 0}, 0 elsewhere.  Returns s (B, Nx, Nz) FP64."""
+    import torch
+    g = objective_gradient(model, y, lam, x)
+    return torch.where(x > 0, x, -torch.clamp(g, min=0.0) / BETA)
+
+
+def make_training_set(model, n, seed, batch=64, noise=0.01, lambda_rel=1e-2, kinds=None, log=None):
+    """(inputs A^T y, targets s*) as (n, Nx, Nz) float32 device tensors.
+
+    Each seeded phantom x gives y = A x + noise (Gaussian, ``noise`` * max|A x|) and
+    lambdaL2 = lambda_rel * ||A^T y||; the target is the stationary state s* of
+    nnls_L2_less for the converged solution (nnls_l2_minimiser, KKT residual <= 2e-7):
+    the network predicts the whole state of the loop -- the image where it is positive
+    and the scaled dual v = grad F / beta where it is zero."""
+    import torch
     disc = model.Discretization
     Nx, Nz = model.shape_image
     kinds = kinds or ("disks", "ellipses", "vessels")
-    ins, tgs = [], []
+    ins = torch.empty((n, Nx, Nz), dtype=torch.float32, device="cuda")
+    tgs = torch.empty_like(ins)
     g = torch.Generator(device="cuda").manual_seed(int(seed))
+    t0 = time.perf_counter()
     for b0 in range(0, n, batch):
         nb = min(batch, n - b0)
         ph = torch.as_tensor(phantom_batch(disc, nb, seed + b0, kinds)).cuda()
         y = model.applyForward(ph)
         y = (y + noise * y.abs().amax(dim=(1, 2), keepdim=True) *
              torch.randn(y.shape, generator=g, device="cuda")).contiguous()
-        aty = model.applyTranspose(y).reshape(nb, -1)
-        lam = lambda_rel * torch.linalg.vector_norm(aty.double(), dim=1)
-        rec = rec_nn_with_L2_reg_batch(model, y, num_iter, lam.cpu().numpy())
-        ins.append(aty.reshape(nb, Nx, Nz))
-        tgs.append(rec.reshape(nb, Nx, Nz))
-    return torch.cat(ins), torch.cat(tgs)
+        aty = model.applyTranspose(y).reshape(nb, Nx, Nz)
+        lam = (lambda_rel * torch.linalg.vector_norm(aty.reshape(nb, -1).double(), dim=1)).cpu().numpy()
+        x, _ = nnls_l2_minimiser(model, y, lam)
+        ins[b0:b0 + nb] = aty
+        tgs[b0:b0 + nb] = stationary_state(model, y, lam, x).float()
+        if log:
+            log(f"training set: {b0 + nb}/{n} ({time.perf_counter() - t0:.1f} s)")
+    return ins, tgs
 
 
-def train(net, inputs, targets, epochs=10, batch=8, lr=1e-3, seed=0, log=None):
-    """Adam over shuffled mini-batches; returns the per-epoch mean losses."""
+def train(net, inputs, targets, epochs=10, batch=8, lr=1e-3, lr_final=None, flip=False, seed=0, log=None):
+    """Adam over shuffled mini-batches; the learning rate decays geometrically from lr to
+    lr_final (default lr) over the epochs; flip=True mirrors each mini-batch along x with
+    probability 1/2 (the arc probe is mirror-symmetric about the z axis, so A'A commutes
+    with the mirror).  Returns the per-epoch mean losses."""
     import torch
     n = inputs.shape[0]
     rng = np.random.default_rng(seed)
     hist = []
+    lr_final = lr if lr_final is None else lr_final
     for ep in range(epochs):
+        lr_ep = lr * (lr_final / lr) ** (ep / max(epochs - 1, 1))
         order = rng.permutation(n)
         tot, cnt = 0.0, 0
         t0 = time.perf_counter()
         for b0 in range(0, n - batch + 1, batch):
             idx = torch.as_tensor(order[b0:b0 + batch], device=inputs.device)
-            loss = net.train_step(inputs[idx].contiguous(), targets[idx].contiguous(), lr=lr)
+            xin, xt = inputs[idx], targets[idx]
+            if flip and rng.uniform() < 0.5:
+                xin, xt = torch.flip(xin, dims=(1,)), torch.flip(xt, dims=(1,))
+            loss = net.train_step(xin.contiguous(), xt.contiguous(), lr=lr_ep)
             tot += loss
             cnt += 1
         hist.append(tot / max(cnt, 1))
         if log:
-            log(f"epoch {ep}: loss {hist[-1]:.5g} ({time.perf_counter() - t0:.1f} s)")
+            log(f"epoch {ep}: loss {hist[-1]:.5g} lr {lr_ep:.2e} ({time.perf_counter() - t0:.1f} s)")
     return hist
 
 
-BETA = 1e-3   # nnls_L2_less.m:34
-
-
 def warm_start_estimate(net, model, y, lambdaL2):
-    """Warm start (w0, v0) for nnls_L2_less from sinograms y (B, D, nT).
+    """Warm start (w0, v0, used) of nnls_L2_less from sinograms y (B, D, nT).
 
-    w0 = max(net(A'y), 0).  v0 is the scaled dual that makes w0 a fixed point of
-    the outer iteration if w0 were the solution: with g = A'(A w0 - y) + lambdaL2 w0
-    (the gradient of the subproblem's data + L2 term, halved), the u-step optimality
-    at u = w = x gives v = g / beta, and the w-step requires v = 0 where x > 0 and
-    v >= 0 where x = 0; so v0 = max(g, 0) / beta on the zero set of w0, 0 elsewhere.
-    Returns (w0, v0) as (B, Nx, Nz) device tensors."""
+    The network maps A'y to the loop's state s0; w0 = max(s0, 0) and v0 = max(-s0, 0)
+    (the form every outer iteration leaves them in, nnls_L2_less.m:68-72).  Fallback to
+    the cold start (w0 = v0 = 0, nnls_L2_less.m:28-29), per item: when s0 is not finite,
+    or when w0 scores a larger objective F than the cold start's w = 0.  ``used`` (bool
+    array [B]) says which items start from the network's state."""
     import torch
     B = y.shape[0]
     Nx, Nz = model.shape_image
     aty = model.applyTranspose(y).reshape(B, Nx, Nz)
-    w0 = torch.clamp(net.forward(aty), min=0.0)
-    lam = torch.as_tensor(np.asarray(lambdaL2, dtype=np.float32), device=w0.device).reshape(-1, 1, 1)
-    r = model.applyForward(w0) - y
-    g = model.applyTranspose(r).reshape(B, Nx, Nz) + lam * w0
-    v0 = torch.where(w0 > 0, torch.zeros_like(g), torch.clamp(g, min=0.0) / BETA)
-    return w0.contiguous(), v0.contiguous()
+    s0 = net.forward(aty)
+    finite = torch.isfinite(s0).reshape(B, -1).all(dim=1)
+    s0 = torch.where(finite.reshape(B, 1, 1), s0, torch.zeros_like(s0))
+    w0 = torch.clamp(s0, min=0.0)
+    v0 = torch.clamp(-s0, min=0.0)
+    f_warm = objective(model, y, lambdaL2, w0)
+    f_cold = 0.5 * (y.double().reshape(B, -1) ** 2).sum(dim=1)
+    used = finite & (f_warm <= f_cold)
+    keep = used.reshape(B, 1, 1)
+    w0 = torch.where(keep, w0, torch.zeros_like(w0))
+    v0 = torch.where(keep, v0, torch.zeros_like(v0))
+    return w0.contiguous(), v0.contiguous(), used.cpu().numpy()
+
+
+def rec_nn_with_L2_reg_learned(net, model, y, num_iter, lambdaL2):
+    """B independent rec_nn_with_L2_reg.m solves (RegL2 = identity) of num_iter outer
+    iterations, each started from the network's state (warm_start_estimate) instead of
+    zero.  Returns (rec (B, Nx, Nz) device tensor, used)."""
+    from .solvers import rec_nn_with_L2_reg_batch
+    w0, v0, used = warm_start_estimate(net, model, y, lambdaL2)
+    return rec_nn_with_L2_reg_batch(model, y, num_iter, lambdaL2, init=w0, init_dual=v0), used
+
+
+def iterations_to_tolerance(model, y, lam, ref, tol, w0=None, v0=None, chunk=25, max_iter=3000):
+    """Outer iterations nnls_L2_less needs, from (w0, v0) (None: cold start), until every
+    item is within ``tol`` relative L2 of ``ref``; the loop runs in chunks of ``chunk``
+    iterations, each continuing from the previous chunk's full state (w, v), so the
+    count is exact to a multiple of ``chunk``.  Returns (k or None, {k: worst error})."""
+    import torch
+    from .solvers import rec_nn_with_L2_reg_batch
+    B = y.shape[0]
+    nref = torch.linalg.vector_norm(ref.reshape(B, -1).double(), dim=1)
+    w, v = w0, v0
+    errs = {}
+    k = 0
+    while k < max_iter:
+        w, v = rec_nn_with_L2_reg_batch(model, y, chunk, lam, init=w, init_dual=v, return_dual=True)
+        k += chunk
+        e = (torch.linalg.vector_norm((w.double() - ref.double()).reshape(B, -1), dim=1) / nref).max().item()
+        errs[k] = e
+        if e <= tol:
+            return k, errs
+    return None, errs

@@ -12,14 +12,14 @@
 #include "mex.h"
 #include "mbrec.h"
 
-static void mbrec_gateway_fail(void)
+static inline void mbrec_gateway_fail(void)
 {
     mexErrMsgIdAndTxt(mbrec_last_error_id(), "%s", mbrec_last_error());
 }
 
-static void mbrec_gateway_at_exit(void) { mbrec_mex_cache_clear(); }
+static inline void mbrec_gateway_at_exit(void) { mbrec_mex_cache_clear(); }
 
-static void mbrec_gateway_register_exit(void)
+static inline void mbrec_gateway_register_exit(void)
 {
     static int done = 0;
     if (!done) {
@@ -29,7 +29,7 @@ static void mbrec_gateway_register_exit(void)
 }
 
 /* reference check: mxIsDouble && !mxIsComplex, else <id> "<name> must be type double." */
-static void mbrec_need_double(const char *mexname, const mxArray *a, int pos)
+static inline void mbrec_need_double(const char *mexname, const mxArray *a, int pos)
 {
     if (!mxIsDouble(a) || mxIsComplex(a)) {
         mbrec_mex_type_error(mexname, pos, 0);
@@ -37,7 +37,7 @@ static void mbrec_need_double(const char *mexname, const mxArray *a, int pos)
     }
 }
 
-static void mbrec_need_int32(const char *mexname, const mxArray *a, int pos)
+static inline void mbrec_need_int32(const char *mexname, const mxArray *a, int pos)
 {
     if (!mxIsInt32(a)) {
         mbrec_mex_type_error(mexname, pos, 1);

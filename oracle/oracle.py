@@ -266,6 +266,48 @@ def zero_phase_response_textbook(n, fs, fmin, fmax):
 
 
 # ---------------------------------------------------------------------------
+# sinogram preprocessing (load_and_preprocess/), numpy FP64
+# ---------------------------------------------------------------------------
+def filter_butter_zero_phase(x, fs, fmin, fmax):
+    """filter_butter_zero_phase.m:17-48 (FFT branch) along the last axis: demean when
+    fmin > 0, then ifft(|H|^2 fft(x)) with |H|^2 from zero_phase_response_textbook (pinned
+    against scipy's Butterworth design)."""
+    x = np.asarray(x, dtype=np.float64)
+    y = x - x.mean(axis=-1, keepdims=True) if fmin > 0 else x
+    H2 = zero_phase_response_textbook(x.shape[-1], fs, fmin, fmax)
+    return np.real(np.fft.ifft(np.fft.fft(y, axis=-1) * H2, axis=-1))
+
+
+def butterworth_window(N, n_butter, low, high):
+    """apply_butterworth_window_to_sinogram.m:5-7: |freqz(b, a, N)| of butter(n_butter,
+    [low high] / N, 'bandpass') at the N points pi k / N, here from scipy's design
+    (zpk form; the tf form MATLAB goes through is the same response)."""
+    from scipy import signal
+    z, p, k = signal.butter(n_butter, [low / N, high / N], btype="bandpass", output="zpk")
+    _, h = signal.freqz_zpk(z, p, k, worN=N)
+    return np.abs(h)
+
+
+def apply_butterworth_window(x, n_butter, low, high):
+    """apply_butterworth_window_to_sinogram.m:8-9 along the last axis."""
+    x = np.asarray(x, dtype=np.float64)
+    mu = x.mean(axis=-1, keepdims=True)
+    return mu + (x - mu) * butterworth_window(x.shape[-1], n_butter, low, high)
+
+
+def interpolate_broken_transducers(x, broken):
+    """interpolate_signals_of_broken_transducers.m:6-18 for x (..., D, nT): scipy's
+    not-a-knot CubicSpline across the working detectors (1-based indices not in broken),
+    evaluated at 1..D with extrapolation, per time sample."""
+    from scipy.interpolate import CubicSpline
+    x = np.asarray(x, dtype=np.float64)
+    D = x.shape[-2]
+    good = np.setdiff1d(np.arange(1, D + 1), np.asarray(broken, dtype=int))
+    cs = CubicSpline(good.astype(float), x[..., good - 1, :], axis=-2, bc_type="not-a-knot", extrapolate=True)
+    return cs(np.arange(1, D + 1, dtype=float))
+
+
+# ---------------------------------------------------------------------------
 # composed model and solvers (FP64 C)
 # ---------------------------------------------------------------------------
 class Model:
@@ -359,6 +401,45 @@ class Model:
         return rec, cnt
 
 
+def nnls_l2_minimiser(model, data, lambdaL2, max_iter=600, tol=1e-9):
+    """FP64 minimiser of 1/2||A x - y_b||^2 + lambda_b/2 ||x||^2 over x >= 0 for every
+    sinogram b of data (W, D, nT): the point nnls_L2_less converges to (rec_nn_with_L2_reg.m
+    with RegL2 = identity; strictly convex for lambda > 0, so unique).  Accelerated projected
+    gradient (FISTA, Beck & Teboulle 2009, with the gradient restart of O'Donoghue & Candes
+    2015), step 1/(||A||^2 + lambda) with ||A|| <= 1.02 for a sigma-normalised model, all
+    sinograms through one W-batched operator call; stops when every item's KKT residual
+    ||x - max(x - grad, 0)|| / ||x|| <= tol.  Pinned by scipy's lsq_linear and the KKT
+    conditions on small dense problems (tests/test_oracle.py).  Returns (x (W, Nx, Nz), kkt)."""
+    data = np.ascontiguousarray(data, dtype=np.float64)
+    W = data.size // (model.D * model.nT)
+    y = data.reshape(W, model.D, model.nT)
+    lam = np.broadcast_to(np.asarray(lambdaL2, dtype=np.float64), (W,)).reshape(W, 1, 1)
+    L = 1.02 + lam
+
+    def grad(x):
+        return model.transpose(model.forward(x) - y) + lam * x
+
+    x = np.zeros((W, model.Nx, model.Nz))
+    z = x.copy()
+    t = np.ones((W, 1, 1))
+    kkt = np.inf
+    for k in range(max_iter):
+        xn = np.maximum(z - grad(z) / L, 0.0)
+        restart = np.sum((z - xn) * (xn - x), axis=(1, 2), keepdims=True) > 0
+        tn = (1 + np.sqrt(1 + 4 * t * t)) / 2
+        mom = np.where(restart, 0.0, (t - 1) / tn)
+        t = np.where(restart, 1.0, tn)
+        z = xn + mom * (xn - x)
+        x = xn
+        if k % 10 == 9:
+            pg = x - np.maximum(x - grad(x), 0.0)
+            kkt = float(np.max(np.linalg.norm(pg.reshape(W, -1), axis=1) /
+                               np.maximum(np.linalg.norm(x.reshape(W, -1), axis=1), 1e-300)))
+            if kkt <= tol:
+                break
+    return x, kkt
+
+
 def full_kernels(Nz, Nx, hz, hx, sigma):
     """Full-length circular prefilter kernels from the oracle's direct DFT."""
     return (prefilter_kernel_1d(gaussian_profile_shifted(Nz, hz, sigma)),
@@ -419,6 +500,93 @@ def lsqr_matlab(A, b, tol, maxit, x0):
     return x
 
 
+def lsqr_matlab_op(Aop, ATop, b, tol, maxit, x0):
+    """lsqr_matlab with the operator given as two callables (x -> A x, u -> A' u)."""
+    class _Op:
+        def __matmul__(self, x):
+            return Aop(x)
+
+        @property
+        def T(self):
+            return _OpT()
+
+    class _OpT:
+        def __matmul__(self, u):
+            return ATop(u)
+
+    return lsqr_matlab(_Op(), b, tol, maxit, x0)
+
+
+def del2_matlab(img):
+    """MATLAB del2(U, 1) of one image given as (Nx, Nz) (MATLAB [Nz x Nx] read in memory
+    order): per dimension the centred second difference halved, (f(i+1) - 2 f(i) + f(i-1))/2,
+    on interior points; at the two ends linear extrapolation of those values (2 g(2) - g(3),
+    2 g(n-1) - g(n-2)) when n > 3, the middle value when n = 3, zero when n < 3; the sum over
+    both dimensions divided by ndims = 2, i.e. L = (D2z + D2x) / 4 (laplacian_per_wavelength.m:4-6).
+    Pinned by exactness on cubic polynomials (edges included) and a hand-computed 3-point case."""
+    U = np.asarray(img, dtype=np.float64)
+    out = np.zeros_like(U)
+    for ax in (0, 1):
+        f = np.moveaxis(U, ax, 0)
+        n = f.shape[0]
+        g = np.zeros_like(f)
+        if n > 2:
+            g[1:-1] = (f[2:] - 2 * f[1:-1] + f[:-2]) / 2
+        if n > 3:
+            g[0] = 2 * g[1] - g[2]
+            g[-1] = 2 * g[-2] - g[-3]
+        elif n == 3:
+            g[0] = g[1]
+            g[-1] = g[1]
+        out += np.moveaxis(g, 0, ax)
+    return out / 2
+
+
+def rec_nn_with_L2_reg_regs(model, data, num_iter, regs, lsqr_tol=1e-8):
+    """rec_nn_with_L2_reg.m:39-46 with one or two L2 terms: the stacked operator of
+    combine_forward_and_transpose_with_reg.m:6-14 / _with_two_regs.m:8-14,
+    [A x; sqrt(l1) R1 x; sqrt(l2) R2 x] and A'u_A + sqrt(l1) R1T u_1 + sqrt(l2) R2T u_2, inside
+    nnls_L2_less.m (lambda = 1, beta = 1e-3, lsqr(..., 1e-8, 2, [], [], w) on
+    [A_reg; sqrt(beta) I]), all wavelengths stacked (the reference's call).  regs: list of
+    (lambdaL2, R, RT) with R, RT maps of one image (Nx, Nz) -> (Nx, Nz) applied per
+    wavelength.  Returns (rec (W, Nx, Nz), [||A rec - y||^2, ||R_j rec||^2 ...])."""
+    data = np.asarray(data, dtype=np.float64)
+    W = data.size // (model.D * model.nT)
+    shp = (W, model.Nx, model.Nz)
+    P = model.Nx * model.Nz
+    m = data.size
+    tgt = data.ravel()
+    regs = [(float(l), R, RT) for (l, R, RT) in regs if l > 0]
+    per_img = lambda f, x: np.stack([f(xi) for xi in x.reshape(shp)]).ravel()
+    sb = np.sqrt(1e-3)
+    nreg = len(regs)
+
+    def Areg(x):
+        parts = [model.forward(x.reshape(shp)).ravel()]
+        for lam, R, _ in regs:
+            parts.append(np.sqrt(lam) * per_img(R, x))
+        return np.concatenate(parts)
+
+    def ATreg(u):
+        out = model.transpose(u[:m].reshape(W, model.D, model.nT)).ravel()
+        for j, (lam, _, RT) in enumerate(regs):
+            out = out + np.sqrt(lam) * per_img(RT, u[m + j * W * P:m + (j + 1) * W * P])
+        return out
+
+    Aaug = lambda x: np.concatenate([Areg(x), sb * x])
+    ATaug = lambda u: ATreg(u[:m + nreg * W * P]) + sb * u[m + nreg * W * P:]
+    w = np.zeros(W * P)
+    v = np.zeros(W * P)
+    b0 = np.concatenate([tgt, np.zeros(nreg * W * P)])
+    for _ in range(num_iter):
+        u = lsqr_matlab_op(Aaug, ATaug, np.concatenate([b0, sb * (w + v)]), lsqr_tol, 2, w)
+        w = np.maximum(u - v, 0.0)
+        v = v + w - u
+    r = model.forward(w.reshape(shp)).ravel() - tgt
+    lc = [float(r @ r)] + [float(np.sum(per_img(R, w) ** 2)) for _, R, _ in regs]
+    return w.reshape(shp), lc
+
+
 def lsqr_krylov(A, b, maxit, x0):
     """Exact LSQR iterate: argmin ||b - A x|| over x in x0 + span{(A'A)^j A' r0, j < maxit}."""
     r0 = b - A @ x0
@@ -462,7 +630,8 @@ def spmr_polar(fov_x, fov_z, xd, zd):
     Z = np.asarray(fov_z, dtype=np.float64).ravel()[None]
     xq, zq = np.asarray(xd, dtype=np.float64)[:, None], np.asarray(zd, dtype=np.float64)[:, None]
     dist = np.sqrt((X - xq) ** 2 + (Z - zq) ** 2)
-    ang = np.arccos(((xq - X) * xq + (zq - Z) * zq) / (np.sqrt(xq ** 2 + zq ** 2) * dist))
+    # the cosine is clipped to [-1, 1]: a rounding excess past 1 would make acos complex
+    ang = np.arccos(np.clip(((xq - X) * xq + (zq - Z) * zq) / (np.sqrt(xq ** 2 + zq ** 2) * dist), -1.0, 1.0))
     return dist, ang
 
 

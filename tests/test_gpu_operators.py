@@ -250,8 +250,9 @@ def test_sigma_matches_dense_svd():
     E = np.eye(P).reshape(P, 10, 12)
     A = orc1.forward(E).reshape(P, -1).T
     s_true = np.linalg.svd(A, compute_uv=False)[0]
-    s_dev = m.estimate_sigma(iters=500)
-    assert abs(s_dev - s_true) / s_true < 1e-5, (s_dev, s_true)
+    s_dev = m.estimate_sigma()
+    assert abs(s_dev - s_true) / s_true < 1e-6, (s_dev, s_true)
+    assert abs(m.normalizationFactor - s_true) / s_true < 1e-6
 
 
 @pytest.mark.parametrize("per_det", [False, True])

@@ -354,6 +354,34 @@ def test_nnls_L2_fixed_point_is_nonnegative_least_squares():
     assert rel_l2(rec.ravel(), ref) < 1e-3
 
 
+def test_nnls_l2_minimiser_matches_lsq_linear_and_kkt():
+    """The FP64 minimiser (accelerated projected gradient) against scipy's bounded
+    least squares on the stacked system [A; sqrt(lambda) I] x ~ [y; 0], x >= 0, with the
+    tiny model's dense operator, and its KKT conditions (gradient >= 0 where x = 0,
+    = 0 where x > 0); two sinograms with their own lambda in one call."""
+    from scipy.optimize import lsq_linear
+    m = _tiny_model(seed=31)                                   # sigma = 3
+    s0 = np.linalg.svd(_dense_of(m), compute_uv=False)[0]
+    m1 = O.Model(m.Nz, m.Nx, m.tt, m.nT, m.R, sigma=3.0 * s0 / 0.999, kz=m.kz, kx=m.kx, ir=m.ir)
+    A1 = _dense_of(m1)                                         # ||A1|| = 0.999, as a normalised model
+    g = rng(32)
+    y = g.standard_normal((2, m.D * m.nT))
+    lam = np.array([0.05, 0.3])
+    x, kkt = O.nnls_l2_minimiser(m1, y.reshape(2, m.D, m.nT), lam, max_iter=5000, tol=1e-11)
+    assert kkt <= 1e-11
+    P = m.Nz * m.Nx
+    for b in range(2):
+        S = np.vstack([A1, np.sqrt(lam[b]) * np.eye(P)])
+        want = lsq_linear(S, np.concatenate([y[b], np.zeros(P)]), bounds=(0, np.inf), tol=1e-14,
+                          lsmr_tol="auto", max_iter=10000).x
+        xb = x[b].ravel()
+        active = want < 1e-12 * want.max()
+        assert 0 < active.sum() < P                            # some bounds active, some not
+        np.testing.assert_allclose(xb, want, atol=1e-7 * np.abs(want).max())
+        grad = A1.T @ (A1 @ xb - y[b]) + lam[b] * xb
+        assert np.all(grad[xb == 0] >= -1e-8) and np.abs(grad[xb > 0]).max() < 1e-8
+
+
 def test_nnls_L2_warm_start_oracle():
     """Warm-start form (w0, v0) of nnls_L2_less: zero start = the cold solve; the KKT pair
     (x*, g(x*)/beta on the zero set) of the bound-constrained minimiser is a fixed point;
@@ -465,3 +493,37 @@ def test_nnls_L1_matches_cg_polynomial_restatement(eta):
     assert rel_l2(rec.ravel(), want) < 1e-8
     # 1 start A'y + per outer iteration: A'y, 2 x 2 products of (A'A + I), 1 check product
     assert tuple(cnt) == (15 * 5, 1 + 15 * 6)
+
+
+# --- L2 regularisers ------------------------------------------------------------
+def test_del2_exact_on_cubics_edges_included():
+    """del2 reproduces (1/4) Laplacian of any cubic exactly on a uniform unit grid, at the
+    edges too (the second differences of a cubic are linear, so their extrapolation is
+    exact)."""
+    Nx, Nz = 7, 9
+    x = np.arange(Nx, dtype=float)[:, None]
+    z = np.arange(Nz, dtype=float)[None, :]
+    p = 0.3 * x ** 3 - 2 * x ** 2 * z + 0.5 * z ** 3 + x * z - 4 * z + 1.0
+    lap = (1.8 * x - 4 * z) + (-0 + 3.0 * z)          # d2/dx2 + d2/dz2 of p
+    np.testing.assert_allclose(O.del2_matlab(p), lap / 4, atol=1e-12)
+
+
+def test_del2_three_point_and_short_dimensions():
+    """n = 3 along z: the middle second difference is used at both ends; n = 2 along x:
+    zero contribution.  U (Nx = 2, Nz = 3) = [[1, 4, 2], [0, 1, 5]]:
+    z second differences halved: (2 - 8 + 1)/2 = -2.5 and (5 - 2 + 0)/2 = 1.5, so
+    del2 = [[-2.5]*3, [1.5]*3] / 2."""
+    U = np.array([[1.0, 4.0, 2.0], [0.0, 1.0, 5.0]])
+    want = np.array([[-2.5, -2.5, -2.5], [1.5, 1.5, 1.5]]) / 2
+    np.testing.assert_allclose(O.del2_matlab(U), want, atol=1e-15)
+
+
+def test_regs_oracle_with_identity_matches_the_C_oracle():
+    """The numpy stacked-regulariser loop with RegL2 = identity equals the C oracle's
+    rec_nn_with_L2_reg (pinned above) -- same stacking, same lsqr recurrences."""
+    m = _tiny_model(seed=41)
+    y = rng(42).standard_normal((2, m.D, m.nT))
+    rec, lc = O.rec_nn_with_L2_reg_regs(m, y, 6, [(0.3, lambda a: a, lambda a: a)])
+    ro, lco, _ = m.rec_nn_with_L2_reg(y, 6, 0.3)
+    assert rel_l2(rec, ro) < 1e-12
+    assert abs(lc[0] - lco[0]) < 1e-9 * lco[0] and abs(lc[1] - lco[1]) < 1e-9 * lco[1]
