@@ -102,7 +102,9 @@ int mbrec_plan_set_sir(mbrec_plan plan, const int32_t *sirIndices, const double 
 /* Impulse response IR [Lir] or [Lir x D] (per_detector), MSOTmodel.m:259-269. */
 int mbrec_plan_set_ir(mbrec_plan plan, const double *ir, int Lir, int per_detector);
 /* Separable circular prefilter (Discretization.m:54-72): taps kz[0..2Kz],
- * kx[0..2Kx] for offsets -K..K of the kernel k[n] = ifft(1/phi_hat)[n]. */
+ * kx[0..2Kx] for offsets -K..K of the kernel k[n] = ifft(1/phi_hat)[n], K <= N/2 per
+ * axis.  With K = N/2 (N even) offsets -K and +K reach the same sample: the caller puts
+ * the kernel's N/2 coefficient in one of them and 0 in the other. */
 int mbrec_plan_set_prefilter(mbrec_plan plan, const double *kz, int Kz, const double *kx, int Kx);
 /* model_normalization_factor (MSOTmodel.m:271-287). */
 int mbrec_plan_set_sigma(mbrec_plan plan, double sigma);
@@ -129,7 +131,9 @@ int mbrec_plan_autotune(mbrec_plan plan, int B, int reps, mbrec_stream s, int *c
  * record-lane kernel with that LDS batch chunk (a multiple of 4), < 0 the
  * batch-lane kernel with lane groups of -value (16, 32 or 64) sinograms.  Plans
  * without SIR (pull form): 1, 4, 8, 16 bins per thread pass of that many
- * sinograms; 2, 3 record streaming for one sinogram (1 or 4 bins per lane);
+ * sinograms; 2, 3 record streaming for one sinogram (1 or 4 bins per lane); 5, 6, 7 the
+ * tile form for one sinogram with 8, 4, 2 detectors per workgroup (4 x 4-tile blocks,
+ * 32-bit records staged in LDS, window partials summed per bin; 6 is the default there);
  * -(LB + 100 mode) batch-lane with lane groups LB and mode 0 (plain order), 1
  * (XCD-aware order), 2 (XCD-aware, 16 pairs in flight); -364 / -464 / -564 /
  * -664 / -764 / -864 wave-uniform, -964 / -1064 / -1164 wave-uniform with raw
@@ -245,6 +249,13 @@ int mbrec_rec_nn_with_TV_reg_dev(mbrec_plan plan, const float *data, int B, int 
                                  double regParam, int inner_iterations, int inner_min_iterations, float *rec,
                                  double *lcurve, mbrec_stream s);
 
+/* SB_ITV2 (reconstruction/SB_ITV2.m:1-40): u = arg min 1/2||u - g||^2 + mu ITV_B(u) by split
+ * Bregman for B square N x N images (device, FP32 in/out, [B][N][N] with z fastest), one
+ * mu per image (host array).  The inner cgs(I + B'B, ., 1e-5, 100) of SB_ITV2.m:31 is the
+ * exact DCT Neumann solve; N <= 2048.  iterations (host, nullable) = split-Bregman steps
+ * taken by the slowest image. */
+int mbrec_sb_itv2_dev(const float *g, int N, int B, const double *mu, float *u, int *iterations, mbrec_stream s);
+
 /* ---- 3. Secondary reconstruction paths (FP64) ------------------------------ */
 
 /* dmas_cf (reconstruction_backprojection/dmas_cf.m:19-107): delay-multiply-and-sum
@@ -275,7 +286,51 @@ int mbrec_bp_kernel_dev(const double *t, int Nt, double *K, mbrec_stream s);
 int mbrec_upsample_detectors_dev(const double *U, int Dup, int D, const double *sig, int nT, int nbatch,
                                  double *out, mbrec_stream s);
 
-/* ---- 4. Preprocessing (load_and_preprocess/, FP64) ------------------------- */
+/* ---- 4. Geometry tables ---------------------------------------------------- */
+
+/* calculate_travel_time_indices (models/MSOTmodel.m:350-382) with calculate_travel_time /
+ * refraction (basic_tools/refraction.m:10-24) and fermat_principle (fermat_principle.m:12-32):
+ * xpix, zpix device [P] pixel coordinates; xdet, zdet host [D]; membrane_z (host [nm], or
+ * NULL for a single speed of sound): the membrane z = m(x) sampled at membrane_x0 + i
+ * membrane_dx (linear in between, clamped).  A pixel with z <= m(x) (or with no membrane)
+ * travels straight at c_above; otherwise the time is minimised over the membrane crossing
+ * x in [-radius, radius] (c_below below, c_above above).  tau (device [D][P]) = t fs - delay
+ * - ncrop; x_intersections (device [D][P], may be NULL): the crossing x, NaN for straight
+ * paths.  FP64. */
+int mbrec_travel_times_dev(const double *xpix, const double *zpix, int P, const double *xdet, const double *zdet,
+                           int D, double c_below, double c_above, double radius, const double *membrane_z, int nm,
+                           double membrane_x0, double membrane_dx, double fs, double delay, double ncrop,
+                           double *tau, double *x_intersections, mbrec_stream s);
+
+/* get_SPMR_and_SPMR_indices (MSOTmodel.m:463-480) and the polar grid
+ * (map_grid_point_coordinates_to_polar_coordinates_for_SPMR, :573-597;
+ * calculate_required_SPMR_distances_and_angels, :599-616): xv, zv device pixel coordinates
+ * [P], or per detector [D][P] (per_detector = 1: the virtual positions of the dual speed of
+ * sound); xdet, zdet host [D].  grid (host [7]) receives {dmin, dmax, distance step,
+ * largest angle (deg), angle step (deg), number of distances, number of angles}; sidx
+ * (device [D][P], may be NULL) the 1-based indices (round(1 + (dist - dmin)/dd) - 1)
+ * n_angle + 1 + round(angle_deg / da). */
+int mbrec_spmr_grid_dev(const double *xv, const double *zv, int per_detector, int P, const double *xdet,
+                        const double *zdet, int D, double resolution_fov, double c, double *grid, int32_t *sidx,
+                        mbrec_stream s);
+/* The indices alone for a given grid (a precomputed table's, MSOTmodel.m:470-478). */
+int mbrec_spmr_index_dev(const double *xv, const double *zv, int per_detector, int P, const double *xdet,
+                         const double *zdet, int D, double dmin, double dd, double da, int n_angle, int32_t *sidx,
+                         mbrec_stream s);
+/* The polar SPMR table on the distance x angle grid (host dists [n_dist], angles_deg
+ * [n_angle]), replacing calculate_polarSPMR.m's Field II computation: the element's width
+ * split into n_sub (<= 32) point sub-elements, each arrival-time offset deposited as a
+ * linear hat on the Ls-sample grid (centre Ls/2), convolved with the pixel-model N-shape
+ * taps nshape[Ln] (tap j at sample offset j + nshape_offset).  table device [n_dist *
+ * n_angle][Ls] (angle fastest = MATLAB SPMR [Ls x N_angle*N_dist]); per entry, samples
+ * below trim x the entry's largest magnitude are set to zero (calculate_polarSPMR.m:152-161,
+ * trim 1e-4 there) and windows (device [n_dist * n_angle][2] int32) receive the 1-based
+ * first / last non-zero sample (an empty window is (1, 0)). */
+int mbrec_spmr_table_dev(const double *dists, int n_dist, const double *angles_deg, int n_angle, double element_width,
+                         int n_sub, double c, double fs, const double *nshape, int Ln, int nshape_offset, int Ls,
+                         double trim, double *table, int32_t *windows, mbrec_stream s);
+
+/* ---- 5. Preprocessing (load_and_preprocess/, FP64) ------------------------- */
 
 /* filter_butter_zero_phase (load_and_preprocess/filter_butter_zero_phase.m:1-53, FFT
  * branch): sig, out: device FP64 [ncols][nT] (time fastest, MATLAB [nT x cols]); demean

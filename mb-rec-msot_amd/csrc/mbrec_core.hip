@@ -122,6 +122,8 @@ void host_decode(int32_t q, const BinGeom &g, int &nn, int &m)
 // order inside a bin), each stored as (pixel, interpolation weight u), with CSR
 // offsets [D][NB + 1] into the record array.  Bin j of detector d then receives
 // (1 - u) c from its own pairs and u c from the pairs of bin j - 1.
+static int build_tile_forward(Plan *p, const std::vector<int32_t> &slot_pix, const std::vector<int32_t> &tq);
+
 static int build_pull_records(Plan *p, const std::vector<int32_t> &slot_pix, const std::vector<int32_t> &tq)
 {
     const BinGeom &g = p->geom;
@@ -191,6 +193,92 @@ static int build_pull_records(Plan *p, const std::vector<int32_t> &slot_pix, con
         (rc = dalloc(&p->d_rec1, rec1.size(), &p->rec_bytes)) || (rc = upload(p->d_rec1, rec1.data(), rec1.size())))
         return rc;
     p->table_bytes += p->rec_bytes;
+    return build_tile_forward(p, slot_pix, tq);
+}
+
+// Tile forward tables for one sinogram (see the Plan fields).  Pairs as the pull records
+// take them (nn in [0, nT + 2K), j = 3 nn + m, u exactly as decode_tau); blocks of 4 x 4
+// device tiles in tile order; plans without a 2-D grid (Nx == 1) get none.
+static int build_tile_forward(Plan *p, const std::vector<int32_t> &slot_pix, const std::vector<int32_t> &tq)
+{
+    void *old[] = {p->d_tfrec, p->d_tfroff, p->d_tfdesc, p->d_tfpart};
+    for (void *q : old)
+        if (q) (void)hipFree(q);
+    p->d_tfrec = nullptr;
+    p->d_tfroff = nullptr;
+    p->d_tfdesc = nullptr;
+    p->d_tfpart = nullptr;
+    p->table_bytes -= p->tf_bytes;
+    p->tf_bytes = 0;
+    p->tf_nbz = p->tf_nbx = 0;
+    if (p->Nx == 1 || p->TZ != 16) return MBREC_OK;
+    const BinGeom &g = p->geom;
+    const int span = g.nT + 2 * g.K;
+    const int ntz = (p->Nz + 15) / 16, ntx = (p->Nx + 15) / 16;
+    const int nbz = (ntz + 3) / 4, nbx = (ntx + 3) / 4, nblk = nbz * nbx;
+    const float scale = std::ldexp(1.0f, -g.F);
+    std::vector<uint32_t> rec;
+    std::vector<uint16_t> roff;
+    std::vector<int4> desc((size_t)nblk * p->D + 1);
+    std::vector<int> js(4096);
+    std::vector<uint32_t> rs(4096), srt(4096);
+    std::vector<int> cnt;
+    for (int bx = 0; bx < nbx; ++bx)
+        for (int bz = 0; bz < nbz; ++bz) {
+            const int T = bx * nbz + bz;
+            for (int d = 0; d < p->D; ++d) {
+                int n = 0, jmin = INT32_MAX, jmax = INT32_MIN;
+                for (int lt = 0; lt < 16; ++lt) {
+                    const int tx = 4 * bx + lt / 4, tz = 4 * bz + lt % 4;
+                    if (tx >= ntx || tz >= ntz) continue;
+                    const int tile = tx * ntz + tz;
+                    for (int k = 0; k < 256; ++k) {
+                        const int sl = tile * 256 + k;
+                        if (slot_pix[sl] < 0) continue;
+                        const int32_t q = tq[(size_t)d * p->Pd + sl];
+                        int nn, m;
+                        host_decode(q, g, nn, m);
+                        if ((unsigned)nn >= (unsigned)span) continue;
+                        const int j = 3 * nn + m;
+                        const float fr = (float)(q & ((1 << g.F) - 1)) * scale;
+                        const float ph = m == 0 ? 0.0f : (m == 1 ? g.phi1 : g.phi2);
+                        const float iw = m == 0 ? g.iw0 : (m == 1 ? g.iw1 : g.iw2);
+                        const float u = (fr - ph) * iw;
+                        long long u20 = std::llrint((double)u * (double)(1 << 20));
+                        u20 = std::min<long long>(std::max<long long>(u20, 0), (1 << 20) - 1);
+                        js[n] = j;
+                        rs[n] = (uint32_t)(lt * 256 + k) | ((uint32_t)u20 << 12);
+                        jmin = std::min(jmin, j);
+                        jmax = std::max(jmax, j);
+                        ++n;
+                    }
+                }
+                const size_t rstart = rec.size(), ostart = roff.size();
+                if (rstart > (size_t)INT32_MAX || ostart > (size_t)INT32_MAX - 70000) return MBREC_OK;   // too large: no tile form
+                const int nrow = n ? jmax - jmin + 1 : 0;
+                if (nrow + 1 > kTfMaxRows) return MBREC_OK;   // windows too long for the kernel's LDS: no tile form
+                desc[(size_t)T * p->D + d] = make_int4((int)rstart, (int)ostart, n ? jmin : 0, nrow);
+                // counting sort by row; offsets relative to rstart (n <= 4096 fits uint16)
+                cnt.assign(nrow + 1, 0);
+                for (int i = 0; i < n; ++i) cnt[js[i] - jmin + 1]++;
+                for (int r = 0; r < nrow; ++r) cnt[r + 1] += cnt[r];
+                for (int r = 0; r <= nrow; ++r) roff.push_back((uint16_t)cnt[r]);
+                for (int i = 0; i < n; ++i) srt[cnt[js[i] - jmin]++] = rs[i];
+                rec.insert(rec.end(), srt.begin(), srt.begin() + n);
+            }
+        }
+    desc[(size_t)nblk * p->D] = make_int4((int)rec.size(), (int)roff.size(), 0, 0);
+    rec.push_back(0u);
+    roff.push_back(0);
+    int rc;
+    if ((rc = dalloc(&p->d_tfrec, rec.size(), &p->tf_bytes)) || (rc = dalloc(&p->d_tfroff, roff.size(), &p->tf_bytes)) ||
+        (rc = dalloc(&p->d_tfdesc, desc.size(), &p->tf_bytes)) || (rc = dalloc(&p->d_tfpart, roff.size(), &p->tf_bytes)) ||
+        (rc = upload(p->d_tfrec, rec.data(), rec.size())) || (rc = upload(p->d_tfroff, roff.data(), roff.size())) ||
+        (rc = upload(p->d_tfdesc, desc.data(), desc.size())))
+        return rc;
+    p->table_bytes += p->tf_bytes;
+    p->tf_nbz = nbz;
+    p->tf_nbx = nbx;
     return MBREC_OK;
 }
 
@@ -408,7 +496,17 @@ int build_records(Plan *p, const std::vector<int32_t> &slot_pix, const std::vect
             rec[r] = make_int2(slot_pix[sl], tq[at]);
             if (sidx) rk[r] = sidx[at];
         }
-    // replace the previous tables
+    // replace the previous tables (the tile forward serves plans without SIR only)
+    void *oldtf[] = {p->d_tfrec, p->d_tfroff, p->d_tfdesc, p->d_tfpart};
+    for (void *q : oldtf)
+        if (q) (void)hipFree(q);
+    p->d_tfrec = nullptr;
+    p->d_tfroff = nullptr;
+    p->d_tfdesc = nullptr;
+    p->d_tfpart = nullptr;
+    p->table_bytes -= p->tf_bytes;
+    p->tf_bytes = 0;
+    p->tf_nbz = p->tf_nbx = 0;
     void *old[] = {p->d_rec, p->d_rec_off, p->d_rec_k, p->d_rec1};
     for (void *q : old)
         if (q) (void)hipFree(q);
@@ -599,7 +697,8 @@ static int build_tile_windows(Plan *p, const std::vector<int32_t> &tq)
 static void free_plan(Plan *p)
 {
     void *ptrs[] = {p->d_pix, p->d_tq, p->d_sidx, p->d_sir, p->d_swin, p->d_ir, p->d_irf, p->d_ir64, p->d_irf64,
-                    p->d_kz,  p->d_kx, p->d_rec, p->d_rec_off, p->d_rec_k, p->d_rec1, p->d_twin, p->d_ptab, p->d_ttab, p->d_srec, p->d_srec_off, p->d_sirw, p->d_stwin, p->d_stab};
+                    p->d_kz,  p->d_kx, p->d_rec, p->d_rec_off, p->d_rec_k, p->d_rec1, p->d_twin, p->d_ptab, p->d_ttab, p->d_srec, p->d_srec_off, p->d_sirw, p->d_stwin, p->d_stab,
+                    p->d_tfrec, p->d_tfroff, p->d_tfdesc, p->d_tfpart};
     for (void *q : ptrs)
         if (q) (void)hipFree(q);
     free_work(p);
@@ -795,7 +894,9 @@ int mbrec_plan_set_prefilter(mbrec_plan h, const double *kz, int Kz, const doubl
         return set_error(MBREC_ERR_ARG, "mbrec:args", "plan_set_prefilter: invalid arguments");
     if (h->p.pins) return set_error(MBREC_ERR_ARG, "mbrec:planPinned", "plan is held by a batch graph");
     Plan *p = &h->p;
-    if (2 * Kz + 1 > p->Nz || 2 * Kx + 1 > p->Nx)
+    // K <= N/2: every offset wraps at most once (at K = N/2, N even, the offsets -K and +K
+    // reach the same sample; the caller zeroes one of them)
+    if (2 * Kz > p->Nz || 2 * Kx > p->Nx)
         return set_error(MBREC_ERR_ARG, "mbrec:prefilter",
                          "prefilter taps (%d, %d) wrap around the %dx%d grid", 2 * Kz + 1, 2 * Kx + 1,
                          p->Nz, p->Nx);

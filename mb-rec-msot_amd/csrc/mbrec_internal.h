@@ -56,6 +56,10 @@ struct TuneChoice {
     int scatter_bc = 0;  // scatter LDS batch chunk
 };
 
+// tile forward (one sinogram): rows of a (block, detector) window the kernel stages in LDS;
+// a plan whose windows are longer gets no tile form
+constexpr int kTfMaxRows = 2048;
+
 struct Plan {
     int Nz = 0, Nx = 0, P = 0, D = 0, nT = 0;
     double R = 0.0;
@@ -80,6 +84,17 @@ struct Plan {
     int Wb = 384, nbk = 0, lo_off = 0, hi_off = 1, WL = 0;
     int2 *d_rec = nullptr;       // [records] (pixel, fixed-point travel time) / (pixel, u)
     int2 *d_rec1 = nullptr;      // [records] (slot, u): the pull records with slot indices (one sinogram)
+    // tile forward for one sinogram: blocks of 4 x 4 device tiles (64 x 64 pixels, 4096 slots);
+    // per (block T, detector d) the pairs sorted by bin j, each a 32-bit record (slot in the
+    // block: 12 bits | u in units of 2^-20: 20 bits), row offsets (uint16, relative to the
+    // record start) over the window [jmin, jmin + nrow), and a window partial of nrow + 1 bins
+    uint32_t *d_tfrec = nullptr;
+    uint16_t *d_tfroff = nullptr;
+    int4 *d_tfdesc = nullptr;    // [nblocks][D] + 1 (record start, offset / partial start, jmin, nrow);
+                                 // the extra entry's record start = the total (run lengths by difference)
+    float *d_tfpart = nullptr;   // partial windows (one sinogram), summed per bin by a second pass
+    int tf_nbz = 0, tf_nbx = 0;  // blocks along z / x
+    size_t tf_bytes = 0;
     int32_t *d_rec_off = nullptr;   // [D][nbk + 1] record offsets / [D][NB + 1]
     int32_t *d_rec_k = nullptr;  // [records] 0-based SIR index (SIR plans)
     // SIR forward in pull form: one record per (pixel, detector) pair with a tap in the

@@ -2564,16 +2564,18 @@ __global__ __launch_bounds__(1024) void k_gather_tas(const int4 *__restrict__ st
 // Transpose without SIR for one sinogram, LDS tile form: WG = one 256-slot tile (thread k:
 // slot k) x one of G detector groups.  The group's window origins go to LDS first; per
 // detector the tile's bin window (at most tile_zr <= 256 rows; one row per thread) is
-// staged in LDS, double buffered, with the window rows and travel times of the next two
-// detectors in flight; each thread decodes its travel time and interpolates its two bins
-// from LDS (Kahan sum over detectors, as the record-lane gather).
-template <int G>
+// staged in LDS, double buffered, with the window rows and travel times of the next PF
+// detectors in flight (a register ring); each thread decodes its travel time and
+// interpolates its two bins from LDS (Kahan sum over detectors, as the record-lane gather).
+// DPB detectors share one barrier (their windows staged together; PF a multiple of DPB).
+template <int G, int PF, int DPB = 1>
 __global__ __launch_bounds__(256) void k_gather_ta1(const int32_t *__restrict__ tq, const int2 *__restrict__ twin,
                                                     const int32_t *__restrict__ pix, const float *__restrict__ bins,
                                                     float *__restrict__ out, BinGeom g, int D, int Pd, int P,
                                                     float scale, int ZR)
 {
-    __shared__ float win[2][256];
+    static_assert(PF % DPB == 0, "the ring holds whole barrier steps");
+    __shared__ float win[2][DPB * 256];
     __shared__ int los[256];
     const int tile = blockIdx.x, grp = blockIdx.y, k = threadIdx.x;
     const int DG = (D + G - 1) / G, dbeg = grp * DG, dend = min(D, dbeg + DG);
@@ -2588,31 +2590,43 @@ __global__ __launch_bounds__(256) void k_gather_ta1(const int32_t *__restrict__ 
     };
     auto tq_of = [&](int d) -> int32_t { return tq[(size_t)d * Pd + slot]; };
     float acc = 0.0f, cmp = 0.0f;
-    if (dbeg < dend) {
-        int32_t q0 = tq_of(dbeg), q1 = 0;
-        float w0 = row_of(dbeg), w1 = 0.0f;
-        if (dbeg + 1 < dend) {
-            q1 = tq_of(dbeg + 1);
-            w1 = row_of(dbeg + 1);
-        }
-        for (int d = dbeg; d < dend; ++d) {
-            float *wb = win[(d - dbeg) & 1];
-            wb[k] = w0;
-            // the buffer written here was last read two detectors ago: every thread has
-            // passed the previous detector's barrier since
+    // register ring: slot i holds detector d0 + i's travel time and window row until its
+    // step, then is refilled with detector d0 + i + PF
+    int32_t qa[PF];
+    float wa[PF];
+#pragma unroll
+    for (int i = 0; i < PF; ++i) {
+        const int d = dbeg + i;
+        qa[i] = d < dend ? tq_of(d) : 0;
+        wa[i] = d < dend ? row_of(d) : 0.0f;
+    }
+    for (int d0 = dbeg; d0 < dend; d0 += PF) {
+#pragma unroll
+        for (int i = 0; i < PF; i += DPB) {
+            const int d = d0 + i;
+            if (d >= dend) break;   // uniform over the workgroup
+            float *wb = win[((d - dbeg) / DPB) & 1];
+#pragma unroll
+            for (int h = 0; h < DPB; ++h) wb[h * 256 + k] = wa[i + h];
+            // the buffer written here was last read two steps ago: every thread has
+            // passed the previous step's barrier since
             __syncthreads();
-            const int32_t qd = q0;
-            q0 = q1;
-            w0 = w1;
-            if (d + 2 < dend) {
-                q1 = tq_of(d + 2);
-                w1 = row_of(d + 2);
-            }
-            const Decoded dc = decode_tau(qd, g);
-            if ((unsigned)dc.nn < (unsigned)span) {
-                const int r = 3 * dc.nn + dc.m - los[d - dbeg];
-                const float g0 = wb[r], g1 = wb[r + 1];
-                kahan_add(acc, cmp, fmaf(dc.u, g1 - g0, g0));
+#pragma unroll
+            for (int h = 0; h < DPB; ++h) {
+                const int dd = d + h;
+                const int32_t qd = qa[i + h];
+                if (dd + PF < dend) {
+                    qa[i + h] = tq_of(dd + PF);
+                    wa[i + h] = row_of(dd + PF);
+                }
+                if (dd < dend) {
+                    const Decoded dc = decode_tau(qd, g);
+                    if ((unsigned)dc.nn < (unsigned)span) {
+                        const int r = 3 * dc.nn + dc.m - los[dd - dbeg];
+                        const float g0 = wb[h * 256 + r], g1 = wb[h * 256 + r + 1];
+                        kahan_add(acc, cmp, fmaf(dc.u, g1 - g0, g0));
+                    }
+                }
             }
         }
     }
@@ -2812,6 +2826,7 @@ static int scatter_choice(const Plan *p, int Bp)
 // 8 pairs in flight; 1: XCD-aware order, 8 in flight; 2: XCD-aware, 16 in flight
 static bool pull_choice_ok(const Plan *p, int c, int Bp)
 {
+    if (c == 5 || c == 6 || c == 7) return Bp == 1 && p->d_tfrec && p->d_img_slot;
     if (c > 0) return c == 1 || ((c == 2 || c == 3) && Bp == 1) || ((c == 4 || c == 8 || c == 16) && Bp % c == 0);
     if (c < 0) {
         const int lb = (-c) % 100, mode = (-c) / 100;
@@ -2847,6 +2862,159 @@ __global__ __launch_bounds__(256) void k_img_to_slots(const int32_t *__restrict_
     if (sl >= Pd) return;
     const int px = pix[sl];
     out[sl] = px >= 0 ? img[px] : 0.0f;
+}
+
+// Tile forward for one sinogram, pass 1: workgroup = one 4 x 4-tile block (its 4096 slot
+// values staged in LDS) x a group of detectors; thread per window row r of a (block,
+// detector) pair: partial[r] = sum of (1 - u) v over the records of row r + sum of u v over
+// the records of row r - 1 (rows = bins jmin + r; the window has nrow + 1 partials).
+// The pair's records (<= 4096) and row offsets (<= kTfMaxRows) are copied to LDS with
+// coalesced loads; the next detector's are loaded into registers while this one's rows
+// are summed, so the rows read records and pixel values from LDS only.
+// G detectors per workgroup (choices 5, 6, 7: G = 8, 4, 2)
+template <int G>
+__global__ __launch_bounds__(256) void k_fwd_tile1(const float *__restrict__ img_slot, const uint32_t *__restrict__ rec,
+                                                   const uint16_t *__restrict__ roff, const int4 *__restrict__ desc,
+                                                   float *__restrict__ part, int nbz, int ntz, int ntx, int D)
+{
+    constexpr int RPT = 4096 / 256, OPT = kTfMaxRows / 256;   // records / offsets per thread
+    __shared__ float v[4096];
+    __shared__ uint32_t rb[4096];
+    __shared__ uint16_t ob[kTfMaxRows];
+    __shared__ float edge[OPT][4];   // u part of the last row of each wave, per row slot
+    __shared__ int4 ed[G + 1];
+    const int T = blockIdx.x, bx = T / nbz, bz = T - bx * nbz;
+    const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+    for (int i = t; i < 4096; i += 256) {
+        const int lt = i >> 8, tx = 4 * bx + (lt >> 2), tz = 4 * bz + (lt & 3);
+        v[i] = (tx < ntx && tz < ntz) ? img_slot[(size_t)(tx * ntz + tz) * 256 + (i & 255)] : 0.0f;
+    }
+    const int d0 = blockIdx.y * G, d1 = min(D, d0 + G);
+    // the group's descriptors (and the next entry, for the run lengths) up front: the
+    // record loads of a detector then wait on no other global load
+    if (t <= d1 - d0) ed[t] = desc[(size_t)T * D + d0 + t];
+    __syncthreads();
+    // records of (T, d) are [e.x, e.x + n), n = the next entry's start - e.x (the table ends
+    // with a sentinel entry); offsets [e.y, e.y + nrow].  Two register sets: the loads of
+    // the next two detectors are in flight while one detector's rows are summed.
+    uint32_t ra[RPT], rb2[RPT];
+    uint16_t oa[OPT], ob2[OPT];
+    auto fetch = [&](int di, uint32_t (&rr)[RPT], uint16_t (&orr)[OPT]) __attribute__((always_inline)) {
+        const int4 e = ed[di];
+        const int n = ed[di + 1].x - e.x;
+        const uint32_t *r0 = rec + e.x;
+#pragma unroll
+        for (int i = 0; i < RPT; ++i) {
+            const int k = t + 256 * i;
+            rr[i] = k < n ? r0[k] : 0u;
+        }
+        const uint16_t *o0 = roff + e.y;
+#pragma unroll
+        for (int i = 0; i < OPT; ++i) {
+            const int k = t + 256 * i;
+            orr[i] = k <= e.w ? o0[k] : (uint16_t)0;
+        }
+    };
+    const float su = 1.0f / 1048576.0f;
+    auto step = [&](int d, uint32_t (&rr)[RPT], uint16_t (&orr)[OPT]) __attribute__((always_inline)) {
+        const int4 e = ed[d - d0];
+        __syncthreads();   // the previous detector's rows done with rb / ob / edge (and v written)
+#pragma unroll
+        for (int i = 0; i < RPT; ++i) rb[t + 256 * i] = rr[i];
+#pragma unroll
+        for (int i = 0; i < OPT; ++i) ob[t + 256 * i] = orr[i];
+        __syncthreads();
+        if (d + 2 < d1) fetch(d + 2 - d0, rr, orr);
+        // thread per row r < nrow: its records [b, c) give (1 - u) v to row r (own) and u v
+        // to row r + 1 (up); partial[r] = own[r] + up[r - 1], up[r - 1] from the previous
+        // lane, or through LDS across waves and row slots.  Records in chunks of 8 whose LDS
+        // reads are all issued before the sums (lanes past c read record b with weight 0).
+        float own[OPT], upl[OPT];
+#pragma unroll
+        for (int i = 0; i < OPT; ++i) {
+            const int r = t + 256 * i;
+            float a0 = 0.0f, a1 = 0.0f;
+            if (r < e.w) {
+                const int b = ob[r], c = ob[r + 1];
+                for (int k0 = b; k0 < c; k0 += 8) {
+                    uint32_t q[8];
+                    float val[8];
+#pragma unroll
+                    for (int j = 0; j < 8; ++j) q[j] = rb[k0 + j < c ? k0 + j : b];
+#pragma unroll
+                    for (int j = 0; j < 8; ++j) val[j] = v[q[j] & 4095u];
+#pragma unroll
+                    for (int j = 0; j < 8; ++j) {
+                        const float u = (float)(q[j] >> 12) * su, x = k0 + j < c ? val[j] : 0.0f;
+                        a0 = fmaf(1.0f - u, x, a0);
+                        a1 = fmaf(u, x, a1);
+                    }
+                }
+            }
+            own[i] = a0;
+            upl[i] = __shfl_up(a1, 1);
+            if (lane == 63) edge[i][wv] = a1;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < OPT; ++i) {
+            const int r = t + 256 * i;
+            if (r <= e.w) {
+                const float prev = lane > 0 ? upl[i] : (wv > 0 ? edge[i][wv - 1] : (i > 0 ? edge[i - 1][3] : 0.0f));
+                part[e.y + r] = (r < e.w ? own[i] : 0.0f) + prev;
+            }
+        }
+    };
+    if (d0 < d1) fetch(0, ra, oa);
+    if (d0 + 1 < d1) fetch(1, rb2, ob2);
+    for (int d = d0; d < d1; d += 2) {
+        step(d, ra, oa);
+        if (d + 1 < d1) step(d + 1, rb2, ob2);
+    }
+}
+
+// pass 2: bins[d][j] = sum over blocks T (in order) of their partial covering j; a
+// workgroup takes 1024 bins of one detector, 4 per thread (independent loads in flight)
+__global__ __launch_bounds__(256) void k_fwd_tile1_sum(const float *__restrict__ part, const int4 *__restrict__ desc,
+                                                       float *__restrict__ bins, int NB, int nblk, int D)
+{
+    __shared__ int4 w[256];
+    const int d = blockIdx.y;
+    const int j0 = blockIdx.x * 1024;
+    float acc[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    for (int t0 = 0; t0 < nblk; t0 += 256) {
+        __syncthreads();
+        if (t0 + (int)threadIdx.x < nblk) w[threadIdx.x] = desc[(size_t)(t0 + threadIdx.x) * D + d];
+        __syncthreads();
+        const int nt = min(256, nblk - t0);
+        for (int t = 0; t < nt; ++t) {
+            const int4 e = w[t];
+            if (e.w == 0 || e.z > j0 + 1023 || e.z + e.w < j0) continue;   // window misses these bins
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int r = j0 + (int)threadIdx.x + 256 * q - e.z;
+                if (r >= 0 && r <= e.w) acc[q] += part[e.y + r];
+            }
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int j = j0 + (int)threadIdx.x + 256 * q;
+        if (j < NB) bins[(size_t)d * NB + j] = acc[q];
+    }
+}
+
+template <int G>
+static void fwd_tile1(Plan *p, const float *imgI, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_img_to_slots, dim3((p->Pd + 255) / 256), dim3(256), 0, s, p->d_pix, imgI, p->d_img_slot,
+                       p->Pd);
+    const int ntz = (p->Nz + 15) / 16, ntx = (p->Nx + 15) / 16, nblk = p->tf_nbz * p->tf_nbx;
+    hipLaunchKernelGGL(k_fwd_tile1<G>, dim3(nblk, (p->D + G - 1) / G), dim3(256), 0, s,
+                       (const float *)p->d_img_slot, p->d_tfrec, p->d_tfroff, p->d_tfdesc, p->d_tfpart, p->tf_nbz,
+                       ntz, ntx, p->D);
+    hipLaunchKernelGGL(k_fwd_tile1_sum, dim3((p->geom.NB + 1023) / 1024, p->D), dim3(256), 0, s,
+                       (const float *)p->d_tfpart, p->d_tfdesc, p->d_bins, p->geom.NB, nblk, p->D);
 }
 
 template <int NG>
@@ -2921,11 +3089,16 @@ static void fwd_pull_wt(Plan *p, const float *imgI, int Bp, hipStream_t s, int J
 
 static int launch_pull(Plan *p, const float *imgI, int Bp, hipStream_t s)
 {
-    int c = (p->tune.Bp == Bp && p->tune.scatter_bc != 0) ? p->tune.scatter_bc : default_pull_choice(Bp);
-    if (!pull_choice_ok(p, c, Bp)) c = default_pull_choice(Bp);
+    // one sinogram: the tile form (4 detectors per workgroup) when its tables exist
+    const int dflt = (Bp == 1 && pull_choice_ok(p, 6, Bp)) ? 6 : default_pull_choice(Bp);
+    int c = (p->tune.Bp == Bp && p->tune.scatter_bc != 0) ? p->tune.scatter_bc : dflt;
+    if (!pull_choice_ok(p, c, Bp)) c = dflt;
     switch (c) {
     case 1: fwd_pull<1>(p, imgI, Bp, s); break;
     case 2: fwd_pull_b1<1>(p, imgI, s); break;
+    case 5: fwd_tile1<8>(p, imgI, s); break;
+    case 6: fwd_tile1<4>(p, imgI, s); break;
+    case 7: fwd_tile1<2>(p, imgI, s); break;
     case 3: fwd_pull_b1<4>(p, imgI, s); break;
     case 4: fwd_pull<4>(p, imgI, Bp, s); break;
     case 8: fwd_pull<8>(p, imgI, Bp, s); break;
@@ -3288,10 +3461,10 @@ static void gather_tas(Plan *p, float *img, int B, int Bp, float scale, hipStrea
     }
 }
 
-template <int G>
+template <int G, int PF = 2, int DPB = 1>
 static void gather_ta1(Plan *p, float *img, float scale, hipStream_t s)
 {
-    hipLaunchKernelGGL((k_gather_ta1<G>), dim3(p->Pd / 256, G), dim3(256), 0, s, p->d_tq, p->d_twin, p->d_pix,
+    hipLaunchKernelGGL((k_gather_ta1<G, PF, DPB>), dim3(p->Pd / 256, G), dim3(256), 0, s, p->d_tq, p->d_twin, p->d_pix,
                        p->d_bins, G > 1 ? p->d_gpart : img, p->geom, p->D, p->Pd, p->P, scale, p->tile_zr);
     if (G > 1) {
         const size_t n = (size_t)p->P;
@@ -3356,13 +3529,14 @@ static const GatherVariant kGatherVariants[] = {{1, 8, 0},   {1, 4, 0},   {4, 4,
                                                 {0, -10, 2}, {0, -10, 4}, {0, -11, 1}, {0, -11, 2},
                                                 {0, -11, 4}, {0, -12, 1}, {0, -12, 2}, {0, -12, 4},
                                                 {0, -12, 8}, {0, -13, 1}, {0, -13, 2}, {0, -13, 4},
-                                                {0, -13, 8}, {0, -13, 16}};
+                                                {0, -13, 8}, {0, -13, 16}, {0, -14, 4}, {0, -14, 8},
+                                                {0, -14, 16}};
 constexpr int kNumGatherVariants = sizeof(kGatherVariants) / sizeof(kGatherVariants[0]);
 
 static bool gather_variant_ok(const Plan *p, int v, int Bp)
 {
     const GatherVariant &g = kGatherVariants[v];
-    if (g.u == -13)   // tile gather for one sinogram without SIR: tile windows of at most 256 rows
+    if (g.u == -13 || g.u == -14)   // tile gather for one sinogram without SIR: tile windows of at most 256 rows
         return !p->use_sir && p->d_twin && p->tile_zr >= 64 && p->tile_zr <= 256 && Bp == 1 &&
                (g.lb == 1 || p->d_gpart);
     if (g.u == -12)   // SIR tile gather for one sinogram: the SIR tile tables, G partial images
@@ -3489,7 +3663,10 @@ static int launch_gather(Plan *p, float *img, int B, int Bp, float scale, hipStr
     case 45: gather_ta1<2>(p, img, scale, s); break;
     case 46: gather_ta1<4>(p, img, scale, s); break;
     case 47: gather_ta1<8>(p, img, scale, s); break;
-    default: gather_ta1<16>(p, img, scale, s); break;
+    case 48: gather_ta1<16>(p, img, scale, s); break;
+    case 49: gather_ta1<4, 4, 2>(p, img, scale, s); break;
+    case 50: gather_ta1<8, 4, 2>(p, img, scale, s); break;
+    default: gather_ta1<16, 4, 2>(p, img, scale, s); break;
     }
     MBREC_HIP(hipGetLastError());
     return MBREC_OK;
@@ -3711,7 +3888,7 @@ int autotune(Plan *p, int B, int reps, hipStream_t s, int *gather_out, int *scat
     // scatter: record-lane LDS batch chunks, then batch-lane lane groups
     std::vector<int> cand;
     if (p->pull) {
-        for (int c : {1, 2, 3, 4, 8, 16, -16, -32, -64, -116, -132, -164, -216, -232, -264, -364, -464, -564, -664,
+        for (int c : {6, 5, 7, 1, 2, 3, 4, 8, 16, -16, -32, -64, -116, -132, -164, -216, -232, -264, -364, -464, -564, -664,
                       -764, -864, -964, -1064, -1164})
             if (pull_choice_ok(p, c, Bp) && !(c == 1 && Bp > 16)) cand.push_back(c);
     } else {   // SIR: the pull forms first

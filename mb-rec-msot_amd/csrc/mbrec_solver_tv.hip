@@ -12,21 +12,23 @@
 // The inner linear solve of SB_ITV2, cgs(I + B'B, rhs, 1e-5, 100), is done exactly
 // here: for the DiffOper B (backward differences, zero first row) B'B is the 2-D
 // Neumann Laplacian, diagonalised by the orthonormal DCT-II basis, so
-// (I + B'B)^-1 r = C ((C' R C) ./ (1 + l_i + l_j)) C', four FP64 GEMMs on the matrix
-// cores.  The reference's cgs stops at a relative residual of 1e-5.
+// (I + B'B)^-1 r = C ((C' R C) ./ (1 + l_i + l_j)) C' with l_k = 4 sin^2(pi k / 2N).
+// The 1-D transforms are FP64 DCTs computed from one complex FFT of length N per two
+// real rows (Makhoul's even/odd reordering; the two rows as real and imaginary part),
+// the FFT a mixed-radix Stockham transform in LDS over N's prime factors (radix 4
+// first), O(N^2 log N) per image for smooth N.  Three passes per solve: along z (the
+// right-hand side g - B'(b - d) formed while loading), along x (forward, divide by the
+// eigenvalues, inverse), along z (inverse).  The reference's cgs stops at a relative
+// residual of 1e-5.
 //
 // Model-facing vectors are FP32, the TV prox is FP64, scalars are FP64.  The loops are
 // host-driven (the split Bregman loop length depends on the data).
 #include "mbrec_reduce.h"
 
-#include <rocblas/rocblas.h>
-
 #include <cmath>
 #include <vector>
 
 namespace mbrec {
-
-int blas_handle(hipStream_t s, rocblas_handle *out);   // mbrec_backproj.hip
 
 // ---- TV prox kernels (images [B][Nx][Nz] column-major = z fastest) -------------
 // g = (double) z
@@ -42,38 +44,212 @@ __global__ void k_tv_load(const float *__restrict__ z, double *__restrict__ g, d
     }
 }
 
-// rhs = g - B'(b - d);  (B' v)(iz,ix) = vz(iz,ix)[iz>0] - vz(iz+1,ix)[iz+1<N] + vx(iz,ix)[ix>0] - vx(iz,ix+1)[ix+1<N]
-__global__ void k_tv_rhs(const double *__restrict__ g, const double *__restrict__ bz, const double *__restrict__ bx,
-                         const double *__restrict__ dz, const double *__restrict__ dx, double *__restrict__ rhs,
-                         int N, int B, const int *__restrict__ active)
+// ---- Neumann solve: DCTs by mixed-radix FFT in LDS -------------------------------
+constexpr int kNmMaxFactors = 24;
+constexpr int kNmLdsPoints = 2048;   // complex points per LDS buffer (two buffers: 64 KB)
+constexpr int kNmMaxN = kNmLdsPoints;
+
+struct NmFactors {
+    int n;
+    int p[kNmMaxFactors];
+};
+
+// position of sample i in Makhoul's reordering: even samples first, odd ones reversed
+__device__ inline int nm_perm(int i, int N) { return (i & 1) ? N - 1 - (i >> 1) : (i >> 1); }
+
+// Stockham FFT of C rows of N complex points (row c at a + c*N), unnormalised,
+// e^{-2 pi i/N} (forward) or e^{+2 pi i/N} (inverse) kernel; tw[j] = e^{-2 pi i j/N}.
+// Stage with radix p after Ns points per sub-transform: output (j, q), j < N/p, q < p,
+// goes to (j / Ns) Ns p + j % Ns + q Ns and is sum_r a[j + r N/p] w^(r t),
+// t = (j % Ns + q Ns) N / (Ns p).  Returns the buffer holding the result.
+__device__ double2 *nm_fft(double2 *a, double2 *b, int N, int C, const NmFactors &f,
+                           const double2 *__restrict__ tw, bool inverse)
 {
-    const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const size_t P = (size_t)N * N;
-    if (idx >= P * B) return;
-    const int b = (int)(idx / P);
-    if (!active[b]) return;
-    const int p = (int)(idx - (size_t)b * P);
-    const int ix = p / N, iz = p - ix * N;
-    const size_t o = (size_t)b * P;
-    auto vz = [&](int z_, int x_) { const size_t k = o + (size_t)x_ * N + z_; return bz[k] - dz[k]; };
-    auto vx = [&](int z_, int x_) { const size_t k = o + (size_t)x_ * N + z_; return bx[k] - dx[k]; };
-    double bt = 0.0;
-    if (iz > 0) bt += vz(iz, ix);
-    if (iz + 1 < N) bt -= vz(iz + 1, ix);
-    if (ix > 0) bt += vx(iz, ix);
-    if (ix + 1 < N) bt -= vx(iz, ix + 1);
-    rhs[idx] = g[idx] - bt;
+    int Ns = 1;
+    for (int s = 0; s < f.n; ++s) {
+        const int p = f.p[s], m = N / p, step = N / (Ns * p);
+        for (int o = threadIdx.x; o < C * N; o += blockDim.x) {
+            const int c = o / N, e = o - c * N;
+            const int q = e / m, j = e - q * m;
+            const int k = j % Ns;
+            const int t = (k + q * Ns) * step;
+            const double2 *src = a + c * N + j;
+            double re = 0.0, im = 0.0;
+            int idx = 0;
+            for (int r = 0; r < p; ++r) {
+                const double2 w = tw[idx];
+                const double wi = inverse ? -w.y : w.y;
+                const double2 x = src[r * m];
+                re += x.x * w.x - x.y * wi;
+                im += x.x * wi + x.y * w.x;
+                idx += t;
+                if (idx >= N) idx -= N;
+            }
+            b[c * N + (j / Ns) * Ns * p + k + q * Ns] = make_double2(re, im);
+        }
+        __syncthreads();
+        double2 *sw = a;
+        a = b;
+        b = sw;
+        Ns *= p;
+    }
+    return a;
 }
 
-// every item (an item past its stop test is solved again from its unchanged right-hand
-// side; nothing downstream reads that result)
-__global__ void k_tv_scale_eig(double *__restrict__ t, const double *__restrict__ den, int N, int B)
+// Orthonormal DCT-II of the two real rows packed in Z = FFT(v1 + i v2) (v the reordered
+// rows): X_a[k] = s_k Re(e^{-i pi k/2N} V_a[k]), V1 = (Z[k] + conj Z[N-k]) / 2,
+// V2 = (Z[k] - conj Z[N-k]) / 2i; s_0 = sqrt(1/N), s_k = sqrt(2/N).
+__device__ inline double2 nm_dct_pair(const double2 *Z, int k, int N, const double2 *__restrict__ qw)
 {
-    const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const size_t P = (size_t)N * N;
-    if (idx >= P * B) return;
-    const int b = (int)(idx / P);
-    t[idx] /= den[idx - (size_t)b * P];
+    const double2 z = Z[k], zc = Z[k == 0 ? 0 : N - k];
+    const double v1r = 0.5 * (z.x + zc.x), v1i = 0.5 * (z.y - zc.y);
+    const double v2r = 0.5 * (z.y + zc.y), v2i = -0.5 * (z.x - zc.x);
+    const double2 q = qw[k];
+    const double s = k == 0 ? sqrt(1.0 / N) : sqrt(2.0 / N);
+    return make_double2(s * (q.x * v1r - q.y * v1i), s * (q.x * v2r - q.y * v2i));
+}
+
+// Inverse (orthonormal DCT-III) input for the two real rows X1, X2, given X_a[k] and
+// X_a[N-k] (0 for k = 0): Y = X / s, W = Y[k] - i Y[N-k], V = e^{i pi k/2N} W, and the
+// packed spectrum V1 + i V2 whose inverse FFT / N is v1 + i v2.
+__device__ inline double2 nm_idct_pair(double2 xk, double2 xr, int k, int N, const double2 *__restrict__ qw)
+{
+    const double is = k == 0 ? sqrt((double)N) : sqrt(0.5 * N);
+    const double isr = sqrt(0.5 * N);
+    const double y1 = xk.x * is, y1r = k == 0 ? 0.0 : xr.x * isr;
+    const double y2 = xk.y * is, y2r = k == 0 ? 0.0 : xr.y * isr;
+    const double2 q = qw[k];                   // conj(q) = e^{i pi k/2N}
+    const double v1r = q.x * y1 - q.y * y1r, v1i = -q.x * y1r - q.y * y1;
+    const double v2r = q.x * y2 - q.y * y2r, v2i = -q.x * y2r - q.y * y2;
+    return make_double2(v1r - v2i, v1i + v2r);
+}
+
+struct NmArgs {
+    int N, C;
+    NmFactors f;
+    const double2 *tw, *qw;
+    const double *lam;
+};
+
+// Pass 1, along z: rows = pairs of image columns (ix, ix + 1), loaded as the right-hand
+// side rhs = g - B'(b - d),
+//   (B' v)(iz,ix) = vz(iz,ix)[iz>0] - vz(iz+1,ix)[iz+1<N] + vx(iz,ix)[ix>0] - vx(iz,ix+1)[ix+1<N],
+// and written as their DCT-II along z to t.  Grid (ceil(ceil(N/2) / C), B).
+__global__ __launch_bounds__(256) void k_nm_z_fwd(const double *__restrict__ g, const double *__restrict__ bz,
+                                                  const double *__restrict__ bx, const double *__restrict__ dz,
+                                                  const double *__restrict__ dx, double *__restrict__ t,
+                                                  const int *__restrict__ active, NmArgs A)
+{
+    extern __shared__ double2 nm_sm[];
+    const int b = blockIdx.y;
+    if (!active[b]) return;
+    const int N = A.N, C = A.C;
+    const size_t o = (size_t)b * N * N;
+    double2 *a0 = nm_sm, *a1 = nm_sm + C * N;
+    const int col0 = blockIdx.x * 2 * C;
+    auto rhs = [&](int iz, int ix) -> double {
+        if (ix >= N) return 0.0;
+        const size_t k = o + (size_t)ix * N + iz;
+        double bt = 0.0;
+        if (iz > 0) bt += bz[k] - dz[k];
+        if (iz + 1 < N) bt -= bz[k + 1] - dz[k + 1];
+        if (ix > 0) bt += bx[k] - dx[k];
+        if (ix + 1 < N) bt -= bx[k + N] - dx[k + N];
+        return g[k] - bt;
+    };
+    for (int e = threadIdx.x; e < C * N; e += blockDim.x) {
+        const int c = e / N, i = e - c * N;
+        const int ix = col0 + 2 * c;
+        a0[c * N + nm_perm(i, N)] = make_double2(rhs(i, ix), rhs(i, ix + 1));
+    }
+    __syncthreads();
+    const double2 *Z = nm_fft(a0, a1, N, C, A.f, A.tw, false);
+    for (int e = threadIdx.x; e < C * N; e += blockDim.x) {
+        const int c = e / N, k = e - c * N;
+        const int ix = col0 + 2 * c;
+        const double2 X = nm_dct_pair(Z + c * N, k, N, A.qw);
+        if (ix < N) t[o + (size_t)ix * N + k] = X.x;
+        if (ix + 1 < N) t[o + (size_t)(ix + 1) * N + k] = X.y;
+    }
+}
+
+// Pass 2, along x: rows = pairs of z frequencies (kz, kz + 1); DCT-II along x, divide by
+// 1 + l_kz + l_kx, DCT-III along x, in place in t.
+__global__ __launch_bounds__(256) void k_nm_x(double *__restrict__ t, const int *__restrict__ active, NmArgs A)
+{
+    extern __shared__ double2 nm_sm[];
+    const int b = blockIdx.y;
+    if (!active[b]) return;
+    const int N = A.N, C = A.C;
+    const size_t o = (size_t)b * N * N;
+    double2 *a0 = nm_sm, *a1 = nm_sm + C * N;
+    const int row0 = blockIdx.x * 2 * C;
+    const int R = 2 * C;
+    // element (kz = row0 + r, ix): consecutive threads take consecutive kz
+    for (int e = threadIdx.x; e < R * N; e += blockDim.x) {
+        const int ix = e / R, r = e - ix * R;
+        const int kz = row0 + r;
+        const double v = kz < N ? t[o + (size_t)ix * N + kz] : 0.0;
+        reinterpret_cast<double *>(a0 + (r >> 1) * N + nm_perm(ix, N))[r & 1] = v;
+    }
+    __syncthreads();
+    double2 *Z = nm_fft(a0, a1, N, C, A.f, A.tw, false);
+    double2 *X = Z == a0 ? a1 : a0;
+    for (int e = threadIdx.x; e < C * N; e += blockDim.x) {
+        const int c = e / N, k = e - c * N;
+        const int kz = row0 + 2 * c;
+        double2 v = nm_dct_pair(Z + c * N, k, N, A.qw);
+        const double lk = A.lam[k];
+        v.x /= 1.0 + A.lam[kz < N ? kz : 0] + lk;
+        v.y /= 1.0 + A.lam[kz + 1 < N ? kz + 1 : 0] + lk;
+        X[c * N + k] = v;
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < C * N; e += blockDim.x) {
+        const int c = e / N, k = e - c * N;
+        const double2 *Xc = X + c * N;
+        Z[c * N + k] = nm_idct_pair(Xc[k], Xc[k == 0 ? 0 : N - k], k, N, A.qw);
+    }
+    __syncthreads();
+    const double2 *z = nm_fft(Z, X, N, C, A.f, A.tw, true);
+    const double inv = 1.0 / N;
+    for (int e = threadIdx.x; e < R * N; e += blockDim.x) {
+        const int ix = e / R, r = e - ix * R;
+        const int kz = row0 + r;
+        if (kz < N) t[o + (size_t)ix * N + kz] = reinterpret_cast<const double *>(z + (r >> 1) * N + nm_perm(ix, N))[r & 1] * inv;
+    }
+}
+
+// Pass 3, along z: rows = pairs of image columns; DCT-III along z from t into u.
+__global__ __launch_bounds__(256) void k_nm_z_inv(const double *__restrict__ t, double *__restrict__ u,
+                                                  const int *__restrict__ active, NmArgs A)
+{
+    extern __shared__ double2 nm_sm[];
+    const int b = blockIdx.y;
+    if (!active[b]) return;
+    const int N = A.N, C = A.C;
+    const size_t o = (size_t)b * N * N;
+    double2 *a0 = nm_sm, *a1 = nm_sm + C * N;
+    const int col0 = blockIdx.x * 2 * C;
+    for (int e = threadIdx.x; e < C * N; e += blockDim.x) {
+        const int c = e / N, k = e - c * N;
+        const int ix = col0 + 2 * c;
+        const int kr = k == 0 ? 0 : N - k;
+        double2 xk = make_double2(0.0, 0.0), xr = make_double2(0.0, 0.0);
+        if (ix < N) { xk.x = t[o + (size_t)ix * N + k]; xr.x = t[o + (size_t)ix * N + kr]; }
+        if (ix + 1 < N) { xk.y = t[o + (size_t)(ix + 1) * N + k]; xr.y = t[o + (size_t)(ix + 1) * N + kr]; }
+        a0[c * N + k] = nm_idct_pair(xk, xr, k, N, A.qw);
+    }
+    __syncthreads();
+    const double2 *z = nm_fft(a0, a1, N, C, A.f, A.tw, true);
+    const double inv = 1.0 / N;
+    for (int e = threadIdx.x; e < C * N; e += blockDim.x) {
+        const int c = e / N, i = e - c * N;
+        const int ix = col0 + 2 * c;
+        const double2 v = z[c * N + nm_perm(i, N)];
+        if (ix < N) u[o + (size_t)ix * N + i] = v.x * inv;
+        if (ix + 1 < N) u[o + (size_t)(ix + 1) * N + i] = v.y * inv;
+    }
 }
 
 // SB_ITV2's stop test on the device: an active item stops once
@@ -308,62 +484,81 @@ static inline int nblk(size_t n) { return (int)std::min<size_t>((n + 255) / 256,
 
 struct TvProx {
     int N = 0, B = 0;
-    double *C = nullptr, *den = nullptr;           // DCT-II basis [N x N], 1 + l_i + l_j
+    double2 *tw = nullptr, *qw = nullptr;          // e^{-2 pi i j/N}, e^{-i pi k/2N}
+    double *lam = nullptr;                         // 4 sin^2(pi k / 2N)
+    NmArgs nm{};
     double *g = nullptr, *u = nullptr, *up = nullptr, *bz = nullptr, *bx = nullptr, *dz = nullptr, *dx = nullptr;
-    double *rhs = nullptr, *t1 = nullptr, *mu = nullptr, *part = nullptr, *sums = nullptr;
+    double *t1 = nullptr, *mu = nullptr, *part = nullptr, *sums = nullptr;
     int *active = nullptr, *cnt = nullptr;
 };
 
+// N's prime factors, radix 4 first (fewer stages), then 2, 3, 5, 7, ...
+static bool nm_factors(int N, NmFactors &f)
+{
+    f.n = 0;
+    int n = N;
+    auto take = [&](int p) {
+        while (n % p == 0) {
+            if (f.n == kNmMaxFactors) return false;
+            f.p[f.n++] = p;
+            n /= p;
+        }
+        return true;
+    };
+    if (!take(4) || !take(2)) return false;
+    for (int p = 3; n > 1; p += 2)
+        if (!take(p)) return false;
+    return true;
+}
+
 static int prox_setup(TvProx &T, TvWork &w, int N, int B, hipStream_t s)
 {
+    if (N > kNmMaxN)
+        return set_error(MBREC_ERR_ARG, "mbrec:tvSize", "rec_nn_with_TV_reg: image side above 2048 is not supported");
     T.N = N;
     T.B = B;
     const size_t P = (size_t)N * N, n = P * B;
     int rc;
-    double **bufs[] = {&T.g, &T.u, &T.up, &T.bz, &T.bx, &T.dz, &T.dx, &T.rhs, &T.t1};
+    double **bufs[] = {&T.g, &T.u, &T.up, &T.bz, &T.bx, &T.dz, &T.dx, &T.t1};
     for (double **b : bufs)
         if ((rc = w.alloc(b, n))) return rc;
-    if ((rc = w.alloc(&T.C, P)) || (rc = w.alloc(&T.den, P)) || (rc = w.alloc(&T.mu, B)) ||
-        (rc = w.alloc(&T.part, (size_t)kMaxQ * B * kRedBlocks)) || (rc = w.alloc(&T.sums, 6 * (size_t)B)) ||
-        (rc = w.alloc(&T.active, B)) || (rc = w.alloc(&T.cnt, 1)))
+    if ((rc = w.alloc(&T.tw, N)) || (rc = w.alloc(&T.qw, N)) || (rc = w.alloc(&T.lam, N)) ||
+        (rc = w.alloc(&T.mu, B)) || (rc = w.alloc(&T.part, (size_t)kMaxQ * B * kRedBlocks)) ||
+        (rc = w.alloc(&T.sums, 6 * (size_t)B)) || (rc = w.alloc(&T.active, B)) || (rc = w.alloc(&T.cnt, 1)))
         return rc;
-    std::vector<double> C(P), den(P), lam(N);
+    std::vector<double2> tw(N), qw(N);
+    std::vector<double> lam(N);
     for (int k = 0; k < N; ++k) {
         lam[k] = 4.0 * std::pow(std::sin(M_PI * k / (2.0 * N)), 2);
-        for (int i = 0; i < N; ++i)
-            C[(size_t)k * N + i] = (k == 0) ? std::sqrt(1.0 / N) : std::sqrt(2.0 / N) * std::cos(M_PI * (i + 0.5) * k / N);
+        tw[k] = make_double2(std::cos(2.0 * M_PI * k / N), -std::sin(2.0 * M_PI * k / N));
+        qw[k] = make_double2(std::cos(M_PI * k / (2.0 * N)), -std::sin(M_PI * k / (2.0 * N)));
     }
-    for (int j = 0; j < N; ++j)
-        for (int i = 0; i < N; ++i) den[(size_t)j * N + i] = 1.0 + lam[i] + lam[j];
-    MBREC_HIP(hipMemcpyAsync(T.C, C.data(), sizeof(double) * P, hipMemcpyHostToDevice, s));
-    MBREC_HIP(hipMemcpyAsync(T.den, den.data(), sizeof(double) * P, hipMemcpyHostToDevice, s));
+    MBREC_HIP(hipMemcpyAsync(T.tw, tw.data(), sizeof(double2) * N, hipMemcpyHostToDevice, s));
+    MBREC_HIP(hipMemcpyAsync(T.qw, qw.data(), sizeof(double2) * N, hipMemcpyHostToDevice, s));
+    MBREC_HIP(hipMemcpyAsync(T.lam, lam.data(), sizeof(double) * N, hipMemcpyHostToDevice, s));
     MBREC_HIP(hipStreamSynchronize(s));
+    T.nm.N = N;
+    T.nm.C = std::max(1, std::min(4, kNmLdsPoints / N));
+    if (!nm_factors(N, T.nm.f))
+        return set_error(MBREC_ERR_ARG, "mbrec:tvSize", "rec_nn_with_TV_reg: too many prime factors in the image side");
+    T.nm.tw = T.tw;
+    T.nm.qw = T.qw;
+    T.nm.lam = T.lam;
     return MBREC_OK;
 }
 
-// u = (I + B'B)^-1 rhs for every item: U = C ((C' R C) ./ den) C' (strided-batched GEMMs)
-static int neumann_solve(TvProx &T, hipStream_t s)
+// u = (I + B'B)^-1 (g - B'(b - d)) for every active item: DCT-II along z (rhs formed on
+// load), along x (forward, divide by 1 + l_kz + l_kx, inverse), DCT-III along z
+static void neumann_solve(TvProx &T, hipStream_t s)
 {
-    rocblas_handle h;
-    int rc = blas_handle(s, &h);
-    if (rc) return rc;
-    const int N = T.N;
-    const rocblas_stride P = (rocblas_stride)N * N;
-    const double one = 1.0, zero = 0.0;
-    auto gemm = [&](rocblas_operation ta, rocblas_operation tb, const double *A, rocblas_stride sa, const double *B_,
-                    rocblas_stride sb, double *Cc) {
-        return rocblas_dgemm_strided_batched(h, ta, tb, N, N, N, &one, A, N, sa, B_, N, sb, &zero, Cc, N, P, T.B) ==
-               rocblas_status_success;
-    };
-    const auto nt = rocblas_operation_none, tr = rocblas_operation_transpose;
-    // t1 = C' R ; R <- t1 C ; R ./= den ; t1 = C R ; U = t1 C'
-    if (!gemm(tr, nt, T.C, 0, T.rhs, P, T.t1) || !gemm(nt, nt, T.t1, P, T.C, 0, T.rhs))
-        return set_error(MBREC_ERR_HIP, "mbrec:rocblas", "rocblas_dgemm_strided_batched failed");
-    hipLaunchKernelGGL(k_tv_scale_eig, dim3((unsigned)((P * T.B + 255) / 256)), dim3(256), 0, s, T.rhs,
-                       (const double *)T.den, N, T.B);
-    if (!gemm(nt, nt, T.C, 0, T.rhs, P, T.t1) || !gemm(nt, tr, T.t1, P, T.C, 0, T.u))
-        return set_error(MBREC_ERR_HIP, "mbrec:rocblas", "rocblas_dgemm_strided_batched failed");
-    return MBREC_OK;
+    const int N = T.N, C = T.nm.C;
+    const dim3 grid((unsigned)(((N + 1) / 2 + C - 1) / C), (unsigned)T.B);
+    const size_t lds = sizeof(double2) * 2 * C * N;
+    hipLaunchKernelGGL(k_nm_z_fwd, grid, dim3(256), lds, s, (const double *)T.g, (const double *)T.bz,
+                       (const double *)T.bx, (const double *)T.dz, (const double *)T.dx, T.t1,
+                       (const int *)T.active, T.nm);
+    hipLaunchKernelGGL(k_nm_x, grid, dim3(256), lds, s, T.t1, (const int *)T.active, T.nm);
+    hipLaunchKernelGGL(k_nm_z_inv, grid, dim3(256), lds, s, (const double *)T.t1, T.u, (const int *)T.active, T.nm);
 }
 
 // x (FP32, [B][N][N]) <- SB_ITV2(z, mu_b) per item; mu host [B]; items with skip[b] are left as they are.
@@ -383,16 +578,12 @@ static int tv_prox(TvProx &T, const float *z, float *x, const std::vector<double
     MBREC_HIP(hipMemcpyAsync(T.active, act.data(), sizeof(int) * B, hipMemcpyHostToDevice, s));
     hipLaunchKernelGGL(k_tv_load, dim3(nblk(n)), dim3(256), 0, s, z, T.g, T.up, T.bz, T.bx, T.dz, T.dx, n);
     hipLaunchKernelGGL(k_tv_zero_cnt, dim3(1), dim3(1), 0, s, T.cnt);
-    int rc;
     for (;;) {
         bool any = false;
         for (int b = 0; b < B; ++b) any = any || act[b];
         if (!any) break;
         for (int c = 0; c < kTvCheck; ++c) {
-            hipLaunchKernelGGL(k_tv_rhs, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, (const double *)T.g,
-                               (const double *)T.bz, (const double *)T.bx, (const double *)T.dz,
-                               (const double *)T.dx, T.rhs, N, B, (const int *)T.active);
-            if ((rc = neumann_solve(T, s))) return rc;
+            neumann_solve(T, s);
             hipLaunchKernelGGL(k_tv_shrink, vgrid(B), dim3(256), 0, s, (const double *)T.u, T.up, T.bz, T.bx, T.dz,
                                T.dx, N, B, (const double *)T.mu, (const int *)T.active, T.part);
             hipLaunchKernelGGL(k_tv_active, dim3(1), dim3(256), 0, s, (const double *)T.part, T.active, B, T.cnt);
@@ -547,4 +738,21 @@ extern "C" int mbrec_rec_nn_with_TV_reg_dev(mbrec_plan h, const float *data, int
         return set_error(MBREC_ERR_ARG, "mbrec:args", "rec_nn_with_TV_reg: invalid arguments");
     return nnls_tv(&h->p, data, B, coupled, num_iter, regParam, inner_iterations, inner_min_iterations, rec, lcurve,
                    (hipStream_t)s);
+}
+
+extern "C" int mbrec_sb_itv2_dev(const float *g, int N, int B, const double *mu, float *u, int *iterations,
+                                 mbrec_stream s)
+{
+    if (!g || !u || !mu || N <= 0 || B <= 0)
+        return set_error(MBREC_ERR_ARG, "mbrec:args", "SB_ITV2: invalid arguments");
+    TvWork w;
+    TvProx T;
+    const hipStream_t st = (hipStream_t)s;
+    int rc;
+    if ((rc = prox_setup(T, w, N, B, st))) return rc;
+    std::vector<double> m(mu, mu + B);
+    std::vector<int> skip(B, 0);
+    if ((rc = tv_prox(T, g, u, m, skip, st, iterations))) return rc;
+    MBREC_HIP(hipStreamSynchronize(st));
+    return MBREC_OK;
 }

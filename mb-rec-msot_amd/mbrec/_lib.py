@@ -60,6 +60,18 @@ SIGNATURES = [
     ("mbrec_rec_nn_with_L2_reg_dev", C.c_int,
      [c_void_p, c_void_p, C.c_int, C.c_int, C.c_int, C.c_double, c_void_p, c_void_p, c_void_p,
       c_void_p]),
+    ("mbrec_travel_times_dev", C.c_int,
+     [c_void_p, c_void_p, C.c_int, c_void_p, c_void_p, C.c_int, C.c_double, C.c_double, C.c_double, c_void_p,
+      C.c_int, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double, c_void_p, c_void_p, c_void_p]),
+    ("mbrec_spmr_grid_dev", C.c_int,
+     [c_void_p, c_void_p, C.c_int, C.c_int, c_void_p, c_void_p, C.c_int, C.c_double, C.c_double, c_void_p,
+      c_void_p, c_void_p]),
+    ("mbrec_spmr_index_dev", C.c_int,
+     [c_void_p, c_void_p, C.c_int, C.c_int, c_void_p, c_void_p, C.c_int, C.c_double, C.c_double, C.c_double,
+      C.c_int, c_void_p, c_void_p]),
+    ("mbrec_spmr_table_dev", C.c_int,
+     [c_void_p, C.c_int, c_void_p, C.c_int, C.c_double, C.c_int, C.c_double, C.c_double, c_void_p, C.c_int,
+      C.c_int, C.c_int, C.c_double, c_void_p, c_void_p, c_void_p]),
     ("mbrec_filter_butter_zero_phase_dev", C.c_int,
      [c_void_p, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, c_void_p, c_void_p]),
     ("mbrec_apply_butterworth_window_dev", C.c_int,
@@ -77,6 +89,7 @@ SIGNATURES = [
     ("mbrec_rec_nn_with_TV_reg_dev", C.c_int,
      [c_void_p, c_void_p, C.c_int, C.c_int, C.c_int, C.c_double, C.c_int, C.c_int, c_void_p, c_void_p,
       c_void_p]),
+    ("mbrec_sb_itv2_dev", C.c_int, [c_void_p, C.c_int, C.c_int, c_void_p, c_void_p, c_void_p, c_void_p]),
     ("mbrec_dmas_cf_dev", C.c_int,
      [c_void_p, c_void_p, C.c_int, c_void_p, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_int,
       C.c_double, C.c_double, c_void_p, c_void_p]),

@@ -79,15 +79,20 @@ class Discretization:
         resid = np.abs(K - np.outer(kz, kx)).max() / np.abs(K).max()
 
         def taps(k):
+            # offsets -Kh..Kh of the circular kernel, Kh the largest offset with a tap above
+            # tol; for an even N the N/2 tap is one sample reached from both sides, so it
+            # is kept once (at +N/2, with 0 at -N/2)
             N = k.shape[0]
             kmax = np.abs(k).max()
-            half = (N - 1) // 2
             Kh = 0
-            for a in range(half, -1, -1):
+            for a in range(N // 2, -1, -1):
                 if max(abs(k[a % N]), abs(k[-a % N])) > tol * kmax:
                     Kh = a
                     break
-            return np.array([k[a % N] for a in range(-Kh, Kh + 1)]), Kh
+            t = np.array([k[a % N] for a in range(-Kh, Kh + 1)])
+            if Kh > 0 and 2 * Kh == N:
+                t[0] = 0.0
+            return t, Kh
 
         tz, Kz = taps(kz)
         tx, Kx = taps(kx)

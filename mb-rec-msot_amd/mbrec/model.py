@@ -21,18 +21,105 @@ from .irf import impulse_response_with_sir, impulse_response_without_sir
 from .probe import Probe
 from .sir import sir_tables_for_model
 
+MEMBRANE_SAMPLES = 8193   # samples of the membrane curve over [-R, R] handed to the device
 
-def travel_time_indices(disc: Discretization, probe: Probe, c: float, ncrop: int) -> np.ndarray:
-    """Single speed of sound (refraction.m:20-24 same-side branch, MSOTmodel.m:380-381):
-    tau[d, x] = |p_x - q_d| / c * fs - delay - ncrop, shape (D, Nx, Nz)."""
-    X = disc.region["xPositionsFov"].T          # (Nx, Nz): memory order x = iz + Nz*ix
-    Z = disc.region["zPositionsFov"].T
-    xd = probe.detector.xPositionsOfTransducers
-    zd = probe.detector.zPositionsOfTransducers
-    dx = xd[:, None, None] - X[None]
-    dz = zd[:, None, None] - Z[None]
-    t = np.sqrt(dx * dx + dz * dz) / c
+
+def acoustic_data(disc: Discretization, probe: Probe, c: float, single_sos: bool = True):
+    """calculate_and_save_acoustic_data_for_model_with_SPMR_or_singSoS (MSOTmodel.m:436-461)
+    on the device (calculate_travel_time.m, refraction.m:10-24, fermat_principle.m:12-32):
+    travel times t[d, x] in seconds -- the straight-path time |p_x - q_d| / c for a single
+    speed of sound; for the dual speed of sound (c below the membrane,
+    probe.coupling.speedOfSound above it) the Fermat minimum over the membrane crossing for
+    pixels below the membrane -- and the crossing x (NaN where the path is straight).
+    Returns (t, x_intersections) as (D, Nx, Nz) arrays."""
+    import torch
+    L = _lib.lib()
+    _lib.check(L.mbrec_init(0))
+    X = np.ascontiguousarray(disc.region["xPositionsFov"].T, dtype=np.float64)   # (Nx, Nz): x = iz + Nz*ix
+    Z = np.ascontiguousarray(disc.region["zPositionsFov"].T, dtype=np.float64)
+    xd = np.ascontiguousarray(probe.detector.xPositionsOfTransducers, dtype=np.float64)
+    zd = np.ascontiguousarray(probe.detector.zPositionsOfTransducers, dtype=np.float64)
+    D, P = xd.size, X.size
+    Xd, Zd = torch.as_tensor(X.ravel()).cuda(), torch.as_tensor(Z.ravel()).cuda()
+    t = torch.empty((D, P), dtype=torch.float64, device="cuda")
+    xi = torch.empty_like(t)
+    R = float(probe.detector.radius)
+    if single_sos:
+        mz, nm, x0, dxm, c_above, c_below = None, 0, 0.0, 1.0, float(c), float(c)
+    else:
+        if probe.membrane is None or probe.coupling.speedOfSound is None:
+            raise ValueError("Dual speed of sound requires membrane model.")
+        xs = np.linspace(-R, R, MEMBRANE_SAMPLES)
+        mz = np.ascontiguousarray(probe.membrane(xs), dtype=np.float64)
+        nm, x0, dxm = MEMBRANE_SAMPLES, -R, xs[1] - xs[0]
+        c_above, c_below = float(probe.coupling.speedOfSound), float(c)
+    _lib.check(L.mbrec_travel_times_dev(Xd.data_ptr(), Zd.data_ptr(), P, xd.ctypes.data, zd.ctypes.data, D, c_below,
+                                        c_above, R, mz.ctypes.data if mz is not None else None, nm, x0, dxm, 1.0,
+                                        0.0, 0.0, t.data_ptr(), xi.data_ptr(), _lib.stream_handle()))
+    return t.cpu().numpy().reshape(D, *X.shape), xi.cpu().numpy().reshape(D, *X.shape)
+
+
+def travel_time_indices(disc: Discretization, probe: Probe, c: float, ncrop: int, single_sos: bool = True,
+                        cache_dir=None):
+    """MSOTmodel.m:350-382: tau[d, x] = t * fs - delayBeforeRecording - ncrop, shape
+    (D, Nx, Nz), t from ``acoustic_data`` or from the acoustic cache (``cache_dir``)."""
+    t, _ = load_or_compute_acoustic_data(disc, probe, c, single_sos, cache_dir)
     return t * probe.DAC.frequency - probe.DAC.delayBeforeRecording - ncrop
+
+
+def _acoustic_hash_fields(disc, probe, c, single_sos):
+    """hash_to_load_and_save_acoustic_data (MSOTmodel.m:289-308): coupling speed of sound
+    (the medium's for a single speed of sound), detector positions, medium, region,
+    useSingleSos and the membrane evaluated along the region's first row (+inf for a
+    single speed of sound)."""
+    xrow = np.asarray(disc.region["xPositionsFov"], dtype=np.float64)[0, :]
+    if single_sos:
+        coupling, mem = float(c), np.full(xrow.shape, np.inf)
+    else:
+        coupling, mem = float(probe.coupling.speedOfSound), np.asarray(probe.membrane(xrow), dtype=np.float64)
+    return {"coupling_speedOfSound": np.float64(coupling),
+            "detector_x": np.asarray(probe.detector.xPositionsOfTransducers, dtype=np.float64),
+            "detector_y": np.asarray(probe.detector.zPositionsOfTransducers, dtype=np.float64),
+            "Medium_speedOfSound": np.float64(c),
+            "region_xPositionsFov": np.asarray(disc.region["xPositionsFov"], dtype=np.float64),
+            "region_zPositionsFov": np.asarray(disc.region["zPositionsFov"], dtype=np.float64),
+            "useSingleSos": np.bool_(single_sos), "membrane": mem}
+
+
+def acoustic_hash(fields) -> str:
+    """MD5 over the fields' names, dtypes, shapes and bytes in name order (the reference uses
+    DataHash of the same struct; the byte stream differs, so files do not interchange)."""
+    import hashlib
+    h = hashlib.md5()
+    for k in sorted(fields):
+        a = np.ascontiguousarray(fields[k])
+        h.update(k.encode())
+        h.update(str(a.dtype).encode())
+        h.update(str(a.shape).encode())
+        h.update(a.tobytes())
+    return h.hexdigest()
+
+
+def load_or_compute_acoustic_data(disc, probe, c, single_sos=True, cache_dir=None):
+    """calculate_travel_time_indices' cache (MSOTmodel.m:351-377): with cache_dir, the
+    travel times and membrane crossings are read from acoustic_file_<hash>.npz when it
+    exists -- after checking that its stored hashed fields equal the current ones ('Hash
+    collision: Delete outdated travel time files') -- and written there otherwise."""
+    import os
+    if cache_dir is None:
+        return acoustic_data(disc, probe, c, single_sos)
+    fields = _acoustic_hash_fields(disc, probe, c, single_sos)
+    path = os.path.join(cache_dir, f"acoustic_file_{acoustic_hash(fields)}.npz")
+    if os.path.exists(path):
+        z = np.load(path, allow_pickle=False)
+        for k, v in fields.items():
+            if not np.array_equal(z["hashed_" + k], v):
+                raise AssertionError("Hash collision: Delete outdated travel time files")
+        return z["travel_times"], z["x_membrane_intersections"]
+    t, xi = acoustic_data(disc, probe, c, single_sos)
+    os.makedirs(cache_dir, exist_ok=True)
+    np.savez(path, travel_times=t, x_membrane_intersections=xi, **{"hashed_" + k: v for k, v in fields.items()})
+    return t, xi
 
 
 class DevicePlan:
@@ -152,17 +239,19 @@ class MSOTmodel:
                  include_sir: bool, use_single_speed_of_sound: bool = True,
                  num_cropped_samples_at_sinogram_start: int = 0, filt_cutoff_min: float = 1e5,
                  filt_cutoff_max: float = 12e6, model_normalization_factor=None,
-                 sigma_iterations: int = 50, progress=None):
-        """progress: optional callable receiving a line after each set-up stage."""
+                 sigma_iterations: int = 50, progress=None, acoustic_cache_dir=None, spmr_dir=None):
+        """progress: optional callable receiving a line after each set-up stage;
+        acoustic_cache_dir: directory of the acoustic_file_<hash>.npz travel-time cache
+        (MSOTmodel.m:351-377), None for no cache; spmr_dir: directory of the polarSPMR files
+        (MSOTmodel.m:482-571), None to generate the SIR table every time."""
         say = progress or (lambda msg: None)
         t_start = time.time()
-        if not use_single_speed_of_sound:
-            raise NotImplementedError("dual speed of sound travel times are not provided")
         self.Probe = probe
         self.Discretization = discretization
         self.Medium = SimpleNamespace(speedOfSound=float(speed_of_sound_below_membrane))
         self.includeSir = bool(include_sir)
-        self.useSingleSpeedOfSound = True
+        self.useSingleSpeedOfSound = bool(use_single_speed_of_sound)
+        single = self.useSingleSpeedOfSound
         ncrop = int(num_cropped_samples_at_sinogram_start)
         self.DataPreprocessing = SimpleNamespace(numCroppedSamplesAtSinogramStart=ncrop,
                                                  filtCutoffMin=filt_cutoff_min,
@@ -170,7 +259,8 @@ class MSOTmodel:
         c = self.Medium.speedOfSound
         fs = probe.DAC.frequency
         nT = int(probe.DAC.numRecordedSamplesPerTransducer - ncrop)
-        tt = travel_time_indices(discretization, probe, c, ncrop)
+        t_sec, x_mem = load_or_compute_acoustic_data(discretization, probe, c, single, acoustic_cache_dir)
+        tt = t_sec * fs - probe.DAC.delayBeforeRecording - ncrop     # MSOTmodel.m:380-381
         Z = discretization.region["zPositionsFov"]
         R = (Z[1, 0] - Z[0, 0]) / (c / fs)                      # MSOTmodel.m:36-38
         self.Data = SimpleNamespace(numTimeSamplesPerTransducer=nT, absoluteTravelTimeIndices=tt,
@@ -181,10 +271,23 @@ class MSOTmodel:
             X = discretization.region["xPositionsFov"].T.ravel()
             Zf = discretization.region["zPositionsFov"].T.ravel()
             res = min(discretization.dx, discretization.dz)
+            c_spmr = c
+            if not single:
+                # virtual pixel positions along the detector-side ray (MSOTmodel.m:582-593)
+                D_ = probe.detector.numOfTransducers
+                c_spmr = probe.coupling.speedOfSound
+                xq = probe.detector.xPositionsOfTransducers[:, None]
+                zq = probe.detector.zPositionsOfTransducers[:, None]
+                xm = x_mem.reshape(D_, -1)
+                zm = probe.membrane(np.nan_to_num(xm))
+                nrm = np.sqrt((xm - xq) ** 2 + (zm - zq) ** 2)
+                run = t_sec.reshape(D_, -1) * c_spmr
+                X = np.where(np.isnan(xm), X[None, :], xq + (xm - xq) / nrm * run)
+                Zf = np.where(np.isnan(xm), Zf[None, :], zq + (zm - zq) / nrm * run)
             table, se, sidx, info = sir_tables_for_model(
                 X, Zf, probe.detector.xPositionsOfTransducers, probe.detector.zPositionsOfTransducers,
-                res, c, fs, probe.detector.elementWidth, probe.detector.numSubElements,
-                discretization.pixelModel)
+                res, c_spmr, fs, probe.detector.elementWidth, probe.detector.numSubElements,
+                discretization.pixelModel, spmr_dir=spmr_dir, probe_id=probe.probeId)
             self.Data.SPMR = table
             self.Data.SPMRTimestepIndexStartAndEnd = se
             self.Data.SPMRIndices = sidx

@@ -48,16 +48,34 @@ def arc_positions(n: int, radius: float, coverage_deg: float):
     return radius * np.cos(ang), -radius * np.sin(ang)
 
 
+def membrane_model_horizontal_example(x):
+    """Probe.m:148-151: the example membrane, a horizontal line at z = -10 mm."""
+    return -0.01 * np.ones_like(np.asarray(x, dtype=np.float64))
+
+
 _REGISTRY = {
+    # coupling medium heavy water at 1397 m/s and the horizontal example membrane (Probe.m:101-106)
     "example_system": dict(n=256, radius=0.060, coverage=145.0, samples=2030, delay=500, fs=4e7,
-                           width=0.2e-3, height=20e-3, sub=8),
+                           width=0.2e-3, height=20e-3, sub=8, coupling_sos=1397.0,
+                           membrane=membrane_model_horizontal_example),
     "arc256_r40_174": dict(n=256, radius=0.040, coverage=174.0, samples=2030, delay=0, fs=4e7,
-                           width=0.5e-3, height=15e-3, sub=8),
+                           width=0.5e-3, height=15e-3, sub=8, coupling_sos=None, membrane=None),
 }
+
+
+@dataclass
+class Coupling:
+    speedOfSound: float | None = None
 
 
 class Probe:
     eirLength = EIR_LENGTH
+
+    def set_custom_membrane_model(self, custom_membrane_model):
+        """Probe.m:88-94: a callable z = m(x) (vectorised over numpy arrays)."""
+        if not callable(custom_membrane_model):
+            raise ValueError("Custom membrane model must be function handle.")
+        self.membrane = custom_membrane_model
 
     def __init__(self, probe_id: str, include_eir: bool = True, include_indiv_eir: bool = False,
                  num_detectors: int | None = None):
@@ -72,6 +90,8 @@ class Probe:
         self.detector = Detector(n, cfg["radius"], cfg["coverage"], x, z, cfg["width"], cfg["height"],
                                  cfg["sub"])
         self.DAC = DAC(cfg["samples"], cfg["delay"], cfg["fs"])
+        self.coupling = Coupling(cfg["coupling_sos"])
+        self.membrane = cfg["membrane"]
         if include_eir:
             self.detector.EIR = gauspuls_eir(cfg["fs"])
         else:

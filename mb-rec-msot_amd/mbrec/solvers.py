@@ -7,7 +7,9 @@ order); ``rec_img`` is (W, Nx, Nz) or (Nx, Nz).
 """
 from __future__ import annotations
 
+import ctypes
 import time
+from ctypes import c_int as C_int
 
 import numpy as np
 
@@ -277,3 +279,26 @@ def rec_nn_with_TV_reg(model, data_filt, num_iter, regParam, *, coupled=True, in
         out = rec.cpu().numpy().astype(np.float64)
         rec = out[0] if np.ndim(data_filt) == 2 else out
     return rec, e_img, e_reg, err, t
+
+
+def SB_ITV2(g, mu):
+    """SB_ITV2.m:1-40: arg min_u 1/2||u - g||^2 + mu ITV(u) by split Bregman on the device.
+
+    g: one square image (N, N) or a stack (B, N, N) in this package's layout ((Nx, Nz),
+    z fastest), numpy or a device tensor; mu a scalar or one value per image.  The inner
+    cgs(I + B'B, ., 1e-5, 100) of SB_ITV2.m:31 is the exact DCT Neumann solve.  Returns
+    (u, iterations) with u of the same kind and shape (FP32 on the device)."""
+    import torch
+    host = not hasattr(g, "data_ptr")
+    t = torch.as_tensor(np.asarray(g, dtype=np.float32)).cuda() if host else g.float()
+    t = t.contiguous()
+    N = t.shape[-1]
+    if t.shape[-2] != N:
+        raise ValueError("SB_ITV2: square images only")
+    B = t.numel() // (N * N)
+    m = np.ascontiguousarray(np.broadcast_to(np.asarray(mu, dtype=np.float64), (B,)))
+    u = torch.empty_like(t)
+    it = C_int()
+    _lib.check(_lib.lib().mbrec_sb_itv2_dev(t.data_ptr(), int(N), int(B), m.ctypes.data, u.data_ptr(),
+                                            ctypes.addressof(it), _lib.stream_handle()))
+    return (u.cpu().numpy().astype(np.float64) if host else u), it.value

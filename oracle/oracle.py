@@ -622,6 +622,32 @@ def travel_times(fov_x, fov_z, xd, zd, c, fs, delay, ncrop):
     return t * fs - delay - ncrop
 
 
+def travel_times_fermat(fov_x, fov_z, xd, zd, c_below, c_above, membrane, radius):
+    """Dual speed of sound (refraction.m:10-24, fermat_principle.m:12-32): for a pixel below
+    the membrane (z > m(x)) the minimum over the crossing x in [-radius, radius] of
+    |p - (x, m(x))| / c_below + |(x, m(x)) - q| / c_above (scipy's bounded Brent search,
+    xatol 1e-12: tighter than fminbnd's default TolX of 1e-4 the reference runs with);
+    straight |q - p| / c_above otherwise.  Returns (t seconds, crossing x or NaN) as (D, P).
+    Pinned by Snell's law, solved as a quartic, for a horizontal membrane (test_geometry.py)."""
+    from scipy.optimize import minimize_scalar
+    X = np.asarray(fov_x, dtype=np.float64).ravel()
+    Z = np.asarray(fov_z, dtype=np.float64).ravel()
+    D, P = len(xd), X.size
+    t = np.empty((D, P))
+    xi = np.full((D, P), np.nan)
+    for d in range(D):
+        for x in range(P):
+            xp, zp, xq, zq = X[x], Z[x], xd[d], zd[d]
+            if not zp > membrane(np.array([xp]))[0]:
+                t[d, x] = np.hypot(xq - xp, zq - zp) / c_above
+                continue
+            f = lambda u: (np.hypot(xp - u, zp - membrane(np.array([u]))[0]) / c_below +
+                           np.hypot(u - xq, membrane(np.array([u]))[0] - zq) / c_above)
+            r = minimize_scalar(f, bounds=(-radius, radius), method="bounded", options={"xatol": 1e-12})
+            t[d, x], xi[d, x] = r.fun, r.x
+    return t, xi
+
+
 def spmr_polar(fov_x, fov_z, xd, zd):
     """MSOTmodel.m:573-597 (single speed of sound): distance from every detector to every
     pixel and the azimuth angle between the detector's inward direction -q and q - p:
@@ -633,6 +659,53 @@ def spmr_polar(fov_x, fov_z, xd, zd):
     # the cosine is clipped to [-1, 1]: a rounding excess past 1 would make acos complex
     ang = np.arccos(np.clip(((xq - X) * xq + (zq - Z) * zq) / (np.sqrt(xq ** 2 + zq ** 2) * dist), -1.0, 1.0))
     return dist, ang
+
+
+def spmr_table_numpy(dists, angles_deg, width, n_sub, c, fs, pixel_model, Ls=91, trim=1e-4):
+    """The SIR table generator (the product's replacement for calculate_polarSPMR.m's Field
+    II step) restated in numpy: per (distance, angle) entry, angle fastest, n_sub point
+    sub-elements across the element width, each arrival offset (d_i - rho)/c fs deposited
+    as a linear hat centred at Ls // 2 (both samples inside the grid), the sum convolved
+    with the N-shape phi(c t)(-t) taken on the same grid where it exceeds 1e-12 of its
+    peak, then per entry (calculate_polarSPMR.m:152-161) samples below trim x the entry's
+    peak zeroed and the window = first / last non-zero sample (1-based; empty: (1, 0)).
+    Pinned by the one-sub-element case (the N-shape itself), the conserved tap sum and the
+    window rule (tests/test_sir.py)."""
+    nd, na = len(dists), len(angles_deg)
+    rho = np.repeat(np.asarray(dists, dtype=np.float64), na)
+    alpha = np.tile(np.asarray(angles_deg, dtype=np.float64), nd) * np.pi / 180.0
+    e = (np.arange(n_sub) + 0.5) * width / n_sub - width / 2.0
+    lat = np.zeros((nd * na, Ls))
+    centre = Ls // 2
+    sa = np.sin(alpha)
+    for ei in e:
+        di = np.sqrt(rho * rho + ei * ei - 2.0 * rho * ei * sa)
+        dt = (di - rho) / c * fs + centre
+        lo = np.floor(dt).astype(np.int64)
+        fr = dt - lo
+        ok = (lo >= 0) & (lo + 1 < Ls)
+        rows = np.nonzero(ok)[0]
+        np.add.at(lat, (rows, lo[ok]), (1.0 - fr[ok]) / n_sub)
+        np.add.at(lat, (rows, lo[ok] + 1), fr[ok] / n_sub)
+    kn = np.arange(-centre, centre + 1)
+    t = kn / fs
+    nsh = pixel_model(c * t) * (-t)
+    keep = np.nonzero(np.abs(nsh) > 1e-12 * np.abs(nsh).max())[0]
+    nsh, off = nsh[keep[0]:keep[-1] + 1], kn[keep[0]]
+    out = np.zeros_like(lat)
+    for j, h in enumerate(nsh):
+        sh = j + off
+        if sh >= 0:
+            out[:, sh:] += h * lat[:, :Ls - sh]
+        else:
+            out[:, :Ls + sh] += h * lat[:, -sh:]
+    mx = np.abs(out).max(axis=1, keepdims=True)
+    out[np.abs(out) < mx * trim] = 0.0
+    nz = out != 0
+    any_ = nz.any(axis=1)
+    first = np.where(any_, nz.argmax(axis=1) + 1, 1)
+    last = np.where(any_, Ls - nz[:, ::-1].argmax(axis=1), 0)
+    return out, np.stack([first, last], axis=1).astype(np.int32)
 
 
 def spmr_indices(dist, ang, dmin, dd, da_deg, n_angle):

@@ -8,7 +8,8 @@ C2  400^2 / 100 um with SIR: forward and transpose within 1e-5 of the oracle, th
 C3  the 224-sinogram batch over 200^2 / 300^2 / 400^2 grids with crop lengths U{1500..2030}
     in one graph: one sinogram per grid within 1e-3 of the cropped-model oracle's solve,
     graph replay = direct launch bit for bit for all 224.
-C4  400^2, batch 64, learned warm start: the warm-started loop run to the iteration count
+C4  400^2, batch 64, learned warm start: fewer outer iterations to 1e-3 than the cold start;
+    the warm-started loop run to the iteration count
     at which the whole batch is within 1e-3 of the device's converged solution; two
     sinograms checked against the oracle's FP64 converged solution (1e-3, plus the FP32
     operator's own 1e-5 scale).
@@ -181,6 +182,9 @@ def test_c4_learned_warm_start_400():
     k, _ = iterations_to_tolerance(m, y, lam, ref, 1e-3, w0, v0)
     _log(f"C4: {k} outer iterations from the network's state ({int(used.sum())}/{B} items use it)")
     assert k is not None
+    k_cold, _ = iterations_to_tolerance(m, y, lam, ref, 1e-3)
+    _log(f"C4: cold start {k_cold} outer iterations: the network saves {k_cold - k} ({k_cold / k:.2f}x)")
+    assert k_cold is not None and k < k_cold
     rec = rec_nn_with_L2_reg_batch(m, y, k, lam, init=w0, init_dual=v0)
     orc = _oracle_of(m)
     pick = [0, 37]

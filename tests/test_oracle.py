@@ -165,6 +165,27 @@ def test_prefilter_separable_matches_reference_fft(Nz, Nx):
         np.testing.assert_allclose(a, b, rtol=1e-9, atol=1e-9)
 
 
+@pytest.mark.parametrize("Nz,Nx", [(12, 10), (13, 11), (24, 24), (6, 7)])
+def test_product_prefilter_taps_rebuild_the_circular_kernel(Nz, Nx):
+    """The device prefilter's tap lists (offsets -K..K, the N/2 tap of an even axis kept
+    once), placed circularly, give the oracle's full 2-D kernel (taps below 1e-10 of the
+    largest dropped)."""
+    from mbrec.discretization import Discretization
+    h = 1.25e-4
+    d = Discretization([-(Nx - 1) / 2 * h, (Nx - 1) / 2 * h, -(Nz - 1) / 2 * h, (Nz - 1) / 2 * h], [Nx, Nz])
+    assert 2 * d.Kz <= Nz and 2 * d.Kx <= Nx
+
+    def circ(t, K, N):
+        c = np.zeros(N)
+        for a in range(-K, K + 1):
+            c[a % N] += t[a + K]
+        return c
+    kz, kx = O.full_kernels(Nz, Nx, d.dz, d.dx, d.pixelSigma)
+    full = np.outer(kz, kx)
+    np.testing.assert_allclose(np.outer(circ(d.kz, d.Kz, Nz), circ(d.kx, d.Kx, Nx)), full, rtol=0,
+                               atol=1e-9 * np.abs(full).max())
+
+
 # --- IR design -----------------------------------------------------------------
 def test_butterworth_design_matches_textbook():
     from mbrec.irf import butter_zero_phase_response
@@ -207,23 +228,18 @@ def test_travel_times_hand_values():
     np.testing.assert_allclose(tt[:, 1], [447.2135955 - 15, 985.0, 1612.4515497 - 15], atol=1e-6)
 
 
-def test_travel_times_match_independent_geometry_and_product():
+def test_travel_times_match_independent_geometry():
     """The oracle's table on a 9 x 7 grid and 16 detectors equals the independent helper's
-    and the product's host table (MSOTmodel travel_time_indices)."""
+    (the product's device table is compared with it in test_geometry.py)."""
     from helpers import geometry_tt
     from mbrec.discretization import Discretization
-    from mbrec.model import travel_time_indices
-    from mbrec.probe import Probe
     Nz, Nx, h, D = 9, 7, 1e-4, 16
     disc = Discretization([-(Nx - 1) / 2 * h, (Nx - 1) / 2 * h, -(Nz - 1) / 2 * h, (Nz - 1) / 2 * h], [Nx, Nz])
-    probe = Probe("arc256_r40_174", True, False, num_detectors=D)
-    probe.DAC.delayBeforeRecording = 7
     X, Z = disc.region["xPositionsFov"].T, disc.region["zPositionsFov"].T
     xd, zd = O.arc_detectors(D, 0.040, 174.0)
     tt = O.travel_times(X, Z, xd, zd, 1500.0, 4e7, 7, 3)
     want = geometry_tt(Nz, Nx, h, D, 0.040, 174.0, 1500.0, 4e7, delay=10).reshape(D, -1)
     np.testing.assert_allclose(tt, want, rtol=0, atol=1e-9)
-    np.testing.assert_allclose(travel_time_indices(disc, probe, 1500.0, 3).reshape(D, -1), tt, rtol=0, atol=1e-9)
 
 
 def test_spmr_polar_hand_values():
