@@ -174,6 +174,15 @@ int mbrec_model_forward_window_dev(mbrec_plan plan, const float *img, float *sin
 int mbrec_model_transpose_window_dev(mbrec_plan plan, const float *sino, float *img, int B,
                                      const int32_t *windows, mbrec_stream s);
 
+/* Model transpose that allocates nothing (no counterpart in the reference): for callers
+ * that cannot allocate device memory -- code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc.  Same kernels and result as mbrec_model_transpose_dev;
+ * its scratch (IR-filtered sinograms, bins, prefilter images, detector-group partial
+ * images) is a file-scope device array of 64 MiB per GPU, held until the process ends.
+ * A request that needs more (large B or grids) fails with mbrec:scratchTooSmall.  Every
+ * call on one GPU shares that GPU's array: calls that overlap on one GPU race. */
+int mbrec_model_transpose_noalloc_dev(mbrec_plan plan, const float *sino, float *img, int B, mbrec_stream s);
+
 /* rec_nn_with_L2_reg (reconstruction/rec_nn_with_L2_reg.m:1-66) with RegL2 = identity
  * (mbrec_rec_nn_with_L2_reg_terms_dev below takes other L2 terms)
  * and nnls_L2_less (nnls_L2_less.m:1-101): 50-style outer splitting, 2 LSQR inner

@@ -57,6 +57,7 @@ SIGNATURES = [
     ("mbrec_model_transpose_dev", C.c_int, [c_void_p, c_void_p, c_void_p, C.c_int, c_void_p]),
     ("mbrec_model_forward_window_dev", C.c_int, [c_void_p, c_void_p, c_void_p, C.c_int, c_void_p, c_void_p]),
     ("mbrec_model_transpose_window_dev", C.c_int, [c_void_p, c_void_p, c_void_p, C.c_int, c_void_p, c_void_p]),
+    ("mbrec_model_transpose_noalloc_dev", C.c_int, [c_void_p, c_void_p, c_void_p, C.c_int, c_void_p]),
     ("mbrec_rec_nn_with_L2_reg_dev", C.c_int,
      [c_void_p, c_void_p, C.c_int, C.c_int, C.c_int, C.c_double, c_void_p, c_void_p, c_void_p,
       c_void_p]),
