@@ -52,6 +52,9 @@ def parse():
     ap.add_argument("--chunk", type=int, default=25, help="granularity of the iterations-to-1e-3 search")
     ap.add_argument("--weights", default=None, help="warm-start network weights (.npz)")
     ap.add_argument("--mex", action="store_true", help="per-call time of the four MEX drop-ins at 400^2")
+    ap.add_argument("--pair-b1", action="store_true",
+                    help="one-sinogram operator pair at 400^2 DAS: every forward choice and tile-gather "
+                         "variant timed (HIP events) and checked against the default; prints one JSON line")
     ap.add_argument("--mex-reps", type=int, default=20)
     ap.add_argument("--grid", type=int, default=400)
     ap.add_argument("--sir", action="store_true", help="400x400 with SIR (the SIR parity configuration)")
@@ -104,17 +107,26 @@ def traffic_from(fetch_csv, write_csv, N=400, B=64):
     copy = [n for n in f if "copyBuffer" in n]
     cal_f = CALIB_BYTES / max(f[copy[0]]) if copy else 1.0
     cal_w = CALIB_BYTES / max(w[copy[0]]) if copy else 1.0
-    out = {"calibration": {"copy_bytes": CALIB_BYTES, "fetch_factor": round(cal_f, 4), "write_factor": round(cal_w, 4)}}
-    # the kernel the solve launches most often of each kind (the tuned variant)
-    for kind, pat in (("scatter", "k_fwd_pull"), ("gather", "k_gather")):
-        names = [n for n in f if pat in n and n in w]
-        if not names:
-            continue
-        name = max(names, key=lambda n: len(f[n]))
-        fb, wb = statistics.median(f[name]) * cal_f, statistics.median(w[name]) * cal_w
-        out[f"das_{kind}_B{B}_N{N}"] = {"kernel": name.split("(")[0], "dispatches": len(f[name]),
-                                        "fetch_bytes": fb, "write_bytes": wb, "hbm_bytes_per_launch": fb + wb}
     path = os.path.join(ROOT, "profiles", "traffic.json")
+    out = json.load(open(path)) if os.path.exists(path) else {}
+    out["calibration"] = {"copy_bytes": CALIB_BYTES, "fetch_factor": round(cal_f, 4), "write_factor": round(cal_w, 4)}
+    # every operator kernel of the run (median over its dispatches), under the workload
+    per = {}
+    for name in f:
+        if name in w and ("k_fwd" in name or "k_gather" in name or "k_sum_parts" in name or "k_img_to_slots" in name):
+            fb, wb = statistics.median(f[name]) * cal_f, statistics.median(w[name]) * cal_w
+            per[name.split("(")[0]] = {"dispatches": len(f[name]), "fetch_bytes": fb, "write_bytes": wb,
+                                       "hbm_bytes_per_launch": fb + wb}
+    out[f"kernels_B{B}_N{N}"] = per
+    if B > 1:   # the kernel the solve launches most often of each kind (the tuned variant)
+        for kind, pat in (("scatter", "k_fwd_pull"), ("gather", "k_gather")):
+            names = [n for n in f if pat in n and n in w]
+            if not names:
+                continue
+            name = max(names, key=lambda n: len(f[n]))
+            fb, wb = statistics.median(f[name]) * cal_f, statistics.median(w[name]) * cal_w
+            out[f"das_{kind}_B{B}_N{N}"] = {"kernel": name.split("(")[0], "dispatches": len(f[name]),
+                                            "fetch_bytes": fb, "write_bytes": wb, "hbm_bytes_per_launch": fb + wb}
     with open(path, "w") as fh:
         json.dump(out, fh, indent=1)
     print(json.dumps(out, indent=1))
@@ -522,6 +534,57 @@ def events_ms(fn, steps, warmup):
     return e0.elapsed_time(e1) / steps, out
 
 
+def pair_b1(a, reps=50):
+    """The one-sinogram operator pair at 400^2 / 256 x 2030 (DAS): the forward-pair stage
+    for each one-sinogram forward choice (2, 3: record streaming; 5, 6, 7: tile form with 8,
+    4, 2 detectors per workgroup) and the gather for each tile-gather variant (44-51), HIP
+    events on the launch stream, each result compared with the default choice's; then the
+    model operators (prefilter, IR, 1/sigma included).  Algorithmic bytes per application:
+    4 P D + 4 P + 4 D nT (166.5 MB)."""
+    import torch
+    from mbrec.discretization import Discretization
+    from mbrec.model import MSOTmodel
+    from mbrec.probe import Probe
+    N, h = a.grid, 1e-4
+    half = (N - 1) / 2 * h
+    m = MSOTmodel(Probe("arc256_r40_174", True, False), Discretization([-half, half], N), 1500.0, False, True, 0,
+                  1e5, 12e6, 1.0)
+    D, nT = m.shape_sinogram
+    alg = 4.0 * (m.Discretization.numberOfPixels * D + m.Discretization.numberOfPixels + D * nT)
+    g = torch.Generator(device="cuda").manual_seed(11)
+    x = torch.rand((1, N, N), generator=g, dtype=torch.float32, device="cuda")
+    sg = torch.randn((1, D, nT), generator=g, dtype=torch.float32, device="cuda")
+    rel = lambda u, w: float(torch.linalg.vector_norm((u - w).double()) / torch.linalg.vector_norm(w.double()))
+    out = {"forward": {}, "gather": {}}
+    m.plan.set_launch(1, -1, 0)
+    y_ref, x_ref = m.applyForward(x), m.applyTranspose(sg)
+    for ch in (2, 3, 5, 6, 7):
+        m.plan.set_launch(1, -1, ch)
+        ms = m.plan.time_stage("scatter", 1, reps)
+        out["forward"][ch] = {"us": round(ms * 1e3, 2), "alg_GB/s": round(alg / (ms * 1e-3) / 1e9, 1),
+                              "rel_diff_vs_default": rel(m.applyForward(x), y_ref)}
+        log(f"forward choice {ch}: {ms * 1e3:.1f} us")
+    for gv in range(44, 55):
+        m.plan.set_launch(1, gv, 0)
+        ms = m.plan.time_stage("gather", 1, reps)
+        out["gather"][gv] = {"us": round(ms * 1e3, 2), "alg_GB/s": round(alg / (ms * 1e-3) / 1e9, 1),
+                             "rel_diff_vs_default": rel(m.applyTranspose(sg), x_ref)}
+        log(f"gather variant {gv}: {ms * 1e3:.1f} us")
+    m.plan.set_launch(1, -1, 0)
+    for name, fn, arg in (("model_forward_us", m.applyForward, x), ("model_transpose_us", m.applyTranspose, sg)):
+        ms, _ = events_ms(lambda: fn(arg), reps, 5)
+        out[name] = round(ms * 1e3, 2)
+    out.update({"workload": f"{N}x{N} px, {D} ch x {nT} samples, DAS, one sinogram",
+                "algorithmic_bytes": alg, "hbm_peak_GB/s": HBM_PEAK_GBS})
+    if a.traffic_calibration:
+        xc = torch.empty(CALIB_BYTES // 4, dtype=torch.float32, device="cuda").fill_(1.0)
+        torch.cuda.synchronize()
+        yc = xc.clone()
+        torch.cuda.synchronize()
+        del xc, yc
+    print(json.dumps(out), flush=True)
+
+
 def metric_bench(a):
     """The headline metric: recon/s of batch-64 NNLS-L2 at 400^2 x 256 ch reaching 1e-3
     relative L2 of the converged solution, learned warm start vs plain cold start."""
@@ -529,7 +592,8 @@ def metric_bench(a):
     import torch
 
     from mbrec.solvers import rec_nn_with_L2_reg_batch
-    from mbrec.warmstart import UNet, iterations_to_tolerance, nnls_l2_minimiser, warm_start_estimate
+    from mbrec.warmstart import (UNet, iterations_to_tolerance, nnls_l2_minimiser, rec_nn_with_L2_reg_learned,
+                                 warm_start_estimate)
 
     weights = a.weights or os.path.join(ROOT, "mb-rec-msot_amd", "mbrec", "weights", f"unet_state_{a.grid}.npz")
     if not os.path.exists(weights):
@@ -553,9 +617,14 @@ def metric_bench(a):
     if k_plain is None or k_learned is None:
         raise SystemExit("a solve did not reach the tolerance")
 
+    judged = {}
+
     def learned_step():
-        w, v, _ = warm_start_estimate(net, model, y, lam)
-        return rec_nn_with_L2_reg_batch(model, y, k_learned, lam, init=w, init_dual=v)
+        # network, fallback test, cold start as judge over the first 10 iterations (its
+        # own 10 cold iterations are extra work inside the timed step), then the rest
+        rec, used_j = rec_nn_with_L2_reg_learned(net, model, y, k_learned, lam, judge_iters=10)
+        judged["used"] = used_j
+        return rec
 
     def plain_step():
         return rec_nn_with_L2_reg_batch(model, y, k_plain, lam)
@@ -577,10 +646,13 @@ def metric_bench(a):
            "config": {"workload": f"batch {B} independent rec_nn_with_L2_reg solves (RegL2 = identity, "
                                   f"lambdaL2 = 1e-2 ||A'y||), {a.grid}x{a.grid} px at 100 um, no SIR, {D} ch x {nT} "
                                   f"samples, each to rel.err <= {tol:g} of the converged solution; learned "
-                                  f"warm start (U-Net state estimate) vs plain cold start",
+                                  f"warm start (U-Net state estimate, cold start as fallback and judge) vs "
+                                  f"plain cold start",
                       "batch": B, "grid": a.grid, "detectors": D, "samples": nT, "sir": False, "tol": tol},
            "learned": {"recon_per_s": round(value, 4), "outer_iterations": k_learned, "worst_rel_err": e_learned,
-                       "items_from_network": int(used.sum()), "ms_per_batch": round(ms_learned, 2),
+                       "items_from_network": int(used.sum()),
+                       "items_kept_by_judge": int(judged["used"].sum()), "judge_iterations": 10,
+                       "ms_per_batch": round(ms_learned, 2),
                        "wall_ms_per_batch": round(wall / max(a.steps + a.warmup, 1), 2)},
            "plain": {"recon_per_s": round(plain_value, 4), "outer_iterations": k_plain, "worst_rel_err": e_plain,
                      "ms_per_batch": round(ms_plain, 2)},
@@ -708,11 +780,12 @@ def main():
         traffic_from(*a.traffic_from, N=a.grid, B=a.batch)
         return
     if a.gpus != 1:
-        print(f"bench.py runs on one GPU only; --gpus {a.gpus} is not supported in this build "
-              "(batches of independent solves shard across GPUs with no exchange; see DESIGN.md)",
-              flush=True)
+        print(f"bench.py runs on one GPU only; --gpus {a.gpus} is not supported "
+              "(batches of independent solves would shard across GPUs with no exchange)", flush=True)
         sys.exit(2)
-    if a.mex:
+    if a.pair_b1:
+        pair_b1(a)
+    elif a.mex:
         mex_bench(a)
     elif a.mixed:
         mixed(a)

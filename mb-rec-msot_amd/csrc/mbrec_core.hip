@@ -691,6 +691,20 @@ static int build_tile_windows(Plan *p, const std::vector<int32_t> &tq)
     if ((rc = dalloc(&p->d_twin, w.size(), &p->table_bytes)) || (rc = upload(p->d_twin, w.data(), w.size())) ||
         (rc = dalloc(&p->d_ttab, tt.size(), &p->table_bytes)) || (rc = upload(p->d_ttab, tt.data(), tt.size())))
         return rc;
+    // the same pairs packed into 32 bits for the one-sinogram gather: row (< 512) | u23 << 9,
+    // u rounded to the nearest 2^-23 (within half an FP32 ulp at u ~ 1)
+    if (p->tile_zr <= 256) {
+        std::vector<uint32_t> t1(tt.size());
+        for (size_t i = 0; i < tt.size(); ++i) {
+            float u;
+            std::memcpy(&u, &tt[i].y, sizeof(u));
+            long long u23 = std::llrint((double)u * (double)(1 << 23));
+            u23 = std::min<long long>(std::max<long long>(u23, 0), (1 << 23) - 1);
+            t1[i] = (uint32_t)tt[i].x | ((uint32_t)u23 << 9);
+        }
+        if ((rc = dalloc(&p->d_tq1, t1.size(), &p->table_bytes)) || (rc = upload(p->d_tq1, t1.data(), t1.size())))
+            return rc;
+    }
     return MBREC_OK;
 }
 
@@ -698,7 +712,7 @@ static void free_plan(Plan *p)
 {
     void *ptrs[] = {p->d_pix, p->d_tq, p->d_sidx, p->d_sir, p->d_swin, p->d_ir, p->d_irf, p->d_ir64, p->d_irf64,
                     p->d_kz,  p->d_kx, p->d_rec, p->d_rec_off, p->d_rec_k, p->d_rec1, p->d_twin, p->d_ptab, p->d_ttab, p->d_srec, p->d_srec_off, p->d_sirw, p->d_stwin, p->d_stab,
-                    p->d_tfrec, p->d_tfroff, p->d_tfdesc, p->d_tfpart};
+                    p->d_tfrec, p->d_tfroff, p->d_tfdesc, p->d_tfpart, p->d_tq1};
     for (void *q : ptrs)
         if (q) (void)hipFree(q);
     free_work(p);

@@ -72,6 +72,8 @@ struct Plan {
     int tile_wmax = 0;           // longest tile window
     int2 *d_ttab = nullptr;      // [Pd / 256][D][256] (bin row inside the tile window, or tile_zr; u bits)
     int tile_zr = 0;             // first of the two zero rows after a staged tile window (64 * ceil(wmax / 64))
+    uint32_t *d_tq1 = nullptr;   // [Pd / 256][D][256] packed (row inside the tile window, or tile_zr: 9 bits |
+                                 // u in units of 2^-23: 23 bits); tile-major stream of the one-sinogram gather
     BinGeom geom{};
     HatTaps taps{};
     // forward scatter: per detector, (pixel, travel time) records grouped into buckets

@@ -2638,6 +2638,63 @@ __global__ __launch_bounds__(256) void k_gather_ta1(const int32_t *__restrict__ 
         out[px] = scale * (acc - cmp);
 }
 
+// As k_gather_ta1, reading the packed tile-major pair table (row inside the tile's window |
+// u in 23 bits): each workgroup's stream is one contiguous run per detector group, and the
+// pair needs no decode; rows outside the bin range point at the zero rows tile_zr, +1.
+template <int G>
+__global__ __launch_bounds__(256) void k_gather_tp1(const uint32_t *__restrict__ tq1, const int2 *__restrict__ twin,
+                                                    const int32_t *__restrict__ pix, const float *__restrict__ bins,
+                                                    float *__restrict__ out, int D, int NB, int P, float scale, int ZR)
+{
+    __shared__ float win[2][258];
+    __shared__ int los[256];
+    const int tile = blockIdx.x, grp = blockIdx.y, k = threadIdx.x;
+    const int DG = (D + G - 1) / G, dbeg = grp * DG, dend = min(D, dbeg + DG);
+    const size_t nrows = (size_t)D * NB + 2;   // bins rows incl. the two zero rows
+    const size_t slot = (size_t)tile * 256 + k;
+    if (k < dend - dbeg) los[k] = twin[(size_t)tile * D + dbeg + k].x;   // DG <= 256
+    if (k < 4) win[k >> 1][256 + (k & 1)] = 0.0f;
+    __syncthreads();
+    auto row_of = [&](int d) -> float {
+        const size_t r = (size_t)d * NB + los[d - dbeg] + k;
+        return (k < ZR && r < nrows) ? bins[r] : 0.0f;
+    };
+    const uint32_t *tq = tq1 + (size_t)tile * D * 256 + k;
+    const float su = 1.0f / 8388608.0f;
+    float acc = 0.0f, cmp = 0.0f;
+    if (dbeg < dend) {
+        uint32_t q0 = tq[(size_t)dbeg * 256], q1 = 0;
+        float w0 = row_of(dbeg), w1 = 0.0f;
+        if (dbeg + 1 < dend) {
+            q1 = tq[(size_t)(dbeg + 1) * 256];
+            w1 = row_of(dbeg + 1);
+        }
+        for (int d = dbeg; d < dend; ++d) {
+            float *wb = win[(d - dbeg) & 1];
+            wb[k] = w0;
+            // the buffer written here was last read two detectors ago: every thread has
+            // passed the previous detector's barrier since
+            __syncthreads();
+            const uint32_t qd = q0;
+            q0 = q1;
+            w0 = w1;
+            if (d + 2 < dend) {
+                q1 = tq[(size_t)(d + 2) * 256];
+                w1 = row_of(d + 2);
+            }
+            const int r = (int)(qd & 511u);
+            const float u = (float)(qd >> 9) * su, g0 = wb[r], g1 = wb[r + 1];
+            kahan_add(acc, cmp, fmaf(u, g1 - g0, g0));
+        }
+    }
+    const int px = pix[slot];
+    if (px < 0) return;
+    if (G > 1)
+        out[(size_t)grp * P + px] = acc - cmp;
+    else
+        out[px] = scale * (acc - cmp);
+}
+
 // SIR transpose for one sinogram, LDS tile form: WG = one 256-slot tile (thread k: slot k)
 // x one of G detector groups.  Per detector the tile's window of bin rows (the SIR tile
 // table's, at most 256 rows) is copied to LDS (one row per thread, double buffered); the
@@ -2974,27 +3031,58 @@ __global__ __launch_bounds__(256) void k_fwd_tile1(const float *__restrict__ img
 }
 
 // pass 2: bins[d][j] = sum over blocks T (in order) of their partial covering j; a
-// workgroup takes 1024 bins of one detector, 4 per thread (independent loads in flight)
+// workgroup takes 1024 bins of one detector, 4 per thread.  The blocks whose windows meet
+// those bins are listed first (in block order), then read four at a time so that 16 loads
+// per thread are in flight before the sums (which keep the block order).
 __global__ __launch_bounds__(256) void k_fwd_tile1_sum(const float *__restrict__ part, const int4 *__restrict__ desc,
                                                        float *__restrict__ bins, int NB, int nblk, int D)
 {
     __shared__ int4 w[256];
+    __shared__ int cov[256];
+    __shared__ int ncov, wcnt[4];
     const int d = blockIdx.y;
     const int j0 = blockIdx.x * 1024;
     float acc[4] = {0.0f, 0.0f, 0.0f, 0.0f};
     for (int t0 = 0; t0 < nblk; t0 += 256) {
         __syncthreads();
-        if (t0 + (int)threadIdx.x < nblk) w[threadIdx.x] = desc[(size_t)(t0 + threadIdx.x) * D + d];
-        __syncthreads();
         const int nt = min(256, nblk - t0);
-        for (int t = 0; t < nt; ++t) {
-            const int4 e = w[t];
-            if (e.w == 0 || e.z > j0 + 1023 || e.z + e.w < j0) continue;   // window misses these bins
-#pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                const int r = j0 + (int)threadIdx.x + 256 * q - e.z;
-                if (r >= 0 && r <= e.w) acc[q] += part[e.y + r];
+        if ((int)threadIdx.x < nt) w[threadIdx.x] = desc[(size_t)(t0 + threadIdx.x) * D + d];
+        __syncthreads();
+        {   // compact the blocks whose window meets these bins, keeping their order
+            const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+            bool c = false;
+            if (t < nt) {
+                const int4 e = w[t];
+                c = e.w > 0 && e.z <= j0 + 1023 && e.z + e.w >= j0;
             }
+            const unsigned long long m = __ballot(c);
+            if (lane == 0) wcnt[wv] = __popcll(m);
+            __syncthreads();
+            int off = __popcll(m & ((1ull << lane) - 1ull));
+            for (int i = 0; i < wv; ++i) off += wcnt[i];
+            if (c) cov[off] = t;
+            if (t == 0) ncov = wcnt[0] + wcnt[1] + wcnt[2] + wcnt[3];
+        }
+        __syncthreads();
+        const int n = ncov;
+        for (int i0 = 0; i0 < n; i0 += 4) {
+            float val[4][4];
+#pragma unroll
+            for (int ii = 0; ii < 4; ++ii) {
+                const int i = i0 + ii;
+                const int4 e = i < n ? w[cov[i]] : make_int4(0, 0, 0, -1);
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const int r = j0 + (int)threadIdx.x + 256 * q - e.z;
+                    val[ii][q] = (r >= 0 && r <= e.w) ? part[e.y + r] : 0.0f;
+                }
+            }
+#pragma unroll
+            for (int ii = 0; ii < 4; ++ii)
+                if (i0 + ii < n) {
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) acc[q] += val[ii][q];
+                }
         }
     }
 #pragma unroll
@@ -3461,6 +3549,18 @@ static void gather_tas(Plan *p, float *img, int B, int Bp, float scale, hipStrea
     }
 }
 
+template <int G>
+static void gather_tp1(Plan *p, float *img, float scale, hipStream_t s)
+{
+    hipLaunchKernelGGL((k_gather_tp1<G>), dim3(p->Pd / 256, G), dim3(256), 0, s, p->d_tq1, p->d_twin, p->d_pix,
+                       p->d_bins, G > 1 ? p->d_gpart : img, p->D, p->geom.NB, p->P, scale, p->tile_zr);
+    if (G > 1) {
+        const size_t n = (size_t)p->P;
+        hipLaunchKernelGGL(k_sum_parts<G>, dim3(std::min<size_t>(4096, (n + 255) / 256)), dim3(256), 0, s,
+                           (const float *)p->d_gpart, img, n, scale);
+    }
+}
+
 template <int G, int PF = 2, int DPB = 1>
 static void gather_ta1(Plan *p, float *img, float scale, hipStream_t s)
 {
@@ -3530,12 +3630,15 @@ static const GatherVariant kGatherVariants[] = {{1, 8, 0},   {1, 4, 0},   {4, 4,
                                                 {0, -11, 4}, {0, -12, 1}, {0, -12, 2}, {0, -12, 4},
                                                 {0, -12, 8}, {0, -13, 1}, {0, -13, 2}, {0, -13, 4},
                                                 {0, -13, 8}, {0, -13, 16}, {0, -14, 4}, {0, -14, 8},
-                                                {0, -14, 16}};
+                                                {0, -14, 16}, {0, -15, 4}, {0, -15, 8}, {0, -15, 16}};
 constexpr int kNumGatherVariants = sizeof(kGatherVariants) / sizeof(kGatherVariants[0]);
 
 static bool gather_variant_ok(const Plan *p, int v, int Bp)
 {
     const GatherVariant &g = kGatherVariants[v];
+    if (g.u == -15)   // packed tile-major pair table (built when the tile windows fit 256 rows)
+        return !p->use_sir && p->d_tq1 && p->d_twin && p->tile_zr >= 64 && p->tile_zr <= 256 && Bp == 1 &&
+               p->d_gpart;
     if (g.u == -13 || g.u == -14)   // tile gather for one sinogram without SIR: tile windows of at most 256 rows
         return !p->use_sir && p->d_twin && p->tile_zr >= 64 && p->tile_zr <= 256 && Bp == 1 &&
                (g.lb == 1 || p->d_gpart);
@@ -3666,7 +3769,10 @@ static int launch_gather(Plan *p, float *img, int B, int Bp, float scale, hipStr
     case 48: gather_ta1<16>(p, img, scale, s); break;
     case 49: gather_ta1<4, 4, 2>(p, img, scale, s); break;
     case 50: gather_ta1<8, 4, 2>(p, img, scale, s); break;
-    default: gather_ta1<16, 4, 2>(p, img, scale, s); break;
+    case 51: gather_ta1<16, 4, 2>(p, img, scale, s); break;
+    case 52: gather_tp1<4>(p, img, scale, s); break;
+    case 53: gather_tp1<8>(p, img, scale, s); break;
+    default: gather_tp1<16>(p, img, scale, s); break;
     }
     MBREC_HIP(hipGetLastError());
     return MBREC_OK;

@@ -298,12 +298,42 @@ def warm_start_estimate(net, model, y, lambdaL2):
     return w0.contiguous(), v0.contiguous(), used.cpu().numpy()
 
 
-def rec_nn_with_L2_reg_learned(net, model, y, num_iter, lambdaL2):
+def judge_warm_start(model, y, lambdaL2, w0, v0, used, k=10):
+    """The plain cold start as the judge of a warm start: both run k outer iterations of
+    nnls_L2_less; per item the run whose last iteration moved the state s = w - v less is
+    kept (for a contraction the distance to the fixed point scales with the step).
+    Returns (w, v, used) after k iterations -- passing w, v back as init / init_dual
+    continues that item's iteration sequence -- with used False where the cold run won."""
+    import torch
+    from .solvers import rec_nn_with_L2_reg_batch
+    B = y.shape[0]
+
+    def run(wa, va):
+        w1, v1 = rec_nn_with_L2_reg_batch(model, y, k - 1, lambdaL2, init=wa, init_dual=va, return_dual=True)
+        w2, v2 = rec_nn_with_L2_reg_batch(model, y, 1, lambdaL2, init=w1, init_dual=v1, return_dual=True)
+        s1, s2 = (w1 - v1).double().reshape(B, -1), (w2 - v2).double().reshape(B, -1)
+        return w2, v2, torch.linalg.vector_norm(s2 - s1, dim=1)
+
+    ww, vw, sw = run(w0, v0)
+    wc, vc, sc = run(None, None)
+    keep = torch.as_tensor(used, device=ww.device) & torch.isfinite(sw) & (sw <= sc)
+    sel = keep.reshape(B, 1, 1)
+    return (torch.where(sel, ww, wc).contiguous(), torch.where(sel, vw, vc).contiguous(),
+            keep.cpu().numpy())
+
+
+def rec_nn_with_L2_reg_learned(net, model, y, num_iter, lambdaL2, judge_iters=10):
     """B independent rec_nn_with_L2_reg.m solves (RegL2 = identity) of num_iter outer
     iterations, each started from the network's state (warm_start_estimate) instead of
-    zero.  Returns (rec (B, Nx, Nz) device tensor, used)."""
+    zero, with the cold start as fallback and judge (judge_warm_start over the first
+    judge_iters iterations; 0 skips the judge).  Returns (rec (B, Nx, Nz) device tensor,
+    used)."""
     from .solvers import rec_nn_with_L2_reg_batch
     w0, v0, used = warm_start_estimate(net, model, y, lambdaL2)
+    k = min(int(judge_iters), int(num_iter))
+    if k >= 2:
+        w0, v0, used = judge_warm_start(model, y, lambdaL2, w0, v0, used, k)
+        num_iter -= k
     return rec_nn_with_L2_reg_batch(model, y, num_iter, lambdaL2, init=w0, init_dual=v0), used
 
 

@@ -435,3 +435,12 @@ def test_warm_start_state_training_and_fallback():
     assert not used[0] and not used[1] and used[2:].all()
     assert float(w0[:2].abs().max()) == 0.0 and float(v0[:2].abs().max()) == 0.0
     assert torch.equal(w0[2:], torch.clamp(bad[2:], min=0.0))
+    # the cold start as judge: exact states are kept, a finite state with a huge dual (it
+    # passes the objective test: its image is 0) loses to the cold start after 10 iterations
+    from mbrec.warmstart import judge_warm_start
+    v_bad = v0.clone()
+    v_bad[3] = 1e4 * float(v0[2:].abs().max())
+    wj, vj, keep = judge_warm_start(m, y, lam, w0, v_bad, used, k=10)
+    assert not keep[0] and not keep[1] and not keep[3] and keep[2] and keep[4:].all()
+    wc, vc = rec_nn_with_L2_reg_batch(m, y, 10, lam, return_dual=True)
+    assert torch.equal(wj[3], wc[3]) and torch.equal(vj[3], vc[3])
