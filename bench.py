@@ -537,7 +537,7 @@ def events_ms(fn, steps, warmup):
 def pair_b1(a, reps=50):
     """The one-sinogram operator pair at 400^2 / 256 x 2030 (DAS): the forward-pair stage
     for each one-sinogram forward choice (2, 3: record streaming; 5, 6, 7: tile form with 8,
-    4, 2 detectors per workgroup) and the gather for each tile-gather variant (44-51), HIP
+    4, 2 detectors per workgroup) and the gather for each tile-gather variant (44-54), HIP
     events on the launch stream, each result compared with the default choice's; then the
     model operators (prefilter, IR, 1/sigma included).  Algorithmic bytes per application:
     4 P D + 4 P + 4 D nT (166.5 MB)."""

@@ -11,6 +11,7 @@ stationary state s* (stationary_state) are computed; the loop is then started fr
   - the exact image x* with v = 0 (what seeding only the image gives),
   - s* (1 + eps n) and, for comparison, x* (1 + eps n) with its KKT dual,
 with n a smooth field (Gaussian noise averaged over 9 x 9 pixels, unit standard deviation),
+and s* (1 + eps w) with w white Gaussian noise (which error scales the loop removes slowly),
 and the worst relative L2 error to x* is reported after k outer iterations.
 """
 from __future__ import annotations
@@ -72,6 +73,10 @@ def main(argv=None):
         s0 = s * (1 + eps * n)
         res[f"state_eps_{eps}"] = curve(f"s* (1 + {eps} n)", f32(torch.clamp(s0, min=0.0)),
                                         f32(torch.clamp(-s0, min=0.0)), grid)
+        wn = torch.randn(B, Nx, Nz, generator=gen, device="cuda", dtype=torch.float64)
+        s0 = s * (1 + eps * wn)
+        res[f"state_white_eps_{eps}"] = curve(f"s* (1 + {eps} white)", f32(torch.clamp(s0, min=0.0)),
+                                              f32(torch.clamp(-s0, min=0.0)), grid)
         x0 = torch.clamp(xs * (1 + eps * n), min=0.0)
         g0 = objective_gradient(model, y, lam, x0)
         v0 = torch.where(x0 <= 0, torch.clamp(g0, min=0.0) / BETA, torch.zeros_like(g0))
