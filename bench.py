@@ -617,13 +617,13 @@ def metric_bench(a):
     if k_plain is None or k_learned is None:
         raise SystemExit("a solve did not reach the tolerance")
 
-    judged = {}
+    arbitrated = {}
 
     def learned_step():
-        # network, fallback test, cold start as judge over the first 10 iterations (its
+        # network, fallback test, cold start as arbiter over the first 10 iterations (its
         # own 10 cold iterations are extra work inside the timed step), then the rest
-        rec, used_j = rec_nn_with_L2_reg_learned(net, model, y, k_learned, lam, judge_iters=10)
-        judged["used"] = used_j
+        rec, used_j = rec_nn_with_L2_reg_learned(net, model, y, k_learned, lam, arbiter_iters=10)
+        arbitrated["used"] = used_j
         return rec
 
     def plain_step():
@@ -646,12 +646,12 @@ def metric_bench(a):
            "config": {"workload": f"batch {B} independent rec_nn_with_L2_reg solves (RegL2 = identity, "
                                   f"lambdaL2 = 1e-2 ||A'y||), {a.grid}x{a.grid} px at 100 um, no SIR, {D} ch x {nT} "
                                   f"samples, each to rel.err <= {tol:g} of the converged solution; learned "
-                                  f"warm start (U-Net state estimate, cold start as fallback and judge) vs "
+                                  f"warm start (U-Net state estimate, cold start as fallback and arbiter) vs "
                                   f"plain cold start",
                       "batch": B, "grid": a.grid, "detectors": D, "samples": nT, "sir": False, "tol": tol},
            "learned": {"recon_per_s": round(value, 4), "outer_iterations": k_learned, "worst_rel_err": e_learned,
                        "items_from_network": int(used.sum()),
-                       "items_kept_by_judge": int(judged["used"].sum()), "judge_iterations": 10,
+                       "items_kept_by_arbiter": int(arbitrated["used"].sum()), "arbiter_iterations": 10,
                        "ms_per_batch": round(ms_learned, 2),
                        "wall_ms_per_batch": round(wall / max(a.steps + a.warmup, 1), 2)},
            "plain": {"recon_per_s": round(plain_value, 4), "outer_iterations": k_plain, "worst_rel_err": e_plain,

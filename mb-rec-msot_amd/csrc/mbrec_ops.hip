@@ -3694,8 +3694,9 @@ static bool gather_variant_ok(const Plan *p, int v, int Bp)
 
 static int default_gather_variant(const Plan *p, int Bp)
 {
-    if (Bp == 1)   // the tile forms in 8 detector groups (SIR, then without), else record-lane
-        for (int v : {43, 47, 0})
+    if (Bp == 1)   // the tile forms (SIR in 8 detector groups; without SIR the packed-table form
+                   // in 16, then the decoding form in 8), else record-lane
+        for (int v : {43, 54, 47, 0})
             if (gather_variant_ok(p, v, Bp)) return v;
     if (p->use_sir && Bp >= 16)
         for (int v : {39, 38, 37})   // SIR tile gather in four detector groups, two, one

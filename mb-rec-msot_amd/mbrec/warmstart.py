@@ -7,7 +7,7 @@ which seeds the outer iterate of ``rec_nn_with_L2_reg`` (hook at
 rec_nn_with_L2_reg.m:41, where A^T y is already available).  The solver always
 finishes from the network's output, so the result is an NNLS solution
 whatever the network predicts; the plain cold-start solve stays both the
-fallback and the judge of the warm start's error.
+fallback and the arbiter of the warm start's error.
 
 Training data are produced by the same plain solver from seeded phantoms
 (``make_training_set``); the network is trained on the device by its own
@@ -298,8 +298,8 @@ def warm_start_estimate(net, model, y, lambdaL2):
     return w0.contiguous(), v0.contiguous(), used.cpu().numpy()
 
 
-def judge_warm_start(model, y, lambdaL2, w0, v0, used, k=10):
-    """The plain cold start as the judge of a warm start: both run k outer iterations of
+def arbitrate_warm_start(model, y, lambdaL2, w0, v0, used, k=10):
+    """The plain cold start as the arbiter of a warm start: both run k outer iterations of
     nnls_L2_less; per item the run whose last iteration moved the state s = w - v less is
     kept (for a contraction the distance to the fixed point scales with the step).
     Returns (w, v, used) after k iterations -- passing w, v back as init / init_dual
@@ -322,17 +322,17 @@ def judge_warm_start(model, y, lambdaL2, w0, v0, used, k=10):
             keep.cpu().numpy())
 
 
-def rec_nn_with_L2_reg_learned(net, model, y, num_iter, lambdaL2, judge_iters=10):
+def rec_nn_with_L2_reg_learned(net, model, y, num_iter, lambdaL2, arbiter_iters=10):
     """B independent rec_nn_with_L2_reg.m solves (RegL2 = identity) of num_iter outer
     iterations, each started from the network's state (warm_start_estimate) instead of
-    zero, with the cold start as fallback and judge (judge_warm_start over the first
-    judge_iters iterations; 0 skips the judge).  Returns (rec (B, Nx, Nz) device tensor,
+    zero, with the cold start as fallback and arbiter (arbitrate_warm_start over the first
+    arbiter_iters iterations; 0 skips the arbitration).  Returns (rec (B, Nx, Nz) device tensor,
     used)."""
     from .solvers import rec_nn_with_L2_reg_batch
     w0, v0, used = warm_start_estimate(net, model, y, lambdaL2)
-    k = min(int(judge_iters), int(num_iter))
+    k = min(int(arbiter_iters), int(num_iter))
     if k >= 2:
-        w0, v0, used = judge_warm_start(model, y, lambdaL2, w0, v0, used, k)
+        w0, v0, used = arbitrate_warm_start(model, y, lambdaL2, w0, v0, used, k)
         num_iter -= k
     return rec_nn_with_L2_reg_batch(model, y, num_iter, lambdaL2, init=w0, init_dual=v0), used
 

@@ -193,6 +193,46 @@ def test_every_launch_variant_matches_oracle(sir, B):
     plan.set_launch(B, -1, 0)
 
 
+def test_one_sinogram_tile_variants_match_oracle():
+    """Every one-sinogram tile form on a real arc geometry (the tile windows fit the LDS only
+    for smooth travel times): forward choices 2, 3 (record streaming) and 5, 6, 7 (tile form),
+    gather variants 44-54 (LDS tile gathers, packed tile-major table), each against the
+    oracle's DAS loops."""
+    import torch
+    _log("one-sinogram tile variants 96x80")
+    m = _geometry_model(96, 80, 40, 700, 1e-4, sir=False, sigma=1.0)
+    orc = _oracle_of(m)
+    plan = m.plan
+    D, nT = m.shape_sinogram
+    x = rng(31).uniform(0, 1, (1, 80, 96))
+    sg = rng(32).standard_normal((1, D, nT))
+    tt = orc.tt.T   # [P x D]
+    yo = np.transpose(O.forward_das(x.reshape(1, -1), tt, nT, orc.R), (0, 2, 1))
+    xto = O.transpose_das(np.transpose(sg, (0, 2, 1)), tt, orc.R)
+    xd = torch.tensor(x, dtype=torch.float32).cuda()
+    sd = torch.tensor(sg, dtype=torch.float32).cuda()
+    done = []
+    ref = None
+    for gv in range(44, 55):
+        try:
+            plan.set_launch(1, gv, 6)
+        except Exception:
+            continue
+        xt = plan.transpose(sd, model=False).cpu().numpy()
+        assert rel_l2(xt.reshape(1, -1), xto) < TOL_OP, gv
+        if gv == 47:
+            ref = xt
+        if gv == 53:   # the packed table in 8 detector groups, as variant 47 (u rounded to 2^-23)
+            assert ref is not None and rel_l2(xt, ref) < 1e-6
+        done.append(gv)
+    assert {47, 53}.issubset(done), done
+    for ch in (2, 3, 5, 6, 7):
+        plan.set_launch(1, 47, ch)
+        y = plan.forward(xd, model=False).cpu().numpy()
+        assert rel_l2(y, yo) < TOL_OP, ch
+    plan.set_launch(1, -1, 0)
+
+
 def _geometry_model(Nz, Nx, D, nT, h, sir=False, sigma=None):
     """Model with the benchmark geometry scaled down, built by the product setup code."""
     from mbrec.discretization import Discretization
@@ -435,12 +475,12 @@ def test_warm_start_state_training_and_fallback():
     assert not used[0] and not used[1] and used[2:].all()
     assert float(w0[:2].abs().max()) == 0.0 and float(v0[:2].abs().max()) == 0.0
     assert torch.equal(w0[2:], torch.clamp(bad[2:], min=0.0))
-    # the cold start as judge: exact states are kept, a finite state with a huge dual (it
+    # the cold start as arbiter: exact states are kept, a finite state with a huge dual (it
     # passes the objective test: its image is 0) loses to the cold start after 10 iterations
-    from mbrec.warmstart import judge_warm_start
+    from mbrec.warmstart import arbitrate_warm_start
     v_bad = v0.clone()
     v_bad[3] = 1e4 * float(v0[2:].abs().max())
-    wj, vj, keep = judge_warm_start(m, y, lam, w0, v_bad, used, k=10)
+    wj, vj, keep = arbitrate_warm_start(m, y, lam, w0, v_bad, used, k=10)
     assert not keep[0] and not keep[1] and not keep[3] and keep[2] and keep[4:].all()
     wc, vc = rec_nn_with_L2_reg_batch(m, y, 10, lam, return_dual=True)
     assert torch.equal(wj[3], wc[3]) and torch.equal(vj[3], vc[3])
