@@ -484,3 +484,10 @@ def test_warm_start_state_training_and_fallback():
     assert not keep[0] and not keep[1] and not keep[3] and keep[2] and keep[4:].all()
     wc, vc = rec_nn_with_L2_reg_batch(m, y, 10, lam, return_dual=True)
     assert torch.equal(wj[3], wc[3]) and torch.equal(vj[3], vc[3])
+    # the learned solve chains estimate, fallback, arbiter and the remaining iterations
+    from mbrec.warmstart import rec_nn_with_L2_reg_learned
+    rec_l, used_l = rec_nn_with_L2_reg_learned(Fixed(bad), m, y, 15, lam, arbiter_iters=10)
+    w0, v0, used = warm_start_estimate(Fixed(bad), m, y, lam)
+    wa, va, ka = arbitrate_warm_start(m, y, lam, w0, v0, used, k=10)
+    assert torch.equal(rec_l, rec_nn_with_L2_reg_batch(m, y, 5, lam, init=wa, init_dual=va))
+    assert np.array_equal(used_l, ka)
