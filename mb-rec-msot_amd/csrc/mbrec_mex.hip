@@ -19,6 +19,8 @@
 
 #include <cstring>
 #include <list>
+#include <algorithm>
+#include <cmath>
 #include <mutex>
 #include <vector>
 
@@ -102,6 +104,38 @@ static std::mutex g_cache_mu;
 static std::list<CacheEntry> g_cache;   // most recently used first
 static const size_t kCacheMax = 4;
 
+// The MEX calls get P pixels without the grid shape.  The plan's device pixel order (16 x 16
+// tiles) only pays off with the true column length Nz, so it is read off the travel times
+// of the first detector: along a column (x = iz + Nz ix) they change smoothly, and they jump
+// where one column ends and the next begins.  Nz = the common spacing of the jumps, accepted
+// only if it divides P, every jump sits at a multiple of it, and the step between columns
+// is smooth too; otherwise one column of P pixels (any factorisation gives the same result;
+// the shape only decides which kernels are fast).
+static int infer_column_length(const double *tt, int P)
+{
+    if (P < 64) return P;
+    std::vector<double> dv(P - 1);
+    for (int x = 0; x + 1 < P; ++x) dv[x] = std::fabs(tt[x + 1] - tt[x]);
+    std::vector<double> srt(dv);
+    std::nth_element(srt.begin(), srt.begin() + srt.size() / 2, srt.end());
+    const double med = srt[srt.size() / 2];
+    if (!(med > 0.0) || !std::isfinite(med)) return P;
+    int first = -1;
+    for (int x = 0; x + 1 < P; ++x)
+        if (dv[x] > 8.0 * med) { first = x; break; }
+    if (first < 0) return P;
+    const int Nz = first + 1;
+    if (Nz < 8 || P % Nz != 0 || P / Nz < 2) return P;
+    double across = 0.0;   // mean step between neighbouring columns vs along a column
+    for (int x = 0; x + 1 < P; ++x) {
+        const bool boundary = (x + 1) % Nz == 0;
+        if (!boundary && dv[x] > 8.0 * med) return P;   // a jump inside a column
+        if (x + Nz < P) across += std::fabs(tt[x + Nz] - tt[x]);
+    }
+    across /= (double)(P - Nz);
+    return across < 8.0 * med ? Nz : P;
+}
+
 // the cache entry of these tables, with staging for W wavelengths; the returned entry
 // stays valid while g_cache_mu is held by the caller
 static int get_entry(int P, int D, int nT, double R, const double *tt, const int32_t *sidx, const double *sir,
@@ -122,7 +156,8 @@ static int get_entry(int P, int D, int nT, double R, const double *tt, const int
         }
     if (!e) {
         mbrec_plan plan = nullptr;
-        int rc = mbrec_plan_create(&plan, P, 1, D, nT, R, tt);
+        const int Nz = infer_column_length(tt, P);
+        int rc = mbrec_plan_create(&plan, Nz, P / Nz, D, nT, R, tt);
         if (rc) return rc;
         if (sidx && (rc = mbrec_plan_set_sir(plan, sidx, sir, Ls, Nsir, se))) {
             mbrec_plan_destroy(plan);

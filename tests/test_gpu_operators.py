@@ -48,6 +48,25 @@ def test_mex_das_matches_oracle(W, P, D, nT, R, spill):
     assert rel_l2(xt, O.transpose_das(s, tt, R)) < TOL_OP
 
 
+@pytest.mark.parametrize("W", [1, 3])
+def test_mex_das_on_an_arc_geometry(W):
+    """The DAS drop-ins with the travel times of a real grid (96 x 80 pixels, 40 detectors of
+    the arc): the MEX path reads the column length off the table and runs the 2-D tile
+    kernels; results against the oracle's loops."""
+    from mbrec.mex import forwardDelayAndSumMex, transposeDelayAndSumMex
+    _log(f"mex_das on an arc geometry W={W}")
+    m = _geometry_model(96, 80, 40, 700, 1e-4, sir=False, sigma=1.0)
+    orc = _oracle_of(m)
+    tt = np.asfortranarray(orc.tt.T)   # [P x D]
+    D, nT = m.shape_sinogram
+    x = rng(41).uniform(0, 1, (W, tt.shape[0]))
+    y = forwardDelayAndSumMex(x, tt, np.int32(nT), orc.R)
+    assert rel_l2(y, O.forward_das(x, tt, nT, orc.R)) < TOL_OP
+    sg = rng(42).standard_normal((W, nT, D))
+    xt = transposeDelayAndSumMex(sg, tt, orc.R)
+    assert rel_l2(xt, O.transpose_das(sg, tt, orc.R)) < TOL_OP
+
+
 @pytest.mark.parametrize("W,P,D,nT,R,Ls", [(1, 500, 8, 200, 2.6666666666666665, 9),
                                            (2, 333, 6, 150, 1.6, 15),
                                            (5, 70, 3, 90, 3.3333333333333335, 4)])
