@@ -537,7 +537,8 @@ def events_ms(fn, steps, warmup):
 def pair_b1(a, reps=50):
     """The one-sinogram operator pair at 400^2 / 256 x 2030 (DAS): the forward-pair stage
     for each one-sinogram forward choice (2, 3: record streaming; 5, 6, 7: tile form with 8,
-    4, 2 detectors per workgroup) and the gather for each tile-gather variant (44-54), HIP
+    4, 2 detectors per workgroup, thread per row; 9, 10: its lean form; 11, 12, 13: the lean
+    form over rows sorted by record count, 11 the default) and the gather for each tile-gather variant (44-54), HIP
     events on the launch stream, each result compared with the default choice's; then the
     model operators (prefilter, IR, 1/sigma included).  Algorithmic bytes per application:
     4 P D + 4 P + 4 D nT (166.5 MB)."""
@@ -558,7 +559,7 @@ def pair_b1(a, reps=50):
     out = {"forward": {}, "gather": {}}
     m.plan.set_launch(1, -1, 0)
     y_ref, x_ref = m.applyForward(x), m.applyTranspose(sg)
-    for ch in (2, 3, 5, 6, 7):
+    for ch in (2, 3, 5, 6, 7, 9, 10, 11, 12, 13):
         m.plan.set_launch(1, -1, ch)
         ms = m.plan.time_stage("scatter", 1, reps)
         out["forward"][ch] = {"us": round(ms * 1e3, 2), "alg_GB/s": round(alg / (ms * 1e-3) / 1e9, 1),

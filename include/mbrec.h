@@ -133,7 +133,11 @@ int mbrec_plan_autotune(mbrec_plan plan, int B, int reps, mbrec_stream s, int *c
  * without SIR (pull form): 1, 4, 8, 16 bins per thread pass of that many
  * sinograms; 2, 3 record streaming for one sinogram (1 or 4 bins per lane); 5, 6, 7 the
  * tile form for one sinogram with 8, 4, 2 detectors per workgroup (4 x 4-tile blocks,
- * 32-bit records staged in LDS, window partials summed per bin; 6 is the default there);
+ * 32-bit records staged in LDS, window partials summed per bin); 9, 10 its lean form
+ * (scalar descriptors, unconditional loads, sums of v and t v per row; 2, 4 detectors
+ * per workgroup); 11, 12, 13 the lean form over rows sorted by record count (the same
+ * sums; 2, 2, 4 detectors per workgroup, records read in chunks of 4, 2, 2; 11 is the
+ * default for one sinogram);
  * -(LB + 100 mode) batch-lane with lane groups LB and mode 0 (plain order), 1
  * (XCD-aware order), 2 (XCD-aware, 16 pairs in flight); -364 / -464 / -564 /
  * -664 / -764 / -864 wave-uniform, -964 / -1064 / -1164 wave-uniform with raw

@@ -201,13 +201,15 @@ static int build_pull_records(Plan *p, const std::vector<int32_t> &slot_pix, con
 // device tiles in tile order; plans without a 2-D grid (Nx == 1) get none.
 static int build_tile_forward(Plan *p, const std::vector<int32_t> &slot_pix, const std::vector<int32_t> &tq)
 {
-    void *old[] = {p->d_tfrec, p->d_tfroff, p->d_tfdesc, p->d_tfpart};
+    void *old[] = {p->d_tfrec, p->d_tfroff, p->d_tfdesc, p->d_tfpart, p->d_tfsrec, p->d_tfsro};
     for (void *q : old)
         if (q) (void)hipFree(q);
     p->d_tfrec = nullptr;
     p->d_tfroff = nullptr;
     p->d_tfdesc = nullptr;
     p->d_tfpart = nullptr;
+    p->d_tfsrec = nullptr;
+    p->d_tfsro = nullptr;
     p->table_bytes -= p->tf_bytes;
     p->tf_bytes = 0;
     p->tf_nbz = p->tf_nbx = 0;
@@ -217,9 +219,10 @@ static int build_tile_forward(Plan *p, const std::vector<int32_t> &slot_pix, con
     const int ntz = (p->Nz + 15) / 16, ntx = (p->Nx + 15) / 16;
     const int nbz = (ntz + 3) / 4, nbx = (ntx + 3) / 4, nblk = nbz * nbx;
     const float scale = std::ldexp(1.0f, -g.F);
-    std::vector<uint32_t> rec;
+    std::vector<uint32_t> rec, srec, sro;
     std::vector<uint16_t> roff;
     std::vector<int4> desc((size_t)nblk * p->D + 1);
+    std::vector<int> order;
     std::vector<int> js(4096);
     std::vector<uint32_t> rs(4096), srt(4096);
     std::vector<int> cnt;
@@ -265,16 +268,35 @@ static int build_tile_forward(Plan *p, const std::vector<int32_t> &slot_pix, con
                 for (int r = 0; r <= nrow; ++r) roff.push_back((uint16_t)cnt[r]);
                 for (int i = 0; i < n; ++i) srt[cnt[js[i] - jmin]++] = rs[i];
                 rec.insert(rec.end(), srt.begin(), srt.begin() + n);
+                // rows by decreasing record count (ties in row order); row r's records are
+                // srt[roff[ostart + r] .. roff[ostart + r + 1])
+                order.resize(nrow);
+                for (int r = 0; r < nrow; ++r) order[r] = r;
+                const uint16_t *ro = roff.data() + ostart;
+                std::stable_sort(order.begin(), order.end(),
+                                 [ro](int a, int b) { return ro[a + 1] - ro[a] > ro[b + 1] - ro[b]; });
+                uint32_t at = 0;
+                for (int k = 0; k < nrow; ++k) {
+                    const int r = order[k];
+                    sro.push_back(at | ((uint32_t)r << 16));
+                    for (int i = ro[r]; i < ro[r + 1]; ++i) srec.push_back(srt[i]);
+                    at += ro[r + 1] - ro[r];
+                }
+                sro.push_back((uint32_t)n);
             }
         }
     desc[(size_t)nblk * p->D] = make_int4((int)rec.size(), (int)roff.size(), 0, 0);
     rec.push_back(0u);
     roff.push_back(0);
+    srec.push_back(0u);
+    sro.push_back(0u);
     int rc;
     if ((rc = dalloc(&p->d_tfrec, rec.size(), &p->tf_bytes)) || (rc = dalloc(&p->d_tfroff, roff.size(), &p->tf_bytes)) ||
         (rc = dalloc(&p->d_tfdesc, desc.size(), &p->tf_bytes)) || (rc = dalloc(&p->d_tfpart, roff.size(), &p->tf_bytes)) ||
+        (rc = dalloc(&p->d_tfsrec, srec.size(), &p->tf_bytes)) || (rc = dalloc(&p->d_tfsro, sro.size(), &p->tf_bytes)) ||
         (rc = upload(p->d_tfrec, rec.data(), rec.size())) || (rc = upload(p->d_tfroff, roff.data(), roff.size())) ||
-        (rc = upload(p->d_tfdesc, desc.data(), desc.size())))
+        (rc = upload(p->d_tfdesc, desc.data(), desc.size())) || (rc = upload(p->d_tfsrec, srec.data(), srec.size())) ||
+        (rc = upload(p->d_tfsro, sro.data(), sro.size())))
         return rc;
     p->table_bytes += p->tf_bytes;
     p->tf_nbz = nbz;
@@ -497,13 +519,15 @@ int build_records(Plan *p, const std::vector<int32_t> &slot_pix, const std::vect
             if (sidx) rk[r] = sidx[at];
         }
     // replace the previous tables (the tile forward serves plans without SIR only)
-    void *oldtf[] = {p->d_tfrec, p->d_tfroff, p->d_tfdesc, p->d_tfpart};
+    void *oldtf[] = {p->d_tfrec, p->d_tfroff, p->d_tfdesc, p->d_tfpart, p->d_tfsrec, p->d_tfsro};
     for (void *q : oldtf)
         if (q) (void)hipFree(q);
     p->d_tfrec = nullptr;
     p->d_tfroff = nullptr;
     p->d_tfdesc = nullptr;
     p->d_tfpart = nullptr;
+    p->d_tfsrec = nullptr;
+    p->d_tfsro = nullptr;
     p->table_bytes -= p->tf_bytes;
     p->tf_bytes = 0;
     p->tf_nbz = p->tf_nbx = 0;
@@ -712,7 +736,7 @@ static void free_plan(Plan *p)
 {
     void *ptrs[] = {p->d_pix, p->d_tq, p->d_sidx, p->d_sir, p->d_swin, p->d_ir, p->d_irf, p->d_ir64, p->d_irf64,
                     p->d_kz,  p->d_kx, p->d_rec, p->d_rec_off, p->d_rec_k, p->d_rec1, p->d_twin, p->d_ptab, p->d_ttab, p->d_srec, p->d_srec_off, p->d_sirw, p->d_stwin, p->d_stab,
-                    p->d_tfrec, p->d_tfroff, p->d_tfdesc, p->d_tfpart, p->d_tq1};
+                    p->d_tfrec, p->d_tfroff, p->d_tfdesc, p->d_tfpart, p->d_tfsrec, p->d_tfsro, p->d_tq1};
     for (void *q : ptrs)
         if (q) (void)hipFree(q);
     free_work(p);

@@ -95,6 +95,11 @@ struct Plan {
     int4 *d_tfdesc = nullptr;    // [nblocks][D] + 1 (record start, offset / partial start, jmin, nrow);
                                  // the extra entry's record start = the total (run lengths by difference)
     float *d_tfpart = nullptr;   // partial windows (one sinogram), summed per bin by a second pass
+    // the same records with each pair's rows in order of decreasing record count (a row's
+    // records keep their order), and per sorted position (first record | row << 16), entry
+    // nrow = (record count, 0); indexed like d_tfrec / d_tfroff
+    uint32_t *d_tfsrec = nullptr;
+    uint32_t *d_tfsro = nullptr;
     int tf_nbz = 0, tf_nbx = 0;  // blocks along z / x
     size_t tf_bytes = 0;
     int32_t *d_rec_off = nullptr;   // [D][nbk + 1] record offsets / [D][NB + 1]

@@ -2883,7 +2883,8 @@ static int scatter_choice(const Plan *p, int Bp)
 // 8 pairs in flight; 1: XCD-aware order, 8 in flight; 2: XCD-aware, 16 in flight
 static bool pull_choice_ok(const Plan *p, int c, int Bp)
 {
-    if (c == 5 || c == 6 || c == 7) return Bp == 1 && p->d_tfrec && p->d_img_slot;
+    if (c == 5 || c == 6 || c == 7 || c == 9 || c == 10) return Bp == 1 && p->d_tfrec && p->d_img_slot;
+    if (c >= 11 && c <= 13) return Bp == 1 && p->d_tfsrec && p->d_img_slot;
     if (c > 0) return c == 1 || ((c == 2 || c == 3) && Bp == 1) || ((c == 4 || c == 8 || c == 16) && Bp % c == 0);
     if (c < 0) {
         const int lb = (-c) % 100, mode = (-c) / 100;
@@ -2921,6 +2922,21 @@ __global__ __launch_bounds__(256) void k_img_to_slots(const int32_t *__restrict_
     out[sl] = px >= 0 ? img[px] : 0.0f;
 }
 
+// The 4096 slot values of a 4 x 4-tile block (tile k of the block: slots 256 k ..) into
+// LDS, 0 for tiles past the grid; all 16 loads of a thread in flight before the writes
+__device__ __forceinline__ void stage_block_image(float *v, const float *__restrict__ img_slot, int bx, int bz, int ntx,
+                                                  int ntz, int t)
+{
+    float x[16];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+        const int tx = 4 * bx + (k >> 2), tz = 4 * bz + (k & 3);
+        x[k] = (tx < ntx && tz < ntz) ? img_slot[(size_t)(tx * ntz + tz) * 256 + t] : 0.0f;
+    }
+#pragma unroll
+    for (int k = 0; k < 16; ++k) v[t + 256 * k] = x[k];
+}
+
 // Tile forward for one sinogram, pass 1: workgroup = one 4 x 4-tile block (its 4096 slot
 // values staged in LDS) x a group of detectors; thread per window row r of a (block,
 // detector) pair: partial[r] = sum of (1 - u) v over the records of row r + sum of u v over
@@ -2942,10 +2958,7 @@ __global__ __launch_bounds__(256) void k_fwd_tile1(const float *__restrict__ img
     __shared__ int4 ed[G + 1];
     const int T = blockIdx.x, bx = T / nbz, bz = T - bx * nbz;
     const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
-    for (int i = t; i < 4096; i += 256) {
-        const int lt = i >> 8, tx = 4 * bx + (lt >> 2), tz = 4 * bz + (lt & 3);
-        v[i] = (tx < ntx && tz < ntz) ? img_slot[(size_t)(tx * ntz + tz) * 256 + (i & 255)] : 0.0f;
-    }
+    stage_block_image(v, img_slot, bx, bz, ntx, ntz, t);
     const int d0 = blockIdx.y * G, d1 = min(D, d0 + G);
     // the group's descriptors (and the next entry, for the run lengths) up front: the
     // record loads of a detector then wait on no other global load
@@ -3092,15 +3105,219 @@ __global__ __launch_bounds__(256) void k_fwd_tile1_sum(const float *__restrict__
     }
 }
 
-template <int G>
+// Tile forward for one sinogram, pass 1, lean form of k_fwd_tile1 (same tables, same
+// partials): the descriptors are read as scalars, the record and row-offset loads of a
+// detector are unconditional (indices clamped to the pair's last entry; entries past it
+// are never read back), a row's records are read from LDS at immediate offsets in chunks
+// of C, and a row sums s_x = sum v and s_t = sum t v (t = u 2^20, the record's integer u),
+// so that its own part is s_x - s_t 2^-20 and its up part s_t 2^-20: four VALU
+// operations per record for the sums instead of six.
+template <int G, int C>
+__global__ __launch_bounds__(256) void k_fwd_tile1r(const float *__restrict__ img_slot, const uint32_t *__restrict__ rec,
+                                                    const uint16_t *__restrict__ roff, const int4 *__restrict__ desc,
+                                                    float *__restrict__ part, int nbz, int ntz, int ntx, int D)
+{
+    constexpr int RPT = 4096 / 256, OPT = kTfMaxRows / 256;
+    __shared__ float v[4096];
+    __shared__ uint32_t rb[4096 + C];   // C entries of slack: a chunk may run past the pair's last record
+    __shared__ uint16_t ob[kTfMaxRows];
+    __shared__ float edge[OPT][4];
+    const int T = blockIdx.x, bx = T / nbz, bz = T - bx * nbz;
+    const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+    stage_block_image(v, img_slot, bx, bz, ntx, ntz, t);
+    if (t < C) rb[4096 + t] = 0u;
+    const int d0 = blockIdx.y * G, d1 = min(D, d0 + G);
+    const int4 *dp = desc + (size_t)T * D;   // entry d + 1 always exists (sentinel after the last pair)
+    uint32_t ra[RPT], rb2[RPT];
+    uint16_t oa[OPT], ob2[OPT];
+    auto fetch = [&](int d, uint32_t (&rr)[RPT], uint16_t (&orr)[OPT]) __attribute__((always_inline)) {
+        const int4 e = dp[d];
+        const int nl = max(dp[d + 1].x - e.x - 1, 0);   // the pair's last record (the next entry's first if none)
+        const uint32_t *r0 = rec + e.x;
+#pragma unroll
+        for (int i = 0; i < RPT; ++i) rr[i] = r0[min(t + 256 * i, nl)];
+        const uint16_t *o0 = roff + e.y;
+#pragma unroll
+        for (int i = 0; i < OPT; ++i) orr[i] = o0[min(t + 256 * i, e.w)];
+    };
+    const float su = 1.0f / 1048576.0f;
+    auto step = [&](int d, uint32_t (&rr)[RPT], uint16_t (&orr)[OPT]) __attribute__((always_inline)) {
+        const int4 e = dp[d];
+        __syncthreads();   // the previous detector's rows done with rb / ob / edge (and v written)
+#pragma unroll
+        for (int i = 0; i < RPT; ++i) rb[t + 256 * i] = rr[i];
+#pragma unroll
+        for (int i = 0; i < OPT; ++i) ob[t + 256 * i] = orr[i];
+        __syncthreads();
+        if (d + 2 < d1) fetch(d + 2, rr, orr);
+        float own[OPT], upl[OPT];
+#pragma unroll
+        for (int i = 0; i < OPT; ++i) {
+            const int r = t + 256 * i;
+            float sx = 0.0f, st = 0.0f;
+            if (r < e.w) {
+                const int b = ob[r], c = ob[r + 1];
+                for (int k0 = b; k0 < c; k0 += C) {
+                    const int rem = c - k0;
+                    uint32_t q[C];
+                    float val[C];
+#pragma unroll
+                    for (int j = 0; j < C; ++j) q[j] = rb[k0 + j];
+#pragma unroll
+                    for (int j = 0; j < C; ++j) val[j] = v[q[j] & 4095u];
+#pragma unroll
+                    for (int j = 0; j < C; ++j) {
+                        const float x = j < rem ? val[j] : 0.0f;
+                        sx += x;
+                        st = fmaf((float)(q[j] >> 12), x, st);
+                    }
+                }
+            }
+            const float a1 = st * su;
+            own[i] = sx - a1;
+            upl[i] = __shfl_up(a1, 1);
+            if (lane == 63) edge[i][wv] = a1;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < OPT; ++i) {
+            const int r = t + 256 * i;
+            if (r <= e.w) {
+                const float prev = lane > 0 ? upl[i] : (wv > 0 ? edge[i][wv - 1] : (i > 0 ? edge[i - 1][3] : 0.0f));
+                part[e.y + r] = (r < e.w ? own[i] : 0.0f) + prev;
+            }
+        }
+    };
+    if (d0 < d1) fetch(d0, ra, oa);
+    if (d0 + 1 < d1) fetch(d0 + 1, rb2, ob2);
+    for (int d = d0; d < d1; d += 2) {
+        step(d, ra, oa);
+        if (d + 1 < d1) step(d + 1, rb2, ob2);
+    }
+}
+
+// Tile forward for one sinogram, pass 1, sorted-row form (the partials of k_fwd_tile1): each
+// pair's rows come in order of decreasing record count (d_tfsrec / d_tfsro), so a block of
+// 64 consecutive sorted positions (lane = position) has nearly equal record counts and its
+// lanes run nearly the same number of iterations (the natural row order wastes about half
+// of them).  The blocks of a pair go to the four waves in snake order (0 1 2 3 3 2 1 0 ..),
+// which balances their descending costs; each lane sums s_x = sum v and s_t = sum t v over
+// its row's records (as k_fwd_tile1r, records in the same order: identical sums), the rows'
+// own and up parts go to LDS by row, and the window partials are written in row order.
+template <int G, int C>
+__global__ __launch_bounds__(256) void k_fwd_tile1s(const float *__restrict__ img_slot, const uint32_t *__restrict__ srec,
+                                                    const uint32_t *__restrict__ sro, const int4 *__restrict__ desc,
+                                                    float *__restrict__ part, int nbz, int ntz, int ntx, int D)
+{
+    constexpr int RPT = 4096 / 256, OPT = kTfMaxRows / 256, KB = kTfMaxRows / 64 / 4;   // blocks per wave
+    // one array (40 KB: four workgroups per CU): slot values, records (after the row sums:
+    // own part by row at [0, 2048), up part at [2048, 4096)), sorted positions; a chunk
+    // that runs past the last record reads position entries, with weight 0
+    __shared__ uint32_t smem[4096 + 4096 + kTfMaxRows];
+    float *v = reinterpret_cast<float *>(smem);
+    uint32_t *rb = smem + 4096, *pb = smem + 8192;
+    const int T = blockIdx.x, bx = T / nbz, bz = T - bx * nbz;
+    const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+    stage_block_image(v, img_slot, bx, bz, ntx, ntz, t);
+    const int d0 = blockIdx.y * G, d1 = min(D, d0 + G);
+    const int4 *dp = desc + (size_t)T * D;   // entry d + 1 always exists (sentinel after the last pair)
+    uint32_t ra[RPT], rb2[RPT], oa[OPT], ob2[OPT];
+    auto fetch = [&](int d, uint32_t (&rr)[RPT], uint32_t (&orr)[OPT]) __attribute__((always_inline)) {
+        const int4 e = dp[d];
+        const int nl = max(dp[d + 1].x - e.x - 1, 0);   // the pair's last record (the next entry's first if none)
+        const uint32_t *r0 = srec + e.x;
+#pragma unroll
+        for (int i = 0; i < RPT; ++i) rr[i] = r0[min(t + 256 * i, nl)];
+        const uint32_t *o0 = sro + e.y;
+#pragma unroll
+        for (int i = 0; i < OPT; ++i) orr[i] = o0[min(t + 256 * i, e.w)];
+    };
+    const float su = 1.0f / 1048576.0f;
+    auto step = [&](int d, uint32_t (&rr)[RPT], uint32_t (&orr)[OPT]) __attribute__((always_inline)) {
+        const int4 e = dp[d];
+        __syncthreads();   // the previous detector done with rb / pb (and v written)
+#pragma unroll
+        for (int i = 0; i < RPT; ++i) rb[t + 256 * i] = rr[i];
+#pragma unroll
+        for (int i = 0; i < OPT; ++i) pb[t + 256 * i] = orr[i];
+        __syncthreads();
+        if (d + 2 < d1) fetch(d + 2, rr, orr);
+        const int nb = (e.w + 63) >> 6;   // blocks of 64 sorted positions
+        float own[KB], up[KB];
+        int row[KB];
+#pragma unroll
+        for (int k = 0; k < KB; ++k) {
+            const int blk = 4 * k + ((k & 1) ? 3 - wv : wv);
+            float sx = 0.0f, st = 0.0f;
+            row[k] = -1;
+            if (blk < nb) {
+                const int p = 64 * blk + lane;
+                if (p < e.w) {
+                    const uint32_t a = pb[p];
+                    const int b = a & 0xffffu, c = pb[p + 1] & 0xffffu;
+                    row[k] = a >> 16;
+                    for (int k0 = b; k0 < c; k0 += C) {
+                        const int rem = c - k0;
+                        uint32_t q[C];
+                        float val[C];
+#pragma unroll
+                        for (int j = 0; j < C; ++j) q[j] = rb[k0 + j];
+#pragma unroll
+                        for (int j = 0; j < C; ++j) val[j] = v[q[j] & 4095u];
+#pragma unroll
+                        for (int j = 0; j < C; ++j) {
+                            const float x = j < rem ? val[j] : 0.0f;
+                            sx += x;
+                            st = fmaf((float)(q[j] >> 12), x, st);
+                        }
+                    }
+                }
+            }
+            up[k] = st * su;
+            own[k] = sx - up[k];
+        }
+        __syncthreads();   // every record read: rb takes the row parts
+#pragma unroll
+        for (int k = 0; k < KB; ++k)
+            if (row[k] >= 0) {
+                rb[row[k]] = __float_as_uint(own[k]);
+                rb[2048 + row[k]] = __float_as_uint(up[k]);
+            }
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < OPT; ++i) {
+            const int r = t + 256 * i;
+            if (r <= e.w)
+                part[e.y + r] = (r < e.w ? __uint_as_float(rb[r]) : 0.0f) + (r > 0 ? __uint_as_float(rb[2047 + r]) : 0.0f);
+        }
+    };
+    if (d0 < d1) fetch(d0, ra, oa);
+    if (d0 + 1 < d1) fetch(d0 + 1, rb2, ob2);
+    for (int d = d0; d < d1; d += 2) {
+        step(d, ra, oa);
+        if (d + 1 < d1) step(d + 1, rb2, ob2);
+    }
+}
+
+// PASS1 < 0: k_fwd_tile1<G>; PASS1 > 0: k_fwd_tile1r<G, PASS1>; PASS1 <= -2: k_fwd_tile1s<G, -PASS1 - 1>
+template <int G, int PASS1>
 static void fwd_tile1(Plan *p, const float *imgI, hipStream_t s)
 {
     hipLaunchKernelGGL(k_img_to_slots, dim3((p->Pd + 255) / 256), dim3(256), 0, s, p->d_pix, imgI, p->d_img_slot,
                        p->Pd);
     const int ntz = (p->Nz + 15) / 16, ntx = (p->Nx + 15) / 16, nblk = p->tf_nbz * p->tf_nbx;
-    hipLaunchKernelGGL(k_fwd_tile1<G>, dim3(nblk, (p->D + G - 1) / G), dim3(256), 0, s,
-                       (const float *)p->d_img_slot, p->d_tfrec, p->d_tfroff, p->d_tfdesc, p->d_tfpart, p->tf_nbz,
-                       ntz, ntx, p->D);
+    if constexpr (PASS1 <= -2)
+        hipLaunchKernelGGL((k_fwd_tile1s<G, -PASS1 - 1>), dim3(nblk, (p->D + G - 1) / G), dim3(256), 0, s,
+                           (const float *)p->d_img_slot, p->d_tfsrec, p->d_tfsro, p->d_tfdesc, p->d_tfpart, p->tf_nbz,
+                           ntz, ntx, p->D);
+    else if constexpr (PASS1 < 0)
+        hipLaunchKernelGGL(k_fwd_tile1<G>, dim3(nblk, (p->D + G - 1) / G), dim3(256), 0, s,
+                           (const float *)p->d_img_slot, p->d_tfrec, p->d_tfroff, p->d_tfdesc, p->d_tfpart, p->tf_nbz,
+                           ntz, ntx, p->D);
+    else
+        hipLaunchKernelGGL((k_fwd_tile1r<G, PASS1>), dim3(nblk, (p->D + G - 1) / G), dim3(256), 0, s,
+                           (const float *)p->d_img_slot, p->d_tfrec, p->d_tfroff, p->d_tfdesc, p->d_tfpart, p->tf_nbz,
+                           ntz, ntx, p->D);
     hipLaunchKernelGGL(k_fwd_tile1_sum, dim3((p->geom.NB + 1023) / 1024, p->D), dim3(256), 0, s,
                        (const float *)p->d_tfpart, p->d_tfdesc, p->d_bins, p->geom.NB, nblk, p->D);
 }
@@ -3177,16 +3394,21 @@ static void fwd_pull_wt(Plan *p, const float *imgI, int Bp, hipStream_t s, int J
 
 static int launch_pull(Plan *p, const float *imgI, int Bp, hipStream_t s)
 {
-    // one sinogram: the tile form (4 detectors per workgroup) when its tables exist
-    const int dflt = (Bp == 1 && pull_choice_ok(p, 6, Bp)) ? 6 : default_pull_choice(Bp);
+    // one sinogram: the tile form over sorted rows (2 detectors per workgroup) when its tables exist
+    const int dflt = (Bp == 1 && pull_choice_ok(p, 11, Bp)) ? 11 : default_pull_choice(Bp);
     int c = (p->tune.Bp == Bp && p->tune.scatter_bc != 0) ? p->tune.scatter_bc : dflt;
     if (!pull_choice_ok(p, c, Bp)) c = dflt;
     switch (c) {
     case 1: fwd_pull<1>(p, imgI, Bp, s); break;
     case 2: fwd_pull_b1<1>(p, imgI, s); break;
-    case 5: fwd_tile1<8>(p, imgI, s); break;
-    case 6: fwd_tile1<4>(p, imgI, s); break;
-    case 7: fwd_tile1<2>(p, imgI, s); break;
+    case 5: fwd_tile1<8, -1>(p, imgI, s); break;
+    case 6: fwd_tile1<4, -1>(p, imgI, s); break;
+    case 7: fwd_tile1<2, -1>(p, imgI, s); break;
+    case 9: fwd_tile1<2, 4>(p, imgI, s); break;
+    case 10: fwd_tile1<4, 4>(p, imgI, s); break;
+    case 11: fwd_tile1<2, -5>(p, imgI, s); break;
+    case 12: fwd_tile1<2, -3>(p, imgI, s); break;
+    case 13: fwd_tile1<4, -3>(p, imgI, s); break;
     case 3: fwd_pull_b1<4>(p, imgI, s); break;
     case 4: fwd_pull<4>(p, imgI, Bp, s); break;
     case 8: fwd_pull<8>(p, imgI, Bp, s); break;
@@ -3995,7 +4217,7 @@ int autotune(Plan *p, int B, int reps, hipStream_t s, int *gather_out, int *scat
     // scatter: record-lane LDS batch chunks, then batch-lane lane groups
     std::vector<int> cand;
     if (p->pull) {
-        for (int c : {6, 5, 7, 1, 2, 3, 4, 8, 16, -16, -32, -64, -116, -132, -164, -216, -232, -264, -364, -464, -564, -664,
+        for (int c : {11, 12, 13, 9, 10, 6, 5, 7, 1, 2, 3, 4, 8, 16, -16, -32, -64, -116, -132, -164, -216, -232, -264, -364, -464, -564, -664,
                       -764, -864, -964, -1064, -1164})
             if (pull_choice_ok(p, c, Bp) && !(c == 1 && Bp > 16)) cand.push_back(c);
     } else {   // SIR: the pull forms first

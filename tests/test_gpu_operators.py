@@ -197,8 +197,8 @@ def test_every_launch_variant_matches_oracle(sir, B):
     sd = torch.tensor(sg, dtype=torch.float32).cuda()
     n_ok = 0
     for gv in range(_lib.lib().mbrec_num_gather_variants()):
-        for sc in (1, 2, 3, 4, 5, 6, 7, 8, 16, -16, -32, -64, -116, -164, -216, -264, -364, -464, -564, -664, -764, -864,
-                   -964, -1064, -1164, 1000, 1001, 1002, 1003, 1004):
+        for sc in (1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 16, -16, -32, -64, -116, -164, -216, -264, -364, -464,
+                   -564, -664, -764, -864, -964, -1064, -1164, 1000, 1001, 1002, 1003, 1004):
             try:
                 plan.set_launch(B, gv, sc)
             except Exception:
@@ -214,7 +214,9 @@ def test_every_launch_variant_matches_oracle(sir, B):
 
 def test_one_sinogram_tile_variants_match_oracle():
     """Every one-sinogram tile form on a real arc geometry (the tile windows fit the LDS only
-    for smooth travel times): forward choices 2, 3 (record streaming) and 5, 6, 7 (tile form),
+    for smooth travel times): forward choices 2, 3 (record streaming), 5, 6, 7 (tile form, thread
+    per row), 9, 10 (its lean form) and 11, 12, 13 (the lean form over rows sorted by record
+    count: the same sums as 9, bit for bit),
     gather variants 44-54 (LDS tile gathers, packed tile-major table), each against the
     oracle's DAS loops."""
     import torch
@@ -245,10 +247,15 @@ def test_one_sinogram_tile_variants_match_oracle():
             assert ref is not None and rel_l2(xt, ref) < 1e-6
         done.append(gv)
     assert {47, 53}.issubset(done), done
-    for ch in (2, 3, 5, 6, 7):
+    lean = None
+    for ch in (2, 3, 5, 6, 7, 9, 10, 11, 12, 13):
         plan.set_launch(1, 47, ch)
         y = plan.forward(xd, model=False).cpu().numpy()
         assert rel_l2(y, yo) < TOL_OP, ch
+        if ch == 9:
+            lean = y
+        if ch >= 10:   # chunk size, detector grouping and row order leave each row's sum unchanged
+            assert np.array_equal(y, lean), ch
     plan.set_launch(1, -1, 0)
 
 
