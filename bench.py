@@ -24,7 +24,8 @@ scaled to one plain solve of the same iteration count.
 
 Other modes: --fixed-iters N (and --sir) time a fixed-iteration plain solve; --mixed the
 224-sinogram mixed-grid batch; --tv600 the TV configuration; --secondary the secondary
-paths; --mex the per-call time of the MEX drop-ins.
+paths; --mex the per-call time of the MEX drop-ins; --pair-b1 the one-sinogram operator
+pair's launch choices; --copy-bw the box's device copy rate.
 """
 import argparse
 import json
@@ -55,6 +56,8 @@ def parse():
     ap.add_argument("--pair-b1", action="store_true",
                     help="one-sinogram operator pair at 400^2 DAS: every forward choice and tile-gather "
                          "variant timed (HIP events) and checked against the default; prints one JSON line")
+    ap.add_argument("--copy-bw", action="store_true",
+                    help="time a 1 GiB device-to-device copy (the box's practical HBM rate); prints one JSON line")
     ap.add_argument("--mex-reps", type=int, default=20)
     ap.add_argument("--grid", type=int, default=400)
     ap.add_argument("--sir", action="store_true", help="400x400 with SIR (the SIR parity configuration)")
@@ -586,6 +589,17 @@ def pair_b1(a, reps=50):
     print(json.dumps(out), flush=True)
 
 
+def copy_bw(reps=20):
+    """Practical HBM rate: a 1 GiB device copy (read + write traffic) timed with HIP events."""
+    import torch
+    n = 1 << 28
+    x = torch.empty(n, dtype=torch.float32, device="cuda").fill_(1.0)
+    y = torch.empty_like(x)
+    ms, _ = events_ms(lambda: y.copy_(x), reps, 3)
+    print(json.dumps({"copy_bytes": 4 * n, "ms": round(ms, 4), "read_plus_write_GB/s": round(8 * n / (ms * 1e-3) / 1e9, 1),
+                      "hbm_peak_GB/s": HBM_PEAK_GBS}), flush=True)
+
+
 def metric_bench(a):
     """The headline metric: recon/s of batch-64 NNLS-L2 at 400^2 x 256 ch reaching 1e-3
     relative L2 of the converged solution, learned warm start vs plain cold start."""
@@ -784,7 +798,9 @@ def main():
         print(f"bench.py runs on one GPU only; --gpus {a.gpus} is not supported "
               "(batches of independent solves would shard across GPUs with no exchange)", flush=True)
         sys.exit(2)
-    if a.pair_b1:
+    if a.copy_bw:
+        copy_bw()
+    elif a.pair_b1:
         pair_b1(a)
     elif a.mex:
         mex_bench(a)

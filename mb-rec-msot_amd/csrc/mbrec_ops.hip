@@ -158,7 +158,7 @@ using PfAcc = float;
 // KT > 0: K = KT known at compile time, each thread forms kFiltR consecutive
 // outputs of a column from one register window of kFiltR + 2 KT samples
 // (staged columns carry kFiltR extra samples so the last window stays inside).
-template <bool IL, int KT>
+template <bool IL, int KT, int COLS = kFzCols>
 __global__ __launch_bounds__(256) void k_filt_z(const float *__restrict__ in, float *__restrict__ out,
                                                 const float *__restrict__ k, int K, int Nz, int Nx, int B, int Bp,
                                                 int transp)
@@ -173,15 +173,15 @@ __global__ __launch_bounds__(256) void k_filt_z(const float *__restrict__ in, fl
     size_t col0 = 0;
     if (IL) {
         ixc = blockIdx.x;
-        b0 = blockIdx.y * kFzCols;
+        b0 = blockIdx.y * COLS;
     } else {
-        col0 = (size_t)blockIdx.x * kFzCols;
+        col0 = (size_t)blockIdx.x * COLS;
     }
     for (int i = threadIdx.x; i < 2 * K + 1; i += blockDim.x) tk[i] = (PfAcc)k[i];
     const int W = Nz + 2 * K + (KT ? kFiltR : 0);
     const float invW = 1.0f / (float)W;
     stage_batched<16, 256>(
-        threadIdx.x, kFzCols * W,
+        threadIdx.x, COLS * W,
         [&](int e) {
             int c, i;
             divmod_small(e, W, invW, c, i);
@@ -198,15 +198,16 @@ __global__ __launch_bounds__(256) void k_filt_z(const float *__restrict__ in, fl
         });
     __syncthreads();
     // out[i] = sum_a k[a + K] xs[i + K - sgn a]: forward reads xs[i + 2K - t], transpose xs[i + t] (t = a + K)
-    if (KT) {   // 16 threads per column, blocks of kFiltR outputs
+    if (KT) {   // planar: 256 / COLS threads per column, blocks of kFiltR outputs
         constexpr int NT = 2 * (KT ? KT : 1) + 1, NW = kFiltR + NT - 1;
         PfAcc tr[NT];
 #pragma unroll
         for (int t = 0; t < NT; ++t) tr[t] = tk[t];
-        const int c = IL ? threadIdx.x % kFzCols : threadIdx.x / 16, ib = IL ? threadIdx.x / kFzCols : threadIdx.x % 16;
+        constexpr int TPC = 256 / COLS;   // threads per column (planar)
+        const int c = IL ? threadIdx.x % COLS : threadIdx.x / TPC, ib = IL ? threadIdx.x / COLS : threadIdx.x % TPC;
         const bool live = IL ? b0 + c < Bp : col0 + c < (size_t)B * Nx;
         if (!live) return;
-        for (int iz0 = ib * kFiltR; iz0 < Nz; iz0 += 16 * kFiltR) {
+        for (int iz0 = ib * kFiltR; iz0 < Nz; iz0 += (IL ? 16 : TPC) * kFiltR) {
             const float *x = xs + c * S + iz0;
             PfAcc w[NW], acc[kFiltR];
 #pragma unroll
@@ -235,10 +236,10 @@ __global__ __launch_bounds__(256) void k_filt_z(const float *__restrict__ in, fl
         return;
     }
     if (IL) {
-        const int c = threadIdx.x % kFzCols;
+        const int c = threadIdx.x % COLS;
         const int b = b0 + c;
         if (b >= Bp) return;
-        for (int iz = threadIdx.x / kFzCols; iz < Nz; iz += 256 / kFzCols) {
+        for (int iz = threadIdx.x / COLS; iz < Nz; iz += 256 / COLS) {
             const float *x = xs + c * S + iz;
             PfAcc acc = 0;
             if (transp)
@@ -248,7 +249,7 @@ __global__ __launch_bounds__(256) void k_filt_z(const float *__restrict__ in, fl
             out[((size_t)ixc * Nz + iz) * Bp + b] = b < B ? (float)acc : 0.0f;
         }
     } else {
-        for (int c = 0; c < kFzCols; ++c) {
+        for (int c = 0; c < COLS; ++c) {
             if (col0 + c >= (size_t)B * Nx) break;
             for (int iz = threadIdx.x; iz < Nz; iz += blockDim.x) {
                 const float *x = xs + c * S + iz;
@@ -4012,17 +4013,34 @@ static void prefilter_passes(Plan *p, const float *in, float *out, float *tmp, i
     const dim3 gx((p->Nz + kFxRows - 1) / kFxRows, B);
     const int tr = transp ? 1 : 0;
     const dim3 gzp(((size_t)B * p->Nx + kFzCols - 1) / kFzCols);
+    // one sinogram: 4 columns per workgroup (64 threads each), so that a 400-column image
+    // spreads over 100 workgroups instead of 25
+    constexpr int C1 = 4;
+    const dim3 gz1((p->Nx + C1 - 1) / C1);
+    const size_t ldz1 = tapb + sizeof(float) * (size_t)C1 * ((p->Nz + 2 * p->Kz + (KZ ? kFiltR : 0)) | 1);
+    const bool one = B == 1 && KZ > 0;
     if (!transp) {   // forward: x pass then z pass (the z pass writes the interleaved layout)
         hipLaunchKernelGGL((k_filt_x<KX>), gx, dim3(256), ldx, s, in, tmp, p->d_kx, p->Kx, p->Nz, p->Nx, 0);
-        if (interleaved_out)   // (measured: the register-blocked form is slower for this layout)
+        // interleaved output [P][Bp]: for one sinogram it is the planar layout, which the
+        // register-blocked planar pass writes with every thread busy (the interleaved pass
+        // gives one of its 16 column lanes work there); for batches the interleaved pass
+        // (measured: the register-blocked form is slower for that layout)
+        if (one)
+            hipLaunchKernelGGL((k_filt_z<false, KZ, C1>), gz1, dim3(256), ldz1, s, (const float *)tmp, out, p->d_kz,
+                               p->Kz, p->Nz, p->Nx, B, Bp, 0);
+        else if (interleaved_out)
             hipLaunchKernelGGL((k_filt_z<true, 0>), dim3(p->Nx, (Bp + kFzCols - 1) / kFzCols), dim3(256), ldz0, s,
                                (const float *)tmp, out, p->d_kz, p->Kz, p->Nz, p->Nx, B, Bp, 0);
         else
             hipLaunchKernelGGL((k_filt_z<false, KZ>), gzp, dim3(256), ldz, s, (const float *)tmp, out, p->d_kz, p->Kz,
                                p->Nz, p->Nx, B, Bp, 0);
     } else {   // transpose: z^T then x^T
-        hipLaunchKernelGGL((k_filt_z<false, KZ>), gzp, dim3(256), ldz, s, in, tmp, p->d_kz, p->Kz, p->Nz, p->Nx, B,
-                           Bp, tr);
+        if (one)
+            hipLaunchKernelGGL((k_filt_z<false, KZ, C1>), gz1, dim3(256), ldz1, s, in, tmp, p->d_kz, p->Kz, p->Nz,
+                               p->Nx, B, Bp, tr);
+        else
+            hipLaunchKernelGGL((k_filt_z<false, KZ>), gzp, dim3(256), ldz, s, in, tmp, p->d_kz, p->Kz, p->Nz, p->Nx,
+                               B, Bp, tr);
         hipLaunchKernelGGL((k_filt_x<KX>), gx, dim3(256), ldx, s, (const float *)tmp, out, p->d_kx, p->Kx, p->Nz,
                            p->Nx, tr);
     }
