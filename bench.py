@@ -578,6 +578,12 @@ def pair_b1(a, reps=50):
     for name, fn, arg in (("model_forward_us", m.applyForward, x), ("model_transpose_us", m.applyTranspose, sg)):
         ms, _ = events_ms(lambda: fn(arg), reps, 5)
         out[name] = round(ms * 1e3, 2)
+    m.plan.set_fir_kernel(True)   # the IR filter on the FP64 matrix cores, for comparison
+    for name, fn, arg in (("model_forward_fir_mfma_us", m.applyForward, x),
+                          ("model_transpose_fir_mfma_us", m.applyTranspose, sg)):
+        ms, _ = events_ms(lambda: fn(arg), reps, 5)
+        out[name] = round(ms * 1e3, 2)
+    m.plan.set_fir_kernel(False)
     out.update({"workload": f"{N}x{N} px, {D} ch x {nT} samples, DAS, one sinogram",
                 "algorithmic_bytes": alg, "hbm_peak_GB/s": HBM_PEAK_GBS})
     if a.traffic_calibration:

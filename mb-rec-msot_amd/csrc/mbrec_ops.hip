@@ -3621,6 +3621,11 @@ static void z_to_bins_bc(Plan *p, const float *z, int B, int Bp, hipStream_t s, 
 
 static int launch_z_to_bins(Plan *p, const float *z, int B, int Bp, hipStream_t s, const int2 *win = nullptr)
 {
+    if (Bp == 1) {   // one sinogram: the chunk form with a chunk of one (taps in registers, same sums)
+        z_to_bins_bc<1>(p, z, B, Bp, s, win);
+        MBREC_HIP(hipGetLastError());
+        return MBREC_OK;
+    }
     if (Bp >= 16) {
         if (Bp >= 64) z_to_bins_bc<64>(p, z, B, Bp, s, win);
         else if (Bp >= 32) z_to_bins_bc<32>(p, z, B, Bp, s, win);
