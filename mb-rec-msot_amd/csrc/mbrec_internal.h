@@ -160,6 +160,15 @@ struct Plan {
 
 static inline int batch_stride(int B) { return B <= 1 ? 1 : (B + 3) / 4 * 4; }
 
+// q = e / d, r = e % d for 0 <= e < 2^24, 0 < d < 2^16 (float reciprocal, one correction step)
+__device__ __forceinline__ void divmod_small(int e, int d, float inv, int &q, int &r)
+{
+    q = (int)((float)e * inv);
+    r = e - q * d;
+    if (r < 0) { --q; r += d; }
+    else if (r >= d) { ++q; r -= d; }
+}
+
 // error state -------------------------------------------------------------
 int set_error(int code, const char *id, const char *fmt, ...);
 int hip_check(hipError_t e, const char *what);

@@ -86,15 +86,6 @@ __device__ __forceinline__ void stage_batched(int t, int n, Ld &&ld, St &&st)
     }
 }
 
-// q = e / d, r = e % d for 0 <= e < 2^24, 0 < d < 2^16 (float reciprocal, one correction step)
-__device__ __forceinline__ void divmod_small(int e, int d, float inv, int &q, int &r)
-{
-    q = (int)((float)e * inv);
-    r = e - q * d;
-    if (r < 0) { --q; r += d; }
-    else if (r >= d) { ++q; r -= d; }
-}
-
 // ---------------------------------------------------------------------------
 // planar [B][P] -> interleaved [P][Bp] (padding columns zeroed)
 // ---------------------------------------------------------------------------

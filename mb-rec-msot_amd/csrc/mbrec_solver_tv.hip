@@ -62,30 +62,51 @@ __device__ inline int nm_perm(int i, int N) { return (i & 1) ? N - 1 - (i >> 1) 
 // Stage with radix p after Ns points per sub-transform: output (j, q), j < N/p, q < p,
 // goes to (j / Ns) Ns p + j % Ns + q Ns and is sum_r a[j + r N/p] w^(r t),
 // t = (j % Ns + q Ns) N / (Ns p).  Returns the buffer holding the result.
+// one output of a radix-p stage: sum_r src[r m] w^(r t) (P > 0: p known at compile time)
+template <int P>
+__device__ __forceinline__ double2 nm_bfly(const double2 *src, int p, int m, int t, int N,
+                                           const double2 *__restrict__ tw, bool inverse)
+{
+    double re = 0.0, im = 0.0;
+    int idx = 0;
+#pragma unroll
+    for (int r = 0; r < (P ? P : p); ++r) {
+        const double2 w = tw[idx];
+        const double wi = inverse ? -w.y : w.y;
+        const double2 x = src[r * m];
+        re += x.x * w.x - x.y * wi;
+        im += x.x * wi + x.y * w.x;
+        idx += t;
+        if (idx >= N) idx -= N;
+    }
+    return make_double2(re, im);
+}
+
 __device__ double2 *nm_fft(double2 *a, double2 *b, int N, int C, const NmFactors &f,
                            const double2 *__restrict__ tw, bool inverse)
 {
     int Ns = 1;
+    const float invN = 1.0f / (float)N;
     for (int s = 0; s < f.n; ++s) {
         const int p = f.p[s], m = N / p, step = N / (Ns * p);
+        const float invm = 1.0f / (float)m, invNs = 1.0f / (float)Ns;   // the stage's divisions as reciprocals
         for (int o = threadIdx.x; o < C * N; o += blockDim.x) {
-            const int c = o / N, e = o - c * N;
-            const int q = e / m, j = e - q * m;
-            const int k = j % Ns;
+            int c, e, q, j, jn, k;
+            divmod_small(o, N, invN, c, e);
+            divmod_small(e, m, invm, q, j);
+            divmod_small(j, Ns, invNs, jn, k);
             const int t = (k + q * Ns) * step;
             const double2 *src = a + c * N + j;
-            double re = 0.0, im = 0.0;
-            int idx = 0;
-            for (int r = 0; r < p; ++r) {
-                const double2 w = tw[idx];
-                const double wi = inverse ? -w.y : w.y;
-                const double2 x = src[r * m];
-                re += x.x * w.x - x.y * wi;
-                im += x.x * wi + x.y * w.x;
-                idx += t;
-                if (idx >= N) idx -= N;
+            double2 y;
+            switch (p) {   // uniform: the radices nm_factors produces first
+            case 2: y = nm_bfly<2>(src, p, m, t, N, tw, inverse); break;
+            case 3: y = nm_bfly<3>(src, p, m, t, N, tw, inverse); break;
+            case 4: y = nm_bfly<4>(src, p, m, t, N, tw, inverse); break;
+            case 5: y = nm_bfly<5>(src, p, m, t, N, tw, inverse); break;
+            case 7: y = nm_bfly<7>(src, p, m, t, N, tw, inverse); break;
+            default: y = nm_bfly<0>(src, p, m, t, N, tw, inverse); break;
             }
-            b[c * N + (j / Ns) * Ns * p + k + q * Ns] = make_double2(re, im);
+            b[c * N + jn * Ns * p + k + q * Ns] = y;
         }
         __syncthreads();
         double2 *sw = a;
@@ -538,7 +559,10 @@ static int prox_setup(TvProx &T, TvWork &w, int N, int B, hipStream_t s)
     MBREC_HIP(hipMemcpyAsync(T.lam, lam.data(), sizeof(double) * N, hipMemcpyHostToDevice, s));
     MBREC_HIP(hipStreamSynchronize(s));
     T.nm.N = N;
-    T.nm.C = std::max(1, std::min(4, kNmLdsPoints / N));
+    // row pairs per workgroup: about 512 complex points (at 600^2 one pair, 19 KB of LDS,
+    // eight workgroups per CU; measured 0.58 ms per Neumann solve of 32 images against
+    // 1.00 ms with three pairs, 57.6 KB and two workgroups per CU)
+    T.nm.C = std::max(1, std::min(4, 512 / N));
     if (!nm_factors(N, T.nm.f))
         return set_error(MBREC_ERR_ARG, "mbrec:tvSize", "rec_nn_with_TV_reg: too many prime factors in the image side");
     T.nm.tw = T.tw;
