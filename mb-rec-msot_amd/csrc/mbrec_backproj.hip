@@ -131,19 +131,50 @@ __global__ __launch_bounds__(256) void k_dmas_gather(const double *__restrict__ 
 // ---------------------------------------------------------------------------
 // backproject_waveeq
 // ---------------------------------------------------------------------------
-// radial integrals: S = cumtrapz(t, s/c) ./ t along time, t_k = (k + 1 + cropped) dt
-__global__ void k_cumtrapz(const double *__restrict__ s, double *__restrict__ out, int rows, int nT, double c,
-                           double dt, int cropped)
+// radial integrals: S = cumtrapz(t, s/c) ./ t along time, t_k = (k + 1 + cropped) dt.
+// A workgroup per row: thread i takes 8 consecutive samples of a 2048-sample chunk, forms
+// the trapezoid increments dt (s[k-1]/c + s[k]/c) / 2 and their running sum over its 8,
+// a block scan of the thread totals gives each thread its offset (the chunk's total carried
+// into the next chunk).  The partial sums are grouped by the scan instead of strictly left
+// to right: FP64 rounding differences only.
+__global__ __launch_bounds__(256) void k_cumtrapz(const double *__restrict__ s, double *__restrict__ out, int rows,
+                                                  int nT, double c, double dt, int cropped)
 {
-    const int r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= rows) return;
+    __shared__ double sc[256];
+    const int r = blockIdx.x, t = threadIdx.x;
+    if (r >= rows) return;   // the grid is one workgroup per row: uniform
     const double *a = s + (size_t)r * nT;
     double *o = out + (size_t)r * nT;
-    double acc = 0.0;
-    o[0] = 0.0;
-    for (int k = 1; k < nT; ++k) {
-        acc += dt * (a[k - 1] / c + a[k] / c) / 2.0;
-        o[k] = acc / ((k + 1 + cropped) * dt);
+    double carry = 0.0;
+    for (int k0 = 0; k0 < nT; k0 += 2048) {
+        const int kb = k0 + 8 * t;
+        double inc[8];
+        double qp = (kb >= 1 && kb - 1 < nT) ? a[kb - 1] / c : 0.0;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            const int k = kb + j;
+            const double q = k < nT ? a[k] / c : 0.0;
+            inc[j] = (k >= 1 && k < nT) ? dt * (qp + q) / 2.0 : 0.0;
+            qp = q;
+        }
+#pragma unroll
+        for (int j = 1; j < 8; ++j) inc[j] += inc[j - 1];
+        sc[t] = inc[7];
+        __syncthreads();
+        for (int off = 1; off < 256; off <<= 1) {   // inclusive scan of the thread totals
+            const double v = t >= off ? sc[t - off] : 0.0;
+            __syncthreads();
+            sc[t] += v;
+            __syncthreads();
+        }
+        const double base = carry + (t > 0 ? sc[t - 1] : 0.0);
+        carry += sc[255];
+        __syncthreads();   // sc is rewritten by the next chunk
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            const int k = kb + j;
+            if (k < nT) o[k] = k == 0 ? 0.0 : (base + inc[j]) / ((k + 1 + cropped) * dt);
+        }
     }
 }
 
@@ -242,13 +273,32 @@ __global__ void k_antisym(const double *__restrict__ M, double *__restrict__ K, 
     K[idx] = M[idx] - M[(size_t)j * Nt + k];
 }
 
+// scratch of one call, stream-ordered (no device-wide synchronisation on allocation or
+// release, which a plain hipFree would add to every call)
 struct DBuf {
+    hipStream_t s;
     std::vector<void *> all;
-    ~DBuf() { for (void *p : all) (void)hipFree(p); }
+    explicit DBuf(hipStream_t st) : s(st)
+    {
+        // keep up to 256 MB of released scratch in the device's default pool between calls,
+        // so that a call's allocations do not map fresh memory each time (once per device)
+        static bool warmed[64] = {};
+        int dev = 0;
+        if (hipGetDevice(&dev) == hipSuccess && dev >= 0 && dev < 64 && !warmed[dev]) {
+            hipMemPool_t pool;
+            if (hipDeviceGetDefaultMemPool(&pool, dev) == hipSuccess) {
+                uint64_t keep = (uint64_t)256 << 20;
+                (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep);
+            }
+            (void)hipGetLastError();
+            warmed[dev] = true;
+        }
+    }
+    ~DBuf() { for (void *p : all) (void)hipFreeAsync(p, s); }
     template <typename T>
     int alloc(T **p, size_t n)
     {
-        hipError_t e = hipMalloc((void **)p, (n ? n : 1) * sizeof(T));
+        hipError_t e = hipMallocAsync((void **)p, (n ? n : 1) * sizeof(T), s);
         if (e != hipSuccess) {
             (void)hipGetLastError();
             return set_error(MBREC_ERR_HIP, "mbrec:alloc", "allocation of %zu bytes failed", n * sizeof(T));
@@ -276,7 +326,7 @@ int mbrec_dmas_cf_dev(const double *xpix, const double *zpix, int P, const float
     std::vector<double> ang, hx(D), hz(D);
     detector_angles(D, coverage_deg, ang);
     for (int p = 0; p < D; ++p) { hx[p] = std::cos(ang[p]) * radius; hz[p] = std::sin(ang[p]) * radius; }
-    DBuf b;
+    DBuf b(s);
     double *qx, *qz, *sn, *sh;
     unsigned int *mx;
     int *err;
@@ -325,7 +375,7 @@ int mbrec_backproject_waveeq_dev(const double *xv, const double *zv, int Nz, int
     }
     double psum = 0.0;
     for (int i = 0; i < pm * pn; ++i) psum += psf_host[i];
-    DBuf b;
+    DBuf b(s);
     double *qx, *qz, *qc, *qs, *psf, *s1, *s2, *swl, *ux, *uy, *xp, *zp, *dxv, *dzv;
     int *err;
     const size_t n = (size_t)W * D * nT;
@@ -353,7 +403,7 @@ int mbrec_backproject_waveeq_dev(const double *xv, const double *zv, int Nz, int
     MBREC_HIP(hipMemcpyAsync(zp, hzp.data(), sizeof(double) * P, hipMemcpyHostToDevice, s));
     MBREC_HIP(hipMemsetAsync(err, 0, sizeof(int), s));
     const double dt = 1.0 / fs, t1 = (1 + cropped) * dt;
-    hipLaunchKernelGGL(k_cumtrapz, dim3(blocks_for((size_t)W * D)), dim3(256), 0, s, sig, s1, W * D, nT, c, dt,
+    hipLaunchKernelGGL(k_cumtrapz, dim3(W * D), dim3(256), 0, s, sig, s1, W * D, nT, c, dt,
                        cropped);
     hipLaunchKernelGGL(k_conv_same, dim3(blocks_for(n)), dim3(256), 0, s, (const double *)s1, s2, W, D, nT,
                        (const double *)psf, pm, pn, 1.0 / psum);
@@ -385,7 +435,7 @@ int mbrec_bp_kernel_dev(const double *t_host, int Nt, double *K, mbrec_stream st
         const double wtr = (l == 0 || l == Nt - 1) ? 0.5 * dl : dl;   // trapz weights
         wl[l] = wtr * lam[l];
     }
-    DBuf b;
+    DBuf b(s);
     double *t, *dlam, *dwl, *A, *Bm, *M;
     const size_t nn = (size_t)Nt * Nt;
     int rc;
