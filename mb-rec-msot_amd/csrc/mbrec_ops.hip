@@ -255,24 +255,24 @@ __global__ __launch_bounds__(256) void k_filt_z(const float *__restrict__ in, fl
     }
 }
 
-template <int KT>
+template <int KT, int ROWS = kFxRows>
 __global__ __launch_bounds__(256) void k_filt_x(const float *__restrict__ in, float *__restrict__ out,
                                                 const float *__restrict__ k, int K, int Nz, int Nx, int transp)
 {
     extern __shared__ float sm[];
     PfAcc *tk = reinterpret_cast<PfAcc *>(sm);
-    float *xs = sm + 2 * (2 * K + 1 + 1);   // [Nx + 2K (+ kFiltR)][kFxRows]
-    const int iz0 = blockIdx.x * kFxRows, b = blockIdx.y;
+    float *xs = sm + 2 * (2 * K + 1 + 1);   // [Nx + 2K (+ kFiltR)][ROWS]
+    const int iz0 = blockIdx.x * ROWS, b = blockIdx.y;
     const size_t P = (size_t)Nz * Nx;
     const float *src = in + (size_t)b * P;
     for (int i = threadIdx.x; i < 2 * K + 1; i += blockDim.x) tk[i] = (PfAcc)k[i];
-    const int r = threadIdx.x % kFxRows;
+    const int r = threadIdx.x % ROWS;
     const bool rv = iz0 + r < Nz;
-    // element e = i * kFxRows + r: row i of the wrapped x axis, z row iz0 + r
+    // element e = i * ROWS + r: row i of the wrapped x axis, z row iz0 + r
     stage_batched<16, 256>(
-        threadIdx.x, (Nx + 2 * K + (KT ? kFiltR : 0)) * kFxRows,
+        threadIdx.x, (Nx + 2 * K + (KT ? kFiltR : 0)) * ROWS,
         [&](int e) {
-            const int i = e / kFxRows, rr = e % kFxRows;
+            const int i = e / ROWS, rr = e % ROWS;
             if (i >= Nx + 2 * K) return 0.0f;   // window overrun of the last block (outputs not stored)
             int j = i - K;
             j = j < 0 ? j + Nx : (j >= Nx ? j - Nx : j);
@@ -287,11 +287,11 @@ __global__ __launch_bounds__(256) void k_filt_x(const float *__restrict__ in, fl
         PfAcc tr[NT];
 #pragma unroll
         for (int t = 0; t < NT; ++t) tr[t] = tk[t];
-        for (int ix0 = (threadIdx.x / kFxRows) * kFiltR; ix0 < Nx; ix0 += (256 / kFxRows) * kFiltR) {
-            const float *x = xs + ix0 * kFxRows + r;
+        for (int ix0 = (threadIdx.x / ROWS) * kFiltR; ix0 < Nx; ix0 += (256 / ROWS) * kFiltR) {
+            const float *x = xs + ix0 * ROWS + r;
             PfAcc w[NW], acc[kFiltR];
 #pragma unroll
-            for (int i = 0; i < NW; ++i) w[i] = (PfAcc)x[i * kFxRows];
+            for (int i = 0; i < NW; ++i) w[i] = (PfAcc)x[i * ROWS];
 #pragma unroll
             for (int q = 0; q < kFiltR; ++q) {
                 acc[q] = 0;
@@ -311,13 +311,13 @@ __global__ __launch_bounds__(256) void k_filt_x(const float *__restrict__ in, fl
         }
         return;
     }
-    for (int ix = threadIdx.x / kFxRows; ix < Nx; ix += 256 / kFxRows) {
-        const float *x = xs + ix * kFxRows + r;
+    for (int ix = threadIdx.x / ROWS; ix < Nx; ix += 256 / ROWS) {
+        const float *x = xs + ix * ROWS + r;
         PfAcc acc = 0;
         if (transp)
-            for (int t = 0; t <= 2 * K; ++t) acc = fma(tk[t], (PfAcc)x[t * kFxRows], acc);
+            for (int t = 0; t <= 2 * K; ++t) acc = fma(tk[t], (PfAcc)x[t * ROWS], acc);
         else
-            for (int t = 0; t <= 2 * K; ++t) acc = fma(tk[t], (PfAcc)x[(2 * K - t) * kFxRows], acc);
+            for (int t = 0; t <= 2 * K; ++t) acc = fma(tk[t], (PfAcc)x[(2 * K - t) * ROWS], acc);
         dst[(size_t)ix * Nz] = (float)acc;
     }
 }
@@ -545,7 +545,11 @@ __host__ __device__ constexpr int fir16_window(int Lp) { return 16 * fir16_nblk(
 
 // LPT > 0: the padded tap count Lp known at compile time (the tap loop unrolls fully, the
 // register window needs no moves and the LDS offsets are immediates); LPT = 0: any L.
-template <int LPT>
+// SPLIT: for few rows (one sinogram: 256 tiles for the whole filter) both halves of the
+// workgroup take the same tile, half 0 the first ceil(Lp / 32) groups of 16 taps and half
+// 1 the rest, and the two FP64 partial sums are added at the end (one tile per workgroup,
+// twice the workgroups; the sum of each half in tap order, then the two halves).
+template <int LPT, bool SPLIT = false>
 __global__ __launch_bounds__(256) void k_fir16(const float *__restrict__ x, float *__restrict__ out,
                                                const double *__restrict__ taps64, int L, int per_det, int off, int D,
                                                int nT, int rows, float scale, const int2 *__restrict__ win)
@@ -553,9 +557,9 @@ __global__ __launch_bounds__(256) void k_fir16(const float *__restrict__ x, floa
     extern __shared__ float sm[];
     const int Lp = LPT ? LPT : fir16_padded_taps(L), nx = fir16_window(Lp), NBLK = fir16_nblk(Lp);
     const int hf = __builtin_amdgcn_readfirstlane(threadIdx.x >> 7), lt = threadIdx.x & 127;
-    float *xs = sm + (size_t)hf * nx;
+    float *xs = SPLIT ? sm : sm + (size_t)hf * nx;
     const int ntile = (nT + kFirSpan - 1) / kFirSpan;
-    const int tile = blockIdx.x * 2 + hf;
+    const int tile = SPLIT ? (int)blockIdx.x : (int)blockIdx.x * 2 + hf;
     const bool live = tile < rows * ntile;
     const int row = live ? tile / ntile : 0;
     const int t0 = live ? (tile - row * ntile) * kFirSpan : 0;
@@ -568,13 +572,15 @@ __global__ __launch_bounds__(256) void k_fir16(const float *__restrict__ x, floa
     // xs[m] = x[xbase + m], xbase = t0 + off - Lp - 15
     const int xbase = t0 + off - Lp - 15;
     const float *xr = x + (size_t)row * nT;
-    stage_batched<8, 128>(
-        lt, nx,
-        [&](int i) {
-            const int t = xbase + i;
-            return (live && t >= lo && t < hi) ? xr[t] : 0.0f;
-        },
-        [&](int i, float v) { xs[(i & 15) * NBLK + (i >> 4)] = v; });
+    auto ldx = [&](int i) {
+        const int t = xbase + i;
+        return (live && t >= lo && t < hi) ? xr[t] : 0.0f;
+    };
+    auto stx = [&](int i, float v) { xs[(i & 15) * NBLK + (i >> 4)] = v; };
+    if constexpr (SPLIT)
+        stage_batched<8, 256>((int)threadIdx.x, nx, ldx, stx);
+    else
+        stage_batched<8, 128>(lt, nx, ldx, stx);
     // the row's taps, FP64 and zero-padded to Lp, read as scalars (uniform per wave)
     const double *tk = taps64 + (per_det ? (size_t)(row % D) * Lp : 0);
     __syncthreads();
@@ -584,11 +590,16 @@ __global__ __launch_bounds__(256) void k_fir16(const float *__restrict__ x, floa
     // this one moved up by 8 with 8 new elements below (lanes 0..7 of a block after a
     // group at an even multiple of 8, lanes 8..15 of the block before after an odd one).
     const int blk = lt + Lp / 16;
+    // tap groups [i0a, i0b) of this thread (all of them unless SPLIT); the register window
+    // at the start of the group at i0 is W[w] = element (w + 8, blk - i0 / 16) for w < 8 and
+    // (w - 8, blk - i0 / 16 + 1) for w >= 8 (the state the shifts below leave)
+    const int Sp = 16 * ((Lp / 16 + 1) / 2);
+    const int i0a = SPLIT && hf ? Sp : 0;
     double W[23], acc[kFirOut];
 #pragma unroll
-    for (int w = 0; w < 8; ++w) W[w] = (double)xs[(w + 8) * NBLK + blk];
+    for (int w = 0; w < 8; ++w) W[w] = (double)xs[(w + 8) * NBLK + blk - i0a / 16];
 #pragma unroll
-    for (int w = 8; w < 23; ++w) W[w] = (double)xs[(w - 8) * NBLK + blk + 1];
+    for (int w = 8; w < 23; ++w) W[w] = (double)xs[(w - 8) * NBLK + blk - i0a / 16 + 1];
 #pragma unroll
     for (int k = 0; k < kFirOut; ++k) acc[k] = 0.0;
     auto group = [&](int g) {
@@ -611,13 +622,44 @@ __global__ __launch_bounds__(256) void k_fir16(const float *__restrict__ x, floa
         group(i0 + 8);
         if (i0 + 16 < Lp) shift(blk - i0 / 16 - 1, 8);
     };
-    if constexpr (LPT > 0) {
+    if constexpr (SPLIT && LPT > 0) {
+        constexpr int S = 16 * ((LPT / 16 + 1) / 2);
+        if (hf == 0) {
+#pragma unroll
+            for (int i0 = 0; i0 < S; i0 += 16) step(i0);
+        } else {
+#pragma unroll
+            for (int i0 = S; i0 < LPT; i0 += 16) step(i0);
+        }
+    } else if constexpr (SPLIT) {
+        for (int i0 = i0a; i0 < (hf ? Lp : Sp); i0 += 16) step(i0);
+    } else if constexpr (LPT > 0) {
 #pragma unroll
         for (int i0 = 0; i0 < LPT; i0 += 16) step(i0);
     } else {
         for (int i0 = 0; i0 < Lp; i0 += 16) step(i0);   // Lp is a multiple of 16
     }
     __syncthreads();   // xs is reused for the outputs (same layout)
+    if constexpr (SPLIT) {   // half 1's partial sums through LDS (past xs) into half 0's
+        double *ex = reinterpret_cast<double *>(sm + nx);
+        if (hf) {
+#pragma unroll
+            for (int k = 0; k < kFirOut; ++k) ex[k * 128 + lt] = acc[k];
+        }
+        __syncthreads();
+        if (!hf) {
+#pragma unroll
+            for (int k = 0; k < kFirOut; ++k) {
+                const int t = t0 + 16 * lt + k;
+                xs[k * NBLK + lt] = (t >= lo && t < hi) ? scale * (float)(acc[k] + ex[k * 128 + lt]) : 0.0f;
+            }
+        }
+        __syncthreads();
+        if (!live) return;
+        float *orow = out + (size_t)row * nT;
+        for (int i = threadIdx.x; i < kFirSpan && t0 + i < nT; i += 256) orow[t0 + i] = xs[(i & 15) * NBLK + (i >> 4)];
+        return;
+    }
 #pragma unroll
     for (int k = 0; k < kFirOut; ++k) {
         const int t = t0 + 16 * lt + k;
@@ -3567,6 +3609,19 @@ static int launch_fir(Plan *p, const float *x, float *out, int B, bool corr, flo
     if (lds16 <= kLdsBudget && p->d_ir64) {
         const int rows = B * p->D, ntile = (p->nT + kFirSpan - 1) / kFirSpan;
         const size_t nblk = ((size_t)rows * ntile + 1) / 2;
+        // few tiles (fewer than two per CU): one tile per workgroup, the taps split between its halves
+        const size_t ldsp = sizeof(float) * (size_t)fir16_window(Lp) + sizeof(double) * 128 * kFirOut;
+        if ((size_t)rows * ntile < 512 && ldsp <= kLdsBudget) {
+            const dim3 gp((unsigned)((size_t)rows * ntile));
+            if (Lp == 208)
+                hipLaunchKernelGGL((k_fir16<208, true>), gp, dim3(256), ldsp, s, x, out, corr ? p->d_irf64 : p->d_ir64,
+                                   L, p->ir_per_det, off, p->D, p->nT, rows, scale, win);
+            else
+                hipLaunchKernelGGL((k_fir16<0, true>), gp, dim3(256), ldsp, s, x, out, corr ? p->d_irf64 : p->d_ir64,
+                                   L, p->ir_per_det, off, p->D, p->nT, rows, scale, win);
+            MBREC_HIP(hipGetLastError());
+            return MBREC_OK;
+        }
         if (Lp == 208)   // the 201-tap IR of the default IR design
             hipLaunchKernelGGL(k_fir16<208>, dim3((unsigned)nblk), dim3(256), lds16, s, x, out,
                                corr ? p->d_irf64 : p->d_ir64, L, p->ir_per_det, off, p->D, p->nT, rows, scale, win);
@@ -4007,6 +4062,11 @@ static void prefilter_passes(Plan *p, const float *in, float *out, float *tmp, i
     const size_t ldz0 = tapb + sizeof(float) * (size_t)kFzCols * ((p->Nz + 2 * p->Kz) | 1);
     const size_t ldx = tapb + sizeof(float) * (size_t)kFxRows * (p->Nx + 2 * p->Kx + (KX ? kFiltR : 0));
     const dim3 gx((p->Nz + kFxRows - 1) / kFxRows, B);
+    // one sinogram: 4 z rows per workgroup (100 workgroups at 400^2 instead of 25)
+    constexpr int R1 = 4;
+    const dim3 gx1((p->Nz + R1 - 1) / R1, 1);
+    const size_t ldx1 = tapb + sizeof(float) * (size_t)R1 * (p->Nx + 2 * p->Kx + (KX ? kFiltR : 0));
+    const bool onex = B == 1 && KX > 0;
     const int tr = transp ? 1 : 0;
     const dim3 gzp(((size_t)B * p->Nx + kFzCols - 1) / kFzCols);
     // one sinogram: 4 columns per workgroup (64 threads each), so that a 400-column image
@@ -4016,7 +4076,10 @@ static void prefilter_passes(Plan *p, const float *in, float *out, float *tmp, i
     const size_t ldz1 = tapb + sizeof(float) * (size_t)C1 * ((p->Nz + 2 * p->Kz + (KZ ? kFiltR : 0)) | 1);
     const bool one = B == 1 && KZ > 0;
     if (!transp) {   // forward: x pass then z pass (the z pass writes the interleaved layout)
-        hipLaunchKernelGGL((k_filt_x<KX>), gx, dim3(256), ldx, s, in, tmp, p->d_kx, p->Kx, p->Nz, p->Nx, 0);
+        if (onex)
+            hipLaunchKernelGGL((k_filt_x<KX, R1>), gx1, dim3(256), ldx1, s, in, tmp, p->d_kx, p->Kx, p->Nz, p->Nx, 0);
+        else
+            hipLaunchKernelGGL((k_filt_x<KX>), gx, dim3(256), ldx, s, in, tmp, p->d_kx, p->Kx, p->Nz, p->Nx, 0);
         // interleaved output [P][Bp]: for one sinogram it is the planar layout, which the
         // register-blocked planar pass writes with every thread busy (the interleaved pass
         // gives one of its 16 column lanes work there); for batches the interleaved pass
@@ -4037,8 +4100,12 @@ static void prefilter_passes(Plan *p, const float *in, float *out, float *tmp, i
         else
             hipLaunchKernelGGL((k_filt_z<false, KZ>), gzp, dim3(256), ldz, s, in, tmp, p->d_kz, p->Kz, p->Nz, p->Nx,
                                B, Bp, tr);
-        hipLaunchKernelGGL((k_filt_x<KX>), gx, dim3(256), ldx, s, (const float *)tmp, out, p->d_kx, p->Kx, p->Nz,
-                           p->Nx, tr);
+        if (onex)
+            hipLaunchKernelGGL((k_filt_x<KX, R1>), gx1, dim3(256), ldx1, s, (const float *)tmp, out, p->d_kx, p->Kx,
+                               p->Nz, p->Nx, tr);
+        else
+            hipLaunchKernelGGL((k_filt_x<KX>), gx, dim3(256), ldx, s, (const float *)tmp, out, p->d_kx, p->Kx, p->Nz,
+                               p->Nx, tr);
     }
 }
 
