@@ -425,7 +425,9 @@ constexpr int kYTile = 128;
 
 // DIRECT: read the bins [D][NB][Bp] themselves (pull forward) instead of the
 // overlapping bucket windows
-template <bool DIRECT>
+// NW > 0: taps per phase known at compile time (the taps in registers instead of indexed in
+// the kernel arguments at run time; the same sums in the same order)
+template <bool DIRECT, int NW = 0>
 __global__ __launch_bounds__(256) void k_bins_to_y(
     const float *__restrict__ hpart, float *__restrict__ y, HatTaps taps, BinGeom g, int Wb, int nbk,
     int lo_off, int WL, int Bp, int D, int B, int Bc, float scale, const int2 *__restrict__ win)
@@ -457,13 +459,29 @@ __global__ __launch_bounds__(256) void k_bins_to_y(
     }
     __syncthreads();
     const int nr = t1 - t0;
+    float wr[3 * (NW ? NW : 1)];
+    if constexpr (NW > 0) {
+#pragma unroll
+        for (int m = 0; m < 3; ++m)
+#pragma unroll
+            for (int q = 0; q < NW; ++q) wr[m * NW + q] = taps.w[m][q];
+    }
     for (int i = threadIdx.x; i < nr * nb; i += blockDim.x) {
         const int b = i / nr, r = t0 + (i - b * nr);
         const float *bs = sm + b * njp;
         float acc = 0.0f;
-        for (int m = 0; m < 3; ++m) {
-            const int jb = 3 * (r - taps.kmin + g.K) + m - jlo;
-            for (int q = 0; q < taps.nw; ++q) acc = fmaf(taps.w[m][q], bs[jb - 3 * q], acc);
+        if constexpr (NW > 0) {
+#pragma unroll
+            for (int m = 0; m < 3; ++m) {
+                const int jb = 3 * (r - taps.kmin + g.K) + m - jlo;
+#pragma unroll
+                for (int q = 0; q < NW; ++q) acc = fmaf(wr[m * NW + q], bs[jb - 3 * q], acc);
+            }
+        } else {
+            for (int m = 0; m < 3; ++m) {
+                const int jb = 3 * (r - taps.kmin + g.K) + m - jlo;
+                for (int q = 0; q < taps.nw; ++q) acc = fmaf(taps.w[m][q], bs[jb - 3 * q], acc);
+            }
         }
         if (win) {
             const int2 wb = win[b0 + b];
@@ -3581,10 +3599,27 @@ static int launch_bins_to_y(Plan *p, float *y, int B, int Bp, float scale, hipSt
     // LDS chunk of sinograms: at most kHatLds bytes, so several workgroups share a CU
     const int Bc = std::max(1, std::min(B, (int)(kHatLds / (sizeof(float) * (size_t)(nj | 1)))));
     dim3 grid((p->nT + kYTile - 1) / kYTile, p->D, (B + Bc - 1) / Bc);
-    if (forward_writes_bins(p, Bp))
-        hipLaunchKernelGGL(k_bins_to_y<true>, grid, dim3(256), sizeof(float) * (size_t)Bc * (nj | 1), s, p->d_bins, y,
-                           p->taps, p->geom, p->Wb, p->nbk, p->lo_off, p->WL, Bp, p->D, B, Bc, scale, win);
-    else
+    const size_t lds = sizeof(float) * (size_t)Bc * (nj | 1);
+    if (forward_writes_bins(p, Bp)) {
+        switch (p->taps.nw) {   // 2 floor(R) + 3 taps per phase
+        case 5:
+            hipLaunchKernelGGL((k_bins_to_y<true, 5>), grid, dim3(256), lds, s, p->d_bins, y, p->taps, p->geom, p->Wb,
+                               p->nbk, p->lo_off, p->WL, Bp, p->D, B, Bc, scale, win);
+            break;
+        case 7:
+            hipLaunchKernelGGL((k_bins_to_y<true, 7>), grid, dim3(256), lds, s, p->d_bins, y, p->taps, p->geom, p->Wb,
+                               p->nbk, p->lo_off, p->WL, Bp, p->D, B, Bc, scale, win);
+            break;
+        case 9:
+            hipLaunchKernelGGL((k_bins_to_y<true, 9>), grid, dim3(256), lds, s, p->d_bins, y, p->taps, p->geom, p->Wb,
+                               p->nbk, p->lo_off, p->WL, Bp, p->D, B, Bc, scale, win);
+            break;
+        default:
+            hipLaunchKernelGGL((k_bins_to_y<true>), grid, dim3(256), lds, s, p->d_bins, y, p->taps, p->geom, p->Wb,
+                               p->nbk, p->lo_off, p->WL, Bp, p->D, B, Bc, scale, win);
+            break;
+        }
+    } else
         hipLaunchKernelGGL(k_bins_to_y<false>, grid, dim3(256), sizeof(float) * (size_t)Bc * (nj | 1), s, p->d_hpart,
                            y, p->taps, p->geom, p->Wb, p->nbk, p->lo_off, p->WL, Bp, p->D, B, Bc, scale, win);
     MBREC_HIP(hipGetLastError());
