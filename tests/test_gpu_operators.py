@@ -175,7 +175,7 @@ def test_one_sinogram_forward_larger_image():
 
 
 @pytest.mark.parametrize("sir", [False, True])
-@pytest.mark.parametrize("B", [1, 8, 20, 64])
+@pytest.mark.parametrize("B", [1, 8, 20, 32, 64])
 def test_every_launch_variant_matches_oracle(sir, B):
     """Each gather variant and scatter choice valid for B gives the oracle's result."""
     from mbrec import _lib
