@@ -12,6 +12,7 @@
 
 #include <rocblas/rocblas.h>
 
+#include <atomic>
 #include <cmath>
 #include <mutex>
 #include <vector>
@@ -282,16 +283,16 @@ struct DBuf {
     {
         // keep up to 256 MB of released scratch in the device's default pool between calls,
         // so that a call's allocations do not map fresh memory each time (once per device)
-        static bool warmed[64] = {};
+        static std::atomic<unsigned long long> warmed{0};
         int dev = 0;
-        if (hipGetDevice(&dev) == hipSuccess && dev >= 0 && dev < 64 && !warmed[dev]) {
+        if (hipGetDevice(&dev) == hipSuccess && dev >= 0 && dev < 64 &&
+            !(warmed.fetch_or(1ull << dev) & (1ull << dev))) {
             hipMemPool_t pool;
             if (hipDeviceGetDefaultMemPool(&pool, dev) == hipSuccess) {
                 uint64_t keep = (uint64_t)256 << 20;
                 (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep);
             }
             (void)hipGetLastError();
-            warmed[dev] = true;
         }
     }
     ~DBuf() { for (void *p : all) (void)hipFreeAsync(p, s); }
