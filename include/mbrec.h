@@ -408,6 +408,11 @@ int mbrec_unet_forward(mbrec_unet net, const float *in, float *out, int B, int H
 /* loss = mean((net(in/s) - target/s)^2); gradients of it (host FP32, may be NULL). */
 int mbrec_unet_gradients(mbrec_unet net, const float *in, const float *target, int B, int H, int W,
                          float *grad_host, double *loss_host, mbrec_stream s);
+/* on = 1: the loss (of mbrec_unet_gradients and mbrec_unet_train_step) weights item b by
+ * w_b = mean_b' P_b' / P_b, P_b the mean of (target_b / s_b)^2, so that every item counts by
+ * its relative error; on = 0 (the default): the plain mean.  No counterpart in the reference
+ * (the warm start is new). */
+int mbrec_unet_set_relative_loss(mbrec_unet net, int on);
 /* One Adam step on that loss; loss_host (may be NULL) receives the loss before the step. */
 int mbrec_unet_train_step(mbrec_unet net, const float *in, const float *target, int B, int H, int W, float lr,
                           float beta1, float beta2, float eps, double *loss_host, mbrec_stream s);

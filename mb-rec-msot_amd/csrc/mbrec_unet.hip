@@ -318,14 +318,46 @@ __global__ void k_head(const uint16_t *__restrict__ a, const float *__restrict__
     if (out) out[p] = acc * s[p / HW];
 }
 
-// loss (mean squared error of the normalised output) and head backward:
-// d = on - target/s;  loss += d^2;  g_on = 2 d / NP;  ga[p][ci] = g_on w[ci] (a > 0);
-// dw[ci] += g_on a[p][ci];  db += g_on
+// relative loss: pw[b] = mean over item b of (target / s_b)^2, then the weights
+// wt[b] = mean_b' pw[b'] / pw[b] (1 for an item without target power)
+__global__ __launch_bounds__(1024) void k_target_power(const float *__restrict__ target, const float *__restrict__ s,
+                                                      float *__restrict__ pw, int HW)
+{
+    const float *src = target + (size_t)blockIdx.x * HW;
+    const double sb = s[blockIdx.x];
+    double acc = 0.0;
+    for (int i = threadIdx.x; i < HW; i += blockDim.x) {
+        const double v = src[i] / sb;
+        acc += v * v;
+    }
+    __shared__ double red[1024];
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int k = blockDim.x / 2; k > 0; k >>= 1) {
+        if ((int)threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) pw[blockIdx.x] = (float)(red[0] / HW);
+}
+
+__global__ void k_loss_weights(float *__restrict__ wt, int B)
+{
+    if (threadIdx.x != 0) return;
+    double m = 0.0;
+    for (int b = 0; b < B; ++b) m += wt[b];
+    m /= B;
+    for (int b = 0; b < B; ++b) wt[b] = wt[b] > 0.0f ? (float)(m / wt[b]) : 1.0f;
+}
+
+// loss (mean squared error of the normalised output, item b weighted by wt[b] if wt) and
+// head backward: d = on - target/s;  loss += wt d^2;  g_on = 2 wt d / NP;
+// ga[p][ci] = g_on w[ci] (a > 0);  dw[ci] += g_on a[p][ci];  db += g_on
 __global__ __launch_bounds__(256) void k_head_loss_bwd(const uint16_t *__restrict__ a, const float *__restrict__ w,
                                                        const float *__restrict__ on, const float *__restrict__ target,
-                                                       const float *__restrict__ s, uint16_t *__restrict__ ga,
-                                                       float *__restrict__ dw, float *__restrict__ db,
-                                                       double *__restrict__ loss, int HW, int NP, int C)
+                                                       const float *__restrict__ s, const float *__restrict__ wt,
+                                                       uint16_t *__restrict__ ga, float *__restrict__ dw,
+                                                       float *__restrict__ db, double *__restrict__ loss, int HW, int NP,
+                                                       int C)
 {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     __shared__ float red[65][256];
@@ -334,8 +366,9 @@ __global__ __launch_bounds__(256) void k_head_loss_bwd(const uint16_t *__restric
     float gon = 0.0f, l2 = 0.0f;
     if (p < NP) {
         const float d = on[p] - target[p] / s[p / HW];
-        l2 = d * d;
-        gon = 2.0f * d / (float)NP;
+        const float wi = wt ? wt[p / HW] : 1.0f;
+        l2 = wi * d * d;
+        gon = 2.0f * wi * d / (float)NP;
         for (int c0 = 0; c0 < C; c0 += 8) {
             const u16x8 v = *reinterpret_cast<const u16x8 *>(a + (size_t)p * C + c0);
             u16x8 o;
@@ -521,6 +554,8 @@ struct UNet {
     bool grads = false;
     std::vector<void *> bufs;
     float *s = nullptr, *in_n = nullptr, *on = nullptr;
+    float *lw = nullptr;   // per-item loss weights (relative loss)
+    int rel_loss = 0;      // 1: every item's squared error divided by its own normalised target power
     double *loss = nullptr;
     Feat a1, cat1, d1a, d1b, p1, a2, cat2, d2a, d2b, p2, a3, cat3, d3a, d3b, p3, a4, a4b;
     Feat g_a1, g_cat1, g_d1a, g_d1b, g_p1, g_a2, g_cat2, g_d2a, g_d2b, g_p2, g_a3, g_cat3, g_d3a, g_d3b, g_p3, g_a4,
@@ -561,7 +596,8 @@ static int ensure_ws(UNet *u, int B, int H, int W, bool grads)
     const int c = u->c;
     const size_t P1 = (size_t)B * H * W, P2 = P1 / 4, P3 = P1 / 16, P4 = P1 / 64;
     int rc;
-    if ((rc = walloc(u, (void **)&u->s, sizeof(float) * B)) || (rc = walloc(u, (void **)&u->in_n, sizeof(float) * P1)) ||
+    if ((rc = walloc(u, (void **)&u->s, sizeof(float) * B)) || (rc = walloc(u, (void **)&u->lw, sizeof(float) * B)) ||
+        (rc = walloc(u, (void **)&u->in_n, sizeof(float) * P1)) ||
         (rc = walloc(u, (void **)&u->on, sizeof(float) * P1)) || (rc = walloc(u, (void **)&u->loss, sizeof(double))) ||
         (rc = feat(u, u->a1, P1, c)) || (rc = feat(u, u->cat1, P1, 3 * c)) || (rc = feat(u, u->d1a, P1, c)) ||
         (rc = feat(u, u->d1b, P1, c)) || (rc = feat(u, u->p1, P2, c)) || (rc = feat(u, u->a2, P2, 2 * c)) ||
@@ -692,8 +728,13 @@ static int backward(UNet *u, const float *target, int B, int H, int W, hipStream
     const int H2 = H / 2, W2 = W / 2, H3 = H / 4, W3 = W / 4, H4 = H / 8, W4 = W / 8;
     hipLaunchKernelGGL(k_fill, dim3(nb(u->nparam)), dim3(256), 0, s, u->gr, 0.0f, u->nparam);
     hipLaunchKernelGGL(k_fill, dim3(1), dim3(64), 0, s, (float *)u->loss, 0.0f, (size_t)2);
+    if (u->rel_loss) {
+        hipLaunchKernelGGL(k_target_power, dim3(B), dim3(1024), 0, s, target, (const float *)u->s, u->lw, HW);
+        hipLaunchKernelGGL(k_loss_weights, dim3(1), dim3(64), 0, s, u->lw, B);
+    }
     hipLaunchKernelGGL(k_head_loss_bwd, dim3(nb(P1)), dim3(256), 0, s, u->d1b.p, u->th + u->L[HEAD].w_off, u->on,
-                       target, u->s, u->g_d1b.p, GW(u, HEAD), GB(u, HEAD), u->loss, HW, (int)P1, c);
+                       target, u->s, u->rel_loss ? (const float *)u->lw : nullptr, u->g_d1b.p, GW(u, HEAD),
+                       GB(u, HEAD), u->loss, HW, (int)P1, c);
     // level 1 decoder
     wgrad3(u->g_d1b, 0, u->d1a, 0, c, c, GW(u, D1B), GB(u, D1B), B, H, W, s);
     conv3(u->g_d1b, 0, c, WT(u, D1B), nullptr, u->g_d1a, 0, c, B, H, W, 1, &u->d1a, 0, s);
@@ -913,6 +954,13 @@ int mbrec_unet_gradients(mbrec_unet h, const float *in, const float *target, int
         MBREC_HIP(hipMemcpy(&l, u->loss, sizeof(double), hipMemcpyDeviceToHost));
         *loss_host = l / ((double)B * H * W);
     }
+    return MBREC_OK;
+}
+
+int mbrec_unet_set_relative_loss(mbrec_unet h, int on)
+{
+    if (!h) return set_error(MBREC_ERR_ARG, "mbrec:args", "unet_set_relative_loss: null argument");
+    h->u.rel_loss = on ? 1 : 0;
     return MBREC_OK;
 }
 

@@ -116,6 +116,7 @@ SIGNATURES = [
     ("mbrec_unet_forward", C.c_int, [c_void_p, c_void_p, c_void_p, C.c_int, C.c_int, C.c_int, c_void_p]),
     ("mbrec_unet_gradients", C.c_int,
      [c_void_p, c_void_p, c_void_p, C.c_int, C.c_int, C.c_int, c_void_p, c_double_p, c_void_p]),
+    ("mbrec_unet_set_relative_loss", C.c_int, [c_void_p, C.c_int]),
     ("mbrec_unet_train_step", C.c_int,
      [c_void_p, c_void_p, c_void_p, C.c_int, C.c_int, C.c_int, C.c_float, C.c_float, C.c_float, C.c_float,
       c_double_p, c_void_p]),

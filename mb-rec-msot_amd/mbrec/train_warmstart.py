@@ -75,6 +75,8 @@ def main(argv=None):
     ap.add_argument("--chunk", type=int, default=25)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--init", default=None, help="start from these weights")
+    ap.add_argument("--relative-loss", action="store_true",
+                    help="weight each item by the inverse of its normalised target power")
     ap.add_argument("--report", default=None)
     a = ap.parse_args(argv)
     import torch
@@ -89,6 +91,8 @@ def main(argv=None):
     t_data = time.perf_counter() - t0
     log(f"training set: {a.train} stationary states in {t_data:.1f} s")
     net = UNet.load(a.init) if a.init else UNet(a.base, seed=a.seed)
+    if a.relative_loss:
+        net.set_relative_loss(True)
     t0 = time.perf_counter()
     hist = train(net, xin, xt, epochs=a.epochs, batch=a.batch, lr=a.lr, lr_final=a.lr_final, flip=True,
                  seed=a.seed, log=log)
@@ -119,7 +123,8 @@ def main(argv=None):
     log(f"cold: {k_cold} outer iterations to {a.tol:g}")
     k_warm, errs_warm = iterations_to_tolerance(model, y, lam, ref, a.tol, w0, v0, chunk=a.chunk)
     log(f"learned: {k_warm} outer iterations to {a.tol:g}")
-    rep = {"train": a.train, "epochs": a.epochs, "base_channels": a.base, "loss": hist, "t_data_s": t_data,
+    rep = {"train": a.train, "epochs": a.epochs, "base_channels": a.base, "relative_loss": a.relative_loss,
+           "loss": hist, "t_data_s": t_data,
            "t_train_s": t_train, "tol": a.tol, "chunk": a.chunk, "ref_kkt": kkt,
            "state_err_median": float(e_state.median()), "state_err_max": float(e_state.max()),
            "image_err_median": float(e_img.median()), "used": int(used.sum()),

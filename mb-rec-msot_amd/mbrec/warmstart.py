@@ -115,6 +115,12 @@ class UNet:
                                                    C.byref(loss), _lib.stream_handle()))
         return float(loss.value), grad
 
+    def set_relative_loss(self, on=True):
+        """Weight every item's squared error by the inverse of its normalised target power
+        (mean over the batch of the powers / its own), so that items count by their relative
+        error; off: the plain mean."""
+        _lib.check(_lib.lib().mbrec_unet_set_relative_loss(self.h, 1 if on else 0))
+
     def train_step(self, x, target, lr=1e-3, beta1=0.9, beta2=0.999, eps=1e-8, want_loss=True):
         t, g = self._img(x), self._img(target)
         B, H, W = t.shape

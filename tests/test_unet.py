@@ -190,6 +190,40 @@ def test_unet_gradients_match_oracle(require_gpu):
 
 
 @pytest.mark.gpu
+def test_unet_relative_loss_weights_items(require_gpu):
+    """Relative loss of a two-item batch = (w0 L0 + w1 L1) / 2 with w_b = mean(P) / P_b,
+    P_b = mean((t_b / s_b)^2), s_b = rms(x_b); each L_b and its gradient from the plain
+    loss of that item alone."""
+    from mbrec.warmstart import UNet
+    print("unet relative loss", flush=True)
+    c, th, x, t = _net_case()
+    t = t.copy()
+    t[1] *= 3.0                     # item 1: nine times the normalised target power
+    net = UNet(c)
+    net.set_params(th)
+    per = [net.gradients(x[b:b + 1], t[b:b + 1]) for b in range(2)]
+    plain_loss, plain_grad = net.gradients(x, t)
+    assert abs(plain_loss - 0.5 * (per[0][0] + per[1][0])) < 1e-4 * plain_loss
+    xd, td = x.astype(np.float64), t.astype(np.float64)
+    s = np.sqrt((xd ** 2).mean(axis=(1, 2)))
+    P = ((td / s[:, None, None]) ** 2).mean(axis=(1, 2))
+    w = P.mean() / P
+    assert w[0] > 4 * w[1]
+    net.set_relative_loss(True)
+    loss, grad = net.gradients(x, t)
+    want_loss = 0.5 * (w[0] * per[0][0] + w[1] * per[1][0])
+    want_grad = 0.5 * (w[0] * per[0][1].astype(np.float64) + w[1] * per[1][1].astype(np.float64))
+    assert abs(loss - want_loss) < 1e-3 * want_loss
+    assert rel_l2(grad, want_grad) < 1e-2, rel_l2(grad, want_grad)
+    assert rel_l2(plain_grad, want_grad) > 0.1
+    net.set_relative_loss(False)
+    loss2, grad2 = net.gradients(x, t)
+    # weight gradients are summed with FP32 atomics: equal up to the order of the additions
+    assert abs(loss2 - plain_loss) < 1e-6 * plain_loss
+    assert rel_l2(grad2, plain_grad) < 1e-5
+
+
+@pytest.mark.gpu
 def test_unet_adam_steps_reduce_loss(require_gpu):
     from mbrec.warmstart import UNet
     print("unet training steps", flush=True)
