@@ -654,6 +654,8 @@ def metric_bench(a):
     ms_learned, rec_l = events_ms(learned_step, a.steps, a.warmup)
     wall = (time.perf_counter() - t_wall) * 1e3
     ms_plain, rec_p = events_ms(plain_step, 1, 0)
+    # the estimate alone (A'y, network, fallback test), part of every learned step above
+    ms_net, _ = events_ms(lambda: warm_start_estimate(net, model, y, lam), 3, 1)
     e_learned, e_plain = worst(rec_l), worst(rec_p)
     value = B / (ms_learned / 1e3)
     plain_value = B / (ms_plain / 1e3)
@@ -674,7 +676,8 @@ def metric_bench(a):
                        "items_from_network": int(used.sum()),
                        "items_kept_by_arbiter": int(arbitrated["used"].sum()), "arbiter_iterations": 10,
                        "ms_per_batch": round(ms_learned, 2),
-                       "wall_ms_per_batch": round(wall / max(a.steps + a.warmup, 1), 2)},
+                       "wall_ms_per_batch": round(wall / max(a.steps + a.warmup, 1), 2),
+                       "estimate_ms": round(ms_net, 2)},
            "plain": {"recon_per_s": round(plain_value, 4), "outer_iterations": k_plain, "worst_rel_err": e_plain,
                      "ms_per_batch": round(ms_plain, 2)},
            "speedup_learned_vs_plain": round(value / plain_value, 3),

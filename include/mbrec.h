@@ -116,10 +116,11 @@ int mbrec_plan_tau_frac_bits(mbrec_plan plan);
 
 /* Autotuner: times every valid launch variant of the transpose gather (sinograms
  * per pass x detector unroll) and of the forward scatter (LDS batch chunk) for a
- * batch of B on this plan (reps launches each, HIP events on stream s) and keeps
- * the fastest for later calls with the same batch stride.  choice (may be
- * NULL) receives {gather variant, scatter chunk}.  Results do not depend on the
- * choice beyond FP32 summation order. */
+ * batch of B on this plan (reps launches each, HIP events on stream s, after >= 20 ms
+ * of untimed work so that the GPU's clocks have ramped up), times the three fastest
+ * of each again in two alternating rounds, and keeps the fastest for later calls with
+ * the same batch stride.  choice (may be NULL) receives {gather variant, scatter
+ * chunk}.  Results do not depend on the choice beyond FP32 summation order. */
 int mbrec_plan_autotune(mbrec_plan plan, int B, int reps, mbrec_stream s, int *choice);
 /* Set the launch choice for batches of B by hand: gather variant in
  * [0, mbrec_num_gather_variants()) or -1 (automatic); forward choice 0:

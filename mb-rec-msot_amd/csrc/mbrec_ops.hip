@@ -129,16 +129,20 @@ __global__ __launch_bounds__(256) void k_filt_axis(
 
 // ---------------------------------------------------------------------------
 // separable circular prefilter, LDS-staged passes (FP64 sums as above).
-// z pass: 16 columns of Nz samples per workgroup, each staged once with K
-// wrapped samples on either side.  Planar output [B][P]; interleaved output
-// [P][Bp] (IL) takes the 16 columns as 16 consecutive sinograms of one x
-// column, so its stores are 64-byte runs (columns b >= B are written as zero).
-// x pass: one sinogram's kFxRows consecutive z rows over all Nx columns per
+// z pass: COLS columns of Nz samples per workgroup, each staged once with K
+// wrapped samples on either side.  Planar output [B][P] (float4 stores where
+// aligned); interleaved output [P][Bp] (IL) takes the columns as consecutive
+// sinograms of one x column, so its stores are 64-byte runs for 16 columns
+// (columns b >= B are written as zero).
+// x pass: one sinogram's ROWS consecutive z rows over all Nx columns per
 // workgroup, planar in and out.
 // forward: out[i] = sum_a k[a] in[(i - a) mod N];  transpose: in[(i + a) mod N].
 // ---------------------------------------------------------------------------
 constexpr int kFzCols = 16;
 constexpr int kFxRows = 16;
+// batches: 16 columns per z-pass workgroup (planar and interleaved output), 32 z rows per
+// x-pass workgroup (128-byte runs per x; 35 us against 39 us for 16 rows at 64 x 400^2)
+constexpr int kFzColsB = 16, kFzColsIL = 16, kFxRowsB = 32;
 
 constexpr int kFiltR = 8;   // outputs per thread of the register-blocked prefilter passes
 // prefilter sums: FP32 (the taps decay geometrically from a dominant centre tap, so an
@@ -152,7 +156,7 @@ using PfAcc = float;
 template <bool IL, int KT, int COLS = kFzCols>
 __global__ __launch_bounds__(256) void k_filt_z(const float *__restrict__ in, float *__restrict__ out,
                                                 const float *__restrict__ k, int K, int Nz, int Nx, int B, int Bp,
-                                                int transp)
+                                                int transp, int vst = 0)
 {
     extern __shared__ float sm[];
     const int S = (Nz + 2 * K + (KT ? kFiltR : 0)) | 1;   // column stride in LDS
@@ -170,23 +174,34 @@ __global__ __launch_bounds__(256) void k_filt_z(const float *__restrict__ in, fl
     }
     for (int i = threadIdx.x; i < 2 * K + 1; i += blockDim.x) tk[i] = (PfAcc)k[i];
     const int W = Nz + 2 * K + (KT ? kFiltR : 0);
-    const float invW = 1.0f / (float)W;
-    stage_batched<16, 256>(
-        threadIdx.x, COLS * W,
-        [&](int e) {
-            int c, i;
-            divmod_small(e, W, invW, c, i);
-            if (i >= Nz + 2 * K) return 0.0f;   // window overrun of the last block (outputs not stored)
-            int j = i - K;
-            j = j < 0 ? j + Nz : (j >= Nz ? j - Nz : j);
-            if (IL) return b0 + c < B ? in[(size_t)(b0 + c) * P + (size_t)ixc * Nz + j] : 0.0f;
-            return col0 + c < (size_t)B * Nx ? in[(col0 + c) * Nz + j] : 0.0f;
-        },
-        [&](int e, float v) {
-            int c, i;
-            divmod_small(e, W, invW, c, i);
-            xs[c * S + i] = v;
-        });
+    // column c: sample i of the staged window holds in[(i - K) mod Nz] (0 past Nz + 2K, the
+    // overrun of the last register window, whose outputs are not stored).  Column bases are
+    // uniform; a column past the data (IL: sinogram >= B; planar: past B * Nx) reads column
+    // 0 and stages zeros.  Thread t stages samples t, t + 256, ... of every column, its
+    // wrapped source offset computed once per sample for all COLS columns.
+    const float *colp[COLS];
+    bool cok[COLS];
+#pragma unroll
+    for (int c = 0; c < COLS; ++c) {
+        if (IL) {
+            cok[c] = b0 + c < B;
+            colp[c] = in + (cok[c] ? (size_t)(b0 + c) * P : 0) + (size_t)ixc * Nz;
+        } else {
+            cok[c] = col0 + c < (size_t)B * Nx;
+            colp[c] = in + (cok[c] ? (col0 + c) * Nz : 0);
+        }
+    }
+    for (int i = threadIdx.x; i < W; i += 256) {
+        const bool pad = i >= Nz + 2 * K;
+        int j = i - K;
+        j = j < 0 ? j + Nz : (j >= Nz ? j - Nz : j);
+        j = pad ? 0 : j;
+        float v[COLS];
+#pragma unroll
+        for (int c = 0; c < COLS; ++c) v[c] = colp[c][(unsigned)j];
+#pragma unroll
+        for (int c = 0; c < COLS; ++c) xs[c * S + i] = (pad || !cok[c]) ? 0.0f : v[c];
+    }
     __syncthreads();
     // out[i] = sum_a k[a + K] xs[i + K - sgn a]: forward reads xs[i + 2K - t], transpose xs[i + t] (t = a + K)
     if (KT) {   // planar: 256 / COLS threads per column, blocks of kFiltR outputs
@@ -198,7 +213,7 @@ __global__ __launch_bounds__(256) void k_filt_z(const float *__restrict__ in, fl
         const int c = IL ? threadIdx.x % COLS : threadIdx.x / TPC, ib = IL ? threadIdx.x / COLS : threadIdx.x % TPC;
         const bool live = IL ? b0 + c < Bp : col0 + c < (size_t)B * Nx;
         if (!live) return;
-        for (int iz0 = ib * kFiltR; iz0 < Nz; iz0 += (IL ? 16 : TPC) * kFiltR) {
+        for (int iz0 = ib * kFiltR; iz0 < Nz; iz0 += (IL ? 256 / COLS : TPC) * kFiltR) {
             const float *x = xs + c * S + iz0;
             PfAcc w[NW], acc[kFiltR];
 #pragma unroll
@@ -213,6 +228,13 @@ __global__ __launch_bounds__(256) void k_filt_z(const float *__restrict__ in, fl
 #pragma unroll
                     for (int t = 0; t < NT; ++t) acc[q] = fma(tr[t], w[q + NT - 1 - t], acc[q]);
                 }
+            }
+            if (!IL && vst && iz0 + kFiltR <= Nz) {   // 16-byte aligned (checked by the launcher)
+                float4 *o4 = reinterpret_cast<float4 *>(out + (col0 + c) * Nz + iz0);
+#pragma unroll
+                for (int q = 0; q < kFiltR; q += 4)
+                    o4[q / 4] = make_float4((float)acc[q], (float)acc[q + 1], (float)acc[q + 2], (float)acc[q + 3]);
+                continue;
             }
 #pragma unroll
             for (int q = 0; q < kFiltR; ++q) {
@@ -4093,54 +4115,62 @@ static void prefilter_passes(Plan *p, const float *in, float *out, float *tmp, i
                              bool interleaved_out, int Bp, hipStream_t s)
 {
     const size_t tapb = sizeof(float) * 2 * (size_t)(2 * std::max(p->Kz, p->Kx) + 2);
-    const size_t ldz = tapb + sizeof(float) * (size_t)kFzCols * ((p->Nz + 2 * p->Kz + (KZ ? kFiltR : 0)) | 1);
-    const size_t ldz0 = tapb + sizeof(float) * (size_t)kFzCols * ((p->Nz + 2 * p->Kz) | 1);
-    const size_t ldx = tapb + sizeof(float) * (size_t)kFxRows * (p->Nx + 2 * p->Kx + (KX ? kFiltR : 0));
-    const dim3 gx((p->Nz + kFxRows - 1) / kFxRows, B);
+    const size_t ldz = tapb + sizeof(float) * (size_t)kFzColsB * ((p->Nz + 2 * p->Kz + (KZ ? kFiltR : 0)) | 1);
+    const size_t ldz0 = tapb + sizeof(float) * (size_t)kFzColsIL * ((p->Nz + 2 * p->Kz) | 1);
+    const size_t ldzi = tapb + sizeof(float) * (size_t)kFzColsIL * ((p->Nz + 2 * p->Kz + (KZ ? kFiltR : 0)) | 1);
+    const size_t ldx = tapb + sizeof(float) * (size_t)kFxRowsB * (p->Nx + 2 * p->Kx + (KX ? kFiltR : 0));
+    const dim3 gx((p->Nz + kFxRowsB - 1) / kFxRowsB, B);
     // one sinogram: 4 z rows per workgroup (100 workgroups at 400^2 instead of 25)
     constexpr int R1 = 4;
     const dim3 gx1((p->Nz + R1 - 1) / R1, 1);
     const size_t ldx1 = tapb + sizeof(float) * (size_t)R1 * (p->Nx + 2 * p->Kx + (KX ? kFiltR : 0));
     const bool onex = B == 1 && KX > 0;
     const int tr = transp ? 1 : 0;
-    const dim3 gzp(((size_t)B * p->Nx + kFzCols - 1) / kFzCols);
+    const dim3 gzp(((size_t)B * p->Nx + kFzColsB - 1) / kFzColsB);
     // one sinogram: 4 columns per workgroup (64 threads each), so that a 400-column image
     // spreads over 100 workgroups instead of 25
     constexpr int C1 = 4;
     const dim3 gz1((p->Nx + C1 - 1) / C1);
     const size_t ldz1 = tapb + sizeof(float) * (size_t)C1 * ((p->Nz + 2 * p->Kz + (KZ ? kFiltR : 0)) | 1);
     const bool one = B == 1 && KZ > 0;
+    // float4 stores of the planar z pass: columns of Nz % 4 == 0 samples from a 16-byte aligned base
+    auto vec_ok = [&](const float *o) {
+        return (KZ > 0 && p->Nz % 4 == 0 && ((uintptr_t)o & 15) == 0) ? 1 : 0;
+    };
     if (!transp) {   // forward: x pass then z pass (the z pass writes the interleaved layout)
         if (onex)
             hipLaunchKernelGGL((k_filt_x<KX, R1>), gx1, dim3(256), ldx1, s, in, tmp, p->d_kx, p->Kx, p->Nz, p->Nx, 0);
         else
-            hipLaunchKernelGGL((k_filt_x<KX>), gx, dim3(256), ldx, s, in, tmp, p->d_kx, p->Kx, p->Nz, p->Nx, 0);
+            hipLaunchKernelGGL((k_filt_x<KX, kFxRowsB>), gx, dim3(256), ldx, s, in, tmp, p->d_kx, p->Kx, p->Nz, p->Nx,
+                               0);
         // interleaved output [P][Bp]: for one sinogram it is the planar layout, which the
         // register-blocked planar pass writes with every thread busy (the interleaved pass
-        // gives one of its 16 column lanes work there); for batches the interleaved pass
-        // (measured: the register-blocked form is slower for that layout)
+        // gives one of its 16 column lanes work there); for batches the interleaved pass,
+        // register-blocked (32.6 us at 64 x 400^2 against 56 us for the runtime-tap loop,
+        // whose two LDS reads per FMA bound it)
         if (one)
             hipLaunchKernelGGL((k_filt_z<false, KZ, C1>), gz1, dim3(256), ldz1, s, (const float *)tmp, out, p->d_kz,
                                p->Kz, p->Nz, p->Nx, B, Bp, 0);
         else if (interleaved_out)
-            hipLaunchKernelGGL((k_filt_z<true, 0>), dim3(p->Nx, (Bp + kFzCols - 1) / kFzCols), dim3(256), ldz0, s,
-                               (const float *)tmp, out, p->d_kz, p->Kz, p->Nz, p->Nx, B, Bp, 0);
+            hipLaunchKernelGGL((k_filt_z<true, KZ, kFzColsIL>), dim3(p->Nx, (Bp + kFzColsIL - 1) / kFzColsIL),
+                               dim3(256), KZ ? ldzi : ldz0, s, (const float *)tmp, out, p->d_kz, p->Kz, p->Nz, p->Nx,
+                               B, Bp, 0, 0);
         else
-            hipLaunchKernelGGL((k_filt_z<false, KZ>), gzp, dim3(256), ldz, s, (const float *)tmp, out, p->d_kz, p->Kz,
-                               p->Nz, p->Nx, B, Bp, 0);
+            hipLaunchKernelGGL((k_filt_z<false, KZ, kFzColsB>), gzp, dim3(256), ldz, s, (const float *)tmp, out,
+                               p->d_kz, p->Kz, p->Nz, p->Nx, B, Bp, 0, vec_ok(out));
     } else {   // transpose: z^T then x^T
         if (one)
             hipLaunchKernelGGL((k_filt_z<false, KZ, C1>), gz1, dim3(256), ldz1, s, in, tmp, p->d_kz, p->Kz, p->Nz,
                                p->Nx, B, Bp, tr);
         else
-            hipLaunchKernelGGL((k_filt_z<false, KZ>), gzp, dim3(256), ldz, s, in, tmp, p->d_kz, p->Kz, p->Nz, p->Nx,
-                               B, Bp, tr);
+            hipLaunchKernelGGL((k_filt_z<false, KZ, kFzColsB>), gzp, dim3(256), ldz, s, in, tmp, p->d_kz, p->Kz,
+                               p->Nz, p->Nx, B, Bp, tr, vec_ok(tmp));
         if (onex)
             hipLaunchKernelGGL((k_filt_x<KX, R1>), gx1, dim3(256), ldx1, s, (const float *)tmp, out, p->d_kx, p->Kx,
                                p->Nz, p->Nx, tr);
         else
-            hipLaunchKernelGGL((k_filt_x<KX>), gx, dim3(256), ldx, s, (const float *)tmp, out, p->d_kx, p->Kx, p->Nz,
-                               p->Nx, tr);
+            hipLaunchKernelGGL((k_filt_x<KX, kFxRowsB>), gx, dim3(256), ldx, s, (const float *)tmp, out, p->d_kx,
+                               p->Kx, p->Nz, p->Nx, tr);
     }
 }
 
@@ -4149,7 +4179,7 @@ static int launch_prefilter(Plan *p, const float *in, float *out, float *tmp, in
 {
     const size_t tapb = sizeof(float) * 2 * (size_t)(2 * std::max(p->Kz, p->Kx) + 2);
     const size_t ldz = tapb + sizeof(float) * (size_t)kFzCols * ((p->Nz + 2 * p->Kz + kFiltR) | 1);
-    const size_t ldx = tapb + sizeof(float) * (size_t)kFxRows * (p->Nx + 2 * p->Kx + kFiltR);
+    const size_t ldx = tapb + sizeof(float) * (size_t)std::max(kFxRows, kFxRowsB) * (p->Nx + 2 * p->Kx + kFiltR);
     if (ldz <= kLdsBudget && ldx <= kLdsBudget) {
         if (transp && interleaved_out)
             return set_error(MBREC_ERR_ARG, "mbrec:internal", "interleaved output of the transposed prefilter");
@@ -4278,8 +4308,13 @@ int time_stage(Plan *p, int which, int B, int reps, hipStream_t s, float *ms)
 
 // ---------------------------------------------------------------------------
 // autotuner: time every valid gather variant and scatter batch chunk for a
-// batch of B on this plan (HIP events, reps launches each after one warm-up)
-// and keep the fastest; later launches with the same batch stride use them.
+// batch of B on this plan (HIP events, reps launches each after one warm-up),
+// then time the three fastest of each stage again in two alternating rounds and
+// keep the fastest; later launches with the same batch stride use them.  The
+// default gather runs untimed for >= 20 ms first: an idle GPU's clocks ramp up
+// over its first milliseconds of work, and the candidate timed first (the
+// default) was otherwise measured slow (batch 64 at 400^2: 0.92 ms chosen over
+// the default's 0.85 ms).
 // ---------------------------------------------------------------------------
 int autotune(Plan *p, int B, int reps, hipStream_t s, int *gather_out, int *scatter_bc_out)
 {
@@ -4311,6 +4346,27 @@ int autotune(Plan *p, int B, int reps, hipStream_t s, int *gather_out, int *scat
         ms /= reps;
         return MBREC_OK;
     };
+    // the three fastest of a first pass (ms, choice), timed again in two alternating
+    // rounds with every candidate's better round kept; returns the fastest choice
+    auto refine = [&](std::vector<std::pair<float, int>> &res, auto &&set, auto &&fn, int &out) -> int {
+        if (res.empty()) return MBREC_OK;
+        std::sort(res.begin(), res.end());
+        if (res.size() > 3) res.resize(3);
+        std::vector<float> t(res.size(), 1e30f);
+        for (int round = 0; round < 2; ++round)
+            for (size_t k = 0; k < res.size(); ++k) {
+                set(res[k].second);
+                float ms = 0.0f;
+                const int r = timeit(fn, ms, 1e30f);
+                if (r) return r;
+                t[k] = std::min(t[k], ms);
+            }
+        size_t bk = 0;
+        for (size_t k = 1; k < res.size(); ++k)
+            if (t[k] < t[bk]) bk = k;
+        out = res[bk].second;
+        return MBREC_OK;
+    };
     const TuneChoice saved = p->tune;
     p->tune.Bp = Bp;
     // gather
@@ -4320,14 +4376,37 @@ int autotune(Plan *p, int B, int reps, hipStream_t s, int *gather_out, int *scat
     if (gather_variant_ok(p, best_v, Bp)) order.push_back(best_v);
     for (int v = 0; v < kNumGatherVariants; ++v)
         if (v != best_v && gather_variant_ok(p, v, Bp)) order.push_back(v);
+    const auto gather_fn = [&]() { return launch_gather(p, p->d_img_tmp, B, Bp, 1.0f, s); };
+    if (!order.empty()) {   // clock ramp-up: the first candidate untimed for >= 20 ms (at most 400 launches)
+        p->tune.gather = order.front();
+        float el = 0.0f;
+        MBREC_HIP(hipEventRecord(a, s));
+        for (int n = 1; n <= 400 && el < 20.0f; ++n) {
+            if ((rc = gather_fn())) {
+                p->tune = saved;
+                return rc;
+            }
+            if (n % 4 == 0) {
+                MBREC_HIP(hipEventRecord(b, s));
+                MBREC_HIP(hipEventSynchronize(b));
+                MBREC_HIP(hipEventElapsedTime(&el, a, b));
+            }
+        }
+    }
+    std::vector<std::pair<float, int>> res;
     for (int v : order) {
         p->tune.gather = v;
         float ms = 0.0f;
-        if ((rc = timeit([&]() { return launch_gather(p, p->d_img_tmp, B, Bp, 1.0f, s); }, ms, best))) {
+        if ((rc = timeit(gather_fn, ms, best))) {
             p->tune = saved;
             return rc;
         }
+        res.emplace_back(ms, v);
         if (ms < best) { best = ms; best_v = v; }
+    }
+    if ((rc = refine(res, [&](int v) { p->tune.gather = v; }, gather_fn, best_v))) {
+        p->tune = saved;
+        return rc;
     }
     p->tune.gather = best_v;
     // scatter: record-lane LDS batch chunks, then batch-lane lane groups
@@ -4346,14 +4425,21 @@ int autotune(Plan *p, int B, int reps, hipStream_t s, int *gather_out, int *scat
     }
     best = 1e30f;
     int best_bc = cand.front();
+    const auto scatter_fn = [&]() { return launch_scatter(p, p->d_imgI, B, Bp, s); };
+    res.clear();
     for (int c : cand) {
         p->tune.scatter_bc = c;
         float ms = 0.0f;
-        if ((rc = timeit([&]() { return launch_scatter(p, p->d_imgI, B, Bp, s); }, ms, best))) {
+        if ((rc = timeit(scatter_fn, ms, best))) {
             p->tune = saved;
             return rc;
         }
+        res.emplace_back(ms, c);
         if (ms < best) { best = ms; best_bc = c; }
+    }
+    if ((rc = refine(res, [&](int c) { p->tune.scatter_bc = c; }, scatter_fn, best_bc))) {
+        p->tune = saved;
+        return rc;
     }
     p->tune.scatter_bc = best_bc;
     (void)hipEventDestroy(a);

@@ -212,6 +212,32 @@ def test_every_launch_variant_matches_oracle(sir, B):
     plan.set_launch(B, -1, 0)
 
 
+@pytest.mark.parametrize("sir", [False, True])
+@pytest.mark.parametrize("B", [1, 64])
+def test_autotuned_launch_matches_oracle(sir, B):
+    """The autotuner (ramp-up, first pass, refinement of the three fastest) returns a valid
+    launch choice for B, and the operators under it give the oracle's result."""
+    Nz, Nx, D, nT, R = 24, 20, 10, 200, 2.6666666666666665
+    _log(f"autotune sir={sir} B={B}")
+    plan, tt, kw, torch = _plan_problem(Nz, Nx, D, nT, R, seed=60 + B, sir=sir)
+    gv, sc = plan.autotune(B, 2)
+    plan.set_launch(B, gv, sc)   # raises if the choice is not valid for B
+    x = rng(17).uniform(0, 1, (B, Nx, Nz))
+    sg = rng(18).standard_normal((B, D, nT))
+    xo, sgm = x.reshape(B, -1), np.transpose(sg, (0, 2, 1))
+    if sir:
+        sidx, s, se = (a.T for a in kw["sir"])
+        yo = np.transpose(O.forward_sir(xo, sidx, tt, s, nT, se, R), (0, 2, 1))
+        xto = O.transpose_sir(sgm, sidx, tt, s, se, R)
+    else:
+        yo = np.transpose(O.forward_das(xo, tt, nT, R), (0, 2, 1))
+        xto = O.transpose_das(sgm, tt, R)
+    y = plan.forward(torch.tensor(x, dtype=torch.float32).cuda(), model=False).cpu().numpy()
+    xt = plan.transpose(torch.tensor(sg, dtype=torch.float32).cuda(), model=False).cpu().numpy()
+    assert rel_l2(y, yo) < TOL_OP, (gv, sc)
+    assert rel_l2(xt.reshape(B, -1), xto) < TOL_OP, (gv, sc)
+
+
 def test_one_sinogram_tile_variants_match_oracle():
     """Every one-sinogram tile form on a real arc geometry (the tile windows fit the LDS only
     for smooth travel times): forward choices 2, 3 (record streaming), 5, 6, 7 (tile form, thread
