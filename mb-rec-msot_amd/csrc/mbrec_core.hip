@@ -715,6 +715,30 @@ static int build_tile_windows(Plan *p, const std::vector<int32_t> &tq)
     if ((rc = dalloc(&p->d_twin, w.size(), &p->table_bytes)) || (rc = upload(p->d_twin, w.data(), w.size())) ||
         (rc = dalloc(&p->d_ttab, tt.size(), &p->table_bytes)) || (rc = upload(p->d_ttab, tt.data(), tt.size())))
         return rc;
+    // launch order of the batch tile gather: the tiles listed in serpentine strips of four
+    // tile rows (z), the list cut into 8 contiguous runs, one per XCD (workgroup L runs on
+    // XCD L % 8 as its (L / 8)-th); the workgroups an XCD holds at once then cover a compact
+    // patch of neighbouring tiles, whose bin windows for one detector overlap in its L2
+    if (p->Nx > 1 && p->TZ == 16) {
+        const int ntz = (p->Nz + 15) / 16, ntx = (p->Nx + 15) / 16;
+        std::vector<int32_t> seq;
+        seq.reserve(ntile);
+        for (int z0 = 0, strip = 0; z0 < ntz; z0 += 4, ++strip)
+            for (int i = 0; i < ntx; ++i) {
+                const int tx = (strip & 1) ? ntx - 1 - i : i;
+                for (int tz = z0; tz < std::min(ntz, z0 + 4); ++tz) seq.push_back(tx * ntz + tz);
+            }
+        const int ntp = 8 * ((ntile + 7) / 8), m = ntp / 8;
+        std::vector<int32_t> ord(ntp);
+        for (int L = 0; L < ntp; ++L) {
+            const int idx = (L % 8) * m + L / 8;
+            ord[L] = idx < ntile ? seq[idx] : -1;
+        }
+        if ((rc = dalloc(&p->d_tord, ord.size(), &p->table_bytes)) ||
+            (rc = upload(p->d_tord, ord.data(), ord.size())))
+            return rc;
+        p->tile_launch = ntp;
+    }
     // the same pairs packed into 32 bits for the one-sinogram gather: row (< 512) | u23 << 9,
     // u rounded to the nearest 2^-23 (within half an FP32 ulp at u ~ 1)
     if (p->tile_zr <= 256) {
@@ -736,7 +760,7 @@ static void free_plan(Plan *p)
 {
     void *ptrs[] = {p->d_pix, p->d_tq, p->d_sidx, p->d_sir, p->d_swin, p->d_ir, p->d_irf, p->d_ir64, p->d_irf64,
                     p->d_kz,  p->d_kx, p->d_rec, p->d_rec_off, p->d_rec_k, p->d_rec1, p->d_twin, p->d_ptab, p->d_ttab, p->d_srec, p->d_srec_off, p->d_sirw, p->d_stwin, p->d_stab,
-                    p->d_tfrec, p->d_tfroff, p->d_tfdesc, p->d_tfpart, p->d_tfsrec, p->d_tfsro, p->d_tq1};
+                    p->d_tfrec, p->d_tfroff, p->d_tfdesc, p->d_tfpart, p->d_tfsrec, p->d_tfsro, p->d_tq1, p->d_tord};
     for (void *q : ptrs)
         if (q) (void)hipFree(q);
     free_work(p);

@@ -72,6 +72,9 @@ struct Plan {
     int tile_wmax = 0;           // longest tile window
     int2 *d_ttab = nullptr;      // [Pd / 256][D][256] (bin row inside the tile window, or tile_zr; u bits)
     int tile_zr = 0;             // first of the two zero rows after a staged tile window (64 * ceil(wmax / 64))
+    int32_t *d_tord = nullptr;   // [tile_launch] tile of each workgroup of the batch tile gather, or -1
+                                 // (XCD-aware order; null: workgroup = tile)
+    int tile_launch = 0;         // workgroups per batch chunk and detector group of that gather
     uint32_t *d_tq1 = nullptr;   // [Pd / 256][D][256] packed (row inside the tile window, or tile_zr: 9 bits |
                                  // u in units of 2^-23: 23 bits); tile-major stream of the one-sinogram gather
     BinGeom geom{};

@@ -2401,12 +2401,16 @@ template <int G, int NCH>
 __global__ __launch_bounds__(1024) void k_gather_ta(
     const int2 *__restrict__ ttab, const int2 *__restrict__ twin, const int32_t *__restrict__ pix,
     const float *__restrict__ bins, float *__restrict__ out, int D, int NB, int P, int B, int Bp, float scale,
-    int bins_bytes, int ttab_bytes)
+    int bins_bytes, int ttab_bytes, const int32_t *__restrict__ tord)
 {
     using L = TaLayout<NCH>;
     __shared__ __attribute__((aligned(16))) float s0[L::BUF], s1[L::BUF], s2[L::BUF];
     __shared__ int2 tw[kTaMaxD];   // the tile's window origins
-    const int tile = blockIdx.x, grp = blockIdx.y % G, b0 = (blockIdx.y / G) * 64;
+    // tord: the tile of each workgroup (XCD-aware order; -1: a padding workgroup, which
+    // exits before any barrier), or null: workgroup x = tile x
+    const int tile = tord ? tord[blockIdx.x] : (int)blockIdx.x;
+    if (tile < 0) return;
+    const int grp = blockIdx.y % G, b0 = (blockIdx.y / G) * 64;
     const int lane = threadIdx.x & 63, wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int DG = (D + G - 1) / G, dbeg = grp * DG, dend = min(D, dbeg + DG);
     for (int i = threadIdx.x; i < 2 * 64; i += 1024) {   // the zero rows of the three buffers
@@ -3833,9 +3837,10 @@ static void gather_ta_n(Plan *p, float *img, int B, int Bp, float scale, hipStre
     const size_t lds = 0;   // static LDS
     const int bytes = (int)(sizeof(float) * ((size_t)p->D * p->geom.NB + 2) * Bp);
     const int tbytes = (int)(sizeof(int2) * (size_t)(p->Pd / 256) * p->D * 256);
-    hipLaunchKernelGGL((k_gather_ta<G, NCH>), dim3(p->Pd / 256, G * nbc), dim3(1024), lds, s, p->d_ttab, p->d_twin,
-                       p->d_pix, p->d_bins, G > 1 ? p->d_gpart : img, p->D, p->geom.NB, p->P, B, Bp, scale, bytes,
-                       tbytes);
+    const int nwg = p->d_tord ? p->tile_launch : p->Pd / 256;
+    hipLaunchKernelGGL((k_gather_ta<G, NCH>), dim3(nwg, G * nbc), dim3(1024), lds, s, p->d_ttab, p->d_twin, p->d_pix,
+                       p->d_bins, G > 1 ? p->d_gpart : img, p->D, p->geom.NB, p->P, B, Bp, scale, bytes, tbytes,
+                       (const int32_t *)p->d_tord);
     if (G > 1) {
         const size_t n = (size_t)B * p->P;
         hipLaunchKernelGGL(k_sum_parts<G>, dim3(std::min<size_t>(4096, (n + 255) / 256)), dim3(256), 0, s,
