@@ -4155,7 +4155,7 @@ static void prefilter_passes(Plan *p, const float *in, float *out, float *tmp, i
         // whose two LDS reads per FMA bound it)
         if (one)
             hipLaunchKernelGGL((k_filt_z<false, KZ, C1>), gz1, dim3(256), ldz1, s, (const float *)tmp, out, p->d_kz,
-                               p->Kz, p->Nz, p->Nx, B, Bp, 0);
+                               p->Kz, p->Nz, p->Nx, B, Bp, 0, vec_ok(out));
         else if (interleaved_out)
             hipLaunchKernelGGL((k_filt_z<true, KZ, kFzColsIL>), dim3(p->Nx, (Bp + kFzColsIL - 1) / kFzColsIL),
                                dim3(256), KZ ? ldzi : ldz0, s, (const float *)tmp, out, p->d_kz, p->Kz, p->Nz, p->Nx,
@@ -4166,7 +4166,7 @@ static void prefilter_passes(Plan *p, const float *in, float *out, float *tmp, i
     } else {   // transpose: z^T then x^T
         if (one)
             hipLaunchKernelGGL((k_filt_z<false, KZ, C1>), gz1, dim3(256), ldz1, s, in, tmp, p->d_kz, p->Kz, p->Nz,
-                               p->Nx, B, Bp, tr);
+                               p->Nx, B, Bp, tr, vec_ok(tmp));
         else
             hipLaunchKernelGGL((k_filt_z<false, KZ, kFzColsB>), gzp, dim3(256), ldz, s, in, tmp, p->d_kz, p->Kz,
                                p->Nz, p->Nx, B, Bp, tr, vec_ok(tmp));
