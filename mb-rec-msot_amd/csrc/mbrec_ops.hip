@@ -128,7 +128,7 @@ __global__ __launch_bounds__(256) void k_filt_axis(
 }
 
 // ---------------------------------------------------------------------------
-// separable circular prefilter, LDS-staged passes (FP64 sums as above).
+// separable circular prefilter, LDS-staged passes (sums in PfAcc, FP32: see below).
 // z pass: COLS columns of Nz samples per workgroup, each staged once with K
 // wrapped samples on either side.  Planar output [B][P] (float4 stores where
 // aligned); interleaved output [P][Bp] (IL) takes the columns as consecutive
