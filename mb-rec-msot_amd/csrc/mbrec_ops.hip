@@ -3615,7 +3615,9 @@ static int launch_bins_to_y(Plan *p, float *y, int B, int Bp, float scale, hipSt
                             const int2 *win = nullptr)
 {
     if (Bp >= 16) {
-        if (Bp >= 64) bins_to_y_bc<64, 64>(p, y, B, Bp, scale, s, win);
+        // batches of 64 and more: 32 sinograms x 64 samples per workgroup (28 KB of LDS, five
+        // workgroups per CU: 110 us at 64 x 400^2 against 138 us for 64 x 64, 56 KB, two per CU)
+        if (Bp >= 64) bins_to_y_bc<32, 64>(p, y, B, Bp, scale, s, win);
         else if (Bp >= 32) bins_to_y_bc<32, 128>(p, y, B, Bp, scale, s, win);
         else bins_to_y_bc<16, 128>(p, y, B, Bp, scale, s, win);
         MBREC_HIP(hipGetLastError());
