@@ -28,7 +28,7 @@ def gauspuls_eir(fs: float, fc: float = 4e6, bw: float = 1.0, bwr: float = -6.0,
 
 def butter_zero_phase_response(n: int, fs: float, fmin: float, fmax: float) -> np.ndarray:
     """|H(e^{i 2 pi k/n})|^2 of the cascaded Butterworth filters, k = 0..n-1
-    (filter_butter_zero_phase.m:74-96: freqz(..., n, 'whole', fs))."""
+    (filter_butter_zero_phase.m:23-45: butter / zp2sos, then freqz(..., n, 'whole', fs))."""
     zs, ps, k = [], [], 1.0
     if fmax > 0:
         z, p, kk = signal.butter(8, fmax * 2.0 / fs, btype="low", output="zpk")
