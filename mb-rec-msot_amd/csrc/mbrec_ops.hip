@@ -3616,7 +3616,7 @@ static int launch_bins_to_y(Plan *p, float *y, int B, int Bp, float scale, hipSt
 {
     if (Bp >= 16) {
         // batches of 64 and more: 32 sinograms x 64 samples per workgroup (28 KB of LDS, five
-        // workgroups per CU: 110 us at 64 x 400^2 against 138 us for 64 x 64, 56 KB, two per CU)
+        // workgroups per CU: 113 us at 64 x 400^2 against 134 us for 64 x 64, 56 KB, two per CU)
         if (Bp >= 64) bins_to_y_bc<32, 64>(p, y, B, Bp, scale, s, win);
         else if (Bp >= 32) bins_to_y_bc<32, 128>(p, y, B, Bp, scale, s, win);
         else bins_to_y_bc<16, 128>(p, y, B, Bp, scale, s, win);
