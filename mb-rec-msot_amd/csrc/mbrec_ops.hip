@@ -141,7 +141,7 @@ __global__ __launch_bounds__(256) void k_filt_axis(
 constexpr int kFzCols = 16;
 constexpr int kFxRows = 16;
 // batches: 16 columns per z-pass workgroup (planar and interleaved output), 32 z rows per
-// x-pass workgroup (128-byte runs per x; 35 us against 39 us for 16 rows at 64 x 400^2)
+// x-pass workgroup (128-byte runs per x; 34 us against 37 us for 16 rows at 64 x 400^2)
 constexpr int kFzColsB = 16, kFzColsIL = 16, kFxRowsB = 32;
 
 constexpr int kFiltR = 8;   // outputs per thread of the register-blocked prefilter passes
@@ -4153,7 +4153,7 @@ static void prefilter_passes(Plan *p, const float *in, float *out, float *tmp, i
         // interleaved output [P][Bp]: for one sinogram it is the planar layout, which the
         // register-blocked planar pass writes with every thread busy (the interleaved pass
         // gives one of its 16 column lanes work there); for batches the interleaved pass,
-        // register-blocked (32.6 us at 64 x 400^2 against 56 us for the runtime-tap loop,
+        // register-blocked (32 us at 64 x 400^2 against 56 us for the runtime-tap loop,
         // whose two LDS reads per FMA bound it)
         if (one)
             hipLaunchKernelGGL((k_filt_z<false, KZ, C1>), gz1, dim3(256), ldz1, s, (const float *)tmp, out, p->d_kz,
